@@ -1,0 +1,860 @@
+/*
+ * CPU oracle for the VFS pressure-projection path -- TEST INFRASTRUCTURE ONLY.
+ * See vfs_oracle.h for the layout conventions.  Each function restates the
+ * arithmetic of the cited reference lines (trunglendsu/VFS_v3p2) in FP64,
+ * keeping the reference's branch structure, operand order and accumulation
+ * order wherever the result is meant to be compared bit for bit.
+ */
+#include "vfs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TH 0.1 /* poisson_threshold, poisson.c:16 */
+
+#define IX(g, i, j, k) ((((size_t)(k)) * (size_t)(g)->my + (size_t)(j)) * (size_t)(g)->mx + (size_t)(i))
+
+double or_threshold(void) { return TH; }
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+static int is_bnd(const or_grid *g, int i, int j, int k) {
+  return i == 0 || i == g->mx - 1 || j == 0 || j == g->my - 1 || k == 0 || k == g->mz - 1;
+}
+
+/* periodic image of a DMDA point (poisson.c:153-174); returns 1 if any axis was mapped */
+static int periodic_image(const or_grid *g, int i, int j, int k, int *a, int *b, int *c, int flag[3]) {
+  *a = i; *b = j; *c = k;
+  flag[0] = flag[1] = flag[2] = 0;
+  if (g->per[0] && i == 0) { *a = g->mx - 2; flag[0] = 1; }
+  else if (g->per[0] && i == g->mx - 1) { *a = 1; flag[0] = 1; }
+  if (g->per[1] && j == 0) { *b = g->my - 2; flag[1] = 1; }
+  else if (g->per[1] && j == g->my - 1) { *b = 1; flag[1] = 1; }
+  if (g->per[2] && k == 0) { *c = g->mz - 2; flag[2] = 1; }
+  else if (g->per[2] && k == g->mz - 1) { *c = 1; flag[2] = 1; }
+  return flag[0] | flag[1] | flag[2];
+}
+
+static void periodic_copy_scalar(const or_grid *g, double *f) {
+  if (!(g->per[0] || g->per[1] || g->per[2])) return;
+  for (int k = 0; k < g->mz; k++)
+    for (int j = 0; j < g->my; j++)
+      for (int i = 0; i < g->mx; i++) {
+        int a, b, c, fl[3];
+        if (periodic_image(g, i, j, k, &a, &b, &c, fl)) f[IX(g, i, j, k)] = f[IX(g, a, b, c)];
+      }
+}
+
+/* ---------------------------------------------------------------- setup_lidx2 */
+int or_setup_gid(const or_grid *g, const double *nvert, double *gid) {
+  size_t n = (size_t)g->mx * g->my * g->mz;
+  for (size_t p = 0; p < n; p++) gid[p] = -1.0;
+  int cnt = 0;
+  for (int k = 0; k < g->mz; k++)
+    for (int j = 0; j < g->my; j++)
+      for (int i = 0; i < g->mx; i++) {
+        if (is_bnd(g, i, j, k)) continue;
+        size_t p = IX(g, i, j, k);
+        if ((int)nvert[p] > TH) continue; /* poisson.c:239 casts to int */
+        gid[p] = (double)cnt++;
+      }
+  periodic_copy_scalar(g, gid); /* poisson.c:298-331 */
+  return cnt;
+}
+
+/* ---------------------------------------------------------- face coefficients */
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+void or_face_coeffs(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                    const double *jcsi, const double *jeta, const double *jzet,
+                    const double *kcsi, const double *keta, const double *kzet,
+                    const double *iaj, const double *jaj, const double *kaj, double *g9) {
+  size_t n = (size_t)g->mx * g->my * g->mz;
+  memset(g9, 0, 9 * n * sizeof(double));
+  for (int k = 0; k < g->mz; k++)
+    for (int j = 0; j < g->my; j++)
+      for (int i = 0; i < g->mx; i++) {
+        int a, b, c, fl[3];
+        periodic_image(g, i, j, k, &a, &b, &c, fl);
+        size_t s = IX(g, a, b, c), p = IX(g, i, j, k);
+        /* operand order of poisson.c:1132-1140: (v.x*w.x + v.y*w.y + v.z*w.z) * aj */
+        g9[0 * n + p] = dot3(&icsi[3 * s], &icsi[3 * s]) * iaj[s];
+        g9[1 * n + p] = dot3(&ieta[3 * s], &icsi[3 * s]) * iaj[s];
+        g9[2 * n + p] = dot3(&izet[3 * s], &icsi[3 * s]) * iaj[s];
+        g9[3 * n + p] = dot3(&jcsi[3 * s], &jeta[3 * s]) * jaj[s];
+        g9[4 * n + p] = dot3(&jeta[3 * s], &jeta[3 * s]) * jaj[s];
+        g9[5 * n + p] = dot3(&jzet[3 * s], &jeta[3 * s]) * jaj[s];
+        g9[6 * n + p] = dot3(&kcsi[3 * s], &kzet[3 * s]) * kaj[s];
+        g9[7 * n + p] = dot3(&keta[3 * s], &kzet[3 * s]) * kaj[s];
+        g9[8 * n + p] = dot3(&kzet[3 * s], &kzet[3 * s]) * kaj[s];
+      }
+}
+
+/* ------------------------------------------------------------ operator rows */
+/* slot index of offset (di,dj,dk), poisson.c:956-974 */
+static int slot_of(int di, int dj, int dk) {
+  static const int tab[3][3][3] = {
+      /* dk=-1 */ {{-1, OR_BS, -1}, {OR_BW, OR_BP, OR_BE}, {-1, OR_BN, -1}},
+      /* dk= 0 */ {{OR_SW, OR_SP, OR_SE}, {OR_WP, OR_CP, OR_EP}, {OR_NW, OR_NP, OR_NE}},
+      /* dk=+1 */ {{-1, OR_TS, -1}, {OR_TW, OR_TP, OR_TE}, {-1, OR_TN, -1}}};
+  return tab[dk + 1][dj + 1][di + 1];
+}
+
+/* Cross-difference case for the face between cells p and q (q = p +/- e_d), taken
+ * in direction e (poisson.c:1182-1230 and the 11 rotated copies). */
+static int cross_case(const or_grid *g, const double *nv, size_t p, size_t q, size_t se, int pos_e,
+                      int m_e, int per_e) {
+  double sum_hi = nv[p + se] + nv[q + se];
+  double sum_lo = nv[p - se] + nv[q - se];
+  if ((pos_e == m_e - 2 && !per_e) || sum_hi > TH) {
+    if (sum_lo < TH && pos_e != 1) return OR_X_LOWER;
+    return OR_X_ZERO;
+  } else if ((pos_e == 1 && !per_e) || sum_lo > TH) {
+    if (sum_hi < TH) return OR_X_UPPER;
+    return OR_X_ZERO;
+  }
+  (void)g;
+  return OR_X_CENTRAL;
+}
+
+void or_stencil(const or_grid *g, const double *g9, const double *nvert, double *vol19,
+                uint8_t *rowflag, uint8_t *cases) {
+  const size_t n = (size_t)g->mx * g->my * g->mz;
+  const size_t stride[3] = {1, (size_t)g->mx, (size_t)g->mx * g->my};
+  const int m[3] = {g->mx, g->my, g->mz};
+  memset(vol19, 0, 19 * n * sizeof(double));
+  memset(rowflag, 0, n);
+  memset(cases, 0, 18 * n);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        const size_t c = IX(g, i, j, k);
+        /* rows exist only for fluid cells: poisson.c:1158-1169 */
+        if (nvert[c] > 0.1) continue;
+        rowflag[c] = 1;
+        const int pos[3] = {i, j, k};
+        double vol[19];
+        for (int s = 0; s < 19; s++) vol[s] = 0.;
+        /* faces in the reference's order: i+, i-, j+, j-, k+, k- */
+        for (int d = 0; d < 3; d++)
+          for (int sgn = 1; sgn >= -1; sgn -= 2) {
+            const int face = 2 * d + (sgn > 0 ? 0 : 1);
+            const size_t q = sgn > 0 ? c + stride[d] : c - stride[d];
+            const int edge = sgn > 0 ? (pos[d] != m[d] - 2) : (pos[d] != 1);
+            if (!(nvert[q] < TH && (edge || g->per[d]))) continue;
+            cases[18 * c + 3 * face] = 1;
+            const size_t owner = sgn > 0 ? c : q; /* face coefficients live on the lower cell */
+            int xi = 0;
+            for (int e = 0; e < 3; e++) {
+              const double gv = g9[(size_t)(3 * d + e) * n + owner];
+              int off_q[3] = {0, 0, 0};
+              off_q[d] = sgn;
+              if (e == d) {
+                /* g_dd (phi_q - phi_c) term, e.g. poisson.c:1183-1184 */
+                vol[OR_CP] -= gv;
+                vol[slot_of(off_q[0], off_q[1], off_q[2])] += gv;
+                continue;
+              }
+              const int cc = cross_case(g, nvert, c, q, stride[e], pos[e], m[e], g->per[e]);
+              cases[18 * c + 3 * face + 1 + xi] = (uint8_t)cc;
+              xi++;
+              if (cc == OR_X_ZERO) continue;
+              const double w = (cc == OR_X_CENTRAL) ? 0.25 : 0.5;
+              const double t = w * gv; /* exact: w is a power of two */
+              int hi = (cc == OR_X_LOWER) ? 0 : 1;  /* upper row offset in e */
+              int lo = (cc == OR_X_UPPER) ? 0 : -1; /* lower row offset in e */
+              int o1[3] = {0, 0, 0}, o2[3], o3[3] = {0, 0, 0}, o4[3];
+              /* the row gets sgn * F; F's cross part is t * (phi[c+hi]+phi[q+hi]-phi[c+lo]-phi[q+lo]) */
+              o1[e] = hi;
+              memcpy(o2, off_q, sizeof o2); o2[e] += hi;
+              o3[e] = lo;
+              memcpy(o4, off_q, sizeof o4); o4[e] += lo;
+              if (sgn > 0) {
+                vol[slot_of(o1[0], o1[1], o1[2])] += t;
+                vol[slot_of(o2[0], o2[1], o2[2])] += t;
+                vol[slot_of(o3[0], o3[1], o3[2])] -= t;
+                vol[slot_of(o4[0], o4[1], o4[2])] -= t;
+              } else {
+                vol[slot_of(o1[0], o1[1], o1[2])] -= t;
+                vol[slot_of(o2[0], o2[1], o2[2])] -= t;
+                vol[slot_of(o3[0], o3[1], o3[2])] += t;
+                vol[slot_of(o4[0], o4[1], o4[2])] += t;
+              }
+            }
+          }
+        for (int s = 0; s < 19; s++) vol19[19 * c + s] = vol[s];
+      }
+}
+
+static const int SLOT_OFF[19][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+                                    {0, 0, 1},  {0, 0, -1},  {1, 1, 0},  {1, -1, 0}, {-1, 1, 0},
+                                    {-1, -1, 0}, {0, 1, 1},  {0, 1, -1}, {0, -1, 1}, {0, -1, -1},
+                                    {1, 0, 1},  {1, 0, -1},  {-1, 0, 1}, {-1, 0, -1}};
+
+int or_assemble_csr(const or_grid *g, const double *vol19, const uint8_t *rowflag,
+                    const double *gid, int drop, int *rowptr, int *col, double *val) {
+  int nnz = 0, row = 0;
+  rowptr[0] = 0;
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (!rowflag[c]) continue;
+        int r = (int)gid[c];
+        if (r != row) return -1; /* rows are numbered in this loop order */
+        int start = nnz;
+        for (int s = 0; s < 19; s++) {
+          size_t q = IX(g, i + SLOT_OFF[s][0], j + SLOT_OFF[s][1], k + SLOT_OFF[s][2]);
+          int idx = (int)gid[q];
+          double v = vol19[19 * c + s];
+          /* poisson.c:1608 (drop=1); drop=0 keeps every nonzero coefficient */
+          const double cut = drop ? 1.e-10 : 0.;
+          if ((fabs(v) > cut && idx >= 0) || s == OR_CP) {
+            int dup = -1;
+            for (int t = start; t < nnz; t++)
+              if (col[t] == idx) dup = t;
+            if (dup >= 0) { val[dup] = v; continue; } /* SetValues overwrites */
+            col[nnz] = idx;
+            val[nnz] = v;
+            nnz++;
+          }
+        }
+        row++;
+        rowptr[row] = nnz;
+      }
+  return nnz;
+}
+
+void or_csr_matvec(int nrow, const int *rowptr, const int *col, const double *val,
+                   const double *x, double *y) {
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < nrow; r++) {
+    double s = 0.;
+    for (int t = rowptr[r]; t < rowptr[r + 1]; t++) s += val[t] * x[col[t]];
+    y[r] = s;
+  }
+}
+
+void or_stencil_apply(const or_grid *g, const double *vol19, const uint8_t *rowflag,
+                      const double *phi, double *y) {
+  size_t n = (size_t)g->mx * g->my * g->mz;
+  memset(y, 0, n * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (!rowflag[c]) continue;
+        double s = 0.;
+        for (int t = 0; t < 19; t++)
+          s += vol19[19 * c + t] * phi[IX(g, i + SLOT_OFF[t][0], j + SLOT_OFF[t][1], k + SLOT_OFF[t][2])];
+        y[c] = s;
+      }
+}
+
+/* ----------------------------------------------------------------------- RHS */
+void or_rhs(const or_grid *g, const double *ucont, const double *nvert, double dt, double st,
+            double coeff, double *b) {
+  size_t n = (size_t)g->mx * g->my * g->mz;
+  memset(b, 0, n * sizeof(double));
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (nvert[c] >= 0.1) continue; /* poisson_hypre.c:346 */
+        double val = 0;
+        val -= ucont[3 * c + 0];
+        if (i == 1 && g->per[0]) val += ucont[3 * IX(g, mx - 2, j, k) + 0];
+        else val += ucont[3 * IX(g, i - 1, j, k) + 0];
+        val -= ucont[3 * c + 1];
+        if (j == 1 && g->per[1]) val += ucont[3 * IX(g, i, my - 2, k) + 1];
+        else val += ucont[3 * IX(g, i, j - 1, k) + 1];
+        val -= ucont[3 * c + 2];
+        if (k == 1 && g->per[2]) val += ucont[3 * IX(g, i, j, mz - 2) + 2];
+        else val += ucont[3 * IX(g, i, j, k - 1) + 2];
+        val *= -1.0 / dt * st * coeff; /* poisson_hypre.c:376 */
+        b[c] = val;
+      }
+}
+
+/* ---------------------------------------------------------------- VolumeFlux */
+static double norm3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double *csi,
+                    const double *eta, const double *zet, const double *nvert, double out[4]) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const size_t st[3] = {1, (size_t)mx, (size_t)mx * my};
+  const double *met[3] = {csi, eta, zet};
+  const int mm[3] = {mx, my, mz};
+  if (immersed == 0) {
+    /* poisson.c:2634-2828 */
+    for (int pass = 0; pass < 2; pass++) {
+      double flux = 0, area = 0;
+      for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+          for (int i = 1; i < mx - 1; i++) {
+            size_t c = IX(g, i, j, k);
+            int pos[3] = {i, j, k};
+            for (int d = 0; d < 3; d++) {
+              double nq = nvert[c + st[d]];
+              if (pos[d] > mm[d] - 2) continue;
+              if (nvert[c] < TH && nq > TH) { flux += ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+              if (nvert[c] > TH && nq < TH) { flux -= ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+            }
+          }
+      out[2 * pass] = flux;
+      out[2 * pass + 1] = area;
+      if (pass == 1) break;
+      double corr = (area > 1.e-15) ? flux / area : 0.;
+      for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+          for (int i = 1; i < mx - 1; i++) {
+            size_t c = IX(g, i, j, k);
+            int pos[3] = {i, j, k};
+            for (int d = 0; d < 3; d++) {
+              double nq = nvert[c + st[d]];
+              if (pos[d] > mm[d] - 2) continue;
+              if (nvert[c] < TH && nq > TH) { ucont[3 * c + d] -= norm3(&met[d][3 * c]) * corr; ucont[3 * c + d] = 0; }
+              if (nvert[c] > TH && nq < TH) { ucont[3 * c + d] += norm3(&met[d][3 * c]) * corr; ucont[3 * c + d] = 0; }
+            }
+          }
+    }
+    return;
+  }
+  /* immersed == 1: poisson.c:2898-3240 with flg = 0 */
+  const double eps = 1.e-10, ibmval = 1.1;
+  double flux = 0, area = 0;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        int pos[3] = {i, j, k};
+        for (int d = 0; d < 3; d++) {
+          double nq = nvert[c + st[d]];
+          if (!(pos[d] < mm[d] - 2)) continue;
+          if (nvert[c] < 0.1 && nq > 0.1 && nq < ibmval) {
+            if (fabs(ucont[3 * c + d]) > eps) { flux += ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+            else ucont[3 * c + d] = 0.;
+          }
+        }
+        if (nvert[c] > 0.1 && nvert[c] < ibmval)
+          for (int d = 0; d < 3; d++) {
+            double nq = nvert[c + st[d]];
+            if (!(pos[d] < mm[d] - 2)) continue;
+            if (nq < 0.1) {
+              if (fabs(ucont[3 * c + d]) > eps) { flux -= ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+              else ucont[3 * c + d] = 0.;
+            }
+          }
+      }
+  out[0] = flux;
+  out[1] = area;
+  double corr = (area > 1.e-15) ? flux / area : 0.;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        int pos[3] = {i, j, k};
+        for (int d = 0; d < 3; d++) {
+          double nq = nvert[c + st[d]];
+          if (!(pos[d] < mm[d] - 2)) continue;
+          if (nvert[c] < 0.1 && nq > 0.1 && nq < ibmval && fabs(ucont[3 * c + d]) > eps)
+            ucont[3 * c + d] -= norm3(&met[d][3 * c]) * corr;
+        }
+        if (nvert[c] > 0.1 && nvert[c] < 1.1)
+          for (int d = 0; d < 3; d++) {
+            double nq = nvert[c + st[d]];
+            if (!(pos[d] < mm[d] - 2)) continue;
+            if (nq < 0.1 && fabs(ucont[3 * c + d]) > eps) ucont[3 * c + d] += norm3(&met[d][3 * c]) * corr;
+          }
+      }
+  flux = 0; area = 0;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        int pos[3] = {i, j, k};
+        for (int d = 0; d < 3; d++) {
+          double nq = nvert[c + st[d]];
+          if (!(pos[d] < mm[d] - 2)) continue;
+          if (nvert[c] < 0.1 && nq > 0.1 && nq < ibmval) { flux += ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+        }
+        if (nvert[c] > 0.1 && nvert[c] < ibmval)
+          for (int d = 0; d < 3; d++) {
+            double nq = nvert[c + st[d]];
+            if (!(pos[d] < mm[d] - 2)) continue;
+            if (nq < 0.1) { flux -= ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+          }
+      }
+  out[2] = flux;
+  out[3] = area;
+}
+
+/* ------------------------------------------------------ null space, scatter */
+void or_remove_nullspace(const or_grid *g, const double *nvert, const double *gid, double *x) {
+  double sum = 0;
+  int cnt = 0;
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (nvert[c] < 0.1) { sum += x[(int)gid[c]]; cnt++; }
+      }
+  double val = -sum / (double)cnt;
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (nvert[c] < 0.1) x[(int)gid[c]] += val;
+      }
+}
+
+void or_convert_phi(const or_grid *g, const double *nvert, const double *phi2, double *phi) {
+  size_t n = (size_t)g->mx * g->my * g->mz;
+  memset(phi, 0, n * sizeof(double));
+  int pos = 0;
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if ((int)nvert[c] > TH) phi[c] = 0;
+        else phi[c] = phi2[pos++];
+      }
+  periodic_copy_scalar(g, phi);
+}
+
+void or_update_pressure(const or_grid *g, const double *nvert, double *p, double *phi) {
+  for (int k = 1; k < g->mz - 1; k++)
+    for (int j = 1; j < g->my - 1; j++)
+      for (int i = 1; i < g->mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (nvert[c] < 0.1) p[c] += phi[c];
+      }
+  periodic_copy_scalar(g, p);
+  periodic_copy_scalar(g, phi);
+}
+
+/* ---------------------------------------------------------------- Projection */
+/* cross difference of phi for the face (p,q) in direction e, poisson.c:2234-2257 */
+static double proj_cross(const or_grid *g, const double *nv, const double *phi, size_t p, size_t q,
+                         size_t se, int pos_e, int m_e, int per_e) {
+  int cc = cross_case(g, nv, p, q, se, pos_e, m_e, per_e);
+  switch (cc) {
+    case OR_X_LOWER: return (phi[p] + phi[q] - phi[p - se] - phi[q - se]) * 0.5;
+    case OR_X_UPPER: return (phi[p + se] + phi[q + se] - phi[p] - phi[q]) * 0.5;
+    case OR_X_CENTRAL: return (phi[p + se] + phi[q + se] - phi[p - se] - phi[q - se]) * 0.25;
+    default: return 0.;
+  }
+}
+
+void or_projection(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                   const double *jcsi, const double *jeta, const double *jzet,
+                   const double *kcsi, const double *keta, const double *kzet,
+                   const double *iaj, const double *jaj, const double *kaj,
+                   const double *nvert, const double *phi, double *ucont, double dt, double st,
+                   double coeff) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const size_t sx = 1, sy = (size_t)mx, sz = (size_t)mx * my;
+  const int pi = g->per[0], pj = g->per[1], pk = g->per[2];
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        const size_t c = IX(g, i, j, k);
+        /* i face, poisson.c:2221-2265 */
+        if (j > 0 && j < my - 1 && k > 0 && k < mz - 1)
+          if ((i > 0 && i < mx - 2) || (pi && i == mx - 2)) {
+            const size_t q = c + sx;
+            double dpdc = phi[q] - phi[c];
+            if (i == mx - 2 && pi) dpdc = phi[IX(g, 1, j, k)] - phi[c];
+            double dpde = proj_cross(g, nvert, phi, c, q, sy, j, my, pj);
+            double dpdz = proj_cross(g, nvert, phi, c, q, sz, k, mz, pk);
+            if ((nvert[c] + nvert[q]) < TH) {
+              const double *a = &icsi[3 * c], *b = &ieta[3 * c], *z = &izet[3 * c];
+              ucont[3 * c + 0] -= (dpdc * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) * iaj[c] +
+                                   dpde * (b[0] * a[0] + b[1] * a[1] + b[2] * a[2]) * iaj[c] +
+                                   dpdz * (z[0] * a[0] + z[1] * a[1] + z[2] * a[2]) * iaj[c]) *
+                                  dt * st / coeff / 1.0;
+            }
+          }
+        /* j face, poisson.c:2267-2314 */
+        if (i > 0 && i < mx - 1 && k > 0 && k < mz - 1)
+          if ((j > 0 && j < my - 2) || (pj && j == my - 2)) {
+            const size_t q = c + sy;
+            double dpdc = proj_cross(g, nvert, phi, c, q, sx, i, mx, pi);
+            double dpde;
+            if (j == my - 2 && pj) dpde = phi[IX(g, i, 1, k)] - phi[c];
+            else dpde = phi[q] - phi[c];
+            double dpdz = proj_cross(g, nvert, phi, c, q, sz, k, mz, pk);
+            if ((nvert[c] + nvert[q]) < TH) {
+              const double *a = &jcsi[3 * c], *b = &jeta[3 * c], *z = &jzet[3 * c];
+              ucont[3 * c + 1] -= (dpdc * (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * jaj[c] +
+                                   dpde * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) * jaj[c] +
+                                   dpdz * (z[0] * b[0] + z[1] * b[1] + z[2] * b[2]) * jaj[c]) *
+                                  dt * st / coeff / 1.0;
+            }
+          }
+        /* k face, poisson.c:2316-2363 */
+        if (i > 0 && i < mx - 1 && j > 0 && j < my - 1)
+          if ((k > 0 && k < mz - 2) || (pk && k == mz - 2)) {
+            const size_t q = c + sz;
+            double dpdc = proj_cross(g, nvert, phi, c, q, sx, i, mx, pi);
+            double dpde = proj_cross(g, nvert, phi, c, q, sy, j, my, pj);
+            double dpdz;
+            if (k == mz - 2 && pk) dpdz = phi[IX(g, i, j, 1)] - phi[c];
+            else dpdz = phi[q] - phi[c];
+            if ((nvert[c] + nvert[q]) < TH) {
+              const double *a = &kcsi[3 * c], *b = &keta[3 * c], *z = &kzet[3 * c];
+              ucont[3 * c + 2] -= (dpdc * (a[0] * z[0] + a[1] * z[1] + a[2] * z[2]) * kaj[c] +
+                                   dpde * (b[0] * z[0] + b[1] * z[1] + b[2] * z[2]) * kaj[c] +
+                                   dpdz * (z[0] * z[0] + z[1] * z[1] + z[2] * z[2]) * kaj[c]) *
+                                  dt * st / coeff / 1.0;
+            }
+          }
+      }
+  /* periodic ghost faces, poisson.c:2401-2441 */
+  if (pi || pj || pk)
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          int a, b, cc, fl[3];
+          if (!periodic_image(g, i, j, k, &a, &b, &cc, fl)) continue;
+          size_t p = IX(g, i, j, k), s = IX(g, a, b, cc);
+          if (fl[0]) ucont[3 * p + 0] = ucont[3 * s + 0];
+          if (fl[1]) ucont[3 * p + 1] = ucont[3 * s + 1];
+          if (fl[2]) ucont[3 * p + 2] = ucont[3 * s + 2];
+        }
+}
+
+/* --------------------------------------------------------------- Contra2Cart */
+void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, const double *eta,
+                    const double *zet, const double *nvert, double *ucont_in, double *ucat) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const size_t n = (size_t)mx * my * mz;
+  const int anyper = g->per[0] || g->per[1] || g->per[2];
+  /* local copy of Ucont with whole-struct periodic ghost copies, rhs.c:115-142 */
+  double *lu = (double *)malloc(3 * n * sizeof(double));
+  memcpy(lu, ucont_in, 3 * n * sizeof(double));
+  if (anyper)
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          int a, b, c, fl[3];
+          if (!periodic_image(g, i, j, k, &a, &b, &c, fl)) continue;
+          size_t p = IX(g, i, j, k), s = IX(g, a, b, c);
+          lu[3 * p + 0] = lu[3 * s + 0]; lu[3 * p + 1] = lu[3 * s + 1]; lu[3 * p + 2] = lu[3 * s + 2];
+        }
+  /* core Cramer solve, rhs.c:144-189 */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if (!(nvert[c] < 0.1)) continue;
+        const double *m0 = &csi[3 * c], *m1 = &eta[3 * c], *m2 = &zet[3 * c];
+        double q0 = 0.5 * (lu[3 * IX(g, i - 1, j, k) + 0] + lu[3 * c + 0]);
+        double q1 = 0.5 * (lu[3 * IX(g, i, j - 1, k) + 1] + lu[3 * c + 1]);
+        double q2 = 0.5 * (lu[3 * IX(g, i, j, k - 1) + 2] + lu[3 * c + 2]);
+        double det = m0[0] * (m1[1] * m2[2] - m1[2] * m2[1]) - m0[1] * (m1[0] * m2[2] - m1[2] * m2[0]) +
+                     m0[2] * (m1[0] * m2[1] - m1[1] * m2[0]);
+        double det0 = q0 * (m1[1] * m2[2] - m1[2] * m2[1]) - q1 * (m0[1] * m2[2] - m0[2] * m2[1]) +
+                      q2 * (m0[1] * m1[2] - m0[2] * m1[1]);
+        double det1 = -q0 * (m1[0] * m2[2] - m1[2] * m2[0]) + q1 * (m0[0] * m2[2] - m0[2] * m2[0]) -
+                      q2 * (m0[0] * m1[2] - m0[2] * m1[0]);
+        double det2 = q0 * (m1[0] * m2[1] - m1[1] * m2[0]) - q1 * (m0[0] * m2[1] - m0[1] * m2[0]) +
+                      q2 * (m0[0] * m1[1] - m0[1] * m1[0]);
+        ucat[3 * c + 0] = det0 / det;
+        ucat[3 * c + 1] = det1 / det;
+        ucat[3 * c + 2] = det2 / det;
+      }
+  /* periodic copy, rhs.c:198-241 */
+  if (anyper)
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          int a, b, c, fl[3];
+          if (!periodic_image(g, i, j, k, &a, &b, &c, fl)) continue;
+          size_t p = IX(g, i, j, k), s = IX(g, a, b, c);
+          for (int d = 0; d < 3; d++) ucat[3 * p + d] = ucat[3 * s + d];
+        }
+  /* boundary values: solid cells and no-slip walls, rhs.c:257-260, 491-515 */
+  double *lc = lu; /* reuse the buffer as the snapshot of Ucat */
+  memcpy(lc, ucat, 3 * n * sizeof(double));
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        size_t c = IX(g, i, j, k);
+        if ((int)(nvert[c] + 0.1) == 3) {
+          ucat[3 * c] = ucat[3 * c + 1] = ucat[3 * c + 2] = 0.;
+          continue;
+        }
+        size_t src = (size_t)-1;
+        if (i == 0 && bctype[0] == 1 && (j != 0 && k != 0)) src = IX(g, i + 1, j, k);
+        if (i == mx - 1 && bctype[1] == 1 && (j != 0 && k != 0)) src = IX(g, i - 1, j, k);
+        if (src != (size_t)-1) for (int d = 0; d < 3; d++) ucat[3 * c + d] = -lc[3 * src + d];
+        src = (size_t)-1;
+        if (j == 0 && bctype[2] == 1 && (i != 0 && k != 0)) src = IX(g, i, j + 1, k);
+        if (j == my - 1 && bctype[3] == 1 && (i != 0 && k != 0)) src = IX(g, i, j - 1, k);
+        if (src != (size_t)-1) for (int d = 0; d < 3; d++) ucat[3 * c + d] = -lc[3 * src + d];
+        src = (size_t)-1;
+        if (k == 0 && bctype[4] == 1 && (i != 0 && j != 0)) src = IX(g, i, j, k + 1);
+        if (k == mz - 1 && bctype[5] == 1 && (i != 0 && j != 0)) src = IX(g, i, j, k - 1);
+        if (src != (size_t)-1) for (int d = 0; d < 3; d++) ucat[3 * c + d] = -lc[3 * src + d];
+      }
+  /* final periodic copy, rhs.c:645-673 */
+  if (anyper)
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          int a, b, c, fl[3];
+          if (!periodic_image(g, i, j, k, &a, &b, &c, fl)) continue;
+          size_t p = IX(g, i, j, k), s = IX(g, a, b, c);
+          for (int d = 0; d < 3; d++) ucat[3 * p + d] = ucat[3 * s + d];
+        }
+  free(lu);
+}
+
+/* ---------------------------------------------------------------- Divergence */
+double or_divergence(const or_grid *g, const double *u, const double *aj, const double *nvert,
+                     double *div) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const size_t sy = (size_t)mx, sz = (size_t)mx * my;
+  double mx_div = 0.;
+  if (div) memset(div, 0, (size_t)mx * my * mz * sizeof(double));
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        double d = fabs((u[3 * c] - u[3 * (c - 1)] + u[3 * c + 1] - u[3 * (c - sy) + 1] + u[3 * c + 2] -
+                         u[3 * (c - sz) + 2]) *
+                        aj[c]);
+        if (nvert[c] + nvert[c + sz] + nvert[c - sz] + nvert[c + sy] + nvert[c - sy] + nvert[c + 1] +
+                nvert[c - 1] >
+            0.1)
+          d = 0.;
+        if (div) div[c] = d;
+        if (d > mx_div) mx_div = d;
+      }
+  return mx_div;
+}
+
+/* ------------------------------------------------------------ nvert encoding */
+void or_nvert_classify(const or_grid *g, const int bctype[6], double *nv) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  /* dilation, ibm.c:331-353 (order independent: it only ever writes 2) */
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        size_t c = IX(g, i, j, k);
+        if (nv[c] < 0) nv[c] = 0;
+        int ip = i < mx - 1 ? i + 1 : i, im = i > 0 ? i - 1 : i;
+        int jp = j < my - 1 ? j + 1 : j, jm = j > 0 ? j - 1 : j;
+        int kp = k < mz - 1 ? k + 1 : k, km = k > 0 ? k - 1 : k;
+        if ((int)(nv[c] + 0.5) != 4)
+          for (int kk = km; kk < kp + 1; kk++)
+            for (int jj = jm; jj < jp + 1; jj++)
+              for (int ii = im; ii < ip + 1; ii++)
+                if ((int)(nv[IX(g, ii, jj, kk)] + 0.5) == 4) nv[c] = nv[c] > 2 ? nv[c] : 2;
+      }
+  /* remap 2->1, 4->3 on the interior, ibm.c:540-547 */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        if ((int)(nv[c] + 0.5) == 2) nv[c] = 1;
+        if ((int)(nv[c] + 0.5) == 4) nv[c] = 3;
+      }
+  /* wall-model face layers, ibm.c:550-568 */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k);
+        int pos[6] = {i == 1, i == mx - 2, j == 1, j == my - 2, k == 1, k == mz - 2};
+        for (int f = 0; f < 6; f++)
+          if ((bctype[f] == -1 || bctype[f] == -2) && pos[f]) nv[c] = 1;
+      }
+}
+
+/* ------------------------------------------------------------- MG primitives */
+void or_mg_restrict(const or_grid *gf, const or_grid *gc, const double *nvf, const double *nvc,
+                    const double *x, double *f) {
+  for (int k = 0; k < gc->mz; k++)
+    for (int j = 0; j < gc->my; j++)
+      for (int i = 0; i < gc->mx; i++) {
+        size_t c = IX(gc, i, j, k);
+        if (is_bnd(gc, i, j, k)) { f[c] = 0.; continue; }
+        int ih = 2 * i, jh = 2 * j, kh = 2 * k;
+#define W(a, b, cc) (x[IX(gf, a, b, cc)] * fmax(0., 1 - nvf[IX(gf, a, b, cc)]))
+        f[c] = 0.125 * (W(ih, jh, kh) + W(ih - 1, jh, kh) + W(ih, jh - 1, kh) + W(ih, jh, kh - 1) +
+                        W(ih - 1, jh - 1, kh) + W(ih, jh - 1, kh - 1) + W(ih - 1, jh, kh - 1) +
+                        W(ih - 1, jh - 1, kh - 1));
+#undef W
+        if (nvc[c] > 0.1) f[c] = 0.;
+      }
+}
+
+void or_mg_prolong(const or_grid *gf, const or_grid *gc, const double *nvf, const double *nvc,
+                   const double *x, double *f) {
+  const int mx = gf->mx, my = gf->my, mz = gf->mz;
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        int ic = (i + 1) / 2, jc = (j + 1) / 2, kc = (k + 1) / 2;
+        int ia = (i - 2 * ic) == 0 ? 1 : -1;
+        int ja = (j - 2 * jc) == 0 ? 1 : -1;
+        int ka = (k - 2 * kc) == 0 ? 1 : -1;
+        if (i == 1 || i == mx - 2) ia = 0;
+        if (j == 1 || j == my - 2) ja = 0;
+        if (k == 1 || k == mz - 2) ka = 0;
+        if (ka == -1 && nvc[IX(gc, ic, jc, kc - 1)] > 0.1) ka = 0;
+        else if (ka == 1 && nvc[IX(gc, ic, jc, kc + 1)] > 0.1) ka = 0;
+        if (ja == -1 && nvc[IX(gc, ic, jc - 1, kc)] > 0.1) ja = 0;
+        else if (ja == 1 && nvc[IX(gc, ic, jc + 1, kc)] > 0.1) ja = 0;
+        if (ia == -1 && nvc[IX(gc, ic - 1, jc, kc)] > 0.1) ia = 0;
+        else if (ia == 1 && nvc[IX(gc, ic + 1, jc, kc)] > 0.1) ia = 0;
+#define X(a, b, cc) x[IX(gc, a, b, cc)]
+        f[IX(gf, i, j, k)] = (X(ic, jc, kc) * 9 + X(ic, jc + ja, kc) * 3 + X(ic + ia, jc, kc) * 3 +
+                              X(ic + ia, jc + ja, kc)) * 3. / 64. +
+                             (X(ic, jc, kc + ka) * 9 + X(ic, jc + ja, kc + ka) * 3 + X(ic + ia, jc, kc + ka) * 3 +
+                              X(ic + ia, jc + ja, kc + ka)) / 64.;
+#undef X
+      }
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        size_t c = IX(gf, i, j, k);
+        if (is_bnd(gf, i, j, k)) f[c] = 0.;
+        if (nvf[c] > 0.1) f[c] = 0.;
+      }
+}
+
+void or_mg_nvert_restrict(const or_grid *gf, const or_grid *gc, const double *nvf, double *nvc) {
+  size_t n = (size_t)gc->mx * gc->my * gc->mz;
+  for (size_t p = 0; p < n; p++) nvc[p] = 0.;
+  for (int k = 1; k < gc->mz - 1; k++)
+    for (int j = 1; j < gc->my - 1; j++)
+      for (int i = 1; i < gc->mx - 1; i++) {
+        int ih = 2 * i, jh = 2 * j, kh = 2 * k;
+#define V(a, b, cc) nvf[IX(gf, a, b, cc)]
+        if (V(ih, jh, kh) * V(ih - 1, jh, kh) * V(ih, jh - 1, kh) * V(ih, jh, kh - 1) * V(ih - 1, jh - 1, kh) *
+                V(ih - 1, jh, kh - 1) * V(ih, jh - 1, kh - 1) * V(ih - 1, jh - 1, kh - 1) >
+            0.1) {
+          size_t c = IX(gc, i, j, k);
+          nvc[c] = nvc[c] > 1. ? nvc[c] : 1.;
+        }
+#undef V
+      }
+  /* isolated coarse fluid cells, poisson.c:2083-2097 (sequential in-place sweep) */
+  const size_t sy = (size_t)gc->mx, sz = (size_t)gc->mx * gc->my;
+  for (int k = 1; k < gc->mz - 1; k++)
+    for (int j = 1; j < gc->my - 1; j++)
+      for (int i = 1; i < gc->mx - 1; i++) {
+        size_t c = IX(gc, i, j, k);
+        if (nvc[c] < 0.1 && nvc[c + 1] + nvc[c - 1] > 1.1 && nvc[c + sy] + nvc[c - sy] > 1.1 &&
+            nvc[c + sz] + nvc[c - sz] > 1.1)
+          nvc[c] = 1.;
+      }
+}
+
+int or_fully_blocked(const or_grid *g, const double *nvert, int *kske) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  int multi = 1;
+  for (int j = 1; j < my - 1; j++)
+    for (int i = 1; i < mx - 1; i++) {
+      int blocked = 0;
+      for (int k = 0; k < mz; k++)
+        if (nvert[IX(g, i, j, k)] > 0.1) {
+          if (!blocked) { kske[2 * (j * mx + i)] = k; blocked = 1; }
+          else if (k < kske[2 * (j * mx + i)]) kske[2 * (j * mx + i)] = k;
+        }
+      if (!blocked) multi = 0;
+    }
+  return multi;
+}
+
+/* ---------------------------------------------------------------- GMRES(m) */
+static double vdot(int n, const double *a, const double *b) {
+  double s = 0.;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+int or_gmres(int n, const int *rowptr, const int *col, const double *val, const double *b,
+             double *x, int m, double rtol, int maxit, int precond, double *rel) {
+  double *V = (double *)malloc((size_t)(m + 1) * n * sizeof(double));
+  double *Z = (double *)malloc((size_t)m * n * sizeof(double));
+  double *H = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = (double *)malloc(m * sizeof(double)), *sn = (double *)malloc(m * sizeof(double));
+  double *gv = (double *)malloc((m + 1) * sizeof(double)), *y = (double *)malloc(m * sizeof(double));
+  double *dinv = (double *)malloc(n * sizeof(double));
+  double *w = (double *)malloc(n * sizeof(double));
+  for (int r = 0; r < n; r++) {
+    double d = 1.;
+    if (precond == 1)
+      for (int t = rowptr[r]; t < rowptr[r + 1]; t++)
+        if (col[t] == r && val[t] != 0.) d = 1. / val[t];
+    dinv[r] = d;
+  }
+  double bn = sqrt(vdot(n, b, b));
+  if (bn == 0.) bn = 1.;
+  int it = 0;
+  double res = 0.;
+  for (;;) {
+    or_csr_matvec(n, rowptr, col, val, x, w);
+    for (int i = 0; i < n; i++) V[i] = b[i] - w[i];
+    double beta = sqrt(vdot(n, V, V));
+    res = beta / bn;
+    if (res <= rtol || it >= maxit) break;
+    for (int i = 0; i < n; i++) V[i] /= beta;
+    for (int i = 0; i <= m; i++) gv[i] = 0.;
+    gv[0] = beta;
+    int j;
+    for (j = 0; j < m && it < maxit; j++, it++) {
+      double *zj = Z + (size_t)j * n, *vj = V + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
+      for (int i = 0; i < n; i++) zj[i] = dinv[i] * vj[i];
+      or_csr_matvec(n, rowptr, col, val, zj, vn);
+      for (int l = 0; l <= j; l++) { /* modified Gram-Schmidt */
+        double h = vdot(n, vn, V + (size_t)l * n);
+        H[l * m + j] = h;
+        const double *vl = V + (size_t)l * n;
+        for (int i = 0; i < n; i++) vn[i] -= h * vl[i];
+      }
+      double hn = sqrt(vdot(n, vn, vn));
+      H[(j + 1) * m + j] = hn;
+      if (hn != 0.) for (int i = 0; i < n; i++) vn[i] /= hn;
+      for (int l = 0; l < j; l++) {
+        double t1 = cs[l] * H[l * m + j] + sn[l] * H[(l + 1) * m + j];
+        double t2 = -sn[l] * H[l * m + j] + cs[l] * H[(l + 1) * m + j];
+        H[l * m + j] = t1; H[(l + 1) * m + j] = t2;
+      }
+      double a = H[j * m + j], bb = H[(j + 1) * m + j], r = sqrt(a * a + bb * bb);
+      cs[j] = r == 0. ? 1. : a / r; sn[j] = r == 0. ? 0. : bb / r;
+      H[j * m + j] = r; H[(j + 1) * m + j] = 0.;
+      gv[j + 1] = -sn[j] * gv[j]; gv[j] = cs[j] * gv[j];
+      if (fabs(gv[j + 1]) / bn <= rtol) { j++; it++; break; }
+    }
+    for (int l = j - 1; l >= 0; l--) {
+      double s = gv[l];
+      for (int q = l + 1; q < j; q++) s -= H[l * m + q] * y[q];
+      y[l] = s / H[l * m + l];
+    }
+    for (int l = 0; l < j; l++) {
+      const double *zl = Z + (size_t)l * n;
+      for (int i = 0; i < n; i++) x[i] += y[l] * zl[i];
+    }
+  }
+  if (rel) *rel = res;
+  free(V); free(Z); free(H); free(cs); free(sn); free(gv); free(y); free(dinv); free(w);
+  return it;
+}

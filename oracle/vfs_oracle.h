@@ -1,0 +1,132 @@
+/*
+ * CPU oracle for the VFS pressure-projection path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C FP64 restatement of the reference's live pressure path
+ * (trunglendsu/VFS_v3p2: poisson.c, poisson_hypre.c, rhs.c, main.c, ibm.c),
+ * written to check the HIP engine.  Only the test suite, the smoke check and
+ * the CPU-baseline leg of the benchmark may load it.  The engine never calls
+ * it: the product path is the HIP library under vfs_v3p2_amd/.
+ *
+ * Layout: every field is a DMDA array [mz][my][mx] with i fastest, exactly the
+ * single-rank view the reference gets from DMDAVecGetArray.  Index 0 and m-1
+ * of each axis are the boundary/ghost planes; interior cells are 1..m-2.
+ * Vector fields (Cmpnts, variables.h:37-39) are AoS {x,y,z} doubles.
+ * per[d] != 0 means the axis is periodic (i_periodic or ii_periodic: plane 0
+ * aliases plane m-2 and plane m-1 aliases plane 1).
+ *
+ * Parity status of each function is listed in oracle/README_ORACLE.md.
+ */
+#ifndef VFS_ORACLE_H
+#define VFS_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int mx, my, mz; /* DMDA extents, nx+2 etc. */
+  int per[3];     /* periodic flags (i, j, k) */
+} or_grid;
+
+/* stencil slot order of poisson.c:956-974 */
+enum { OR_CP = 0, OR_EP, OR_WP, OR_NP, OR_SP, OR_TP, OR_BP, OR_NE, OR_SE, OR_NW, OR_SW,
+       OR_TN, OR_BN, OR_TS, OR_BS, OR_TE, OR_BE, OR_TW, OR_BW };
+
+/* cross-term case codes (per face, per cross direction) */
+enum { OR_X_ZERO = 0, OR_X_CENTRAL = 1, OR_X_LOWER = 2, OR_X_UPPER = 3 };
+
+double or_threshold(void); /* poisson.c:16 */
+
+/* setup_lidx2 (poisson.c:187-334): fluid numbering in k,j,i order; gid=-1 elsewhere.
+ * Returns the number of unknowns. */
+int or_setup_gid(const or_grid *g, const double *nvert, double *gid);
+
+/* Face coefficients g11..g33 (poisson.c:1105-1141).  g9 holds 9 DMDA arrays in the
+ * order g11,g12,g13,g21,g22,g23,g31,g32,g33.  Points outside [l?s-1, l?e] stay 0. */
+void or_face_coeffs(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                    const double *jcsi, const double *jeta, const double *jzet,
+                    const double *kcsi, const double *keta, const double *kzet,
+                    const double *iaj, const double *jaj, const double *kaj, double *g9);
+
+/* Row coefficients of PoissonLHSNew before the |v|<=1e-10 drop (poisson.c:1146-1558).
+ * vol19: 19 doubles per DMDA point (zero where no row).  rowflag: 1 where a row exists.
+ * cases: 18 bytes per DMDA point: for faces (i+,i-,j+,j-,k+,k-) x (active, cross1, cross2)
+ * where cross1/cross2 are the two cross directions in increasing axis order. */
+void or_stencil(const or_grid *g, const double *g9, const double *nvert, double *vol19,
+                uint8_t *rowflag, uint8_t *cases);
+
+/* CSR assembly (poisson.c:1586-1616).  drop=1 applies the reference's |v|<=1e-10
+ * drop rule (poisson.c:1608); drop=0 keeps every nonzero coefficient (the
+ * conservative operator the projection is consistent with).  Arrays sized
+ * nrow+1 / 19*nrow.  Returns nnz, or -1 if the row numbering disagrees with gid. */
+int or_assemble_csr(const or_grid *g, const double *vol19, const uint8_t *rowflag,
+                    const double *gid, int drop, int *rowptr, int *col, double *val);
+
+void or_csr_matvec(int nrow, const int *rowptr, const int *col, const double *val,
+                   const double *x, double *y);
+
+/* Undropped matrix-free apply on DMDA arrays; phi must hold periodic ghost images. */
+void or_stencil_apply(const or_grid *g, const double *vol19, const uint8_t *rowflag,
+                      const double *phi, double *y);
+
+/* PoissonRHS2_hypre (poisson_hypre.c:342-429): b on fluid cells (DMDA layout). */
+void or_rhs(const or_grid *g, const double *ucont, const double *nvert, double dt, double st,
+            double coeff, double *b);
+
+/* VolumeFlux (poisson.c:2634-2828 when immersed==0; poisson.c:2898-3240 flg=0 when
+ * immersed==1).  out[0..3] = flux, area before correction, flux, area after. */
+void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double *csi,
+                    const double *eta, const double *zet, const double *nvert, double out[4]);
+
+/* Remove_Nullspace (poisson_hypre.c:207-270) on the compact vector. */
+void or_remove_nullspace(const or_grid *g, const double *nvert, const double *gid, double *x);
+
+/* Convert_Phi2_Phi (poisson.c:97-185). */
+void or_convert_phi(const or_grid *g, const double *nvert, const double *phi2, double *phi);
+
+/* UpdatePressure (poisson.c:2522-2614): P += Phi, then periodic ghost copies of P and Phi. */
+void or_update_pressure(const or_grid *g, const double *nvert, double *p, double *phi);
+
+/* Projection (poisson.c:2202-2441) without the trailing Contra2Cart call. */
+void or_projection(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                   const double *jcsi, const double *jeta, const double *jzet,
+                   const double *kcsi, const double *keta, const double *kzet,
+                   const double *iaj, const double *jaj, const double *kaj,
+                   const double *nvert, const double *phi, double *ucont, double dt, double st,
+                   double coeff);
+
+/* Contra2Cart (rhs.c:50-260, 491-515, 645-673): bctype 1 (no-slip) and periodic only. */
+void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, const double *eta,
+                    const double *zet, const double *nvert, double *ucont, double *ucat);
+
+/* Divergence (main.c:1279-1347): returns max |aj div u| over eligible cells; div may be NULL. */
+double or_divergence(const or_grid *g, const double *ucont, const double *aj, const double *nvert,
+                     double *div);
+
+/* nvert encoding (ibm.c:334-356, 540-568): input 4 inside / 0 outside; output {0,1,3}. */
+void or_nvert_classify(const or_grid *g, const int bctype[6], double *nvert);
+
+/* MG primitives (poisson.c:336-669, 2004-2106, 1918-2002).  Coarse grid gc has
+ * extents derived by IM_c=(IM_f+1)/2; all axes fully coarsened. */
+void or_mg_restrict(const or_grid *gf, const or_grid *gc, const double *nvert_f,
+                    const double *nvert_c, const double *xf, double *fc);
+void or_mg_prolong(const or_grid *gf, const or_grid *gc, const double *nvert_f,
+                   const double *nvert_c, const double *xc, double *ff);
+void or_mg_nvert_restrict(const or_grid *gf, const or_grid *gc, const double *nvert_f,
+                          double *nvert_c);
+int or_fully_blocked(const or_grid *g, const double *nvert, int *kske);
+
+/* Right-preconditioned restarted GMRES(m) on the compact CSR system, FP64.
+ * precond: 0 none, 1 Jacobi.  Stops at ||b-Ax||/||b|| <= rtol.  Returns iterations,
+ * writes the final relative residual to *rel. */
+int or_gmres(int n, const int *rowptr, const int *col, const double *val, const double *b,
+             double *x, int m, double rtol, int maxit, int precond, double *rel);
+
+int or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
