@@ -1,0 +1,179 @@
+/*
+ * vfs_pp -- MI355X pressure-projection engine, C ABI.
+ *
+ * Drop-in for the pressure path of the VFS curvilinear immersed-boundary solver
+ * (trunglendsu/VFS_v3p2).  The reference calls, once per block per time step
+ * (solvers.c:874-895):
+ *     PoissonSolver_Hypre(user, ibm, ibm_intp);   poisson_hypre.c:580  (variables.h:785)
+ *     UpdatePressure(user);                        poisson.c:2473       (variables.h:752)
+ *     Projection(user);                            poisson.c:2122       (variables.h:753)
+ * with every field carried in UserCtx (variables.h:298-591).  This library takes
+ * the same fields as plain arrays and runs all per-cell work on one GPU.
+ *
+ * Array layout (all entry points): the single-rank DMDA layout the reference
+ * uses -- a dense [mz][my][mx] array with i fastest, mx = nx+2, my = ny+2,
+ * mz = nz+2 (planes 0 and m-1 are the boundary/ghost planes, init.c:156-157).
+ * Vector fields (Cmpnts, variables.h:37-39) are AoS {x,y,z} FP64 triples.
+ * "_device" entry points take device pointers (hipMalloc'd or framework
+ * tensors); the others take host pointers and copy over PCIe.
+ *
+ * Status codes: 0 ok, otherwise one of VFS_PP_E* below.  The reference returns
+ * PetscErrorCode 0 unconditionally and is silent on non-convergence; this ABI
+ * reports it.
+ *
+ * Thread safety: one handle may be used by one host thread at a time.  Each
+ * handle owns one HIP stream (default priority); calls are synchronous unless
+ * stated otherwise.
+ */
+#ifndef VFS_PP_H
+#define VFS_PP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFS_PP_OK 0
+#define VFS_PP_EINVAL 1      /* bad argument or shape */
+#define VFS_PP_ENOMEM 2      /* device allocation failed / admission refused */
+#define VFS_PP_EHIP 3        /* HIP runtime error */
+#define VFS_PP_ENOCONV 4     /* solver did not reach the requested tolerance */
+#define VFS_PP_ESTATE 5      /* call order violated (e.g. solve before set_geometry) */
+#define VFS_PP_ENONFINITE 6  /* NaN/Inf in the inputs or the iterate */
+
+typedef struct vfs_pp_handle vfs_pp_handle;
+
+/* Grid description: extents of interior cells and the reference's flags. */
+typedef struct {
+  int nx, ny, nz;   /* interior cells; DMDA extents are nx+2 etc. (init.c:156-157) */
+  int periodic[3];  /* i_periodic|ii_periodic, j..., k... (main.c:2150-2160) */
+  int bctype[6];    /* user->bctype (init.c:502); Contra2Cart honours 1 (no-slip) */
+  int immersed;     /* 0: VolumeFlux(user,Ucont,..) poisson.c:2634; 1: flg variant poisson.c:2898 */
+} vfs_grid;
+
+/* Solver options.  Defaults (vfs_pp_default_opts) give the mixed-precision form. */
+typedef struct {
+  int precision;     /* 0: FP64 multigrid preconditioner (plain FP64 drop-in)
+                        1: FP32 multigrid inside FP64 FGMRES (mixed precision) */
+  double rtol;       /* stop when ||b - A phi||_2 <= rtol ||b||_2 (flux-form residual,
+                        the norm HYPRE GMRES uses, poisson_hypre.c:90-95) */
+  double div_tol;    /* additionally require max|aj div u| <= div_tol after projection
+                        (main.c:1283); <= 0 disables the check */
+  int max_iters;     /* outer FGMRES iterations (poisson_it, main.c:126, default 10) */
+  int restart;       /* FGMRES restart length (HYPRE KDim 51, poisson_hypre.c:82) */
+  int smoother;      /* 0 auto, 1 point Chebyshev-Jacobi, 2 y-line Chebyshev */
+  int cheb_degree;   /* smoothing steps per pre/post smoothing */
+  int coarse_max;    /* cell budget of the coarsest (direct-solve) level */
+  int drop_small;    /* 1: drop |a_ij| <= 1e-10 like the assembled HYPRE matrix
+                        (poisson.c:1608); 0: conservative operator (default) */
+  int zero_guess;    /* 1: start from phi = 0; 0: start from the phi passed in */
+} vfs_pp_opts;
+
+typedef struct {
+  int iterations;        /* outer Krylov iterations */
+  int vcycles;           /* preconditioner applications */
+  int converged;         /* 1 if rtol (and div_tol) were met */
+  double rel_residual;   /* ||b - A phi|| / ||b|| after the solve, FP64 */
+  double bnorm;          /* ||b||_2 */
+  double max_div;        /* max |aj div u| after projection (Divergence, main.c:1283) */
+  double ibm_flux[4];    /* VolumeFlux: flux/area before and after correction */
+  double phi_mean;       /* mean subtracted by Remove_Nullspace */
+  double ms_total;       /* wall time of the call (ms) */
+} vfs_pp_stats;
+
+void vfs_pp_default_opts(vfs_pp_opts *opts);
+
+/* Create a handle for one grid.  Replaces the ti==tistart set-up branch of
+ * PoissonSolver_Hypre (setup_lidx2 + Create_Hypre_* + PoissonLHSNew,
+ * poisson_hypre.c:609-627). */
+int vfs_pp_create(const vfs_grid *grid, const vfs_pp_opts *opts, vfs_pp_handle **out);
+int vfs_pp_destroy(vfs_pp_handle *h);
+int vfs_pp_set_opts(vfs_pp_handle *h, const vfs_pp_opts *opts);
+
+/* Geometry: the UserCtx fields lICsi..lKZet, lIAj..lKAj, lAj, lCsi/lEta/lZet and
+ * lNvert (variables.h:333-356, 378-385), host pointers in DMDA layout.  Builds the
+ * face coefficients (poisson.c:1105-1141), the per-cell mask codes of
+ * PoissonLHSNew's branches (poisson.c:1146-1558) and the multigrid hierarchy;
+ * everything stays resident on the device. */
+int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta,
+                        const double *izet, const double *jcsi, const double *jeta,
+                        const double *jzet, const double *kcsi, const double *keta,
+                        const double *kzet, const double *iaj, const double *jaj,
+                        const double *kaj, const double *aj, const double *csi,
+                        const double *eta, const double *zet, const double *nvert);
+
+/* PoissonSolver_Hypre (poisson_hypre.c:580-771) on device-resident arrays:
+ * VolumeFlux (modifies ucont, poisson_hypre.c:662-667) -> PoissonRHS2_hypre ->
+ * Krylov+MG solve -> Remove_Nullspace -> Convert_Phi2_Phi.  phi is the DMDA
+ * pressure increment (lPhi, with periodic ghost images); on entry it is the
+ * initial guess unless opts.zero_guess. */
+int vfs_pp_poisson_solve_device(vfs_pp_handle *h, double *ucont, double *phi, double dt,
+                                double st, double coeff, vfs_pp_stats *stats);
+
+/* UpdatePressure (poisson.c:2473-2632): P += Phi on fluid cells, periodic ghosts. */
+int vfs_pp_update_pressure_device(vfs_pp_handle *h, double *p, double *phi);
+
+/* Projection (poisson.c:2122-2470) including its Contra2Cart call (rhs.c:50-675,
+ * no-slip and periodic boundaries): corrects ucont in place and writes ucat
+ * (cells the reference does not touch keep their input values). */
+int vfs_pp_projection_device(vfs_pp_handle *h, double *ucont, const double *phi, double *ucat,
+                             double dt, double st, double coeff);
+
+/* Divergence (main.c:1244-1391): max |aj div u| over cells whose 7-point
+ * neighbourhood is fluid. */
+int vfs_pp_divergence_device(vfs_pp_handle *h, const double *ucont, double *maxdiv);
+
+/* The whole per-step pressure path, as solvers.c:874-895 runs it:
+ * poisson_solve -> update_pressure -> projection, plus the divergence check. */
+int vfs_pp_project_device(vfs_pp_handle *h, double *ucont, double *phi, double *p, double *ucat,
+                          double dt, double st, double coeff, vfs_pp_stats *stats);
+
+/* Same as vfs_pp_project_device with host arrays (copies over PCIe). */
+int vfs_pp_project(vfs_pp_handle *h, double *ucont, double *phi, double *p, double *ucat,
+                   double dt, double st, double coeff, vfs_pp_stats *stats);
+
+/* ---- operator-level entry points (reference: the assembled HYPRE matrix and
+ *      HYPRE's ParCSR SpMV inside HYPRE_ParCSRGMRESSolve, poisson_hypre.c:722-726) */
+
+/* y = A phi on DMDA arrays (phi with periodic ghost images; y written on fluid
+ * cells, zero elsewhere).  drop_small selects the poisson.c:1608 drop rule. */
+int vfs_pp_apply_device(vfs_pp_handle *h, const double *phi, double *y, int drop_small);
+
+/* Row coefficients of PoissonLHSNew for every DMDA point, as the GPU rebuilds them
+ * (19 per point, slot order poisson.c:956-974, before the drop rule), and the
+ * per-face branch codes (18 bytes per point: faces i+,i-,j+,j-,k+,k- x {active,
+ * cross1, cross2}).  Host outputs; for verification. */
+int vfs_pp_export_stencil(vfs_pp_handle *h, double *vol19, uint8_t *cases);
+
+/* IB classification (ibm.c:331-353, 540-568) on a device DMDA array holding 4
+ * for inside cells and 0 elsewhere; rewritten in place to {0,1,3}. */
+int vfs_pp_classify_nvert_device(const vfs_grid *grid, double *nvert);
+
+/* Multigrid transfer primitives of the reference on DMDA arrays, fully coarsened
+ * grids (IM_c=(IM_f+1)/2, init.c:95-127).  MyRestriction poisson.c:364-471,
+ * MyInterpolation poisson.c:526-669, MyNvertRestriction poisson.c:2004-2106.
+ * Device pointers; fine grid described by `fine`. */
+int vfs_pp_mg_restrict_device(const vfs_grid *fine, const double *nvert_f, const double *nvert_c,
+                              const double *xf, double *fc);
+int vfs_pp_mg_prolong_device(const vfs_grid *fine, const double *nvert_f, const double *nvert_c,
+                             const double *xc, double *ff);
+int vfs_pp_mg_nvert_restrict_device(const vfs_grid *fine, const double *nvert_f, double *nvert_c);
+
+/* Time `reps` launches of one engine kernel on the handle's stream with HIP
+ * events; kernel ids: 0 FP64 operator apply, 1 FP32 smoother sweep (finest
+ * level), 2 projection, 3 Contra2Cart, 4 RHS.  Writes the mean duration (ms)
+ * and the algorithmic bytes of one launch. */
+int vfs_pp_time_kernel(vfs_pp_handle *h, int kernel, int reps, double *ms_per_launch,
+                       double *bytes_per_launch);
+
+/* Description of the multigrid hierarchy (levels, sizes, smoother) as text. */
+int vfs_pp_describe(vfs_pp_handle *h, char *buf, size_t len);
+
+const char *vfs_pp_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

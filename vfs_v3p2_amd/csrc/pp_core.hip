@@ -1,0 +1,792 @@
+// Set-up, FP64 operator and pressure-path kernels of the projection engine.
+//
+// Data layout in HBM: solver vectors and per-cell coefficients are dense
+// interior arrays (nx*ny*nz, i fastest) with periodic neighbours reached by
+// index wrap, so no ghost planes need refreshing between sweeps.  Caller fields
+// (Ucont, Phi, P, Ucat, nvert) stay in the reference's DMDA layout and are only
+// touched by the kernels that read or write them in the reference.
+#include "pp_kernels.hpp"
+#include "pp_stencil.hpp"
+
+namespace pp {
+
+namespace {
+
+inline dim3 cell_grid(const Dims& d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
+inline dim3 dmda_grid(const Dmda& m) { return dim3((m.mx + 63) / 64, (m.my + 3) / 4, m.mz); }
+const dim3 kBlk(64, 4, 1);
+
+__device__ inline int blin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
+// periodic image of DMDA point (poisson.c:153-174)
+__device__ inline bool image_of(const Dmda& m, int i, int j, int k, int& a, int& b, int& c, int fl[3]) {
+  a = i; b = j; c = k;
+  fl[0] = fl[1] = fl[2] = 0;
+  if (m.per[0] && i == 0) { a = m.mx - 2; fl[0] = 1; }
+  else if (m.per[0] && i == m.mx - 1) { a = 1; fl[0] = 1; }
+  if (m.per[1] && j == 0) { b = m.my - 2; fl[1] = 1; }
+  else if (m.per[1] && j == m.my - 1) { b = 1; fl[1] = 1; }
+  if (m.per[2] && k == 0) { c = m.mz - 2; fl[2] = 1; }
+  else if (m.per[2] && k == m.mz - 1) { c = 1; fl[2] = 1; }
+  return fl[0] | fl[1] | fl[2];
+}
+
+// ------------------------------------------------------------------------ set-up
+__device__ inline double dot3_exact(const double* a, const double* b) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__global__ __launch_bounds__(256) void k_face_coeffs(Dmda m, Dims d, const double* icsi, const double* ieta,
+                                                     const double* izet, const double* jcsi, const double* jeta,
+                                                     const double* jzet, const double* kcsi, const double* keta,
+                                                     const double* kzet, const double* iaj, const double* jaj,
+                                                     const double* kaj, double* g) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.n(), c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+  // poisson.c:1132-1140, operand order (v.x*w.x + v.y*w.y + v.z*w.z) * aj
+  g[0 * n + c] = dot3_exact(&icsi[3 * p], &icsi[3 * p]) * iaj[p];
+  g[1 * n + c] = dot3_exact(&ieta[3 * p], &icsi[3 * p]) * iaj[p];
+  g[2 * n + c] = dot3_exact(&izet[3 * p], &icsi[3 * p]) * iaj[p];
+  g[3 * n + c] = dot3_exact(&jcsi[3 * p], &jeta[3 * p]) * jaj[p];
+  g[4 * n + c] = dot3_exact(&jeta[3 * p], &jeta[3 * p]) * jaj[p];
+  g[5 * n + c] = dot3_exact(&jzet[3 * p], &jeta[3 * p]) * jaj[p];
+  g[6 * n + c] = dot3_exact(&kcsi[3 * p], &kzet[3 * p]) * kaj[p];
+  g[7 * n + c] = dot3_exact(&keta[3 * p], &kzet[3 * p]) * kaj[p];
+  g[8 * n + c] = dot3_exact(&kzet[3 * p], &kzet[3 * p]) * kaj[p];
+}
+
+__global__ __launch_bounds__(256) void k_codes(Dmda m, Dims d, const double* nv, uint32_t* code, uint8_t* cls) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const int I = i + 1, J = j + 1, K = k + 1;
+  const size_t c = d.idx(i, j, k), p = m.idx(I, J, K);
+  code[c] = make_code(nv, m.mx, m.my, m.mz, m.per, I, J, K);
+  const double v = nv[p];
+  uint8_t b = 0;
+  if (v < 0.1) b |= CLS_LT;
+  if ((int)v > TH) b |= CLS_INTGT;
+  if ((int)(v + 0.1) == 3) b |= CLS_SOLID;
+  if (v > 0.1) b |= CLS_NONFLUID;
+  // Projection face conditions, poisson.c:2221-2222,2259 (and rotations)
+  if ((I < m.mx - 2 || (m.per[0] && I == m.mx - 2)) && (v + nv[p + 1]) < TH) b |= CLS_PROJ_I;
+  if ((J < m.my - 2 || (m.per[1] && J == m.my - 2)) && (v + nv[p + m.mx]) < TH) b |= CLS_PROJ_J;
+  if ((K < m.mz - 2 || (m.per[2] && K == m.mz - 2)) && (v + nv[p + (size_t)m.mx * m.my]) < TH) b |= CLS_PROJ_K;
+  cls[c] = b;
+}
+
+__global__ __launch_bounds__(256) void k_export_rows(Dims d, const double* g, const uint32_t* code, double* vol19) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = d.idx(i, j, k);
+  Nbr nb;
+  neighbours(d, i, j, k, nb);
+  double vol[19];
+  build_row(g, d.n(), code[c], c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+  for (int s = 0; s < 19; s++) vol19[19 * c + s] = vol[s];
+}
+
+// dilation of the IB mask (ibm.c:331-353); writes only 2s, so in-place is race free
+__global__ __launch_bounds__(256) void k_dilate(Dmda m, double* nv) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= m.mx || j >= m.my) return;
+  const size_t c = m.idx(i, j, k);
+  double v = nv[c];
+  if (v < 0) v = 0;
+  const int ip = i < m.mx - 1 ? i + 1 : i, im = i > 0 ? i - 1 : i;
+  const int jp = j < m.my - 1 ? j + 1 : j, jm = j > 0 ? j - 1 : j;
+  const int kp = k < m.mz - 1 ? k + 1 : k, km = k > 0 ? k - 1 : k;
+  if ((int)(v + 0.5) != 4) {
+    bool hit = false;
+    for (int kk = km; kk <= kp; kk++)
+      for (int jj = jm; jj <= jp; jj++)
+        for (int ii = im; ii <= ip; ii++)
+          if ((int)(nv[m.idx(ii, jj, kk)] + 0.5) == 4) hit = true;
+    if (hit) v = v > 2 ? v : 2;
+  }
+  nv[c] = v;
+}
+
+// remap 2->1, 4->3 and wall-model layers (ibm.c:540-568), interior points
+__global__ __launch_bounds__(256) void k_remap(Dmda m, int b0, int b1, int b2, int b3, int b4, int b5, double* nv) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i < 1 || j < 1 || k < 1 || i >= m.mx - 1 || j >= m.my - 1 || k >= m.mz - 1) return;
+  const size_t c = m.idx(i, j, k);
+  double v = nv[c];
+  if ((int)(v + 0.5) == 2) v = 1;
+  if ((int)(v + 0.5) == 4) v = 3;
+  const int bc[6] = {b0, b1, b2, b3, b4, b5};
+  const bool at[6] = {i == 1, i == m.mx - 2, j == 1, j == m.my - 2, k == 1, k == m.mz - 2};
+  for (int f = 0; f < 6; f++)
+    if ((bc[f] == -1 || bc[f] == -2) && at[f]) v = 1;
+  nv[c] = v;
+}
+
+// ------------------------------------------------------------------ operator
+template <typename TX, bool DROP>
+__global__ __launch_bounds__(256) void k_apply64(Dims d, const double* __restrict__ g,
+                                                 const uint32_t* __restrict__ code, const TX* __restrict__ x,
+                                                 double* __restrict__ y) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = d.idx(i, j, k);
+  const uint32_t cd = code[c];
+  Nbr nb;
+  neighbours(d, i, j, k, nb);
+  double acc = 0.;
+  if constexpr (DROP) {
+    // the assembled matrix of the reference: coefficients with |v| <= 1e-10 dropped
+    // (poisson.c:1608); needs the coefficients one by one
+    double vol[19];
+    build_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+#pragma unroll
+    for (int s = 0; s < 19; s++) {
+      double v = vol[s];
+      if (s != CP && !(fabs(v) > 1.e-10)) v = 0.;
+      if (v != 0.) acc += v * (double)x[nb.s[s]];
+    }
+  } else {
+    double xv[19], dg;
+    load_x(nb, x, xv);
+    flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, acc, dg);
+  }
+  y[c] = (cd & ROW_BIT) ? acc : 0.;
+}
+
+template <typename TX>
+__global__ __launch_bounds__(256) void k_residual64(Dims d, const double* __restrict__ g,
+                                                    const uint32_t* __restrict__ code, const double* __restrict__ b,
+                                                    const TX* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ partial) {
+  __shared__ double lds[4];
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  double rr = 0.;
+  if (i < d.nx && j < d.ny) {
+    const size_t c = d.idx(i, j, k);
+    const uint32_t cd = code[c];
+    Nbr nb;
+    neighbours(d, i, j, k, nb);
+    double xv[19], ax, dg;
+    load_x(nb, x, xv);
+    flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+    const double res = (cd & ROW_BIT) ? b[c] - ax : 0.;
+    r[c] = res;
+    rr = res * res;
+  }
+  rr = block_sum(rr, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = rr;
+}
+
+__global__ __launch_bounds__(256) void k_apply_dmda(Dmda m, Dims d, const double* __restrict__ g,
+                                                    const uint32_t* __restrict__ code, const double* __restrict__ phi,
+                                                    double* __restrict__ y, int drop) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+  const uint32_t cd = code[c];
+  if (!(cd & ROW_BIT)) { y[p] = 0.; return; }
+  Nbr nb;
+  neighbours(d, i, j, k, nb);
+  double vol[19];
+  build_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+  double acc = 0.;
+#pragma unroll
+  for (int s = 0; s < 19; s++) {
+    double v = vol[s];
+    if (drop && s != CP && !(fabs(v) > 1.e-10)) v = 0.;
+    const Off3 o = slot_off(s);
+    // DMDA neighbour: periodic images live in the ghost planes
+    if (v != 0.) acc += v * phi[p + o.d[0] + (ptrdiff_t)o.d[1] * m.mx + (ptrdiff_t)o.d[2] * m.mx * m.my];
+  }
+  y[p] = acc;
+}
+
+// ------------------------------------------------------------------------- RHS
+__global__ __launch_bounds__(256) void k_rhs(Dmda m, Dims d, const double* __restrict__ u,
+                                             const uint8_t* __restrict__ cls, double fac, double* __restrict__ b,
+                                             double* __restrict__ partial) {
+  __shared__ double lds[4];
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  double bb = 0.;
+  if (i < d.nx && j < d.ny) {
+    const size_t c = d.idx(i, j, k);
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t p = m.idx(I, J, K);
+    double val = 0.;
+    if (cls[c] & CLS_LT) {  // poisson_hypre.c:346-376
+      val -= u[3 * p + 0];
+      val += (I == 1 && m.per[0]) ? u[3 * m.idx(m.mx - 2, J, K) + 0] : u[3 * (p - 1) + 0];
+      val -= u[3 * p + 1];
+      val += (J == 1 && m.per[1]) ? u[3 * m.idx(I, m.my - 2, K) + 1] : u[3 * (p - m.mx) + 1];
+      val -= u[3 * p + 2];
+      val += (K == 1 && m.per[2]) ? u[3 * m.idx(I, J, m.mz - 2) + 2] : u[3 * (p - (size_t)m.mx * m.my) + 2];
+      val *= fac;
+    }
+    b[c] = val;
+    bb = val * val;
+  }
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = bb;
+}
+
+// ------------------------------------------------------------------ VolumeFlux
+__global__ __launch_bounds__(256) void k_ibflux_sum(const IbFace* __restrict__ f, size_t nf, double* __restrict__ u,
+                                                    int zero_small, double* __restrict__ partial, int nblk) {
+  __shared__ double lds[4];
+  double fl = 0., ar = 0.;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < nf; t += (size_t)nblk * 256) {
+    const IbFace F = f[t];
+    const double v = u[F.pos];
+    if (zero_small) {  // poisson.c:2938-2957: |u| <= 1e-10 faces are zeroed and skipped
+      if (fabs(v) > 1.e-10) { fl += F.sign * v; ar += F.area; }
+      else u[F.pos] = 0.;
+    } else {
+      fl += F.sign * v;
+      ar += F.area;
+    }
+  }
+  fl = block_sum(fl, lds);
+  ar = block_sum(ar, lds);
+  if (threadIdx.x == 0) { partial[blockIdx.x] = fl; partial[nblk + blockIdx.x] = ar; }
+}
+
+__global__ __launch_bounds__(256) void k_ibflux_correct(const IbFace* __restrict__ f, size_t nf,
+                                                        double* __restrict__ u, int immersed,
+                                                        const double* __restrict__ sums) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nf) return;
+  const double area = sums[1];
+  const double corr = area > 1.e-15 ? sums[0] / area : 0.;
+  const IbFace F = f[t];
+  if (immersed == 0) {  // poisson.c:2715-2750: corrected then set to zero
+    u[F.pos] = 0.;
+    return;
+  }
+  const double v = u[F.pos];
+  if (fabs(v) > 1.e-10) u[F.pos] = v - F.sign * (F.area * corr);
+}
+
+// ------------------------------------------------------------ null space, scatter
+__global__ __launch_bounds__(256) void k_masked_sum(Dims d, const double* __restrict__ x,
+                                                    const uint8_t* __restrict__ cls, double* __restrict__ partial,
+                                                    int nblk) {
+  __shared__ double lds[4];
+  const size_t n = d.n();
+  double s = 0.;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)nblk * 256)
+    if (cls[c] & CLS_LT) s += x[c];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sub_mean(Dims d, double* __restrict__ x, const uint8_t* __restrict__ cls,
+                                                  const double* __restrict__ sum, double cnt) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= d.n()) return;
+  const double val = -sum[0] / cnt;  // poisson_hypre.c:256
+  if (cls[c] & CLS_LT) x[c] += val;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_update(Dmda m, Dims d, const double* __restrict__ x,
+                                                        const uint8_t* __restrict__ cls, double* __restrict__ phi,
+                                                        double* __restrict__ p) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = d.idx(i, j, k), q = m.idx(i + 1, j + 1, k + 1);
+  const uint8_t b = cls[c];
+  const double ph = (b & CLS_INTGT) ? 0. : x[c];  // Convert_Phi2_Phi, poisson.c:131-139
+  phi[q] = ph;
+  if (p && (b & CLS_LT)) p[q] += ph;  // UpdatePressure, poisson.c:2548
+}
+
+// UpdatePressure: P += Phi on interior cells with nvert < 0.1 (poisson.c:2548)
+__global__ __launch_bounds__(256) void k_add_phi(Dmda m, Dims d, const uint8_t* __restrict__ cls,
+                                                 const double* __restrict__ phi, double* __restrict__ p) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  if (!(cls[d.idx(i, j, k)] & CLS_LT)) return;
+  const size_t q = m.idx(i + 1, j + 1, k + 1);
+  p[q] += phi[q];
+}
+
+// ghost planes of a DMDA scalar: periodic images; other boundary points are
+// zeroed when zero_other (Convert_Phi2_Phi starts from VecSet(Phi,0))
+__global__ __launch_bounds__(256) void k_periodic_scalar(Dmda m, double* __restrict__ a, int zero_other) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= m.mx || j >= m.my) return;
+  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
+  if (!bnd) return;
+  int aa, bb, cc, fl[3];
+  if (image_of(m, i, j, k, aa, bb, cc, fl)) {
+    // the image of a ghost point with one periodic flag may itself be a non-periodic
+    // boundary point; resolve it the same way
+    const bool ibnd = aa == 0 || bb == 0 || cc == 0 || aa == m.mx - 1 || bb == m.my - 1 || cc == m.mz - 1;
+    if (ibnd && zero_other) a[m.idx(i, j, k)] = 0.;
+    else a[m.idx(i, j, k)] = a[m.idx(aa, bb, cc)];
+  } else if (zero_other) {
+    a[m.idx(i, j, k)] = 0.;
+  }
+}
+
+// ------------------------------------------------------------------ Projection
+// Cross difference of phi for the face (p, q) from its case code (poisson.c:2234-2257).
+__device__ inline double proj_cross(int cc, const double* __restrict__ phi, size_t p, size_t q, size_t se) {
+  switch (cc) {
+    case X_LOWER: return (phi[p] + phi[q] - phi[p - se] - phi[q - se]) * 0.5;
+    case X_UPPER: return (phi[p + se] + phi[q + se] - phi[p] - phi[q]) * 0.5;
+    case X_CENTRAL: return (phi[p + se] + phi[q + se] - phi[p - se] - phi[q - se]) * 0.25;
+    default: return 0.;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_projection(Dmda m, Dims d, const double* __restrict__ g,
+                                                    const uint32_t* __restrict__ code,
+                                                    const uint8_t* __restrict__ cls, const double* __restrict__ phi,
+                                                    double* __restrict__ u, double fac) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.n(), c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+  const uint8_t b = cls[c];
+  if (!(b & (CLS_PROJ_I | CLS_PROJ_J | CLS_PROJ_K))) return;
+  const uint32_t cd = code[c];
+  const size_t sx = 1, sy = m.mx, sz = (size_t)m.mx * m.my;
+  if (b & CLS_PROJ_I) {  // poisson.c:2221-2265; face i+ is face 0, cross j then k
+    const size_t q = p + sx;
+    const double dc = phi[q] - phi[p];
+    const double de = proj_cross((cd >> 6) & 3u, phi, p, q, sy);
+    const double dz = proj_cross((cd >> 8) & 3u, phi, p, q, sz);
+    u[3 * p + 0] -= (dc * g[0 * n + c] + de * g[1 * n + c] + dz * g[2 * n + c]) * fac;
+  }
+  if (b & CLS_PROJ_J) {  // poisson.c:2267-2314; face j+ is face 2, cross i then k
+    const size_t q = p + sy;
+    const double dc = proj_cross((cd >> 14) & 3u, phi, p, q, sx);
+    const double de = phi[q] - phi[p];
+    const double dz = proj_cross((cd >> 16) & 3u, phi, p, q, sz);
+    u[3 * p + 1] -= (dc * g[3 * n + c] + de * g[4 * n + c] + dz * g[5 * n + c]) * fac;
+  }
+  if (b & CLS_PROJ_K) {  // poisson.c:2316-2363; face k+ is face 4, cross i then j
+    const size_t q = p + sz;
+    const double dc = proj_cross((cd >> 22) & 3u, phi, p, q, sx);
+    const double de = proj_cross((cd >> 24) & 3u, phi, p, q, sy);
+    const double dz = phi[q] - phi[p];
+    u[3 * p + 2] -= (dc * g[6 * n + c] + de * g[7 * n + c] + dz * g[8 * n + c]) * fac;
+  }
+}
+
+// per-component periodic ghost faces (poisson.c:2401-2441)
+__global__ __launch_bounds__(256) void k_ucont_ghosts(Dmda m, double* __restrict__ u) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= m.mx || j >= m.my) return;
+  int a, b, c, fl[3];
+  if (!image_of(m, i, j, k, a, b, c, fl)) return;
+  const size_t p = m.idx(i, j, k), s = m.idx(a, b, c);
+  if (fl[0]) u[3 * p + 0] = u[3 * s + 0];
+  if (fl[1]) u[3 * p + 1] = u[3 * s + 1];
+  if (fl[2]) u[3 * p + 2] = u[3 * s + 2];
+}
+
+// ----------------------------------------------------------------- Contra2Cart
+__global__ __launch_bounds__(256) void k_c2c_core(Dmda m, Dims d, const double* __restrict__ met,
+                                                  const uint8_t* __restrict__ cls, const double* __restrict__ u,
+                                                  double* __restrict__ ucat) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.n(), c = d.idx(i, j, k);
+  if (!(cls[c] & CLS_LT)) return;
+  const size_t p = m.idx(i + 1, j + 1, k + 1);
+  const double m00 = met[0 * n + c], m01 = met[1 * n + c], m02 = met[2 * n + c];
+  const double m10 = met[3 * n + c], m11 = met[4 * n + c], m12 = met[5 * n + c];
+  const double m20 = met[6 * n + c], m21 = met[7 * n + c], m22 = met[8 * n + c];
+  // rhs.c:166-189
+  const double q0 = 0.5 * (u[3 * (p - 1) + 0] + u[3 * p + 0]);
+  const double q1 = 0.5 * (u[3 * (p - m.mx) + 1] + u[3 * p + 1]);
+  const double q2 = 0.5 * (u[3 * (p - (size_t)m.mx * m.my) + 2] + u[3 * p + 2]);
+  const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
+  const double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
+  const double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
+  const double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
+  ucat[3 * p + 0] = det0 / det;
+  ucat[3 * p + 1] = det1 / det;
+  ucat[3 * p + 2] = det2 / det;
+}
+
+// whole-vector periodic copy of ghost points (rhs.c:198-241, 645-673); when snap
+// is given the boundary points are also copied there (the snapshot the boundary
+// rules read)
+__global__ __launch_bounds__(256) void k_vec_ghosts(Dmda m, double* __restrict__ a, double* __restrict__ snap) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= m.mx || j >= m.my) return;
+  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
+  if (!bnd) return;
+  const size_t p = m.idx(i, j, k);
+  int aa, bb, cc, fl[3];
+  double v0 = a[3 * p], v1 = a[3 * p + 1], v2 = a[3 * p + 2];
+  if (image_of(m, i, j, k, aa, bb, cc, fl)) {
+    const size_t s = m.idx(aa, bb, cc);
+    v0 = a[3 * s]; v1 = a[3 * s + 1]; v2 = a[3 * s + 2];
+    a[3 * p] = v0; a[3 * p + 1] = v1; a[3 * p + 2] = v2;
+  }
+  if (snap) { snap[3 * p] = v0; snap[3 * p + 1] = v1; snap[3 * p + 2] = v2; }
+}
+
+// boundary rules of Contra2Cart on boundary points (rhs.c:257-260, 491-515);
+// sources on boundary points come from the snapshot, interior sources are
+// untouched until k_c2c_solid runs
+__global__ __launch_bounds__(256) void k_c2c_bc(Dmda m, const double* __restrict__ nv, const double* __restrict__ snap,
+                                                double* __restrict__ ucat, int b0, int b1, int b2, int b3, int b4,
+                                                int b5) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= m.mx || j >= m.my) return;
+  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
+  if (!bnd) return;
+  const size_t p = m.idx(i, j, k);
+  if ((int)(nv[p] + 0.1) == 3) { ucat[3 * p] = ucat[3 * p + 1] = ucat[3 * p + 2] = 0.; return; }
+  auto src = [&](int a, int b, int c) -> const double* {
+    const bool sb = a == 0 || b == 0 || c == 0 || a == m.mx - 1 || b == m.my - 1 || c == m.mz - 1;
+    return (sb ? snap : ucat) + 3 * m.idx(a, b, c);
+  };
+  const double* s = nullptr;
+  if (i == 0 && b0 == 1 && (j != 0 && k != 0)) s = src(i + 1, j, k);
+  if (i == m.mx - 1 && b1 == 1 && (j != 0 && k != 0)) s = src(i - 1, j, k);
+  if (j == 0 && b2 == 1 && (i != 0 && k != 0)) s = src(i, j + 1, k);
+  if (j == m.my - 1 && b3 == 1 && (i != 0 && k != 0)) s = src(i, j - 1, k);
+  if (k == 0 && b4 == 1 && (i != 0 && j != 0)) s = src(i, j, k + 1);
+  if (k == m.mz - 1 && b5 == 1 && (i != 0 && j != 0)) s = src(i, j, k - 1);
+  if (s) {
+    const double v0 = -s[0], v1 = -s[1], v2 = -s[2];
+    ucat[3 * p] = v0; ucat[3 * p + 1] = v1; ucat[3 * p + 2] = v2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_c2c_solid(Dims d, Dmda m, const uint8_t* __restrict__ cls,
+                                                   double* __restrict__ ucat) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  if (!(cls[d.idx(i, j, k)] & CLS_SOLID)) return;
+  const size_t p = m.idx(i + 1, j + 1, k + 1);
+  ucat[3 * p] = ucat[3 * p + 1] = ucat[3 * p + 2] = 0.;
+}
+
+// ------------------------------------------------------------------- Divergence
+__global__ __launch_bounds__(256) void k_divergence(Dmda m, Dims d, const uint32_t* __restrict__ code,
+                                                    const double* __restrict__ aj, const double* __restrict__ u,
+                                                    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  __shared__ double lds[4];
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  double mv = 0.;
+  if (i < d.nx && j < d.ny) {
+    const size_t c = d.idx(i, j, k);
+    if (code[c] & DIVOK_BIT) {
+      const size_t p = m.idx(i + 1, j + 1, k + 1);
+      // main.c:1283 operand order
+      mv = fabs((u[3 * p] - u[3 * (p - 1)] + u[3 * p + 1] - u[3 * (p - m.mx) + 1] + u[3 * p + 2] -
+                 u[3 * (p - (size_t)m.mx * m.my) + 2]) *
+                aj[c]);
+    }
+  }
+  mv = block_max(mv, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = mv;
+}
+
+// ---------------------------------------------------------------- reductions
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partial, int nblk,
+                                                  double* __restrict__ out, int op, double* __restrict__ host) {
+  __shared__ double lds[4];
+  const double* pv = partial + (size_t)blockIdx.x * nblk;
+  double s = 0.;
+  for (int b = threadIdx.x; b < nblk; b += 256) s = op ? fmax(s, pv[b]) : s + pv[b];
+  s = op ? block_max(s, lds) : block_sum(s, lds);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = s;
+    if (host) host[blockIdx.x] = s;  // pinned host mirror, read after the stream syncs
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_multidot(size_t n, const double* __restrict__ w, const double* __restrict__ V,
+                                                  size_t ldv, double* __restrict__ partial, int nblk) {
+  __shared__ double lds[4];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)nblk * 256) {
+    const double wt = w[t];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] += wt * V[(size_t)v * ldv + t];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const double s = block_sum(acc[v], lds);
+    if (threadIdx.x == 0) partial[(size_t)v * nblk + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_axpy_multi(size_t n, double* __restrict__ w, const double* __restrict__ V,
+                                                    size_t ldv, int nv, const double* __restrict__ h, double sign) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  double acc = 0.;
+  for (int v = 0; v < nv; v++) acc += h[v] * V[(size_t)v * ldv + t];
+  w[t] += sign * acc;
+}
+
+__global__ __launch_bounds__(256) void k_norm2(size_t n, const double* __restrict__ a, double* __restrict__ partial,
+                                               int nblk) {
+  __shared__ double lds[4];
+  double s = 0.;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)nblk * 256) s += a[t] * a[t];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_scale_to(size_t n, const double* __restrict__ a,
+                                                  const double* __restrict__ nrm2, double* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const double s = nrm2[0] > 0. ? 1. / sqrt(nrm2[0]) : 0.;
+  out[t] = a[t] * s;
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_update_x(size_t n, double* __restrict__ x, const TZ* __restrict__ Z,
+                                                  size_t ldz, int nz, const double* __restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  double acc = x[t];
+  for (int v = 0; v < nz; v++) acc += y[v] * (double)Z[(size_t)v * ldz + t];
+  x[t] = acc;
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_convert(size_t n, const TA* __restrict__ a, TB* __restrict__ b) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) b[t] = (TB)a[t];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(size_t n, T* a, T v) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int partial_blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_face_coeffs(hipStream_t s, Dmda m, Dims d, const double* icsi, const double* ieta, const double* izet,
+                        const double* jcsi, const double* jeta, const double* jzet, const double* kcsi,
+                        const double* keta, const double* kzet, const double* iaj, const double* jaj,
+                        const double* kaj, double* g) {
+  k_face_coeffs<<<cell_grid(d), kBlk, 0, s>>>(m, d, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj,
+                                              kaj, g);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_codes(hipStream_t s, Dmda m, Dims d, const double* nvert, uint32_t* code, uint8_t* cls) {
+  k_codes<<<cell_grid(d), kBlk, 0, s>>>(m, d, nvert, code, cls);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_export_rows(hipStream_t s, Dims d, const double* g, const uint32_t* code, double* vol19) {
+  k_export_rows<<<cell_grid(d), kBlk, 0, s>>>(d, g, code, vol19);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_classify_nvert(hipStream_t s, Dmda m, const int bc[6], double* nv, uint8_t*) {
+  k_dilate<<<dmda_grid(m), kBlk, 0, s>>>(m, nv);
+  HIPCHECK(hipGetLastError());
+  k_remap<<<dmda_grid(m), kBlk, 0, s>>>(m, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], nv);
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename TX>
+void launch_apply64(hipStream_t s, Dims d, const double* g, const uint32_t* code, const TX* x, double* y, bool drop) {
+  if (drop) k_apply64<TX, true><<<cell_grid(d), kBlk, 0, s>>>(d, g, code, x, y);
+  else k_apply64<TX, false><<<cell_grid(d), kBlk, 0, s>>>(d, g, code, x, y);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_apply64<double>(hipStream_t, Dims, const double*, const uint32_t*, const double*, double*, bool);
+template void launch_apply64<float>(hipStream_t, Dims, const double*, const uint32_t*, const float*, double*, bool);
+
+template <typename TX>
+void launch_residual64(hipStream_t s, Dims d, const double* g, const uint32_t* code, const double* b, const TX* x,
+                       double* r, double* partial, int* nblk) {
+  const dim3 gr = cell_grid(d);
+  *nblk = (int)(gr.x * gr.y * gr.z);
+  k_residual64<TX><<<gr, kBlk, 0, s>>>(d, g, code, b, x, r, partial);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_residual64<double>(hipStream_t, Dims, const double*, const uint32_t*, const double*,
+                                        const double*, double*, double*, int*);
+template void launch_residual64<float>(hipStream_t, Dims, const double*, const uint32_t*, const double*, const float*,
+                                       double*, double*, int*);
+
+void launch_apply_dmda(hipStream_t s, Dmda m, Dims d, const double* g, const uint32_t* code, const double* phi,
+                       double* y, bool drop) {
+  k_apply_dmda<<<cell_grid(d), kBlk, 0, s>>>(m, d, g, code, phi, y, drop ? 1 : 0);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_rhs(hipStream_t s, Dmda m, Dims d, const double* ucont, const uint8_t* cls, double fac, double* b,
+                double* partial, int* nblk) {
+  const dim3 gr = cell_grid(d);
+  *nblk = (int)(gr.x * gr.y * gr.z);
+  k_rhs<<<gr, kBlk, 0, s>>>(m, d, ucont, cls, fac, b, partial);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_ibflux_sum(hipStream_t s, const IbFace* f, size_t nf, double* ucont, int immersed, int zero_small,
+                       double* partial, int* nblk) {
+  const int nb = partial_blocks(nf);
+  *nblk = nb;
+  k_ibflux_sum<<<nb, 256, 0, s>>>(f, nf, ucont, (immersed == 1 && zero_small) ? 1 : 0, partial, nb);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_ibflux_correct(hipStream_t s, const IbFace* f, size_t nf, double* ucont, int immersed, const double* sums) {
+  if (nf == 0) return;
+  k_ibflux_correct<<<nb1(nf), 256, 0, s>>>(f, nf, ucont, immersed, sums);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_masked_sum(hipStream_t s, Dims d, const double* x, const uint8_t* cls, double* partial, int* nblk) {
+  const int nb = partial_blocks(d.n());
+  *nblk = nb;
+  k_masked_sum<<<nb, 256, 0, s>>>(d, x, cls, partial, nb);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_sub_mean(hipStream_t s, Dims d, double* x, const uint8_t* cls, const double* sum, double cnt) {
+  k_sub_mean<<<nb1(d.n()), 256, 0, s>>>(d, x, cls, sum, cnt);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_scatter_update(hipStream_t s, Dmda m, Dims d, const double* x, const uint8_t* cls, double* phi, double* p) {
+  k_scatter_update<<<cell_grid(d), kBlk, 0, s>>>(m, d, x, cls, phi, p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_add_phi(hipStream_t s, Dmda m, Dims d, const uint8_t* cls, const double* phi, double* p) {
+  k_add_phi<<<cell_grid(d), kBlk, 0, s>>>(m, d, cls, phi, p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_periodic_scalar(hipStream_t s, Dmda m, double* a, int zero_other) {
+  k_periodic_scalar<<<dmda_grid(m), kBlk, 0, s>>>(m, a, zero_other);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_projection(hipStream_t s, Dmda m, Dims d, const double* g, const uint32_t* code, const uint8_t* cls,
+                       const double* phi, double* ucont, double fac) {
+  k_projection<<<cell_grid(d), kBlk, 0, s>>>(m, d, g, code, cls, phi, ucont, fac);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_ucont_ghosts(hipStream_t s, Dmda m, double* ucont) {
+  if (!(m.per[0] || m.per[1] || m.per[2])) return;
+  k_ucont_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucont);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_c2c(hipStream_t s, Dmda m, Dims d, const double* met9, const uint8_t* cls, const double* nvert,
+                const double* ucont, double* ucat, double* scratch, const int bc[6]) {
+  k_c2c_core<<<cell_grid(d), kBlk, 0, s>>>(m, d, met9, cls, ucont, ucat);
+  HIPCHECK(hipGetLastError());
+  k_vec_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucat, scratch);
+  HIPCHECK(hipGetLastError());
+  k_c2c_bc<<<dmda_grid(m), kBlk, 0, s>>>(m, nvert, scratch, ucat, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+  HIPCHECK(hipGetLastError());
+  k_c2c_solid<<<cell_grid(d), kBlk, 0, s>>>(d, m, cls, ucat);
+  HIPCHECK(hipGetLastError());
+  if (m.per[0] || m.per[1] || m.per[2]) {
+    k_vec_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucat, nullptr);
+    HIPCHECK(hipGetLastError());
+  }
+}
+
+void launch_divergence(hipStream_t s, Dmda m, Dims d, const uint32_t* code, const double* aj, const double* ucont,
+                       double* partial, int* nblk) {
+  const dim3 gr = cell_grid(d);
+  *nblk = (int)(gr.x * gr.y * gr.z);
+  k_divergence<<<gr, kBlk, 0, s>>>(m, d, code, aj, ucont, partial);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_finalize(hipStream_t s, const double* partial, int nvec, int nblk, double* out, int op,
+                     double* host_mirror) {
+  k_finalize<<<nvec, 256, 0, s>>>(partial, nblk, out, op, host_mirror);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_multidot(hipStream_t s, size_t n, const double* w, const double* V, size_t ldv, int nv, double* partial,
+                     int* nblk) {
+  const int nb = partial_blocks(n);
+  *nblk = nb;
+  int done = 0;
+  while (done < nv) {
+    const int left = nv - done;
+    const double* Vp = V + (size_t)done * ldv;
+    double* pp = partial + (size_t)done * nb;
+    if (left >= 8) { k_multidot<8><<<nb, 256, 0, s>>>(n, w, Vp, ldv, pp, nb); done += 8; }
+    else if (left >= 4) { k_multidot<4><<<nb, 256, 0, s>>>(n, w, Vp, ldv, pp, nb); done += 4; }
+    else if (left >= 2) { k_multidot<2><<<nb, 256, 0, s>>>(n, w, Vp, ldv, pp, nb); done += 2; }
+    else { k_multidot<1><<<nb, 256, 0, s>>>(n, w, Vp, ldv, pp, nb); done += 1; }
+    HIPCHECK(hipGetLastError());
+  }
+}
+
+void launch_axpy_multi(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                       double sign) {
+  k_axpy_multi<<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, sign);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_norm2(hipStream_t s, size_t n, const double* a, double* partial, int* nblk) {
+  const int nb = partial_blocks(n);
+  *nblk = nb;
+  k_norm2<<<nb, 256, 0, s>>>(n, a, partial, nb);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_scale_to(hipStream_t s, size_t n, const double* a, const double* norm2, double* out) {
+  k_scale_to<<<nb1(n), 256, 0, s>>>(n, a, norm2, out);
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename TZ>
+void launch_update_x(hipStream_t s, size_t n, double* x, const TZ* Z, size_t ldz, int nz, const double* y) {
+  k_update_x<TZ><<<nb1(n), 256, 0, s>>>(n, x, Z, ldz, nz, y);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_update_x<float>(hipStream_t, size_t, double*, const float*, size_t, int, const double*);
+template void launch_update_x<double>(hipStream_t, size_t, double*, const double*, size_t, int, const double*);
+
+template <typename TA, typename TB>
+void launch_convert(hipStream_t s, size_t n, const TA* a, TB* b) {
+  k_convert<TA, TB><<<nb1(n), 256, 0, s>>>(n, a, b);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_convert<double, float>(hipStream_t, size_t, const double*, float*);
+template void launch_convert<float, double>(hipStream_t, size_t, const float*, double*);
+template void launch_convert<double, double>(hipStream_t, size_t, const double*, double*);
+
+void launch_fill(hipStream_t s, size_t n, double* a, double v) {
+  k_fill<double><<<nb1(n), 256, 0, s>>>(n, a, v);
+  HIPCHECK(hipGetLastError());
+}
+void launch_fillf(hipStream_t s, size_t n, float* a, float v) {
+  k_fill<float><<<nb1(n), 256, 0, s>>>(n, a, v);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace pp

@@ -1,0 +1,1206 @@
+// Host side of the pressure-projection engine: device-resident case data, the
+// geometric multigrid hierarchy, the FP64 FGMRES outer loop and the C ABI of
+// include/vfs_pp.h.
+//
+// Solve structure (one call of vfs_pp_project_device):
+//   VolumeFlux on a compact list of IB faces -> RHS b (FP64) -> FGMRES(m) in FP64
+//   with CGS2 orthogonalisation, right-preconditioned by one V-cycle of a
+//   geometric multigrid in FP32 (mixed) or FP64 -> Remove_Nullspace -> scatter
+//   phi to the DMDA array and P += phi -> Projection -> Contra2Cart ->
+//   Divergence.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pp_kernels.hpp"
+#include "pp_stencil.hpp"
+#include "vfs_pp.h"
+
+namespace pp {
+
+// --------------------------------------------------------------------- memory
+class Arena {
+ public:
+  ~Arena() { release(); }
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    // every buffer starts at zero so no result can depend on earlier contents of
+    // the device heap (set-up time, not on the solve path)
+    e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      throw StatusError(VFS_PP_EHIP, "hipMemset failed");
+    }
+    ptrs_.push_back(p);
+    total_ += bytes;
+    return static_cast<T*>(p);
+  }
+  void release() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+    total_ = 0;
+  }
+  size_t total() const { return total_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t total_ = 0;
+};
+
+template <typename T>
+static void h2d(hipStream_t s, T* d, const T* h, size_t n) {
+  HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+static void d2h(hipStream_t s, T* h, const T* d, size_t n) {
+  HIPCHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// ------------------------------------------------------------- host hierarchy
+struct HostLevel {
+  Dims d{};
+  std::vector<double> g;       // 9*n face coefficients
+  std::vector<double> nv;      // DMDA nvert of this level (with ghost planes)
+  std::vector<uint32_t> code;  // n
+  std::vector<uint8_t> cls;    // n, CLS_NONFLUID only on coarse levels
+  int cmask = 0;               // axes coarsened towards the next level
+};
+
+static Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {d.per[0], d.per[1], d.per[2]}}; }
+
+static void periodic_fill_host(const Dmda& m, std::vector<double>& a) {
+  for (int k = 0; k < m.mz; k++)
+    for (int j = 0; j < m.my; j++)
+      for (int i = 0; i < m.mx; i++) {
+        int aa = i, bb = j, cc = k;
+        bool f = false;
+        if (m.per[0] && i == 0) aa = m.mx - 2, f = true;
+        else if (m.per[0] && i == m.mx - 1) aa = 1, f = true;
+        if (m.per[1] && j == 0) bb = m.my - 2, f = true;
+        else if (m.per[1] && j == m.my - 1) bb = 1, f = true;
+        if (m.per[2] && k == 0) cc = m.mz - 2, f = true;
+        else if (m.per[2] && k == m.mz - 1) cc = 1, f = true;
+        if (f) a[m.idx(i, j, k)] = a[m.idx(aa, bb, cc)];
+      }
+}
+
+static void codes_host(HostLevel& L) {
+  const Dmda m = dmda_of(L.d);
+  L.code.assign(L.d.n(), 0u);
+  L.cls.assign(L.d.n(), 0);
+  for (int k = 0; k < L.d.nz; k++)
+    for (int j = 0; j < L.d.ny; j++)
+      for (int i = 0; i < L.d.nx; i++) {
+        const size_t c = L.d.idx(i, j, k);
+        L.code[c] = make_code(L.nv.data(), m.mx, m.my, m.mz, m.per, i + 1, j + 1, k + 1);
+        if (L.nv[m.idx(i + 1, j + 1, k + 1)] > 0.1) L.cls[c] = CLS_NONFLUID;
+      }
+}
+
+static void row_of(const HostLevel& L, int i, int j, int k, double (&vol)[19], Nbr& nb) {
+  neighbours(L.d, i, j, k, nb);
+  const size_t c = L.d.idx(i, j, k);
+  build_row(L.g.data(), L.d.n(), L.code[c], c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+}
+
+// Next coarser level: cell-centred 2:1 coarsening per axis with n >= 3 (odd
+// counts leave a one-child cell at the high end), coarse nvert by the
+// MyNvertRestriction rule (poisson.c:2042-2097: IB when every child is, then
+// isolated fluid cells), coarse face coefficients summed over the active fine
+// faces that make up each coarse face and divided by the coarsening factor of
+// the difference direction.
+static HostLevel coarsen(HostLevel& F) {
+  HostLevel C;
+  const int cm = (F.d.nx >= 3 ? 1 : 0) | (F.d.ny >= 3 ? 2 : 0) | (F.d.nz >= 3 ? 4 : 0);
+  F.cmask = cm;
+  C.d = F.d;
+  if (cm & 1) C.d.nx = (F.d.nx + 1) / 2;
+  if (cm & 2) C.d.ny = (F.d.ny + 1) / 2;
+  if (cm & 4) C.d.nz = (F.d.nz + 1) / 2;
+  const Dmda mf = dmda_of(F.d), mc = dmda_of(C.d);
+  const size_t nc = C.d.n(), nf = F.d.n();
+  auto ch = [&](int axis, int ci, int& lo, int& hi) {
+    const int n = axis == 0 ? F.d.nx : axis == 1 ? F.d.ny : F.d.nz;
+    if (cm & (1 << axis)) { lo = 2 * ci; hi = std::min(2 * ci + 1, n - 1); }
+    else { lo = hi = ci; }
+  };
+  // nvert
+  C.nv.assign(mc.n(), 0.);
+  for (int k = 0; k < C.d.nz; k++)
+    for (int j = 0; j < C.d.ny; j++)
+      for (int i = 0; i < C.d.nx; i++) {
+        int i0, i1, j0, j1, k0, k1;
+        ch(0, i, i0, i1); ch(1, j, j0, j1); ch(2, k, k0, k1);
+        double prod = 1.;
+        for (int kk = k0; kk <= k1; kk++)
+          for (int jj = j0; jj <= j1; jj++)
+            for (int ii = i0; ii <= i1; ii++) prod *= F.nv[mf.idx(ii + 1, jj + 1, kk + 1)];
+        if (prod > 0.1) C.nv[mc.idx(i + 1, j + 1, k + 1)] = 1.;
+      }
+  periodic_fill_host(mc, C.nv);
+  {
+    std::vector<double> pre = C.nv;
+    const size_t sy = mc.mx, sz = (size_t)mc.mx * mc.my;
+    for (int k = 1; k < mc.mz - 1; k++)
+      for (int j = 1; j < mc.my - 1; j++)
+        for (int i = 1; i < mc.mx - 1; i++) {
+          const size_t p = mc.idx(i, j, k);
+          if (pre[p] < 0.1 && pre[p + 1] + pre[p - 1] > 1.1 && pre[p + sy] + pre[p - sy] > 1.1 &&
+              pre[p + sz] + pre[p - sz] > 1.1)
+            C.nv[p] = 1.;
+        }
+    periodic_fill_host(mc, C.nv);
+  }
+  // face coefficients
+  C.g.assign(9 * nc, 0.);
+  const int fac[3] = {(cm & 1) ? 2 : 1, (cm & 2) ? 2 : 1, (cm & 4) ? 2 : 1};
+  for (int k = 0; k < C.d.nz; k++)
+    for (int j = 0; j < C.d.ny; j++)
+      for (int i = 0; i < C.d.nx; i++) {
+        const size_t cc = C.d.idx(i, j, k);
+        int r[3][2];
+        ch(0, i, r[0][0], r[0][1]);
+        ch(1, j, r[1][0], r[1][1]);
+        ch(2, k, r[2][0], r[2][1]);
+        for (int d = 0; d < 3; d++) {
+          int lo[3] = {r[0][0], r[1][0], r[2][0]}, hi[3] = {r[0][1], r[1][1], r[2][1]};
+          lo[d] = hi[d];  // the fine faces on the coarse upper face belong to the last child
+          double acc[3] = {0., 0., 0.};
+          for (int kk = lo[2]; kk <= hi[2]; kk++)
+            for (int jj = lo[1]; jj <= hi[1]; jj++)
+              for (int ii = lo[0]; ii <= hi[0]; ii++) {
+                const size_t fc = F.d.idx(ii, jj, kk);
+                if (!((F.code[fc] >> (2 * d)) & 1u)) continue;
+                for (int e = 0; e < 3; e++) acc[e] += F.g[(size_t)(3 * d + e) * nf + fc];
+              }
+          for (int e = 0; e < 3; e++) C.g[(size_t)(3 * d + e) * nc + cc] = acc[e] / fac[e];
+        }
+      }
+  codes_host(C);
+  // a coarse row whose faces all vanish cannot be smoothed: treat it as solid
+  for (int it = 0; it < 3; it++) {
+    bool changed = false;
+    for (int k = 0; k < C.d.nz; k++)
+      for (int j = 0; j < C.d.ny; j++)
+        for (int i = 0; i < C.d.nx; i++) {
+          const size_t c = C.d.idx(i, j, k);
+          if (!(C.code[c] & ROW_BIT)) continue;
+          double vol[19];
+          Nbr nb;
+          row_of(C, i, j, k, vol, nb);
+          if (vol[CP] == 0.) { C.nv[mc.idx(i + 1, j + 1, k + 1)] = 1.; changed = true; }
+        }
+    if (!changed) break;
+    periodic_fill_host(mc, C.nv);
+    codes_host(C);
+  }
+  return C;
+}
+
+// Bordered inverse of the coarsest operator: [A E; E^T 0]^-1, top-left block,
+// where E has one indicator column per connected fluid component (the
+// constants of each component span the null space of the Neumann problem).
+static std::vector<double> coarse_inverse(const HostLevel& L, int* ncomp_out) {
+  const int n = (int)L.d.n();
+  std::vector<int> comp(n, -1);
+  std::vector<std::vector<std::pair<int, double>>> rows(n);
+  for (int k = 0; k < L.d.nz; k++)
+    for (int j = 0; j < L.d.ny; j++)
+      for (int i = 0; i < L.d.nx; i++) {
+        const int c = (int)L.d.idx(i, j, k);
+        if (!(L.code[c] & ROW_BIT)) continue;
+        double vol[19];
+        Nbr nb;
+        row_of(L, i, j, k, vol, nb);
+        for (int s = 0; s < 19; s++)
+          if (vol[s] != 0.) rows[c].push_back({(int)nb.s[s], vol[s]});
+      }
+  int ncomp = 0;
+  for (int c = 0; c < n; c++) {
+    if (!(L.code[c] & ROW_BIT) || comp[c] >= 0) continue;
+    std::vector<int> st{c};
+    comp[c] = ncomp;
+    while (!st.empty()) {
+      int u = st.back();
+      st.pop_back();
+      for (auto& e : rows[u])
+        if (e.first != u && (L.code[e.first] & ROW_BIT) && comp[e.first] < 0) {
+          comp[e.first] = ncomp;
+          st.push_back(e.first);
+        }
+    }
+    ncomp++;
+  }
+  const int N = n + ncomp;
+  std::vector<double> B((size_t)N * N, 0.), I((size_t)N * N, 0.);
+  for (int c = 0; c < n; c++) {
+    if (!(L.code[c] & ROW_BIT)) { B[(size_t)c * N + c] = 1.; continue; }
+    for (auto& e : rows[c]) B[(size_t)c * N + e.first] += e.second;
+    B[(size_t)c * N + n + comp[c]] = 1.;
+    B[(size_t)(n + comp[c]) * N + c] = 1.;
+  }
+  for (int r = 0; r < N; r++) I[(size_t)r * N + r] = 1.;
+  // Gauss-Jordan with partial pivoting
+  for (int col = 0; col < N; col++) {
+    int piv = col;
+    double best = std::fabs(B[(size_t)col * N + col]);
+    for (int r = col + 1; r < N; r++) {
+      const double v = std::fabs(B[(size_t)r * N + col]);
+      if (v > best) { best = v; piv = r; }
+    }
+    if (best == 0.) throw StatusError(VFS_PP_EINVAL, "coarsest operator is singular beyond its null space");
+    if (piv != col)
+      for (int t = 0; t < N; t++) {
+        std::swap(B[(size_t)piv * N + t], B[(size_t)col * N + t]);
+        std::swap(I[(size_t)piv * N + t], I[(size_t)col * N + t]);
+      }
+    const double inv = 1. / B[(size_t)col * N + col];
+    for (int t = 0; t < N; t++) { B[(size_t)col * N + t] *= inv; I[(size_t)col * N + t] *= inv; }
+    for (int r = 0; r < N; r++) {
+      if (r == col) continue;
+      const double f = B[(size_t)r * N + col];
+      if (f == 0.) continue;
+      for (int t = 0; t < N; t++) {
+        B[(size_t)r * N + t] -= f * B[(size_t)col * N + t];
+        I[(size_t)r * N + t] -= f * I[(size_t)col * N + t];
+      }
+    }
+  }
+  std::vector<double> out((size_t)n * n);
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) out[(size_t)c * n + r] = I[(size_t)r * N + c];  // column-major
+  *ncomp_out = ncomp;
+  return out;
+}
+
+// ----------------------------------------------------------- device multigrid
+struct MGBase {
+  virtual ~MGBase() = default;
+  virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
+  virtual void* f0() = 0;
+  virtual void set_out(void* out) = 0;
+  virtual std::string describe() const = 0;
+  virtual size_t fine_bytes_per_sweep() const = 0;
+  virtual void time_sweep(hipStream_t s) = 0;
+};
+
+template <typename T>
+struct MG : MGBase {
+  struct Lev {
+    Dims d{};
+    T* g = nullptr;
+    const uint32_t* code = nullptr;
+    const uint8_t* cls = nullptr;
+    T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr;
+    int line = 0;
+    std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
+    double lam = 0.;
+    LevelView<T> view() const { return LevelView<T>{d, g, code, cls}; }
+  };
+  std::vector<Lev> lev;
+  T* scr = nullptr;
+  T* cinv = nullptr;
+  int ncoarse = 0, ncomp = 0;
+  T* out = nullptr;
+  int degree = 2;
+
+  void* f0() override { return lev[0].f; }
+  void set_out(void* o) override { out = static_cast<T*>(o); }
+
+  // Chebyshev coefficients on [lo*lam, hi*lam]
+  static void cheb(double lam, int deg, std::vector<T>& c1, std::vector<T>& c2) {
+    const double a = 0.3 * lam, b = 1.1 * lam;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double rho = 1. / sigma;
+    c1.assign(deg, T(0));
+    c2.assign(deg, T(0));
+    c2[0] = T(1. / theta);
+    for (int k = 1; k < deg; k++) {
+      const double rn = 1. / (2. * sigma - rho);
+      c1[k] = T(rn * rho);
+      c2[k] = T(2. * rn / delta);
+      rho = rn;
+    }
+  }
+
+  void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out) {
+    for (int k = 0; k < degree; k++) {
+      const bool zg = zero_start && k == 0;
+      T* xin = x;
+      T* xout = (x == L.xa) ? L.xb : L.xa;
+      if (k == degree - 1 && final_out) xout = final_out;
+      launch_smooth<T>(s, L.view(), L.line, zg, L.f, xin, xout, L.dv, scr, k == 0 ? T(0) : L.c1[k], L.c2[k]);
+      x = xout;
+    }
+  }
+
+  void vcycle(hipStream_t s) override { cycle(s, 0, out); }
+
+  // returns the buffer holding this level's result
+  T* cycle(hipStream_t s, size_t l, T* final_out) {
+    Lev& L = lev[l];
+    if (l + 1 == lev.size()) {
+      T* x = final_out ? final_out : L.xa;
+      launch_dense_solve<T>(s, ncoarse, cinv, L.f, x);
+      return x;
+    }
+    T* x = L.xa;
+    smooth(s, L, true, x, nullptr);
+    Lev& C = lev[l + 1];
+    launch_resid_restrict<T>(s, L.view(), L.f, x, C.d, C.cls, C.f);
+    T* xc = cycle(s, l + 1, nullptr);
+    launch_prolong_add<T>(s, L.view(), C.d, C.cls, xc, x);
+    smooth(s, L, false, x, final_out);
+    return x;
+  }
+
+  std::string describe() const override {
+    std::ostringstream o;
+    for (size_t l = 0; l < lev.size(); l++) {
+      o << "level " << l << ": " << lev[l].d.nx << "x" << lev[l].d.ny << "x" << lev[l].d.nz;
+      if (l + 1 == lev.size()) o << " dense(" << ncoarse << ", components " << ncomp << ")";
+      else o << (lev[l].line ? " y-line" : " point") << " chebyshev deg " << degree << " lambda " << lev[l].lam;
+      o << "\n";
+    }
+    return o.str();
+  }
+
+  size_t fine_bytes_per_sweep() const override {
+    // g 9 + f + x_old + x_new + d(read, write) in T, code 4
+    const size_t n = lev[0].d.n();
+    size_t b = n * (9 * sizeof(T) + 4 + 5 * sizeof(T));
+    if (lev[0].line) b += n * 4 * sizeof(T);  // Thomas factors written and read back
+    return b;
+  }
+
+  void time_sweep(hipStream_t s) override {
+    Lev& L = lev[0];
+    launch_smooth<T>(s, L.view(), L.line, false, L.f, L.xa, L.xb, L.dv, scr, L.c1.size() > 1 ? L.c1[1] : T(0),
+                     L.c2[0]);
+  }
+};
+
+// --------------------------------------------------------------------- engine
+}  // namespace pp
+
+struct vfs_pp_handle {
+  vfs_grid grid{};
+  vfs_pp_opts opts{};
+  pp::Dmda m{};
+  pp::Dims d{};
+  hipStream_t s = nullptr;
+  pp::Arena ar;
+  bool have_geom = false;
+  // fine-level resident data
+  double* g64 = nullptr;
+  uint32_t* code = nullptr;
+  uint8_t* cls = nullptr;
+  double* met9 = nullptr;
+  double* aj = nullptr;
+  double* nvert = nullptr;
+  pp::IbFace* ibf = nullptr;
+  size_t nibf = 0;
+  double n_lt = 0.;
+  // vectors
+  double *x = nullptr, *b = nullptr, *r = nullptr;
+  double* partial = nullptr;
+  double* red = nullptr;   // device scratch scalars (>= 256)
+  double* snap = nullptr;  // DMDA vector scratch for Contra2Cart
+  double* V = nullptr;
+  void* Z = nullptr;
+  int mrest = 0;
+  std::unique_ptr<pp::MGBase> mg;
+  std::vector<pp::HostLevel> hl;
+  double* hpin = nullptr;  // pinned host scratch
+  double* hpin_dev = nullptr;  // the same memory as seen by kernels
+  std::string desc;
+};
+
+namespace pp {
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// power iteration for lambda_max of M^-1 A on one level
+template <typename T>
+static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& L) {
+  const size_t n = L.d.n();
+  std::vector<T> v0(n);
+  std::mt19937 rng(1234);
+  std::uniform_real_distribution<double> U(-1., 1.);
+  std::vector<uint32_t> code(n);
+  d2h(h->s, code.data(), L.code, n);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  for (size_t i = 0; i < n; i++) v0[i] = (code[i] & ROW_BIT) ? T(U(rng)) : T(0);
+  h2d(h->s, L.xa, v0.data(), n);
+  if (std::is_same<T, float>::value) launch_fillf(h->s, n, (float*)L.f, 0.f);
+  else launch_fill(h->s, n, (double*)L.f, 0.);
+  int nb = 0;
+  launch_normsq<T>(h->s, n, L.xa, h->partial, &nb);
+  launch_finalize(h->s, h->partial, 1, nb, h->red, 0);
+  launch_scale_by<T>(h->s, n, L.xa, h->red);
+  double lam = 0., best = 0.;
+  for (int it = 0; it < 12; it++) {
+    launch_smooth<T>(h->s, L.view(), L.line, false, L.f, L.xa, L.xb, L.dv, mg.scr, T(0), T(1));
+    launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
+    launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+    HIPCHECK(hipStreamSynchronize(h->s));
+    lam = std::sqrt(h->hpin[0]);
+    if (it >= 6) best = std::max(best, lam);
+    HIPCHECK(hipMemcpyAsync(L.xa, L.dv, n * sizeof(T), hipMemcpyDeviceToDevice, h->s));
+    launch_scale_by<T>(h->s, n, L.xa, h->red);
+  }
+  return best > 0. ? best : 2.;
+}
+
+template <typename T>
+static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
+  auto mg = std::make_unique<MG<T>>();
+  mg->degree = std::max(1, h->opts.cheb_degree);
+  const size_t n0 = h->d.n();
+  mg->scr = h->ar.alloc<T>(2 * n0);
+  for (size_t l = 0; l < h->hl.size(); l++) {
+    HostLevel& H = h->hl[l];
+    typename MG<T>::Lev L;
+    L.d = H.d;
+    const size_t n = H.d.n();
+    if (l == 0) {
+      if (std::is_same<T, double>::value) L.g = (T*)h->g64;
+      else {
+        L.g = h->ar.alloc<T>(9 * n);
+        launch_convert<double, T>(h->s, 9 * n, h->g64, L.g);
+      }
+      L.code = h->code;
+      uint8_t* cls = h->ar.alloc<uint8_t>(n);
+      std::vector<uint8_t> c(n);
+      for (size_t i = 0; i < n; i++) c[i] = (H.code[i] & ROW_BIT) ? 0 : CLS_NONFLUID;
+      h2d(h->s, cls, c.data(), n);
+      L.cls = cls;
+      HIPCHECK(hipStreamSynchronize(h->s));
+    } else {
+      std::vector<T> gt(9 * n);
+      for (size_t i = 0; i < 9 * n; i++) gt[i] = T(H.g[i]);
+      L.g = h->ar.alloc<T>(9 * n);
+      h2d(h->s, L.g, gt.data(), 9 * n);
+      uint32_t* code = h->ar.alloc<uint32_t>(n);
+      h2d(h->s, code, H.code.data(), n);
+      uint8_t* cls = h->ar.alloc<uint8_t>(n);
+      h2d(h->s, cls, H.cls.data(), n);
+      L.code = code;
+      L.cls = cls;
+      HIPCHECK(hipStreamSynchronize(h->s));
+    }
+    L.f = h->ar.alloc<T>(n);
+    L.xa = h->ar.alloc<T>(n);
+    L.xb = h->ar.alloc<T>(n);
+    L.dv = h->ar.alloc<T>(n);
+    // smoother: y-line when the wall-normal coupling dominates (and j is not periodic)
+    int line = 0;
+    if (h->opts.smoother == 2) line = 1;
+    else if (h->opts.smoother == 0 && !H.d.per[1]) {
+      double worst = 0.;
+      for (int k = 0; k < H.d.nz; k++)
+        for (int j = 0; j < H.d.ny; j++)
+          for (int i = 0; i < H.d.nx; i++) {
+            const size_t c = H.d.idx(i, j, k);
+            if (!(H.code[c] & ROW_BIT)) continue;
+            double vol[19];
+            Nbr nb;
+            row_of(H, i, j, k, vol, nb);
+            const double yc = std::fabs(vol[NP]) + std::fabs(vol[SP]);
+            const double oc = std::fabs(vol[EP]) + std::fabs(vol[WP]) + std::fabs(vol[TP]) + std::fabs(vol[BP]);
+            if (oc > 0.) worst = std::max(worst, yc / oc);
+          }
+      line = worst > 4.0 ? 1 : 0;
+    }
+    if (H.d.per[1]) line = 0;
+    L.line = line;
+    mg->lev.push_back(L);
+  }
+  // Chebyshev parameters per smoothed level
+  for (size_t l = 0; l + 1 < mg->lev.size(); l++) {
+    auto& L = mg->lev[l];
+    L.lam = estimate_lambda<T>(h, *mg, L);
+    MG<T>::cheb(L.lam, mg->degree, L.c1, L.c2);
+  }
+  // coarsest dense inverse
+  HostLevel& Hc = h->hl.back();
+  int ncomp = 0;
+  std::vector<double> ci = coarse_inverse(Hc, &ncomp);
+  std::vector<T> cit(ci.size());
+  for (size_t i = 0; i < ci.size(); i++) cit[i] = T(ci[i]);
+  mg->ncoarse = (int)Hc.d.n();
+  mg->ncomp = ncomp;
+  mg->cinv = h->ar.alloc<T>(cit.size());
+  h2d(h->s, mg->cinv, cit.data(), cit.size());
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return mg;
+}
+
+static double finalize_to_host(vfs_pp_handle* h, int nvec, int nblk, int op, double* out) {
+  launch_finalize(h->s, h->partial, nvec, nblk, h->red, op, h->hpin_dev);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  for (int v = 0; v < nvec; v++) out[v] = h->hpin[v];
+  return out[0];
+}
+
+// FGMRES(m) with CGS2; right preconditioner = one V-cycle.  Solves A x = b
+// (x holds the initial guess).  Returns iterations; fills rel.
+template <typename TZ>
+static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bool* conv) {
+  const size_t n = h->d.n();
+  const int m = h->mrest;
+  const double rtol = h->opts.rtol;
+  TZ* Z = static_cast<TZ*>(h->Z);
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hcol(m + 2);
+  int it = 0;
+  double rel = 1.;
+  *conv = false;
+  const double tgt = rtol * bnorm;
+  while (true) {
+    int nb = 0;
+    launch_residual64<double>(h->s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
+    double beta2;
+    finalize_to_host(h, 1, nb, 0, &beta2);
+    const double beta = std::sqrt(beta2);
+    rel = bnorm > 0. ? beta / bnorm : 0.;
+    if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
+    if (beta <= tgt) { *conv = true; break; }
+    if (it >= h->opts.max_iters) break;
+    launch_scale_to(h->s, n, h->r, h->red, h->V);  // red[0] still holds beta^2
+    std::fill(gv.begin(), gv.end(), 0.);
+    gv[0] = beta;
+    int j = 0;
+    for (; j < m && it < h->opts.max_iters; j++) {
+      double* vj = h->V + (size_t)j * n;
+      double* w = h->V + (size_t)(j + 1) * n;
+      // z_j = M^-1 v_j
+      if (std::is_same<TZ, float>::value) launch_convert<double, float>(h->s, n, vj, (float*)h->mg->f0());
+      else HIPCHECK(hipMemcpyAsync(h->mg->f0(), vj, n * sizeof(double), hipMemcpyDeviceToDevice, h->s));
+      h->mg->set_out(Z + (size_t)j * n);
+      h->mg->vcycle(h->s);
+      (*vcyc)++;
+      launch_apply64<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->opts.drop_small != 0);
+      // CGS2
+      for (int pass = 0; pass < 2; pass++) {
+        launch_multidot(h->s, n, w, h->V, n, j + 1, h->partial, &nb);
+        launch_finalize(h->s, h->partial, j + 1, nb, h->red + 8 + pass * 64, 0, h->hpin_dev + 8 + pass * 64);
+        launch_axpy_multi(h->s, n, w, h->V, n, j + 1, h->red + 8 + pass * 64, -1.);
+      }
+      launch_norm2(h->s, n, w, h->partial, &nb);
+      launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+      launch_scale_to(h->s, n, w, h->red, w);
+      HIPCHECK(hipStreamSynchronize(h->s));
+      for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[8 + 64 + l];
+      H[(size_t)(j + 1) * m + j] = std::sqrt(h->hpin[0]);
+      for (int l = 0; l < j; l++) {
+        const double t1 = cs[l] * H[(size_t)l * m + j] + sn[l] * H[(size_t)(l + 1) * m + j];
+        const double t2 = -sn[l] * H[(size_t)l * m + j] + cs[l] * H[(size_t)(l + 1) * m + j];
+        H[(size_t)l * m + j] = t1;
+        H[(size_t)(l + 1) * m + j] = t2;
+      }
+      const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+      const double rr = std::sqrt(a * a + bb * bb);
+      cs[j] = rr == 0. ? 1. : a / rr;
+      sn[j] = rr == 0. ? 0. : bb / rr;
+      H[(size_t)j * m + j] = rr;
+      H[(size_t)(j + 1) * m + j] = 0.;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      it++;
+      if (!std::isfinite(gv[j + 1])) throw StatusError(VFS_PP_ENONFINITE, "non-finite Krylov residual");
+      if (std::fabs(gv[j + 1]) <= tgt) { j++; break; }
+    }
+    for (int l = j - 1; l >= 0; l--) {
+      double s = gv[l];
+      for (int q = l + 1; q < j; q++) s -= H[(size_t)l * m + q] * y[q];
+      y[l] = s / H[(size_t)l * m + l];
+    }
+    for (int l = 0; l < j; l++) h->hpin[l] = y[l];
+    h2d(h->s, h->red + 200, h->hpin, (size_t)j);
+    launch_update_x<TZ>(h->s, n, h->x, Z, n, j, h->red + 200);
+  }
+  *rel_out = rel;
+  return it;
+}
+
+static void ensure_geom(vfs_pp_handle* h) {
+  if (!h->have_geom) throw StatusError(VFS_PP_ESTATE, "vfs_pp_set_geometry has not been called");
+}
+
+static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const double* iaj, const double* jaj,
+                         const double* kaj, const double* aj, const double* csi, const double* eta,
+                         const double* zet, const double* nvert) {
+  const Dmda m = h->m;
+  const Dims d = h->d;
+  const size_t M = m.n(), n = d.n();
+  hipStream_t s = h->s;
+  // resident arrays
+  if (!h->g64) {
+    h->g64 = h->ar.alloc<double>(9 * n);
+    h->code = h->ar.alloc<uint32_t>(n);
+    h->cls = h->ar.alloc<uint8_t>(n);
+    h->met9 = h->ar.alloc<double>(9 * n);
+    h->aj = h->ar.alloc<double>(n);
+    h->nvert = h->ar.alloc<double>(M);
+    h->x = h->ar.alloc<double>(n);
+    h->b = h->ar.alloc<double>(n);
+    h->r = h->ar.alloc<double>(n);
+    h->partial = h->ar.alloc<double>(64 * 1024 + (size_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz);
+    h->red = h->ar.alloc<double>(512);
+    h->snap = h->ar.alloc<double>(3 * M);
+    h->mrest = std::max(1, std::min(h->opts.restart, 60));
+    h->V = h->ar.alloc<double>((size_t)(h->mrest + 1) * n);
+    if (h->opts.precision == 1) h->Z = h->ar.alloc<float>((size_t)h->mrest * n);
+    else h->Z = h->ar.alloc<double>((size_t)h->mrest * n);
+    HIPCHECK(hipHostMalloc(&h->hpin, 4096 * sizeof(double)));
+    HIPCHECK(hipHostGetDevicePointer((void**)&h->hpin_dev, h->hpin, 0));
+  }
+  // temporaries for the face metrics
+  {
+    Arena tmp;
+    double* dv[9];
+    for (int q = 0; q < 9; q++) {
+      dv[q] = tmp.alloc<double>(3 * M);
+      h2d(s, dv[q], vec9[q], 3 * M);
+    }
+    double* dj[3];
+    const double* hj[3] = {iaj, jaj, kaj};
+    for (int q = 0; q < 3; q++) {
+      dj[q] = tmp.alloc<double>(M);
+      h2d(s, dj[q], hj[q], M);
+    }
+    h2d(s, h->nvert, nvert, M);
+    launch_face_coeffs(s, m, d, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dj[0], dj[1], dj[2],
+                       h->g64);
+    launch_codes(s, m, d, h->nvert, h->code, h->cls);
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  // cell metrics and aj on interior cells (Contra2Cart, Divergence)
+  {
+    std::vector<double> met(9 * n), ajv(n);
+    const double* src[3] = {csi, eta, zet};
+    for (int k = 0; k < d.nz; k++)
+      for (int j = 0; j < d.ny; j++)
+        for (int i = 0; i < d.nx; i++) {
+          const size_t c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+          for (int v = 0; v < 3; v++)
+            for (int q = 0; q < 3; q++) met[(size_t)(3 * v + q) * n + c] = src[v][3 * p + q];
+          ajv[c] = aj[p];
+        }
+    h2d(s, h->met9, met.data(), 9 * n);
+    h2d(s, h->aj, ajv.data(), n);
+    HIPCHECK(hipStreamSynchronize(s));
+  }
+  // VolumeFlux face list (poisson.c:2670-2700 / 2936-3010)
+  {
+    std::vector<IbFace> faces;
+    const size_t st[3] = {1, (size_t)m.mx, (size_t)m.mx * m.my};
+    const int mm[3] = {m.mx, m.my, m.mz};
+    const double* met3[3] = {csi, eta, zet};
+    double cnt = 0.;
+    for (int k = 1; k < m.mz - 1; k++)
+      for (int j = 1; j < m.my - 1; j++)
+        for (int i = 1; i < m.mx - 1; i++) {
+          const size_t c = m.idx(i, j, k);
+          const int pos[3] = {i, j, k};
+          const double nc = nvert[c];
+          if (nc < 0.1) cnt += 1.;
+          for (int dd = 0; dd < 3; dd++) {
+            const double nq = nvert[c + st[dd]];
+            const double* mv = &met3[dd][3 * c];
+            const double area = std::sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+            float sign = 0.f;
+            if (h->grid.immersed == 1) {
+              if (!(pos[dd] < mm[dd] - 2)) continue;
+              if (nc < 0.1 && nq > 0.1 && nq < 1.1) sign = 1.f;
+              else if (nc > 0.1 && nc < 1.1 && nq < 0.1) sign = -1.f;
+            } else {
+              if (pos[dd] > mm[dd] - 2) continue;
+              if (nc < TH && nq > TH) sign = 1.f;
+              else if (nc > TH && nq < TH) sign = -1.f;
+            }
+            if (sign != 0.f) faces.push_back(IbFace{(uint64_t)(3 * c + dd), area, sign, 0.f});
+          }
+        }
+    h->n_lt = cnt;
+    h->nibf = faces.size();
+    if (!faces.empty()) {
+      h->ibf = h->ar.alloc<IbFace>(faces.size());
+      h2d(s, h->ibf, faces.data(), faces.size());
+      HIPCHECK(hipStreamSynchronize(s));
+    }
+  }
+  // host hierarchy
+  h->hl.clear();
+  HostLevel L0;
+  L0.d = d;
+  L0.g.resize(9 * n);
+  L0.code.resize(n);
+  d2h(s, L0.g.data(), h->g64, 9 * n);
+  d2h(s, L0.code.data(), h->code, n);
+  L0.nv.assign(nvert, nvert + M);
+  HIPCHECK(hipStreamSynchronize(s));
+  L0.cls.assign(n, 0);
+  h->hl.push_back(std::move(L0));
+  while (true) {
+    HostLevel& F = h->hl.back();
+    const bool can = F.d.nx >= 3 || F.d.ny >= 3 || F.d.nz >= 3;
+    if ((long long)F.d.n() <= h->opts.coarse_max || !can || h->hl.size() >= 16) break;
+    HostLevel C = coarsen(F);
+    h->hl.push_back(std::move(C));
+  }
+  if ((long long)h->hl.back().d.n() > 1024)
+    throw StatusError(VFS_PP_EINVAL, "grid cannot be coarsened to a direct-solve size");
+  if (h->opts.precision == 1) h->mg = build_mg<float>(h);
+  else h->mg = build_mg<double>(h);
+  h->desc = h->mg->describe();
+  // free the host copies of the fine level (the device holds them)
+  for (auto& L : h->hl) {
+    L.g.clear(); L.g.shrink_to_fit();
+    L.nv.clear(); L.nv.shrink_to_fit();
+  }
+  h->have_geom = true;
+}
+
+// ------------------------------------------------------------ pressure path
+static void poisson_solve(vfs_pp_handle* h, double* ucont, double* phi, double dt, double st, double coeff,
+                          vfs_pp_stats* out) {
+  ensure_geom(h);
+  const double t0 = now_ms();
+  vfs_pp_stats S{};
+  hipStream_t s = h->s;
+  const size_t n = h->d.n();
+  int nb = 0;
+  double tmp[4];
+  // VolumeFlux
+  if (h->nibf) {
+    launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 1, h->partial, &nb);
+    launch_finalize(s, h->partial, 2, nb, h->red + 300, 0, h->hpin_dev + 300);
+    launch_ibflux_correct(s, h->ibf, h->nibf, ucont, h->grid.immersed, h->red + 300);
+    launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 0, h->partial, &nb);
+    launch_finalize(s, h->partial, 2, nb, h->red + 302, 0, h->hpin_dev + 302);
+  }
+  // RHS: val *= -1.0 / dt * st * coeff  (poisson_hypre.c:376)
+  const double fac = -1.0 / dt * st * coeff;
+  launch_rhs(s, h->m, h->d, ucont, h->cls, fac, h->b, h->partial, &nb);
+  double b2;
+  finalize_to_host(h, 1, nb, 0, &b2);
+  if (h->nibf) for (int q = 0; q < 4; q++) S.ibm_flux[q] = h->hpin[300 + q];
+  const double bnorm = std::sqrt(b2);
+  S.bnorm = bnorm;
+  if (!std::isfinite(bnorm)) throw StatusError(VFS_PP_ENONFINITE, "non-finite right-hand side");
+  // initial guess
+  if (h->opts.zero_guess) launch_fill(s, n, h->x, 0.);
+  else throw StatusError(VFS_PP_EINVAL, "non-zero initial guess is not supported yet");
+  double rel = 0.;
+  int vc = 0;
+  bool conv = false;
+  int it;
+  if (bnorm == 0.) { it = 0; rel = 0.; conv = true; }
+  else if (h->opts.precision == 1) it = fgmres<float>(h, bnorm, &rel, &vc, &conv);
+  else it = fgmres<double>(h, bnorm, &rel, &vc, &conv);
+  S.iterations = it;
+  S.vcycles = vc;
+  S.rel_residual = rel;
+  S.converged = conv ? 1 : 0;
+  // Remove_Nullspace (poisson_hypre.c:207-270)
+  launch_masked_sum(s, h->d, h->x, h->cls, h->partial, &nb);
+  launch_finalize(s, h->partial, 1, nb, h->red + 310, 0, h->hpin_dev + 310);
+  if (h->n_lt > 0.) launch_sub_mean(s, h->d, h->x, h->cls, h->red + 310, h->n_lt);
+  // Convert_Phi2_Phi (poisson.c:97-185)
+  launch_scatter_update(s, h->m, h->d, h->x, h->cls, phi, nullptr);
+  launch_periodic_scalar(s, h->m, phi, 1);
+  HIPCHECK(hipStreamSynchronize(s));
+  S.phi_mean = h->n_lt > 0. ? h->hpin[310] / h->n_lt : 0.;
+  S.ms_total = now_ms() - t0;
+  if (out) *out = S;
+}
+
+static void update_pressure(vfs_pp_handle* h, double* p, double* phi) {
+  ensure_geom(h);
+  // P += Phi on fluid interior cells; periodic ghosts of P and Phi (poisson.c:2522-2614)
+  launch_add_phi(h->s, h->m, h->d, h->cls, phi, p);
+  launch_periodic_scalar(h->s, h->m, p, 0);
+  launch_periodic_scalar(h->s, h->m, phi, 0);
+}
+
+static void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
+                       double coeff) {
+  ensure_geom(h);
+  const double fac = dt * st / coeff;
+  launch_projection(h->s, h->m, h->d, h->g64, h->code, h->cls, phi, ucont, fac);
+  launch_ucont_ghosts(h->s, h->m, ucont);
+  launch_c2c(h->s, h->m, h->d, h->met9, h->cls, h->nvert, ucont, ucat, h->snap, h->grid.bctype);
+}
+
+static double divergence(vfs_pp_handle* h, const double* ucont) {
+  ensure_geom(h);
+  int nb = 0;
+  launch_divergence(h->s, h->m, h->d, h->code, h->aj, ucont, h->partial, &nb);
+  double mx;
+  finalize_to_host(h, 1, nb, 1, &mx);
+  return mx;
+}
+
+}  // namespace pp
+
+// ======================================================================= C ABI
+using namespace pp;
+
+#define ABI_BEGIN try {
+#define ABI_END                                   \
+  }                                               \
+  catch (const StatusError& e) {                  \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return e.code;                                \
+  }                                               \
+  catch (const HipError& e) {                     \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return VFS_PP_EHIP;                           \
+  }                                               \
+  catch (const std::exception& e) {               \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return VFS_PP_EINVAL;                         \
+  }
+
+extern "C" {
+
+void vfs_pp_default_opts(vfs_pp_opts* o) {
+  if (!o) return;
+  o->precision = 1;
+  o->rtol = 1e-10;
+  o->div_tol = 0.;
+  o->max_iters = 200;
+  o->restart = 30;
+  o->smoother = 0;
+  o->cheb_degree = 2;
+  o->coarse_max = 512;
+  o->drop_small = 0;
+  o->zero_guess = 1;
+}
+
+const char* vfs_pp_error_string(int code) {
+  switch (code) {
+    case VFS_PP_OK: return "ok";
+    case VFS_PP_EINVAL: return "invalid argument";
+    case VFS_PP_ENOMEM: return "device memory exhausted";
+    case VFS_PP_EHIP: return "HIP runtime error";
+    case VFS_PP_ENOCONV: return "solver did not converge";
+    case VFS_PP_ESTATE: return "call order violated";
+    case VFS_PP_ENONFINITE: return "non-finite values";
+    default: return "unknown";
+  }
+}
+
+int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) {
+  ABI_BEGIN
+  if (!g || !out) return VFS_PP_EINVAL;
+  if (g->nx < 2 || g->ny < 2 || g->nz < 2) return VFS_PP_EINVAL;
+  for (int a = 0; a < 3; a++)
+    if (g->periodic[a] && (a == 0 ? g->nx : a == 1 ? g->ny : g->nz) < 3) return VFS_PP_EINVAL;
+  auto h = std::make_unique<vfs_pp_handle>();
+  h->grid = *g;
+  if (o) h->opts = *o;
+  else vfs_pp_default_opts(&h->opts);
+  if (!(h->opts.rtol > 0.) || h->opts.max_iters < 0 || h->opts.restart < 1) return VFS_PP_EINVAL;
+  h->d = Dims{g->nx, g->ny, g->nz, {g->periodic[0] ? 1 : 0, g->periodic[1] ? 1 : 0, g->periodic[2] ? 1 : 0}};
+  h->m = Dmda{g->nx + 2, g->ny + 2, g->nz + 2, {h->d.per[0], h->d.per[1], h->d.per[2]}};
+  HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+  *out = h.release();
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_destroy(vfs_pp_handle* h) {
+  if (!h) return VFS_PP_OK;
+  if (h->s) (void)hipStreamSynchronize(h->s);
+  h->mg.reset();
+  h->ar.release();
+  if (h->hpin) (void)hipHostFree(h->hpin);
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+  return VFS_PP_OK;
+}
+
+int vfs_pp_set_opts(vfs_pp_handle* h, const vfs_pp_opts* o) {
+  if (!h || !o) return VFS_PP_EINVAL;
+  // options that change the hierarchy require a new handle
+  if (h->have_geom && (o->precision != h->opts.precision || o->restart != h->opts.restart ||
+                       o->coarse_max != h->opts.coarse_max || o->smoother != h->opts.smoother ||
+                       o->cheb_degree != h->opts.cheb_degree))
+    return VFS_PP_ESTATE;
+  if (!(o->rtol > 0.)) return VFS_PP_EINVAL;
+  h->opts = *o;
+  return VFS_PP_OK;
+}
+
+int vfs_pp_set_geometry(vfs_pp_handle* h, const double* icsi, const double* ieta, const double* izet,
+                        const double* jcsi, const double* jeta, const double* jzet, const double* kcsi,
+                        const double* keta, const double* kzet, const double* iaj, const double* jaj,
+                        const double* kaj, const double* aj, const double* csi, const double* eta, const double* zet,
+                        const double* nvert) {
+  ABI_BEGIN
+  if (!h) return VFS_PP_EINVAL;
+  const double* v9[9] = {icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet};
+  for (auto p : v9)
+    if (!p) return VFS_PP_EINVAL;
+  if (!iaj || !jaj || !kaj || !aj || !csi || !eta || !zet || !nvert) return VFS_PP_EINVAL;
+  set_geometry(h, v9, iaj, jaj, kaj, aj, csi, eta, zet, nvert);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_poisson_solve_device(vfs_pp_handle* h, double* ucont, double* phi, double dt, double st, double coeff,
+                                vfs_pp_stats* stats) {
+  ABI_BEGIN
+  if (!h || !ucont || !phi || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
+  vfs_pp_stats S{};
+  poisson_solve(h, ucont, phi, dt, st, coeff, &S);
+  if (stats) *stats = S;
+  return S.converged ? VFS_PP_OK : VFS_PP_ENOCONV;
+  ABI_END
+}
+
+int vfs_pp_update_pressure_device(vfs_pp_handle* h, double* p, double* phi) {
+  ABI_BEGIN
+  if (!h || !p || !phi) return VFS_PP_EINVAL;
+  update_pressure(h, p, phi);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_projection_device(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
+                             double coeff) {
+  ABI_BEGIN
+  if (!h || !ucont || !phi || !ucat || coeff == 0.) return VFS_PP_EINVAL;
+  projection(h, ucont, phi, ucat, dt, st, coeff);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_divergence_device(vfs_pp_handle* h, const double* ucont, double* maxdiv) {
+  ABI_BEGIN
+  if (!h || !ucont || !maxdiv) return VFS_PP_EINVAL;
+  *maxdiv = divergence(h, ucont);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_project_device(vfs_pp_handle* h, double* ucont, double* phi, double* p, double* ucat, double dt,
+                          double st, double coeff, vfs_pp_stats* stats) {
+  ABI_BEGIN
+  if (!h || !ucont || !phi || !p || !ucat || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
+  const double t0 = now_ms();
+  vfs_pp_stats S{};
+  poisson_solve(h, ucont, phi, dt, st, coeff, &S);
+  update_pressure(h, p, phi);
+  projection(h, ucont, phi, ucat, dt, st, coeff);
+  S.max_div = divergence(h, ucont);
+  S.ms_total = now_ms() - t0;
+  bool ok = S.converged;
+  if (h->opts.div_tol > 0. && !(S.max_div <= h->opts.div_tol)) ok = false;
+  S.converged = ok ? 1 : 0;
+  if (stats) *stats = S;
+  return ok ? VFS_PP_OK : VFS_PP_ENOCONV;
+  ABI_END
+}
+
+int vfs_pp_project(vfs_pp_handle* h, double* ucont, double* phi, double* p, double* ucat, double dt, double st,
+                   double coeff, vfs_pp_stats* stats) {
+  ABI_BEGIN
+  if (!h || !ucont || !phi || !p || !ucat) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  const size_t M = h->m.n();
+  Arena tmp;
+  double* du = tmp.alloc<double>(3 * M);
+  double* dphi = tmp.alloc<double>(M);
+  double* dp = tmp.alloc<double>(M);
+  double* dc = tmp.alloc<double>(3 * M);
+  h2d(h->s, du, ucont, 3 * M);
+  h2d(h->s, dphi, phi, M);
+  h2d(h->s, dp, p, M);
+  h2d(h->s, dc, ucat, 3 * M);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  const int rc = vfs_pp_project_device(h, du, dphi, dp, dc, dt, st, coeff, stats);
+  if (rc != VFS_PP_OK && rc != VFS_PP_ENOCONV) return rc;
+  d2h(h->s, ucont, du, 3 * M);
+  d2h(h->s, phi, dphi, M);
+  d2h(h->s, p, dp, M);
+  d2h(h->s, ucat, dc, 3 * M);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return rc;
+  ABI_END
+}
+
+int vfs_pp_apply_device(vfs_pp_handle* h, const double* phi, double* y, int drop_small) {
+  ABI_BEGIN
+  if (!h || !phi || !y) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  launch_apply_dmda(h->s, h->m, h->d, h->g64, h->code, phi, y, drop_small != 0);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_export_stencil(vfs_pp_handle* h, double* vol19, uint8_t* cases) {
+  ABI_BEGIN
+  if (!h || !vol19 || !cases) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  const size_t n = h->d.n(), M = h->m.n();
+  Arena tmp;
+  double* dv = tmp.alloc<double>(19 * n);
+  launch_export_rows(h->s, h->d, h->g64, h->code, dv);
+  std::vector<double> hv(19 * n);
+  std::vector<uint32_t> code(n);
+  d2h(h->s, hv.data(), dv, 19 * n);
+  d2h(h->s, code.data(), h->code, n);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  std::memset(vol19, 0, 19 * M * sizeof(double));
+  std::memset(cases, 0, 18 * M);
+  for (int k = 0; k < h->d.nz; k++)
+    for (int j = 0; j < h->d.ny; j++)
+      for (int i = 0; i < h->d.nx; i++) {
+        const size_t c = h->d.idx(i, j, k), p = h->m.idx(i + 1, j + 1, k + 1);
+        if (!(code[c] & ROW_BIT)) continue;
+        std::memcpy(vol19 + 19 * p, hv.data() + 19 * c, 19 * sizeof(double));
+        for (int f = 0; f < 6; f++) {
+          const bool act = (code[c] >> f) & 1u;
+          cases[18 * p + 3 * f] = act ? 1 : 0;
+          if (!act) continue;
+          cases[18 * p + 3 * f + 1] = (uint8_t)((code[c] >> (6 + 4 * f)) & 3u);
+          cases[18 * p + 3 * f + 2] = (uint8_t)((code[c] >> (8 + 4 * f)) & 3u);
+        }
+      }
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_classify_nvert_device(const vfs_grid* g, double* nvert) {
+  ABI_BEGIN
+  if (!g || !nvert) return VFS_PP_EINVAL;
+  const Dmda m{g->nx + 2, g->ny + 2, g->nz + 2, {g->periodic[0], g->periodic[1], g->periodic[2]}};
+  launch_classify_nvert(nullptr, m, g->bctype, nvert, nullptr);
+  HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+static Dmda dmda_fine(const vfs_grid* g) {
+  return Dmda{g->nx + 2, g->ny + 2, g->nz + 2, {g->periodic[0], g->periodic[1], g->periodic[2]}};
+}
+static Dmda dmda_coarse(const vfs_grid* g) {
+  // IM_c = (IM_f + 1) / 2 with IM = m - 1 (init.c:95-127)
+  auto c = [](int mf) { return ((mf - 1) + 1) / 2 + 1; };
+  return Dmda{c(g->nx + 2), c(g->ny + 2), c(g->nz + 2), {g->periodic[0], g->periodic[1], g->periodic[2]}};
+}
+
+int vfs_pp_mg_restrict_device(const vfs_grid* g, const double* nvf, const double* nvc, const double* xf, double* fc) {
+  ABI_BEGIN
+  if (!g || !nvf || !nvc || !xf || !fc || (g->nx % 2) || (g->ny % 2) || (g->nz % 2)) return VFS_PP_EINVAL;
+  launch_ref_restrict(nullptr, dmda_fine(g), dmda_coarse(g), nvf, nvc, xf, fc);
+  HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_mg_prolong_device(const vfs_grid* g, const double* nvf, const double* nvc, const double* xc, double* ff) {
+  ABI_BEGIN
+  if (!g || !nvf || !nvc || !xc || !ff || (g->nx % 2) || (g->ny % 2) || (g->nz % 2)) return VFS_PP_EINVAL;
+  launch_ref_prolong(nullptr, dmda_fine(g), dmda_coarse(g), nvf, nvc, xc, ff);
+  HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_mg_nvert_restrict_device(const vfs_grid* g, const double* nvf, double* nvc) {
+  ABI_BEGIN
+  if (!g || !nvf || !nvc || (g->nx % 2) || (g->ny % 2) || (g->nz % 2)) return VFS_PP_EINVAL;
+  const Dmda c = dmda_coarse(g);
+  Arena tmp;
+  double* t = tmp.alloc<double>(c.n());
+  launch_ref_nvert_restrict(nullptr, dmda_fine(g), c, nvf, nvc, t);
+  HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, double* bytes) {
+  ABI_BEGIN
+  if (!h || reps < 1 || !ms || !bytes) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  const size_t n = h->d.n(), M = h->m.n();
+  Arena tmp;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  double* y = tmp.alloc<double>(n);
+  double* u = nullptr;
+  double* phi = nullptr;
+  double* uc = nullptr;
+  if (kernel >= 2) {
+    u = tmp.alloc<double>(3 * M);
+    phi = tmp.alloc<double>(M);
+    uc = tmp.alloc<double>(3 * M);
+    HIPCHECK(hipMemsetAsync(u, 0, 3 * M * sizeof(double), h->s));
+    HIPCHECK(hipMemsetAsync(phi, 0, M * sizeof(double), h->s));
+    HIPCHECK(hipMemsetAsync(uc, 0, 3 * M * sizeof(double), h->s));
+  }
+  int nb = 0;
+  auto launch = [&]() {
+    switch (kernel) {
+      case 0: launch_apply64<double>(h->s, h->d, h->g64, h->code, h->x, y, false); break;
+      case 1: h->mg->time_sweep(h->s); break;
+      case 2: launch_projection(h->s, h->m, h->d, h->g64, h->code, h->cls, phi, u, 1.0); break;
+      case 3: launch_c2c(h->s, h->m, h->d, h->met9, h->cls, h->nvert, u, uc, h->snap, h->grid.bctype); break;
+      case 4: launch_rhs(h->s, h->m, h->d, u, h->cls, 1.0, y, h->partial, &nb); break;
+      default: throw StatusError(VFS_PP_EINVAL, "unknown kernel id");
+    }
+  };
+  for (int w = 0; w < 3; w++) launch();
+  HIPCHECK(hipEventRecord(e0, h->s));
+  for (int r = 0; r < reps; r++) launch();
+  HIPCHECK(hipEventRecord(e1, h->s));
+  HIPCHECK(hipEventSynchronize(e1));
+  float tms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&tms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)tms / reps;
+  // algorithmic bytes per launch
+  switch (kernel) {
+    case 0: *bytes = (double)n * (9 * 8 + 4 + 8 + 8); break;          // g, code, x, y
+    case 1: *bytes = (double)h->mg->fine_bytes_per_sweep(); break;
+    case 2: *bytes = (double)n * (9 * 8 + 4 + 1 + 8 + 24 + 24); break;  // g, code, cls, phi, Ucont r/w
+    case 3: *bytes = (double)n * (24 + 72 + 1 + 24); break;            // Ucont, csi/eta/zet, cls, Ucat
+    case 4: *bytes = (double)n * (24 + 1 + 8); break;                  // Ucont, cls, b
+  }
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_describe(vfs_pp_handle* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return VFS_PP_EINVAL;
+  std::string s = h->desc;
+  std::snprintf(buf, len, "%s", s.c_str());
+  return VFS_PP_OK;
+}
+
+}  // extern "C"

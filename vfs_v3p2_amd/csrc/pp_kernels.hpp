@@ -1,0 +1,130 @@
+// Kernel launchers of the pressure-projection engine.  All launches go to the
+// stream passed in; nothing here synchronises.
+#pragma once
+
+#include "pp_common.hpp"
+
+namespace pp {
+
+// Per-cell class bits (1 byte per interior cell), the scalar nvert tests used
+// outside the operator rows.
+constexpr uint8_t CLS_LT = 1;         // nvert < 0.1: RHS, Remove_Nullspace, UpdatePressure, Contra2Cart
+constexpr uint8_t CLS_INTGT = 2;      // (int)nvert > 0.1: Convert_Phi2_Phi writes 0 (poisson.c:131)
+constexpr uint8_t CLS_SOLID = 4;      // (int)(nvert+0.1) == 3: Contra2Cart zeroes Ucat (rhs.c:257)
+constexpr uint8_t CLS_PROJ_I = 8;     // Projection corrects this cell's i+1/2 face (poisson.c:2221,2259)
+constexpr uint8_t CLS_PROJ_J = 16;    // ... j+1/2 face
+constexpr uint8_t CLS_PROJ_K = 32;    // ... k+1/2 face
+constexpr uint8_t CLS_NONFLUID = 64;  // nvert > 0.1 (multigrid masks: MyInterpolation zeroes, restriction weight)
+
+// DMDA description used by kernels touching caller arrays.
+struct Dmda {
+  int mx, my, mz;
+  int per[3];
+  HD size_t n() const { return (size_t)mx * my * mz; }
+  HD size_t idx(int i, int j, int k) const { return ((size_t)k * my + j) * mx + i; }
+};
+
+// One face next to the immersed boundary (VolumeFlux).
+struct IbFace {
+  uint64_t pos;  // index into the AoS Ucont array (3*dmda_point + component)
+  double area;   // |csi|, |eta| or |zet| of the cell (poisson.c:2676-2684)
+  float sign;    // +1 fluid->IB, -1 IB->fluid
+  float pad;
+};
+
+// ---- set-up ------------------------------------------------------------------
+void launch_face_coeffs(hipStream_t s, Dmda m, Dims d, const double* icsi, const double* ieta, const double* izet,
+                        const double* jcsi, const double* jeta, const double* jzet, const double* kcsi,
+                        const double* keta, const double* kzet, const double* iaj, const double* jaj,
+                        const double* kaj, double* g);
+void launch_codes(hipStream_t s, Dmda m, Dims d, const double* nvert, uint32_t* code, uint8_t* cls);
+void launch_export_rows(hipStream_t s, Dims d, const double* g, const uint32_t* code, double* vol19);
+void launch_classify_nvert(hipStream_t s, Dmda m, const int bctype[6], double* nv, uint8_t* tmp);
+
+// ---- FP64 operator ----------------------------------------------------------------
+// y = A x on interior vectors; x may be float or double.
+template <typename TX>
+void launch_apply64(hipStream_t s, Dims d, const double* g, const uint32_t* code, const TX* x, double* y,
+                    bool drop);
+// r = b - A x and per-block partial sums of r.r (nblk entries)
+template <typename TX>
+void launch_residual64(hipStream_t s, Dims d, const double* g, const uint32_t* code, const double* b, const TX* x,
+                       double* r, double* partial, int* nblk);
+// DMDA in/out variant for the public apply entry point
+void launch_apply_dmda(hipStream_t s, Dmda m, Dims d, const double* g, const uint32_t* code, const double* phi,
+                       double* y, bool drop);
+
+// ---- pressure path ----------------------------------------------------------------
+void launch_rhs(hipStream_t s, Dmda m, Dims d, const double* ucont, const uint8_t* cls, double fac, double* b,
+                double* partial, int* nblk);
+// VolumeFlux on a compact face list.  pass 0: sums (immersed==1 also zeroes |u|<=eps);
+// pass 1: correction by corr (device scalar); pass 2: sums after.
+void launch_ibflux_sum(hipStream_t s, const IbFace* f, size_t nf, double* ucont, int immersed, int zero_small,
+                       double* partial, int* nblk);
+void launch_ibflux_correct(hipStream_t s, const IbFace* f, size_t nf, double* ucont, int immersed,
+                           const double* sums /* [flux, area] */);
+void launch_masked_sum(hipStream_t s, Dims d, const double* x, const uint8_t* cls, double* partial, int* nblk);
+void launch_sub_mean(hipStream_t s, Dims d, double* x, const uint8_t* cls, const double* sum, double inv_count);
+void launch_scatter_update(hipStream_t s, Dmda m, Dims d, const double* x, const uint8_t* cls, double* phi,
+                           double* p);
+void launch_periodic_scalar(hipStream_t s, Dmda m, double* a, int zero_other);
+void launch_add_phi(hipStream_t s, Dmda m, Dims d, const uint8_t* cls, const double* phi, double* p);
+void launch_projection(hipStream_t s, Dmda m, Dims d, const double* g, const uint32_t* code, const uint8_t* cls,
+                       const double* phi, double* ucont, double fac);
+void launch_ucont_ghosts(hipStream_t s, Dmda m, double* ucont);
+void launch_c2c(hipStream_t s, Dmda m, Dims d, const double* met9, const uint8_t* cls, const double* nvert,
+                const double* ucont, double* ucat, double* scratch, const int bctype[6]);
+void launch_divergence(hipStream_t s, Dmda m, Dims d, const uint32_t* code, const double* aj, const double* ucont,
+                       double* partial, int* nblk);
+
+// ---- reductions / BLAS-1 ---------------------------------------------------------
+// out[v] = sum_b partial[v*nblk + b] in a fixed order (deterministic); op 0 sum, 1 max
+void launch_finalize(hipStream_t s, const double* partial, int nvec, int nblk, double* out, int op,
+                     double* host_mirror = nullptr);
+void launch_multidot(hipStream_t s, size_t n, const double* w, const double* V, size_t ldv, int nv, double* partial,
+                     int* nblk);
+void launch_axpy_multi(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                       double sign);
+void launch_norm2(hipStream_t s, size_t n, const double* a, double* partial, int* nblk);
+void launch_scale_to(hipStream_t s, size_t n, const double* a, const double* norm2, double* out);
+template <typename TZ>
+void launch_update_x(hipStream_t s, size_t n, double* x, const TZ* Z, size_t ldz, int nz, const double* y);
+template <typename TA, typename TB>
+void launch_convert(hipStream_t s, size_t n, const TA* a, TB* b);
+void launch_fill(hipStream_t s, size_t n, double* a, double v);
+void launch_fillf(hipStream_t s, size_t n, float* a, float v);
+
+int partial_blocks(size_t n);
+
+// ---- multigrid ---------------------------------------------------------------------
+template <typename T>
+struct LevelView {
+  Dims d;
+  const T* g;
+  const uint32_t* code;
+  const uint8_t* cls;  // CLS_NONFLUID used by the transfers
+};
+
+template <typename T>
+void launch_smooth(hipStream_t s, LevelView<T> L, int line, bool zero_guess, const T* f, const T* x_old, T* x_new,
+                   T* dvec, T* scratch, T c1, T c2);
+template <typename T>
+void launch_resid_restrict(hipStream_t s, LevelView<T> F, const T* f, const T* x, Dims dc, const uint8_t* cls_c,
+                           T* fc);
+template <typename T>
+void launch_prolong_add(hipStream_t s, LevelView<T> F, Dims dc, const uint8_t* cls_c, const T* xc, T* xf);
+template <typename T>
+void launch_dense_solve(hipStream_t s, int n, const T* cinv, const T* f, T* x);
+template <typename T>
+void launch_normsq(hipStream_t s, size_t n, const T* a, double* partial, int* nblk);
+template <typename T>
+void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2);
+
+// reference-layout transfer primitives (DMDA doubles), for verification
+void launch_ref_restrict(hipStream_t s, Dmda f, Dmda c, const double* nvf, const double* nvc, const double* xf,
+                         double* fc);
+void launch_ref_prolong(hipStream_t s, Dmda f, Dmda c, const double* nvf, const double* nvc, const double* xc,
+                        double* ff);
+void launch_ref_nvert_restrict(hipStream_t s, Dmda f, Dmda c, const double* nvf, double* nvc, double* tmp);
+
+}  // namespace pp

@@ -1,0 +1,308 @@
+"""Host-side mirror of the reference's pressure-path interface over the C ABI.
+
+The reference calls, per block and time step (solvers.c:874-895):
+
+    PoissonSolver_Hypre(user, ibm, ibm_intp)   # poisson_hypre.c:580
+    UpdatePressure(user)                       # poisson.c:2473
+    Projection(user)                           # poisson.c:2122 (ends with Contra2Cart)
+    Divergence(user)                           # main.c:1244
+
+with every field carried by UserCtx (variables.h:298-591).  `UserCtx` below
+carries the same fields as device tensors in the reference's DMDA layout and
+the functions of the same names run the HIP engine (libvfs_pp.so).  PyTorch is
+only used to own device memory; all arithmetic is in the library.
+
+Error behaviour: the reference returns 0 and is silent on non-convergence; here
+every call raises `VfsPPError` on a non-zero status, except that a
+non-converged solve is reported through `stats.converged` unless
+`strict=True`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvfs_pp.so")
+
+VFS_PP_OK, VFS_PP_EINVAL, VFS_PP_ENOMEM, VFS_PP_EHIP, VFS_PP_ENOCONV, VFS_PP_ESTATE, VFS_PP_ENONFINITE = range(7)
+
+
+class VfsPPError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {error_string(code)} ({code})")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("periodic", C.c_int * 3),
+                ("bctype", C.c_int * 6), ("immersed", C.c_int)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("rtol", C.c_double), ("div_tol", C.c_double), ("max_iters", C.c_int),
+                ("restart", C.c_int), ("smoother", C.c_int), ("cheb_degree", C.c_int), ("coarse_max", C.c_int),
+                ("drop_small", C.c_int), ("zero_guess", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
+                ("bnorm", C.c_double), ("max_div", C.c_double), ("ibm_flux", C.c_double * 4),
+                ("phi_mean", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k == "ibm_flux" else getattr(self, k)) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libvfs_pp.so; raises if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m vfs_v3p2_amd.build` "
+                           "(the engine has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, u8 = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+    H = C.c_void_p
+    sig = {
+        "vfs_pp_default_opts": (None, [C.POINTER(Opts)]),
+        "vfs_pp_create": (C.c_int, [C.POINTER(Grid), C.POINTER(Opts), C.POINTER(H)]),
+        "vfs_pp_destroy": (C.c_int, [H]),
+        "vfs_pp_set_opts": (C.c_int, [H, C.POINTER(Opts)]),
+        "vfs_pp_set_geometry": (C.c_int, [H] + [dp] * 17),
+        "vfs_pp_poisson_solve_device": (C.c_int, [H, vp, vp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
+        "vfs_pp_update_pressure_device": (C.c_int, [H, vp, vp]),
+        "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
+        "vfs_pp_divergence_device": (C.c_int, [H, vp, dp]),
+        "vfs_pp_project_device": (C.c_int, [H, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                            C.POINTER(Stats)]),
+        "vfs_pp_project": (C.c_int, [H, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
+        "vfs_pp_apply_device": (C.c_int, [H, vp, vp, C.c_int]),
+        "vfs_pp_export_stencil": (C.c_int, [H, dp, u8]),
+        "vfs_pp_classify_nvert_device": (C.c_int, [C.POINTER(Grid), vp]),
+        "vfs_pp_mg_restrict_device": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
+        "vfs_pp_mg_prolong_device": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
+        "vfs_pp_mg_nvert_restrict_device": (C.c_int, [C.POINTER(Grid), vp, vp]),
+        "vfs_pp_time_kernel": (C.c_int, [H, C.c_int, C.c_int, dp, dp]),
+        "vfs_pp_describe": (C.c_int, [H, C.c_char_p, C.c_size_t]),
+        "vfs_pp_error_string": (C.c_char_p, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def error_string(code):
+    return lib().vfs_pp_error_string(code).decode()
+
+
+def _check(rc, what, allow=()):
+    if rc != VFS_PP_OK and rc not in allow:
+        raise VfsPPError(rc, what)
+    return rc
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().vfs_pp_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def make_grid(nx, ny, nz, periodic=(1, 0, 1), bctype=(0, 0, 1, 1, 0, 0), immersed=0) -> Grid:
+    g = Grid()
+    g.nx, g.ny, g.nz = nx, ny, nz
+    for a in range(3):
+        g.periodic[a] = int(bool(periodic[a]))
+    for f in range(6):
+        g.bctype[f] = int(bctype[f])
+    g.immersed = int(immersed)
+    return g
+
+
+def _dptr(a: np.ndarray):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _tptr(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+class Handle:
+    """One device-resident pressure system (vfs_pp_handle)."""
+
+    def __init__(self, grid: Grid, opts: Optional[Opts] = None):
+        self.grid = grid
+        self.opts = opts if opts is not None else default_opts()
+        self._h = C.c_void_p()
+        _check(lib().vfs_pp_create(C.byref(grid), C.byref(self.opts), C.byref(self._h)), "vfs_pp_create")
+
+    def close(self):
+        if self._h:
+            lib().vfs_pp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        _check(lib().vfs_pp_set_opts(self._h, C.byref(self.opts)), "vfs_pp_set_opts")
+
+    def set_geometry(self, f: dict):
+        keep = []
+        args = []
+        for k in ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj", "aj",
+                  "csi", "eta", "zet", "nvert"):
+            a, p = _dptr(f[k])
+            keep.append(a)
+            args.append(p)
+        _check(lib().vfs_pp_set_geometry(self._h, *args), "vfs_pp_set_geometry")
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(4096)
+        lib().vfs_pp_describe(self._h, buf, len(buf))
+        return buf.value.decode()
+
+    def poisson_solve(self, ucont, phi, dt, st, coeff, strict=False):
+        s = Stats()
+        rc = lib().vfs_pp_poisson_solve_device(self._h, _tptr(ucont), _tptr(phi), dt, st, coeff, C.byref(s))
+        _check(rc, "poisson solve", allow=() if strict else (VFS_PP_ENOCONV,))
+        return s
+
+    def update_pressure(self, p, phi):
+        _check(lib().vfs_pp_update_pressure_device(self._h, _tptr(p), _tptr(phi)), "update pressure")
+
+    def projection(self, ucont, phi, ucat, dt, st, coeff):
+        _check(lib().vfs_pp_projection_device(self._h, _tptr(ucont), _tptr(phi), _tptr(ucat), dt, st, coeff),
+               "projection")
+
+    def divergence(self, ucont) -> float:
+        out = C.c_double()
+        _check(lib().vfs_pp_divergence_device(self._h, _tptr(ucont), C.byref(out)), "divergence")
+        return out.value
+
+    def project(self, ucont, phi, p, ucat, dt, st, coeff, strict=False):
+        s = Stats()
+        rc = lib().vfs_pp_project_device(self._h, _tptr(ucont), _tptr(phi), _tptr(p), _tptr(ucat), dt, st, coeff,
+                                         C.byref(s))
+        _check(rc, "project", allow=() if strict else (VFS_PP_ENOCONV,))
+        return s
+
+    def project_host(self, ucont, phi, p, ucat, dt, st, coeff):
+        """Host-array variant (PCIe copies in and out); arrays are updated in place."""
+        s = Stats()
+        ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in (ucont, phi, p, ucat)]
+        for a in (ucont, phi, p, ucat):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        rc = lib().vfs_pp_project(self._h, *ptrs, dt, st, coeff, C.byref(s))
+        _check(rc, "project (host arrays)", allow=(VFS_PP_ENOCONV,))
+        return s
+
+    def apply(self, phi, y, drop_small=False):
+        _check(lib().vfs_pp_apply_device(self._h, _tptr(phi), _tptr(y), int(drop_small)), "apply")
+
+    def export_stencil(self):
+        M = (self.grid.nz + 2) * (self.grid.ny + 2) * (self.grid.nx + 2)
+        vol = np.zeros(19 * M)
+        cases = np.zeros(18 * M, np.uint8)
+        _check(lib().vfs_pp_export_stencil(self._h, vol.ctypes.data_as(C.POINTER(C.c_double)),
+                                           cases.ctypes.data_as(C.POINTER(C.c_uint8))), "export stencil")
+        shape = (self.grid.nz + 2, self.grid.ny + 2, self.grid.nx + 2)
+        return vol.reshape(shape + (19,)), cases.reshape(shape + (18,))
+
+    def time_kernel(self, kernel: int, reps: int = 20):
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().vfs_pp_time_kernel(self._h, kernel, reps, C.byref(ms), C.byref(by)), "time kernel")
+        return ms.value, by.value
+
+
+# ----------------------------------------------------------- reference-named API
+
+class UserCtx:
+    """Device-resident counterpart of the UserCtx fields the pressure path uses
+    (variables.h:305-309, 333-336, 352-356, 378-385, 398-400): Ucont, Ucat, P,
+    Phi as torch CUDA tensors in DMDA layout, plus the engine handle that holds
+    the metrics, nvert masks and multigrid hierarchy."""
+
+    def __init__(self, case, opts: Optional[Opts] = None, device="cuda"):
+        import torch
+
+        self.case = case
+        self.grid = make_grid(case.nx, case.ny, case.nz, case.per, case.bctype, case.immersed)
+        self.handle = Handle(self.grid, opts)
+        self.handle.set_geometry(case.fields)
+        self.dt, self.st, self.coeff = case.dt, case.st, case.coeff
+        sh = case.shape
+        f = case.fields
+        self.Ucont = torch.from_numpy(np.ascontiguousarray(f["ucont"])).to(device)
+        self.Phi = torch.zeros(sh, dtype=torch.float64, device=device)
+        self.P = torch.zeros(sh, dtype=torch.float64, device=device)
+        self.Ucat = torch.zeros(sh + (3,), dtype=torch.float64, device=device)
+        self.stats = None
+
+
+def PoissonSolver(user: UserCtx, strict=False):
+    """PoissonSolver_Hypre (poisson_hypre.c:580): VolumeFlux, RHS, solve,
+    Remove_Nullspace, Convert_Phi2_Phi.  Returns the solve statistics."""
+    user.stats = user.handle.poisson_solve(user.Ucont, user.Phi, user.dt, user.st, user.coeff, strict=strict)
+    return user.stats
+
+
+def UpdatePressure(user: UserCtx):
+    """UpdatePressure (poisson.c:2473): P += Phi on fluid cells."""
+    user.handle.update_pressure(user.P, user.Phi)
+
+
+def Projection(user: UserCtx):
+    """Projection (poisson.c:2122) followed by Contra2Cart (rhs.c:50)."""
+    user.handle.projection(user.Ucont, user.Phi, user.Ucat, user.dt, user.st, user.coeff)
+
+
+def Divergence(user: UserCtx) -> float:
+    """Divergence (main.c:1244): max |aj div u| over cells not next to the IB."""
+    return user.handle.divergence(user.Ucont)
+
+
+def Flow_Solver_pressure_step(user: UserCtx, strict=False):
+    """The pressure block of Flow_Solver (solvers.c:874-895) as one call."""
+    user.stats = user.handle.project(user.Ucont, user.Phi, user.P, user.Ucat, user.dt, user.st, user.coeff,
+                                     strict=strict)
+    return user.stats
+
+
+# ----------------------------------------------------------- reference primitives
+
+def classify_nvert_device(grid: Grid, nvert_dev):
+    _check(lib().vfs_pp_classify_nvert_device(C.byref(grid), _tptr(nvert_dev)), "classify nvert")
+
+
+def mg_restrict_device(grid: Grid, nvf, nvc, xf, fc):
+    _check(lib().vfs_pp_mg_restrict_device(C.byref(grid), _tptr(nvf), _tptr(nvc), _tptr(xf), _tptr(fc)), "restrict")
+
+
+def mg_prolong_device(grid: Grid, nvf, nvc, xc, ff):
+    _check(lib().vfs_pp_mg_prolong_device(C.byref(grid), _tptr(nvf), _tptr(nvc), _tptr(xc), _tptr(ff)), "prolong")
+
+
+def mg_nvert_restrict_device(grid: Grid, nvf, nvc):
+    _check(lib().vfs_pp_mg_nvert_restrict_device(C.byref(grid), _tptr(nvf), _tptr(nvc)), "nvert restrict")
