@@ -1,0 +1,174 @@
+"""Benchmark: pressure-projection solves/s on one MI355X.
+
+Workload (one step = one full pressure projection, the reference's
+PoissonSolver_Hypre + UpdatePressure + Projection + Divergence, solvers.c:874-930):
+the 192x96x128 terrain grid with a Gaussian hill (h=0.3, s=0.5), algebraic
+grid-line blending (non-zero cross metrics, non-symmetric operator), a voxelised
+IB sphere (r=0.15) and a seeded divergent Ucont.  Inputs are resident in HBM
+before timing starts; every step projects its own copy of the field.
+
+Forms: mixed (FP32 multigrid preconditioner inside FP64 FGMRES; the headline
+`value`) and plain FP64 (FP64 multigrid), both to max|aj div u| <= 1e-8.
+
+Prints one JSON line (see README of the benchmark contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
+DIV_TOL = 1e-8
+RTOL = 2e-11  # flux-form relative residual; leaves margin under DIV_TOL
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU work for the baseline sample")
+    return ap.parse_args()
+
+
+def run_form(E, torch, case, precision, steps, warmup):
+    g = E.make_grid(case.nx, case.ny, case.nz, case.per, case.bctype, case.immersed)
+    opts = E.default_opts(precision=precision, rtol=RTOL, div_tol=DIV_TOL)
+    h = E.Handle(g, opts)
+    t0 = time.perf_counter()
+    h.set_geometry(case.fields)
+    setup_s = time.perf_counter() - t0
+    sh = case.shape
+    base = torch.from_numpy(np.ascontiguousarray(case.fields["ucont"])).cuda()
+    n = steps + warmup
+    us = [base.clone() for _ in range(n)]
+    phi = torch.zeros(sh, dtype=torch.float64, device="cuda")
+    p = torch.zeros_like(phi)
+    ucat = torch.zeros(sh + (3,), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for w in range(warmup):
+        st = h.project(us[w], phi, p, ucat, case.dt, case.st, case.coeff, strict=True)
+    torch.cuda.synchronize()
+    lat = []
+    stats = []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        st = h.project(us[warmup + k], phi, p, ucat, case.dt, case.st, case.coeff, strict=True)
+        lat.append(time.perf_counter() - a)
+        stats.append((st.iterations, st.rel_residual, st.max_div))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    return h, dict(total=total, lat=lat, stats=stats, setup_s=setup_s, desc=h.describe())
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs every system on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; systems would shard one per GPU without communication)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from vfs_v3p2_amd import cases
+    from vfs_v3p2_amd import engine as E
+
+    E.lib()  # fail loudly without the HIP library
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    case = cases.hill_case(192, 96, 128, h=0.3, s=0.5, sphere=(0.6, 0.75, 0.0, 0.15), seed=0)
+
+    h_mx, mx = run_form(E, torch, case, 1, args.steps, args.warmup)
+    # dominant kernel of the mixed solve: the finest-level FP32 smoother sweep
+    ms_sm, by_sm = h_mx.time_kernel(1, 100)
+    ms_ap, by_ap = h_mx.time_kernel(0, 100)
+    h_mx.close()
+    h64, f64 = run_form(E, torch, case, 0, args.steps, args.warmup)
+    ms_ap64, by_ap64 = h64.time_kernel(0, 100)
+    ms_sm64, by_sm64 = h64.time_kernel(1, 100)
+    h64.close()
+
+    lat = np.array(mx["lat"]) * 1e3
+    value = args.steps / mx["total"]
+    fp64_value = args.steps / f64["total"]
+    its = [s[0] for s in mx["stats"]]
+    divs = [s[2] for s in mx["stats"]]
+    divs64 = [s[2] for s in f64["stats"]]
+    sm_gbs = by_sm / (ms_sm * 1e-3) / 1e9
+    out = {
+        "metric": "Poisson solves/s & p99 latency at div<=1e-8, mixed vs FP64; stencil HBM GB/s",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mx["total"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 multigrid preconditioner)",
+        "data": "synthetic (seeded generator, vfs_v3p2_amd/cases.py)",
+        "config": {
+            "workload": "terrain/hill 192x96x128 + IB sphere r=0.15, full pressure projection "
+                        "(VolumeFlux, RHS, FGMRES+MG solve to max|div|<=1e-8, null space, P+=phi, "
+                        "Projection, Contra2Cart, Divergence)",
+            "grid": [case.nx, case.ny, case.nz],
+            "form": "mixed: FP32 geometric multigrid inside FP64 FGMRES",
+            "rtol": RTOL,
+            "div_tol": DIV_TOL,
+        },
+        "p50_ms": round(float(np.percentile(lat, 50)), 4),
+        "p99_ms": round(float(np.percentile(lat, 99)), 4),
+        "iterations": int(np.median(its)),
+        "max_div": float(max(divs)),
+        "fp64": {
+            "value": round(fp64_value, 3),
+            "unit": "solves/s",
+            "p99_ms": round(float(np.percentile(np.array(f64["lat"]) * 1e3, 99)), 4),
+            "iterations": int(np.median([s[0] for s in f64["stats"]])),
+            "max_div": float(max(divs64)),
+            "apply_gbs": round(by_ap64 / (ms_ap64 * 1e-3) / 1e9, 1),
+            "smoother_gbs": round(by_sm64 / (ms_sm64 * 1e-3) / 1e9, 1),
+        },
+        "mixed_over_fp64": round(value / fp64_value, 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_smooth_point<float> (finest-level Chebyshev-Jacobi sweep)",
+            "achieved": round(sm_gbs, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(sm_gbs / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "ms_per_launch": round(ms_sm, 5),
+            "bytes_per_launch": by_sm,
+            "fp64_apply_gbs": round(by_ap / (ms_ap * 1e-3) / 1e9, 1),
+        },
+        "setup_s": round(mx["setup_s"], 3),
+        "cpu_baseline": None,
+    }
+    if max(divs) > DIV_TOL or max(divs64) > DIV_TOL:
+        out["error"] = "max|div u| above 1e-8"
+    if not args.no_cpu:
+        try:
+            from oracle import cpu_baseline
+
+            out["cpu_baseline"] = cpu_baseline.measure(case, budget_s=args.cpu_budget)
+        except Exception as e:  # the GPU figures stand on their own
+            out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

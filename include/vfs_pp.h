@@ -69,6 +69,11 @@ typedef struct {
   int drop_small;    /* 1: drop |a_ij| <= 1e-10 like the assembled HYPRE matrix
                         (poisson.c:1608); 0: conservative operator (default) */
   int zero_guess;    /* 1: start from phi = 0; 0: start from the phi passed in */
+  double cheb_lo;    /* Chebyshev smoothing interval [cheb_lo, cheb_hi] x lambda_max(M^-1 A) */
+  double cheb_hi;
+  int orthog;        /* 0: one-pass classical Gram-Schmidt fused into the operator pass,
+                        with a second pass on cancellation; 1: two-pass CGS2 */
+  int verbose;       /* 1: print the residual history of each solve to stderr */
 } vfs_pp_opts;
 
 typedef struct {

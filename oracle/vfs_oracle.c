@@ -858,3 +858,478 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
   free(V); free(Z); free(H); free(cs); free(sn); free(gv); free(y); free(dinv); free(w);
   return it;
 }
+
+/* ===================================================================== CPU MG
+ * CPU port of the engine's solver, for the CPU baseline and for checking the
+ * device multigrid: geometric multigrid (coarse face coefficients summed over
+ * the active fine faces and divided by the coarsening factor of the difference
+ * direction, coarse nvert by the reference's MyNvertRestriction rule, Chebyshev
+ * smoothing preconditioned by the diagonal, summed restriction of fluid-child
+ * residuals, trilinear prolongation with the (27,9,9,3,9,3,3,1)/64 weights,
+ * bordered dense solve on the coarsest level) inside FP64 FGMRES(m) with CGS2.
+ * Vectors are interior arrays (nx*ny*nz, i fastest) with periodic wrap. */
+
+typedef struct {
+  int nx, ny, nz, per[3];
+  size_t n;
+  or_grid g;        /* DMDA view of this level */
+  double *g9;       /* DMDA face coefficients */
+  double *nv;       /* DMDA nvert */
+  double *vol;      /* DMDA row coefficients, 19 per point */
+  uint8_t *row, *cases;
+  double *f, *x, *xt, *dv, *r;
+  double c1[16], c2[16], lam;
+  int cmask;
+} mg_lev;
+
+struct or_mg {
+  int nl, degree;
+  mg_lev L[16];
+  double *cinv; /* coarsest bordered inverse, n x n row-major */
+};
+
+static size_t lidx(const mg_lev *L, int i, int j, int k) { return ((size_t)k * L->ny + j) * L->nx + i; }
+static size_t ldm(const mg_lev *L, int i, int j, int k) { return IX(&L->g, i + 1, j + 1, k + 1); }
+static int wr(int i, int n, int per) { return i < 0 ? (per ? i + n : 0) : (i >= n ? (per ? i - n : n - 1) : i); }
+
+/* y = A x on interior vectors using the level's row coefficients */
+static double row_apply(const mg_lev *L, int i, int j, int k, const double *x) {
+  size_t p = ldm(L, i, j, k);
+  if (!L->row[p]) return 0.;
+  double s = 0.;
+  for (int t = 0; t < 19; t++) {
+    double v = L->vol[19 * p + t];
+    if (v == 0.) continue;
+    int a = wr(i + SLOT_OFF[t][0], L->nx, L->per[0]);
+    int b = wr(j + SLOT_OFF[t][1], L->ny, L->per[1]);
+    int c = wr(k + SLOT_OFF[t][2], L->nz, L->per[2]);
+    s += v * x[lidx(L, a, b, c)];
+  }
+  return s;
+}
+
+static void lev_rows(mg_lev *L) {
+  size_t M = (size_t)L->g.mx * L->g.my * L->g.mz;
+  L->vol = (double *)malloc(19 * M * sizeof(double));
+  L->row = (uint8_t *)malloc(M);
+  L->cases = (uint8_t *)malloc(18 * M);
+  or_stencil(&L->g, L->g9, L->nv, L->vol, L->row, L->cases);
+}
+
+static void lev_alloc(mg_lev *L) {
+  L->n = (size_t)L->nx * L->ny * L->nz;
+  L->f = (double *)calloc(L->n, sizeof(double));
+  L->x = (double *)calloc(L->n, sizeof(double));
+  L->xt = (double *)calloc(L->n, sizeof(double));
+  L->dv = (double *)calloc(L->n, sizeof(double));
+  L->r = (double *)calloc(L->n, sizeof(double));
+}
+
+static void dm_periodic(const or_grid *g, double *a) { periodic_copy_scalar(g, a); }
+
+static void coarsen_level(mg_lev *F, mg_lev *C) {
+  int cm = (F->nx >= 3 ? 1 : 0) | (F->ny >= 3 ? 2 : 0) | (F->nz >= 3 ? 4 : 0);
+  F->cmask = cm;
+  C->nx = (cm & 1) ? (F->nx + 1) / 2 : F->nx;
+  C->ny = (cm & 2) ? (F->ny + 1) / 2 : F->ny;
+  C->nz = (cm & 4) ? (F->nz + 1) / 2 : F->nz;
+  for (int a = 0; a < 3; a++) C->per[a] = F->per[a];
+  C->g.mx = C->nx + 2; C->g.my = C->ny + 2; C->g.mz = C->nz + 2;
+  for (int a = 0; a < 3; a++) C->g.per[a] = C->per[a];
+  size_t M = (size_t)C->g.mx * C->g.my * C->g.mz;
+  C->nv = (double *)calloc(M, sizeof(double));
+  C->g9 = (double *)calloc(9 * M, sizeof(double));
+  const int fn[3] = {F->nx, F->ny, F->nz};
+  for (int k = 0; k < C->nz; k++)
+    for (int j = 0; j < C->ny; j++)
+      for (int i = 0; i < C->nx; i++) {
+        int lo[3], hi[3], ci[3] = {i, j, k};
+        for (int a = 0; a < 3; a++) {
+          if (cm & (1 << a)) { lo[a] = 2 * ci[a]; hi[a] = 2 * ci[a] + 1 < fn[a] ? 2 * ci[a] + 1 : fn[a] - 1; }
+          else lo[a] = hi[a] = ci[a];
+        }
+        double prod = 1.;
+        for (int kk = lo[2]; kk <= hi[2]; kk++)
+          for (int jj = lo[1]; jj <= hi[1]; jj++)
+            for (int ii = lo[0]; ii <= hi[0]; ii++) prod *= F->nv[ldm(F, ii, jj, kk)];
+        if (prod > 0.1) C->nv[ldm(C, i, j, k)] = 1.;
+        for (int d = 0; d < 3; d++) {
+          int l2[3] = {lo[0], lo[1], lo[2]};
+          l2[d] = hi[d];
+          double acc[3] = {0, 0, 0};
+          for (int kk = l2[2]; kk <= hi[2]; kk++)
+            for (int jj = l2[1]; jj <= hi[1]; jj++)
+              for (int ii = l2[0]; ii <= hi[0]; ii++) {
+                size_t p = ldm(F, ii, jj, kk);
+                if (!F->cases[18 * p + 3 * (2 * d)]) continue; /* active upper face */
+                for (int e = 0; e < 3; e++) acc[e] += F->g9[(size_t)(3 * d + e) * ((size_t)F->g.mx * F->g.my * F->g.mz) + p];
+              }
+          for (int e = 0; e < 3; e++) {
+            const double fac = (cm & (1 << e)) ? 2. : 1.;
+            C->g9[(size_t)(3 * d + e) * M + ldm(C, i, j, k)] = acc[e] / fac;
+          }
+        }
+      }
+  dm_periodic(&C->g, C->nv);
+  {
+    double *pre = (double *)malloc(M * sizeof(double));
+    memcpy(pre, C->nv, M * sizeof(double));
+    const size_t sy = C->g.mx, sz = (size_t)C->g.mx * C->g.my;
+    for (int k = 1; k < C->g.mz - 1; k++)
+      for (int j = 1; j < C->g.my - 1; j++)
+        for (int i = 1; i < C->g.mx - 1; i++) {
+          size_t p = IX(&C->g, i, j, k);
+          if (pre[p] < 0.1 && pre[p + 1] + pre[p - 1] > 1.1 && pre[p + sy] + pre[p - sy] > 1.1 &&
+              pre[p + sz] + pre[p - sz] > 1.1)
+            C->nv[p] = 1.;
+        }
+    free(pre);
+    dm_periodic(&C->g, C->nv);
+  }
+  for (int e = 0; e < 9; e++) dm_periodic(&C->g, C->g9 + (size_t)e * M);
+  lev_rows(C);
+  /* rows with no coupling become solid */
+  for (int it = 0; it < 3; it++) {
+    int changed = 0;
+    for (int k = 0; k < C->nz; k++)
+      for (int j = 0; j < C->ny; j++)
+        for (int i = 0; i < C->nx; i++) {
+          size_t p = ldm(C, i, j, k);
+          if (C->row[p] && C->vol[19 * p] == 0.) { C->nv[p] = 1.; changed = 1; }
+        }
+    if (!changed) break;
+    dm_periodic(&C->g, C->nv);
+    free(C->vol); free(C->row); free(C->cases);
+    lev_rows(C);
+  }
+}
+
+/* one Chebyshev step: z = D^-1 (f - A x), d = c1 d + c2 z, x_new = x + d */
+static void cheb_step(mg_lev *L, const double *xin, double *xout, double c1, double c2, int zero_guess) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < L->nz; k++)
+    for (int j = 0; j < L->ny; j++)
+      for (int i = 0; i < L->nx; i++) {
+        size_t c = lidx(L, i, j, k), p = ldm(L, i, j, k);
+        if (!L->row[p]) { xout[c] = 0.; L->dv[c] = 0.; continue; }
+        double diag = L->vol[19 * p];
+        double r = L->f[c] - (zero_guess ? 0. : row_apply(L, i, j, k, xin));
+        double z = diag != 0. ? r / diag : 0.;
+        double dn = (zero_guess || c1 == 0.) ? c2 * z : c1 * L->dv[c] + c2 * z;
+        L->dv[c] = dn;
+        xout[c] = (zero_guess ? 0. : xin[c]) + dn;
+      }
+}
+
+static double vnorm(size_t n, const double *a) {
+  double s = 0.;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (size_t i = 0; i < n; i++) s += a[i] * a[i];
+  return sqrt(s);
+}
+
+static double power_lambda(mg_lev *L) {
+  unsigned s = 1234u;
+  for (size_t c = 0; c < L->n; c++) {
+    s = s * 1664525u + 1013904223u;
+    L->x[c] = ((double)(s >> 8) / (double)(1u << 24)) * 2. - 1.;
+  }
+  memset(L->f, 0, L->n * sizeof(double));
+  double best = 0.;
+  for (int it = 0; it < 12; it++) {
+    double nx = vnorm(L->n, L->x);
+    for (size_t c = 0; c < L->n; c++) L->x[c] /= nx;
+    cheb_step(L, L->x, L->xt, 0., 1., 0);
+    double lam = vnorm(L->n, L->dv);
+    if (it >= 6 && lam > best) best = lam;
+    memcpy(L->x, L->dv, L->n * sizeof(double));
+  }
+  return best > 0. ? best : 2.;
+}
+
+static void cheb_coeffs(mg_lev *L, int deg, double lo, double hi) {
+  double a = lo * L->lam, b = hi * L->lam;
+  double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta, rho = 1. / sigma;
+  L->c1[0] = 0.; L->c2[0] = 1. / theta;
+  for (int k = 1; k < deg; k++) {
+    double rn = 1. / (2. * sigma - rho);
+    L->c1[k] = rn * rho; L->c2[k] = 2. * rn / delta; rho = rn;
+  }
+}
+
+static double *bordered_inverse(mg_lev *L) {
+  int n = (int)L->n;
+  int *comp = (int *)malloc(n * sizeof(int));
+  for (int c = 0; c < n; c++) comp[c] = -1;
+  int *stack = (int *)malloc(n * sizeof(int)), nc = 0;
+  for (int k0 = 0; k0 < L->nz; k0++)
+    for (int j0 = 0; j0 < L->ny; j0++)
+      for (int i0 = 0; i0 < L->nx; i0++) {
+        int c0 = (int)lidx(L, i0, j0, k0);
+        if (!L->row[ldm(L, i0, j0, k0)] || comp[c0] >= 0) continue;
+        int sp = 0;
+        stack[sp++] = c0; comp[c0] = nc;
+        while (sp) {
+          int u = stack[--sp];
+          int i = u % L->nx, j = (u / L->nx) % L->ny, k = u / (L->nx * L->ny);
+          size_t p = ldm(L, i, j, k);
+          for (int t = 1; t < 19; t++) {
+            if (L->vol[19 * p + t] == 0.) continue;
+            int a = wr(i + SLOT_OFF[t][0], L->nx, L->per[0]), b = wr(j + SLOT_OFF[t][1], L->ny, L->per[1]),
+                cc = wr(k + SLOT_OFF[t][2], L->nz, L->per[2]);
+            int v = (int)lidx(L, a, b, cc);
+            if (L->row[ldm(L, a, b, cc)] && comp[v] < 0) { comp[v] = nc; stack[sp++] = v; }
+          }
+        }
+        nc++;
+      }
+  int N = n + nc;
+  double *B = (double *)calloc((size_t)N * N, sizeof(double)), *I = (double *)calloc((size_t)N * N, sizeof(double));
+  for (int k = 0; k < L->nz; k++)
+    for (int j = 0; j < L->ny; j++)
+      for (int i = 0; i < L->nx; i++) {
+        int c = (int)lidx(L, i, j, k);
+        size_t p = ldm(L, i, j, k);
+        if (!L->row[p]) { B[(size_t)c * N + c] = 1.; continue; }
+        for (int t = 0; t < 19; t++) {
+          double v = L->vol[19 * p + t];
+          if (v == 0.) continue;
+          int a = wr(i + SLOT_OFF[t][0], L->nx, L->per[0]), b = wr(j + SLOT_OFF[t][1], L->ny, L->per[1]),
+              cc = wr(k + SLOT_OFF[t][2], L->nz, L->per[2]);
+          B[(size_t)c * N + lidx(L, a, b, cc)] += v;
+        }
+        B[(size_t)c * N + n + comp[c]] = 1.;
+        B[(size_t)(n + comp[c]) * N + c] = 1.;
+      }
+  for (int r = 0; r < N; r++) I[(size_t)r * N + r] = 1.;
+  for (int col = 0; col < N; col++) {
+    int piv = col;
+    double best = fabs(B[(size_t)col * N + col]);
+    for (int r = col + 1; r < N; r++)
+      if (fabs(B[(size_t)r * N + col]) > best) { best = fabs(B[(size_t)r * N + col]); piv = r; }
+    if (piv != col)
+      for (int t = 0; t < N; t++) {
+        double tmp = B[(size_t)piv * N + t]; B[(size_t)piv * N + t] = B[(size_t)col * N + t]; B[(size_t)col * N + t] = tmp;
+        tmp = I[(size_t)piv * N + t]; I[(size_t)piv * N + t] = I[(size_t)col * N + t]; I[(size_t)col * N + t] = tmp;
+      }
+    double inv = 1. / B[(size_t)col * N + col];
+    for (int t = 0; t < N; t++) { B[(size_t)col * N + t] *= inv; I[(size_t)col * N + t] *= inv; }
+    for (int r = 0; r < N; r++) {
+      if (r == col) continue;
+      double f = B[(size_t)r * N + col];
+      if (f == 0.) continue;
+      for (int t = 0; t < N; t++) { B[(size_t)r * N + t] -= f * B[(size_t)col * N + t]; I[(size_t)r * N + t] -= f * I[(size_t)col * N + t]; }
+    }
+  }
+  double *out = (double *)malloc((size_t)n * n * sizeof(double));
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) out[(size_t)r * n + c] = I[(size_t)r * N + c];
+  free(B); free(I); free(comp); free(stack);
+  return out;
+}
+
+or_mg *or_mg_create(const or_grid *g, const double *g9, const double *nvert, int degree, double lo, double hi,
+                    int coarse_max) {
+  or_mg *mg = (or_mg *)calloc(1, sizeof(or_mg));
+  mg->degree = degree;
+  mg_lev *L0 = &mg->L[0];
+  L0->nx = g->mx - 2; L0->ny = g->my - 2; L0->nz = g->mz - 2;
+  for (int a = 0; a < 3; a++) L0->per[a] = g->per[a];
+  L0->g = *g;
+  size_t M = (size_t)g->mx * g->my * g->mz;
+  L0->g9 = (double *)malloc(9 * M * sizeof(double));
+  memcpy(L0->g9, g9, 9 * M * sizeof(double));
+  L0->nv = (double *)malloc(M * sizeof(double));
+  memcpy(L0->nv, nvert, M * sizeof(double));
+  lev_rows(L0);
+  lev_alloc(L0);
+  mg->nl = 1;
+  while (mg->nl < 16) {
+    mg_lev *F = &mg->L[mg->nl - 1];
+    int can = F->nx >= 3 || F->ny >= 3 || F->nz >= 3;
+    if ((long)F->n <= coarse_max || !can) break;
+    mg_lev *C = &mg->L[mg->nl];
+    coarsen_level(F, C);
+    lev_alloc(C);
+    mg->nl++;
+  }
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    mg->L[l].lam = power_lambda(&mg->L[l]);
+    cheb_coeffs(&mg->L[l], degree, lo, hi);
+  }
+  mg->cinv = bordered_inverse(&mg->L[mg->nl - 1]);
+  return mg;
+}
+
+void or_mg_destroy(or_mg *mg) {
+  if (!mg) return;
+  for (int l = 0; l < mg->nl; l++) {
+    mg_lev *L = &mg->L[l];
+    free(L->g9); free(L->nv); free(L->vol); free(L->row); free(L->cases);
+    free(L->f); free(L->x); free(L->xt); free(L->dv); free(L->r);
+  }
+  free(mg->cinv);
+  free(mg);
+}
+
+static void mg_cycle(or_mg *mg, int l, double *out) {
+  mg_lev *L = &mg->L[l];
+  if (l == mg->nl - 1) {
+    int n = (int)L->n;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < n; r++) {
+      double s = 0.;
+      for (int c = 0; c < n; c++) s += mg->cinv[(size_t)r * n + c] * L->f[c];
+      out[r] = s;
+    }
+    return;
+  }
+  double *x = L->x, *y = L->xt;
+  for (int k = 0; k < mg->degree; k++) {
+    cheb_step(L, x, y, L->c1[k], L->c2[k], k == 0);
+    double *t = x; x = y; y = t;
+  }
+  mg_lev *C = &mg->L[l + 1];
+  const int cm = L->cmask;
+  /* restriction of fluid-child residuals */
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < C->nz; k++)
+    for (int j = 0; j < C->ny; j++)
+      for (int i = 0; i < C->nx; i++) {
+        size_t cc = lidx(C, i, j, k);
+        if (C->nv[ldm(C, i, j, k)] > 0.1) { C->f[cc] = 0.; continue; }
+        int ci[3] = {i, j, k}, fn[3] = {L->nx, L->ny, L->nz}, lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+          if (cm & (1 << a)) { lo[a] = 2 * ci[a]; hi[a] = 2 * ci[a] + 1 < fn[a] ? 2 * ci[a] + 1 : fn[a] - 1; }
+          else lo[a] = hi[a] = ci[a];
+        }
+        double s = 0.;
+        for (int kk = lo[2]; kk <= hi[2]; kk++)
+          for (int jj = lo[1]; jj <= hi[1]; jj++)
+            for (int ii = lo[0]; ii <= hi[0]; ii++)
+              if (L->row[ldm(L, ii, jj, kk)]) s += L->f[lidx(L, ii, jj, kk)] - row_apply(L, ii, jj, kk, x);
+        C->f[cc] = s;
+      }
+  double *xc = (double *)malloc(C->n * sizeof(double));
+  mg_cycle(mg, l + 1, xc);
+  /* prolongation and correction */
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < L->nz; k++)
+    for (int j = 0; j < L->ny; j++)
+      for (int i = 0; i < L->nx; i++) {
+        if (L->nv[ldm(L, i, j, k)] > 0.1) continue;
+        int f[3] = {i, j, k}, fn[3] = {L->nx, L->ny, L->nz}, cn[3] = {C->nx, C->ny, C->nz}, c0[3], o[3], nb[3];
+        for (int a = 0; a < 3; a++) {
+          if (!(cm & (1 << a))) { c0[a] = f[a]; o[a] = 0; }
+          else {
+            c0[a] = f[a] / 2;
+            o[a] = (f[a] & 1) ? 1 : -1;
+            if (!L->per[a] && (f[a] == 0 || f[a] == fn[a] - 1)) o[a] = 0;
+          }
+          int q = c0[a] + o[a];
+          nb[a] = q < 0 ? q + cn[a] : (q >= cn[a] ? q - cn[a] : q);
+        }
+        if (o[2] && C->nv[ldm(C, c0[0], c0[1], nb[2])] > 0.1) o[2] = 0;
+        if (o[1] && C->nv[ldm(C, c0[0], nb[1], c0[2])] > 0.1) o[1] = 0;
+        if (o[0] && C->nv[ldm(C, nb[0], c0[1], c0[2])] > 0.1) o[0] = 0;
+        int i2 = o[0] ? nb[0] : c0[0], j2 = o[1] ? nb[1] : c0[1], k2 = o[2] ? nb[2] : c0[2];
+#define XC(a, b, cc) xc[lidx(C, a, b, cc)]
+        double v = (XC(c0[0], c0[1], c0[2]) * 9 + XC(c0[0], j2, c0[2]) * 3 + XC(i2, c0[1], c0[2]) * 3 + XC(i2, j2, c0[2])) * (3. / 64.) +
+                   (XC(c0[0], c0[1], k2) * 9 + XC(c0[0], j2, k2) * 3 + XC(i2, c0[1], k2) * 3 + XC(i2, j2, k2)) * (1. / 64.);
+#undef XC
+        x[lidx(L, i, j, k)] += v;
+      }
+  free(xc);
+  for (int k = 0; k < mg->degree; k++) {
+    cheb_step(L, x, y, L->c1[k], L->c2[k], 0);
+    double *t = x; x = y; y = t;
+  }
+  memcpy(out, x, L->n * sizeof(double));
+}
+
+void or_mg_vcycle(or_mg *mg, const double *r, double *z) {
+  memcpy(mg->L[0].f, r, mg->L[0].n * sizeof(double));
+  mg_cycle(mg, 0, z);
+}
+
+size_t or_mg_n(const or_mg *mg) { return mg->L[0].n; }
+int or_mg_levels(const or_mg *mg) { return mg->nl; }
+
+/* FGMRES(m) + CGS2 on interior vectors, right preconditioned by one V-cycle.
+ * b, x interior (x holds the initial guess).  Returns iterations. */
+int or_mg_fgmres(or_mg *mg, const double *b, double *x, int m, double rtol, int maxit, double *rel, int *vcyc) {
+  mg_lev *L0 = &mg->L[0];
+  size_t n = L0->n;
+  double *V = (double *)malloc((size_t)(m + 1) * n * sizeof(double));
+  double *Z = (double *)malloc((size_t)m * n * sizeof(double));
+  double *H = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = (double *)malloc(m * sizeof(double)), *sn = (double *)malloc(m * sizeof(double));
+  double *gv = (double *)malloc((m + 1) * sizeof(double)), *y = (double *)malloc(m * sizeof(double));
+  double bn = vnorm(n, b);
+  int it = 0;
+  double res = 1.;
+  *vcyc = 0;
+  for (;;) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < L0->nz; k++)
+      for (int j = 0; j < L0->ny; j++)
+        for (int i = 0; i < L0->nx; i++) {
+          size_t c = lidx(L0, i, j, k);
+          V[c] = L0->row[ldm(L0, i, j, k)] ? b[c] - row_apply(L0, i, j, k, x) : 0.;
+        }
+    double beta = vnorm(n, V);
+    res = bn > 0 ? beta / bn : 0.;
+    if (beta <= rtol * bn || it >= maxit) break;
+    for (size_t i = 0; i < n; i++) V[i] /= beta;
+    for (int i = 0; i <= m; i++) gv[i] = 0.;
+    gv[0] = beta;
+    int j;
+    for (j = 0; j < m && it < maxit; j++) {
+      double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+      or_mg_vcycle(mg, vj, zj);
+      (*vcyc)++;
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < L0->nz; k++)
+        for (int jj = 0; jj < L0->ny; jj++)
+          for (int i = 0; i < L0->nx; i++) w[lidx(L0, i, jj, k)] = row_apply(L0, i, jj, k, zj);
+      for (int l = 0; l <= j; l++) H[l * m + j] = 0.;
+      for (int pass = 0; pass < 2; pass++)
+        for (int l = 0; l <= j; l++) {
+          const double *vl = V + (size_t)l * n;
+          double h = 0.;
+#pragma omp parallel for reduction(+ : h) schedule(static)
+          for (size_t i = 0; i < n; i++) h += w[i] * vl[i];
+#pragma omp parallel for schedule(static)
+          for (size_t i = 0; i < n; i++) w[i] -= h * vl[i];
+          H[l * m + j] += h;
+        }
+      double hn = vnorm(n, w);
+      H[(j + 1) * m + j] = hn;
+      if (hn != 0.) for (size_t i = 0; i < n; i++) w[i] /= hn;
+      for (int l = 0; l < j; l++) {
+        double t1 = cs[l] * H[l * m + j] + sn[l] * H[(l + 1) * m + j];
+        double t2 = -sn[l] * H[l * m + j] + cs[l] * H[(l + 1) * m + j];
+        H[l * m + j] = t1; H[(l + 1) * m + j] = t2;
+      }
+      double a = H[j * m + j], bb = H[(j + 1) * m + j], r = sqrt(a * a + bb * bb);
+      cs[j] = r == 0. ? 1. : a / r; sn[j] = r == 0. ? 0. : bb / r;
+      H[j * m + j] = r; H[(j + 1) * m + j] = 0.;
+      gv[j + 1] = -sn[j] * gv[j]; gv[j] = cs[j] * gv[j];
+      it++;
+      if (fabs(gv[j + 1]) <= rtol * bn) { j++; break; }
+    }
+    for (int l = j - 1; l >= 0; l--) {
+      double s = gv[l];
+      for (int q = l + 1; q < j; q++) s -= H[l * m + q] * y[q];
+      y[l] = s / H[l * m + l];
+    }
+    for (int l = 0; l < j; l++) {
+      const double *zl = Z + (size_t)l * n;
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < n; i++) x[i] += y[l] * zl[i];
+    }
+  }
+  if (rel) *rel = res;
+  free(V); free(Z); free(H); free(cs); free(sn); free(gv); free(y);
+  return it;
+}

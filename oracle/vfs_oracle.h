@@ -126,6 +126,18 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
 
 int or_num_threads(void);
 
+/* CPU port of the engine's solver (geometric multigrid + FGMRES/CGS2), on
+ * interior vectors (nx*ny*nz, i fastest).  g9 and nvert are the DMDA arrays of
+ * or_face_coeffs and the case. */
+typedef struct or_mg or_mg;
+or_mg *or_mg_create(const or_grid *g, const double *g9, const double *nvert, int degree, double lo, double hi,
+                    int coarse_max);
+void or_mg_destroy(or_mg *mg);
+void or_mg_vcycle(or_mg *mg, const double *r, double *z);
+size_t or_mg_n(const or_mg *mg);
+int or_mg_levels(const or_mg *mg);
+int or_mg_fgmres(or_mg *mg, const double *b, double *x, int m, double rtol, int maxit, double *rel, int *vcyc);
+
 #ifdef __cplusplus
 }
 #endif

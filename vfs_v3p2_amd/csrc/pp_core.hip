@@ -573,6 +573,104 @@ __global__ __launch_bounds__(256) void k_fill(size_t n, T* a, T v) {
   if (t < n) a[t] = v;
 }
 
+
+// w = A z (flux form) and per-block partial sums of w.V_l (l < nv) and w.w; the
+// first Krylov reduction of classical Gram-Schmidt fused into the operator pass.
+template <typename TZ>
+__global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __restrict__ g,
+                                                    const uint32_t* __restrict__ code, const TZ* __restrict__ z,
+                                                    double* __restrict__ w, const double* __restrict__ V, size_t ldv,
+                                                    int nv, double* __restrict__ partial) {
+  __shared__ double lds[4];
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  const int nblk = gridDim.x * gridDim.y * gridDim.z, b = blin();
+  const bool in = i < d.nx && j < d.ny;
+  size_t c = 0;
+  double wv = 0.;
+  if (in) {
+    c = d.idx(i, j, k);
+    const uint32_t cd = code[c];
+    Nbr nb;
+    neighbours(d, i, j, k, nb);
+    double xv[19], ax, dg;
+    load_x(nb, z, xv);
+    flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+    wv = (cd & ROW_BIT) ? ax : 0.;
+    w[c] = wv;
+  }
+  for (int l = 0; l <= nv; l++) {
+    double p = 0.;
+    if (in) p = (l < nv) ? wv * V[(size_t)l * ldv + c] : wv * wv;
+    p = block_sum(p, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[(size_t)l * nblk + b] = p;
+  }
+}
+
+
+// CGS2 second pass, fused: w1 = w - sum_l h_l V_l (in place) and per-block
+// partial sums of w1.V_l (l < nv) and w1.w1.  One element per thread; the second
+// read of V_l[t] hits the cache line the first one brought in.
+__global__ __launch_bounds__(256) void k_axpy_dots(size_t n, double* __restrict__ w, const double* __restrict__ V,
+                                                   size_t ldv, int nv, const double* __restrict__ h,
+                                                   double* __restrict__ partial) {
+  __shared__ double lds[4];
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int nblk = gridDim.x;
+  double w1 = 0.;
+  if (t < n) {
+    w1 = w[t];
+    for (int l = 0; l < nv; l++) w1 -= h[l] * V[(size_t)l * ldv + t];
+    w[t] = w1;
+  }
+  for (int l = 0; l <= nv; l++) {
+    double p = 0.;
+    if (t < n) p = (l < nv) ? w1 * V[(size_t)l * ldv + t] : w1 * w1;
+    p = block_sum(p, lds);
+    if (threadIdx.x == 0) partial[(size_t)l * nblk + blockIdx.x] = p;
+  }
+}
+
+// red[1] = ||w - V h|| = sqrt(w.w - h.h), red[2] = 1/red[1] with h = red[off..off+nv)
+// and w.w = red[off+nv]; mirrored to host
+__global__ void k_gs_norm(double* __restrict__ red, int off, int nv, double* __restrict__ host) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double* h = red + off;
+  double hh = 0.;
+  for (int l = 0; l < nv; l++) hh += h[l] * h[l];
+  const double s = h[nv] - hh;  // h[nv] holds w.w
+  const double nrm = s > 0. ? sqrt(s) : 0.;
+  red[1] = nrm;
+  red[2] = nrm > 0. ? 1. / nrm : 0.;
+  if (host) { host[1] = nrm; host[2] = red[2]; }
+}
+
+// v = (w - sum_l h_l V_l) * inv (in place over w); f = (TF)v for the next V-cycle
+template <typename TF>
+__global__ __launch_bounds__(256) void k_axpy_scale(size_t n, double* __restrict__ w, const double* __restrict__ V,
+                                                    size_t ldv, int nv, const double* __restrict__ h,
+                                                    const double* __restrict__ inv, TF* __restrict__ f) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  double acc = w[t];
+  for (int l = 0; l < nv; l++) acc -= h[l] * V[(size_t)l * ldv + t];
+  const double v = acc * inv[0];
+  w[t] = v;
+  if (f) f[t] = (TF)v;
+}
+
+// v = a / sqrt(norm2); f = (TF)v
+template <typename TF>
+__global__ __launch_bounds__(256) void k_scale_conv(size_t n, const double* __restrict__ a,
+                                                    const double* __restrict__ nrm2, double* __restrict__ v,
+                                                    TF* __restrict__ f) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const double s = nrm2[0] > 0. ? 1. / sqrt(nrm2[0]) : 0.;
+  const double x = a[t] * s;
+  v[t] = x;
+  if (f) f[t] = (TF)x;
+}
+
 inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -788,5 +886,49 @@ void launch_fillf(hipStream_t s, size_t n, float* a, float v) {
   k_fill<float><<<nb1(n), 256, 0, s>>>(n, a, v);
   HIPCHECK(hipGetLastError());
 }
+
+template <typename TZ>
+void launch_apply_dots(hipStream_t s, Dims d, const double* g, const uint32_t* code, const TZ* z, double* w,
+                       const double* V, size_t ldv, int nv, double* partial, int* nblk) {
+  const dim3 gr = cell_grid(d);
+  *nblk = (int)(gr.x * gr.y * gr.z);
+  k_apply_dots<TZ><<<gr, kBlk, 0, s>>>(d, g, code, z, w, V, ldv, nv, partial);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_apply_dots<float>(hipStream_t, Dims, const double*, const uint32_t*, const float*, double*,
+                                       const double*, size_t, int, double*, int*);
+template void launch_apply_dots<double>(hipStream_t, Dims, const double*, const uint32_t*, const double*, double*,
+                                        const double*, size_t, int, double*, int*);
+
+void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                      double* partial, int* nblk) {
+  *nblk = (int)nb1(n);
+  k_axpy_dots<<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_gs_norm(hipStream_t s, double* red, int off, int nv, double* host) {
+  k_gs_norm<<<1, 64, 0, s>>>(red, off, nv, host);
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename TF>
+void launch_axpy_scale(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                       const double* inv, TF* f) {
+  k_axpy_scale<TF><<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, inv, f);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_axpy_scale<float>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
+                                       const double*, float*);
+template void launch_axpy_scale<double>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
+                                        const double*, double*);
+
+template <typename TF>
+void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f) {
+  k_scale_conv<TF><<<nb1(n), 256, 0, s>>>(n, a, nrm2, v, f);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_scale_conv<float>(hipStream_t, size_t, const double*, const double*, double*, float*);
+template void launch_scale_conv<double>(hipStream_t, size_t, const double*, const double*, double*, double*);
 
 }  // namespace pp

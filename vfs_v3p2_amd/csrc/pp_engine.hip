@@ -39,11 +39,13 @@ class Arena {
       throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
     // every buffer starts at zero so no result can depend on earlier contents of
-    // the device heap (set-up time, not on the solve path)
-    e = hipMemset(p, 0, bytes);
+    // the device heap.  The memset goes to the null stream and is waited for
+    // here: the engine's own stream is non-blocking and would otherwise race it.
+    e = hipMemsetAsync(p, 0, bytes, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
       (void)hipFree(p);
-      throw StatusError(VFS_PP_EHIP, "hipMemset failed");
+      throw StatusError(VFS_PP_EHIP, "zeroing a new buffer failed");
     }
     ptrs_.push_back(p);
     total_ += bytes;
@@ -322,8 +324,8 @@ struct MG : MGBase {
   void set_out(void* o) override { out = static_cast<T*>(o); }
 
   // Chebyshev coefficients on [lo*lam, hi*lam]
-  static void cheb(double lam, int deg, std::vector<T>& c1, std::vector<T>& c2) {
-    const double a = 0.3 * lam, b = 1.1 * lam;
+  static void cheb(double lam, int deg, double lo, double hi, std::vector<T>& c1, std::vector<T>& c2) {
+    const double a = lo * lam, b = hi * lam;
     const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
     double rho = 1. / sigma;
     c1.assign(deg, T(0));
@@ -418,6 +420,8 @@ struct vfs_pp_handle {
   // vectors
   double *x = nullptr, *b = nullptr, *r = nullptr;
   double* partial = nullptr;
+  size_t partial_cap = 0;  // doubles in `partial`
+  size_t stencil_blocks = 0;  // blocks of one cell-grid launch
   double* red = nullptr;   // device scratch scalars (>= 256)
   double* snap = nullptr;  // DMDA vector scratch for Contra2Cart
   double* V = nullptr;
@@ -536,7 +540,7 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   for (size_t l = 0; l + 1 < mg->lev.size(); l++) {
     auto& L = mg->lev[l];
     L.lam = estimate_lambda<T>(h, *mg, L);
-    MG<T>::cheb(L.lam, mg->degree, L.c1, L.c2);
+    MG<T>::cheb(L.lam, mg->degree, h->opts.cheb_lo, h->opts.cheb_hi, L.c1, L.c2);
   }
   // coarsest dense inverse
   HostLevel& Hc = h->hl.back();
@@ -550,6 +554,12 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   h2d(h->s, mg->cinv, cit.data(), cit.size());
   HIPCHECK(hipStreamSynchronize(h->s));
   return mg;
+}
+
+static void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk) {
+  if (nvec * nblk > h->partial_cap)
+    throw StatusError(VFS_PP_EINVAL, "reduction scratch too small (" + std::to_string(nvec * nblk) + " > " +
+                                         std::to_string(h->partial_cap) + ")");
 }
 
 static double finalize_to_host(vfs_pp_handle* h, int nvec, int nblk, int op, double* out) {
@@ -580,34 +590,54 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
     const double beta = std::sqrt(beta2);
     rel = bnorm > 0. ? beta / bnorm : 0.;
     if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
+    if (h->opts.verbose) std::fprintf(stderr, "vfs_pp: restart at it %d  true rel %.3e\n", it, rel);
     if (beta <= tgt) { *conv = true; break; }
     if (it >= h->opts.max_iters) break;
-    launch_scale_to(h->s, n, h->r, h->red, h->V);  // red[0] still holds beta^2
+    // v_0 = r / beta and its copy as the first V-cycle input
+    launch_scale_conv<TZ>(h->s, n, h->r, h->red, h->V, static_cast<TZ*>(h->mg->f0()));
     std::fill(gv.begin(), gv.end(), 0.);
     gv[0] = beta;
     int j = 0;
     for (; j < m && it < h->opts.max_iters; j++) {
-      double* vj = h->V + (size_t)j * n;
       double* w = h->V + (size_t)(j + 1) * n;
-      // z_j = M^-1 v_j
-      if (std::is_same<TZ, float>::value) launch_convert<double, float>(h->s, n, vj, (float*)h->mg->f0());
-      else HIPCHECK(hipMemcpyAsync(h->mg->f0(), vj, n * sizeof(double), hipMemcpyDeviceToDevice, h->s));
+      // z_j = M^-1 v_j (the V-cycle reads v_j from its level-0 right-hand side)
       h->mg->set_out(Z + (size_t)j * n);
       h->mg->vcycle(h->s);
       (*vcyc)++;
-      launch_apply64<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->opts.drop_small != 0);
-      // CGS2
-      for (int pass = 0; pass < 2; pass++) {
-        launch_multidot(h->s, n, w, h->V, n, j + 1, h->partial, &nb);
-        launch_finalize(h->s, h->partial, j + 1, nb, h->red + 8 + pass * 64, 0, h->hpin_dev + 8 + pass * 64);
-        launch_axpy_multi(h->s, n, w, h->V, n, j + 1, h->red + 8 + pass * 64, -1.);
+      double sub = 0.;
+      if (h->opts.orthog == 1) {
+        // two-pass classical Gram-Schmidt (CGS2)
+        launch_apply64<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, false);
+        for (int pass = 0; pass < 2; pass++) {
+          need_partials(h, (size_t)j + 1, (size_t)partial_blocks(n));
+          launch_multidot(h->s, n, w, h->V, n, j + 1, h->partial, &nb);
+          launch_finalize(h->s, h->partial, j + 1, nb, h->red + 8 + pass * 64, 0, h->hpin_dev + 8 + pass * 64);
+          launch_axpy_multi(h->s, n, w, h->V, n, j + 1, h->red + 8 + pass * 64, -1.);
+        }
+        launch_norm2(h->s, n, w, h->partial, &nb);
+        launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+        launch_scale_conv<TZ>(h->s, n, w, h->red, w, static_cast<TZ*>(h->mg->f0()));
+        HIPCHECK(hipStreamSynchronize(h->s));
+        for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[8 + 64 + l];
+        sub = std::sqrt(h->hpin[0]);
+      } else {
+        // CGS2 in three passes over the basis: w = A z_j with h1 = V^T w fused into
+        // the operator pass; w1 = w - V h1 with h2 = V^T w1 and |w1|^2 fused; then
+        // v_{j+1} = (w1 - V h2) / sqrt(|w1|^2 - |h2|^2) together with the next
+        // V-cycle's input.  After the first pass w1 is nearly orthogonal to V,
+        // so the last norm has no cancellation.
+        need_partials(h, (size_t)j + 2, std::max(h->stencil_blocks, (n + 255) / 256));
+        launch_apply_dots<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->V, n, j + 1, h->partial, &nb);
+        launch_finalize(h->s, h->partial, j + 1, nb, h->red + 8, 0, h->hpin_dev + 8);
+        launch_axpy_dots(h->s, n, w, h->V, n, j + 1, h->red + 8, h->partial, &nb);
+        launch_finalize(h->s, h->partial, j + 2, nb, h->red + 72, 0, h->hpin_dev + 72);
+        launch_gs_norm(h->s, h->red, 72, j + 1, h->hpin_dev);
+        launch_axpy_scale<TZ>(h->s, n, w, h->V, n, j + 1, h->red + 72, h->red + 2, static_cast<TZ*>(h->mg->f0()));
+        HIPCHECK(hipStreamSynchronize(h->s));
+        for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[72 + l];
+        sub = h->hpin[1];
       }
-      launch_norm2(h->s, n, w, h->partial, &nb);
-      launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
-      launch_scale_to(h->s, n, w, h->red, w);
-      HIPCHECK(hipStreamSynchronize(h->s));
-      for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[8 + 64 + l];
-      H[(size_t)(j + 1) * m + j] = std::sqrt(h->hpin[0]);
+      H[(size_t)(j + 1) * m + j] = sub;
       for (int l = 0; l < j; l++) {
         const double t1 = cs[l] * H[(size_t)l * m + j] + sn[l] * H[(size_t)(l + 1) * m + j];
         const double t2 = -sn[l] * H[(size_t)l * m + j] + cs[l] * H[(size_t)(l + 1) * m + j];
@@ -624,6 +654,9 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
       gv[j] = cs[j] * gv[j];
       it++;
       if (!std::isfinite(gv[j + 1])) throw StatusError(VFS_PP_ENONFINITE, "non-finite Krylov residual");
+      if (h->opts.verbose)
+        std::fprintf(stderr, "vfs_pp: it %3d  est rel %.3e  h_jj %.3e  sub %.3e\n", it, std::fabs(gv[j + 1]) / bnorm,
+                     H[(size_t)j * m + j], sub);
       if (std::fabs(gv[j + 1]) <= tgt) { j++; break; }
     }
     for (int l = j - 1; l >= 0; l--) {
@@ -661,10 +694,15 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
     h->x = h->ar.alloc<double>(n);
     h->b = h->ar.alloc<double>(n);
     h->r = h->ar.alloc<double>(n);
-    h->partial = h->ar.alloc<double>(64 * 1024 + (size_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz);
+    h->mrest = std::max(1, std::min(h->opts.restart, 60));
+    // per-block partial sums: up to mrest+2 vectors per launch (fused Krylov
+    // reductions over the cell grid), or 1024 blocks for grid-stride reductions
+    h->stencil_blocks = (size_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz;
+    h->partial_cap =
+        (size_t)(h->mrest + 4) * std::max<size_t>(std::max<size_t>(h->stencil_blocks, (n + 255) / 256), 1024) + 4096;
+    h->partial = h->ar.alloc<double>(h->partial_cap);
     h->red = h->ar.alloc<double>(512);
     h->snap = h->ar.alloc<double>(3 * M);
-    h->mrest = std::max(1, std::min(h->opts.restart, 60));
     h->V = h->ar.alloc<double>((size_t)(h->mrest + 1) * n);
     if (h->opts.precision == 1) h->Z = h->ar.alloc<float>((size_t)h->mrest * n);
     else h->Z = h->ar.alloc<double>((size_t)h->mrest * n);
@@ -889,10 +927,14 @@ void vfs_pp_default_opts(vfs_pp_opts* o) {
   o->max_iters = 200;
   o->restart = 30;
   o->smoother = 0;
-  o->cheb_degree = 2;
+  o->cheb_degree = 3;
   o->coarse_max = 512;
   o->drop_small = 0;
   o->zero_guess = 1;
+  o->cheb_lo = 0.15;
+  o->cheb_hi = 1.1;
+  o->orthog = 0;
+  o->verbose = 0;
 }
 
 const char* vfs_pp_error_string(int code) {
@@ -919,6 +961,7 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
   if (o) h->opts = *o;
   else vfs_pp_default_opts(&h->opts);
   if (!(h->opts.rtol > 0.) || h->opts.max_iters < 0 || h->opts.restart < 1) return VFS_PP_EINVAL;
+  if (!(h->opts.cheb_lo > 0. && h->opts.cheb_lo < h->opts.cheb_hi)) return VFS_PP_EINVAL;
   h->d = Dims{g->nx, g->ny, g->nz, {g->periodic[0] ? 1 : 0, g->periodic[1] ? 1 : 0, g->periodic[2] ? 1 : 0}};
   h->m = Dmda{g->nx + 2, g->ny + 2, g->nz + 2, {h->d.per[0], h->d.per[1], h->d.per[2]}};
   HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));

@@ -96,6 +96,21 @@ void launch_fillf(hipStream_t s, size_t n, float* a, float v);
 
 int partial_blocks(size_t n);
 
+// fused Krylov step: w = A z with w.V_l (l < nv) and w.w partials (nv+1 vectors)
+template <typename TZ>
+void launch_apply_dots(hipStream_t s, Dims d, const double* g, const uint32_t* code, const TZ* z, double* w,
+                       const double* V, size_t ldv, int nv, double* partial, int* nblk);
+// CGS2 second pass: w -= V h and w.V_l, w.w partials (nv+1 vectors, n/256 blocks)
+void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                      double* partial, int* nblk);
+// red[1] = sqrt(red[off+nv] - sum red[off+l]^2), red[2] = 1/red[1]
+void launch_gs_norm(hipStream_t s, double* red, int off, int nv, double* host);
+template <typename TF>
+void launch_axpy_scale(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
+                       const double* inv, TF* f);
+template <typename TF>
+void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f);
+
 // ---- multigrid ---------------------------------------------------------------------
 template <typename T>
 struct LevelView {
