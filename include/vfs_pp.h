@@ -68,7 +68,10 @@ typedef struct {
   int coarse_max;    /* cell budget of the coarsest (direct-solve) level */
   int drop_small;    /* 1: drop |a_ij| <= 1e-10 like the assembled HYPRE matrix
                         (poisson.c:1608); 0: conservative operator (default) */
-  int zero_guess;    /* 1: start from phi = 0; 0: start from the phi passed in */
+  int zero_guess;    /* 1: start from phi = 0 (default);
+                        0: start from the phi passed in, as the reference does from its
+                        second time step on (its HYPRE solution vector persists between
+                        calls, poisson_hypre.c:565-574, 722) */
   double cheb_lo;    /* Chebyshev smoothing interval [cheb_lo, cheb_hi] x lambda_max(M^-1 A) */
   double cheb_hi;
   int orthog;        /* 0: one-pass classical Gram-Schmidt fused into the operator pass,
@@ -153,7 +156,9 @@ int vfs_pp_apply_device(vfs_pp_handle *h, const double *phi, double *y, int drop
 int vfs_pp_export_stencil(vfs_pp_handle *h, double *vol19, uint8_t *cases);
 
 /* IB classification (ibm.c:331-353, 540-568) on a device DMDA array holding 4
- * for inside cells and 0 elsewhere; rewritten in place to {0,1,3}. */
+ * for inside cells and 0 elsewhere; rewritten in place: interior cells to
+ * {0,1,3}; boundary-plane points next to an inside cell keep the dilation's 2,
+ * as in the reference (its 2->1 remap covers the interior only). */
 int vfs_pp_classify_nvert_device(const vfs_grid *grid, double *nvert);
 
 /* Multigrid transfer primitives of the reference on DMDA arrays, fully coarsened

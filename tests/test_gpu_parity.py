@@ -1,0 +1,353 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Bars: stencil coefficients, branch codes, IB classification and the reference
+multigrid primitives bit-exact; operator applications to rounding; pressure
+increment phi within relative L2 1e-8 of the oracle's FP64 GMRES solution after
+null-space removal, max|div u| <= 1e-8, and the projected fields consistent with
+the oracle's projection of the engine's own phi to rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfs_v3p2_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+PHI_TOL = 1e-8
+DIV_TOL = 1e-8
+
+
+def affine_case(nx=10, ny=9, nz=8, seed=0):
+    """Non-periodic grid under a full (sheared) affine map with a random flux field
+    whose boundary faces carry no flux (a consistent Neumann problem)."""
+    M = np.array([[1.0, 0.35, -0.2], [0.15, 0.8, 0.1], [-0.25, 0.3, 1.2]])
+    ii, jj, kk = np.meshgrid(np.arange(nx + 1) / nx, np.arange(ny + 1) / ny, np.arange(nz + 1) / nz, indexing="xy")
+    N = np.stack([ii, jj, kk], -1).transpose(2, 0, 1, 3) @ M.T
+    co = cases.dmda_coords(N)
+    f = cases.form_metrics(co, (0, 0, 0))
+    f["nvert"] = np.zeros((nz + 2, ny + 2, nx + 2))
+    rng = np.random.default_rng(seed)
+    u = np.zeros((nz + 2, ny + 2, nx + 2, 3))
+    u[1:-1, 1:-1, 1:-2, 0] = rng.standard_normal((nz, ny, nx - 1))
+    u[1:-1, 1:-2, 1:-1, 1] = rng.standard_normal((nz, ny - 1, nx))
+    u[1:-2, 1:-1, 1:-1, 2] = rng.standard_normal((nz - 1, ny, nx))
+    f["ucont"] = u
+    return cases.Case(name="affine", nx=nx, ny=ny, nz=nz, per=(0, 0, 0), bctype=(0,) * 6, immersed=0, fields=f)
+
+
+def _sphere_immersed0():
+    c = cases.hill_case(24, 16, 16, h=0.3, sphere=(0.6, 0.75, 0.0, 0.3), seed=1)
+    c.immersed = 0
+    c.name += "-immersed0"
+    return c
+
+
+CASES = {
+    "channel": lambda: cases.channel_case(16, 16, 12, beta=2.0, seed=0),
+    "hill_sphere": lambda: cases.hill_case(24, 16, 16, h=0.3, sphere=(0.6, 0.75, 0.0, 0.3), seed=0),
+    "hill_sphere_immersed0": _sphere_immersed0,
+    "towers": lambda: cases.towers_case(48, 24, 24, ratio=1.05, rows=2, cols=2, radius_cells=3, seed=2),
+    "affine": affine_case,
+}
+_cache = {}
+
+
+def get_case(name):
+    if name not in _cache:
+        c = CASES[name]()
+        _cache[name] = (c, O.Assembled(c, drop=False), O.reference_projection(c, rtol=1e-13))
+    return _cache[name]
+
+
+def handle_for(E, c, **kw):
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), E.default_opts(**kw))
+    h.set_geometry(c.fields)
+    return h
+
+
+def relL2(a, b, mask):
+    return float(np.linalg.norm((a - b)[mask]) / np.linalg.norm(b[mask]))
+
+
+@pytest.fixture(scope="module")
+def E(torch_cuda):
+    from vfs_v3p2_amd import engine
+
+    engine.lib()
+    return engine
+
+
+# ------------------------------------------------------------------ operator
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stencil_bit_exact(E, name):
+    c, A, _ = get_case(name)
+    h = handle_for(E, c)
+    vol, cs = h.export_stencil()
+    h.close()
+    np.testing.assert_array_equal(cs, A.cases)
+    np.testing.assert_array_equal(vol, A.vol)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_apply_matches_oracle(E, name, torch_cuda):
+    torch = torch_cuda
+    c, A, _ = get_case(name)
+    h = handle_for(E, c)
+    rng = np.random.default_rng(11)
+    phi = rng.standard_normal(c.shape)
+    cases.periodic_fill(phi, c.per)
+    y_ref = O.stencil_apply(A.g, A.vol, A.rowflag, phi)
+    tol = 1e-13 * np.abs(A.vol).max() * np.abs(phi).max() * 19
+    pd = torch.from_numpy(phi).cuda()
+    y = torch.zeros_like(pd)
+    h.apply(pd, y, drop_small=False)
+    fl = A.fluid_cells
+    np.testing.assert_allclose(y.cpu().numpy()[fl], y_ref[fl], rtol=0, atol=tol)
+    # the assembled matrix with the |a| <= 1e-10 drop rule
+    rp, col, val = O.assemble_csr(A.g, A.vol, A.rowflag, A.gid, A.n, drop=True)
+    yd_ref = O.csr_matvec(rp, col, val, A.to_compact(phi))
+    h.apply(pd, y, drop_small=True)
+    np.testing.assert_allclose(y.cpu().numpy()[fl], yd_ref, rtol=0, atol=tol)
+    h.close()
+
+
+# ------------------------------------------------------- reference primitives
+
+def _inside_mask(shape, seed):
+    c = cases.hill_case(shape[2], shape[1], shape[0], h=0.3, sphere=None, seed=0)
+    co = cases.dmda_coords(cases.make_nodes(shape[2], shape[1], shape[0], -3 + np.arange(shape[2] + 1) * 6 / shape[2],
+                                            np.arange(shape[1] + 1) / shape[1],
+                                            -2 + np.arange(shape[0] + 1) * 4 / shape[0], hill_h=0.3))
+    ins = cases.inside_sphere(cases.cell_centres(co), (0.4, 0.7, 0.1), 0.5)
+    rng = np.random.default_rng(seed)
+    ins |= rng.random(ins.shape) < 0.03  # isolated speckles exercise the neighbour rules
+    return cases.interior_only(ins), c
+
+
+@pytest.mark.parametrize("bctype", [(0, 0, 1, 1, 0, 0), (1, 1, 1, 1, 1, 1)])
+def test_classify_nvert_bit_exact(E, torch_cuda, bctype):
+    torch = torch_cuda
+    ins, c = _inside_mask((16, 12, 20), 3)
+    nv = np.where(ins, 4.0, 0.0)
+    g = E.make_grid(20, 12, 16, c.per, bctype)
+    d = torch.from_numpy(nv.copy()).cuda()
+    E.classify_nvert_device(g, d)
+    O.nvert_classify(O.grid(nv.shape, c.per), bctype, nv)
+    np.testing.assert_array_equal(d.cpu().numpy(), nv)
+
+
+def test_reference_multigrid_primitives_bit_exact(E, torch_cuda):
+    torch = torch_cuda
+    ins, c = _inside_mask((16, 12, 20), 5)
+    nvf = np.where(ins, 4.0, 0.0)
+    gf = O.grid(nvf.shape, (0, 0, 0))
+    O.nvert_classify(gf, c.bctype, nvf)
+    g = E.make_grid(20, 12, 16, (0, 0, 0), c.bctype)
+    gc = O.grid(E.coarse_dmda_shape(g), (0, 0, 0))
+    nvc_ref = O.mg_nvert_restrict(gf, gc, nvf)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    nvc = torch.zeros(E.coarse_dmda_shape(g), dtype=torch.float64, device="cuda")
+    E.mg_nvert_restrict_device(g, T(nvf), nvc)
+    np.testing.assert_array_equal(nvc.cpu().numpy(), nvc_ref)
+    rng = np.random.default_rng(2)
+    xf = rng.standard_normal(nvf.shape)
+    xc = rng.standard_normal(nvc_ref.shape)
+    fc = torch.zeros_like(nvc)
+    E.mg_restrict_device(g, T(nvf), T(nvc_ref), T(xf), fc)
+    np.testing.assert_array_equal(fc.cpu().numpy(), O.mg_restrict(gf, gc, nvf, nvc_ref, xf))
+    ff = torch.zeros(nvf.shape, dtype=torch.float64, device="cuda")
+    E.mg_prolong_device(g, T(nvf), T(nvc_ref), T(xc), ff)
+    np.testing.assert_array_equal(ff.cpu().numpy(), O.mg_prolong(gf, gc, nvf, nvc_ref, xc))
+
+
+# --------------------------------------------------------- full projection
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("name", list(CASES))
+def test_projection_parity(E, name, precision):
+    c, A, ref = get_case(name)
+    print(f"case {name} {c.nx}x{c.ny}x{c.nz} precision {precision}", flush=True)
+    user = E.UserCtx(c, E.default_opts(precision=precision, rtol=1e-12))
+    st = E.Flow_Solver_pressure_step(user, strict=True)
+    fl = A.fluid_cells
+    phi = user.Phi.cpu().numpy()
+    assert relL2(phi, ref["phi"], fl) <= PHI_TOL
+    assert relL2(user.P.cpu().numpy(), ref["p"], fl) <= PHI_TOL
+    assert st.max_div <= DIV_TOL and ref["maxdiv"] <= DIV_TOL
+    # projection and Contra2Cart of the engine's own phi, by the oracle
+    u = np.ascontiguousarray(c.fields["ucont"].copy())
+    vf = O.volume_flux(A.g, c.immersed, u, c.fields)
+    np.testing.assert_allclose(st.ibm_flux, vf[:4], rtol=1e-12, atol=1e-15)
+    O.projection(A.g, c.fields, np.ascontiguousarray(phi), u, c.dt, c.st, c.coeff)
+    ucat = np.zeros_like(u)
+    O.contra2cart(A.g, c.bctype, c.fields, u, ucat)
+    ug = user.Ucont.cpu().numpy()
+    np.testing.assert_allclose(ug, u, rtol=0, atol=1e-13 * np.abs(u).max())
+    np.testing.assert_allclose(user.Ucat.cpu().numpy(), ucat, rtol=0, atol=1e-13 * np.abs(ucat).max())
+    md = O.divergence(A.g, ug, np.ascontiguousarray(c.fields["aj"]), A.nvert)
+    assert abs(md - st.max_div) <= 1e-12 * max(md, 1e-300) + 1e-20
+    assert abs(E.Divergence(user) - st.max_div) <= 1e-15
+    user.handle.close()
+
+
+def test_drop_rule_solve(E):
+    """drop_small=1 solves the assembled HYPRE matrix (|a| <= 1e-10 dropped)."""
+    c = cases.hill_case(16, 10, 12, h=0.3, sphere=(0.5, 0.8, 0.0, 0.4), seed=6)
+    A = O.Assembled(c, drop=True)
+    ref = O.reference_projection(c, rtol=1e-12, drop=True)
+    user = E.UserCtx(c, E.default_opts(precision=0, rtol=1e-11, drop_small=1))
+    st = E.Flow_Solver_pressure_step(user, strict=True)
+    assert relL2(user.Phi.cpu().numpy(), ref["phi"], A.fluid_cells) <= PHI_TOL
+    assert st.max_div <= DIV_TOL
+    user.handle.close()
+
+
+def test_deterministic_across_handles_and_calls(E, torch_cuda):
+    torch = torch_cuda
+    c, A, _ = get_case("hill_sphere")
+    outs = []
+    for _ in range(2):
+        h = handle_for(E, c)
+        for _ in range(2):
+            u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+            phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+            p = torch.zeros_like(phi)
+            ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+            h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+            outs.append((phi.cpu().numpy(), u.cpu().numpy(), ucat.cpu().numpy()))
+        h.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_warm_start_from_previous_phi(E, torch_cuda):
+    """zero_guess=0 starts from the Phi passed in, as the reference's persistent
+    HYPRE vector does: the next step's solution is the same, in fewer iterations."""
+    torch = torch_cuda
+    c, A, _ = get_case("hill_sphere")
+    h = handle_for(E, c, rtol=1e-12)
+    u0 = np.ascontiguousarray(c.fields["ucont"])
+    u1 = u0 * (1.0 + 0.01 * np.random.default_rng(4).standard_normal(u0.shape))
+    cases.set_periodic_flux_ghosts(u1, c.per)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+    h.poisson_solve(dev(u0), phi, c.dt, c.st, c.coeff, strict=True)
+    phi_cold = torch.zeros_like(phi)
+    cold = h.poisson_solve(dev(u1), phi_cold, c.dt, c.st, c.coeff, strict=True)
+    h.set_opts(zero_guess=0)
+    warm = h.poisson_solve(dev(u1), phi, c.dt, c.st, c.coeff, strict=True)
+    assert warm.iterations < cold.iterations
+    assert relL2(phi.cpu().numpy(), phi_cold.cpu().numpy(), A.fluid_cells) <= PHI_TOL
+    h.close()
+
+
+def test_host_array_entry_matches_device(E, torch_cuda):
+    torch = torch_cuda
+    c, A, _ = get_case("towers")
+    h = handle_for(E, c)
+    u = np.ascontiguousarray(c.fields["ucont"].copy())
+    phi = np.zeros(c.shape)
+    p = np.zeros(c.shape)
+    ucat = np.zeros(c.shape + (3,))
+    h.project_host(u, phi, p, ucat, c.dt, c.st, c.coeff)
+    ud = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+    phid = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+    pd = torch.zeros_like(phid)
+    ucatd = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+    h.project(ud, phid, pd, ucatd, c.dt, c.st, c.coeff)
+    for a, b in ((u, ud), (phi, phid), (p, pd), (ucat, ucatd)):
+        np.testing.assert_array_equal(a, b.cpu().numpy())
+    h.close()
+
+
+def test_error_behaviour(E, torch_cuda):
+    torch = torch_cuda
+    with pytest.raises(E.VfsPPError) as ei:
+        E.Handle(E.make_grid(1, 8, 8))
+    assert ei.value.code == E.VFS_PP_EINVAL
+    c, A, _ = get_case("channel")
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed))
+    u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+    phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+    with pytest.raises(E.VfsPPError) as ei:
+        h.poisson_solve(u, phi, c.dt, c.st, c.coeff)
+    assert ei.value.code == E.VFS_PP_ESTATE
+    h.set_geometry(c.fields)
+    bad = u.clone()
+    bad[3, 4, 5, 0] = float("nan")
+    with pytest.raises(E.VfsPPError) as ei:
+        h.poisson_solve(bad, phi, c.dt, c.st, c.coeff)
+    assert ei.value.code == E.VFS_PP_ENONFINITE
+    with pytest.raises(ValueError):
+        h.poisson_solve(u[:-1].contiguous(), phi, c.dt, c.st, c.coeff)
+    # an iteration cap too small to converge is reported, not silent
+    h.set_opts(max_iters=1, rtol=1e-14)
+    with pytest.raises(E.VfsPPError) as ei:
+        h.poisson_solve(u.clone(), phi, c.dt, c.st, c.coeff, strict=True)
+    assert ei.value.code == E.VFS_PP_ENOCONV
+    st = h.poisson_solve(u.clone(), phi, c.dt, c.st, c.coeff)
+    assert st.converged == 0 and st.iterations == 1
+    h.close()
+
+
+# --------------------------------------------- the 192x96x128 terrain system
+
+@pytest.fixture(scope="module")
+def big():
+    c = cases.hill_case(192, 96, 128, h=0.3, s=0.5, sphere=(0.6, 0.75, 0.0, 0.15), seed=0)
+    return c, O.Assembled(c, drop=False)
+
+
+@pytest.fixture(scope="module")
+def big_runs(E, big, torch_cuda):
+    c, A = big
+    out = {}
+    for prec in (1, 0):
+        print(f"hill 192x96x128 + sphere, precision {prec}", flush=True)
+        user = E.UserCtx(c, E.default_opts(precision=prec, rtol=2e-11))
+        st = E.Flow_Solver_pressure_step(user, strict=True)
+        out[prec] = (st, user.Phi.cpu().numpy(), user.Ucont.cpu().numpy(), user.Ucat.cpu().numpy())
+        user.handle.close()
+    return out
+
+
+def test_big_divergence_free_both_forms(big, big_runs):
+    c, A = big
+    for prec in (1, 0):
+        st, phi, u, _ = big_runs[prec]
+        assert st.converged and st.max_div <= DIV_TOL
+        md = O.divergence(A.g, u, np.ascontiguousarray(c.fields["aj"]), A.nvert)
+        assert md <= DIV_TOL and abs(md - st.max_div) <= 1e-12 * md + 1e-20
+        # null space removed: fluid mean of phi is zero
+        lt = (A.nvert < 0.1)[1:-1, 1:-1, 1:-1]
+        assert abs(phi[1:-1, 1:-1, 1:-1][lt].mean()) <= 1e-12 * np.abs(phi).max()
+
+
+def test_big_mixed_equals_fp64(big, big_runs):
+    c, A = big
+    assert relL2(big_runs[1][1], big_runs[0][1], A.fluid_cells) <= PHI_TOL
+
+
+def test_big_against_cpu_multigrid_oracle(big, big_runs):
+    """The oracle's FP64 multigrid FGMRES on the CPU reaches the same phi."""
+    c, A = big
+    mg = O.CpuMG(A)
+    r = O.cpu_projection(c, A, mg, rtol=1e-12)
+    mg.close()
+    assert relL2(big_runs[0][1], r["phi"], A.fluid_cells) <= PHI_TOL
+    assert relL2(big_runs[1][1], r["phi"], A.fluid_cells) <= PHI_TOL
+
+
+def test_big_projection_consistent_with_oracle(big, big_runs):
+    c, A = big
+    st, phi, ug, ucg = big_runs[1]
+    u = np.ascontiguousarray(c.fields["ucont"].copy())
+    O.volume_flux(A.g, c.immersed, u, c.fields)
+    O.projection(A.g, c.fields, np.ascontiguousarray(phi), u, c.dt, c.st, c.coeff)
+    np.testing.assert_allclose(ug, u, rtol=0, atol=1e-13 * np.abs(u).max())
+    ucat = np.zeros_like(u)
+    O.contra2cart(A.g, c.bctype, c.fields, u, ucat)
+    np.testing.assert_allclose(ucg, ucat, rtol=0, atol=1e-13 * np.abs(ucat).max())

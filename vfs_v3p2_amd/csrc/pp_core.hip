@@ -301,6 +301,18 @@ __global__ __launch_bounds__(256) void k_scatter_update(Dmda m, Dims d, const do
   if (p && (b & CLS_LT)) p[q] += ph;  // UpdatePressure, poisson.c:2548
 }
 
+// Initial guess from the caller's Phi: the inverse of Convert_Phi2_Phi.  The
+// reference keeps its HYPRE solution vector between time steps and GMRES starts
+// from it (poisson_hypre.c:565-574, 722); that vector equals Phi on every cell
+// that carries an unknown.
+__global__ __launch_bounds__(256) void k_gather_phi(Dmda m, Dims d, const double* __restrict__ phi,
+                                                    const uint8_t* __restrict__ cls, double* __restrict__ x) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = d.idx(i, j, k);
+  x[c] = (cls[c] & CLS_INTGT) ? 0. : phi[m.idx(i + 1, j + 1, k + 1)];
+}
+
 // UpdatePressure: P += Phi on interior cells with nvert < 0.1 (poisson.c:2548)
 __global__ __launch_bounds__(256) void k_add_phi(Dmda m, Dims d, const uint8_t* __restrict__ cls,
                                                  const double* __restrict__ phi, double* __restrict__ p) {
@@ -771,6 +783,11 @@ void launch_sub_mean(hipStream_t s, Dims d, double* x, const uint8_t* cls, const
 
 void launch_scatter_update(hipStream_t s, Dmda m, Dims d, const double* x, const uint8_t* cls, double* phi, double* p) {
   k_scatter_update<<<cell_grid(d), kBlk, 0, s>>>(m, d, x, cls, phi, p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_gather_phi(hipStream_t s, Dmda m, Dims d, const double* phi, const uint8_t* cls, double* x) {
+  k_gather_phi<<<cell_grid(d), kBlk, 0, s>>>(m, d, phi, cls, x);
   HIPCHECK(hipGetLastError());
 }
 

@@ -845,7 +845,7 @@ static void poisson_solve(vfs_pp_handle* h, double* ucont, double* phi, double d
   if (!std::isfinite(bnorm)) throw StatusError(VFS_PP_ENONFINITE, "non-finite right-hand side");
   // initial guess
   if (h->opts.zero_guess) launch_fill(s, n, h->x, 0.);
-  else throw StatusError(VFS_PP_EINVAL, "non-zero initial guess is not supported yet");
+  else launch_gather_phi(s, h->m, h->d, phi, h->cls, h->x);
   double rel = 0.;
   int vc = 0;
   bool conv = false;

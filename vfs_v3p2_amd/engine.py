@@ -137,11 +137,26 @@ def _dptr(a: np.ndarray):
     return a, a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _tptr(t):
+def _tptr(t, shape=None, what="array"):
+    """Device pointer of a contiguous FP64 CUDA tensor, checked against `shape`."""
     if t is None:
-        return None
-    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+        raise ValueError(f"{what}: a device tensor is required")
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what}: expected a CUDA tensor, got {type(t).__name__} on {getattr(t, 'device', 'host')}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: tensor must be contiguous")
+    if str(t.dtype) != "torch.float64":
+        raise ValueError(f"{what}: expected float64, got {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
     return C.c_void_p(t.data_ptr())
+
+
+def dmda_shape(grid: Grid, vector=False):
+    """DMDA array shape (mz, my, mx[, 3]) of a grid: interior cells plus one ghost
+    plane on each side (init.c:156-157)."""
+    sh = (grid.nz + 2, grid.ny + 2, grid.nx + 2)
+    return sh + (3,) if vector else sh
 
 
 class Handle:
@@ -174,10 +189,19 @@ class Handle:
         args = []
         for k in ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj", "aj",
                   "csi", "eta", "zet", "nvert"):
+            want = dmda_shape(self.grid, vector=k not in ("iaj", "jaj", "kaj", "aj", "nvert"))
+            if tuple(np.shape(f[k])) != want:
+                raise ValueError(f"set_geometry: {k} has shape {np.shape(f[k])}, expected {want}")
             a, p = _dptr(f[k])
             keep.append(a)
             args.append(p)
         _check(lib().vfs_pp_set_geometry(self._h, *args), "vfs_pp_set_geometry")
+
+    def _s(self, t, what):
+        return _tptr(t, dmda_shape(self.grid), what)
+
+    def _v(self, t, what):
+        return _tptr(t, dmda_shape(self.grid, vector=True), what)
 
     def describe(self) -> str:
         buf = C.create_string_buffer(4096)
@@ -186,41 +210,46 @@ class Handle:
 
     def poisson_solve(self, ucont, phi, dt, st, coeff, strict=False):
         s = Stats()
-        rc = lib().vfs_pp_poisson_solve_device(self._h, _tptr(ucont), _tptr(phi), dt, st, coeff, C.byref(s))
+        rc = lib().vfs_pp_poisson_solve_device(self._h, self._v(ucont, "ucont"), self._s(phi, "phi"), dt, st, coeff,
+                                               C.byref(s))
         _check(rc, "poisson solve", allow=() if strict else (VFS_PP_ENOCONV,))
         return s
 
     def update_pressure(self, p, phi):
-        _check(lib().vfs_pp_update_pressure_device(self._h, _tptr(p), _tptr(phi)), "update pressure")
+        _check(lib().vfs_pp_update_pressure_device(self._h, self._s(p, "p"), self._s(phi, "phi")), "update pressure")
 
     def projection(self, ucont, phi, ucat, dt, st, coeff):
-        _check(lib().vfs_pp_projection_device(self._h, _tptr(ucont), _tptr(phi), _tptr(ucat), dt, st, coeff),
+        _check(lib().vfs_pp_projection_device(self._h, self._v(ucont, "ucont"), self._s(phi, "phi"), self._v(ucat, "ucat"),
+                                             dt, st, coeff),
                "projection")
 
     def divergence(self, ucont) -> float:
         out = C.c_double()
-        _check(lib().vfs_pp_divergence_device(self._h, _tptr(ucont), C.byref(out)), "divergence")
+        _check(lib().vfs_pp_divergence_device(self._h, self._v(ucont, "ucont"), C.byref(out)), "divergence")
         return out.value
 
     def project(self, ucont, phi, p, ucat, dt, st, coeff, strict=False):
         s = Stats()
-        rc = lib().vfs_pp_project_device(self._h, _tptr(ucont), _tptr(phi), _tptr(p), _tptr(ucat), dt, st, coeff,
-                                         C.byref(s))
+        rc = lib().vfs_pp_project_device(self._h, self._v(ucont, "ucont"), self._s(phi, "phi"), self._s(p, "p"),
+                                         self._v(ucat, "ucat"), dt, st, coeff, C.byref(s))
         _check(rc, "project", allow=() if strict else (VFS_PP_ENOCONV,))
         return s
 
     def project_host(self, ucont, phi, p, ucat, dt, st, coeff):
         """Host-array variant (PCIe copies in and out); arrays are updated in place."""
         s = Stats()
+        for nm, a, vec in (("ucont", ucont, True), ("phi", phi, False), ("p", p, False), ("ucat", ucat, True)):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+                raise ValueError(f"{nm}: expected a C-contiguous float64 numpy array")
+            if a.shape != dmda_shape(self.grid, vec):
+                raise ValueError(f"{nm}: expected shape {dmda_shape(self.grid, vec)}, got {a.shape}")
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in (ucont, phi, p, ucat)]
-        for a in (ucont, phi, p, ucat):
-            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
         rc = lib().vfs_pp_project(self._h, *ptrs, dt, st, coeff, C.byref(s))
         _check(rc, "project (host arrays)", allow=(VFS_PP_ENOCONV,))
         return s
 
     def apply(self, phi, y, drop_small=False):
-        _check(lib().vfs_pp_apply_device(self._h, _tptr(phi), _tptr(y), int(drop_small)), "apply")
+        _check(lib().vfs_pp_apply_device(self._h, self._s(phi, "phi"), self._s(y, "y"), int(drop_small)), "apply")
 
     def export_stencil(self):
         M = (self.grid.nz + 2) * (self.grid.ny + 2) * (self.grid.nx + 2)
@@ -293,17 +322,32 @@ def Flow_Solver_pressure_step(user: UserCtx, strict=False):
 
 # ----------------------------------------------------------- reference primitives
 
+def coarse_dmda_shape(grid: Grid):
+    """DMDA shape of the reference's next coarser level, IM_c = (IM_f + 1) / 2 with
+    IM = m - 1 (init.c:95-127); the fine interior extents must be even."""
+    if grid.nx % 2 or grid.ny % 2 or grid.nz % 2:
+        raise ValueError("the reference multigrid primitives need even interior extents")
+    return tuple((m - 1 + 1) // 2 + 1 for m in dmda_shape(grid))
+
+
 def classify_nvert_device(grid: Grid, nvert_dev):
-    _check(lib().vfs_pp_classify_nvert_device(C.byref(grid), _tptr(nvert_dev)), "classify nvert")
+    _check(lib().vfs_pp_classify_nvert_device(C.byref(grid), _tptr(nvert_dev, dmda_shape(grid), "nvert")),
+           "classify nvert")
 
 
 def mg_restrict_device(grid: Grid, nvf, nvc, xf, fc):
-    _check(lib().vfs_pp_mg_restrict_device(C.byref(grid), _tptr(nvf), _tptr(nvc), _tptr(xf), _tptr(fc)), "restrict")
+    F, Cs = dmda_shape(grid), coarse_dmda_shape(grid)
+    _check(lib().vfs_pp_mg_restrict_device(C.byref(grid), _tptr(nvf, F, "nvert_f"), _tptr(nvc, Cs, "nvert_c"),
+                                           _tptr(xf, F, "xf"), _tptr(fc, Cs, "fc")), "restrict")
 
 
 def mg_prolong_device(grid: Grid, nvf, nvc, xc, ff):
-    _check(lib().vfs_pp_mg_prolong_device(C.byref(grid), _tptr(nvf), _tptr(nvc), _tptr(xc), _tptr(ff)), "prolong")
+    F, Cs = dmda_shape(grid), coarse_dmda_shape(grid)
+    _check(lib().vfs_pp_mg_prolong_device(C.byref(grid), _tptr(nvf, F, "nvert_f"), _tptr(nvc, Cs, "nvert_c"),
+                                          _tptr(xc, Cs, "xc"), _tptr(ff, F, "ff")), "prolong")
 
 
 def mg_nvert_restrict_device(grid: Grid, nvf, nvc):
-    _check(lib().vfs_pp_mg_nvert_restrict_device(C.byref(grid), _tptr(nvf), _tptr(nvc)), "nvert restrict")
+    F, Cs = dmda_shape(grid), coarse_dmda_shape(grid)
+    _check(lib().vfs_pp_mg_nvert_restrict_device(C.byref(grid), _tptr(nvf, F, "nvert_f"), _tptr(nvc, Cs, "nvert_c")),
+           "nvert restrict")
