@@ -10,7 +10,9 @@ before timing starts; every step projects its own copy of the field.
 Forms: mixed (FP32 multigrid preconditioner inside FP64 FGMRES; the headline
 `value`) and plain FP64 (FP64 multigrid), both to max|aj div u| <= 1e-8.
 
-Prints one JSON line (see README of the benchmark contract in DESIGN.md).
+Prints one JSON line: solves/s of the mixed form as `value`, p50/p99 latency,
+the FP64 form beside it, the per-kernel HBM figures (`roofline`) and a CPU
+baseline timed on this host's cores (`cpu_baseline`).
 """
 from __future__ import annotations
 
@@ -29,6 +31,23 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
 DIV_TOL = 1e-8
 RTOL = 2e-11  # flux-form relative residual; leaves margin under DIV_TOL
+
+
+# vfs_pp_time_kernel ids
+KERNELS = {0: "apply_fp64", 1: "smoother_fp32", 2: "projection", 3: "contra2cart", 4: "rhs", 5: "copy_1GiB"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "smoother_fp32_hill192x96x128_hbm_traffic.txt")
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the finest FP32 smoother sweep, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); None if absent."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -91,14 +110,19 @@ def main():
     case = cases.hill_case(192, 96, 128, h=0.3, s=0.5, sphere=(0.6, 0.75, 0.0, 0.15), seed=0)
 
     h_mx, mx = run_form(E, torch, case, 1, args.steps, args.warmup)
-    # dominant kernel of the mixed solve: the finest-level FP32 smoother sweep
-    ms_sm, by_sm = h_mx.time_kernel(1, 100)
-    ms_ap, by_ap = h_mx.time_kernel(0, 100)
+    # every engine kernel timed in isolation with HIP events on the handle's stream
+    kern = {}
+    for kid, name in KERNELS.items():
+        ms, by = h_mx.time_kernel(kid, 100 if kid != 5 else 20)
+        kern[name] = dict(ms_per_launch=round(ms, 5), algorithmic_bytes=by,
+                          gbs=round(by / (ms * 1e-3) / 1e9, 1), frac=round(by / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
     h_mx.close()
     h64, f64 = run_form(E, torch, case, 0, args.steps, args.warmup)
-    ms_ap64, by_ap64 = h64.time_kernel(0, 100)
     ms_sm64, by_sm64 = h64.time_kernel(1, 100)
     h64.close()
+    kern["smoother_fp64"] = dict(ms_per_launch=round(ms_sm64, 5), algorithmic_bytes=by_sm64,
+                                 gbs=round(by_sm64 / (ms_sm64 * 1e-3) / 1e9, 1),
+                                 frac=round(by_sm64 / (ms_sm64 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
 
     lat = np.array(mx["lat"]) * 1e3
     value = args.steps / mx["total"]
@@ -106,7 +130,7 @@ def main():
     its = [s[0] for s in mx["stats"]]
     divs = [s[2] for s in mx["stats"]]
     divs64 = [s[2] for s in f64["stats"]]
-    sm_gbs = by_sm / (ms_sm * 1e-3) / 1e9
+    sm = kern["smoother_fp32"]
     out = {
         "metric": "Poisson solves/s & p99 latency at div<=1e-8, mixed vs FP64; stencil HBM GB/s",
         "value": round(value, 3),
@@ -139,21 +163,20 @@ def main():
             "p99_ms": round(float(np.percentile(np.array(f64["lat"]) * 1e3, 99)), 4),
             "iterations": int(np.median([s[0] for s in f64["stats"]])),
             "max_div": float(max(divs64)),
-            "apply_gbs": round(by_ap64 / (ms_ap64 * 1e-3) / 1e9, 1),
-            "smoother_gbs": round(by_sm64 / (ms_sm64 * 1e-3) / 1e9, 1),
         },
         "mixed_over_fp64": round(value / fp64_value, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_smooth_point<float> (finest-level Chebyshev-Jacobi sweep)",
-            "achieved": round(sm_gbs, 1),
+            "kernel": "k_smooth_point<float> (finest-level Chebyshev-Jacobi sweep, 49 algorithmic B/cell)",
+            "achieved": sm["gbs"],
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
-            "frac": round(sm_gbs / PEAK_HBM_GBS, 4),
-            "traffic": None,
-            "ms_per_launch": round(ms_sm, 5),
-            "bytes_per_launch": by_sm,
-            "fp64_apply_gbs": round(by_ap / (ms_ap * 1e-3) / 1e9, 1),
+            "frac": sm["frac"],
+            "traffic": traffic_from_profile(),
+            "ms_per_launch": sm["ms_per_launch"],
+            "algorithmic_bytes_per_launch": sm["algorithmic_bytes"],
+            "copy_gbs": kern["copy_1GiB"]["gbs"],
+            "kernels": kern,
         },
         "setup_s": round(mx["setup_s"], 3),
         "cpu_baseline": None,

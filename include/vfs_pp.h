@@ -74,8 +74,10 @@ typedef struct {
                         calls, poisson_hypre.c:565-574, 722) */
   double cheb_lo;    /* Chebyshev smoothing interval [cheb_lo, cheb_hi] x lambda_max(M^-1 A) */
   double cheb_hi;
-  int orthog;        /* 0: one-pass classical Gram-Schmidt fused into the operator pass,
-                        with a second pass on cancellation; 1: two-pass CGS2 */
+  int orthog;        /* Gram-Schmidt of the Krylov basis, both classical with
+                        re-orthogonalisation (CGS2): 0 fused into three passes over
+                        the basis (the first with the operator apply); 1 plain,
+                        one pass per step */
   int verbose;       /* 1: print the residual history of each solve to stderr */
 } vfs_pp_opts;
 
@@ -172,9 +174,12 @@ int vfs_pp_mg_prolong_device(const vfs_grid *fine, const double *nvert_f, const 
 int vfs_pp_mg_nvert_restrict_device(const vfs_grid *fine, const double *nvert_f, double *nvert_c);
 
 /* Time `reps` launches of one engine kernel on the handle's stream with HIP
- * events; kernel ids: 0 FP64 operator apply, 1 FP32 smoother sweep (finest
- * level), 2 projection, 3 Contra2Cart, 4 RHS.  Writes the mean duration (ms)
- * and the algorithmic bytes of one launch. */
+ * events; kernel ids: 0 FP64 operator apply, 1 finest-level smoother sweep (in
+ * the multigrid precision), 2 projection, 3 Contra2Cart, 4 RHS, 5 a plain
+ * 1 GiB device copy (the bandwidth reference).  Writes the mean duration (ms)
+ * and the algorithmic bytes of one launch: every distinct array once per cell,
+ * with a 1-byte mask (apply 89, smoother 49 FP32 / 97 FP64, projection 129,
+ * Contra2Cart 121, RHS 33 bytes per cell; copy 16 bytes per element). */
 int vfs_pp_time_kernel(vfs_pp_handle *h, int kernel, int reps, double *ms_per_launch,
                        double *bytes_per_launch);
 

@@ -50,6 +50,26 @@ HD inline int wrap_p(int i, int n, int per) { return i < n - 1 ? i + 1 : (per ? 
 
 constexpr int BLOCK = 256;
 
+// Workgroups go to the 8 XCDs round-robin by linear id (workgroup w runs on XCD
+// w % 8), and each XCD has its own L2.  xcd_block() renumbers the grid so that
+// XCD x runs one contiguous range of tiles in (i, j, k) order: tiles adjacent in
+// j and k, which re-read each other's halo rows and lower-face coefficients, then
+// run close together in time on one L2.  Any renumbering is a bijection of the
+// grid, so results do not depend on it.
+__device__ inline uint3 xcd_block() {
+  const uint32_t nbx = gridDim.x, nby = gridDim.y, pl = nbx * nby;
+  const uint32_t nb = pl * gridDim.z;
+  const uint32_t w = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const uint32_t q = nb >> 3, r = nb & 7u, x = w & 7u, l = w >> 3;
+  const uint32_t t = x * q + (x < r ? x : r) + l;
+  return make_uint3(t % nbx, (t % pl) / nbx, t / pl);
+}
+// (i, j, k) of this thread for the 64 x 4 x 1 cell tiles of a cell_grid() launch.
+#define CELL_IJK(i, j, k)                                                                          \
+  const uint3 xb_ = xcd_block();                                                                   \
+  const int i = (int)xb_.x * 64 + (int)threadIdx.x, j = (int)xb_.y * 4 + (int)threadIdx.y,        \
+            k = (int)xb_.z
+
 // wave64 reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll

@@ -129,7 +129,7 @@ template <typename TX, bool DROP>
 __global__ __launch_bounds__(256) void k_apply64(Dims d, const double* __restrict__ g,
                                                  const uint32_t* __restrict__ code, const TX* __restrict__ x,
                                                  double* __restrict__ y) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
   const size_t c = d.idx(i, j, k);
   const uint32_t cd = code[c];
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_residual64(Dims d, const double* __rest
                                                     const TX* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ partial) {
   __shared__ double lds[4];
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   double rr = 0.;
   if (i < d.nx && j < d.ny) {
     const size_t c = d.idx(i, j, k);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_residual64(Dims d, const double* __rest
 __global__ __launch_bounds__(256) void k_apply_dmda(Dmda m, Dims d, const double* __restrict__ g,
                                                     const uint32_t* __restrict__ code, const double* __restrict__ phi,
                                                     double* __restrict__ y, int drop) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
   const size_t c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
   const uint32_t cd = code[c];
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_rhs(Dmda m, Dims d, const double* __res
                                              const uint8_t* __restrict__ cls, double fac, double* __restrict__ b,
                                              double* __restrict__ partial) {
   __shared__ double lds[4];
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   double bb = 0.;
   if (i < d.nx && j < d.ny) {
     const size_t c = d.idx(i, j, k);
@@ -357,34 +357,40 @@ __global__ __launch_bounds__(256) void k_projection(Dmda m, Dims d, const double
                                                     const uint32_t* __restrict__ code,
                                                     const uint8_t* __restrict__ cls, const double* __restrict__ phi,
                                                     double* __restrict__ u, double fac) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
   const size_t n = d.n(), c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
   const uint8_t b = cls[c];
   if (!(b & (CLS_PROJ_I | CLS_PROJ_J | CLS_PROJ_K))) return;
   const uint32_t cd = code[c];
   const size_t sx = 1, sy = m.mx, sz = (size_t)m.mx * m.my;
+  // the cell's three face fluxes are read, corrected and stored together, so each
+  // cache line of the AoS array is written once
+  double u0 = u[3 * p + 0], u1 = u[3 * p + 1], u2 = u[3 * p + 2];
   if (b & CLS_PROJ_I) {  // poisson.c:2221-2265; face i+ is face 0, cross j then k
     const size_t q = p + sx;
     const double dc = phi[q] - phi[p];
     const double de = proj_cross((cd >> 6) & 3u, phi, p, q, sy);
     const double dz = proj_cross((cd >> 8) & 3u, phi, p, q, sz);
-    u[3 * p + 0] -= (dc * g[0 * n + c] + de * g[1 * n + c] + dz * g[2 * n + c]) * fac;
+    u0 -= (dc * g[0 * n + c] + de * g[1 * n + c] + dz * g[2 * n + c]) * fac;
   }
   if (b & CLS_PROJ_J) {  // poisson.c:2267-2314; face j+ is face 2, cross i then k
     const size_t q = p + sy;
     const double dc = proj_cross((cd >> 14) & 3u, phi, p, q, sx);
     const double de = phi[q] - phi[p];
     const double dz = proj_cross((cd >> 16) & 3u, phi, p, q, sz);
-    u[3 * p + 1] -= (dc * g[3 * n + c] + de * g[4 * n + c] + dz * g[5 * n + c]) * fac;
+    u1 -= (dc * g[3 * n + c] + de * g[4 * n + c] + dz * g[5 * n + c]) * fac;
   }
   if (b & CLS_PROJ_K) {  // poisson.c:2316-2363; face k+ is face 4, cross i then j
     const size_t q = p + sz;
     const double dc = proj_cross((cd >> 22) & 3u, phi, p, q, sx);
     const double de = proj_cross((cd >> 24) & 3u, phi, p, q, sy);
     const double dz = phi[q] - phi[p];
-    u[3 * p + 2] -= (dc * g[6 * n + c] + de * g[7 * n + c] + dz * g[8 * n + c]) * fac;
+    u2 -= (dc * g[6 * n + c] + de * g[7 * n + c] + dz * g[8 * n + c]) * fac;
   }
+  u[3 * p + 0] = u0;
+  u[3 * p + 1] = u1;
+  u[3 * p + 2] = u2;
 }
 
 // per-component periodic ghost faces (poisson.c:2401-2441)
@@ -404,7 +410,7 @@ __global__ __launch_bounds__(256) void k_c2c_core(Dmda m, Dims d, const double* 
                                                   const uint8_t* __restrict__ cls, const double* __restrict__ u,
                                                   double* __restrict__ ucat) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
   const size_t n = d.n(), c = d.idx(i, j, k);
   if (!(cls[c] & CLS_LT)) return;
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(256) void k_divergence(Dmda m, Dims d, const uint32
                                                     double* __restrict__ partial) {
 #pragma clang fp contract(off)
   __shared__ double lds[4];
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   double mv = 0.;
   if (i < d.nx && j < d.ny) {
     const size_t c = d.idx(i, j, k);
@@ -505,14 +511,34 @@ __global__ __launch_bounds__(256) void k_divergence(Dmda m, Dims d, const uint32
 }
 
 // ---------------------------------------------------------------- reductions
-__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partial, int nblk,
-                                                  double* __restrict__ out, int op, double* __restrict__ host) {
-  __shared__ double lds[4];
+// One 1024-thread block per vector sums (op 0) or maxes (op 1) its nblk block
+// partials in a fixed order: four independent accumulators per thread keep the
+// loads in flight, then the 16 wave results meet in LDS.
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partial, int nblk,
+                                                   double* __restrict__ out, int op, double* __restrict__ host) {
+  __shared__ double lds[16];
   const double* pv = partial + (size_t)blockIdx.x * nblk;
-  double s = 0.;
-  for (int b = threadIdx.x; b < nblk; b += 256) s = op ? fmax(s, pv[b]) : s + pv[b];
-  s = op ? block_max(s, lds) : block_sum(s, lds);
-  if (threadIdx.x == 0) {
+  const int t = threadIdx.x;
+  double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+  int b = t;
+  if (op) {
+    for (; b + 3072 < nblk; b += 4096) {
+      a0 = fmax(a0, pv[b]); a1 = fmax(a1, pv[b + 1024]); a2 = fmax(a2, pv[b + 2048]); a3 = fmax(a3, pv[b + 3072]);
+    }
+    for (; b < nblk; b += 1024) a0 = fmax(a0, pv[b]);
+    a0 = wave_max(fmax(fmax(a0, a1), fmax(a2, a3)));
+  } else {
+    for (; b + 3072 < nblk; b += 4096) {
+      a0 += pv[b]; a1 += pv[b + 1024]; a2 += pv[b + 2048]; a3 += pv[b + 3072];
+    }
+    for (; b < nblk; b += 1024) a0 += pv[b];
+    a0 = wave_sum((a0 + a1) + (a2 + a3));
+  }
+  if ((t & 63) == 0) lds[t >> 6] = a0;
+  __syncthreads();
+  if (t == 0) {
+    double s = lds[0];
+    for (int w = 1; w < 16; w++) s = op ? fmax(s, lds[w]) : s + lds[w];
     out[blockIdx.x] = s;
     if (host) host[blockIdx.x] = s;  // pinned host mirror, read after the stream syncs
   }
@@ -585,6 +611,42 @@ __global__ __launch_bounds__(256) void k_fill(size_t n, T* a, T v) {
   if (t < n) a[t] = v;
 }
 
+// Plain device copy, 16 bytes per work-item: the bandwidth reference that kernel
+// timings and memory-counter readings are calibrated against.
+__global__ __launch_bounds__(256) void k_copy16(size_t n2, const double2* __restrict__ a, double2* __restrict__ b) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n2) b[t] = a[t];
+}
+
+// Per-block sums of a * V_l[c] for l < nv and of a * a (slot nv), written to
+// partial[l * nblk + b].  Eight vectors at a time are loaded together, reduced
+// inside each wave by shuffles and parked in LDS; one barrier, then thread l
+// adds the four wave sums of slot l.  256-thread blocks, nv < 64.
+__device__ inline void block_dots(bool in, size_t c, double a, const double* __restrict__ V, size_t ldv, int nv,
+                                  double* __restrict__ partial, int nblk, int b, double (*lds)[64]) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int lane = t & 63, w = t >> 6;
+  for (int l0 = 0; l0 <= nv; l0 += 8) {
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int l = l0 + q;
+      double v = 0.;
+      if (in && l < nv) v = a * V[(size_t)l * ldv + c];
+      else if (in && l == nv) v = a * a;
+      p[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) p[q] = wave_sum(p[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (l0 + q <= nv) lds[w][l0 + q] = p[q];
+    }
+  }
+  __syncthreads();
+  if (t <= nv) partial[(size_t)t * nblk + b] = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
+}
 
 // w = A z (flux form) and per-block partial sums of w.V_l (l < nv) and w.w; the
 // first Krylov reduction of classical Gram-Schmidt fused into the operator pass.
@@ -593,8 +655,8 @@ __global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __rest
                                                     const uint32_t* __restrict__ code, const TZ* __restrict__ z,
                                                     double* __restrict__ w, const double* __restrict__ V, size_t ldv,
                                                     int nv, double* __restrict__ partial) {
-  __shared__ double lds[4];
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  __shared__ double lds[4][64];
+  CELL_IJK(i, j, k);
   const int nblk = gridDim.x * gridDim.y * gridDim.z, b = blin();
   const bool in = i < d.nx && j < d.ny;
   size_t c = 0;
@@ -610,12 +672,7 @@ __global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __rest
     wv = (cd & ROW_BIT) ? ax : 0.;
     w[c] = wv;
   }
-  for (int l = 0; l <= nv; l++) {
-    double p = 0.;
-    if (in) p = (l < nv) ? wv * V[(size_t)l * ldv + c] : wv * wv;
-    p = block_sum(p, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) partial[(size_t)l * nblk + b] = p;
-  }
+  block_dots(in, c, wv, V, ldv, nv, partial, nblk, b, lds);
 }
 
 
@@ -625,7 +682,7 @@ __global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __rest
 __global__ __launch_bounds__(256) void k_axpy_dots(size_t n, double* __restrict__ w, const double* __restrict__ V,
                                                    size_t ldv, int nv, const double* __restrict__ h,
                                                    double* __restrict__ partial) {
-  __shared__ double lds[4];
+  __shared__ double lds[4][64];
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   const int nblk = gridDim.x;
   double w1 = 0.;
@@ -634,12 +691,7 @@ __global__ __launch_bounds__(256) void k_axpy_dots(size_t n, double* __restrict_
     for (int l = 0; l < nv; l++) w1 -= h[l] * V[(size_t)l * ldv + t];
     w[t] = w1;
   }
-  for (int l = 0; l <= nv; l++) {
-    double p = 0.;
-    if (t < n) p = (l < nv) ? w1 * V[(size_t)l * ldv + t] : w1 * w1;
-    p = block_sum(p, lds);
-    if (threadIdx.x == 0) partial[(size_t)l * nblk + blockIdx.x] = p;
-  }
+  block_dots(t < n, t, w1, V, ldv, nv, partial, nblk, blockIdx.x, lds);
 }
 
 // red[1] = ||w - V h|| = sqrt(w.w - h.h), red[2] = 1/red[1] with h = red[off..off+nv)
@@ -839,7 +891,7 @@ void launch_divergence(hipStream_t s, Dmda m, Dims d, const uint32_t* code, cons
 
 void launch_finalize(hipStream_t s, const double* partial, int nvec, int nblk, double* out, int op,
                      double* host_mirror) {
-  k_finalize<<<nvec, 256, 0, s>>>(partial, nblk, out, op, host_mirror);
+  k_finalize<<<nvec, 1024, 0, s>>>(partial, nblk, out, op, host_mirror);
   HIPCHECK(hipGetLastError());
 }
 
@@ -897,6 +949,11 @@ template void launch_convert<double, double>(hipStream_t, size_t, const double*,
 
 void launch_fill(hipStream_t s, size_t n, double* a, double v) {
   k_fill<double><<<nb1(n), 256, 0, s>>>(n, a, v);
+  HIPCHECK(hipGetLastError());
+}
+void launch_copy(hipStream_t s, size_t n, const double* a, double* b) {
+  if (n % 2) throw StatusError(1, "copy length must be even");
+  k_copy16<<<nb1(n / 2), 256, 0, s>>>(n / 2, reinterpret_cast<const double2*>(a), reinterpret_cast<double2*>(b));
   HIPCHECK(hipGetLastError());
 }
 void launch_fillf(hipStream_t s, size_t n, float* a, float v) {

@@ -296,7 +296,6 @@ struct MGBase {
   virtual void* f0() = 0;
   virtual void set_out(void* out) = 0;
   virtual std::string describe() const = 0;
-  virtual size_t fine_bytes_per_sweep() const = 0;
   virtual void time_sweep(hipStream_t s) = 0;
 };
 
@@ -307,7 +306,7 @@ struct MG : MGBase {
     T* g = nullptr;
     const uint32_t* code = nullptr;
     const uint8_t* cls = nullptr;
-    T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr;
+    T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr, *dinv = nullptr;
     int line = 0;
     std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
     double lam = 0.;
@@ -339,14 +338,24 @@ struct MG : MGBase {
     }
   }
 
+  // `degree` Chebyshev steps; from a zero start the point smoother fuses the
+  // first two steps into one pass
   void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out) {
-    for (int k = 0; k < degree; k++) {
-      const bool zg = zero_start && k == 0;
+    int k = 0;
+    while (k < degree) {
+      int mode = SM_NEXT, adv = 1;
+      if (zero_start && k == 0) {
+        if (!L.line && degree >= 2) { mode = SM_FIRST2; adv = 2; }
+        else mode = SM_ZERO;
+      }
+      const int last = k + adv - 1;  // the last step this launch performs
       T* xin = x;
       T* xout = (x == L.xa) ? L.xb : L.xa;
-      if (k == degree - 1 && final_out) xout = final_out;
-      launch_smooth<T>(s, L.view(), L.line, zg, L.f, xin, xout, L.dv, scr, k == 0 ? T(0) : L.c1[k], L.c2[k]);
+      if (last == degree - 1 && final_out) xout = final_out;
+      const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
+      launch_smooth<T>(s, L.view(), L.line, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
       x = xout;
+      k += adv;
     }
   }
 
@@ -381,18 +390,10 @@ struct MG : MGBase {
     return o.str();
   }
 
-  size_t fine_bytes_per_sweep() const override {
-    // g 9 + f + x_old + x_new + d(read, write) in T, code 4
-    const size_t n = lev[0].d.n();
-    size_t b = n * (9 * sizeof(T) + 4 + 5 * sizeof(T));
-    if (lev[0].line) b += n * 4 * sizeof(T);  // Thomas factors written and read back
-    return b;
-  }
-
   void time_sweep(hipStream_t s) override {
     Lev& L = lev[0];
-    launch_smooth<T>(s, L.view(), L.line, false, L.f, L.xa, L.xb, L.dv, scr, L.c1.size() > 1 ? L.c1[1] : T(0),
-                     L.c2[0]);
+    launch_smooth<T>(s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
+                     L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
   }
 };
 
@@ -460,7 +461,7 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
   launch_scale_by<T>(h->s, n, L.xa, h->red);
   double lam = 0., best = 0.;
   for (int it = 0; it < 12; it++) {
-    launch_smooth<T>(h->s, L.view(), L.line, false, L.f, L.xa, L.xb, L.dv, mg.scr, T(0), T(1));
+    launch_smooth<T>(h->s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, mg.scr, T(0), T(1), T(0));
     launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
     launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
     HIPCHECK(hipStreamSynchronize(h->s));
@@ -513,6 +514,8 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     L.xa = h->ar.alloc<T>(n);
     L.xb = h->ar.alloc<T>(n);
     L.dv = h->ar.alloc<T>(n);
+    L.dinv = h->ar.alloc<T>(n);
+    launch_dinv<T>(h->s, L.view(), L.dinv);
     // smoother: y-line when the wall-normal coupling dominates (and j is not periodic)
     int line = 0;
     if (h->opts.smoother == 2) line = 1;
@@ -1194,11 +1197,14 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
-  double* y = tmp.alloc<double>(n);
+  // kernel 5 copies 2^27 doubles (1 GiB), far beyond the 256 MB last-level cache
+  const size_t ncopy = (size_t)1 << 27;
+  double* y = tmp.alloc<double>(kernel == 5 ? ncopy : n);
+  double* src = kernel == 5 ? tmp.alloc<double>(ncopy) : nullptr;
   double* u = nullptr;
   double* phi = nullptr;
   double* uc = nullptr;
-  if (kernel >= 2) {
+  if (kernel >= 2 && kernel <= 4) {
     u = tmp.alloc<double>(3 * M);
     phi = tmp.alloc<double>(M);
     uc = tmp.alloc<double>(3 * M);
@@ -1214,6 +1220,7 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
       case 2: launch_projection(h->s, h->m, h->d, h->g64, h->code, h->cls, phi, u, 1.0); break;
       case 3: launch_c2c(h->s, h->m, h->d, h->met9, h->cls, h->nvert, u, uc, h->snap, h->grid.bctype); break;
       case 4: launch_rhs(h->s, h->m, h->d, u, h->cls, 1.0, y, h->partial, &nb); break;
+      case 5: launch_copy(h->s, ncopy, src, y); break;
       default: throw StatusError(VFS_PP_EINVAL, "unknown kernel id");
     }
   };
@@ -1227,13 +1234,19 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms = (double)tms / reps;
-  // algorithmic bytes per launch
+  // Algorithmic bytes per launch: each distinct array once per cell, halos from
+  // cache, a 1-byte mask per cell.  (The engine's kernels move somewhat more:
+  // a 4-byte branch-code word instead of the mask, and the Chebyshev direction.)
+  const double N = (double)n;
   switch (kernel) {
-    case 0: *bytes = (double)n * (9 * 8 + 4 + 8 + 8); break;          // g, code, x, y
-    case 1: *bytes = (double)h->mg->fine_bytes_per_sweep(); break;
-    case 2: *bytes = (double)n * (9 * 8 + 4 + 1 + 8 + 24 + 24); break;  // g, code, cls, phi, Ucont r/w
-    case 3: *bytes = (double)n * (24 + 72 + 1 + 24); break;            // Ucont, csi/eta/zet, cls, Ucat
-    case 4: *bytes = (double)n * (24 + 1 + 8); break;                  // Ucont, cls, b
+    case 0: *bytes = N * (8 + 72 + 1 + 8); break;  // phi, g11..g33, mask -> y           = 89
+    case 1:                                          // x, b, g, mask -> x   FP32 49 / FP64 97
+      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 36 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+      break;
+    case 2: *bytes = N * (8 + 72 + 1 + 24 + 24); break;  // phi, g, mask, Ucont r/w   = 129
+    case 3: *bytes = N * (24 + 72 + 1 + 24); break;      // Ucont, csi/eta/zet, mask -> Ucat = 121
+    case 4: *bytes = N * (24 + 1 + 8); break;            // Ucont, mask -> b        = 33
+    case 5: *bytes = (double)ncopy * 16.; break;         // read + write
   }
   return VFS_PP_OK;
   ABI_END

@@ -93,6 +93,7 @@ void launch_update_x(hipStream_t s, size_t n, double* x, const TZ* Z, size_t ldz
 template <typename TA, typename TB>
 void launch_convert(hipStream_t s, size_t n, const TA* a, TB* b);
 void launch_fill(hipStream_t s, size_t n, double* a, double v);
+void launch_copy(hipStream_t s, size_t n, const double* a, double* b);
 void launch_fillf(hipStream_t s, size_t n, float* a, float v);
 
 int partial_blocks(size_t n);
@@ -121,9 +122,14 @@ struct LevelView {
   const uint8_t* cls;  // CLS_NONFLUID used by the transfers
 };
 
+// Chebyshev smoothing launch modes (see k_smooth_point)
+enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2 };
+
 template <typename T>
-void launch_smooth(hipStream_t s, LevelView<T> L, int line, bool zero_guess, const T* f, const T* x_old, T* x_new,
-                   T* dvec, T* scratch, T c1, T c2);
+void launch_smooth(hipStream_t s, LevelView<T> L, int line, int mode, const T* f, const T* dinv, const T* x_old,
+                   T* x_new, T* dv, T* scr, T c1, T c2, T c0);
+template <typename T>
+void launch_dinv(hipStream_t s, LevelView<T> L, T* dinv);
 template <typename T>
 void launch_resid_restrict(hipStream_t s, LevelView<T> F, const T* f, const T* x, Dims dc, const uint8_t* cls_c,
                            T* fc);

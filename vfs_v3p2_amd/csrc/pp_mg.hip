@@ -29,33 +29,61 @@ __device__ __forceinline__ T row_times(const T (&vol)[19], const Nbr& nb, const 
   return acc;
 }
 
-// One Chebyshev step preconditioned by the diagonal:
-//   z = D^-1 (f - A x_old);  d = c1 d + c2 z;  x_new = x_old + d
-template <typename T, bool ZG>
-__global__ __launch_bounds__(256) void k_smooth_point(LevelView<T> L, const T* __restrict__ f,
-                                                      const T* __restrict__ x_old, T* __restrict__ x_new,
-                                                      T* __restrict__ dv, T c1, T c2) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+// D^-1 of the level operator, 0 on cells that carry no unknown.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dinv(LevelView<T> L, T* __restrict__ dinv) {
+  CELL_IJK(i, j, k);
   if (i >= L.d.nx || j >= L.d.ny) return;
   const size_t c = L.d.idx(i, j, k);
   const uint32_t cd = L.code[c];
   Nbr nb;
   neighbours(L.d, i, j, k, nb);
   T xv[19];
-  if constexpr (ZG) {
 #pragma unroll
-    for (int s = 0; s < 19; s++) xv[s] = T(0);
+  for (int s = 0; s < 19; s++) xv[s] = T(0);
+  T ax, diag;
+  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+  dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
+}
+
+// Chebyshev smoothing preconditioned by the diagonal.  One step is
+//   z = D^-1 (f - A x_old);  d = c1 d + c2 z;  x_new = x_old + d
+// (c1 == 0 starts a new sequence and does not read d).  Modes:
+//   SM_NEXT    one step from x_old;
+//   SM_ZERO    the first step from x = 0:  x_new = d = c2 D^-1 f;
+//   SM_FIRST2  the first two steps from x = 0 in one pass: the first step's
+//              x1 = d1 = c0 D^-1 f is formed on the fly at the 19 neighbours from
+//              f and D^-1, then one step from x1 with (c1, c2).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_smooth_point(LevelView<T> L, const T* __restrict__ f,
+                                                      const T* __restrict__ dinv, const T* __restrict__ x_old,
+                                                      T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2, T c0) {
+  CELL_IJK(i, j, k);
+  if (i >= L.d.nx || j >= L.d.ny) return;
+  const size_t c = L.d.idx(i, j, k);
+  if constexpr (MODE == SM_ZERO) {
+    const T v = c2 * (f[c] * dinv[c]);
+    x_new[c] = v;
+    dv[c] = v;
+    return;
+  }
+  const uint32_t cd = L.code[c];
+  Nbr nb;
+  neighbours(L.d, i, j, k, nb);
+  T xv[19];
+  if constexpr (MODE == SM_FIRST2) {
+#pragma unroll
+    for (int s = 0; s < 19; s++) xv[s] = c0 * (f[nb.s[s]] * dinv[nb.s[s]]);
   } else {
     load_x(nb, x_old, xv);
   }
   T ax, diag;
   flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
   const T fc = f[c];
-  // c1 == 0 starts a new Chebyshev sequence: the previous direction is not read
-  const T dold = (ZG || c1 == T(0)) ? T(0) : dv[c];
+  const T dold = (MODE == SM_FIRST2) ? xv[CP] : (c1 == T(0) ? T(0) : dv[c]);
   if (!(cd & ROW_BIT)) { x_new[c] = T(0); dv[c] = T(0); return; }
   const T z = diag != T(0) ? (fc - ax) / diag : T(0);
-  const T dn = ZG ? c2 * z : c1 * dold + c2 * z;
+  const T dn = c1 * dold + c2 * z;
   dv[c] = dn;
   x_new[c] = xv[CP] + dn;
 }
@@ -146,7 +174,8 @@ __global__ __launch_bounds__(256) void k_resid_restrict_blk(LevelView<T> F, cons
                                                             const uint8_t* __restrict__ clsc, T* __restrict__ fc) {
   __shared__ T lds[2][2][32];
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int i = blockIdx.x * 64 + tx, j = blockIdx.y * 2 + ty, k = blockIdx.z * 2 + tz;
+  const uint3 xb = xcd_block();
+  const int i = (int)xb.x * 64 + tx, j = (int)xb.y * 2 + ty, k = (int)xb.z * 2 + tz;
   T r = T(0);
   if (i < F.d.nx && j < F.d.ny && k < F.d.nz) {
     const size_t c = F.d.idx(i, j, k);
@@ -162,7 +191,7 @@ __global__ __launch_bounds__(256) void k_resid_restrict_blk(LevelView<T> F, cons
   if ((tx & 1) == 0) lds[tz][ty][tx >> 1] = r;
   __syncthreads();
   if (ty == 0 && tz == 0 && tx < 32) {
-    const int ic = blockIdx.x * 32 + tx, jc = blockIdx.y, kc = blockIdx.z;
+    const int ic = (int)xb.x * 32 + tx, jc = (int)xb.y, kc = (int)xb.z;
     if (ic < dc.nx && jc < dc.ny && kc < dc.nz) {
       const size_t C = dc.idx(ic, jc, kc);
       const T v = (lds[0][0][tx] + lds[0][1][tx]) + (lds[1][0][tx] + lds[1][1][tx]);
@@ -184,7 +213,7 @@ __device__ inline int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i -
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong_add(LevelView<T> F, Dims dc, const uint8_t* __restrict__ clsc,
                                                      int cmask, const T* __restrict__ xc, T* __restrict__ xf) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  CELL_IJK(i, j, k);
   if (i >= F.d.nx || j >= F.d.ny) return;
   const size_t c = F.d.idx(i, j, k);
   if (F.cls[c] & CLS_NONFLUID) return;
@@ -340,15 +369,30 @@ inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 template <typename T>
-void launch_smooth(hipStream_t s, LevelView<T> L, int line, bool zg, const T* f, const T* x_old, T* x_new, T* dv,
-                   T* scr, T c1, T c2) {
+void launch_smooth(hipStream_t s, LevelView<T> L, int line, int mode, const T* f, const T* dinv, const T* x_old,
+                   T* x_new, T* dv, T* scr, T c1, T c2, T c0) {
   if (line) {
-    if (zg) k_smooth_line<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, scr, c1, c2);
+    if (mode == SM_FIRST2) throw StatusError(1, "the line smoother has no fused first-two-steps mode");
+    if (mode == SM_ZERO) k_smooth_line<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, scr, c1, c2);
     else k_smooth_line<T, false><<<line_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, scr, c1, c2);
   } else {
-    if (zg) k_smooth_point<T, true><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, c1, c2);
-    else k_smooth_point<T, false><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, c1, c2);
+    switch (mode) {
+      case SM_ZERO:
+        k_smooth_point<T, SM_ZERO><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_new, dv, c1, c2, c0);
+        break;
+      case SM_FIRST2:
+        k_smooth_point<T, SM_FIRST2><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_new, dv, c1, c2, c0);
+        break;
+      default:
+        k_smooth_point<T, SM_NEXT><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_new, dv, c1, c2, c0);
+    }
   }
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_dinv(hipStream_t s, LevelView<T> L, T* dinv) {
+  k_dinv<T><<<cell_grid(L.d), kBlk, 0, s>>>(L, dinv);
   HIPCHECK(hipGetLastError());
 }
 
@@ -394,7 +438,9 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
 }
 
 #define INST(T)                                                                                                     \
-  template void launch_smooth<T>(hipStream_t, LevelView<T>, int, bool, const T*, const T*, T*, T*, T*, T, T);       \
+  template void launch_smooth<T>(hipStream_t, LevelView<T>, int, int, const T*, const T*, const T*, T*, T*, T*, T, T, \
+                                 T);                                                                              \
+  template void launch_dinv<T>(hipStream_t, LevelView<T>, T*);                                                     \
   template void launch_resid_restrict<T>(hipStream_t, LevelView<T>, const T*, const T*, Dims, const uint8_t*, T*); \
   template void launch_prolong_add<T>(hipStream_t, LevelView<T>, Dims, const uint8_t*, const T*, T*);              \
   template void launch_dense_solve<T>(hipStream_t, int, const T*, const T*, T*);                                   \

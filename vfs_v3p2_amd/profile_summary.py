@@ -1,12 +1,19 @@
-"""Summarise a rocprofv3 kernel trace of the engine (CSV output).
+"""Summarise rocprofv3 output (CSV) of the engine.
 
-    python -m vfs_v3p2_amd.profile_summary <run>_kernel_trace.csv [--pmc <run>_counter_collection.csv]
+    python -m vfs_v3p2_amd.profile_summary <run>_kernel_trace.csv
+    python -m vfs_v3p2_amd.profile_summary --fetch <fetch>_counter_collection.csv \
+        --write <write>_counter_collection.csv --cells N
 
-Prints, as markdown: per-(kernel, grid) launch counts and durations, the
-per-solve kernel time of each solve (a solve runs from its RHS kernel to its
+Trace mode prints, as markdown: per-(kernel, grid) launch counts and durations,
+the kernel time of each solve (a solve runs from its RHS kernel to its
 divergence kernel), and the per-solve breakdown by kernel for the first and the
 last group of solves (the benchmark runs the mixed form first, then FP64).
-With --pmc, also the mean counter value per dispatch for each (kernel, grid).
+
+Traffic mode reads one FETCH_SIZE and one WRITE_SIZE pass (KiB per dispatch),
+scales each by what the same counter reads for the 1 GiB device copy kernel
+(k_copy16: 1 GiB read, 1 GiB written), and prints HBM-side bytes per launch and
+per cell for every kernel whose grid covers N cells, plus a machine-readable
+line for the finest FP32 smoother sweep.
 """
 from __future__ import annotations
 
@@ -43,12 +50,49 @@ def solves(rows):
     return out
 
 
+def _counters(path):
+    acc = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        acc[(_short(r["Kernel_Name"]), int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+COPY = ("k_copy16", 1 << 26)  # 2^26 work-items x 16 B = 1 GiB read and 1 GiB written
+SMOOTHER = "k_smooth_point<float, false>"
+
+
+def traffic(fetch_csv, write_csv, cells, top):
+    F, W = _counters(fetch_csv), _counters(write_csv)
+    if COPY not in F or COPY not in W:
+        raise SystemExit("the copy kernel k_copy16 is missing from the counter runs")
+    fs = (1 << 30) / (F[COPY][0] * 1024.0)
+    ws = (1 << 30) / (W[COPY][0] * 1024.0)
+    print(f"Calibration on k_copy16 (1 GiB read + 1 GiB written): FETCH_SIZE reads {F[COPY][0]:.0f} KiB "
+          f"-> scale {fs:.4f}; WRITE_SIZE reads {W[COPY][0]:.0f} KiB -> scale {ws:.4f}\n")
+    print("| kernel | grid | dispatches | fetched MB | written MB | B/cell |")
+    print("|---|---|---|---|---|---|")
+    keys = sorted((k for k in F if k[1] == cells), key=lambda k: -(F[k][0] * fs + W.get(k, (0, 0))[0] * ws))
+    for k in keys[:top]:
+        f = F[k][0] * 1024 * fs
+        w = W.get(k, (0.0, 0))[0] * 1024 * ws
+        print(f"| {k[0]} | {k[1]} | {F[k][1]} | {f / 1e6:.2f} | {w / 1e6:.2f} | {(f + w) / cells:.1f} |")
+    sm = (SMOOTHER, cells)
+    if sm in F and sm in W:
+        t = F[sm][0] * 1024 * fs + W[sm][0] * 1024 * ws
+        print(f"\ntraffic_bytes_per_launch: {t:.0f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("trace")
-    ap.add_argument("--pmc", default=None)
+    ap.add_argument("trace", nargs="?")
+    ap.add_argument("--fetch", default=None)
+    ap.add_argument("--write", default=None)
+    ap.add_argument("--cells", type=int, default=192 * 96 * 128)
     ap.add_argument("--top", type=int, default=25)
     a = ap.parse_args()
+    if a.fetch:
+        traffic(a.fetch, a.write, a.cells, a.top)
+        return
     rows = load_trace(a.trace)
     by = collections.defaultdict(list)
     for r in rows:
@@ -86,17 +130,6 @@ def main():
             for (k, g), v in sorted(agg.items(), key=lambda kv: -kv[1])[:a.top]:
                 print(f"| {k} | {g[0]}x{g[1]}x{g[2]} | {cnt[(k, g)] // len(chosen)} | {v / 1e3:.1f} | "
                       f"{100 * v / tot:.1f}% |")
-    if a.pmc:
-        acc = collections.defaultdict(list)
-        for r in csv.DictReader(open(a.pmc)):
-            g = (int(r["Grid_Size"]) if "Grid_Size" in r else 0)
-            acc[(_short(r["Kernel_Name"]), r.get("Grid_Size", ""), r["Counter_Name"])].append(
-                float(r["Counter_Value"]))
-        print("\n## Counters (mean per dispatch)\n")
-        print("| kernel | grid size | counter | dispatches | mean |")
-        print("|---|---|---|---|---|")
-        for (k, g, c), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:a.top]:
-            print(f"| {k} | {g} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
 
 
 if __name__ == "__main__":
