@@ -19,70 +19,9 @@
 #include <string>
 #include <vector>
 
-#include "pp_kernels.hpp"
-#include "pp_stencil.hpp"
-#include "vfs_pp.h"
+#include "pp_engine.hpp"
 
 namespace pp {
-
-// --------------------------------------------------------------------- memory
-class Arena {
- public:
-  ~Arena() { release(); }
-  template <typename T>
-  T* alloc(size_t n) {
-    void* p = nullptr;
-    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
-    }
-    // every buffer starts at zero so no result can depend on earlier contents of
-    // the device heap.  The memset goes to the null stream and is waited for
-    // here: the engine's own stream is non-blocking and would otherwise race it.
-    e = hipMemsetAsync(p, 0, bytes, nullptr);
-    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
-    if (e != hipSuccess) {
-      (void)hipFree(p);
-      throw StatusError(VFS_PP_EHIP, "zeroing a new buffer failed");
-    }
-    ptrs_.push_back(p);
-    total_ += bytes;
-    return static_cast<T*>(p);
-  }
-  void release() {
-    for (void* p : ptrs_) (void)hipFree(p);
-    ptrs_.clear();
-    total_ = 0;
-  }
-  size_t total() const { return total_; }
-
- private:
-  std::vector<void*> ptrs_;
-  size_t total_ = 0;
-};
-
-template <typename T>
-static void h2d(hipStream_t s, T* d, const T* h, size_t n) {
-  HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-}
-template <typename T>
-static void d2h(hipStream_t s, T* h, const T* d, size_t n) {
-  HIPCHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
-}
-
-// ------------------------------------------------------------- host hierarchy
-struct HostLevel {
-  Dims d{};
-  std::vector<double> g;       // 9*n face coefficients
-  std::vector<double> nv;      // DMDA nvert of this level (with ghost planes)
-  std::vector<uint32_t> code;  // n
-  std::vector<uint8_t> cls;    // n, CLS_NONFLUID only on coarse levels
-  int cmask = 0;               // axes coarsened towards the next level
-};
-
-static Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {d.per[0], d.per[1], d.per[2]}}; }
 
 static void periodic_fill_host(const Dmda& m, std::vector<double>& a) {
   for (int k = 0; k < m.mz; k++)
@@ -289,155 +228,8 @@ static std::vector<double> coarse_inverse(const HostLevel& L, int* ncomp_out) {
   return out;
 }
 
-// ----------------------------------------------------------- device multigrid
-struct MGBase {
-  virtual ~MGBase() = default;
-  virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
-  virtual void* f0() = 0;
-  virtual void set_out(void* out) = 0;
-  virtual std::string describe() const = 0;
-  virtual void time_sweep(hipStream_t s) = 0;
-};
-
-template <typename T>
-struct MG : MGBase {
-  struct Lev {
-    Dims d{};
-    T* g = nullptr;
-    const uint32_t* code = nullptr;
-    const uint8_t* cls = nullptr;
-    T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr, *dinv = nullptr;
-    int line = 0;
-    std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
-    double lam = 0.;
-    LevelView<T> view() const { return LevelView<T>{d, g, code, cls}; }
-  };
-  std::vector<Lev> lev;
-  T* scr = nullptr;
-  T* cinv = nullptr;
-  int ncoarse = 0, ncomp = 0;
-  T* out = nullptr;
-  int degree = 2;
-
-  void* f0() override { return lev[0].f; }
-  void set_out(void* o) override { out = static_cast<T*>(o); }
-
-  // Chebyshev coefficients on [lo*lam, hi*lam]
-  static void cheb(double lam, int deg, double lo, double hi, std::vector<T>& c1, std::vector<T>& c2) {
-    const double a = lo * lam, b = hi * lam;
-    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
-    double rho = 1. / sigma;
-    c1.assign(deg, T(0));
-    c2.assign(deg, T(0));
-    c2[0] = T(1. / theta);
-    for (int k = 1; k < deg; k++) {
-      const double rn = 1. / (2. * sigma - rho);
-      c1[k] = T(rn * rho);
-      c2[k] = T(2. * rn / delta);
-      rho = rn;
-    }
-  }
-
-  // `degree` Chebyshev steps; from a zero start the point smoother fuses the
-  // first two steps into one pass
-  void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out) {
-    int k = 0;
-    while (k < degree) {
-      int mode = SM_NEXT, adv = 1;
-      if (zero_start && k == 0) {
-        if (!L.line && degree >= 2) { mode = SM_FIRST2; adv = 2; }
-        else mode = SM_ZERO;
-      }
-      const int last = k + adv - 1;  // the last step this launch performs
-      T* xin = x;
-      T* xout = (x == L.xa) ? L.xb : L.xa;
-      if (last == degree - 1 && final_out) xout = final_out;
-      const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
-      launch_smooth<T>(s, L.view(), L.line, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
-      x = xout;
-      k += adv;
-    }
-  }
-
-  void vcycle(hipStream_t s) override { cycle(s, 0, out); }
-
-  // returns the buffer holding this level's result
-  T* cycle(hipStream_t s, size_t l, T* final_out) {
-    Lev& L = lev[l];
-    if (l + 1 == lev.size()) {
-      T* x = final_out ? final_out : L.xa;
-      launch_dense_solve<T>(s, ncoarse, cinv, L.f, x);
-      return x;
-    }
-    T* x = L.xa;
-    smooth(s, L, true, x, nullptr);
-    Lev& C = lev[l + 1];
-    launch_resid_restrict<T>(s, L.view(), L.f, x, C.d, C.cls, C.f);
-    T* xc = cycle(s, l + 1, nullptr);
-    launch_prolong_add<T>(s, L.view(), C.d, C.cls, xc, x);
-    smooth(s, L, false, x, final_out);
-    return x;
-  }
-
-  std::string describe() const override {
-    std::ostringstream o;
-    for (size_t l = 0; l < lev.size(); l++) {
-      o << "level " << l << ": " << lev[l].d.nx << "x" << lev[l].d.ny << "x" << lev[l].d.nz;
-      if (l + 1 == lev.size()) o << " dense(" << ncoarse << ", components " << ncomp << ")";
-      else o << (lev[l].line ? " y-line" : " point") << " chebyshev deg " << degree << " lambda " << lev[l].lam;
-      o << "\n";
-    }
-    return o.str();
-  }
-
-  void time_sweep(hipStream_t s) override {
-    Lev& L = lev[0];
-    launch_smooth<T>(s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
-                     L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
-  }
-};
-
 // --------------------------------------------------------------------- engine
-}  // namespace pp
-
-struct vfs_pp_handle {
-  vfs_grid grid{};
-  vfs_pp_opts opts{};
-  pp::Dmda m{};
-  pp::Dims d{};
-  hipStream_t s = nullptr;
-  pp::Arena ar;
-  bool have_geom = false;
-  // fine-level resident data
-  double* g64 = nullptr;
-  uint32_t* code = nullptr;
-  uint8_t* cls = nullptr;
-  double* met9 = nullptr;
-  double* aj = nullptr;
-  double* nvert = nullptr;
-  pp::IbFace* ibf = nullptr;
-  size_t nibf = 0;
-  double n_lt = 0.;
-  // vectors
-  double *x = nullptr, *b = nullptr, *r = nullptr;
-  double* partial = nullptr;
-  size_t partial_cap = 0;  // doubles in `partial`
-  size_t stencil_blocks = 0;  // blocks of one cell-grid launch
-  double* red = nullptr;   // device scratch scalars (>= 256)
-  double* snap = nullptr;  // DMDA vector scratch for Contra2Cart
-  double* V = nullptr;
-  void* Z = nullptr;
-  int mrest = 0;
-  std::unique_ptr<pp::MGBase> mg;
-  std::vector<pp::HostLevel> hl;
-  double* hpin = nullptr;  // pinned host scratch
-  double* hpin_dev = nullptr;  // the same memory as seen by kernels
-  std::string desc;
-};
-
-namespace pp {
-
-static double now_ms() {
+double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
@@ -559,7 +351,7 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   return mg;
 }
 
-static void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk) {
+void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk) {
   if (nvec * nblk > h->partial_cap)
     throw StatusError(VFS_PP_EINVAL, "reduction scratch too small (" + std::to_string(nvec * nblk) + " > " +
                                          std::to_string(h->partial_cap) + ")");
@@ -675,7 +467,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
   return it;
 }
 
-static void ensure_geom(vfs_pp_handle* h) {
+void ensure_geom(vfs_pp_handle* h) {
   if (!h->have_geom) throw StatusError(VFS_PP_ESTATE, "vfs_pp_set_geometry has not been called");
 }
 
@@ -820,60 +612,83 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
 }
 
 // ------------------------------------------------------------ pressure path
-static void poisson_solve(vfs_pp_handle* h, double* ucont, double* phi, double dt, double st, double coeff,
-                          vfs_pp_stats* out) {
+// Scratch slots of red[] / hpin[] used outside the Krylov loop
+constexpr int RED_IBF = 300;   // VolumeFlux: 4 values
+constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
+constexpr int RED_B2 = 320;    // ||b||^2
+constexpr int RED_DIV = 330;   // max |aj div u|
+
+// VolumeFlux and RHS, launches only; ||b||^2 lands in hpin[RED_B2] once the
+// stream is synchronised.
+void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff) {
   ensure_geom(h);
-  const double t0 = now_ms();
-  vfs_pp_stats S{};
   hipStream_t s = h->s;
-  const size_t n = h->d.n();
   int nb = 0;
-  double tmp[4];
-  // VolumeFlux
   if (h->nibf) {
     launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 1, h->partial, &nb);
-    launch_finalize(s, h->partial, 2, nb, h->red + 300, 0, h->hpin_dev + 300);
-    launch_ibflux_correct(s, h->ibf, h->nibf, ucont, h->grid.immersed, h->red + 300);
+    launch_finalize(s, h->partial, 2, nb, h->red + RED_IBF, 0, h->hpin_dev + RED_IBF);
+    launch_ibflux_correct(s, h->ibf, h->nibf, ucont, h->grid.immersed, h->red + RED_IBF);
     launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 0, h->partial, &nb);
-    launch_finalize(s, h->partial, 2, nb, h->red + 302, 0, h->hpin_dev + 302);
+    launch_finalize(s, h->partial, 2, nb, h->red + RED_IBF + 2, 0, h->hpin_dev + RED_IBF + 2);
   }
   // RHS: val *= -1.0 / dt * st * coeff  (poisson_hypre.c:376)
   const double fac = -1.0 / dt * st * coeff;
   launch_rhs(s, h->m, h->d, ucont, h->cls, fac, h->b, h->partial, &nb);
-  double b2;
-  finalize_to_host(h, 1, nb, 0, &b2);
-  if (h->nibf) for (int q = 0; q < 4; q++) S.ibm_flux[q] = h->hpin[300 + q];
-  const double bnorm = std::sqrt(b2);
-  S.bnorm = bnorm;
-  if (!std::isfinite(bnorm)) throw StatusError(VFS_PP_ENONFINITE, "non-finite right-hand side");
-  // initial guess
-  if (h->opts.zero_guess) launch_fill(s, n, h->x, 0.);
-  else launch_gather_phi(s, h->m, h->d, phi, h->cls, h->x);
+  launch_finalize(s, h->partial, 1, nb, h->red + RED_B2, 0, h->hpin_dev + RED_B2);
+}
+
+// After the prologue's stream sync: ||b||, VolumeFlux figures, initial guess.
+void solve_start(vfs_pp_handle* h, double* phi, vfs_pp_stats* S) {
+  if (h->nibf)
+    for (int q = 0; q < 4; q++) S->ibm_flux[q] = h->hpin[RED_IBF + q];
+  S->bnorm = std::sqrt(h->hpin[RED_B2]);
+  if (!std::isfinite(S->bnorm)) throw StatusError(VFS_PP_ENONFINITE, "non-finite right-hand side");
+  if (h->opts.zero_guess) launch_fill(h->s, h->d.n(), h->x, 0.);
+  else launch_gather_phi(h->s, h->m, h->d, phi, h->cls, h->x);
+}
+
+// Remove_Nullspace (poisson_hypre.c:207-270) and Convert_Phi2_Phi
+// (poisson.c:97-185), launches only; the subtracted mean is hpin[RED_MEAN] / n_lt
+// after the next stream sync.
+void solve_epilogue(vfs_pp_handle* h, double* phi, vfs_pp_stats* S) {
+  hipStream_t s = h->s;
+  int nb = 0;
+  launch_masked_sum(s, h->d, h->x, h->cls, h->partial, &nb);
+  launch_finalize(s, h->partial, 1, nb, h->red + RED_MEAN, 0, h->hpin_dev + RED_MEAN);
+  if (h->n_lt > 0.) launch_sub_mean(s, h->d, h->x, h->cls, h->red + RED_MEAN, h->n_lt);
+  launch_scatter_update(s, h->m, h->d, h->x, h->cls, phi, nullptr);
+  launch_periodic_scalar(s, h->m, phi, 1);
+  (void)S;
+}
+
+static double phi_mean_of(const vfs_pp_handle* h) { return h->n_lt > 0. ? h->hpin[RED_MEAN] / h->n_lt : 0.; }
+
+static void poisson_solve(vfs_pp_handle* h, double* ucont, double* phi, double dt, double st, double coeff,
+                          vfs_pp_stats* out) {
+  const double t0 = now_ms();
+  vfs_pp_stats S{};
+  solve_prologue(h, ucont, dt, st, coeff);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  solve_start(h, phi, &S);
   double rel = 0.;
   int vc = 0;
   bool conv = false;
   int it;
-  if (bnorm == 0.) { it = 0; rel = 0.; conv = true; }
-  else if (h->opts.precision == 1) it = fgmres<float>(h, bnorm, &rel, &vc, &conv);
-  else it = fgmres<double>(h, bnorm, &rel, &vc, &conv);
+  if (S.bnorm == 0.) { it = 0; rel = 0.; conv = true; }
+  else if (h->opts.precision == 1) it = fgmres<float>(h, S.bnorm, &rel, &vc, &conv);
+  else it = fgmres<double>(h, S.bnorm, &rel, &vc, &conv);
   S.iterations = it;
   S.vcycles = vc;
   S.rel_residual = rel;
   S.converged = conv ? 1 : 0;
-  // Remove_Nullspace (poisson_hypre.c:207-270)
-  launch_masked_sum(s, h->d, h->x, h->cls, h->partial, &nb);
-  launch_finalize(s, h->partial, 1, nb, h->red + 310, 0, h->hpin_dev + 310);
-  if (h->n_lt > 0.) launch_sub_mean(s, h->d, h->x, h->cls, h->red + 310, h->n_lt);
-  // Convert_Phi2_Phi (poisson.c:97-185)
-  launch_scatter_update(s, h->m, h->d, h->x, h->cls, phi, nullptr);
-  launch_periodic_scalar(s, h->m, phi, 1);
-  HIPCHECK(hipStreamSynchronize(s));
-  S.phi_mean = h->n_lt > 0. ? h->hpin[310] / h->n_lt : 0.;
+  solve_epilogue(h, phi, &S);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  S.phi_mean = phi_mean_of(h);
   S.ms_total = now_ms() - t0;
   if (out) *out = S;
 }
 
-static void update_pressure(vfs_pp_handle* h, double* p, double* phi) {
+void update_pressure(vfs_pp_handle* h, double* p, double* phi) {
   ensure_geom(h);
   // P += Phi on fluid interior cells; periodic ghosts of P and Phi (poisson.c:2522-2614)
   launch_add_phi(h->s, h->m, h->d, h->cls, phi, p);
@@ -881,7 +696,7 @@ static void update_pressure(vfs_pp_handle* h, double* p, double* phi) {
   launch_periodic_scalar(h->s, h->m, phi, 0);
 }
 
-static void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
+void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
                        double coeff) {
   ensure_geom(h);
   const double fac = dt * st / coeff;
@@ -890,13 +705,19 @@ static void projection(vfs_pp_handle* h, double* ucont, const double* phi, doubl
   launch_c2c(h->s, h->m, h->d, h->met9, h->cls, h->nvert, ucont, ucat, h->snap, h->grid.bctype);
 }
 
-static double divergence(vfs_pp_handle* h, const double* ucont) {
+// Divergence (main.c:1244-1391), launches only; the maximum is hpin[RED_DIV]
+// after the next stream sync.
+void divergence_launch(vfs_pp_handle* h, const double* ucont) {
   ensure_geom(h);
   int nb = 0;
   launch_divergence(h->s, h->m, h->d, h->code, h->aj, ucont, h->partial, &nb);
-  double mx;
-  finalize_to_host(h, 1, nb, 1, &mx);
-  return mx;
+  launch_finalize(h->s, h->partial, 1, nb, h->red + RED_DIV, 1, h->hpin_dev + RED_DIV);
+}
+
+static double divergence(vfs_pp_handle* h, const double* ucont) {
+  divergence_launch(h, ucont);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  return h->hpin[RED_DIV];
 }
 
 }  // namespace pp

@@ -1,0 +1,237 @@
+// Shared host-side definitions of the pressure-projection engine: device memory
+// arena, the host multigrid hierarchy, the device multigrid (MG<T>) and the
+// per-case handle.  Used by the single-case engine and by the batched/served
+// engine.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pp_kernels.hpp"
+#include "pp_stencil.hpp"
+#include "vfs_pp.h"
+
+namespace pp {
+
+// --------------------------------------------------------------------- memory
+class Arena {
+ public:
+  ~Arena() { release(); }
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    // every buffer starts at zero so no result can depend on earlier contents of
+    // the device heap.  The memset goes to the null stream and is waited for
+    // here: the engine's own stream is non-blocking and would otherwise race it.
+    e = hipMemsetAsync(p, 0, bytes, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      throw StatusError(VFS_PP_EHIP, "zeroing a new buffer failed");
+    }
+    ptrs_.push_back(p);
+    total_ += bytes;
+    return static_cast<T*>(p);
+  }
+  void release() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+    total_ = 0;
+  }
+  size_t total() const { return total_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t total_ = 0;
+};
+
+template <typename T>
+inline void h2d(hipStream_t s, T* d, const T* h, size_t n) {
+  HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+inline void d2h(hipStream_t s, T* h, const T* d, size_t n) {
+  HIPCHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// ------------------------------------------------------------- host hierarchy
+struct HostLevel {
+  Dims d{};
+  std::vector<double> g;       // 9*n face coefficients
+  std::vector<double> nv;      // DMDA nvert of this level (with ghost planes)
+  std::vector<uint32_t> code;  // n
+  std::vector<uint8_t> cls;    // n, CLS_NONFLUID only on coarse levels
+  int cmask = 0;               // axes coarsened towards the next level
+};
+
+inline Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {d.per[0], d.per[1], d.per[2]}}; }
+
+// ----------------------------------------------------------- device multigrid
+struct MGBase {
+  virtual ~MGBase() = default;
+  virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
+  virtual void* f0() = 0;
+  virtual void set_out(void* out) = 0;
+  virtual std::string describe() const = 0;
+  virtual void time_sweep(hipStream_t s) = 0;
+};
+
+template <typename T>
+struct MG : MGBase {
+  struct Lev {
+    Dims d{};
+    T* g = nullptr;
+    const uint32_t* code = nullptr;
+    const uint8_t* cls = nullptr;
+    T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr, *dinv = nullptr;
+    int line = 0;
+    std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
+    double lam = 0.;
+    LevelView<T> view() const { return LevelView<T>{d, g, code, cls}; }
+  };
+  std::vector<Lev> lev;
+  T* scr = nullptr;
+  T* cinv = nullptr;
+  int ncoarse = 0, ncomp = 0;
+  T* out = nullptr;
+  int degree = 2;
+
+  void* f0() override { return lev[0].f; }
+  void set_out(void* o) override { out = static_cast<T*>(o); }
+
+  // Chebyshev coefficients on [lo*lam, hi*lam]
+  static void cheb(double lam, int deg, double lo, double hi, std::vector<T>& c1, std::vector<T>& c2) {
+    const double a = lo * lam, b = hi * lam;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double rho = 1. / sigma;
+    c1.assign(deg, T(0));
+    c2.assign(deg, T(0));
+    c2[0] = T(1. / theta);
+    for (int k = 1; k < deg; k++) {
+      const double rn = 1. / (2. * sigma - rho);
+      c1[k] = T(rn * rho);
+      c2[k] = T(2. * rn / delta);
+      rho = rn;
+    }
+  }
+
+  // `degree` Chebyshev steps; from a zero start the point smoother fuses the
+  // first two steps into one pass
+  void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out) {
+    int k = 0;
+    while (k < degree) {
+      int mode = SM_NEXT, adv = 1;
+      if (zero_start && k == 0) {
+        if (!L.line && degree >= 2) { mode = SM_FIRST2; adv = 2; }
+        else mode = SM_ZERO;
+      }
+      const int last = k + adv - 1;  // the last step this launch performs
+      T* xin = x;
+      T* xout = (x == L.xa) ? L.xb : L.xa;
+      if (last == degree - 1 && final_out) xout = final_out;
+      const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
+      launch_smooth<T>(s, L.view(), L.line, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
+      x = xout;
+      k += adv;
+    }
+  }
+
+  void vcycle(hipStream_t s) override { cycle(s, 0, out); }
+
+  // returns the buffer holding this level's result
+  T* cycle(hipStream_t s, size_t l, T* final_out) {
+    Lev& L = lev[l];
+    if (l + 1 == lev.size()) {
+      T* x = final_out ? final_out : L.xa;
+      launch_dense_solve<T>(s, ncoarse, cinv, L.f, x);
+      return x;
+    }
+    T* x = L.xa;
+    smooth(s, L, true, x, nullptr);
+    Lev& C = lev[l + 1];
+    launch_resid_restrict<T>(s, L.view(), L.f, x, C.d, C.cls, C.f);
+    T* xc = cycle(s, l + 1, nullptr);
+    launch_prolong_add<T>(s, L.view(), C.d, C.cls, xc, x);
+    smooth(s, L, false, x, final_out);
+    return x;
+  }
+
+  std::string describe() const override {
+    std::ostringstream o;
+    for (size_t l = 0; l < lev.size(); l++) {
+      o << "level " << l << ": " << lev[l].d.nx << "x" << lev[l].d.ny << "x" << lev[l].d.nz;
+      if (l + 1 == lev.size()) o << " dense(" << ncoarse << ", components " << ncomp << ")";
+      else o << (lev[l].line ? " y-line" : " point") << " chebyshev deg " << degree << " lambda " << lev[l].lam;
+      o << "\n";
+    }
+    return o.str();
+  }
+
+  void time_sweep(hipStream_t s) override {
+    Lev& L = lev[0];
+    launch_smooth<T>(s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
+                     L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
+  }
+};
+
+}  // namespace pp
+
+struct vfs_pp_handle {
+  vfs_grid grid{};
+  vfs_pp_opts opts{};
+  pp::Dmda m{};
+  pp::Dims d{};
+  hipStream_t s = nullptr;
+  pp::Arena ar;
+  bool have_geom = false;
+  // fine-level resident data
+  double* g64 = nullptr;
+  uint32_t* code = nullptr;
+  uint8_t* cls = nullptr;
+  double* met9 = nullptr;
+  double* aj = nullptr;
+  double* nvert = nullptr;
+  pp::IbFace* ibf = nullptr;
+  size_t nibf = 0;
+  double n_lt = 0.;
+  // vectors
+  double *x = nullptr, *b = nullptr, *r = nullptr;
+  double* partial = nullptr;
+  size_t partial_cap = 0;  // doubles in `partial`
+  size_t stencil_blocks = 0;  // blocks of one cell-grid launch
+  double* red = nullptr;   // device scratch scalars (>= 256)
+  double* snap = nullptr;  // DMDA vector scratch for Contra2Cart
+  double* V = nullptr;
+  void* Z = nullptr;
+  int mrest = 0;
+  std::unique_ptr<pp::MGBase> mg;
+  std::vector<pp::HostLevel> hl;
+  double* hpin = nullptr;  // pinned host scratch
+  double* hpin_dev = nullptr;  // the same memory as seen by kernels
+  std::string desc;
+};
+
+namespace pp {
+// engine functions shared with the batched engine (pp_engine.hip)
+double now_ms();
+void ensure_geom(vfs_pp_handle* h);
+void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk);
+void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff);
+void solve_start(vfs_pp_handle* h, double* phi, vfs_pp_stats* S);
+void solve_epilogue(vfs_pp_handle* h, double* phi, vfs_pp_stats* S);
+void update_pressure(vfs_pp_handle* h, double* p, double* phi);
+void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
+                double coeff);
+void divergence_launch(vfs_pp_handle* h, const double* ucont);
+}  // namespace pp
