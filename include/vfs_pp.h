@@ -173,6 +173,28 @@ int vfs_pp_mg_prolong_device(const vfs_grid *fine, const double *nvert_f, const 
                              const double *xc, double *ff);
 int vfs_pp_mg_nvert_restrict_device(const vfs_grid *fine, const double *nvert_f, double *nvert_c);
 
+/* ---- batched form: many independent cases solved together.
+ *
+ * No counterpart in the reference (it solves one system per process, state in
+ * process globals, poisson_hypre.c:17-21).  Every kernel launch of a batched
+ * solve covers all member cases through one work list (persistent workers,
+ * XCD-contiguous tile order), and one FGMRES iteration of all members is one
+ * HIP graph launch.  Members keep their own geometry, hierarchy and Krylov
+ * storage; they must share opts.precision and opts.cheb_degree and use point
+ * smoothing (opts.smoother = 1).  A member must not be used on its own while
+ * the batch exists. */
+typedef struct vfs_pp_batch vfs_pp_batch;
+int vfs_pp_batch_create(vfs_pp_handle *const *members, int count, vfs_pp_batch **out);
+int vfs_pp_batch_destroy(vfs_pp_batch *b);
+/* 1 (default): replay each iteration as a captured HIP graph; 0: plain launches */
+int vfs_pp_batch_set_graphs(vfs_pp_batch *b, int on);
+/* vfs_pp_project_device for every member at once; per-member device arrays and
+ * scalars; stats (may be NULL) receives one entry per member.  Returns
+ * VFS_PP_ENOCONV if any member missed rtol or div_tol (see its stats). */
+int vfs_pp_batch_project_device(vfs_pp_batch *b, double *const *ucont, double *const *phi,
+                                double *const *p, double *const *ucat, const double *dt,
+                                const double *st, const double *coeff, vfs_pp_stats *stats);
+
 /* Time `reps` launches of one engine kernel on the handle's stream with HIP
  * events; kernel ids: 0 FP64 operator apply, 1 finest-level smoother sweep (in
  * the multigrid precision), 2 projection, 3 Contra2Cart, 4 RHS, 5 a plain

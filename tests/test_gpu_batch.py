@@ -1,0 +1,105 @@
+"""Batched form: independent cases of uneven sizes solved together through
+vfs_pp_batch_project_device must give each case the result it gets alone (and
+the oracle's), call after call, with and without HIP graph replay."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfs_v3p2_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [((20, 16, 24), 1.5, 0.2, False, 11), ((32, 24, 16), 2.5, 0.35, True, 12),
+         ((16, 16, 16), 1.0, 0.0, False, 13), ((24, 20, 28), 3.0, 0.1, True, 14)]
+_cases = {}
+
+
+def get_cases():
+    if not _cases:
+        for q, sp in enumerate(SPECS):
+            c = cases.ensemble_case(*sp)
+            _cases[q] = (c, O.Assembled(c, drop=False))
+    return [_cases[q] for q in range(len(SPECS))]
+
+
+def relL2(a, b, mask):
+    return float(np.linalg.norm((a - b)[mask]) / np.linalg.norm(b[mask]))
+
+
+@pytest.fixture(scope="module")
+def E(torch_cuda):
+    from vfs_v3p2_amd import engine
+
+    engine.lib()
+    return engine
+
+
+def handles(E, cs, **kw):
+    out = []
+    for c, _ in cs:
+        h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed),
+                     E.default_opts(smoother=1, **kw))
+        h.set_geometry(c.fields)
+        out.append(h)
+    return out
+
+
+def fields(torch, cs):
+    u = [torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda() for c, _ in cs]
+    phi = [torch.zeros(c.shape, dtype=torch.float64, device="cuda") for c, _ in cs]
+    p = [torch.zeros(c.shape, dtype=torch.float64, device="cuda") for c, _ in cs]
+    ucat = [torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda") for c, _ in cs]
+    return u, phi, p, ucat
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_batch_equals_alone_and_oracle(E, torch_cuda, precision):
+    torch = torch_cuda
+    cs = get_cases()
+    print(f"batch of {len(cs)} cases, precision {precision}", flush=True)
+    # alone
+    alone = []
+    for h, (c, A) in zip(handles(E, cs, precision=precision, rtol=1e-12), cs):
+        u, phi, p, ucat = fields(torch, [(c, A)])
+        st = h.project(u[0], phi[0], p[0], ucat[0], c.dt, c.st, c.coeff, strict=True)
+        alone.append((st, phi[0].cpu().numpy(), u[0].cpu().numpy()))
+        h.close()
+    hs = handles(E, cs, precision=precision, rtol=1e-12)
+    B = E.Batch(hs)
+    runs = []
+    for call in range(2):
+        u, phi, p, ucat = fields(torch, cs)
+        sts = B.project(u, phi, p, ucat, [c.dt for c, _ in cs], [c.st for c, _ in cs], [c.coeff for c, _ in cs],
+                        strict=True)
+        runs.append([(s, ph.cpu().numpy(), uu.cpu().numpy(), uc.cpu().numpy()) for s, ph, uu, uc in
+                     zip(sts, phi, u, ucat)])
+    B.close()
+    # plain launches instead of graph replay
+    hs2 = handles(E, cs, precision=precision, rtol=1e-12)
+    B2 = E.Batch(hs2, graphs=False)
+    u, phi, p, ucat = fields(torch, cs)
+    sts2 = B2.project(u, phi, p, ucat, [c.dt for c, _ in cs], [c.st for c, _ in cs], [c.coeff for c, _ in cs],
+                      strict=True)
+    nograph = [ph.cpu().numpy() for ph in phi]
+    B2.close()
+    for q, (c, A) in enumerate(cs):
+        s, ph, uu, uc = runs[0][q]
+        assert s.converged and s.max_div <= 1e-8
+        np.testing.assert_array_equal(ph, runs[1][q][1])  # graph replays agree bit for bit
+        np.testing.assert_array_equal(ph, nograph[q])     # graph == plain launches
+        assert relL2(ph, alone[q][1], A.fluid_cells) <= 1e-9
+        assert abs(s.iterations - alone[q][0].iterations) <= 1
+        ref = O.reference_projection(c, rtol=1e-13)
+        assert relL2(ph, ref["phi"], A.fluid_cells) <= 1e-8
+    for h in hs + hs2:
+        h.close()
+
+
+def test_batch_rejects_mixed_precision(E):
+    cs = get_cases()[:2]
+    h0 = handles(E, cs[:1], precision=1)[0]
+    h1 = handles(E, cs[1:], precision=0)[0]
+    with pytest.raises(E.VfsPPError):
+        E.Batch([h0, h1])
+    h0.close()
+    h1.close()

@@ -1,0 +1,982 @@
+// Batched and served pressure projections: many independent cases (each with
+// its own grid size, metrics, multigrid hierarchy and Krylov storage, held by
+// its own vfs_pp_handle) advance together.
+//
+// * Every kernel launch covers all member cases: the launch's work list holds
+//   one item per case (a level of its hierarchy, or its Krylov vectors) and a
+//   prefix sum of the items' tile counts; persistent workers walk the combined
+//   tile range, each XCD taking one contiguous eighth of it so that neighbouring
+//   tiles of a case share that XCD's L2.
+// * Each case's Krylov step j and an active flag live in a small device array,
+//   so the launch sequence of one FGMRES iteration (the V-cycle of every case,
+//   then the three CGS2 passes) never changes while the membership stays the
+//   same: it is captured once as a HIP graph and replayed every iteration.
+// * The host keeps each case's Hessenberg column, Givens rotations and restart
+//   logic (the same algorithm as the single-case engine) and handles cases that
+//   start, restart or finish between iterations.
+//
+// The batch entry point solves a fixed set of cases; the server admits new
+// requests between iterations (continuous batching) as slots and device memory
+// free up.
+#include <cstring>
+#include <deque>
+#include <map>
+
+#include "pp_engine.hpp"
+#include "pp_krylov_dev.hpp"
+#include "pp_mg_dev.hpp"
+
+namespace pp {
+
+namespace {
+
+// ------------------------------------------------------------------ device side
+__device__ __forceinline__ int find_item(const uint32_t* __restrict__ start, int nitems, uint32_t t) {
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent workers (gridDim.x a multiple of 8): worker w runs on XCD w % 8 and
+// walks that XCD's contiguous eighth of the tile range [0, total).
+#define FOR_TILES(total, t)                                                               \
+  const uint32_t xcd_ = blockIdx.x & 7u, wper_ = gridDim.x >> 3;                         \
+  const uint32_t chunk_ = ((total) + 7u) >> 3;                                            \
+  for (uint32_t u_ = blockIdx.x >> 3, t = xcd_ * chunk_ + u_; u_ < chunk_ && t < (total); \
+       u_ += wper_, t = xcd_ * chunk_ + u_)
+
+// slot state: x = Krylov step j, y = active
+using SlotDev = int2;
+
+template <typename T>
+struct SmoothItem {
+  LevelView<T> L;
+  const T* f;
+  const T* dinv;
+  const T* xin;
+  T* xout;
+  T* dv;
+  T* out_base;  // non-null: the output is out_base + j * out_stride (the case's Z_j)
+  size_t out_stride;
+  T c1, c2, c0;
+  int slot;
+  uint32_t tx, ty;  // 64 x 4 tiles of the level
+};
+
+template <typename T>
+struct RestrictItem {
+  LevelView<T> F;
+  const T* f;
+  const T* x;
+  Dims dc;
+  const uint8_t* clsc;
+  T* fc;
+  int cmask;
+  int slot;
+  uint32_t tx, ty;  // blk: 64x2x2 fine tiles; general: 64x4x1 coarse tiles
+};
+
+template <typename T>
+struct ProlongItem {
+  LevelView<T> F;
+  Dims dc;
+  const uint8_t* clsc;
+  const T* xc;
+  T* xf;
+  int cmask;
+  int slot;
+  uint32_t tx, ty;
+};
+
+template <typename T>
+struct DenseItem {
+  const T* cinv;
+  const T* f;
+  T* x;
+  T* out_base;
+  size_t out_stride;
+  int n;
+  int slot;
+};
+
+template <typename T>
+struct ApplyItem {
+  Dims d;
+  const double* g;
+  const uint32_t* code;
+  const T* Z;
+  double* V;
+  size_t n;
+  int slot;
+  uint32_t tx, ty;
+};
+
+template <typename T>
+struct VecItem {
+  double* V;
+  double* red;
+  double* hpin;
+  T* f0;
+  size_t n;
+  int slot;
+};
+
+struct FinItem {
+  double* red;
+  double* hpin;
+  uint32_t t0, t1;
+  int slot;
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict__ items,
+                                                 const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                 const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const SmoothItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const uint32_t lt = t - start[q];
+    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
+    const int k = (int)(lt / (I.tx * I.ty));
+    if (i >= I.L.d.nx || j >= I.L.d.ny) continue;
+    T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
+    smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_restrict_blk(const RestrictItem<T>* __restrict__ items,
+                                                       const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                       const SlotDev* __restrict__ sd) {
+  __shared__ T lds[2][2][32];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const RestrictItem<T>& I = items[q];
+    if (!sd[I.slot].y) continue;
+    const uint32_t lt = t - start[q];
+    resid_restrict_blk(I.F, I.f, I.x, I.dc, I.clsc, I.fc, (int)(lt % I.tx), (int)((lt / I.tx) % I.ty),
+                       (int)(lt / (I.tx * I.ty)), lds);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __restrict__ items,
+                                                       const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                       const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const RestrictItem<T>& I = items[q];
+    if (!sd[I.slot].y) continue;
+    const uint32_t lt = t - start[q];
+    const int ic = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int jc = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
+    const int kc = (int)(lt / (I.tx * I.ty));
+    if (ic >= I.dc.nx || jc >= I.dc.ny) continue;
+    resid_restrict_cell(I.F, I.f, I.x, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_prolong(const ProlongItem<T>* __restrict__ items,
+                                                  const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                  const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const ProlongItem<T>& I = items[q];
+    if (!sd[I.slot].y) continue;
+    const uint32_t lt = t - start[q];
+    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
+    const int k = (int)(lt / (I.tx * I.ty));
+    if (i >= I.F.d.nx || j >= I.F.d.ny) continue;
+    prolong_add_cell(I.F, I.dc, I.clsc, I.cmask, I.xc, I.xf, i, j, k);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_dense(const DenseItem<T>* __restrict__ items,
+                                                const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                const SlotDev* __restrict__ sd) {
+  __shared__ T fs[1024];
+  __shared__ T part[4][64];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const DenseItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    T* x = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.x;
+    dense_rows(I.n, I.cinv, I.f, x, (int)(t - start[q]), fs, part);
+  }
+}
+
+// w = A z_j (FP64 flux form) into V_{j+1}; per-tile partial sums of w.V_l (l <= j)
+// and w.w at partial[l * total + t]
+template <typename T>
+__global__ __launch_bounds__(256) void kb_apply_dots(const ApplyItem<T>* __restrict__ items,
+                                                     const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                     const SlotDev* __restrict__ sd, double* __restrict__ partial) {
+  __shared__ double lds[4][64];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const ApplyItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const uint32_t lt = t - start[q];
+    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
+    const int k = (int)(lt / (I.tx * I.ty));
+    const bool in = i < I.d.nx && j < I.d.ny;
+    const T* z = I.Z + (size_t)st.x * I.n;
+    double* w = I.V + (size_t)(st.x + 1) * I.n;
+    const size_t c = in ? I.d.idx(i, j, k) : 0;
+    const double wv = in ? apply_cell(I.d, I.g, I.code, z, w, i, j, k) : 0.;
+    block_dots(in, c, wv, I.V, I.n, st.x + 1, partial, total, t, lds);
+  }
+}
+
+// w1 = w - V h1 in place over V_{j+1} (h1 = red[8..]); partial sums of w1.V_l, w1.w1
+template <typename T>
+__global__ __launch_bounds__(256) void kb_axpy_dots(const VecItem<T>* __restrict__ items,
+                                                    const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                    const SlotDev* __restrict__ sd, double* __restrict__ partial) {
+  __shared__ double lds[4][64];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const VecItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
+    const bool in = e < I.n;
+    double* w = I.V + (size_t)(st.x + 1) * I.n;
+    const double w1 = in ? gs_sub(w, I.V, I.n, st.x + 1, I.red + 8, e) : 0.;
+    block_dots(in, e, w1, I.V, I.n, st.x + 1, partial, total, t, lds);
+  }
+}
+
+// v_{j+1} = (w1 - V h2) / ||.|| in place; f0 = (T)v_{j+1}, the next V-cycle's input
+template <typename T>
+__global__ __launch_bounds__(256) void kb_axpy_scale(const VecItem<T>* __restrict__ items,
+                                                     const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                     const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const VecItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
+    if (e >= I.n) continue;
+    axpy_scale_one(I.V + (size_t)(st.x + 1) * I.n, I.V, I.n, st.x + 1, I.red + 72, I.red[2], I.f0, e);
+  }
+}
+
+// One block per (case, vector l < j + 1 + extra): fixed-order sum of the case's
+// tile partials into red[off + l], mirrored to pinned host memory.
+__global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ items, const double* __restrict__ partial,
+                                                   uint32_t total, int off, int extra,
+                                                   const SlotDev* __restrict__ sd) {
+  __shared__ double lds[4];
+  const FinItem& I = items[blockIdx.x];
+  const SlotDev st = sd[I.slot];
+  const int l = blockIdx.y;
+  if (!st.y || l >= st.x + 1 + extra) return;
+  const double* pv = partial + (size_t)l * total;
+  double a0 = 0., a1 = 0.;
+  uint32_t b = I.t0 + threadIdx.x;
+  for (; b + 256 < I.t1; b += 512) { a0 += pv[b]; a1 += pv[b + 256]; }
+  for (; b < I.t1; b += 256) a0 += pv[b];
+  const double s = block_sum(a0 + a1, lds);
+  if (threadIdx.x == 0) {
+    I.red[off + l] = s;
+    I.hpin[off + l] = s;
+  }
+}
+
+__global__ void kb_gs_norm(const FinItem* __restrict__ items, int nitems, const SlotDev* __restrict__ sd) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nitems) return;
+  const FinItem& I = items[q];
+  const SlotDev st = sd[I.slot];
+  if (!st.y) return;
+  gs_norm_one(I.red, 72, st.x + 1, I.hpin);
+}
+
+inline unsigned workers(uint32_t total) {
+  const uint32_t w = (total + 7u) & ~7u;
+  return w < 2048u ? (w < 8u ? 8u : w) : 2048u;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- host side
+template <typename T>
+struct BatchCore {
+  enum State { FREE = 0, START, ITER, FINISH, DONE };
+  struct Slot {
+    vfs_pp_handle* h = nullptr;
+    MG<T>* mg = nullptr;
+    hipStream_t own_stream = nullptr;
+    State state = FREE;
+    int ticket = -1;
+    double *ucont = nullptr, *phi = nullptr, *p = nullptr, *ucat = nullptr;
+    double dt = 0., st = 0., coeff = 0.;
+    vfs_pp_stats S{};
+    double t0 = 0.;
+    int j = 0, it = 0, vc = 0, m = 0;
+    double tgt = 0., rel = 1.;
+    bool conv = false, pending_update = false, fresh = false;
+    int pending_nv = 0;
+    std::vector<double> H, cs, sn, gv, y;
+  };
+
+  hipStream_t s = nullptr;
+  std::vector<Slot> slots;
+  Arena fixed;            // slot state, host-mirrored
+  SlotDev* sd = nullptr;  // device slot state
+  SlotDev* sd_host = nullptr;
+  Arena plan;  // work lists of the current membership
+  double* partial = nullptr;
+  size_t partial_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true;
+  int degree = 0;
+  bool use_graph = true;
+
+  struct Launch {
+    int kind, mode;
+    const void* items;
+    const uint32_t* start;
+    int nitems;
+    uint32_t total;
+  };
+  std::vector<Launch> seq;  // V-cycle + Krylov passes of one iteration
+  Launch apply{}, axd{}, axs{};
+  const FinItem* fin = nullptr;
+  int nfin = 0;
+  uint32_t fin_total_apply = 0, fin_total_vec = 0;
+  FinItem* fin_apply = nullptr;
+  FinItem* fin_vec = nullptr;
+
+  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE };
+
+  explicit BatchCore(int nslots) {
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    slots.resize(nslots);
+    sd = fixed.alloc<SlotDev>(nslots);
+    HIPCHECK(hipHostMalloc(&sd_host, sizeof(SlotDev) * nslots));
+    std::memset(sd_host, 0, sizeof(SlotDev) * nslots);
+  }
+  ~BatchCore() {
+    for (auto& sl : slots)
+      if (sl.h) release(sl);
+    drop_graph();
+    if (sd_host) (void)hipHostFree(sd_host);
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  // a case joins slot q: its launches go to the batch stream while it is here
+  void bind(int q, vfs_pp_handle* h) {
+    Slot& sl = slots[q];
+    ensure_geom(h);
+    auto* mg = dynamic_cast<MG<T>*>(h->mg.get());
+    if (!mg) throw StatusError(VFS_PP_EINVAL, "batch members must share the multigrid precision");
+    for (auto& L : mg->lev)
+      if (L.line) throw StatusError(VFS_PP_EINVAL, "batch members must use point smoothing (opts.smoother = 1)");
+    if (degree == 0) degree = mg->degree;
+    if (mg->degree != degree) throw StatusError(VFS_PP_EINVAL, "batch members must share cheb_degree");
+    sl.h = h;
+    sl.mg = mg;
+    sl.own_stream = h->s;
+    h->s = s;
+    sl.state = DONE;
+    dirty = true;
+  }
+  void release(Slot& sl) {
+    if (sl.h) sl.h->s = sl.own_stream;
+    sl = Slot{};
+    dirty = true;
+  }
+
+  // ---------------------------------------------------------------- the plan
+  template <typename Item>
+  std::pair<const Item*, const uint32_t*> upload(const std::vector<Item>& items, const std::vector<uint32_t>& start) {
+    Item* di = plan.alloc<Item>(items.size());
+    uint32_t* ds = plan.alloc<uint32_t>(start.size());
+    HIPCHECK(hipMemcpyAsync(di, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(ds, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    return {di, ds};
+  }
+  template <typename Item>
+  void add_launch(int kind, int mode, const std::vector<Item>& items, const std::vector<uint32_t>& ntiles) {
+    if (items.empty()) return;
+    std::vector<uint32_t> start(items.size());
+    uint32_t tot = 0;
+    for (size_t q = 0; q < items.size(); q++) { start[q] = tot; tot += ntiles[q]; }
+    if (tot == 0) return;
+    auto pr = upload(items, start);
+    seq.push_back(Launch{kind, mode, pr.first, pr.second, (int)items.size(), tot});
+  }
+
+  static uint32_t tiles64x4(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
+
+  void build_plan() {
+    HIPCHECK(hipStreamSynchronize(s));
+    drop_graph();
+    plan.release();
+    seq.clear();
+    std::vector<int> mem;
+    for (int q = 0; q < (int)slots.size(); q++)
+      if (slots[q].h) mem.push_back(q);
+    if (mem.empty()) { dirty = false; return; }
+    int dmax = 0;
+    for (int q : mem) dmax = std::max(dmax, (int)slots[q].mg->lev.size());
+    // current x buffer per (member, level)
+    std::vector<std::vector<T*>> cur(slots.size());
+    for (int q : mem) cur[q].assign(slots[q].mg->lev.size(), nullptr);
+    auto zbase = [&](int q) { return static_cast<T*>(slots[q].h->Z); };
+    auto smooth_item = [&](int q, int l, const T* xin, T* xout, T c1, T c2, T c0, bool final_out) {
+      auto& L = slots[q].mg->lev[l];
+      SmoothItem<T> it{};
+      it.L = L.view();
+      it.f = L.f;
+      it.dinv = L.dinv;
+      it.xin = xin;
+      it.xout = xout;
+      it.dv = L.dv;
+      it.out_base = final_out ? zbase(q) : nullptr;
+      it.out_stride = L.d.n();
+      it.c1 = c1;
+      it.c2 = c2;
+      it.c0 = c0;
+      it.slot = q;
+      it.tx = (uint32_t)(L.d.nx + 63) / 64;
+      it.ty = (uint32_t)(L.d.ny + 3) / 4;
+      return it;
+    };
+    // descent
+    for (int d = 0; d < dmax; d++) {
+      std::vector<int> sm, dn;
+      for (int q : mem) {
+        const int nl = (int)slots[q].mg->lev.size();
+        if (d < nl - 1) sm.push_back(q);
+        else if (d == nl - 1) dn.push_back(q);
+      }
+      if (!sm.empty()) {
+        // pre-smoothing from zero: FIRST2 (or ZERO for degree 1), then NEXT steps
+        int k = 0;
+        while (k < degree) {
+          const int mode = (k == 0) ? (degree >= 2 ? SM_FIRST2 : SM_ZERO) : SM_NEXT;
+          const int adv = (k == 0 && degree >= 2) ? 2 : 1;
+          const int last = k + adv - 1;
+          std::vector<SmoothItem<T>> items;
+          std::vector<uint32_t> nt;
+          for (int q : sm) {
+            auto& L = slots[q].mg->lev[d];
+            T* xin = (k == 0) ? L.xa : cur[q][d];
+            T* xout = (xin == L.xa) ? L.xb : L.xa;
+            const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
+            items.push_back(smooth_item(q, d, xin, xout, c1, L.c2[last], L.c2[0], false));
+            nt.push_back(tiles64x4(L.d));
+            cur[q][d] = xout;
+          }
+          add_launch(K_SMOOTH, mode, items, nt);
+          k += adv;
+        }
+        // residual + restriction to level d+1
+        std::vector<RestrictItem<T>> rb, rg;
+        std::vector<uint32_t> nb, ng;
+        for (int q : sm) {
+          auto& L = slots[q].mg->lev[d];
+          auto& C = slots[q].mg->lev[d + 1];
+          RestrictItem<T> it{};
+          it.F = L.view();
+          it.f = L.f;
+          it.x = cur[q][d];
+          it.dc = C.d;
+          it.clsc = C.cls;
+          it.fc = C.f;
+          it.cmask = (C.d.nx != L.d.nx ? 1 : 0) | (C.d.ny != L.d.ny ? 2 : 0) | (C.d.nz != L.d.nz ? 4 : 0);
+          it.slot = q;
+          if (it.cmask == 7) {
+            it.tx = (uint32_t)(L.d.nx + 63) / 64;
+            it.ty = (uint32_t)(L.d.ny + 1) / 2;
+            rb.push_back(it);
+            nb.push_back(it.tx * it.ty * (uint32_t)((L.d.nz + 1) / 2));
+          } else {
+            it.tx = (uint32_t)(C.d.nx + 63) / 64;
+            it.ty = (uint32_t)(C.d.ny + 3) / 4;
+            rg.push_back(it);
+            ng.push_back(tiles64x4(C.d));
+          }
+        }
+        add_launch(K_RBLK, 0, rb, nb);
+        add_launch(K_RGEN, 0, rg, ng);
+      }
+      if (!dn.empty()) {
+        std::vector<DenseItem<T>> items;
+        std::vector<uint32_t> nt;
+        for (int q : dn) {
+          auto& L = slots[q].mg->lev[d];
+          DenseItem<T> it{};
+          it.cinv = slots[q].mg->cinv;
+          it.f = L.f;
+          it.n = slots[q].mg->ncoarse;
+          it.slot = q;
+          if (d == 0) {
+            it.out_base = zbase(q);
+            it.out_stride = L.d.n();
+            it.x = nullptr;
+          } else {
+            it.x = L.xa;
+          }
+          cur[q][d] = L.xa;
+          items.push_back(it);
+          nt.push_back((uint32_t)(it.n + 63) / 64);
+        }
+        add_launch(K_DENSE, 0, items, nt);
+      }
+    }
+    // ascent
+    for (int d = dmax - 2; d >= 0; d--) {
+      std::vector<int> up;
+      for (int q : mem)
+        if (d < (int)slots[q].mg->lev.size() - 1) up.push_back(q);
+      if (up.empty()) continue;
+      std::vector<ProlongItem<T>> pi;
+      std::vector<uint32_t> pn;
+      for (int q : up) {
+        auto& L = slots[q].mg->lev[d];
+        auto& C = slots[q].mg->lev[d + 1];
+        ProlongItem<T> it{};
+        it.F = L.view();
+        it.dc = C.d;
+        it.clsc = C.cls;
+        it.xc = cur[q][d + 1];
+        it.xf = cur[q][d];
+        it.cmask = (C.d.nx != L.d.nx ? 1 : 0) | (C.d.ny != L.d.ny ? 2 : 0) | (C.d.nz != L.d.nz ? 4 : 0);
+        it.slot = q;
+        it.tx = (uint32_t)(L.d.nx + 63) / 64;
+        it.ty = (uint32_t)(L.d.ny + 3) / 4;
+        pi.push_back(it);
+        pn.push_back(tiles64x4(L.d));
+      }
+      add_launch(K_PROLONG, 0, pi, pn);
+      for (int k = 0; k < degree; k++) {
+        std::vector<SmoothItem<T>> items;
+        std::vector<uint32_t> nt;
+        for (int q : up) {
+          auto& L = slots[q].mg->lev[d];
+          T* xin = cur[q][d];
+          T* xout = (xin == L.xa) ? L.xb : L.xa;
+          const bool fin = (d == 0 && k == degree - 1);
+          items.push_back(smooth_item(q, d, xin, xout, k == 0 ? T(0) : L.c1[k], L.c2[k], T(0), fin));
+          nt.push_back(tiles64x4(L.d));
+          cur[q][d] = xout;
+        }
+        add_launch(K_SMOOTH, SM_NEXT, items, nt);
+      }
+    }
+    // Krylov passes
+    {
+      std::vector<ApplyItem<T>> ai;
+      std::vector<VecItem<T>> vi;
+      std::vector<uint32_t> an, vn;
+      std::vector<FinItem> fa, fv;
+      uint32_t ta = 0, tv = 0;
+      int mmax = 1;
+      for (int q : mem) {
+        vfs_pp_handle* h = slots[q].h;
+        const size_t n = h->d.n();
+        ApplyItem<T> a{};
+        a.d = h->d;
+        a.g = h->g64;
+        a.code = h->code;
+        a.Z = zbase(q);
+        a.V = h->V;
+        a.n = n;
+        a.slot = q;
+        a.tx = (uint32_t)(h->d.nx + 63) / 64;
+        a.ty = (uint32_t)(h->d.ny + 3) / 4;
+        const uint32_t na = tiles64x4(h->d);
+        fa.push_back(FinItem{h->red, h->hpin_dev, ta, ta + na, q});
+        ta += na;
+        ai.push_back(a);
+        an.push_back(na);
+        VecItem<T> v{};
+        v.V = h->V;
+        v.red = h->red;
+        v.hpin = h->hpin_dev;
+        v.f0 = static_cast<T*>(slots[q].mg->f0());
+        v.n = n;
+        v.slot = q;
+        const uint32_t nv = (uint32_t)((n + 255) / 256);
+        fv.push_back(FinItem{h->red, h->hpin_dev, tv, tv + nv, q});
+        tv += nv;
+        vi.push_back(v);
+        vn.push_back(nv);
+        mmax = std::max(mmax, h->mrest);
+      }
+      const size_t need = (size_t)(mmax + 2) * std::max(ta, tv);
+      partial = plan.alloc<double>(need);
+      partial_cap = need;
+      std::vector<uint32_t> sa(ai.size()), sv(vi.size());
+      uint32_t acc = 0;
+      for (size_t q = 0; q < ai.size(); q++) { sa[q] = acc; acc += an[q]; }
+      acc = 0;
+      for (size_t q = 0; q < vi.size(); q++) { sv[q] = acc; acc += vn[q]; }
+      auto pa = upload(ai, sa);
+      auto pv = upload(vi, sv);
+      apply = Launch{-1, 0, pa.first, pa.second, (int)ai.size(), ta};
+      axd = Launch{-1, 0, pv.first, pv.second, (int)vi.size(), tv};
+      axs = axd;
+      fin_apply = plan.alloc<FinItem>(fa.size());
+      fin_vec = plan.alloc<FinItem>(fv.size());
+      HIPCHECK(hipMemcpyAsync(fin_apply, fa.data(), fa.size() * sizeof(FinItem), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(fin_vec, fv.data(), fv.size() * sizeof(FinItem), hipMemcpyHostToDevice, s));
+      nfin = (int)fa.size();
+      fin_total_apply = ta;
+      fin_total_vec = tv;
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    dirty = false;
+  }
+
+  void launch_seq() {
+    for (const Launch& L : seq) {
+      const unsigned W = workers(L.total);
+      switch (L.kind) {
+        case K_SMOOTH: {
+          auto* it = static_cast<const SmoothItem<T>*>(L.items);
+          if (L.mode == SM_FIRST2)
+            kb_smooth<T, SM_FIRST2><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          else if (L.mode == SM_ZERO)
+            kb_smooth<T, SM_ZERO><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          else
+            kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          break;
+        }
+        case K_RBLK:
+          kb_restrict_blk<T><<<W, dim3(64, 2, 2), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
+                                                          L.nitems, L.total, sd);
+          break;
+        case K_RGEN:
+          kb_restrict_gen<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
+                                                          L.nitems, L.total, sd);
+          break;
+        case K_PROLONG:
+          kb_prolong<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const ProlongItem<T>*>(L.items), L.start, L.nitems,
+                                                     L.total, sd);
+          break;
+        case K_DENSE:
+          kb_dense<T><<<W, 256, 0, s>>>(static_cast<const DenseItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
+          break;
+      }
+      HIPCHECK(hipGetLastError());
+    }
+    // CGS2 in three passes over each case's basis
+    kb_apply_dots<T><<<workers(apply.total), dim3(64, 4, 1), 0, s>>>(
+        static_cast<const ApplyItem<T>*>(apply.items), apply.start, apply.nitems, apply.total, sd, partial);
+    HIPCHECK(hipGetLastError());
+    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 0, sd);
+    HIPCHECK(hipGetLastError());
+    kb_axpy_dots<T><<<workers(axd.total), 256, 0, s>>>(static_cast<const VecItem<T>*>(axd.items), axd.start,
+                                                       axd.nitems, axd.total, sd, partial);
+    HIPCHECK(hipGetLastError());
+    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_vec, partial, fin_total_vec, 72, 1, sd);
+    HIPCHECK(hipGetLastError());
+    kb_gs_norm<<<(nfin + 63) / 64, 64, 0, s>>>(fin_vec, nfin, sd);
+    HIPCHECK(hipGetLastError());
+    kb_axpy_scale<T><<<workers(axs.total), 256, 0, s>>>(static_cast<const VecItem<T>*>(axs.items), axs.start,
+                                                        axs.nitems, axs.total, sd);
+    HIPCHECK(hipGetLastError());
+  }
+
+  void run_iteration() {
+    if (dirty) build_plan();
+    // slot state for this iteration
+    for (size_t q = 0; q < slots.size(); q++) {
+      const Slot& sl = slots[q];
+      sd_host[q] = SlotDev{sl.j, sl.state == ITER ? 1 : 0};
+    }
+    HIPCHECK(hipMemcpyAsync(sd, sd_host, sizeof(SlotDev) * slots.size(), hipMemcpyHostToDevice, s));
+    if (use_graph) {
+      if (!gexec) {
+        HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_seq();
+        HIPCHECK(hipStreamEndCapture(s, &graph));
+        HIPCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      HIPCHECK(hipGraphLaunch(gexec, s));
+    } else {
+      launch_seq();
+    }
+  }
+
+  // ------------------------------------------------------------ per-case logic
+  void begin(int q, double* ucont, double* phi, double* p, double* ucat, double dt, double st, double coeff) {
+    Slot& sl = slots[q];
+    sl.ucont = ucont;
+    sl.phi = phi;
+    sl.p = p;
+    sl.ucat = ucat;
+    sl.dt = dt;
+    sl.st = st;
+    sl.coeff = coeff;
+    sl.S = vfs_pp_stats{};
+    sl.t0 = now_ms();
+    sl.j = sl.it = sl.vc = 0;
+    sl.conv = false;
+    sl.pending_update = false;
+    sl.m = sl.h->mrest;
+    sl.H.assign((size_t)(sl.m + 1) * sl.m, 0.);
+    sl.cs.assign(sl.m, 0.);
+    sl.sn.assign(sl.m, 0.);
+    sl.gv.assign(sl.m + 1, 0.);
+    sl.y.assign(sl.m, 0.);
+    solve_prologue(sl.h, ucont, dt, st, coeff);
+    sl.fresh = true;
+    sl.state = START;
+  }
+
+  // cases whose cycle starts: x update from the finished cycle, true residual
+  void start_cycles() {
+    std::vector<int> qs;
+    bool fresh = false;
+    for (int q = 0; q < (int)slots.size(); q++)
+      if (slots[q].state == START) {
+        qs.push_back(q);
+        fresh = fresh || slots[q].fresh;
+      }
+    if (qs.empty()) return;
+    if (fresh) {
+      // ||b|| and the VolumeFlux figures of new cases, then their initial guess
+      HIPCHECK(hipStreamSynchronize(s));
+      for (int q : qs) {
+        Slot& sl = slots[q];
+        if (!sl.fresh) continue;
+        solve_start(sl.h, sl.phi, &sl.S);
+        sl.tgt = sl.h->opts.rtol * sl.S.bnorm;
+        sl.fresh = false;
+      }
+    }
+    for (int q : qs) {
+      Slot& sl = slots[q];
+      vfs_pp_handle* h = sl.h;
+      if (sl.pending_update) {
+        for (int l = 0; l < sl.pending_nv; l++) h->hpin[200 + l] = sl.y[l];
+        HIPCHECK(hipMemcpyAsync(h->red + 200, h->hpin + 200, sl.pending_nv * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_update_x<T>(s, h->d.n(), h->x, static_cast<const T*>(h->Z), h->d.n(), sl.pending_nv, h->red + 200);
+        sl.pending_update = false;
+      }
+      int nb = 0;
+      launch_residual64<double>(s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
+      launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int q : qs) {
+      Slot& sl = slots[q];
+      vfs_pp_handle* h = sl.h;
+      if (sl.S.bnorm == 0.) { sl.conv = true; sl.rel = 0.; sl.state = FINISH; continue; }
+      const double beta = std::sqrt(h->hpin[0]);
+      if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
+      sl.rel = beta / sl.S.bnorm;
+      if (beta <= sl.tgt) { sl.conv = true; sl.state = FINISH; continue; }
+      if (sl.it >= h->opts.max_iters) { sl.state = FINISH; continue; }
+      launch_scale_conv<T>(s, h->d.n(), h->r, h->red, h->V, static_cast<T*>(sl.mg->f0()));
+      std::fill(sl.gv.begin(), sl.gv.end(), 0.);
+      sl.gv[0] = beta;
+      sl.j = 0;
+      sl.state = ITER;
+    }
+  }
+
+  // after an iteration: Arnoldi column, Givens rotation, cycle end
+  void after_iteration() {
+    for (int q = 0; q < (int)slots.size(); q++) {
+      Slot& sl = slots[q];
+      if (sl.state != ITER) continue;
+      vfs_pp_handle* h = sl.h;
+      const int m = sl.m, j = sl.j;
+      auto H = [&](int r, int c) -> double& { return sl.H[(size_t)r * m + c]; };
+      for (int l = 0; l <= j; l++) H(l, j) = h->hpin[8 + l] + h->hpin[72 + l];
+      H(j + 1, j) = h->hpin[1];
+      for (int l = 0; l < j; l++) {
+        const double t1 = sl.cs[l] * H(l, j) + sl.sn[l] * H(l + 1, j);
+        const double t2 = -sl.sn[l] * H(l, j) + sl.cs[l] * H(l + 1, j);
+        H(l, j) = t1;
+        H(l + 1, j) = t2;
+      }
+      const double a = H(j, j), bb = H(j + 1, j);
+      const double rr = std::sqrt(a * a + bb * bb);
+      sl.cs[j] = rr == 0. ? 1. : a / rr;
+      sl.sn[j] = rr == 0. ? 0. : bb / rr;
+      H(j, j) = rr;
+      H(j + 1, j) = 0.;
+      sl.gv[j + 1] = -sl.sn[j] * sl.gv[j];
+      sl.gv[j] = sl.cs[j] * sl.gv[j];
+      sl.it++;
+      sl.vc++;
+      if (!std::isfinite(sl.gv[j + 1])) throw StatusError(VFS_PP_ENONFINITE, "non-finite Krylov residual");
+      const bool est = std::fabs(sl.gv[j + 1]) <= sl.tgt;
+      if (est || j + 1 == m || sl.it >= h->opts.max_iters) {
+        const int nv = j + 1;
+        for (int l = nv - 1; l >= 0; l--) {
+          double acc = sl.gv[l];
+          for (int c = l + 1; c < nv; c++) acc -= H(l, c) * sl.y[c];
+          sl.y[l] = acc / H(l, l);
+        }
+        sl.pending_update = true;
+        sl.pending_nv = nv;
+        sl.state = START;
+      } else {
+        sl.j = j + 1;
+      }
+    }
+  }
+
+  // cases that are done iterating: null space, Phi, P, projection, divergence
+  void finish_cases(std::vector<int>* done) {
+    std::vector<int> qs;
+    for (int q = 0; q < (int)slots.size(); q++)
+      if (slots[q].state == FINISH) qs.push_back(q);
+    if (qs.empty()) return;
+    for (int q : qs) {
+      Slot& sl = slots[q];
+      solve_epilogue(sl.h, sl.phi, &sl.S);
+      update_pressure(sl.h, sl.p, sl.phi);
+      projection(sl.h, sl.ucont, sl.phi, sl.ucat, sl.dt, sl.st, sl.coeff);
+      divergence_launch(sl.h, sl.ucont);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int q : qs) {
+      Slot& sl = slots[q];
+      vfs_pp_handle* h = sl.h;
+      sl.S.iterations = sl.it;
+      sl.S.vcycles = sl.vc;
+      sl.S.rel_residual = sl.rel;
+      sl.S.phi_mean = h->n_lt > 0. ? h->hpin[RED_MEAN] / h->n_lt : 0.;
+      sl.S.max_div = h->hpin[RED_DIV];
+      bool ok = sl.conv;
+      if (h->opts.div_tol > 0. && !(sl.S.max_div <= h->opts.div_tol)) ok = false;
+      sl.S.converged = ok ? 1 : 0;
+      sl.S.ms_total = now_ms() - sl.t0;
+      sl.state = DONE;
+      if (done) done->push_back(q);
+    }
+  }
+
+  bool any(State st) const {
+    for (const auto& sl : slots)
+      if (sl.state == st) return true;
+    return false;
+  }
+
+  // one round: start cycles, one batched iteration, finish
+  void step(std::vector<int>* done) {
+    start_cycles();
+    finish_cases(done);
+    if (any(ITER)) {
+      run_iteration();
+      HIPCHECK(hipStreamSynchronize(s));
+      after_iteration();
+    }
+    start_cycles();
+    finish_cases(done);
+  }
+};
+
+}  // namespace pp
+
+using namespace pp;
+
+// ======================================================================= batch
+struct vfs_pp_batch {
+  int precision = 1;
+  std::vector<vfs_pp_handle*> members;
+  std::unique_ptr<BatchCore<float>> cf;
+  std::unique_ptr<BatchCore<double>> cd;
+};
+
+template <typename T>
+static int batch_project(BatchCore<T>& core, vfs_pp_batch* b, double* const* ucont, double* const* phi,
+                         double* const* p, double* const* ucat, const double* dt, const double* st,
+                         const double* coeff, vfs_pp_stats* stats) {
+  const int n = (int)b->members.size();
+  for (int q = 0; q < n; q++) core.begin(q, ucont[q], phi[q], p[q], ucat[q], dt[q], st[q], coeff[q]);
+  while (core.any(BatchCore<T>::ITER) || core.any(BatchCore<T>::START) || core.any(BatchCore<T>::FINISH))
+    core.step(nullptr);
+  bool ok = true;
+  for (int q = 0; q < n; q++) {
+    if (stats) stats[q] = core.slots[q].S;
+    if (!core.slots[q].S.converged) ok = false;
+  }
+  return ok ? VFS_PP_OK : VFS_PP_ENOCONV;
+}
+
+extern "C" {
+
+int vfs_pp_batch_create(vfs_pp_handle* const* members, int count, vfs_pp_batch** out) {
+  ABI_BEGIN
+  if (!members || count < 1 || !out) return VFS_PP_EINVAL;
+  auto b = std::make_unique<vfs_pp_batch>();
+  b->precision = members[0] ? members[0]->opts.precision : 1;
+  for (int q = 0; q < count; q++) {
+    if (!members[q]) return VFS_PP_EINVAL;
+    for (int r = 0; r < q; r++)
+      if (members[r] == members[q]) return VFS_PP_EINVAL;
+    if (members[q]->opts.precision != b->precision) return VFS_PP_EINVAL;
+    b->members.push_back(members[q]);
+  }
+  if (b->precision == 1) {
+    b->cf = std::make_unique<BatchCore<float>>(count);
+    for (int q = 0; q < count; q++) b->cf->bind(q, members[q]);
+  } else {
+    b->cd = std::make_unique<BatchCore<double>>(count);
+    for (int q = 0; q < count; q++) b->cd->bind(q, members[q]);
+  }
+  *out = b.release();
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_batch_destroy(vfs_pp_batch* b) {
+  if (!b) return VFS_PP_EINVAL;
+  delete b;
+  return VFS_PP_OK;
+}
+
+int vfs_pp_batch_set_graphs(vfs_pp_batch* b, int on) {
+  if (!b) return VFS_PP_EINVAL;
+  if (b->cf) { b->cf->use_graph = on != 0; b->cf->drop_graph(); }
+  if (b->cd) { b->cd->use_graph = on != 0; b->cd->drop_graph(); }
+  return VFS_PP_OK;
+}
+
+int vfs_pp_batch_project_device(vfs_pp_batch* b, double* const* ucont, double* const* phi, double* const* p,
+                                double* const* ucat, const double* dt, const double* st, const double* coeff,
+                                vfs_pp_stats* stats) {
+  ABI_BEGIN
+  if (!b || !ucont || !phi || !p || !ucat || !dt || !st || !coeff) return VFS_PP_EINVAL;
+  const int n = (int)b->members.size();
+  for (int q = 0; q < n; q++)
+    if (!ucont[q] || !phi[q] || !p[q] || !ucat[q] || !(dt[q] > 0.) || coeff[q] == 0.) return VFS_PP_EINVAL;
+  if (b->cf) return batch_project(*b->cf, b, ucont, phi, p, ucat, dt, st, coeff, stats);
+  return batch_project(*b->cd, b, ucont, phi, p, ucat, dt, st, coeff, stats);
+  ABI_END
+}
+
+}  // extern "C"

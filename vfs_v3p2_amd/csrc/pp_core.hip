@@ -7,6 +7,7 @@
 // touched by the kernels that read or write them in the reference.
 #include "pp_kernels.hpp"
 #include "pp_stencil.hpp"
+#include "pp_krylov_dev.hpp"
 
 namespace pp {
 
@@ -618,36 +619,6 @@ __global__ __launch_bounds__(256) void k_copy16(size_t n2, const double2* __rest
   if (t < n2) b[t] = a[t];
 }
 
-// Per-block sums of a * V_l[c] for l < nv and of a * a (slot nv), written to
-// partial[l * nblk + b].  Eight vectors at a time are loaded together, reduced
-// inside each wave by shuffles and parked in LDS; one barrier, then thread l
-// adds the four wave sums of slot l.  256-thread blocks, nv < 64.
-__device__ inline void block_dots(bool in, size_t c, double a, const double* __restrict__ V, size_t ldv, int nv,
-                                  double* __restrict__ partial, int nblk, int b, double (*lds)[64]) {
-  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-  const int lane = t & 63, w = t >> 6;
-  for (int l0 = 0; l0 <= nv; l0 += 8) {
-    double p[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int l = l0 + q;
-      double v = 0.;
-      if (in && l < nv) v = a * V[(size_t)l * ldv + c];
-      else if (in && l == nv) v = a * a;
-      p[q] = v;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q++) p[q] = wave_sum(p[q]);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        if (l0 + q <= nv) lds[w][l0 + q] = p[q];
-    }
-  }
-  __syncthreads();
-  if (t <= nv) partial[(size_t)t * nblk + b] = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
-}
-
 // w = A z (flux form) and per-block partial sums of w.V_l (l < nv) and w.w; the
 // first Krylov reduction of classical Gram-Schmidt fused into the operator pass.
 template <typename TZ>
@@ -659,22 +630,10 @@ __global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __rest
   CELL_IJK(i, j, k);
   const int nblk = gridDim.x * gridDim.y * gridDim.z, b = blin();
   const bool in = i < d.nx && j < d.ny;
-  size_t c = 0;
-  double wv = 0.;
-  if (in) {
-    c = d.idx(i, j, k);
-    const uint32_t cd = code[c];
-    Nbr nb;
-    neighbours(d, i, j, k, nb);
-    double xv[19], ax, dg;
-    load_x(nb, z, xv);
-    flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
-    wv = (cd & ROW_BIT) ? ax : 0.;
-    w[c] = wv;
-  }
+  const size_t c = in ? d.idx(i, j, k) : 0;
+  const double wv = in ? apply_cell(d, g, code, z, w, i, j, k) : 0.;
   block_dots(in, c, wv, V, ldv, nv, partial, nblk, b, lds);
 }
-
 
 // CGS2 second pass, fused: w1 = w - sum_l h_l V_l (in place) and per-block
 // partial sums of w1.V_l (l < nv) and w1.w1.  One element per thread; the second
@@ -684,28 +643,13 @@ __global__ __launch_bounds__(256) void k_axpy_dots(size_t n, double* __restrict_
                                                    double* __restrict__ partial) {
   __shared__ double lds[4][64];
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-  const int nblk = gridDim.x;
-  double w1 = 0.;
-  if (t < n) {
-    w1 = w[t];
-    for (int l = 0; l < nv; l++) w1 -= h[l] * V[(size_t)l * ldv + t];
-    w[t] = w1;
-  }
-  block_dots(t < n, t, w1, V, ldv, nv, partial, nblk, blockIdx.x, lds);
+  const double w1 = t < n ? gs_sub(w, V, ldv, nv, h, t) : 0.;
+  block_dots(t < n, t, w1, V, ldv, nv, partial, gridDim.x, blockIdx.x, lds);
 }
 
-// red[1] = ||w - V h|| = sqrt(w.w - h.h), red[2] = 1/red[1] with h = red[off..off+nv)
-// and w.w = red[off+nv]; mirrored to host
 __global__ void k_gs_norm(double* __restrict__ red, int off, int nv, double* __restrict__ host) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double* h = red + off;
-  double hh = 0.;
-  for (int l = 0; l < nv; l++) hh += h[l] * h[l];
-  const double s = h[nv] - hh;  // h[nv] holds w.w
-  const double nrm = s > 0. ? sqrt(s) : 0.;
-  red[1] = nrm;
-  red[2] = nrm > 0. ? 1. / nrm : 0.;
-  if (host) { host[1] = nrm; host[2] = red[2]; }
+  gs_norm_one(red, off, nv, host);
 }
 
 // v = (w - sum_l h_l V_l) * inv (in place over w); f = (TF)v for the next V-cycle
@@ -715,11 +659,7 @@ __global__ __launch_bounds__(256) void k_axpy_scale(size_t n, double* __restrict
                                                     const double* __restrict__ inv, TF* __restrict__ f) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  double acc = w[t];
-  for (int l = 0; l < nv; l++) acc -= h[l] * V[(size_t)l * ldv + t];
-  const double v = acc * inv[0];
-  w[t] = v;
-  if (f) f[t] = (TF)v;
+  axpy_scale_one(w, V, ldv, nv, h, inv[0], f, t);
 }
 
 // v = a / sqrt(norm2); f = (TF)v

@@ -612,12 +612,6 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
 }
 
 // ------------------------------------------------------------ pressure path
-// Scratch slots of red[] / hpin[] used outside the Krylov loop
-constexpr int RED_IBF = 300;   // VolumeFlux: 4 values
-constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
-constexpr int RED_B2 = 320;    // ||b||^2
-constexpr int RED_DIV = 330;   // max |aj div u|
-
 // VolumeFlux and RHS, launches only; ||b||^2 lands in hpin[RED_B2] once the
 // stream is synchronised.
 void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff) {
@@ -724,22 +718,6 @@ static double divergence(vfs_pp_handle* h, const double* ucont) {
 
 // ======================================================================= C ABI
 using namespace pp;
-
-#define ABI_BEGIN try {
-#define ABI_END                                   \
-  }                                               \
-  catch (const StatusError& e) {                  \
-    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
-    return e.code;                                \
-  }                                               \
-  catch (const HipError& e) {                     \
-    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
-    return VFS_PP_EHIP;                           \
-  }                                               \
-  catch (const std::exception& e) {               \
-    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
-    return VFS_PP_EINVAL;                         \
-  }
 
 extern "C" {
 

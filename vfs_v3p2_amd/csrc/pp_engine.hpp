@@ -223,6 +223,14 @@ struct vfs_pp_handle {
 };
 
 namespace pp {
+// Slots of red[] / hpin[]: Krylov scalars at 0..2 (norm^2, norm, 1/norm), the
+// two Gram-Schmidt coefficient sets at 8.. and 72.., the small solve's y at
+// 200..; outside the Krylov loop:
+constexpr int RED_IBF = 300;   // VolumeFlux: 4 values
+constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
+constexpr int RED_B2 = 320;    // ||b||^2
+constexpr int RED_DIV = 330;   // max |aj div u|
+
 // engine functions shared with the batched engine (pp_engine.hip)
 double now_ms();
 void ensure_geom(vfs_pp_handle* h);
@@ -235,3 +243,20 @@ void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat
                 double coeff);
 void divergence_launch(vfs_pp_handle* h, const double* ucont);
 }  // namespace pp
+
+// C-ABI wrappers: exceptions become status codes
+#define ABI_BEGIN try {
+#define ABI_END                                     \
+  }                                                 \
+  catch (const pp::StatusError& e) {                \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return e.code;                                  \
+  }                                                 \
+  catch (const pp::HipError& e) {                   \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return VFS_PP_EHIP;                             \
+  }                                                 \
+  catch (const std::exception& e) {                 \
+    std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
+    return VFS_PP_EINVAL;                           \
+  }

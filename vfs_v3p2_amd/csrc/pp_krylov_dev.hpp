@@ -1,0 +1,94 @@
+// Device bodies of the FGMRES basis kernels (classical Gram-Schmidt with
+// re-orthogonalisation, fused into three passes over the basis).  Shared by the
+// single-case kernels (pp_core.hip) and the batched kernels (pp_batch.hip).
+#pragma once
+
+#include "pp_common.hpp"
+#include "pp_stencil.hpp"
+
+namespace pp {
+
+// Per-block sums of a * V_l[c] for l < nv and of a * a (slot nv), written to
+// partial[l * nblk + b].  Eight vectors at a time are loaded together, reduced
+// inside each wave by shuffles and parked in LDS; one barrier, then thread l
+// adds the four wave sums of slot l.  256-thread blocks, nv < 64; every thread
+// of the block must call it.
+__device__ inline void block_dots(bool in, size_t c, double a, const double* __restrict__ V, size_t ldv, int nv,
+                                  double* __restrict__ partial, size_t nblk, size_t b, double (*lds)[64]) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int lane = t & 63, w = t >> 6;
+  for (int l0 = 0; l0 <= nv; l0 += 8) {
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int l = l0 + q;
+      double v = 0.;
+      if (in && l < nv) v = a * V[(size_t)l * ldv + c];
+      else if (in && l == nv) v = a * a;
+      p[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) p[q] = wave_sum(p[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (l0 + q <= nv) lds[w][l0 + q] = p[q];
+    }
+  }
+  __syncthreads();
+  if (t <= nv) partial[(size_t)t * nblk + b] = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
+  __syncthreads();  // lds is reused by the block's next tile
+}
+
+// w[c] = (A z)[c] in flux form on a row cell, 0 elsewhere; returns w[c]
+template <typename TZ>
+__device__ __forceinline__ double apply_cell(const Dims& d, const double* __restrict__ g,
+                                             const uint32_t* __restrict__ code, const TZ* __restrict__ z,
+                                             double* __restrict__ w, int i, int j, int k) {
+  const size_t c = d.idx(i, j, k);
+  const uint32_t cd = code[c];
+  Nbr nb;
+  neighbours(d, i, j, k, nb);
+  double xv[19], ax, dg;
+  load_x(nb, z, xv);
+  flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+  const double wv = (cd & ROW_BIT) ? ax : 0.;
+  w[c] = wv;
+  return wv;
+}
+
+// w1 = w - sum_l h_l V_l at element t (in place); returns w1
+__device__ __forceinline__ double gs_sub(double* __restrict__ w, const double* __restrict__ V, size_t ldv, int nv,
+                                         const double* __restrict__ h, size_t t) {
+  double w1 = w[t];
+  for (int l = 0; l < nv; l++) w1 -= h[l] * V[(size_t)l * ldv + t];
+  w[t] = w1;
+  return w1;
+}
+
+// red[1] = ||w - V h|| = sqrt(w.w - h.h), red[2] = 1/red[1] with h = red[off..off+nv)
+// and w.w = red[off+nv]; mirrored to host.  One thread.
+__device__ __forceinline__ void gs_norm_one(double* __restrict__ red, int off, int nv, double* __restrict__ host) {
+  const double* h = red + off;
+  double hh = 0.;
+  for (int l = 0; l < nv; l++) hh += h[l] * h[l];
+  const double s = h[nv] - hh;  // h[nv] holds w.w
+  const double nrm = s > 0. ? sqrt(s) : 0.;
+  red[1] = nrm;
+  red[2] = nrm > 0. ? 1. / nrm : 0.;
+  if (host) { host[1] = nrm; host[2] = red[2]; }
+}
+
+// v = (w - sum_l h_l V_l) * inv at element t (in place over w); f = (TF)v
+template <typename TF>
+__device__ __forceinline__ void axpy_scale_one(double* __restrict__ w, const double* __restrict__ V, size_t ldv,
+                                               int nv, const double* __restrict__ h, double inv, TF* __restrict__ f,
+                                               size_t t) {
+  double acc = w[t];
+  for (int l = 0; l < nv; l++) acc -= h[l] * V[(size_t)l * ldv + t];
+  const double v = acc * inv;
+  w[t] = v;
+  if (f) f[t] = (TF)v;
+}
+
+}  // namespace pp

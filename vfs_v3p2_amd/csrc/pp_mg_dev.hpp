@@ -1,0 +1,253 @@
+// Device bodies of the multigrid kernels: Chebyshev smoothers, fused residual +
+// restriction, IB-masked prolongation and the dense coarsest solve.  Each body
+// takes explicit cell (or tile) coordinates, so the single-case kernels
+// (pp_mg.hip) and the batched persistent kernels (pp_batch.hip) share the
+// arithmetic.
+//
+// The transfers follow the reference's MG primitives (MyRestriction
+// poisson.c:364-471, MyInterpolation poisson.c:473-669): restriction sums the
+// fluid children of a coarse cell (the coarse operator is in the same integrated
+// flux form as the fine one, so coarse residuals add up), zero on coarse IB
+// cells; prolongation uses the (27,9,9,3,9,3,3,1)/64 weights, one-sided next to
+// coarse IB cells and at non-periodic walls, periodic axes wrap.
+#pragma once
+
+#include "pp_kernels.hpp"
+#include "pp_stencil.hpp"
+
+namespace pp {
+
+// D^-1 of the level operator, 0 on cells that carry no unknown.
+template <typename T>
+__device__ __forceinline__ void dinv_cell(const LevelView<T>& L, T* __restrict__ dinv, int i, int j, int k) {
+  const size_t c = L.d.idx(i, j, k);
+  const uint32_t cd = L.code[c];
+  Nbr nb;
+  neighbours(L.d, i, j, k, nb);
+  T xv[19];
+#pragma unroll
+  for (int s = 0; s < 19; s++) xv[s] = T(0);
+  T ax, diag;
+  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+  dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
+}
+
+// Chebyshev smoothing preconditioned by the diagonal.  One step is
+//   z = D^-1 (f - A x_old);  d = c1 d + c2 z;  x_new = x_old + d
+// (c1 == 0 starts a new sequence and does not read d).  Modes:
+//   SM_NEXT    one step from x_old;
+//   SM_ZERO    the first step from x = 0:  x_new = d = c2 D^-1 f;
+//   SM_FIRST2  the first two steps from x = 0 in one pass: the first step's
+//              x1 = d1 = c0 D^-1 f is formed on the fly at the 19 neighbours from
+//              f and D^-1, then one step from x1 with (c1, c2).
+template <typename T, int MODE>
+__device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T* __restrict__ f,
+                                                  const T* __restrict__ dinv, const T* __restrict__ x_old,
+                                                  T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
+                                                  int j, int k) {
+  const size_t c = L.d.idx(i, j, k);
+  if constexpr (MODE == SM_ZERO) {
+    const T v = c2 * (f[c] * dinv[c]);
+    x_new[c] = v;
+    dv[c] = v;
+    return;
+  }
+  const uint32_t cd = L.code[c];
+  Nbr nb;
+  neighbours(L.d, i, j, k, nb);
+  T xv[19];
+  if constexpr (MODE == SM_FIRST2) {
+#pragma unroll
+    for (int s = 0; s < 19; s++) xv[s] = c0 * (f[nb.s[s]] * dinv[nb.s[s]]);
+  } else {
+    load_x(nb, x_old, xv);
+  }
+  T ax, diag;
+  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+  const T fc = f[c];
+  const T dold = (MODE == SM_FIRST2) ? xv[CP] : (c1 == T(0) ? T(0) : dv[c]);
+  if (!(cd & ROW_BIT)) { x_new[c] = T(0); dv[c] = T(0); return; }
+  const T z = diag != T(0) ? (fc - ax) / diag : T(0);
+  const T dn = c1 * dold + c2 * z;
+  dv[c] = dn;
+  x_new[c] = xv[CP] + dn;
+}
+
+template <typename T>
+__device__ __forceinline__ T row_times(const T (&vol)[19], const Nbr& nb, const T* __restrict__ x) {
+  T acc = T(0);
+#pragma unroll
+  for (int s = 0; s < 19; s++)
+    if (vol[s] != T(0)) acc += vol[s] * x[nb.s[s]];
+  return acc;
+}
+
+// One Chebyshev step preconditioned by the y-line tridiagonal part of A (block
+// Jacobi over (i,k) lines) for the line (i, k); Thomas algorithm with the forward
+// sweep's factors parked in scr (2n).
+template <typename T, bool ZG>
+__device__ __forceinline__ void smooth_line_col(const LevelView<T>& L, const T* __restrict__ f,
+                                                const T* __restrict__ x_old, T* __restrict__ x_new,
+                                                T* __restrict__ dv, T* __restrict__ scr, T c1, T c2, int i, int k) {
+  const size_t n = L.d.n();
+  T* __restrict__ cpv = scr;
+  T* __restrict__ dpv = scr + n;
+  T cprev = T(0), dprev = T(0);
+  for (int j = 0; j < L.d.ny; j++) {
+    const size_t c = L.d.idx(i, j, k);
+    const uint32_t cd = L.code[c];
+    T a = T(0), b = T(1), cc = T(0), r = T(0);
+    if (cd & ROW_BIT) {
+      Nbr nb;
+      neighbours(L.d, i, j, k, nb);
+      T vol[19];
+      build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+      r = f[c];
+      if (!ZG) r -= row_times(vol, nb, x_old);
+      a = vol[SP];
+      b = vol[CP];
+      cc = vol[NP];
+    }
+    const T den = b - a * cprev;
+    const T inv = den != T(0) ? T(1) / den : T(0);
+    cprev = cc * inv;
+    dprev = (r - a * dprev) * inv;
+    cpv[c] = cprev;
+    dpv[c] = dprev;
+  }
+  T znext = T(0);
+  for (int j = L.d.ny - 1; j >= 0; j--) {
+    const size_t c = L.d.idx(i, j, k);
+    const T z = dpv[c] - cpv[c] * znext;
+    znext = z;
+    const T dold = (ZG || c1 == T(0)) ? T(0) : dv[c];
+    const T dn = c1 * dold + c2 * z;
+    dv[c] = dn;
+    x_new[c] = (ZG ? T(0) : x_old[c]) + dn;
+  }
+}
+
+// fc[C] = sum over fluid children of (f - A x); zero on coarse IB cells.  Any
+// coarsening mask; one thread per coarse cell.
+template <typename T>
+__device__ __forceinline__ void resid_restrict_cell(const LevelView<T>& F, const T* __restrict__ f,
+                                                    const T* __restrict__ x, const Dims& dc,
+                                                    const uint8_t* __restrict__ clsc, int cmask, T* __restrict__ fc,
+                                                    int ic, int jc, int kc) {
+  const size_t C = dc.idx(ic, jc, kc);
+  if (clsc[C] & CLS_NONFLUID) { fc[C] = T(0); return; }
+  const int i0 = (cmask & 1) ? 2 * ic : ic, i1 = (cmask & 1) ? min(2 * ic + 1, F.d.nx - 1) : ic;
+  const int j0 = (cmask & 2) ? 2 * jc : jc, j1 = (cmask & 2) ? min(2 * jc + 1, F.d.ny - 1) : jc;
+  const int k0 = (cmask & 4) ? 2 * kc : kc, k1 = (cmask & 4) ? min(2 * kc + 1, F.d.nz - 1) : kc;
+  T sum = T(0);
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        const size_t c = F.d.idx(i, j, k);
+        const uint32_t cd = F.code[c];
+        if (!(cd & ROW_BIT)) continue;
+        Nbr nb;
+        neighbours(F.d, i, j, k, nb);
+        T xv[19], ax, dg;
+        load_x(nb, x, xv);
+        flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+        sum += f[c] - ax;
+      }
+  fc[C] = sum;
+}
+
+// Residual + restriction when all three axes are coarsened, for the 64x2x2 fine
+// tile (bx, by, bz) of a (64,2,2) block: one thread per fine cell, pair sums in i
+// by lane shuffle, the j and k pairs through LDS; the tile covers coarse cells
+// (32 bx .. 32 bx + 31, by, bz).  Every thread of the block must call it.
+template <typename T>
+__device__ __forceinline__ void resid_restrict_blk(const LevelView<T>& F, const T* __restrict__ f,
+                                                   const T* __restrict__ x, const Dims& dc,
+                                                   const uint8_t* __restrict__ clsc, T* __restrict__ fc, int bx,
+                                                   int by, int bz, T (*lds)[2][32]) {
+  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  const int i = bx * 64 + tx, j = by * 2 + ty, k = bz * 2 + tz;
+  T r = T(0);
+  if (i < F.d.nx && j < F.d.ny && k < F.d.nz) {
+    const size_t c = F.d.idx(i, j, k);
+    const uint32_t cd = F.code[c];
+    Nbr nb;
+    neighbours(F.d, i, j, k, nb);
+    T xv[19], ax, dg;
+    load_x(nb, x, xv);
+    flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+    if (cd & ROW_BIT) r = f[c] - ax;
+  }
+  r += __shfl_xor(r, 1, 64);
+  if ((tx & 1) == 0) lds[tz][ty][tx >> 1] = r;
+  __syncthreads();
+  if (ty == 0 && tz == 0 && tx < 32) {
+    const int ic = bx * 32 + tx, jc = by, kc = bz;
+    if (ic < dc.nx && jc < dc.ny && kc < dc.nz) {
+      const size_t C = dc.idx(ic, jc, kc);
+      const T v = (lds[0][0][tx] + lds[0][1][tx]) + (lds[1][0][tx] + lds[1][1][tx]);
+      fc[C] = (clsc[C] & CLS_NONFLUID) ? T(0) : v;
+    }
+  }
+  __syncthreads();  // the LDS tile is reused by the block's next tile
+}
+
+__device__ __forceinline__ int axis_pick(int f, int n, int per, bool coarsened, int& ic) {
+  if (!coarsened) { ic = f; return 0; }
+  ic = f >> 1;
+  int ia = (f & 1) ? 1 : -1;
+  if (!per && (f == 0 || f == n - 1)) ia = 0;
+  return ia;
+}
+__device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// xf += P xc at fine cell (i, j, k): trilinear (27,9,9,3,9,3,3,1)/64, one-sided
+// next to coarse IB cells in the reference's order k, j, i (poisson.c:500-505).
+template <typename T>
+__device__ __forceinline__ void prolong_add_cell(const LevelView<T>& F, const Dims& dc,
+                                                 const uint8_t* __restrict__ clsc, int cmask,
+                                                 const T* __restrict__ xc, T* __restrict__ xf, int i, int j, int k) {
+  const size_t c = F.d.idx(i, j, k);
+  if (F.cls[c] & CLS_NONFLUID) return;
+  int ic, jc, kc;
+  int ia = axis_pick(i, F.d.nx, F.d.per[0], cmask & 1, ic);
+  int ja = axis_pick(j, F.d.ny, F.d.per[1], cmask & 2, jc);
+  int ka = axis_pick(k, F.d.nz, F.d.per[2], cmask & 4, kc);
+  const int in = wrapc(ic + ia, dc.nx), jn = wrapc(jc + ja, dc.ny), kn = wrapc(kc + ka, dc.nz);
+  if (ka != 0 && (clsc[dc.idx(ic, jc, kn)] & CLS_NONFLUID)) ka = 0;
+  if (ja != 0 && (clsc[dc.idx(ic, jn, kc)] & CLS_NONFLUID)) ja = 0;
+  if (ia != 0 && (clsc[dc.idx(in, jc, kc)] & CLS_NONFLUID)) ia = 0;
+  const int i2 = ia ? in : ic, j2 = ja ? jn : jc, k2 = ka ? kn : kc;
+  const T v = (xc[dc.idx(ic, jc, kc)] * T(9) + xc[dc.idx(ic, j2, kc)] * T(3) + xc[dc.idx(i2, jc, kc)] * T(3) +
+               xc[dc.idx(i2, j2, kc)]) * T(3. / 64.) +
+              (xc[dc.idx(ic, jc, k2)] * T(9) + xc[dc.idx(ic, j2, k2)] * T(3) + xc[dc.idx(i2, jc, k2)] * T(3) +
+               xc[dc.idx(i2, j2, k2)]) * T(1. / 64.);
+  xf[c] += v;
+}
+
+// x = Cinv f for rows 64 rb .. 64 rb + 63, Cinv column-major n x n (n <= 1024):
+// four waves take interleaved quarters of the columns, partial sums meet in
+// LDS.  256-thread block; every thread must call it.
+template <typename T>
+__device__ __forceinline__ void dense_rows(int n, const T* __restrict__ cinv, const T* __restrict__ f,
+                                           T* __restrict__ x, int rb, T* fs, T (*part)[64]) {
+  for (int t = threadIdx.x; t < n && t < 1024; t += 256) fs[t] = f[t];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = rb * 64 + lane;
+  T a0 = T(0), a1 = T(0);
+  if (r < n) {
+    int c = w;
+    for (; c + 4 < n; c += 8) {
+      a0 += cinv[(size_t)c * n + r] * fs[c];
+      a1 += cinv[(size_t)(c + 4) * n + r] * fs[c + 4];
+    }
+    for (; c < n; c += 4) a0 += cinv[(size_t)c * n + r] * fs[c];
+  }
+  part[w][lane] = a0 + a1;
+  __syncthreads();
+  if (w == 0 && r < n) x[r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  __syncthreads();  // fs and part are reused by the block's next tile
+}
+
+}  // namespace pp

@@ -94,6 +94,11 @@ def lib():
         "vfs_pp_time_kernel": (C.c_int, [H, C.c_int, C.c_int, dp, dp]),
         "vfs_pp_describe": (C.c_int, [H, C.c_char_p, C.c_size_t]),
         "vfs_pp_error_string": (C.c_char_p, [C.c_int]),
+        "vfs_pp_batch_create": (C.c_int, [C.POINTER(H), C.c_int, C.POINTER(H)]),
+        "vfs_pp_batch_destroy": (C.c_int, [H]),
+        "vfs_pp_batch_set_graphs": (C.c_int, [H, C.c_int]),
+        "vfs_pp_batch_project_device": (C.c_int, [H] + [C.POINTER(C.c_void_p)] * 4 + [dp] * 3 +
+                                        [C.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -264,6 +269,48 @@ class Handle:
         ms, by = C.c_double(), C.c_double()
         _check(lib().vfs_pp_time_kernel(self._h, kernel, reps, C.byref(ms), C.byref(by)), "time kernel")
         return ms.value, by.value
+
+
+class Batch:
+    """Independent cases solved together (vfs_pp_batch): one work list per kernel
+    launch covers every member, one FGMRES iteration of all members is one HIP
+    graph launch.  Members are `Handle`s with equal precision and Chebyshev
+    degree and point smoothing; they must not be used alone while in a batch."""
+
+    def __init__(self, handles, graphs=True):
+        self.handles = list(handles)
+        arr = (C.c_void_p * len(self.handles))(*[h._h.value for h in self.handles])
+        self._b = C.c_void_p()
+        _check(lib().vfs_pp_batch_create(arr, len(self.handles), C.byref(self._b)), "vfs_pp_batch_create")
+        if not graphs:
+            _check(lib().vfs_pp_batch_set_graphs(self._b, 0), "vfs_pp_batch_set_graphs")
+
+    def close(self):
+        if self._b:
+            lib().vfs_pp_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def project(self, ucont, phi, p, ucat, dt, st, coeff, strict=False):
+        """Per-member lists of device tensors and scalars; returns a list of Stats."""
+        n = len(self.handles)
+        ptrs = []
+        for name, arrs, vec in (("ucont", ucont, True), ("phi", phi, False), ("p", p, False), ("ucat", ucat, True)):
+            if len(arrs) != n:
+                raise ValueError(f"{name}: expected {n} arrays")
+            ptrs.append((C.c_void_p * n)(*[_tptr(a, dmda_shape(h.grid, vec), f"{name}[{q}]").value
+                                           for q, (a, h) in enumerate(zip(arrs, self.handles))]))
+        scal = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,))) for v in (dt, st, coeff)]
+        stats = (Stats * n)()
+        rc = lib().vfs_pp_batch_project_device(self._b, *ptrs, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in scal],
+                                               stats)
+        _check(rc, "batch project", allow=() if strict else (VFS_PP_ENOCONV,))
+        return list(stats)
 
 
 # ----------------------------------------------------------- reference-named API
