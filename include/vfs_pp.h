@@ -180,9 +180,9 @@ int vfs_pp_mg_nvert_restrict_device(const vfs_grid *fine, const double *nvert_f,
  * solve covers all member cases through one work list (persistent workers,
  * XCD-contiguous tile order), and one FGMRES iteration of all members is one
  * HIP graph launch.  Members keep their own geometry, hierarchy and Krylov
- * storage; they must share opts.precision and opts.cheb_degree and use point
- * smoothing (opts.smoother = 1).  A member must not be used on its own while
- * the batch exists. */
+ * storage (and their own point or y-line smoother per level); they must share
+ * opts.precision and opts.cheb_degree.  A member must not be used on its own
+ * while the batch exists. */
 typedef struct vfs_pp_batch vfs_pp_batch;
 int vfs_pp_batch_create(vfs_pp_handle *const *members, int count, vfs_pp_batch **out);
 int vfs_pp_batch_destroy(vfs_pp_batch *b);

@@ -37,8 +37,7 @@ def E(torch_cuda):
 def handles(E, cs, **kw):
     out = []
     for c, _ in cs:
-        h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed),
-                     E.default_opts(smoother=1, **kw))
+        h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), E.default_opts(**kw))
         h.set_geometry(c.fields)
         out.append(h)
     return out
@@ -52,19 +51,22 @@ def fields(torch, cs):
     return u, phi, p, ucat
 
 
-@pytest.mark.parametrize("precision", [1, 0])
-def test_batch_equals_alone_and_oracle(E, torch_cuda, precision):
+@pytest.mark.parametrize("precision,smoother", [(1, 1), (0, 1), (1, 2), (0, 0)])
+def test_batch_equals_alone_and_oracle(E, torch_cuda, precision, smoother):
+    """smoother 1: point Chebyshev on every level; 2: y-line on every level; 0:
+    chosen per level (y-line where the wall-normal coupling dominates)."""
     torch = torch_cuda
     cs = get_cases()
-    print(f"batch of {len(cs)} cases, precision {precision}", flush=True)
+    kw = dict(precision=precision, rtol=1e-12, smoother=smoother)
+    print(f"batch of {len(cs)} cases, {kw}", flush=True)
     # alone
     alone = []
-    for h, (c, A) in zip(handles(E, cs, precision=precision, rtol=1e-12), cs):
+    for h, (c, A) in zip(handles(E, cs, **kw), cs):
         u, phi, p, ucat = fields(torch, [(c, A)])
         st = h.project(u[0], phi[0], p[0], ucat[0], c.dt, c.st, c.coeff, strict=True)
         alone.append((st, phi[0].cpu().numpy(), u[0].cpu().numpy()))
         h.close()
-    hs = handles(E, cs, precision=precision, rtol=1e-12)
+    hs = handles(E, cs, **kw)
     B = E.Batch(hs)
     runs = []
     for call in range(2):
@@ -75,7 +77,7 @@ def test_batch_equals_alone_and_oracle(E, torch_cuda, precision):
                      zip(sts, phi, u, ucat)])
     B.close()
     # plain launches instead of graph replay
-    hs2 = handles(E, cs, precision=precision, rtol=1e-12)
+    hs2 = handles(E, cs, **kw)
     B2 = E.Batch(hs2, graphs=False)
     u, phi, p, ucat = fields(torch, cs)
     sts2 = B2.project(u, phi, p, ucat, [c.dt for c, _ in cs], [c.st for c, _ in cs], [c.coeff for c, _ in cs],

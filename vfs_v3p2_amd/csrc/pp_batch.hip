@@ -62,9 +62,11 @@ struct SmoothItem {
   T* dv;
   T* out_base;  // non-null: the output is out_base + j * out_stride (the case's Z_j)
   size_t out_stride;
+  const T* lfac;  // y-line levels: Thomas factors; scr holds the residual and the forward sweep
+  T* scr;
   T c1, c2, c0;
   int slot;
-  uint32_t tx, ty;  // 64 x 4 tiles of the level
+  uint32_t tx, ty;  // point/residual: 64 x 4 cell tiles; line solve: 64 (i) x 4 (k) line tiles
 };
 
 template <typename T>
@@ -148,6 +150,44 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
     if (i >= I.L.d.nx || j >= I.L.d.ny) continue;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
     smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
+  }
+}
+
+// r = f - A x into the item's scratch (y-line levels)
+template <typename T>
+__global__ __launch_bounds__(256) void kb_resid(const SmoothItem<T>* __restrict__ items,
+                                                const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const SmoothItem<T>& I = items[q];
+    if (!sd[I.slot].y) continue;
+    const uint32_t lt = t - start[q];
+    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
+    const int k = (int)(lt / (I.tx * I.ty));
+    if (i >= I.L.d.nx || j >= I.L.d.ny) continue;
+    resid_cell(I.L, I.f, I.xin, I.scr, i, j, k);
+  }
+}
+
+// one Chebyshev step with the y-line preconditioner, one thread per (i, k) line
+template <typename T, bool ZG>
+__global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__ items,
+                                               const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                               const SlotDev* __restrict__ sd) {
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const SmoothItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const uint32_t lt = t - start[q];
+    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
+    const int k = (int)(lt / I.tx) * 4 + (int)threadIdx.y;
+    if (i >= I.L.d.nx || k >= I.L.d.nz) continue;
+    T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
+    const size_t n = I.L.d.n();
+    line_solve_col<T, ZG>(I.L.d, ZG ? I.f : I.scr, I.lfac, I.scr + n, I.xin, xout, I.dv, I.c1, I.c2, i, k);
   }
 }
 
@@ -364,7 +404,7 @@ struct BatchCore {
   FinItem* fin_apply = nullptr;
   FinItem* fin_vec = nullptr;
 
-  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE };
+  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE };
 
   explicit BatchCore(int nslots) {
     HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -394,8 +434,6 @@ struct BatchCore {
     ensure_geom(h);
     auto* mg = dynamic_cast<MG<T>*>(h->mg.get());
     if (!mg) throw StatusError(VFS_PP_EINVAL, "batch members must share the multigrid precision");
-    for (auto& L : mg->lev)
-      if (L.line) throw StatusError(VFS_PP_EINVAL, "batch members must use point smoothing (opts.smoother = 1)");
     if (degree == 0) degree = mg->degree;
     if (mg->degree != degree) throw StatusError(VFS_PP_EINVAL, "batch members must share cheb_degree");
     sl.h = h;
@@ -432,6 +470,74 @@ struct BatchCore {
   }
 
   static uint32_t tiles64x4(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
+  static uint32_t line_tiles(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.nz + 3) / 4); }
+
+  // Launches of one smoothing step k (0-based) at one depth for a group of
+  // members: point levels fuse the first two steps from a zero start; y-line
+  // levels run a residual launch and a line-solve launch per step (the zero-start
+  // step needs no residual).  cur[] tracks each member's current x buffer.
+  void smoothing_step(int d, int k, bool zero_start, bool final_level0, const std::vector<int>& grp,
+                      std::vector<std::vector<T*>>& cur) {
+    std::vector<SmoothItem<T>> pt, rs, lz, lnx;
+    std::vector<uint32_t> npt, nrs, nlz, nlnx;
+    int pmode = -1;
+    for (int q : grp) {
+      auto& L = slots[q].mg->lev[d];
+      const bool line = L.lfac != nullptr;
+      int mode, last = k;
+      if (zero_start && k == 0) {
+        mode = (!line && degree >= 2) ? SM_FIRST2 : SM_ZERO;
+        if (mode == SM_FIRST2) last = 1;
+      } else if (zero_start && k == 1 && !line && degree >= 2) {
+        continue;  // done by the fused first launch
+      } else {
+        mode = SM_NEXT;
+      }
+      T* xin = (zero_start && k == 0) ? L.xa : cur[q][d];
+      T* xout = (xin == L.xa) ? L.xb : L.xa;
+      const bool fin = final_level0 && d == 0 && last == degree - 1;
+      const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
+      SmoothItem<T> it{};
+      it.L = L.view();
+      it.f = L.f;
+      it.dinv = L.dinv;
+      it.xin = xin;
+      it.xout = xout;
+      it.dv = L.dv;
+      it.out_base = fin ? static_cast<T*>(slots[q].h->Z) : nullptr;
+      it.out_stride = L.d.n();
+      it.lfac = L.lfac;
+      it.scr = slots[q].mg->scr;
+      it.c1 = c1;
+      it.c2 = L.c2[last];
+      it.c0 = L.c2[0];
+      it.slot = q;
+      cur[q][d] = xout;
+      if (!line) {
+        it.tx = (uint32_t)(L.d.nx + 63) / 64;
+        it.ty = (uint32_t)(L.d.ny + 3) / 4;
+        pmode = mode;
+        pt.push_back(it);
+        npt.push_back(tiles64x4(L.d));
+      } else {
+        if (mode != SM_ZERO) {
+          SmoothItem<T> r = it;
+          r.tx = (uint32_t)(L.d.nx + 63) / 64;
+          r.ty = (uint32_t)(L.d.ny + 3) / 4;
+          rs.push_back(r);
+          nrs.push_back(tiles64x4(L.d));
+        }
+        it.tx = (uint32_t)(L.d.nx + 63) / 64;
+        it.ty = 0;
+        if (mode == SM_ZERO) { lz.push_back(it); nlz.push_back(line_tiles(L.d)); }
+        else { lnx.push_back(it); nlnx.push_back(line_tiles(L.d)); }
+      }
+    }
+    add_launch(K_SMOOTH, pmode, pt, npt);
+    add_launch(K_RESID, 0, rs, nrs);
+    add_launch(K_LINE, SM_ZERO, lz, nlz);
+    add_launch(K_LINE, SM_NEXT, lnx, nlnx);
+  }
 
   void build_plan() {
     HIPCHECK(hipStreamSynchronize(s));
@@ -448,25 +554,6 @@ struct BatchCore {
     std::vector<std::vector<T*>> cur(slots.size());
     for (int q : mem) cur[q].assign(slots[q].mg->lev.size(), nullptr);
     auto zbase = [&](int q) { return static_cast<T*>(slots[q].h->Z); };
-    auto smooth_item = [&](int q, int l, const T* xin, T* xout, T c1, T c2, T c0, bool final_out) {
-      auto& L = slots[q].mg->lev[l];
-      SmoothItem<T> it{};
-      it.L = L.view();
-      it.f = L.f;
-      it.dinv = L.dinv;
-      it.xin = xin;
-      it.xout = xout;
-      it.dv = L.dv;
-      it.out_base = final_out ? zbase(q) : nullptr;
-      it.out_stride = L.d.n();
-      it.c1 = c1;
-      it.c2 = c2;
-      it.c0 = c0;
-      it.slot = q;
-      it.tx = (uint32_t)(L.d.nx + 63) / 64;
-      it.ty = (uint32_t)(L.d.ny + 3) / 4;
-      return it;
-    };
     // descent
     for (int d = 0; d < dmax; d++) {
       std::vector<int> sm, dn;
@@ -476,26 +563,7 @@ struct BatchCore {
         else if (d == nl - 1) dn.push_back(q);
       }
       if (!sm.empty()) {
-        // pre-smoothing from zero: FIRST2 (or ZERO for degree 1), then NEXT steps
-        int k = 0;
-        while (k < degree) {
-          const int mode = (k == 0) ? (degree >= 2 ? SM_FIRST2 : SM_ZERO) : SM_NEXT;
-          const int adv = (k == 0 && degree >= 2) ? 2 : 1;
-          const int last = k + adv - 1;
-          std::vector<SmoothItem<T>> items;
-          std::vector<uint32_t> nt;
-          for (int q : sm) {
-            auto& L = slots[q].mg->lev[d];
-            T* xin = (k == 0) ? L.xa : cur[q][d];
-            T* xout = (xin == L.xa) ? L.xb : L.xa;
-            const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
-            items.push_back(smooth_item(q, d, xin, xout, c1, L.c2[last], L.c2[0], false));
-            nt.push_back(tiles64x4(L.d));
-            cur[q][d] = xout;
-          }
-          add_launch(K_SMOOTH, mode, items, nt);
-          k += adv;
-        }
+        for (int k = 0; k < degree; k++) smoothing_step(d, k, true, false, sm, cur);
         // residual + restriction to level d+1
         std::vector<RestrictItem<T>> rb, rg;
         std::vector<uint32_t> nb, ng;
@@ -575,20 +643,7 @@ struct BatchCore {
         pn.push_back(tiles64x4(L.d));
       }
       add_launch(K_PROLONG, 0, pi, pn);
-      for (int k = 0; k < degree; k++) {
-        std::vector<SmoothItem<T>> items;
-        std::vector<uint32_t> nt;
-        for (int q : up) {
-          auto& L = slots[q].mg->lev[d];
-          T* xin = cur[q][d];
-          T* xout = (xin == L.xa) ? L.xb : L.xa;
-          const bool fin = (d == 0 && k == degree - 1);
-          items.push_back(smooth_item(q, d, xin, xout, k == 0 ? T(0) : L.c1[k], L.c2[k], T(0), fin));
-          nt.push_back(tiles64x4(L.d));
-          cur[q][d] = xout;
-        }
-        add_launch(K_SMOOTH, SM_NEXT, items, nt);
-      }
+      for (int k = 0; k < degree; k++) smoothing_step(d, k, false, true, up, cur);
     }
     // Krylov passes
     {
@@ -669,6 +724,18 @@ struct BatchCore {
             kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
           break;
         }
+        case K_RESID:
+          kb_resid<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start, L.nitems,
+                                                   L.total, sd);
+          break;
+        case K_LINE:
+          if (L.mode == SM_ZERO)
+            kb_line<T, true><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
+                                                          L.nitems, L.total, sd);
+          else
+            kb_line<T, false><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
+                                                           L.nitems, L.total, sd);
+          break;
         case K_RBLK:
           kb_restrict_blk<T><<<W, dim3(64, 2, 2), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
                                                           L.nitems, L.total, sd);

@@ -253,7 +253,7 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
   launch_scale_by<T>(h->s, n, L.xa, h->red);
   double lam = 0., best = 0.;
   for (int it = 0; it < 12; it++) {
-    launch_smooth<T>(h->s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, mg.scr, T(0), T(1), T(0));
+    launch_smooth<T>(h->s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, mg.scr, T(0), T(1), T(0));
     launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
     launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
     HIPCHECK(hipStreamSynchronize(h->s));
@@ -329,6 +329,10 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     }
     if (H.d.per[1]) line = 0;
     L.line = line;
+    if (line) {
+      L.lfac = h->ar.alloc<T>(3 * n);
+      launch_line_factors<T>(h->s, L.view(), L.lfac);
+    }
     mg->lev.push_back(L);
   }
   // Chebyshev parameters per smoothed level

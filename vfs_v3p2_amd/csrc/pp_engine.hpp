@@ -95,6 +95,7 @@ struct MG : MGBase {
     const uint32_t* code = nullptr;
     const uint8_t* cls = nullptr;
     T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr, *dinv = nullptr;
+    T* lfac = nullptr;  // y-line Thomas factors [a | cp | inv] when line smoothing
     int line = 0;
     std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
     double lam = 0.;
@@ -141,7 +142,7 @@ struct MG : MGBase {
       T* xout = (x == L.xa) ? L.xb : L.xa;
       if (last == degree - 1 && final_out) xout = final_out;
       const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
-      launch_smooth<T>(s, L.view(), L.line, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
+      launch_smooth<T>(s, L.view(), L.lfac, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
       x = xout;
       k += adv;
     }
@@ -180,7 +181,7 @@ struct MG : MGBase {
 
   void time_sweep(hipStream_t s) override {
     Lev& L = lev[0];
-    launch_smooth<T>(s, L.view(), L.line, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
+    launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
                      L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
   }
 };

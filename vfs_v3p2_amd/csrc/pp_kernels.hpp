@@ -126,8 +126,10 @@ struct LevelView {
 enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2 };
 
 template <typename T>
-void launch_smooth(hipStream_t s, LevelView<T> L, int line, int mode, const T* f, const T* dinv, const T* x_old,
-                   T* x_new, T* dv, T* scr, T c1, T c2, T c0);
+void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const T* f, const T* dinv,
+                   const T* x_old, T* x_new, T* dv, T* scr, T c1, T c2, T c0);
+template <typename T>
+void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac);
 template <typename T>
 void launch_dinv(hipStream_t s, LevelView<T> L, T* dinv);
 template <typename T>

@@ -35,13 +35,28 @@ __global__ __launch_bounds__(256) void k_smooth_point(LevelView<T> L, const T* _
   smooth_point_cell<T, MODE>(L, f, dinv, x_old, x_new, dv, c1, c2, c0, i, j, k);
 }
 
-template <typename T, bool ZG>
-__global__ __launch_bounds__(256) void k_smooth_line(LevelView<T> L, const T* __restrict__ f,
-                                                     const T* __restrict__ x_old, T* __restrict__ x_new,
-                                                     T* __restrict__ dv, T* __restrict__ scr, T c1, T c2) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_line_factors(LevelView<T> L, T* __restrict__ lfac) {
   const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y * 4 + threadIdx.y;
   if (i >= L.d.nx || k >= L.d.nz) return;
-  smooth_line_col<T, ZG>(L, f, x_old, x_new, dv, scr, c1, c2, i, k);
+  line_factors_col(L, lfac, i, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_resid(LevelView<T> L, const T* __restrict__ f, const T* __restrict__ x,
+                                               T* __restrict__ r) {
+  CELL_IJK(i, j, k);
+  if (i >= L.d.nx || j >= L.d.ny) return;
+  resid_cell(L, f, x, r, i, j, k);
+}
+
+template <typename T, bool ZG>
+__global__ __launch_bounds__(256) void k_line_solve(Dims d, const T* __restrict__ r, const T* __restrict__ lfac,
+                                                    T* __restrict__ tmp, const T* __restrict__ x_old,
+                                                    T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2) {
+  const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx || k >= d.nz) return;
+  line_solve_col<T, ZG>(d, r, lfac, tmp, x_old, x_new, dv, c1, c2, i, k);
 }
 
 template <typename T>
@@ -189,12 +204,19 @@ inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 template <typename T>
-void launch_smooth(hipStream_t s, LevelView<T> L, int line, int mode, const T* f, const T* dinv, const T* x_old,
-                   T* x_new, T* dv, T* scr, T c1, T c2, T c0) {
-  if (line) {
+void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const T* f, const T* dinv,
+                   const T* x_old, T* x_new, T* dv, T* scr, T c1, T c2, T c0) {
+  if (lfac) {
+    // y-line smoothing: residual over all cells, then one thread per line
+    const size_t n = L.d.n();
     if (mode == SM_FIRST2) throw StatusError(1, "the line smoother has no fused first-two-steps mode");
-    if (mode == SM_ZERO) k_smooth_line<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, scr, c1, c2);
-    else k_smooth_line<T, false><<<line_grid(L.d), kBlk, 0, s>>>(L, f, x_old, x_new, dv, scr, c1, c2);
+    if (mode == SM_ZERO) {
+      k_line_solve<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L.d, f, lfac, scr + n, x_old, x_new, dv, c1, c2);
+    } else {
+      k_resid<T><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, x_old, scr);
+      HIPCHECK(hipGetLastError());
+      k_line_solve<T, false><<<line_grid(L.d), kBlk, 0, s>>>(L.d, scr, lfac, scr + n, x_old, x_new, dv, c1, c2);
+    }
   } else {
     switch (mode) {
       case SM_ZERO:
@@ -207,6 +229,12 @@ void launch_smooth(hipStream_t s, LevelView<T> L, int line, int mode, const T* f
         k_smooth_point<T, SM_NEXT><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_new, dv, c1, c2, c0);
     }
   }
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac) {
+  k_line_factors<T><<<line_grid(L.d), kBlk, 0, s>>>(L, lfac);
   HIPCHECK(hipGetLastError());
 }
 
@@ -258,8 +286,9 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
 }
 
 #define INST(T)                                                                                                     \
-  template void launch_smooth<T>(hipStream_t, LevelView<T>, int, int, const T*, const T*, const T*, T*, T*, T*, T, T, \
-                                 T);                                                                              \
+  template void launch_smooth<T>(hipStream_t, LevelView<T>, const T*, int, const T*, const T*, const T*, T*, T*, T*, \
+                                 T, T, T);                                                                        \
+  template void launch_line_factors<T>(hipStream_t, LevelView<T>, T*);                                            \
   template void launch_dinv<T>(hipStream_t, LevelView<T>, T*);                                                     \
   template void launch_resid_restrict<T>(hipStream_t, LevelView<T>, const T*, const T*, Dims, const uint8_t*, T*); \
   template void launch_prolong_add<T>(hipStream_t, LevelView<T>, Dims, const uint8_t*, const T*, T*);              \

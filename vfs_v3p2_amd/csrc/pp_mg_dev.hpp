@@ -73,37 +73,23 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
   x_new[c] = xv[CP] + dn;
 }
 
+// Thomas factors of the y-line tridiagonal part of A on the line (i, k), from the
+// rows' SP / CP / NP coefficients (cells without an unknown get the identity):
+// lfac = [a | cp | inv], each n long; den_j = b_j - a_j cp_{j-1}, inv_j = 1/den_j,
+// cp_j = c_j inv_j.
 template <typename T>
-__device__ __forceinline__ T row_times(const T (&vol)[19], const Nbr& nb, const T* __restrict__ x) {
-  T acc = T(0);
-#pragma unroll
-  for (int s = 0; s < 19; s++)
-    if (vol[s] != T(0)) acc += vol[s] * x[nb.s[s]];
-  return acc;
-}
-
-// One Chebyshev step preconditioned by the y-line tridiagonal part of A (block
-// Jacobi over (i,k) lines) for the line (i, k); Thomas algorithm with the forward
-// sweep's factors parked in scr (2n).
-template <typename T, bool ZG>
-__device__ __forceinline__ void smooth_line_col(const LevelView<T>& L, const T* __restrict__ f,
-                                                const T* __restrict__ x_old, T* __restrict__ x_new,
-                                                T* __restrict__ dv, T* __restrict__ scr, T c1, T c2, int i, int k) {
+__device__ __forceinline__ void line_factors_col(const LevelView<T>& L, T* __restrict__ lfac, int i, int k) {
   const size_t n = L.d.n();
-  T* __restrict__ cpv = scr;
-  T* __restrict__ dpv = scr + n;
-  T cprev = T(0), dprev = T(0);
+  T cprev = T(0);
   for (int j = 0; j < L.d.ny; j++) {
     const size_t c = L.d.idx(i, j, k);
     const uint32_t cd = L.code[c];
-    T a = T(0), b = T(1), cc = T(0), r = T(0);
+    T a = T(0), b = T(1), cc = T(0);
     if (cd & ROW_BIT) {
       Nbr nb;
       neighbours(L.d, i, j, k, nb);
       T vol[19];
       build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
-      r = f[c];
-      if (!ZG) r -= row_times(vol, nb, x_old);
       a = vol[SP];
       b = vol[CP];
       cc = vol[NP];
@@ -111,14 +97,52 @@ __device__ __forceinline__ void smooth_line_col(const LevelView<T>& L, const T* 
     const T den = b - a * cprev;
     const T inv = den != T(0) ? T(1) / den : T(0);
     cprev = cc * inv;
-    dprev = (r - a * dprev) * inv;
-    cpv[c] = cprev;
-    dpv[c] = dprev;
+    lfac[c] = a;
+    lfac[n + c] = cprev;
+    lfac[2 * n + c] = inv;
+  }
+}
+
+// r = f - A x on cells with an unknown, 0 elsewhere
+template <typename T>
+__device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __restrict__ f, const T* __restrict__ x,
+                                           T* __restrict__ r, int i, int j, int k) {
+  const size_t c = L.d.idx(i, j, k);
+  const uint32_t cd = L.code[c];
+  Nbr nb;
+  neighbours(L.d, i, j, k, nb);
+  T xv[19], ax, dg;
+  load_x(nb, x, xv);
+  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+  r[c] = (cd & ROW_BIT) ? f[c] - ax : T(0);
+}
+
+// One Chebyshev step preconditioned by the y-line tridiagonal part T of A on the
+// line (i, k): z = T^-1 r with the precomputed factors (forward sweep parked in
+// tmp), d = c1 d + c2 z, x_new = x_old + d.  ZG: from x = 0 with r = f.
+template <typename T, bool ZG>
+__device__ __forceinline__ void line_solve_col(const Dims& d, const T* __restrict__ r, const T* __restrict__ lfac,
+                                               T* __restrict__ tmp, const T* __restrict__ x_old,
+                                               T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2, int i,
+                                               int k) {
+  const size_t n = d.n();
+  const T* __restrict__ la = lfac;
+  const T* __restrict__ lcp = lfac + n;
+  const T* __restrict__ linv = lfac + 2 * n;
+  const size_t sy = (size_t)d.nx;
+  const size_t c0 = d.idx(i, 0, k);
+  T dprev = T(0);
+#pragma unroll 4
+  for (int j = 0; j < d.ny; j++) {
+    const size_t c = c0 + (size_t)j * sy;
+    dprev = (r[c] - la[c] * dprev) * linv[c];
+    tmp[c] = dprev;
   }
   T znext = T(0);
-  for (int j = L.d.ny - 1; j >= 0; j--) {
-    const size_t c = L.d.idx(i, j, k);
-    const T z = dpv[c] - cpv[c] * znext;
+#pragma unroll 4
+  for (int j = d.ny - 1; j >= 0; j--) {
+    const size_t c = c0 + (size_t)j * sy;
+    const T z = tmp[c] - lcp[c] * znext;
     znext = z;
     const T dold = (ZG || c1 == T(0)) ? T(0) : dv[c];
     const T dn = c1 * dold + c2 * z;

@@ -275,7 +275,7 @@ class Batch:
     """Independent cases solved together (vfs_pp_batch): one work list per kernel
     launch covers every member, one FGMRES iteration of all members is one HIP
     graph launch.  Members are `Handle`s with equal precision and Chebyshev
-    degree and point smoothing; they must not be used alone while in a batch."""
+    degree; they must not be used alone while in a batch."""
 
     def __init__(self, handles, graphs=True):
         self.handles = list(handles)
