@@ -21,6 +21,8 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
+#include <tuple>
 
 #include "pp_engine.hpp"
 #include "pp_krylov_dev.hpp"
@@ -347,9 +349,29 @@ __global__ void kb_gs_norm(const FinItem* __restrict__ items, int nitems, const 
   gs_norm_one(I.red, 72, st.x + 1, I.hpin);
 }
 
-inline unsigned workers(uint32_t total) {
-  const uint32_t w = (total + 7u) & ~7u;
-  return w < 2048u ? (w < 8u ? 8u : w) : 2048u;
+// workers for a launch of `total` tiles: a multiple of 8, at most `cap` (0: one
+// tile per worker)
+inline unsigned workers(uint32_t total, unsigned cap) {
+  uint32_t w = (total + 7u) & ~7u;
+  if (w < 8u) w = 8u;
+  if (cap && w > cap) w = cap;
+  return w;
+}
+
+// Worker counts chosen by measurement, per (launch kind and precision, mode,
+// log2 of the tile count); shared by every batch and server of the process.
+struct TuneCache {
+  std::mutex mu;
+  std::map<std::tuple<int, int, int>, unsigned> best;
+};
+TuneCache& tune_cache() {
+  static TuneCache c;
+  return c;
+}
+int log2_bucket(uint32_t x) {
+  int b = 0;
+  while (x > 1) { x >>= 1; b++; }
+  return b;
 }
 
 }  // namespace
@@ -395,6 +417,7 @@ struct BatchCore {
     const uint32_t* start;
     int nitems;
     uint32_t total;
+    unsigned cap = 2048;  // worker count limit (0: one tile per worker), set by tune()
   };
   std::vector<Launch> seq;  // V-cycle + Krylov passes of one iteration
   Launch apply{}, axd{}, axs{};
@@ -404,7 +427,8 @@ struct BatchCore {
   FinItem* fin_apply = nullptr;
   FinItem* fin_vec = nullptr;
 
-  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE };
+  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS };
+  bool tuned_once = false;
 
   explicit BatchCore(int nslots) {
     HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -466,7 +490,7 @@ struct BatchCore {
     for (size_t q = 0; q < items.size(); q++) { start[q] = tot; tot += ntiles[q]; }
     if (tot == 0) return;
     auto pr = upload(items, start);
-    seq.push_back(Launch{kind, mode, pr.first, pr.second, (int)items.size(), tot});
+    seq.push_back(Launch{kind, mode, pr.first, pr.second, (int)items.size(), tot, 2048});
   }
 
   static uint32_t tiles64x4(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
@@ -695,9 +719,9 @@ struct BatchCore {
       for (size_t q = 0; q < vi.size(); q++) { sv[q] = acc; acc += vn[q]; }
       auto pa = upload(ai, sa);
       auto pv = upload(vi, sv);
-      apply = Launch{-1, 0, pa.first, pa.second, (int)ai.size(), ta};
-      axd = Launch{-1, 0, pv.first, pv.second, (int)vi.size(), tv};
-      axs = axd;
+      apply = Launch{K_APPLY, 0, pa.first, pa.second, (int)ai.size(), ta, 2048};
+      axd = Launch{K_AXD, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
+      axs = Launch{K_AXS, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
       fin_apply = plan.alloc<FinItem>(fa.size());
       fin_vec = plan.alloc<FinItem>(fv.size());
       HIPCHECK(hipMemcpyAsync(fin_apply, fa.data(), fa.size() * sizeof(FinItem), hipMemcpyHostToDevice, s));
@@ -710,70 +734,139 @@ struct BatchCore {
     dirty = false;
   }
 
-  void launch_seq() {
-    for (const Launch& L : seq) {
-      const unsigned W = workers(L.total);
-      switch (L.kind) {
-        case K_SMOOTH: {
-          auto* it = static_cast<const SmoothItem<T>*>(L.items);
-          if (L.mode == SM_FIRST2)
-            kb_smooth<T, SM_FIRST2><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
-          else if (L.mode == SM_ZERO)
-            kb_smooth<T, SM_ZERO><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
-          else
-            kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
-          break;
-        }
-        case K_RESID:
-          kb_resid<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start, L.nitems,
-                                                   L.total, sd);
-          break;
-        case K_LINE:
-          if (L.mode == SM_ZERO)
-            kb_line<T, true><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
-                                                          L.nitems, L.total, sd);
-          else
-            kb_line<T, false><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
-                                                           L.nitems, L.total, sd);
-          break;
-        case K_RBLK:
-          kb_restrict_blk<T><<<W, dim3(64, 2, 2), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
-                                                          L.nitems, L.total, sd);
-          break;
-        case K_RGEN:
-          kb_restrict_gen<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
-                                                          L.nitems, L.total, sd);
-          break;
-        case K_PROLONG:
-          kb_prolong<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const ProlongItem<T>*>(L.items), L.start, L.nitems,
-                                                     L.total, sd);
-          break;
-        case K_DENSE:
-          kb_dense<T><<<W, 256, 0, s>>>(static_cast<const DenseItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
-          break;
+  void launch_one(const Launch& L) {
+    const unsigned W = workers(L.total, L.cap);
+    switch (L.kind) {
+      case K_SMOOTH: {
+        auto* it = static_cast<const SmoothItem<T>*>(L.items);
+        if (L.mode == SM_FIRST2)
+          kb_smooth<T, SM_FIRST2><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+        else if (L.mode == SM_ZERO)
+          kb_smooth<T, SM_ZERO><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+        else
+          kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+        break;
       }
-      HIPCHECK(hipGetLastError());
+      case K_RESID:
+        kb_resid<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start, L.nitems,
+                                                 L.total, sd);
+        break;
+      case K_LINE:
+        if (L.mode == SM_ZERO)
+          kb_line<T, true><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
+                                                        L.nitems, L.total, sd);
+        else
+          kb_line<T, false><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
+                                                         L.nitems, L.total, sd);
+        break;
+      case K_RBLK:
+        kb_restrict_blk<T><<<W, dim3(64, 2, 2), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
+                                                        L.nitems, L.total, sd);
+        break;
+      case K_RGEN:
+        kb_restrict_gen<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
+                                                        L.nitems, L.total, sd);
+        break;
+      case K_PROLONG:
+        kb_prolong<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const ProlongItem<T>*>(L.items), L.start, L.nitems,
+                                                   L.total, sd);
+        break;
+      case K_DENSE:
+        kb_dense<T><<<W, 256, 0, s>>>(static_cast<const DenseItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
+        break;
+      case K_APPLY:
+        kb_apply_dots<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const ApplyItem<T>*>(L.items), L.start, L.nitems,
+                                                      L.total, sd, partial);
+        break;
+      case K_AXD:
+        kb_axpy_dots<T><<<W, 256, 0, s>>>(static_cast<const VecItem<T>*>(L.items), L.start, L.nitems, L.total, sd,
+                                          partial);
+        break;
+      case K_AXS:
+        kb_axpy_scale<T><<<W, 256, 0, s>>>(static_cast<const VecItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
+        break;
     }
-    // CGS2 in three passes over each case's basis
-    kb_apply_dots<T><<<workers(apply.total), dim3(64, 4, 1), 0, s>>>(
-        static_cast<const ApplyItem<T>*>(apply.items), apply.start, apply.nitems, apply.total, sd, partial);
     HIPCHECK(hipGetLastError());
+  }
+
+  void launch_seq() {
+    for (const Launch& L : seq) launch_one(L);
+    // CGS2 in three passes over each case's basis
+    launch_one(apply);
     kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 0, sd);
     HIPCHECK(hipGetLastError());
-    kb_axpy_dots<T><<<workers(axd.total), 256, 0, s>>>(static_cast<const VecItem<T>*>(axd.items), axd.start,
-                                                       axd.nitems, axd.total, sd, partial);
-    HIPCHECK(hipGetLastError());
+    launch_one(axd);
     kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_vec, partial, fin_total_vec, 72, 1, sd);
     HIPCHECK(hipGetLastError());
     kb_gs_norm<<<(nfin + 63) / 64, 64, 0, s>>>(fin_vec, nfin, sd);
     HIPCHECK(hipGetLastError());
-    kb_axpy_scale<T><<<workers(axs.total), 256, 0, s>>>(static_cast<const VecItem<T>*>(axs.items), axs.start,
-                                                        axs.nitems, axs.total, sd);
-    HIPCHECK(hipGetLastError());
+    launch_one(axs);
+  }
+
+  // Autotuner: each launch's worker count is picked by timing its candidates
+  // (HIP events, best of 3 after a warm-up) and remembered per (kind, mode,
+  // size bucket).  The timed launches run on the cases' scratch state, so this
+  // only happens while no case is in the middle of an iteration.
+  void tune(bool allowed) {
+    std::vector<Launch*> all;
+    for (Launch& L : seq) all.push_back(&L);
+    all.push_back(&apply);
+    all.push_back(&axd);
+    all.push_back(&axs);
+    auto& tc = tune_cache();
+    std::lock_guard<std::mutex> lk(tc.mu);
+    bool need = false;
+    for (Launch* L : all) {
+      auto it = tc.best.find(std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, log2_bucket(L->total)));
+      if (it != tc.best.end()) L->cap = it->second;
+      else need = true;
+    }
+    if (!need || !allowed) return;
+    // every slot active at step 0 while timing
+    for (size_t q = 0; q < slots.size(); q++) sd_host[q] = SlotDev{0, slots[q].h ? 1 : 0};
+    HIPCHECK(hipMemcpyAsync(sd, sd_host, sizeof(SlotDev) * slots.size(), hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    const unsigned cands[] = {0u, 16384u, 8192u, 4096u, 2048u, 1024u};
+    for (Launch* L : all) {
+      const auto key = std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, log2_bucket(L->total));
+      auto it = tc.best.find(key);
+      if (it != tc.best.end()) { L->cap = it->second; continue; }
+      float best_ms = 1e30f;
+      unsigned best = 2048u;
+      unsigned last_w = 0;
+      for (unsigned cap : cands) {
+        const unsigned w = workers(L->total, cap);
+        if (w == last_w) continue;  // same grid as the previous candidate
+        last_w = w;
+        L->cap = cap;
+        launch_one(*L);
+        float tmin = 1e30f;
+        for (int r = 0; r < 3; r++) {
+          HIPCHECK(hipEventRecord(e0, s));
+          launch_one(*L);
+          HIPCHECK(hipEventRecord(e1, s));
+          HIPCHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+          tmin = std::min(tmin, ms);
+        }
+        if (tmin < best_ms) { best_ms = tmin; best = cap; }
+      }
+      L->cap = best;
+      tc.best[key] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHECK(hipStreamSynchronize(s));
   }
 
   void run_iteration() {
-    if (dirty) build_plan();
+    if (dirty) {
+      build_plan();
+      tune(false);
+    }
     // slot state for this iteration
     for (size_t q = 0; q < slots.size(); q++) {
       const Slot& sl = slots[q];
@@ -954,6 +1047,11 @@ struct BatchCore {
 
   // one round: start cycles, one batched iteration, finish
   void step(std::vector<int>* done) {
+    if (dirty) {
+      // (re)build before any cycle starts; tune only if no case is mid-iteration
+      build_plan();
+      tune(!any(ITER));
+    }
     start_cycles();
     finish_cases(done);
     if (any(ITER)) {
