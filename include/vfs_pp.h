@@ -195,6 +195,44 @@ int vfs_pp_batch_project_device(vfs_pp_batch *b, double *const *ucont, double *c
                                 double *const *p, double *const *ucat, const double *dt,
                                 const double *st, const double *coeff, vfs_pp_stats *stats);
 
+/* ---- served form: continuous batching of projection requests.
+ *
+ * No counterpart in the reference.  Cases (grids with their metrics, masks and
+ * multigrid hierarchy) are registered once and stay resident.  Requests
+ * (one projection of a registered case on caller device arrays) queue FIFO;
+ * each vfs_pp_server_progress call admits queued requests into free slots
+ * (at most max_active at once, one per case at a time) when the Krylov
+ * workspace of the case fits the HBM budget, runs one batched FGMRES
+ * iteration of everything in flight (as vfs_pp_batch), and reports the
+ * requests that finished.  Results are the ones vfs_pp_project_device gives. */
+typedef struct vfs_pp_server vfs_pp_server;
+typedef struct {
+  size_t hbm_budget;   /* device bytes for resident cases + workspaces; 0: 80% of free memory */
+  int max_active;      /* requests in flight at once */
+  vfs_pp_opts solver;  /* options of every case (precision and cheb_degree shared) */
+} vfs_pp_server_opts;
+void vfs_pp_server_default_opts(vfs_pp_server_opts *opts);
+int vfs_pp_server_create(const vfs_pp_server_opts *opts, vfs_pp_server **out);
+int vfs_pp_server_destroy(vfs_pp_server *sv);
+/* geometry arrays as vfs_pp_set_geometry; VFS_PP_ENOMEM if the case and one
+ * workspace would not fit the budget */
+int vfs_pp_server_add_case(vfs_pp_server *sv, const vfs_grid *grid, const double *icsi,
+                           const double *ieta, const double *izet, const double *jcsi,
+                           const double *jeta, const double *jzet, const double *kcsi,
+                           const double *keta, const double *kzet, const double *iaj,
+                           const double *jaj, const double *kaj, const double *aj,
+                           const double *csi, const double *eta, const double *zet,
+                           const double *nvert, int *case_id);
+int vfs_pp_server_remove_case(vfs_pp_server *sv, int case_id);
+/* queue one projection; the device arrays must stay valid until it finishes */
+int vfs_pp_server_submit(vfs_pp_server *sv, int case_id, double *ucont, double *phi, double *p,
+                         double *ucat, double dt, double st, double coeff, int *ticket);
+/* admit, iterate once, retire: *ndone finished tickets (and their stats) */
+int vfs_pp_server_progress(vfs_pp_server *sv, int *tickets, vfs_pp_stats *stats, int cap,
+                           int *ndone);
+int vfs_pp_server_status(vfs_pp_server *sv, int *queued, int *active, size_t *resident_bytes,
+                         size_t *workspace_bytes);
+
 /* Time `reps` launches of one engine kernel on the handle's stream with HIP
  * events; kernel ids: 0 FP64 operator apply, 1 finest-level smoother sweep (in
  * the multigrid precision), 2 projection, 3 Contra2Cart, 4 RHS, 5 a plain

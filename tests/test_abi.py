@@ -63,8 +63,9 @@ PROBE = r"""
 #include "vfs_pp.h"
 #define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
 int main(void) {
-  printf("vfs_grid %zu\nvfs_pp_opts %zu\nvfs_pp_stats %zu\n", sizeof(vfs_grid), sizeof(vfs_pp_opts),
-         sizeof(vfs_pp_stats));
+  printf("vfs_grid %zu\nvfs_pp_opts %zu\nvfs_pp_stats %zu\nvfs_pp_server_opts %zu\n", sizeof(vfs_grid),
+         sizeof(vfs_pp_opts), sizeof(vfs_pp_stats), sizeof(vfs_pp_server_opts));
+  F(vfs_pp_server_opts, max_active) F(vfs_pp_server_opts, solver)
   F(vfs_grid, periodic) F(vfs_grid, bctype) F(vfs_grid, immersed)
   F(vfs_pp_opts, rtol) F(vfs_pp_opts, div_tol) F(vfs_pp_opts, max_iters) F(vfs_pp_opts, restart)
   F(vfs_pp_opts, smoother) F(vfs_pp_opts, cheb_degree) F(vfs_pp_opts, coarse_max) F(vfs_pp_opts, drop_small)
@@ -87,7 +88,7 @@ def test_struct_layouts_match_header(tmp_path):
     subprocess.run(["gcc", "-std=c11", f"-I{os.path.dirname(HEADER)}", str(src), "-o", str(exe)], check=True)
     lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
     got = dict(l.rsplit(" ", 1) for l in lines if l)
-    cls = {"vfs_grid": E.Grid, "vfs_pp_opts": E.Opts, "vfs_pp_stats": E.Stats}
+    cls = {"vfs_grid": E.Grid, "vfs_pp_opts": E.Opts, "vfs_pp_stats": E.Stats, "vfs_pp_server_opts": E.ServerOpts}
     for k, v in got.items():
         if "." in k:
             s, m = k.split(".")

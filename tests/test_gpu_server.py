@@ -1,0 +1,130 @@
+"""Served form: requests for resident cases, admitted into a continuous batch
+as slots and HBM free up, must each get the result they get alone."""
+import numpy as np
+import pytest
+
+from vfs_v3p2_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [((20, 16, 24), 1.5, 0.2, False, 21), ((32, 24, 16), 2.5, 0.35, True, 22),
+         ((16, 16, 16), 1.0, 0.0, False, 23), ((24, 20, 28), 3.0, 0.1, True, 24), ((28, 18, 20), 2.0, 0.3, True, 25)]
+_cases = []
+
+
+def get_cases():
+    if not _cases:
+        _cases.extend(cases.ensemble_case(*sp) for sp in SPECS)
+    return _cases
+
+
+def relL2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def E(torch_cuda):
+    from vfs_v3p2_amd import engine
+
+    engine.lib()
+    return engine
+
+
+def requests(torch, cs, nreq, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(nreq):
+        q = int(rng.integers(len(cs)))
+        c = cs[q]
+        u = np.ascontiguousarray(c.fields["ucont"]) * (1.0 + 0.05 * rng.standard_normal(c.fields["ucont"].shape))
+        cases.set_periodic_flux_ghosts(u, c.per)
+        out.append((q, u))
+    return out
+
+
+def solve_alone(E, torch, c, u, opts):
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), E.default_opts(**opts))
+    h.set_geometry(c.fields)
+    ud = torch.from_numpy(u.copy()).cuda()
+    phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+    p = torch.zeros_like(phi)
+    ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+    st = h.project(ud, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+    h.close()
+    return st, phi.cpu().numpy(), ud.cpu().numpy()
+
+
+def run_server(E, torch, cs, reqs, opts, max_active, budget=0, per_round=2):
+    sv = E.Server(max_active=max_active, hbm_budget=budget, **opts)
+    ids = [sv.add_case(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), c.fields) for c in cs]
+    res, arrays, tick2req = {}, {}, {}
+    pending = list(enumerate(reqs))
+    max_queued = 0
+    rounds = 0
+    while pending or len(res) < len(reqs):
+        # requests keep arriving while earlier ones are on the GPU
+        for _ in range(per_round):
+            if not pending:
+                break
+            r, (q, u) = pending.pop(0)
+            c = cs[q]
+            ud = torch.from_numpy(u.copy()).cuda()
+            phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+            p = torch.zeros_like(phi)
+            ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+            t = sv.submit(ids[q], ud, phi, p, ucat, c.dt, c.st, c.coeff)
+            tick2req[t] = r
+            arrays[r] = (ud, phi)
+        max_queued = max(max_queued, sv.status()["queued"])
+        for t, st in sv.progress():
+            res[tick2req[t]] = st
+        rounds += 1
+        assert rounds < 10000
+    st = sv.status()
+    sv.close()
+    return res, {r: (a[1].cpu().numpy(), a[0].cpu().numpy()) for r, a in arrays.items()}, max_queued, st
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_served_requests_equal_alone(E, torch_cuda, precision):
+    torch = torch_cuda
+    cs = get_cases()
+    opts = dict(precision=precision, rtol=1e-12)
+    reqs = requests(torch, cs, 12, seed=precision)
+    print(f"12 requests over {len(cs)} resident cases, 3 slots, precision {precision}", flush=True)
+    res, out, max_queued, st = run_server(E, torch, cs, reqs, opts, max_active=3)
+    assert max_queued > 0  # requests did wait for slots
+    assert st["active"] == 0 and st["queued"] == 0
+    for r, (q, u) in enumerate(reqs):
+        s = res[r]
+        assert s.converged and s.max_div <= 1e-8
+        st_a, phi_a, u_a = solve_alone(E, torch, cs[q], u, opts)
+        phi, uo = out[r]
+        assert relL2(phi, phi_a) <= 1e-9, (r, q)
+        assert relL2(uo, u_a) <= 1e-12, (r, q)
+        assert abs(s.iterations - st_a.iterations) <= 1
+
+
+def test_server_memory_admission(E, torch_cuda):
+    torch = torch_cuda
+    cs = get_cases()[:3]
+    opts = dict(precision=1, rtol=1e-10)
+    # a budget that holds the resident cases and about one workspace: requests run one at a time
+    probe = E.Server(max_active=4, **opts)
+    for c in cs:
+        probe.add_case(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), c.fields)
+    resident = probe.status()["resident_bytes"]
+    probe.close()
+    n = max(c.nx * c.ny * c.nz for c in cs)
+    ws = (30 + 1) * n * 8 + 30 * n * 4
+    reqs = requests(torch, cs, 6, seed=7)
+    res, out, max_queued, st = run_server(E, torch, cs, reqs, opts, max_active=4, budget=resident + int(1.5 * ws),
+                                          per_round=6)
+    assert len(res) == 6 and all(s.converged for s in res.values())
+    assert st["workspace_bytes"] <= int(1.5 * ws)
+    assert max_queued >= 4
+    with pytest.raises(E.VfsPPError) as ei:
+        small = E.Server(max_active=2, hbm_budget=1 << 20, **opts)
+        c = cs[0]
+        small.add_case(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), c.fields)
+    assert ei.value.code == E.VFS_PP_ENOMEM

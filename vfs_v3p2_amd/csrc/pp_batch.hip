@@ -1145,3 +1145,233 @@ int vfs_pp_batch_project_device(vfs_pp_batch* b, double* const* ucont, double* c
 }
 
 }  // extern "C"
+
+// ====================================================================== server
+// Continuous batching: requests queue per resident case; between batched
+// iterations the server admits queued requests into free slots as long as a
+// Krylov workspace for the case fits the HBM budget (resident cases plus the
+// workspace pool), and retires finished ones, whose workspaces go back to the
+// pool for the next request of the same size.
+struct vfs_pp_server {
+  struct Case {
+    vfs_pp_handle* h = nullptr;
+    size_t ws = 0;
+    bool busy = false;
+  };
+  struct Req {
+    int ticket, cid;
+    double *ucont, *phi, *p, *ucat;
+    double dt, st, coeff;
+  };
+  vfs_pp_server_opts o{};
+  std::map<int, Case> cases;
+  std::deque<Req> queue;
+  std::vector<int> slot_ticket, slot_case;
+  std::vector<void*> slot_ws;
+  std::multimap<size_t, void*> pool;  // free workspaces by size
+  size_t resident = 0, ws_held = 0, ws_max = 0;
+  int next_case = 0, next_ticket = 0;
+  std::unique_ptr<BatchCore<float>> cf;
+  std::unique_ptr<BatchCore<double>> cd;
+
+  ~vfs_pp_server() {
+    cf.reset();
+    cd.reset();
+    for (auto& kv : pool) (void)hipFree(kv.second);
+    for (void* w : slot_ws)
+      if (w) (void)hipFree(w);
+    for (auto& kv : cases) vfs_pp_destroy(kv.second.h);
+  }
+
+  void* take_ws(size_t bytes) {
+    auto it = pool.find(bytes);
+    if (it != pool.end()) {
+      void* w = it->second;
+      pool.erase(it);
+      return w;
+    }
+    // make room by releasing pooled workspaces of other sizes
+    while (resident + ws_held + bytes > o.hbm_budget && !pool.empty()) {
+      auto big = std::prev(pool.end());
+      ws_held -= big->first;
+      (void)hipFree(big->second);
+      pool.erase(big);
+    }
+    if (resident + ws_held + bytes > o.hbm_budget) return nullptr;
+    void* w = nullptr;
+    if (hipMalloc(&w, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    ws_held += bytes;
+    return w;
+  }
+
+  template <typename T>
+  int progress(BatchCore<T>& core, int* tickets, vfs_pp_stats* stats, int cap) {
+    // admit: FIFO, skipping requests whose case is already in flight
+    for (auto it = queue.begin(); it != queue.end();) {
+      int q = -1;
+      for (int r = 0; r < (int)slot_ticket.size(); r++)
+        if (slot_ticket[r] < 0) { q = r; break; }
+      if (q < 0) break;
+      Case& c = cases.at(it->cid);
+      if (c.busy) { ++it; continue; }
+      void* w = take_ws(c.ws);
+      if (!w) break;  // wait for memory to free up
+      vfs_pp_handle* h = c.h;
+      h->V = static_cast<double*>(w);
+      h->Z = static_cast<char*>(w) + (size_t)(h->mrest + 1) * h->d.n() * sizeof(double);
+      core.bind(q, h);
+      core.begin(q, it->ucont, it->phi, it->p, it->ucat, it->dt, it->st, it->coeff);
+      c.busy = true;
+      slot_ticket[q] = it->ticket;
+      slot_case[q] = it->cid;
+      slot_ws[q] = w;
+      it = queue.erase(it);
+    }
+    std::vector<int> done;
+    if (core.any(BatchCore<T>::START) || core.any(BatchCore<T>::ITER) || core.any(BatchCore<T>::FINISH))
+      core.step(&done);
+    int nd = 0;
+    for (int q : done) {
+      if (nd < cap) {
+        if (tickets) tickets[nd] = slot_ticket[q];
+        if (stats) stats[nd] = core.slots[q].S;
+      }
+      nd++;
+      Case& c = cases.at(slot_case[q]);
+      c.busy = false;
+      core.release(core.slots[q]);
+      c.h->V = nullptr;
+      c.h->Z = nullptr;
+      pool.emplace(c.ws, slot_ws[q]);
+      slot_ws[q] = nullptr;
+      slot_ticket[q] = -1;
+      slot_case[q] = -1;
+    }
+    if (nd > cap) throw StatusError(VFS_PP_EINVAL, "more finished requests than the output arrays hold");
+    return nd;
+  }
+};
+
+extern "C" {
+
+void vfs_pp_server_default_opts(vfs_pp_server_opts* o) {
+  if (!o) return;
+  o->hbm_budget = 0;
+  o->max_active = 64;
+  vfs_pp_default_opts(&o->solver);
+}
+
+int vfs_pp_server_create(const vfs_pp_server_opts* o, vfs_pp_server** out) {
+  ABI_BEGIN
+  if (!out) return VFS_PP_EINVAL;
+  auto sv = std::make_unique<vfs_pp_server>();
+  if (o) sv->o = *o;
+  else vfs_pp_server_default_opts(&sv->o);
+  if (sv->o.max_active < 1 || sv->o.max_active > 4096) return VFS_PP_EINVAL;
+  if (sv->o.hbm_budget == 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    sv->o.hbm_budget = fr / 10 * 8;  // 80% of what is free now
+  }
+  const int n = sv->o.max_active;
+  sv->slot_ticket.assign(n, -1);
+  sv->slot_case.assign(n, -1);
+  sv->slot_ws.assign(n, nullptr);
+  if (sv->o.solver.precision == 1) sv->cf = std::make_unique<BatchCore<float>>(n);
+  else sv->cd = std::make_unique<BatchCore<double>>(n);
+  *out = sv.release();
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_server_destroy(vfs_pp_server* sv) {
+  if (!sv) return VFS_PP_EINVAL;
+  delete sv;
+  return VFS_PP_OK;
+}
+
+int vfs_pp_server_add_case(vfs_pp_server* sv, const vfs_grid* g, const double* icsi, const double* ieta,
+                           const double* izet, const double* jcsi, const double* jeta, const double* jzet,
+                           const double* kcsi, const double* keta, const double* kzet, const double* iaj,
+                           const double* jaj, const double* kaj, const double* aj, const double* csi,
+                           const double* eta, const double* zet, const double* nvert, int* case_id) {
+  ABI_BEGIN
+  if (!sv || !g || !case_id) return VFS_PP_EINVAL;
+  vfs_pp_handle* h = nullptr;
+  int rc = vfs_pp_create(g, &sv->o.solver, &h);
+  if (rc != VFS_PP_OK) return rc;
+  h->external_ws = true;
+  rc = vfs_pp_set_geometry(h, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, aj, csi, eta,
+                           zet, nvert);
+  if (rc != VFS_PP_OK) {
+    vfs_pp_destroy(h);
+    return rc;
+  }
+  const size_t foot = h->ar.total();
+  const size_t ws = workspace_bytes(h);
+  // every registered case must still be able to run alone
+  if (sv->resident + foot + std::max(ws, sv->ws_max) > sv->o.hbm_budget) {
+    vfs_pp_destroy(h);
+    return VFS_PP_ENOMEM;
+  }
+  sv->resident += foot;
+  sv->ws_max = std::max(sv->ws_max, ws);
+  const int id = sv->next_case++;
+  sv->cases[id] = vfs_pp_server::Case{h, ws, false};
+  *case_id = id;
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_server_remove_case(vfs_pp_server* sv, int case_id) {
+  ABI_BEGIN
+  if (!sv) return VFS_PP_EINVAL;
+  auto it = sv->cases.find(case_id);
+  if (it == sv->cases.end()) return VFS_PP_EINVAL;
+  if (it->second.busy) return VFS_PP_ESTATE;
+  for (const auto& r : sv->queue)
+    if (r.cid == case_id) return VFS_PP_ESTATE;
+  sv->resident -= it->second.h->ar.total();
+  vfs_pp_destroy(it->second.h);
+  sv->cases.erase(it);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_server_submit(vfs_pp_server* sv, int case_id, double* ucont, double* phi, double* p, double* ucat,
+                         double dt, double st, double coeff, int* ticket) {
+  ABI_BEGIN
+  if (!sv || !ucont || !phi || !p || !ucat || !ticket || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
+  if (!sv->cases.count(case_id)) return VFS_PP_EINVAL;
+  const int t = sv->next_ticket++;
+  sv->queue.push_back(vfs_pp_server::Req{t, case_id, ucont, phi, p, ucat, dt, st, coeff});
+  *ticket = t;
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_server_progress(vfs_pp_server* sv, int* tickets, vfs_pp_stats* stats, int cap, int* ndone) {
+  ABI_BEGIN
+  if (!sv || !ndone || cap < 0) return VFS_PP_EINVAL;
+  *ndone = sv->cf ? sv->progress(*sv->cf, tickets, stats, cap) : sv->progress(*sv->cd, tickets, stats, cap);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_server_status(vfs_pp_server* sv, int* queued, int* active, size_t* resident_bytes,
+                         size_t* workspace_bytes_held) {
+  if (!sv) return VFS_PP_EINVAL;
+  int act = 0;
+  for (int t : sv->slot_ticket)
+    if (t >= 0) act++;
+  if (queued) *queued = (int)sv->queue.size();
+  if (active) *active = act;
+  if (resident_bytes) *resident_bytes = sv->resident;
+  if (workspace_bytes_held) *workspace_bytes_held = sv->ws_held;
+  return VFS_PP_OK;
+}
+
+}  // extern "C"

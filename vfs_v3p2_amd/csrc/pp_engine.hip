@@ -502,9 +502,11 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
     h->partial = h->ar.alloc<double>(h->partial_cap);
     h->red = h->ar.alloc<double>(512);
     h->snap = h->ar.alloc<double>(3 * M);
-    h->V = h->ar.alloc<double>((size_t)(h->mrest + 1) * n);
-    if (h->opts.precision == 1) h->Z = h->ar.alloc<float>((size_t)h->mrest * n);
-    else h->Z = h->ar.alloc<double>((size_t)h->mrest * n);
+    if (!h->external_ws) {
+      h->V = h->ar.alloc<double>((size_t)(h->mrest + 1) * n);
+      if (h->opts.precision == 1) h->Z = h->ar.alloc<float>((size_t)h->mrest * n);
+      else h->Z = h->ar.alloc<double>((size_t)h->mrest * n);
+    }
     HIPCHECK(hipHostMalloc(&h->hpin, 4096 * sizeof(double)));
     HIPCHECK(hipHostGetDevicePointer((void**)&h->hpin_dev, h->hpin, 0));
   }

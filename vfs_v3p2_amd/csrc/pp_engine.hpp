@@ -216,6 +216,7 @@ struct vfs_pp_handle {
   double* V = nullptr;
   void* Z = nullptr;
   int mrest = 0;
+  bool external_ws = false;  // V and Z are attached per solve by a server
   std::unique_ptr<pp::MGBase> mg;
   std::vector<pp::HostLevel> hl;
   double* hpin = nullptr;  // pinned host scratch
@@ -261,3 +262,12 @@ void divergence_launch(vfs_pp_handle* h, const double* ucont);
     std::fprintf(stderr, "vfs_pp: %s\n", e.what()); \
     return VFS_PP_EINVAL;                           \
   }
+
+namespace pp {
+// bytes of the Krylov basis (V in FP64, Z in the multigrid precision)
+inline size_t workspace_bytes(const vfs_pp_handle* h) {
+  const size_t n = h->d.n();
+  return (size_t)(h->mrest + 1) * n * sizeof(double) +
+         (size_t)h->mrest * n * (h->opts.precision == 1 ? sizeof(float) : sizeof(double));
+}
+}  // namespace pp

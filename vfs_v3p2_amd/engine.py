@@ -49,6 +49,10 @@ class Opts(C.Structure):
                 ("orthog", C.c_int), ("verbose", C.c_int)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("solver", Opts)]
+
+
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
                 ("bnorm", C.c_double), ("max_div", C.c_double), ("ibm_flux", C.c_double * 4),
@@ -99,6 +103,17 @@ def lib():
         "vfs_pp_batch_set_graphs": (C.c_int, [H, C.c_int]),
         "vfs_pp_batch_project_device": (C.c_int, [H] + [C.POINTER(C.c_void_p)] * 4 + [dp] * 3 +
                                         [C.POINTER(Stats)]),
+        "vfs_pp_server_default_opts": (None, [C.POINTER(ServerOpts)]),
+        "vfs_pp_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(H)]),
+        "vfs_pp_server_destroy": (C.c_int, [H]),
+        "vfs_pp_server_add_case": (C.c_int, [H, C.POINTER(Grid)] + [dp] * 17 + [C.POINTER(C.c_int)]),
+        "vfs_pp_server_remove_case": (C.c_int, [H, C.c_int]),
+        "vfs_pp_server_submit": (C.c_int, [H, C.c_int, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                           C.POINTER(C.c_int)]),
+        "vfs_pp_server_progress": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(Stats), C.c_int,
+                                             C.POINTER(C.c_int)]),
+        "vfs_pp_server_status": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_size_t),
+                                           C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -311,6 +326,88 @@ class Batch:
                                                stats)
         _check(rc, "batch project", allow=() if strict else (VFS_PP_ENOCONV,))
         return list(stats)
+
+
+GEOMETRY_FIELDS = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj",
+                   "aj", "csi", "eta", "zet", "nvert")
+
+
+class Server:
+    """Continuous batching of projection requests (vfs_pp_server): cases stay
+    resident; `submit` queues a projection of a case on device tensors and
+    `progress` admits queued requests into free slots as the HBM budget allows,
+    runs one batched iteration of everything in flight, and returns the
+    requests that finished as (ticket, Stats) pairs."""
+
+    def __init__(self, max_active=64, hbm_budget=0, **solver):
+        o = ServerOpts()
+        lib().vfs_pp_server_default_opts(C.byref(o))
+        o.max_active = int(max_active)
+        o.hbm_budget = int(hbm_budget)
+        for k, v in solver.items():
+            setattr(o.solver, k, v)
+        self.max_active = o.max_active
+        self._s = C.c_void_p()
+        _check(lib().vfs_pp_server_create(C.byref(o), C.byref(self._s)), "vfs_pp_server_create")
+        self._keep = {}
+        self.grids = {}
+
+    def close(self):
+        if self._s:
+            lib().vfs_pp_server_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_case(self, grid: Grid, fields: dict) -> int:
+        arrs, ptrs = [], []
+        for k in GEOMETRY_FIELDS:
+            want = dmda_shape(grid, vector=k not in ("iaj", "jaj", "kaj", "aj", "nvert"))
+            if tuple(np.shape(fields[k])) != want:
+                raise ValueError(f"add_case: {k} has shape {np.shape(fields[k])}, expected {want}")
+            a, p = _dptr(fields[k])
+            arrs.append(a)
+            ptrs.append(p)
+        cid = C.c_int()
+        _check(lib().vfs_pp_server_add_case(self._s, C.byref(grid), *ptrs, C.byref(cid)), "vfs_pp_server_add_case")
+        self.grids[cid.value] = grid
+        return cid.value
+
+    def remove_case(self, cid: int):
+        _check(lib().vfs_pp_server_remove_case(self._s, cid), "vfs_pp_server_remove_case")
+        self.grids.pop(cid, None)
+
+    def submit(self, cid, ucont, phi, p, ucat, dt, st, coeff) -> int:
+        g = self.grids[cid]
+        t = C.c_int()
+        _check(lib().vfs_pp_server_submit(self._s, cid, _tptr(ucont, dmda_shape(g, True), "ucont"),
+                                          _tptr(phi, dmda_shape(g), "phi"), _tptr(p, dmda_shape(g), "p"),
+                                          _tptr(ucat, dmda_shape(g, True), "ucat"), dt, st, coeff, C.byref(t)),
+               "vfs_pp_server_submit")
+        self._keep[t.value] = (ucont, phi, p, ucat)  # the arrays stay alive until the request finishes
+        return t.value
+
+    def progress(self):
+        cap = self.max_active
+        tickets = (C.c_int * cap)()
+        stats = (Stats * cap)()
+        nd = C.c_int()
+        _check(lib().vfs_pp_server_progress(self._s, tickets, stats, cap, C.byref(nd)), "vfs_pp_server_progress")
+        out = []
+        for q in range(nd.value):
+            self._keep.pop(tickets[q], None)
+            out.append((tickets[q], stats[q]))
+        return out
+
+    def status(self):
+        q, a = C.c_int(), C.c_int()
+        r, w = C.c_size_t(), C.c_size_t()
+        lib().vfs_pp_server_status(self._s, C.byref(q), C.byref(a), C.byref(r), C.byref(w))
+        return dict(queued=q.value, active=a.value, resident_bytes=r.value, workspace_bytes=w.value)
 
 
 # ----------------------------------------------------------- reference-named API
