@@ -57,8 +57,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU work for the baseline sample")
-    ap.add_argument("--form", choices=("single", "batch"), default="single",
-                    help="single: the 192x96x128 hill projection (default); batch: 64 uneven cases solved together")
+    ap.add_argument("--form", choices=("single", "batch", "serve"), default="single",
+                    help="single: the 192x96x128 hill projection (default); batch: 64 uneven cases solved "
+                         "together; serve: a Poisson stream of requests over resident cases")
+    ap.add_argument("--requests", type=int, default=150, help="serve form: requests per arrival rate")
+    ap.add_argument("--rates", type=str, default="10,50,100,200,500", help="serve form: arrival rates (1/s)")
     ap.add_argument("--cases", type=int, default=64, help="batch form: number of cases")
     ap.add_argument("--workers", type=int, default=16, help="batch form: host processes generating the cases")
     return ap.parse_args()
@@ -68,6 +71,134 @@ def _gen_case(spec):
     from vfs_v3p2_amd import cases
 
     return cases.ensemble_case(*spec)
+
+
+def _gen_served(spec):
+    from vfs_v3p2_amd import cases
+
+    return cases.served_case(*spec)
+
+
+def bench_serve(args):
+    """Poisson stream of projection requests over 16 resident cases (sizes seeded
+    log-uniform 24^3-256^3; 60% channel, 30% hill + sphere, 10% towers), served by
+    continuous batching; p50/p99 latency (arrival to completion) and solves/s per
+    arrival rate."""
+    import multiprocessing as mp
+
+    from vfs_v3p2_amd import cases
+
+    specs = cases.served_specs()
+    pool = mp.get_context("fork").Pool(max(1, min(args.workers, len(specs))))
+    pending = pool.imap(_gen_served, specs)
+    import torch
+
+    from vfs_v3p2_amd import engine as E
+
+    E.lib()
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    sv = E.Server(max_active=64, precision=1, rtol=RTOL, div_tol=DIV_TOL)
+    res = []
+    for c in pending:
+        g = E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed)
+        cid = sv.add_case(g, c.fields)
+        u1 = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        # a second input field of the same case: its divergent part from another seed
+        rng = np.random.default_rng(cid + 99)
+        u2 = u1 * torch.from_numpy(1.0 + 0.2 * rng.standard_normal(1)).cuda()
+        res.append((cid, c.shape, c.dt, c.st, c.coeff, u1, u2, c.nx * c.ny * c.nz, c.name))
+    pool.close()
+    setup_s = time.perf_counter() - t0
+    ncell = sum(r[7] for r in res)
+    rates = [float(x) for x in args.rates.split(",") if x]
+    rng = np.random.default_rng(2024)
+    per_rate = []
+    # warm-up: every case once (plans, tuning, graphs)
+    for r in res:
+        cid, sh = r[0], r[1]
+        sv.submit(cid, r[5].clone(), torch.zeros(sh, dtype=torch.float64, device="cuda"),
+                  torch.zeros(sh, dtype=torch.float64, device="cuda"),
+                  torch.zeros(sh + (3,), dtype=torch.float64, device="cuda"), r[2], r[3], r[4])
+    while sv.status()["queued"] or sv.status()["active"]:
+        sv.progress()
+    for lam in rates:
+        nreq = args.requests
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+        picks = rng.integers(len(res), size=nreq)
+        mix = rng.uniform(-1.0, 1.0, size=nreq)
+        lat = np.zeros(nreq)
+        divs, its, failed = [], [], []
+        t2r = {}
+        i = done = 0
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        while done < nreq:
+            now = time.perf_counter() - start
+            while i < nreq and arrivals[i] <= now:
+                r = res[picks[i]]
+                cid, sh = r[0], r[1]
+                u = r[5] + mix[i] * r[6]
+                t = sv.submit(cid, u, torch.zeros(sh, dtype=torch.float64, device="cuda"),
+                              torch.zeros(sh, dtype=torch.float64, device="cuda"),
+                              torch.zeros(sh + (3,), dtype=torch.float64, device="cuda"), r[2], r[3], r[4])
+                t2r[t] = i
+                i += 1
+            st = sv.status()
+            if st["queued"] == 0 and st["active"] == 0:
+                if i < nreq:
+                    time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - start))))
+                continue
+            for t, s in sv.progress():
+                q = t2r.pop(t)
+                lat[q] = (time.perf_counter() - start) - arrivals[q]
+                divs.append(s.max_div)
+                its.append(s.iterations)
+                if not s.converged:
+                    failed.append(q)
+                    r = res[picks[q]]
+                    print(f"request {q} (case {r[8]}) missed the tolerance: it {s.iterations} rel {s.rel_residual:.3e} "
+                          f"max_div {s.max_div:.3e} bnorm {s.bnorm:.3e}", file=sys.stderr, flush=True)
+                done += 1
+        span = time.perf_counter() - start
+        per_rate.append(dict(rate=lam, requests=nreq, solves_per_s=round(nreq / span, 2),
+                             p50_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
+                             p99_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
+                             max_div=float(max(divs)), iterations_max=int(max(its)), failed=len(failed)))
+    st = sv.status()
+    sv.close()
+    top = per_rate[-1]
+    out = {
+        "metric": "Poisson solves/s & p99 latency at div<=1e-8, mixed vs FP64; stencil HBM GB/s",
+        "value": top["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": sum(p["requests"] for p in per_rate),
+        "warmup": len(res),
+        "ms_per_step": round(1e3 / top["solves_per_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 multigrid preconditioner)",
+        "data": "synthetic (seeded generator, vfs_v3p2_amd/cases.py)",
+        "config": {
+            "workload": "served stream: Poisson arrivals over 16 resident cases (sizes seeded log-uniform "
+                        "24^3-256^3; 60% stretched channel, 30% hill + IB sphere, 10% IB towers), continuous "
+                        "batching, one step = one request; value = solves/s at the highest arrival rate",
+            "cells_resident": ncell,
+            "cases": [r[8] for r in res],
+            "max_active": 64,
+        },
+        "p50_ms": top["p50_ms"],
+        "p99_ms": top["p99_ms"],
+        "rates": per_rate,
+        "resident_gb": round(st["resident_bytes"] / 1e9, 2),
+        "workspace_gb": round(st["workspace_bytes"] / 1e9, 2),
+        "setup_s": round(setup_s, 1),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):
@@ -234,6 +365,9 @@ def main():
         sys.exit(2)
     if args.form == "batch":
         bench_batch(args)
+        return
+    if args.form == "serve":
+        bench_serve(args)
         return
     import torch
 

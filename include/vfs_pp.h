@@ -23,7 +23,10 @@
  *
  * Thread safety: one handle may be used by one host thread at a time.  Each
  * handle owns one HIP stream (default priority); calls are synchronous unless
- * stated otherwise.
+ * stated otherwise.  Engine streams are blocking streams: their work runs after
+ * work queued earlier on the legacy default stream (e.g. the kernels that wrote
+ * a call's input arrays) and before work queued there afterwards; callers that
+ * produce inputs on other non-blocking streams synchronise those first.
  */
 #ifndef VFS_PP_H
 #define VFS_PP_H

@@ -105,3 +105,18 @@ def test_batch_rejects_mixed_precision(E):
         E.Batch([h0, h1])
     h0.close()
     h1.close()
+
+
+def test_batch_divergence_rule(E, torch_cuda):
+    torch = torch_cuda
+    cs = get_cases()
+    hs = handles(E, cs, precision=1, rtol=1e-6, div_tol=1e-8)
+    B = E.Batch(hs)
+    u, phi, p, ucat = fields(torch, cs)
+    sts = B.project(u, phi, p, ucat, [c.dt for c, _ in cs], [c.st for c, _ in cs], [c.coeff for c, _ in cs],
+                    strict=True)
+    B.close()
+    for s in sts:
+        assert s.converged and s.max_div <= 1e-8
+    for h in hs:
+        h.close()

@@ -351,3 +351,22 @@ def test_big_projection_consistent_with_oracle(big, big_runs):
     ucat = np.zeros_like(u)
     O.contra2cart(A.g, c.bctype, c.fields, u, ucat)
     np.testing.assert_allclose(ucg, ucat, rtol=0, atol=1e-13 * np.abs(ucat).max())
+
+
+def test_divergence_rule_drives_the_solve(E, torch_cuda):
+    """With div_tol set the solve goes on until the projected field meets it,
+    whatever rtol says: max|aj div u| after projection = dt/coeff max|aj (b - A phi)|."""
+    torch = torch_cuda
+    c = cases.channel_case(24, 32, 16, beta=3.0, seed=3)
+    out = {}
+    for tag, dtol in (("loose", 0.0), ("ruled", 1e-8)):
+        h = handle_for(E, c, rtol=1e-6, div_tol=dtol)
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        out[tag] = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff)
+        h.close()
+    assert out["loose"].max_div > 1e-8  # rtol alone does not get there on this grid
+    assert out["ruled"].converged and out["ruled"].max_div <= 1e-8
+    assert out["ruled"].iterations > out["loose"].iterations

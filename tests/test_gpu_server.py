@@ -128,3 +128,31 @@ def test_server_memory_admission(E, torch_cuda):
         c = cs[0]
         small.add_case(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), c.fields)
     assert ei.value.code == E.VFS_PP_ENOMEM
+
+
+def test_inputs_written_just_before_submit_are_seen(E, torch_cuda):
+    """Inputs produced by device kernels immediately before submit (on the
+    default stream) are read after those kernels finish."""
+    torch = torch_cuda
+    c = get_cases()[1]
+    opts = dict(precision=1, rtol=1e-12)
+    base = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+    st_a, phi_a, _ = solve_alone(E, torch, c, (base * 0.75).cpu().numpy(), opts)
+    sv = E.Server(max_active=2, **opts)
+    cid = sv.add_case(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), c.fields)
+    outs = []
+    for rep in range(3):
+        big = torch.ones(1 << 24, dtype=torch.float64, device="cuda").cumsum(0)  # keep the default stream busy
+        u = base * 0.5 + base * 0.25 + big[:1] * 0.0
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        t = sv.submit(cid, u, phi, p, ucat, c.dt, c.st, c.coeff)
+        got = []
+        while not got:
+            got = sv.progress()
+        assert got[0][0] == t and got[0][1].converged
+        outs.append(phi.cpu().numpy())
+    sv.close()
+    for ph in outs:
+        assert relL2(ph, phi_a) <= 1e-9

@@ -113,7 +113,8 @@ def _cross_metrics(dc, de, dz):
     eta = np.stack([dydz * dzdc - dzdz * dydc, -dxdz * dzdc + dzdz * dxdc, dxdz * dydc - dydz * dxdc], -1)
     zet = np.stack([dydc * dzde - dzdc * dyde, -dxdc * dzde + dzdc * dxde, dxdc * dyde - dydc * dxde], -1)
     det = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) + dzdc * (dxde * dydz - dyde * dxdz)
-    return csi, eta, zet, 1.0 / det
+    with np.errstate(divide="ignore"):  # the extrapolated outer plane is not used
+        return csi, eta, zet, 1.0 / det
 
 
 def _diff(c, axis, first, last, periodic_seam):
@@ -517,3 +518,38 @@ def ensemble_case(dims, beta, h, with_sphere, seed):
 
 FIELD_ORDER_VEC = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet")
 FIELD_ORDER_SCA = ("iaj", "jaj", "kaj", "aj")
+
+
+def served_case(kind, dims, seed):
+    """One case of the served stream: 'channel' (tanh-stretched periodic channel,
+    no IB), 'hill' (terrain + IB sphere) or 'towers' (stretched box with an IB
+    tower array), at the given interior extents."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    if kind == "channel":
+        c = channel_case(nx, ny, nz, beta=float(rng.uniform(1.0, 3.0)), seed=seed)
+    elif kind == "hill":
+        c = hill_case(nx, ny, nz, h=float(rng.uniform(0.1, 0.4)), s=0.5, beta=float(rng.uniform(0.0, 2.0)),
+                      sphere=(float(rng.uniform(-1.5, 1.5)), float(rng.uniform(0.7, 1.3)), float(rng.uniform(-1.0, 1.0)),
+                              0.25), seed=seed)
+    elif kind == "towers":
+        c = towers_case(nx, ny, nz, ratio=1.0 + 4.0 / ny, rows=2, cols=3, radius_cells=max(2, nx // 48), height=0.6,
+                        seed=seed)
+    else:
+        raise ValueError(kind)
+    c.name = f"{kind}{nx}x{ny}x{nz}"
+    return c
+
+
+def served_specs(count=16, lo=24, hi=256, seed=5):
+    """Seeded (kind, dims, seed) specs: sizes log-uniform in [lo^3, hi^3] with
+    +-15% per axis, kinds 60% channel / 30% hill / 10% towers."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n0 = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        dims = tuple(max(12, int(round(n0 * (1.0 + rng.uniform(-0.15, 0.15))))) for _ in range(3))
+        u = rng.uniform()
+        kind = "channel" if u < 0.6 else ("hill" if u < 0.9 else "towers")
+        out.append((kind, dims, int(rng.integers(1 << 30))))
+    return out

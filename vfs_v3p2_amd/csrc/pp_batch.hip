@@ -431,7 +431,7 @@ struct BatchCore {
   bool tuned_once = false;
 
   explicit BatchCore(int nslots) {
-    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamDefault));  // blocking: ordered with the legacy default stream
     slots.resize(nslots);
     sd = fixed.alloc<SlotDev>(nslots);
     HIPCHECK(hipHostMalloc(&sd_host, sizeof(SlotDev) * nslots));
@@ -942,9 +942,7 @@ struct BatchCore {
         launch_update_x<T>(s, h->d.n(), h->x, static_cast<const T*>(h->Z), h->d.n(), sl.pending_nv, h->red + 200);
         sl.pending_update = false;
       }
-      int nb = 0;
-      launch_residual64<double>(s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
-      launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+      launch_true_residual(h, s);
     }
     HIPCHECK(hipStreamSynchronize(s));
     for (int q : qs) {
@@ -954,7 +952,7 @@ struct BatchCore {
       const double beta = std::sqrt(h->hpin[0]);
       if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
       sl.rel = beta / sl.S.bnorm;
-      if (beta <= sl.tgt) { sl.conv = true; sl.state = FINISH; continue; }
+      if (residual_verdict(h, beta, sl.S.bnorm, sl.tgt)) { sl.conv = true; sl.state = FINISH; continue; }
       if (sl.it >= h->opts.max_iters) { sl.state = FINISH; continue; }
       launch_scale_conv<T>(s, h->d.n(), h->r, h->red, h->V, static_cast<T*>(sl.mg->f0()));
       std::fill(sl.gv.begin(), sl.gv.end(), 0.);

@@ -511,6 +511,20 @@ __global__ __launch_bounds__(256) void k_divergence(Dmda m, Dims d, const uint32
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = mv;
 }
 
+// max over Divergence-counted cells of |aj r|: with r = b - A phi this is the
+// max |aj div u| the projection of phi leaves (div u_new = dt/coeff (b - A phi)
+// for st = 1), known before the projection runs
+__global__ __launch_bounds__(256) void k_divest(Dims d, const uint32_t* __restrict__ code,
+                                                const double* __restrict__ aj, const double* __restrict__ r,
+                                                double* __restrict__ partial) {
+  __shared__ double lds[4];
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double mv = 0.;
+  if (t < d.n() && (code[t] & DIVOK_BIT)) mv = fabs(aj[t] * r[t]);
+  mv = block_max(mv, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = mv;
+}
+
 // ---------------------------------------------------------------- reductions
 // One 1024-thread block per vector sums (op 0) or maxes (op 1) its nblk block
 // partials in a fixed order: four independent accumulators per thread keep the
@@ -819,6 +833,14 @@ void launch_c2c(hipStream_t s, Dmda m, Dims d, const double* met9, const uint8_t
     k_vec_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucat, nullptr);
     HIPCHECK(hipGetLastError());
   }
+}
+
+void launch_divest(hipStream_t s, Dims d, const uint32_t* code, const double* aj, const double* r, double* partial,
+                   int* nblk) {
+  const unsigned nb = nb1(d.n());
+  *nblk = (int)nb;
+  k_divest<<<nb, 256, 0, s>>>(d, code, aj, r, partial);
+  HIPCHECK(hipGetLastError());
 }
 
 void launch_divergence(hipStream_t s, Dmda m, Dims d, const uint32_t* code, const double* aj, const double* ucont,

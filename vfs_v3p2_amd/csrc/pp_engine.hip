@@ -361,13 +361,6 @@ void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk) {
                                          std::to_string(h->partial_cap) + ")");
 }
 
-static double finalize_to_host(vfs_pp_handle* h, int nvec, int nblk, int op, double* out) {
-  launch_finalize(h->s, h->partial, nvec, nblk, h->red, op, h->hpin_dev);
-  HIPCHECK(hipStreamSynchronize(h->s));
-  for (int v = 0; v < nvec; v++) out[v] = h->hpin[v];
-  return out[0];
-}
-
 // FGMRES(m) with CGS2; right preconditioner = one V-cycle.  Solves A x = b
 // (x holds the initial guess).  Returns iterations; fills rel.
 template <typename TZ>
@@ -380,17 +373,18 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
   int it = 0;
   double rel = 1.;
   *conv = false;
-  const double tgt = rtol * bnorm;
+  double tgt = rtol * bnorm;
   while (true) {
-    int nb = 0;
-    launch_residual64<double>(h->s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
-    double beta2;
-    finalize_to_host(h, 1, nb, 0, &beta2);
-    const double beta = std::sqrt(beta2);
+    int nb = 0;  // block count of the reductions below
+    launch_true_residual(h, h->s);
+    HIPCHECK(hipStreamSynchronize(h->s));
+    const double beta = std::sqrt(h->hpin[0]);
     rel = bnorm > 0. ? beta / bnorm : 0.;
     if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
-    if (h->opts.verbose) std::fprintf(stderr, "vfs_pp: restart at it %d  true rel %.3e\n", it, rel);
-    if (beta <= tgt) { *conv = true; break; }
+    if (h->opts.verbose)
+      std::fprintf(stderr, "vfs_pp: restart at it %d  true rel %.3e  div est %.3e\n", it, rel,
+                   h->div_fac * h->hpin[RED_DIVEST]);
+    if (residual_verdict(h, beta, bnorm, tgt)) { *conv = true; break; }
     if (it >= h->opts.max_iters) break;
     // v_0 = r / beta and its copy as the first V-cycle input
     launch_scale_conv<TZ>(h->s, n, h->r, h->red, h->V, static_cast<TZ*>(h->mg->f0()));
@@ -623,6 +617,10 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
 void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff) {
   ensure_geom(h);
   hipStream_t s = h->s;
+  // div u after projection = dt/coeff (b - A phi) holds for st = 1 (the RHS scales
+  // by st, the projection by st as well); with another st the rule is off
+  h->div_fac = (h->opts.div_tol > 0. && st == 1.) ? dt / coeff : 0.;
+  h->div_goal = 0.5 * h->opts.div_tol;
   int nb = 0;
   if (h->nibf) {
     launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 1, h->partial, &nb);
@@ -772,7 +770,10 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
   if (!(h->opts.cheb_lo > 0. && h->opts.cheb_lo < h->opts.cheb_hi)) return VFS_PP_EINVAL;
   h->d = Dims{g->nx, g->ny, g->nz, {g->periodic[0] ? 1 : 0, g->periodic[1] ? 1 : 0, g->periodic[2] ? 1 : 0}};
   h->m = Dmda{g->nx + 2, g->ny + 2, g->nz + 2, {h->d.per[0], h->d.per[1], h->d.per[2]}};
-  HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+  // a blocking stream: its work is ordered after work the caller queued on the
+  // legacy default stream (PyTorch's default stream, for one) and before work
+  // queued there afterwards, so inputs written just before a call are seen
+  HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamDefault));
   *out = h.release();
   return VFS_PP_OK;
   ABI_END

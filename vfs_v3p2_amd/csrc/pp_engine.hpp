@@ -32,8 +32,7 @@ class Arena {
       throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
     // every buffer starts at zero so no result can depend on earlier contents of
-    // the device heap.  The memset goes to the null stream and is waited for
-    // here: the engine's own stream is non-blocking and would otherwise race it.
+    // the device heap; the memset goes to the null stream and is waited for here
     e = hipMemsetAsync(p, 0, bytes, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
@@ -217,6 +216,9 @@ struct vfs_pp_handle {
   void* Z = nullptr;
   int mrest = 0;
   bool external_ws = false;  // V and Z are attached per solve by a server
+  // divergence part of the stopping rule for the current solve: max |aj div u|
+  // after projection = div_fac * max |aj r|; stop when that is <= div_goal
+  double div_fac = 0., div_goal = 0.;
   std::unique_ptr<pp::MGBase> mg;
   std::vector<pp::HostLevel> hl;
   double* hpin = nullptr;  // pinned host scratch
@@ -232,6 +234,7 @@ constexpr int RED_IBF = 300;   // VolumeFlux: 4 values
 constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
 constexpr int RED_B2 = 320;    // ||b||^2
 constexpr int RED_DIV = 330;   // max |aj div u|
+constexpr int RED_DIVEST = 340;  // max |aj r| at the last true-residual check
 
 // engine functions shared with the batched engine (pp_engine.hip)
 double now_ms();
@@ -269,5 +272,34 @@ inline size_t workspace_bytes(const vfs_pp_handle* h) {
   const size_t n = h->d.n();
   return (size_t)(h->mrest + 1) * n * sizeof(double) +
          (size_t)h->mrest * n * (h->opts.precision == 1 ? sizeof(float) : sizeof(double));
+}
+}  // namespace pp
+
+namespace pp {
+// True residual r = b - A x in FP64 with ||r||^2 into red[0] / hpin[0] and, when
+// the divergence rule is on, max |aj r| into hpin[RED_DIVEST]; launches only.
+inline void launch_true_residual(vfs_pp_handle* h, hipStream_t s) {
+  int nb = 0;
+  launch_residual64<double>(s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
+  launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+  if (h->div_fac > 0.) {
+    launch_divest(s, h->d, h->code, h->aj, h->r, h->partial, &nb);
+    launch_finalize(s, h->partial, 1, nb, h->red + RED_DIVEST, 1, h->hpin_dev + RED_DIVEST);
+  }
+}
+
+// Stopping rule at a true-residual check (after the stream sync): ||r|| <= tgt
+// and, with the divergence rule on, div_fac * max |aj r| <= div_goal.  When only
+// the divergence part fails, tgt is lowered in proportion and the solve goes on;
+// at the FP64 rounding floor it stops and the post-projection check reports.
+inline bool residual_verdict(vfs_pp_handle* h, double beta, double bnorm, double& tgt) {
+  if (!(beta <= tgt)) return false;
+  if (h->div_fac <= 0.) return true;
+  const double dmax = h->div_fac * h->hpin[RED_DIVEST];
+  if (dmax <= h->div_goal) return true;
+  const double floor_t = 1e-14 * bnorm;
+  if (beta <= floor_t) return true;
+  tgt = std::max(floor_t, 0.5 * beta * h->div_goal / dmax);
+  return false;
 }
 }  // namespace pp

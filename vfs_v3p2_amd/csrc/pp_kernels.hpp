@@ -75,6 +75,8 @@ void launch_projection(hipStream_t s, Dmda m, Dims d, const double* g, const uin
 void launch_ucont_ghosts(hipStream_t s, Dmda m, double* ucont);
 void launch_c2c(hipStream_t s, Dmda m, Dims d, const double* met9, const uint8_t* cls, const double* nvert,
                 const double* ucont, double* ucat, double* scratch, const int bctype[6]);
+void launch_divest(hipStream_t s, Dims d, const uint32_t* code, const double* aj, const double* r, double* partial,
+                   int* nblk);
 void launch_divergence(hipStream_t s, Dmda m, Dims d, const uint32_t* code, const double* aj, const double* ucont,
                        double* partial, int* nblk);
 
