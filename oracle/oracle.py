@@ -277,7 +277,7 @@ def reference_projection(case, rtol=1e-13, maxit=200000, precond=1, ucont=None, 
 class CpuMG:
     """CPU port of the engine's multigrid-preconditioned FGMRES (FP64)."""
 
-    def __init__(self, A: "Assembled", degree=3, lo=0.15, hi=1.1, coarse_max=512):
+    def __init__(self, A: "Assembled", degree=3, lo=0.1, hi=1.1, coarse_max=512):
         self.A = A
         self.ptr = lib().or_mg_create(C.byref(A.g), _p(A.g9), _p(A.nvert), degree, lo, hi, coarse_max)
         self.levels = lib().or_mg_levels(self.ptr)

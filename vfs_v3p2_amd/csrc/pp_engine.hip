@@ -737,7 +737,7 @@ void vfs_pp_default_opts(vfs_pp_opts* o) {
   o->coarse_max = 512;
   o->drop_small = 0;
   o->zero_guess = 1;
-  o->cheb_lo = 0.15;
+  o->cheb_lo = 0.1;
   o->cheb_hi = 1.1;
   o->orthog = 0;
   o->verbose = 0;

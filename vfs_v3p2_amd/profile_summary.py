@@ -1,6 +1,7 @@
 """Summarise rocprofv3 output (CSV) of the engine.
 
     python -m vfs_v3p2_amd.profile_summary <run>_kernel_trace.csv
+    python -m vfs_v3p2_amd.profile_summary <run>_kernel_trace.csv --batch
     python -m vfs_v3p2_amd.profile_summary --fetch <fetch>_counter_collection.csv \
         --write <write>_counter_collection.csv --cells N
 
@@ -8,6 +9,10 @@ Trace mode prints, as markdown: per-(kernel, grid) launch counts and durations,
 the kernel time of each solve (a solve runs from its RHS kernel to its
 divergence kernel), and the per-solve breakdown by kernel for the first and the
 last group of solves (the benchmark runs the mixed form first, then FP64).
+
+Batch mode keeps the batched kernels (kb_*, in the multigrid precision given by
+--precision), splits them into calls at gaps of more than 20 ms, and breaks
+down the last call by kernel.
 
 Traffic mode reads one FETCH_SIZE and one WRITE_SIZE pass (KiB per dispatch),
 scales each by what the same counter reads for the 1 GiB device copy kernel
@@ -82,9 +87,40 @@ def traffic(fetch_csv, write_csv, cells, top):
         print(f"\ntraffic_bytes_per_launch: {t:.0f}")
 
 
+def batch(rows, precision, top):
+    kb = [r for r in rows if r["name"].startswith("kb_") and (precision in r["name"] or "<" not in r["name"])]
+    calls, cur = [], []
+    for r in kb:
+        if cur and r["s"] - cur[-1]["e"] > 20e6:
+            calls.append(cur)
+            cur = []
+        cur.append(r)
+    if cur:
+        calls.append(cur)
+    print("## Batched calls (kb_* kernels, split at gaps > 20 ms)\n")
+    print("| call | wall ms (profiled) | kernel ms | launches |")
+    print("|---|---|---|---|")
+    for q, c in enumerate(calls):
+        print(f"| {q} | {(c[-1]['e'] - c[0]['s']) / 1e6:.2f} | {sum(r['d'] for r in c) / 1e6:.2f} | {len(c)} |")
+    c = calls[-1]
+    agg = collections.defaultdict(float)
+    cnt = collections.defaultdict(int)
+    for r in c:
+        agg[r["name"]] += r["d"]
+        cnt[r["name"]] += 1
+    tot = sum(agg.values())
+    print(f"\n## Last call by kernel: {tot / 1e6:.2f} ms kernel time\n")
+    print("| kernel | launches | ms | mean us | share |")
+    print("|---|---|---|---|---|")
+    for k, v in sorted(agg.items(), key=lambda kv: -kv[1])[:top]:
+        print(f"| {k} | {cnt[k]} | {v / 1e6:.2f} | {v / cnt[k] / 1e3:.1f} | {100 * v / tot:.1f}% |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("trace", nargs="?")
+    ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--precision", default="float")
     ap.add_argument("--fetch", default=None)
     ap.add_argument("--write", default=None)
     ap.add_argument("--cells", type=int, default=192 * 96 * 128)
@@ -94,6 +130,9 @@ def main():
         traffic(a.fetch, a.write, a.cells, a.top)
         return
     rows = load_trace(a.trace)
+    if a.batch:
+        batch(rows, a.precision, a.top)
+        return
     by = collections.defaultdict(list)
     for r in rows:
         by[(r["name"], r["grid"])].append(r)
