@@ -117,6 +117,23 @@ int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta
                         const double *kaj, const double *aj, const double *csi,
                         const double *eta, const double *zet, const double *nvert);
 
+/* FormMetrics (metrics.c:4-1290, the NEWMETRIC branch that metrics.c:2 selects)
+ * on the device: from the DMDA node coordinates coor[mz][my][mx][3] (device
+ * array; the extra node plane at index m-1 of each axis is the one the reference
+ * extrapolates) writes the 16 metric arrays vfs_pp_set_geometry takes, all device
+ * arrays in the DMDA layout, bit-exact with the reference's arithmetic.  Stream
+ * ordered on the legacy default stream; returns after the kernel finishes. */
+int vfs_pp_form_metrics_device(const vfs_grid *grid, const double *coor, double *icsi,
+                               double *ieta, double *izet, double *jcsi, double *jeta,
+                               double *jzet, double *kcsi, double *keta, double *kzet,
+                               double *iaj, double *jaj, double *kaj, double *aj, double *csi,
+                               double *eta, double *zet);
+
+/* Grid set-up from device node coordinates and a device nvert array: FormMetrics
+ * on the device followed by everything vfs_pp_set_geometry does, with no host
+ * copy of the face metrics (init.c:900 FormMetrics, then the solver set-up). */
+int vfs_pp_set_grid_device(vfs_pp_handle *h, const double *coor, const double *nvert);
+
 /* PoissonSolver_Hypre (poisson_hypre.c:580-771) on device-resident arrays:
  * VolumeFlux (modifies ucont, poisson_hypre.c:662-667) -> PoissonRHS2_hypre ->
  * Krylov+MG solve -> Remove_Nullspace -> Convert_Phi2_Phi.  phi is the DMDA

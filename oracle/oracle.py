@@ -61,6 +61,7 @@ def lib():
         _lib.or_gmres.restype = C.c_int
         _lib.or_gmres.argtypes = [C.c_int, ip, ip, d, d, d, C.c_int, C.c_double, C.c_int, C.c_int, d]
         _lib.or_num_threads.restype = C.c_int
+        _lib.or_form_metrics.argtypes = [g, d, C.POINTER(d)]
         _lib.or_mg_create.restype = C.c_void_p
         _lib.or_mg_create.argtypes = [g, d, d, C.c_int, C.c_double, C.c_double, C.c_int]
         _lib.or_mg_destroy.argtypes = [C.c_void_p]
@@ -329,3 +330,20 @@ def cpu_projection(case, A: "Assembled", mg: CpuMG, rtol=1e-10, ucont=None):
     contra2cart(g, case.bctype, f, u, ucat)
     maxdiv = divergence(g, u, np.ascontiguousarray(f["aj"]), A.nvert)
     return dict(phi=phi, p=p, ucont=u, ucat=ucat, maxdiv=maxdiv, iters=it, rel=rel, vflux=vf)
+
+
+METRIC_NAMES = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj", "aj",
+                "csi", "eta", "zet")
+
+
+def form_metrics(co, per):
+    """or_form_metrics on DMDA node coordinates co (mz, my, mx, 3): dict of the 16
+    metric arrays in the UserCtx layout."""
+    co = np.ascontiguousarray(co, dtype=np.float64)
+    mz, my, mx, _ = co.shape
+    g = Grid(mx, my, mz, (C.c_int * 3)(*[int(bool(v)) for v in per]))
+    out = {nm: np.zeros((mz, my, mx, 3) if (q < 9 or q >= 13) else (mz, my, mx)) for q, nm in
+           enumerate(METRIC_NAMES)}
+    ptrs = (C.POINTER(C.c_double) * 16)(*[_p(out[nm]) for nm in METRIC_NAMES])
+    lib().or_form_metrics(C.byref(g), _p(co), ptrs)
+    return out

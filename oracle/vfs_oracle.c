@@ -1333,3 +1333,199 @@ int or_mg_fgmres(or_mg *mg, const double *b, double *x, int m, double rtol, int 
   free(V); free(Z); free(H); free(cs); free(sn); free(gv); free(y);
   return it;
 }
+
+/* ---------------------------------------------------------------- FormMetrics */
+typedef struct { double x, y, z; } or_v3;
+
+static or_v3 v3_at(const double *a, size_t p) { or_v3 v = {a[3 * p], a[3 * p + 1], a[3 * p + 2]}; return v; }
+static void v3_put(double *a, size_t p, or_v3 v) { a[3 * p] = v.x; a[3 * p + 1] = v.y; a[3 * p + 2] = v.z; }
+
+/* metric triple and inverse Jacobian from the derivatives along xi, eta, zeta
+ * (metrics.c:700-724 and the same block in the j and k sections) */
+static void metric_triple(or_v3 c, or_v3 e, or_v3 z, or_v3 *csi, or_v3 *eta, or_v3 *zet, double *aj) {
+  csi->x = e.y * z.z - e.z * z.y;
+  csi->y = -e.x * z.z + e.z * z.x;
+  csi->z = e.x * z.y - e.y * z.x;
+  eta->x = z.y * c.z - z.z * c.y;
+  eta->y = -z.x * c.z + z.z * c.x;
+  eta->z = z.x * c.y - z.y * c.x;
+  zet->x = c.y * e.z - c.z * e.y;
+  zet->y = -c.x * e.z + c.z * e.x;
+  zet->z = c.x * e.y - c.y * e.x;
+  double det = c.x * (e.y * z.z - e.z * z.y) - c.y * (e.x * z.z - e.z * z.x) + c.z * (e.x * z.y - e.y * z.x);
+  *aj = 1. / det;
+}
+
+/* one-sided at the ends of [lo, hi], central inside, the periodic seam average at
+ * hi when `seam` (metrics.c:628-700): f(t) is the face-centre value at index t */
+static or_v3 dcent(const double *cent, const or_grid *g, int ax, int lo, int hi, int seam, int i, int j, int k) {
+  int t = ax == 0 ? i : (ax == 1 ? j : k);
+#define CAT(s) v3_at(cent, ax == 0 ? IX(g, (s), j, k) : (ax == 1 ? IX(g, i, (s), k) : IX(g, i, j, (s))))
+  or_v3 r;
+  if (t == lo) {
+    or_v3 a = CAT(t + 1), b = CAT(t);
+    r.x = a.x - b.x; r.y = a.y - b.y; r.z = a.z - b.z;
+  } else if (t == hi) {
+    if (seam) {
+      or_v3 a = CAT(1), b = CAT(0), c = CAT(t), d = CAT(t - 1);
+      r.x = (a.x - b.x + c.x - d.x) * 0.5; r.y = (a.y - b.y + c.y - d.y) * 0.5; r.z = (a.z - b.z + c.z - d.z) * 0.5;
+    } else {
+      or_v3 a = CAT(t), b = CAT(t - 1);
+      r.x = a.x - b.x; r.y = a.y - b.y; r.z = a.z - b.z;
+    }
+  } else {
+    or_v3 a = CAT(t + 1), b = CAT(t - 1);
+    r.x = (a.x - b.x) * 0.5; r.y = (a.y - b.y) * 0.5; r.z = (a.z - b.z) * 0.5;
+  }
+#undef CAT
+  return r;
+}
+
+/* 0.5 * (a + b - c - d) on nodes given by index */
+static or_v3 half4(const double *co, size_t a, size_t b, size_t c, size_t d) {
+  or_v3 r;
+  r.x = 0.5 * (co[3 * a] + co[3 * b] - co[3 * c] - co[3 * d]);
+  r.y = 0.5 * (co[3 * a + 1] + co[3 * b + 1] - co[3 * c + 1] - co[3 * d + 1]);
+  r.z = 0.5 * (co[3 * a + 2] + co[3 * b + 2] - co[3 * c + 2] - co[3 * d + 2]);
+  return r;
+}
+
+void or_form_metrics(const or_grid *g, const double *co, double *const out[16]) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const size_t n = (size_t)mx * my * mz;
+  double *icsi = out[0], *ieta = out[1], *izet = out[2], *jcsi = out[3], *jeta = out[4], *jzet = out[5];
+  double *kcsi = out[6], *keta = out[7], *kzet = out[8], *iaj = out[9], *jaj = out[10], *kaj = out[11];
+  double *aj = out[12], *csi = out[13], *eta = out[14], *zet = out[15];
+  for (int q = 0; q < 16; q++) memset(out[q], 0, ((q < 9 || q >= 13) ? 3 : 1) * n * sizeof(double));
+  double *cent = (double *)calloc(3 * n, sizeof(double));
+
+  /* face centres and face metrics, one face family at a time (metrics.c:613-1004) */
+  for (int fam = 0; fam < 3; fam++) {
+    memset(cent, 0, 3 * n * sizeof(double));
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+          size_t a, b, c, d;
+          if (fam == 0) { /* centx: corners over j and k (613-622) */
+            if (k < 1 || j < 1) continue;
+            a = IX(g, i, j, k); b = IX(g, i, j, k - 1); c = IX(g, i, j - 1, k); d = IX(g, i, j - 1, k - 1);
+          } else if (fam == 1) { /* centy: corners over i and k (739-748) */
+            if (k < 1 || i < 1) continue;
+            a = IX(g, i, j, k); b = IX(g, i, j, k - 1); c = IX(g, i - 1, j, k); d = IX(g, i - 1, j, k - 1);
+          } else { /* centz: corners over i and j (859-868) */
+            if (j < 1 || i < 1) continue;
+            a = IX(g, i, j, k); b = IX(g, i, j - 1, k); c = IX(g, i - 1, j, k); d = IX(g, i - 1, j - 1, k);
+          }
+          const size_t p = IX(g, i, j, k);
+          for (int v = 0; v < 3; v++)
+            cent[3 * p + v] = (co[3 * a + v] + co[3 * b + v] + co[3 * c + v] + co[3 * d + v]) * 0.25;
+        }
+    /* derivative ranges: the face-normal axis runs 0..m-2 (seam when periodic),
+     * the two tangential axes 1..m-2 */
+    int lo[3] = {1, 1, 1}, seam[3] = {0, 0, 0};
+    lo[fam] = 0;
+    seam[fam] = g->per[fam];
+    double *o1 = fam == 0 ? icsi : (fam == 1 ? jcsi : kcsi);
+    double *o2 = fam == 0 ? ieta : (fam == 1 ? jeta : keta);
+    double *o3 = fam == 0 ? izet : (fam == 1 ? jzet : kzet);
+    double *oj = fam == 0 ? iaj : (fam == 1 ? jaj : kaj);
+    for (int k = lo[2]; k <= mz - 2; k++)
+      for (int j = lo[1]; j <= my - 2; j++)
+        for (int i = lo[0]; i <= mx - 2; i++) {
+          or_v3 dc = dcent(cent, g, 0, lo[0], mx - 2, seam[0], i, j, k);
+          or_v3 de = dcent(cent, g, 1, lo[1], my - 2, seam[1], i, j, k);
+          or_v3 dz = dcent(cent, g, 2, lo[2], mz - 2, seam[2], i, j, k);
+          or_v3 a, b, c;
+          double J;
+          metric_triple(dc, de, dz, &a, &b, &c, &J);
+          const size_t p = IX(g, i, j, k);
+          v3_put(o1, p, a); v3_put(o2, p, b); v3_put(o3, p, c);
+          oj[p] = J;
+        }
+  }
+
+  /* NEWMETRIC node-plane area vectors (metrics.c:185-270) */
+  double *ct = (double *)calloc(3 * n, sizeof(double));
+  double *et = (double *)calloc(3 * n, sizeof(double));
+  double *zt = (double *)calloc(3 * n, sizeof(double));
+  for (int k = 1; k <= mz - 2; k++)
+    for (int j = 1; j <= my - 2; j++)
+      for (int i = 0; i <= mx - 2; i++) {
+        or_v3 e = half4(co, IX(g, i, j, k), IX(g, i, j, k - 1), IX(g, i, j - 1, k), IX(g, i, j - 1, k - 1));
+        or_v3 z = half4(co, IX(g, i, j - 1, k), IX(g, i, j, k), IX(g, i, j - 1, k - 1), IX(g, i, j, k - 1));
+        or_v3 r = {e.y * z.z - e.z * z.y, -e.x * z.z + e.z * z.x, e.x * z.y - e.y * z.x};
+        v3_put(ct, IX(g, i, j, k), r);
+      }
+  for (int k = 1; k <= mz - 2; k++)
+    for (int j = 0; j <= my - 2; j++)
+      for (int i = 1; i <= mx - 2; i++) {
+        or_v3 c = half4(co, IX(g, i, j, k), IX(g, i, j, k - 1), IX(g, i - 1, j, k), IX(g, i - 1, j, k - 1));
+        or_v3 z = half4(co, IX(g, i, j, k), IX(g, i - 1, j, k), IX(g, i, j, k - 1), IX(g, i - 1, j, k - 1));
+        or_v3 r = {z.y * c.z - z.z * c.y, -z.x * c.z + z.z * c.x, z.x * c.y - z.y * c.x};
+        v3_put(et, IX(g, i, j, k), r);
+      }
+  for (int k = 0; k <= mz - 2; k++)
+    for (int j = 1; j <= my - 2; j++)
+      for (int i = 1; i <= mx - 2; i++) {
+        or_v3 c = half4(co, IX(g, i, j, k), IX(g, i, j - 1, k), IX(g, i - 1, j, k), IX(g, i - 1, j - 1, k));
+        or_v3 e = half4(co, IX(g, i, j, k), IX(g, i - 1, j, k), IX(g, i, j - 1, k), IX(g, i - 1, j - 1, k));
+        or_v3 r = {c.y * e.z - c.z * e.y, -c.x * e.z + c.z * e.x, c.x * e.y - c.y * e.x};
+        v3_put(zt, IX(g, i, j, k), r);
+      }
+  /* cell metrics as AxByC(0.5, ., 0.5, .) of the two planes (metrics.c:286-296)
+   * and the cell Jacobian from the 8 corners (metrics.c:302-351) */
+  for (int k = 1; k <= mz - 2; k++)
+    for (int j = 1; j <= my - 2; j++)
+      for (int i = 1; i <= mx - 2; i++) {
+        const size_t p = IX(g, i, j, k);
+        const size_t pi = IX(g, i - 1, j, k), pj = IX(g, i, j - 1, k), pk = IX(g, i, j, k - 1);
+        for (int v = 0; v < 3; v++) {
+          csi[3 * p + v] = 0.5 * ct[3 * p + v] + 0.5 * ct[3 * pi + v];
+          eta[3 * p + v] = 0.5 * et[3 * p + v] + 0.5 * et[3 * pj + v];
+          zet[3 * p + v] = 0.5 * zt[3 * p + v] + 0.5 * zt[3 * pk + v];
+        }
+        /* corners: (i|i-1, j|j-1, k|k-1) */
+        const size_t n0 = IX(g, i, j, k), n1 = IX(g, i, j - 1, k), n2 = IX(g, i, j, k - 1),
+                     n3 = IX(g, i, j - 1, k - 1), n4 = IX(g, i - 1, j, k), n5 = IX(g, i - 1, j - 1, k),
+                     n6 = IX(g, i - 1, j, k - 1), n7 = IX(g, i - 1, j - 1, k - 1);
+        or_v3 d[3];
+        for (int v = 0; v < 3; v++) {
+          const double c0 = co[3 * n0 + v], c1 = co[3 * n1 + v], c2 = co[3 * n2 + v], c3 = co[3 * n3 + v],
+                       c4 = co[3 * n4 + v], c5 = co[3 * n5 + v], c6 = co[3 * n6 + v], c7 = co[3 * n7 + v];
+          const double xc = 0.25 * (c0 + c1 + c2 + c3 - c4 - c5 - c6 - c7);
+          const double xe = 0.25 * (c0 + c4 + c2 + c6 - c1 - c5 - c3 - c7);
+          const double xz = 0.25 * (c0 + c1 + c4 + c5 - c2 - c3 - c6 - c7);
+          if (v == 0) { d[0].x = xc; d[1].x = xe; d[2].x = xz; }
+          if (v == 1) { d[0].y = xc; d[1].y = xe; d[2].y = xz; }
+          if (v == 2) { d[0].z = xc; d[1].z = xe; d[2].z = xz; }
+        }
+        or_v3 a, b, c;
+        metric_triple(d[0], d[1], d[2], &a, &b, &c, &aj[p]);
+      }
+  free(ct); free(et); free(zt); free(cent);
+
+  /* boundary planes copy the adjacent interior plane, i then j then k
+   * (metrics.c:374-454), then the periodic images (metrics.c:477-509) */
+  double *cell[4] = {csi, eta, zet, aj};
+  for (int q = 0; q < 4; q++) {
+    const int w = q < 3 ? 3 : 1;
+    double *a = cell[q];
+#define CP(dst, src) for (int v = 0; v < w; v++) a[w * (dst) + v] = a[w * (src) + v]
+    for (int k = 0; k < mz; k++)
+      for (int j = 0; j < my; j++) { CP(IX(g, 0, j, k), IX(g, 1, j, k)); CP(IX(g, mx - 1, j, k), IX(g, mx - 2, j, k)); }
+    for (int k = 0; k < mz; k++)
+      for (int i = 0; i < mx; i++) { CP(IX(g, i, 0, k), IX(g, i, 1, k)); CP(IX(g, i, my - 1, k), IX(g, i, my - 2, k)); }
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) { CP(IX(g, i, j, 0), IX(g, i, j, 1)); CP(IX(g, i, j, mz - 1), IX(g, i, j, mz - 2)); }
+    if (g->per[0])
+      for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++) { CP(IX(g, 0, j, k), IX(g, mx - 2, j, k)); CP(IX(g, mx - 1, j, k), IX(g, 1, j, k)); }
+    if (g->per[1])
+      for (int k = 0; k < mz; k++)
+        for (int i = 0; i < mx; i++) { CP(IX(g, i, 0, k), IX(g, i, my - 2, k)); CP(IX(g, i, my - 1, k), IX(g, i, 1, k)); }
+    if (g->per[2])
+      for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) { CP(IX(g, i, j, 0), IX(g, i, j, mz - 2)); CP(IX(g, i, j, mz - 1), IX(g, i, j, 1)); }
+#undef CP
+  }
+}

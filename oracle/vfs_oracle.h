@@ -124,6 +124,12 @@ int or_fully_blocked(const or_grid *g, const double *nvert, int *kske);
 int or_gmres(int n, const int *rowptr, const int *col, const double *val, const double *b,
              double *x, int m, double rtol, int maxit, int precond, double *rel);
 
+/* FormMetrics (metrics.c:4-1290, NEWMETRIC branch) from the DMDA node
+ * coordinates co[mz][my][mx][3]: out[] = icsi, ieta, izet, jcsi, jeta, jzet,
+ * kcsi, keta, kzet (3 per point), iaj, jaj, kaj, aj (1 per point), csi, eta,
+ * zet (3 per point). */
+void or_form_metrics(const or_grid *g, const double *co, double *const out[16]);
+
 int or_num_threads(void);
 
 /* CPU port of the engine's solver (geometric multigrid + FGMRES/CGS2), on

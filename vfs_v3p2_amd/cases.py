@@ -43,6 +43,7 @@ class Case:
     dt: float = 0.04  # init.c:58
     st: float = 1.0  # init.c:76
     coeff: float = 1.0  # time_coeff() with rans=0, poisson.c:22-30
+    coor: Optional[np.ndarray] = None  # DMDA node coordinates (mz,my,mx,3), kept on request
 
     @property
     def shape(self):
@@ -175,35 +176,38 @@ def form_metrics(co: np.ndarray, per) -> dict:
     f["kaj"] = np.where(m, aj, 0.0)
     del cx, cy, cz, dc, de, dz, csi, eta, zet, aj, m
 
-    # ---- cell metrics (NEWMETRIC): average of node-plane area vectors, metrics.c:215-301
-    def h(a, b):
-        return 0.5 * (a + b)
+    # ---- cell metrics (NEWMETRIC): average of node-plane area vectors, metrics.c:185-296.
+    # Each derivative is 0.5*(a + b - c - d) and each average 0.5*x + 0.5*y (AxByC,
+    # momentum.c:35-40), in the reference's operand order.
+    def d4(a, b, c, d):
+        return 0.5 * (a + b - c - d)
 
-    # node-plane i area vector for cell column (j,k)
-    dxde = h(co[1:, 1:, :], co[:-1, 1:, :]) - h(co[1:, :-1, :], co[:-1, :-1, :])
-    dxdz = h(co[1:, :-1, :], co[1:, 1:, :]) - h(co[:-1, :-1, :], co[:-1, 1:, :])
-    csit = np.stack([dxde[..., 1] * dxdz[..., 2] - dxde[..., 2] * dxdz[..., 1],
-                     -dxde[..., 0] * dxdz[..., 2] + dxde[..., 2] * dxdz[..., 0],
-                     dxde[..., 0] * dxdz[..., 1] - dxde[..., 1] * dxdz[..., 0]], -1)  # (mz-1,my-1,mx,3)
-    dxdc = h(co[1:, :, 1:], co[:-1, :, 1:]) - h(co[1:, :, :-1], co[:-1, :, :-1])
-    dxdz = h(co[1:, :, 1:], co[1:, :, :-1]) - h(co[:-1, :, 1:], co[:-1, :, :-1])
-    etat = np.stack([dxdz[..., 1] * dxdc[..., 2] - dxdz[..., 2] * dxdc[..., 1],
-                     -dxdz[..., 0] * dxdc[..., 2] + dxdz[..., 2] * dxdc[..., 0],
-                     dxdz[..., 0] * dxdc[..., 1] - dxdz[..., 1] * dxdc[..., 0]], -1)  # (mz-1,my,mx-1,3)
-    dxdc = h(co[:, 1:, 1:], co[:, :-1, 1:]) - h(co[:, 1:, :-1], co[:, :-1, :-1])
-    dxde = h(co[:, 1:, 1:], co[:, 1:, :-1]) - h(co[:, :-1, 1:], co[:, :-1, :-1])
-    zett = np.stack([dxdc[..., 1] * dxde[..., 2] - dxdc[..., 2] * dxde[..., 1],
-                     -dxdc[..., 0] * dxde[..., 2] + dxdc[..., 2] * dxde[..., 0],
-                     dxdc[..., 0] * dxde[..., 1] - dxdc[..., 1] * dxde[..., 0]], -1)  # (mz,my-1,mx-1,3)
+    def cross(u, v):
+        return np.stack([u[..., 1] * v[..., 2] - u[..., 2] * v[..., 1],
+                         -u[..., 0] * v[..., 2] + u[..., 2] * v[..., 0],
+                         u[..., 0] * v[..., 1] - u[..., 1] * v[..., 0]], -1)
+
+    # csitmp of node plane i for cell column (j,k): index [k-1, j-1, i]
+    dxde = d4(co[1:, 1:, :], co[:-1, 1:, :], co[1:, :-1, :], co[:-1, :-1, :])
+    dxdz = d4(co[1:, :-1, :], co[1:, 1:, :], co[:-1, :-1, :], co[:-1, 1:, :])
+    csit = cross(dxde, dxdz)  # (mz-1,my-1,mx,3)
+    # etatmp of node plane j for cell row (i,k): index [k-1, j, i-1]
+    dxdc = d4(co[1:, :, 1:], co[:-1, :, 1:], co[1:, :, :-1], co[:-1, :, :-1])
+    dxdz = d4(co[1:, :, 1:], co[1:, :, :-1], co[:-1, :, 1:], co[:-1, :, :-1])
+    etat = cross(dxdz, dxdc)  # (mz-1,my,mx-1,3)
+    # zettmp of node plane k for cell column (i,j): index [k, j-1, i-1]
+    dxdc = d4(co[:, 1:, 1:], co[:, :-1, 1:], co[:, 1:, :-1], co[:, :-1, :-1])
+    dxde = d4(co[:, 1:, 1:], co[:, 1:, :-1], co[:, :-1, 1:], co[:, :-1, :-1])
+    zett = cross(dxdc, dxde)  # (mz,my-1,mx-1,3)
     I = slice(1, mx - 1)
     J = slice(1, my - 1)
     K = slice(1, mz - 1)
     csi = np.zeros((mz, my, mx, 3))
     eta = np.zeros_like(csi)
     zet = np.zeros_like(csi)
-    csi[K, J, I] = 0.5 * (csit[0:mz - 2, 0:my - 2, 1:mx - 1] + csit[0:mz - 2, 0:my - 2, 0:mx - 2])
-    eta[K, J, I] = 0.5 * (etat[0:mz - 2, 1:my - 1, 0:mx - 2] + etat[0:mz - 2, 0:my - 2, 0:mx - 2])
-    zet[K, J, I] = 0.5 * (zett[1:mz - 1, 0:my - 2, 0:mx - 2] + zett[0:mz - 2, 0:my - 2, 0:mx - 2])
+    csi[K, J, I] = 0.5 * csit[0:mz - 2, 0:my - 2, 1:mx - 1] + 0.5 * csit[0:mz - 2, 0:my - 2, 0:mx - 2]
+    eta[K, J, I] = 0.5 * etat[0:mz - 2, 1:my - 1, 0:mx - 2] + 0.5 * etat[0:mz - 2, 0:my - 2, 0:mx - 2]
+    zet[K, J, I] = 0.5 * zett[1:mz - 1, 0:my - 2, 0:mx - 2] + 0.5 * zett[0:mz - 2, 0:my - 2, 0:mx - 2]
     del csit, etat, zett, dxdc, dxde, dxdz
     # cell Jacobian from the 8 corner nodes, metrics.c:305-371
     c8 = [co[1:, 1:, 1:], co[1:, :-1, 1:], co[:-1, 1:, 1:], co[:-1, :-1, 1:],
@@ -421,7 +425,7 @@ def divergent_ucont(nx, ny, nz, f, per, seed, noise=0.05, mean_profile=None, flu
 # ------------------------------------------------------------------------- configs
 
 def build_case(name, nx, ny, nz, x, eta, z, *, hill_h=0.0, hill_s=0.5, top=2.0,
-               ib=None, seed=0, noise=0.05, mean_profile_fn=None, fluct=1.0):
+               ib=None, seed=0, noise=0.05, mean_profile_fn=None, fluct=1.0, keep_coords=False):
     per = (1, 0, 1)
     bct = (0, 0, 1, 1, 0, 0)  # no-slip walls in j; periodic i and k
     N = make_nodes(nx, ny, nz, x, eta, z, hill_h=hill_h, hill_s=hill_s, top=top)
@@ -444,20 +448,22 @@ def build_case(name, nx, ny, nz, x, eta, z, *, hill_h=0.0, hill_s=0.5, top=2.0,
         # that it is uniform along i (divergence-free on a streamwise-uniform grid)
         mp = np.broadcast_to(mean_profile_fn(cent[:, :, 1:2, 1]), cent.shape[:3])
     f["ucont"] = divergent_ucont(nx, ny, nz, f, per, seed, noise=noise, mean_profile=mp, fluct=fluct)
-    del cent, co
-    return Case(name=name, nx=nx, ny=ny, nz=nz, per=per, bctype=bct, immersed=immersed, fields=f)
+    del cent
+    return Case(name=name, nx=nx, ny=ny, nz=nz, per=per, bctype=bct, immersed=immersed, fields=f,
+                coor=co if keep_coords else None)
 
 
-def channel_case(nx=64, ny=64, nz=64, beta=2.0, seed=0, noise=0.05):
+def channel_case(nx=64, ny=64, nz=64, beta=2.0, seed=0, noise=0.05, keep_coords=False):
     """Periodic channel 2pi x 2 x pi, tanh wall stretching in y, no IB."""
     x = np.arange(nx + 1) * (2 * np.pi / nx)
     z = np.arange(nz + 1) * (np.pi / nz)
     eta = 0.5 * (tanh_wall_nodes(ny, beta) + 1.0)
-    return build_case(f"channel{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, top=2.0, seed=seed, noise=noise)
+    return build_case(f"channel{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, top=2.0, seed=seed, noise=noise,
+                      keep_coords=keep_coords)
 
 
 def hill_case(nx=192, ny=96, nz=128, h=0.3, s=0.5, beta=0.0, sphere=(0.6, 0.75, 0.0, 0.15), seed=0,
-              noise=0.05):
+              noise=0.05, keep_coords=False):
     """Gaussian hill under algebraic grid-line blending, optional sphere IB."""
     x = -3.0 + np.arange(nx + 1) * (6.0 / nx)
     z = -2.0 + np.arange(nz + 1) * (4.0 / nz)
@@ -467,7 +473,7 @@ def hill_case(nx=192, ny=96, nz=128, h=0.3, s=0.5, beta=0.0, sphere=(0.6, 0.75, 
         cx, cy, cz, r = sphere
         ib = lambda cent: inside_sphere(cent, (cx, cy, cz), r)
     return build_case(f"hill{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, hill_h=h, hill_s=s, top=2.0, ib=ib,
-                      seed=seed, noise=noise)
+                      seed=seed, noise=noise, keep_coords=keep_coords)
 
 
 def towers_case(nx=512, ny=256, nz=256, ratio=1.02, rows=3, cols=4, radius_cells=4, height=0.6, seed=0,

@@ -302,6 +302,23 @@ __global__ __launch_bounds__(256) void k_scatter_update(Dmda m, Dims d, const do
   if (p && (b & CLS_LT)) p[q] += ph;  // UpdatePressure, poisson.c:2548
 }
 
+// cell metrics csi/eta/zet and aj of the interior cells, SoA (met9[3 v + q])
+__global__ __launch_bounds__(256) void k_gather_cell_metrics(Dmda m, Dims d, const double* __restrict__ csi,
+                                                             const double* __restrict__ eta,
+                                                             const double* __restrict__ zet,
+                                                             const double* __restrict__ aj, double* __restrict__ met9,
+                                                             double* __restrict__ ajc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.n(), c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+  const double* src[3] = {csi, eta, zet};
+#pragma unroll
+  for (int v = 0; v < 3; v++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) met9[(size_t)(3 * v + q) * n + c] = src[v][3 * p + q];
+  ajc[c] = aj[p];
+}
+
 // Initial guess from the caller's Phi: the inverse of Convert_Phi2_Phi.  The
 // reference keeps its HYPRE solution vector between time steps and GMRES starts
 // from it (poisson_hypre.c:565-574, 722); that vector equals Phi on every cell
@@ -789,6 +806,12 @@ void launch_sub_mean(hipStream_t s, Dims d, double* x, const uint8_t* cls, const
 
 void launch_scatter_update(hipStream_t s, Dmda m, Dims d, const double* x, const uint8_t* cls, double* phi, double* p) {
   k_scatter_update<<<cell_grid(d), kBlk, 0, s>>>(m, d, x, cls, phi, p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_gather_cell_metrics(hipStream_t s, Dmda m, Dims d, const double* csi, const double* eta,
+                                const double* zet, const double* aj, double* met9, double* ajc) {
+  k_gather_cell_metrics<<<cell_grid(d), kBlk, 0, s>>>(m, d, csi, eta, zet, aj, met9, ajc);
   HIPCHECK(hipGetLastError());
 }
 

@@ -469,9 +469,11 @@ void ensure_geom(vfs_pp_handle* h) {
   if (!h->have_geom) throw StatusError(VFS_PP_ESTATE, "vfs_pp_set_geometry has not been called");
 }
 
-static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const double* iaj, const double* jaj,
-                         const double* kaj, const double* aj, const double* csi, const double* eta,
-                         const double* zet, const double* nvert) {
+// Set-up from the 16 metric arrays on the device (dmet: icsi..kzet, iaj, jaj, kaj,
+// aj, csi, eta, zet) and nvert on the device, with host copies of csi/eta/zet and
+// nvert for the VolumeFlux face list and the host hierarchy.
+static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const double* dnvert, const double* csi,
+                              const double* eta, const double* zet, const double* nvert) {
   const Dmda m = h->m;
   const Dims d = h->d;
   const size_t M = m.n(), n = d.n();
@@ -504,42 +506,14 @@ static void set_geometry(vfs_pp_handle* h, const double* const* vec9, const doub
     HIPCHECK(hipHostMalloc(&h->hpin, 4096 * sizeof(double)));
     HIPCHECK(hipHostGetDevicePointer((void**)&h->hpin_dev, h->hpin, 0));
   }
-  // temporaries for the face metrics
-  {
-    Arena tmp;
-    double* dv[9];
-    for (int q = 0; q < 9; q++) {
-      dv[q] = tmp.alloc<double>(3 * M);
-      h2d(s, dv[q], vec9[q], 3 * M);
-    }
-    double* dj[3];
-    const double* hj[3] = {iaj, jaj, kaj};
-    for (int q = 0; q < 3; q++) {
-      dj[q] = tmp.alloc<double>(M);
-      h2d(s, dj[q], hj[q], M);
-    }
-    h2d(s, h->nvert, nvert, M);
-    launch_face_coeffs(s, m, d, dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], dv[6], dv[7], dv[8], dj[0], dj[1], dj[2],
-                       h->g64);
-    launch_codes(s, m, d, h->nvert, h->code, h->cls);
-    HIPCHECK(hipStreamSynchronize(s));
-  }
-  // cell metrics and aj on interior cells (Contra2Cart, Divergence)
-  {
-    std::vector<double> met(9 * n), ajv(n);
-    const double* src[3] = {csi, eta, zet};
-    for (int k = 0; k < d.nz; k++)
-      for (int j = 0; j < d.ny; j++)
-        for (int i = 0; i < d.nx; i++) {
-          const size_t c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
-          for (int v = 0; v < 3; v++)
-            for (int q = 0; q < 3; q++) met[(size_t)(3 * v + q) * n + c] = src[v][3 * p + q];
-          ajv[c] = aj[p];
-        }
-    h2d(s, h->met9, met.data(), 9 * n);
-    h2d(s, h->aj, ajv.data(), n);
-    HIPCHECK(hipStreamSynchronize(s));
-  }
+  // face coefficients, branch codes, and the cell metrics and aj on interior cells
+  // (Contra2Cart, Divergence)
+  HIPCHECK(hipMemcpyAsync(h->nvert, dnvert, M * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_face_coeffs(s, m, d, dmet[0], dmet[1], dmet[2], dmet[3], dmet[4], dmet[5], dmet[6], dmet[7], dmet[8], dmet[9],
+                     dmet[10], dmet[11], h->g64);
+  launch_codes(s, m, d, h->nvert, h->code, h->cls);
+  launch_gather_cell_metrics(s, m, d, dmet[13], dmet[14], dmet[15], dmet[12], h->met9, h->aj);
+  HIPCHECK(hipStreamSynchronize(s));
   // VolumeFlux face list (poisson.c:2670-2700 / 2936-3010)
   {
     std::vector<IbFace> faces;
@@ -809,11 +783,58 @@ int vfs_pp_set_geometry(vfs_pp_handle* h, const double* icsi, const double* ieta
                         const double* nvert) {
   ABI_BEGIN
   if (!h) return VFS_PP_EINVAL;
-  const double* v9[9] = {icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet};
-  for (auto p : v9)
+  const double* hmet[16] = {icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, aj, csi, eta, zet};
+  for (auto p : hmet)
     if (!p) return VFS_PP_EINVAL;
-  if (!iaj || !jaj || !kaj || !aj || !csi || !eta || !zet || !nvert) return VFS_PP_EINVAL;
-  set_geometry(h, v9, iaj, jaj, kaj, aj, csi, eta, zet, nvert);
+  if (!nvert) return VFS_PP_EINVAL;
+  const size_t M = h->m.n();
+  Arena tmp;
+  double* dmet[16];
+  for (int q = 0; q < 16; q++) {
+    const size_t len = (q < 9 || q >= 13) ? 3 * M : M;
+    dmet[q] = tmp.alloc<double>(len);
+    h2d(h->s, dmet[q], hmet[q], len);
+  }
+  double* dnv = tmp.alloc<double>(M);
+  h2d(h->s, dnv, nvert, M);
+  set_geometry_core(h, dmet, dnv, csi, eta, zet, nvert);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_form_metrics_device(const vfs_grid* g, const double* coor, double* icsi, double* ieta, double* izet,
+                               double* jcsi, double* jeta, double* jzet, double* kcsi, double* keta, double* kzet,
+                               double* iaj, double* jaj, double* kaj, double* aj, double* csi, double* eta,
+                               double* zet) {
+  ABI_BEGIN
+  if (!g || !coor) return VFS_PP_EINVAL;
+  if (g->nx < 2 || g->ny < 2 || g->nz < 2) return VFS_PP_EINVAL;
+  double* out[16] = {icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, aj, csi, eta, zet};
+  for (auto p : out)
+    if (!p) return VFS_PP_EINVAL;
+  const Dmda m{g->nx + 2, g->ny + 2, g->nz + 2, {g->periodic[0] ? 1 : 0, g->periodic[1] ? 1 : 0, g->periodic[2] ? 1 : 0}};
+  launch_form_metrics(nullptr, m, coor, out);
+  HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_set_grid_device(vfs_pp_handle* h, const double* coor, const double* nvert) {
+  ABI_BEGIN
+  if (!h || !coor || !nvert) return VFS_PP_EINVAL;
+  const size_t M = h->m.n();
+  Arena tmp;
+  double* dmet[16];
+  for (int q = 0; q < 16; q++) dmet[q] = tmp.alloc<double>((q < 9 || q >= 13) ? 3 * M : M);
+  launch_form_metrics(h->s, h->m, coor, dmet);
+  // host copies for the VolumeFlux face list and the host hierarchy
+  std::vector<double> hc(3 * M), he(3 * M), hz(3 * M), hn(M);
+  d2h(h->s, hc.data(), dmet[13], 3 * M);
+  d2h(h->s, he.data(), dmet[14], 3 * M);
+  d2h(h->s, hz.data(), dmet[15], 3 * M);
+  d2h(h->s, hn.data(), nvert, M);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  set_geometry_core(h, dmet, nvert, hc.data(), he.data(), hz.data(), hn.data());
   return VFS_PP_OK;
   ABI_END
 }

@@ -68,6 +68,9 @@ void launch_sub_mean(hipStream_t s, Dims d, double* x, const uint8_t* cls, const
 void launch_scatter_update(hipStream_t s, Dmda m, Dims d, const double* x, const uint8_t* cls, double* phi,
                            double* p);
 void launch_periodic_scalar(hipStream_t s, Dmda m, double* a, int zero_other);
+void launch_gather_cell_metrics(hipStream_t s, Dmda m, Dims d, const double* csi, const double* eta,
+                                const double* zet, const double* aj, double* met9, double* ajc);
+void launch_form_metrics(hipStream_t s, Dmda m, const double* coor, double* const out[16]);
 void launch_gather_phi(hipStream_t s, Dmda m, Dims d, const double* phi, const uint8_t* cls, double* x);
 void launch_add_phi(hipStream_t s, Dmda m, Dims d, const uint8_t* cls, const double* phi, double* p);
 void launch_projection(hipStream_t s, Dmda m, Dims d, const double* g, const uint32_t* code, const uint8_t* cls,

@@ -120,34 +120,65 @@ __device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __res
 // One Chebyshev step preconditioned by the y-line tridiagonal part T of A on the
 // line (i, k): z = T^-1 r with the precomputed factors (forward sweep parked in
 // tmp), d = c1 d + c2 z, x_new = x_old + d.  ZG: from x = 0 with r = f.
+// The line is walked in chunks of LC rows whose inputs are loaded into registers
+// first (independent loads in flight together), then the recurrence runs on them:
+// the sequential sweep is otherwise bound by one load latency per row.
 template <typename T, bool ZG>
 __device__ __forceinline__ void line_solve_col(const Dims& d, const T* __restrict__ r, const T* __restrict__ lfac,
                                                T* __restrict__ tmp, const T* __restrict__ x_old,
                                                T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2, int i,
                                                int k) {
+  constexpr int LC = 16;
   const size_t n = d.n();
   const T* __restrict__ la = lfac;
   const T* __restrict__ lcp = lfac + n;
   const T* __restrict__ linv = lfac + 2 * n;
   const size_t sy = (size_t)d.nx;
   const size_t c0 = d.idx(i, 0, k);
+  const int ny = d.ny;
   T dprev = T(0);
-#pragma unroll 4
-  for (int j = 0; j < d.ny; j++) {
-    const size_t c = c0 + (size_t)j * sy;
-    dprev = (r[c] - la[c] * dprev) * linv[c];
-    tmp[c] = dprev;
+  for (int j0 = 0; j0 < ny; j0 += LC) {
+    T rr[LC], aa[LC], vv[LC];
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      const int j = j0 + q < ny ? j0 + q : ny - 1;  // clamped rows are loaded but not used
+      const size_t c = c0 + (size_t)j * sy;
+      rr[q] = r[c];
+      aa[q] = la[c];
+      vv[q] = linv[c];
+    }
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      if (j0 + q < ny) {
+        dprev = (rr[q] - aa[q] * dprev) * vv[q];
+        tmp[c0 + (size_t)(j0 + q) * sy] = dprev;
+      }
+    }
   }
   T znext = T(0);
-#pragma unroll 4
-  for (int j = d.ny - 1; j >= 0; j--) {
-    const size_t c = c0 + (size_t)j * sy;
-    const T z = tmp[c] - lcp[c] * znext;
-    znext = z;
-    const T dold = (ZG || c1 == T(0)) ? T(0) : dv[c];
-    const T dn = c1 * dold + c2 * z;
-    dv[c] = dn;
-    x_new[c] = (ZG ? T(0) : x_old[c]) + dn;
+  const bool rd = !ZG && c1 != T(0);
+  for (int j1 = ny - 1; j1 >= 0; j1 -= LC) {
+    T tt[LC], pp[LC], dd[LC], xo[LC];
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      const int j = j1 - q >= 0 ? j1 - q : 0;
+      const size_t c = c0 + (size_t)j * sy;
+      tt[q] = tmp[c];
+      pp[q] = lcp[c];
+      dd[q] = rd ? dv[c] : T(0);
+      xo[q] = ZG ? T(0) : x_old[c];
+    }
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      if (j1 - q >= 0) {
+        const size_t c = c0 + (size_t)(j1 - q) * sy;
+        const T z = tt[q] - pp[q] * znext;
+        znext = z;
+        const T dn = c1 * dd[q] + c2 * z;
+        dv[c] = dn;
+        x_new[c] = xo[q] + dn;
+      }
+    }
   }
 }
 

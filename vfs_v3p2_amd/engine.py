@@ -82,6 +82,8 @@ def lib():
         "vfs_pp_destroy": (C.c_int, [H]),
         "vfs_pp_set_opts": (C.c_int, [H, C.POINTER(Opts)]),
         "vfs_pp_set_geometry": (C.c_int, [H] + [dp] * 17),
+        "vfs_pp_form_metrics_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 17),
+        "vfs_pp_set_grid_device": (C.c_int, [H, vp, vp]),
         "vfs_pp_poisson_solve_device": (C.c_int, [H, vp, vp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
         "vfs_pp_update_pressure_device": (C.c_int, [H, vp, vp]),
         "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
@@ -216,6 +218,12 @@ class Handle:
             keep.append(a)
             args.append(p)
         _check(lib().vfs_pp_set_geometry(self._h, *args), "vfs_pp_set_geometry")
+
+    def set_grid_device(self, coor, nvert):
+        """FormMetrics on the device from node coordinates (mz, my, mx, 3) and the
+        set-up of vfs_pp_set_geometry, from device tensors."""
+        _check(lib().vfs_pp_set_grid_device(self._h, self._v(coor, "coor"), self._s(nvert, "nvert")),
+               "vfs_pp_set_grid_device")
 
     def _s(self, t, what):
         return _tptr(t, dmda_shape(self.grid), what)
@@ -472,6 +480,24 @@ def coarse_dmda_shape(grid: Grid):
     if grid.nx % 2 or grid.ny % 2 or grid.nz % 2:
         raise ValueError("the reference multigrid primitives need even interior extents")
     return tuple((m - 1 + 1) // 2 + 1 for m in dmda_shape(grid))
+
+
+METRIC_FIELDS = GEOMETRY_FIELDS[:16]
+
+
+def form_metrics_device(grid: Grid, coor):
+    """FormMetrics on the device: dict of the 16 metric tensors (float64, on the
+    device of `coor`) from DMDA node coordinates (mz, my, mx, 3)."""
+    import torch
+
+    V, S = dmda_shape(grid, vector=True), dmda_shape(grid)
+    cp = _tptr(coor, V, "coor")
+    out = {k: torch.empty(V if k in ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "csi",
+                                     "eta", "zet") else S, dtype=torch.float64, device=coor.device)
+           for k in METRIC_FIELDS}
+    _check(lib().vfs_pp_form_metrics_device(C.byref(grid), cp, *[C.c_void_p(out[k].data_ptr()) for k in METRIC_FIELDS]),
+           "vfs_pp_form_metrics_device")
+    return out
 
 
 def classify_nvert_device(grid: Grid, nvert_dev):
