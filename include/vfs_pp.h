@@ -45,6 +45,7 @@ extern "C" {
 #define VFS_PP_ENOCONV 4     /* solver did not reach the requested tolerance */
 #define VFS_PP_ESTATE 5      /* call order violated (e.g. solve before set_geometry) */
 #define VFS_PP_ENONFINITE 6  /* NaN/Inf in the inputs or the iterate */
+#define VFS_PP_EIO 7         /* grid file missing, truncated or malformed */
 
 typedef struct vfs_pp_handle vfs_pp_handle;
 
@@ -116,6 +117,27 @@ int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta
                         const double *kzet, const double *iaj, const double *jaj,
                         const double *kaj, const double *aj, const double *csi,
                         const double *eta, const double *zet, const double *nvert);
+
+/* Grid files as init.c:251-389 reads them (the reference's gridfile, -binary
+ * and -xyz options): */
+#define VFS_PP_GRID_TEXT 0   /* grid.dat text: block count, per block "IM JM KM" then
+                                all x, all y, all z values, i fastest (fscanf) */
+#define VFS_PP_GRID_BINARY 1 /* the same sequence as native int32 / float64 words */
+#define VFS_PP_GRID_XYZ 2    /* xyz.dat: "IM JM KM", then IM, JM, KM lines whose 1st,
+                                2nd, 3rd columns are X[i], Y[j], Z[k] (one block) */
+
+/* Block count and node counts IM, JM, KM of block `block` (the grid then has
+ * nx = IM-1, ny = JM-1, nz = KM-1 cells); reads the headers only. */
+int vfs_pp_grid_file_info(const char *path, int format, int block, int *nblocks, int *im,
+                          int *jm, int *km);
+
+/* Reads block `block` into the device DMDA coordinate array
+ * coor[KM+1][JM+1][IM+1][3] (grid->nx == IM-1 etc.), each value as
+ * value / cl * l_dim (init.c:339-366; cl is -chact_leng, l_dim is L_dim).  The
+ * plane beyond the last node on each axis, which the reference leaves unread,
+ * is filled by linear extrapolation.  Returns after the data is on the device. */
+int vfs_pp_read_grid_device(const char *path, int format, int block, double cl, double l_dim,
+                            const vfs_grid *grid, double *coor);
 
 /* FormMetrics (metrics.c:4-1290, the NEWMETRIC branch that metrics.c:2 selects)
  * on the device: from the DMDA node coordinates coor[mz][my][mx][3] (device

@@ -104,7 +104,7 @@ def test_default_opts_and_error_strings():
     assert 0 < o.rtol < 1e-6 and o.restart >= 1 and o.max_iters >= 1
     assert 0 < o.cheb_lo < o.cheb_hi
     seen = set()
-    for code in range(7):
+    for code in range(8):
         s = E.error_string(code)
         assert s and s not in seen
         seen.add(s)

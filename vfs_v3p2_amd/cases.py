@@ -477,7 +477,7 @@ def hill_case(nx=192, ny=96, nz=128, h=0.3, s=0.5, beta=0.0, sphere=(0.6, 0.75, 
 
 
 def towers_case(nx=512, ny=256, nz=256, ratio=1.02, rows=3, cols=4, radius_cells=4, height=0.6, seed=0,
-                noise=0.05):
+                noise=0.05, keep_coords=False):
     """Wind-farm-like box: uniform in x and z, geometric stretching in y, an array of
     cylindrical towers, log-law mean flow with 10% fluctuations."""
     Lz = 1.0
@@ -494,7 +494,7 @@ def towers_case(nx=512, ny=256, nz=256, ratio=1.02, rows=3, cols=4, radius_cells
     norm = math.log((1.0 + y0) / y0)
     prof = lambda yc: np.log((np.maximum(yc, 0.0) + y0) / y0) / norm
     return build_case(f"towers{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, top=1.0, ib=ib, seed=seed,
-                      noise=noise, mean_profile_fn=prof, fluct=0.1)
+                      noise=noise, mean_profile_fn=prof, fluct=0.1, keep_coords=keep_coords)
 
 
 def batch_sizes(count=64, lo=32, hi=128, jitter=0.25, seed=3):

@@ -726,6 +726,7 @@ const char* vfs_pp_error_string(int code) {
     case VFS_PP_ENOCONV: return "solver did not converge";
     case VFS_PP_ESTATE: return "call order violated";
     case VFS_PP_ENONFINITE: return "non-finite values";
+    case VFS_PP_EIO: return "grid file unreadable";
     default: return "unknown";
   }
 }

@@ -84,6 +84,9 @@ def lib():
         "vfs_pp_set_geometry": (C.c_int, [H] + [dp] * 17),
         "vfs_pp_form_metrics_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 17),
         "vfs_pp_set_grid_device": (C.c_int, [H, vp, vp]),
+        "vfs_pp_grid_file_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 4),
+        "vfs_pp_read_grid_device": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                              C.POINTER(Grid), vp]),
         "vfs_pp_poisson_solve_device": (C.c_int, [H, vp, vp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
         "vfs_pp_update_pressure_device": (C.c_int, [H, vp, vp]),
         "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
@@ -483,6 +486,28 @@ def coarse_dmda_shape(grid: Grid):
 
 
 METRIC_FIELDS = GEOMETRY_FIELDS[:16]
+GRID_FORMATS = {"text": 0, "binary": 1, "xyz": 2}
+
+
+def grid_file_info(path, fmt="text", block=0):
+    """(nblocks, IM, JM, KM) of a grid file block (init.c:251-314 headers)."""
+    v = [C.c_int() for _ in range(4)]
+    _check(lib().vfs_pp_grid_file_info(os.fsencode(path), GRID_FORMATS[fmt], block, *[C.byref(x) for x in v]),
+           "vfs_pp_grid_file_info")
+    return tuple(x.value for x in v)
+
+
+def read_grid_device(path, fmt="text", block=0, cl=1.0, l_dim=1.0, device="cuda", **grid_kw):
+    """Read a grid-file block into device DMDA coordinates: returns (Grid, coor)
+    with coor of shape (KM+1, JM+1, IM+1, 3); grid_kw go to make_grid."""
+    import torch
+
+    _, im, jm, km = grid_file_info(path, fmt, block)
+    g = make_grid(im - 1, jm - 1, km - 1, **grid_kw)
+    coor = torch.empty(dmda_shape(g, vector=True), dtype=torch.float64, device=device)
+    _check(lib().vfs_pp_read_grid_device(os.fsencode(path), GRID_FORMATS[fmt], block, cl, l_dim, C.byref(g),
+                                         _tptr(coor, None, "coor")), "vfs_pp_read_grid_device")
+    return g, coor
 
 
 def form_metrics_device(grid: Grid, coor):
