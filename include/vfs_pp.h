@@ -118,6 +118,29 @@ int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta
                         const double *kaj, const double *aj, const double *csi,
                         const double *eta, const double *zet, const double *nvert);
 
+/* Immersed-body surfaces (the ibmdataNN files of ibm_io.c:78-300): vertex and
+ * triangle counts from the header. */
+int vfs_pp_ucd_info(const char *path, int *n_v, int *n_elmt);
+
+/* Reads an ibmdata surface into host arrays of n_v vertices (x = value / cl + cx,
+ * ibm_io.c:146-148; the reference's cl is 1/L_dim or -char_length_ibm, and cx..cz
+ * are -x_c/-y_c/-z_c) and n_elmt triangles (0-based vertex indices). */
+int vfs_pp_read_ucd(const char *path, double cl, double cx, double cy, double cz, double *x,
+                    double *y, double *z, int *nv1, int *nv2, int *nv3);
+
+/* ibm_search_advanced (ibm.c:81-570) for body `body` on the device: the ray-parity
+ * search of every interior cell centre against the triangulated surface (host
+ * arrays), then the dilation, remap and wall layers of
+ * vfs_pp_classify_nvert_device.  coor is the device DMDA coordinate array and
+ * nvert the device DMDA nvert array, updated in place; body 0 first resets the
+ * interior values below 5 (ibm.c:247), later bodies accumulate.  The
+ * interpolation list the reference also builds (ibm.c:392-525) serves the
+ * momentum step and is not produced.  Returns EINVAL if a ray meets more than
+ * 32 triangles or 64 directions all hit coincident points. */
+int vfs_pp_ib_search_device(const vfs_grid *grid, const double *coor, int n_v, const double *x,
+                            const double *y, const double *z, int n_elmt, const int *nv1,
+                            const int *nv2, const int *nv3, int body, double *nvert);
+
 /* Grid files as init.c:251-389 reads them (the reference's gridfile, -binary
  * and -xyz options): */
 #define VFS_PP_GRID_TEXT 0   /* grid.dat text: block count, per block "IM JM KM" then

@@ -62,6 +62,10 @@ def lib():
         _lib.or_gmres.argtypes = [C.c_int, ip, ip, d, d, d, C.c_int, C.c_double, C.c_int, C.c_int, d]
         _lib.or_num_threads.restype = C.c_int
         _lib.or_form_metrics.argtypes = [g, d, C.POINTER(d)]
+        _lib.or_ib_search.restype = C.c_int
+        _lib.or_ib_search.argtypes = [g, ip, d, C.c_int, d, d, d, C.c_int, ip, ip, ip, C.c_int, C.c_int, d]
+        _lib.or_ray_s.restype = C.c_double
+        _lib.or_ray_s.argtypes = [C.c_int]
         _lib.or_mg_create.restype = C.c_void_p
         _lib.or_mg_create.argtypes = [g, d, d, C.c_int, C.c_double, C.c_double, C.c_int]
         _lib.or_mg_destroy.argtypes = [C.c_void_p]
@@ -347,3 +351,25 @@ def form_metrics(co, per):
     ptrs = (C.POINTER(C.c_double) * 16)(*[_p(out[nm]) for nm in METRIC_NAMES])
     lib().or_form_metrics(C.byref(g), _p(co), ptrs)
     return out
+
+
+def ib_search(co, per, bctype, verts, tris, nvert, body=0, classify=True):
+    """or_ib_search: ibm_search_advanced for one triangulated body on DMDA node
+    coordinates co; updates and returns nvert (a copy)."""
+    co = np.ascontiguousarray(co, dtype=np.float64)
+    g = grid(co.shape[:3], per)
+    nv = np.ascontiguousarray(nvert, dtype=np.float64).copy()
+    V = np.ascontiguousarray(verts, dtype=np.float64)
+    T = np.ascontiguousarray(tris, dtype=np.int32)
+    x, y, z = (np.ascontiguousarray(V[:, q]) for q in range(3))
+    a, b, c = (np.ascontiguousarray(T[:, q]) for q in range(3))
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    rc = lib().or_ib_search(C.byref(g), _p(bc, C.c_int), _p(co), len(x), _p(x), _p(y), _p(z), len(a),
+                            _p(a, C.c_int), _p(b, C.c_int), _p(c, C.c_int), int(body), int(bool(classify)), _p(nv))
+    if rc:
+        raise RuntimeError(f"or_ib_search status {rc}")
+    return nv
+
+
+def ray_s(seed):
+    return lib().or_ray_s(int(seed))

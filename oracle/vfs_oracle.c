@@ -1529,3 +1529,150 @@ void or_form_metrics(const or_grid *g, const double *co, double *const out[16]) 
 #undef CP
   }
 }
+
+/* ------------------------------------------------------- IB ray-parity search */
+/* s of randomdirection (ibm.c:48-79): srand(seed), then rand() / (RAND_MAX + 1) - 0.5 */
+double or_ray_s(int seed) {
+  srand((unsigned)seed);
+  return rand() / ((double)RAND_MAX + 1) - 0.5;
+}
+
+/* Moller-Trumbore as compgeom.c:17-60 writes it (EPSILON 1e-15) */
+static int tri_hit(const double o[3], const double d[3], const double a[3], const double b[3], const double c[3],
+                   double *t) {
+  double e1[3], e2[3], tv[3], pv[3], qv[3];
+  e1[0] = b[0] - a[0]; e1[1] = b[1] - a[1]; e1[2] = b[2] - a[2];
+  e2[0] = c[0] - a[0]; e2[1] = c[1] - a[1]; e2[2] = c[2] - a[2];
+  pv[0] = d[1] * e2[2] - d[2] * e2[1];
+  pv[1] = d[2] * e2[0] - d[0] * e2[2];
+  pv[2] = d[0] * e2[1] - d[1] * e2[0];
+  double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+  if (det > -1.e-15 && det < 1.e-15) return 0;
+  double inv = 1.0 / det;
+  tv[0] = o[0] - a[0]; tv[1] = o[1] - a[1]; tv[2] = o[2] - a[2];
+  double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+  if (u < 0.0 || u > 1.0) return 0;
+  qv[0] = tv[1] * e1[2] - tv[2] * e1[1];
+  qv[1] = tv[2] * e1[0] - tv[0] * e1[2];
+  qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
+  double v = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * inv;
+  if (v < 0.0 || u + v > 1.0) return 0;
+  *t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+  return 1;
+}
+
+typedef struct { int *v; int n, cap; } or_bin;
+
+static void bin_insert(or_bin *b, int e) { /* insertnode, variables.c:7-27: once per bin */
+  for (int q = 0; q < b->n; q++)
+    if (b->v[q] == e) return;
+  if (b->n == b->cap) {
+    b->cap = b->cap ? 2 * b->cap : 8;
+    b->v = (int *)realloc(b->v, (size_t)b->cap * sizeof(int));
+  }
+  b->v[b->n++] = e;
+}
+
+int or_ib_search(const or_grid *g, const int bctype[6], const double *co, int n_v, const double *xb,
+                 const double *yb, const double *zb, int n_elmt, const int *nv1, const int *nv2, const int *nv3,
+                 int body, int classify, double *nvert) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const int ncx = 40, ncy = 40, ncz = 40; /* ibm.c:100 */
+  /* element normals, ibm_io.c:249-266 */
+  double *nf = (double *)malloc(3 * (size_t)(n_elmt > 0 ? n_elmt : 1) * sizeof(double));
+  for (int e = 0; e < n_elmt; e++) {
+    int a = nv1[e], b = nv2[e], c = nv3[e];
+    double dx12 = xb[b] - xb[a], dy12 = yb[b] - yb[a], dz12 = zb[b] - zb[a];
+    double dx13 = xb[c] - xb[a], dy13 = yb[c] - yb[a], dz13 = zb[c] - zb[a];
+    double nx = dy12 * dz13 - dz12 * dy13, ny = -dx12 * dz13 + dz12 * dx13, nz = dx12 * dy13 - dy12 * dx13;
+    double dr = sqrt(nx * nx + ny * ny + nz * nz);
+    nf[3 * e] = nx / dr; nf[3 * e + 1] = ny / dr; nf[3 * e + 2] = nz / dr;
+  }
+  /* bounding box + 0.05 and the 40^3 bins, ibm.c:128-207 */
+  double x0 = 1.e23, x1 = -1.e23, y0 = 1.e23, y1 = -1.e23, z0 = 1.e23, z1 = -1.e23;
+  for (int q = 0; q < n_v; q++) {
+    x0 = x0 < xb[q] ? x0 : xb[q]; x1 = x1 > xb[q] ? x1 : xb[q];
+    y0 = y0 < yb[q] ? y0 : yb[q]; y1 = y1 > yb[q] ? y1 : yb[q];
+    z0 = z0 < zb[q] ? z0 : zb[q]; z1 = z1 > zb[q] ? z1 : zb[q];
+  }
+  x0 -= 0.05; x1 += 0.05; y0 -= 0.05; y1 += 0.05; z0 -= 0.05; z1 += 0.05;
+  const double dcx = (x1 - x0) / (ncx - 1.), dcy = (y1 - y0) / (ncy - 1.), dcz = (z1 - z0) / (ncz - 1.);
+  or_bin *bins = (or_bin *)calloc((size_t)ncx * ncy * ncz, sizeof(or_bin));
+  for (int e = 0; e < n_elmt; e++) {
+    int a = nv1[e], b = nv2[e], c = nv3[e];
+#define MN3(p) ((p[a] < p[b] ? p[a] : p[b]) < p[c] ? (p[a] < p[b] ? p[a] : p[b]) : p[c])
+#define MX3(p) ((p[a] > p[b] ? p[a] : p[b]) > p[c] ? (p[a] > p[b] ? p[a] : p[b]) : p[c])
+    int i0 = (int)floor((MN3(xb) - x0) / dcx), i1 = (int)floor((MX3(xb) - x0) / dcx) + 1;
+    int j0 = (int)floor((MN3(yb) - y0) / dcy), j1 = (int)floor((MX3(yb) - y0) / dcy) + 1;
+    int k0 = (int)floor((MN3(zb) - z0) / dcz), k1 = (int)floor((MX3(zb) - z0) / dcz) + 1;
+#undef MN3
+#undef MX3
+    i0 = i0 < 0 ? 0 : i0; i1 = i1 > ncx ? ncx : i1;
+    j0 = j0 < 0 ? 0 : j0; j1 = j1 > ncx ? ncy : j1;
+    k0 = k0 < 0 ? 0 : k0; k1 = k1 > ncz ? ncz : k1;
+    for (int k = k0; k < k1; k++)
+      for (int j = j0; j < j1; j++)
+        for (int i = i0; i < i1; i++) bin_insert(&bins[(k * ncy + j) * ncx + i], e);
+  }
+  unsigned char *seen = (unsigned char *)malloc((size_t)(n_elmt > 0 ? n_elmt : 1));
+  double *ht = (double *)malloc(1000 * sizeof(double));
+  int *he = (int *)malloc(1000 * sizeof(int));
+  int status = 0;
+  /* search over interior cell centres, ibm.c:245-270 and point_cell_advanced 1777-1902 */
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        const size_t c = IX(g, i, j, k);
+        if (body == 0 && nvert[c] < 5.) nvert[c] = 0;
+        double p[3];
+        for (int v = 0; v < 3; v++)
+          p[v] = 0.125 * (co[3 * IX(g, i, j, k) + v] + co[3 * IX(g, i, j - 1, k) + v] +
+                          co[3 * IX(g, i, j, k - 1) + v] + co[3 * IX(g, i, j - 1, k - 1) + v] +
+                          co[3 * IX(g, i - 1, j, k) + v] + co[3 * IX(g, i - 1, j - 1, k) + v] +
+                          co[3 * IX(g, i - 1, j, k - 1) + v] + co[3 * IX(g, i - 1, j - 1, k - 1) + v]);
+        if (!(p[0] > x0 && p[0] < x1 && p[1] > y0 && p[1] < y1 && p[2] > z0 && p[2] < z1)) continue;
+        const int ic = (int)floor((p[0] - x0) / dcx), jc = (int)floor((p[1] - y0) / dcy),
+                  kc = (int)floor((p[2] - z0) / dcz);
+        double val = -1;
+        for (int st = 1; st <= 1000 && val < 0; st++) {
+          const double s = or_ray_s(st);
+          double d[3];
+          d[0] = (dcx * (ic + 0.5) + x0 + s * dcx) - p[0];
+          d[1] = (dcy * (jc + 0.5) + y0 + s * dcy) - p[1];
+          d[2] = (z1 + 0.2) - p[2];
+          double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          d[0] /= len; d[1] /= len; d[2] /= len;
+          memset(seen, 0, (size_t)(n_elmt > 0 ? n_elmt : 1));
+          int nh = 0, sing = 0;
+          for (int kk = kc; kk < ncz && !sing; kk++) {
+            const or_bin *b = &bins[(kk * ncy + jc) * ncx + ic];
+            for (int q = 0; q < b->n && !sing; q++) {
+              const int e = b->v[q];
+              if (seen[e]) continue;
+              seen[e] = 1;
+              const double A[3] = {xb[nv1[e]], yb[nv1[e]], zb[nv1[e]]};
+              const double B[3] = {xb[nv2[e]], yb[nv2[e]], zb[nv2[e]]};
+              const double Cc[3] = {xb[nv3[e]], yb[nv3[e]], zb[nv3[e]]};
+              double t;
+              if (tri_hit(p, d, A, B, Cc, &t) && t > 0) {
+                if (nh == 1000) { status = 1; sing = 1; break; }
+                for (int r = 0; r < nh; r++) {
+                  const double *n0 = &nf[3 * he[r]], *n1 = &nf[3 * e];
+                  const double nd = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
+                  if (fabs(t - ht[r]) < 1.e-8 && nd > -0.95) sing = 1;
+                }
+                ht[nh] = t; he[nh] = e; nh++;
+              }
+            }
+          }
+          if (!sing) val = (nh % 2) ? 4 : 0;
+        }
+        if (val < 0) { status = 2; val = 0; }
+        nvert[c] = nvert[c] < val ? val : nvert[c];
+        if (nvert[c] < 0) nvert[c] = 0;
+      }
+  for (int q = 0; q < ncx * ncy * ncz; q++) free(bins[q].v);
+  free(bins); free(seen); free(ht); free(he); free(nf);
+  if (classify) or_nvert_classify(g, bctype, nvert); /* ibm.c:325-353, 540-568 */
+  return status;
+}

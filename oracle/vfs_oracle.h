@@ -130,6 +130,18 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
  * zet (3 per point). */
 void or_form_metrics(const or_grid *g, const double *co, double *const out[16]);
 
+/* ibm_search_advanced (ibm.c:81-570) for body `body` of a triangulated surface
+ * (vertices x/y/z, 0-based triangles nv1..nv3): the ray-parity search of the
+ * interior cell centres (point_cell_advanced ibm.c:1777-1902, randomdirection
+ * ibm.c:48-79, intsect_triangle compgeom.c:17-60) marks inside cells 4; with
+ * `classify` the dilation, remap and wall layers of or_nvert_classify follow.
+ * co is the DMDA node-coordinate array.  Returns 0, or 1 / 2 when a ray met
+ * more than 1000 triangles / no direction was free of coincident hits. */
+int or_ib_search(const or_grid *g, const int bctype[6], const double *co, int n_v, const double *x,
+                 const double *y, const double *z, int n_elmt, const int *nv1, const int *nv2, const int *nv3,
+                 int body, int classify, double *nvert);
+double or_ray_s(int seed); /* randomdirection's s for srand(seed), ibm.c:64-66 */
+
 int or_num_threads(void);
 
 /* CPU port of the engine's solver (geometric multigrid + FGMRES/CGS2), on

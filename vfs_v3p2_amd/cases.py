@@ -304,6 +304,63 @@ def inside_towers(cent, xs, zs, radius, height):
     return m
 
 
+# ------------------------------------------------------- triangulated surfaces
+
+def icosphere(centre, r, level=3):
+    """Closed triangulated sphere (icosahedron subdivided `level` times), outward
+    orientation: (vertices (n, 3), triangles (m, 3) 0-based)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, float) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                v = V[a] + V[b]
+                V.append(v / np.linalg.norm(v))
+                mid[key] = len(V) - 1
+            return mid[key]
+
+        F = [f for a, b, c in F for f in ((a, m(a, b), m(c, a)), (b, m(b, c), m(a, b)), (c, m(c, a), m(b, c)),
+                                           (m(a, b), m(b, c), m(c, a)))]
+    return np.asarray(centre, float)[None, :] + r * np.array(V), np.array(F, dtype=np.int32)
+
+
+def cylinder_surface(xc, zc, r, y0, y1, nseg=24, nlayers=4):
+    """Closed cylinder about the y axis through (xc, zc), from y0 to y1, outward
+    orientation."""
+    a = 2 * np.pi * np.arange(nseg) / nseg
+    ys = np.linspace(y0, y1, nlayers + 1)
+    V = [(xc + r * np.cos(q), y, zc + r * np.sin(q)) for y in ys for q in a]
+    V += [(xc, y0, zc), (xc, y1, zc)]
+    bot, top = len(V) - 2, len(V) - 1
+    F = []
+    for L in range(nlayers):
+        for q in range(nseg):
+            p0, p1 = L * nseg + q, L * nseg + (q + 1) % nseg
+            F += [(p0, p0 + nseg, p1), (p1, p0 + nseg, p1 + nseg)]
+    for q in range(nseg):
+        F.append((bot, q, (q + 1) % nseg))
+        F.append((top, nlayers * nseg + (q + 1) % nseg, nlayers * nseg + q))
+    return np.array(V, float), np.array(F, dtype=np.int32)
+
+
+def write_ucd(path, verts, tris, title="vfs_pp surface"):
+    """ibmdata file as ibm_io.c:106-240 reads it (1-based triangle indices)."""
+    with open(path, "w") as f:
+        f.write(f"# {title}\n#\n#\n{len(verts)} {len(tris)} 0 0 0\n")
+        for q, v in enumerate(verts):
+            f.write(f"{q + 1} {float(v[0])!r} {float(v[1])!r} {float(v[2])!r}\n")
+        for q, t in enumerate(tris):
+            f.write(f"{q + 1} 0 tri {int(t[0]) + 1} {int(t[1]) + 1} {int(t[2]) + 1}\n")
+
+
 def interior_only(mask):
     m = np.zeros_like(mask)
     m[1:-1, 1:-1, 1:-1] = mask[1:-1, 1:-1, 1:-1]

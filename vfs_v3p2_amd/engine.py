@@ -84,6 +84,10 @@ def lib():
         "vfs_pp_set_geometry": (C.c_int, [H] + [dp] * 17),
         "vfs_pp_form_metrics_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 17),
         "vfs_pp_set_grid_device": (C.c_int, [H, vp, vp]),
+        "vfs_pp_ucd_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "vfs_pp_read_ucd": (C.c_int, [C.c_char_p] + [C.c_double] * 4 + [dp] * 3 + [C.POINTER(C.c_int)] * 3),
+        "vfs_pp_ib_search_device": (C.c_int, [C.POINTER(Grid), vp, C.c_int, dp, dp, dp, C.c_int] +
+                                    [C.POINTER(C.c_int)] * 3 + [C.c_int, vp]),
         "vfs_pp_grid_file_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 4),
         "vfs_pp_read_grid_device": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                               C.POINTER(Grid), vp]),
@@ -523,6 +527,34 @@ def form_metrics_device(grid: Grid, coor):
     _check(lib().vfs_pp_form_metrics_device(C.byref(grid), cp, *[C.c_void_p(out[k].data_ptr()) for k in METRIC_FIELDS]),
            "vfs_pp_form_metrics_device")
     return out
+
+
+def read_ucd(path, cl=1.0, centre=(0.0, 0.0, 0.0)):
+    """ibmdata surface file -> (vertices (n_v, 3), triangles (n_elmt, 3), 0-based)."""
+    nv, ne = C.c_int(), C.c_int()
+    _check(lib().vfs_pp_ucd_info(os.fsencode(path), C.byref(nv), C.byref(ne)), "vfs_pp_ucd_info")
+    xyz = [np.zeros(nv.value) for _ in range(3)]
+    tri = [np.zeros(ne.value, dtype=np.int32) for _ in range(3)]
+    _check(lib().vfs_pp_read_ucd(os.fsencode(path), cl, *centre, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in xyz],
+                                 *[t.ctypes.data_as(C.POINTER(C.c_int)) for t in tri]), "vfs_pp_read_ucd")
+    return np.stack(xyz, -1), np.stack(tri, -1)
+
+
+def ib_search_device(grid: Grid, coor, verts, tris, nvert, body=0):
+    """ibm_search_advanced for one triangulated body on the device: updates the
+    device nvert tensor in place (ray parity, dilation, remap, wall layers)."""
+    V = np.ascontiguousarray(verts, dtype=np.float64)
+    T = np.ascontiguousarray(tris, dtype=np.int32)
+    if V.ndim != 2 or V.shape[1] != 3 or T.ndim != 2 or T.shape[1] != 3:
+        raise ValueError("ib_search_device: verts and tris must be (n, 3)")
+    x, y, z = (np.ascontiguousarray(V[:, q]) for q in range(3))
+    a, b, c = (np.ascontiguousarray(T[:, q]) for q in range(3))
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    _check(lib().vfs_pp_ib_search_device(C.byref(grid), _tptr(coor, dmda_shape(grid, vector=True), "coor"), len(x),
+                                         x.ctypes.data_as(dp), y.ctypes.data_as(dp), z.ctypes.data_as(dp), len(a),
+                                         a.ctypes.data_as(ip), b.ctypes.data_as(ip), c.ctypes.data_as(ip), int(body),
+                                         _tptr(nvert, dmda_shape(grid), "nvert")), "vfs_pp_ib_search_device")
 
 
 def classify_nvert_device(grid: Grid, nvert_dev):
