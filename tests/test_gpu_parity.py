@@ -370,3 +370,4 @@ def test_divergence_rule_drives_the_solve(E, torch_cuda):
     assert out["loose"].max_div > 1e-8  # rtol alone does not get there on this grid
     assert out["ruled"].converged and out["ruled"].max_div <= 1e-8
     assert out["ruled"].iterations > out["loose"].iterations
+

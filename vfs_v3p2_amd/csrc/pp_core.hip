@@ -5,6 +5,8 @@
 // index wrap, so no ghost planes need refreshing between sweeps.  Caller fields
 // (Ucont, Phi, P, Ucat, nvert) stay in the reference's DMDA layout and are only
 // touched by the kernels that read or write them in the reference.
+#include <algorithm>
+
 #include "pp_kernels.hpp"
 #include "pp_stencil.hpp"
 #include "pp_krylov_dev.hpp"
@@ -678,6 +680,38 @@ __global__ __launch_bounds__(256) void k_axpy_dots(size_t n, double* __restrict_
   block_dots(t < n, t, w1, V, ldv, nv, partial, gridDim.x, blockIdx.x, lds);
 }
 
+// CGS2 second pass with the block reduction amortised over a grid-stride range
+// of elements per thread: the basis values are loaded once into registers for
+// both w1 = w - V h and the (nv + 1) dot products, which stay in registers
+// (nv + 1 <= S) and are reduced once per block.
+template <int S>
+__global__ __launch_bounds__(256) void k_axpy_dots_acc(size_t n, double* __restrict__ w, const double* __restrict__ V,
+                                                       size_t ldv, int nv, const double* __restrict__ h,
+                                                       double* __restrict__ partial) {
+  __shared__ double lds[4][64];
+  double acc[S];
+#pragma unroll
+  for (int l = 0; l < S; l++) acc[l] = 0.;
+  const size_t stride = (size_t)gridDim.x * 256;
+#pragma unroll 1
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) {
+    double vv[S];
+#pragma unroll
+    for (int l = 0; l < S; l++) vv[l] = l < nv ? V[(size_t)l * ldv + t] : 0.;
+    double w1 = w[t];
+#pragma unroll
+    for (int l = 0; l < S; l++)
+      if (l < nv) w1 -= h[l] * vv[l];
+    w[t] = w1;
+#pragma unroll
+    for (int l = 0; l < S; l++) {
+      if (l < nv) acc[l] += w1 * vv[l];
+      else if (l == nv) acc[l] += w1 * w1;
+    }
+  }
+  block_sums(acc, nv + 1, partial, gridDim.x, blockIdx.x, lds);
+}
+
 __global__ void k_gs_norm(double* __restrict__ red, int off, int nv, double* __restrict__ host) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   gs_norm_one(red, off, nv, host);
@@ -961,8 +995,16 @@ template void launch_apply_dots<double>(hipStream_t, Dims, const double*, const 
 
 void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
                       double* partial, int* nblk) {
-  *nblk = (int)nb1(n);
-  k_axpy_dots<<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+  if (nv + 1 <= 32) {
+    const unsigned nb = std::min<unsigned>(nb1(n), 2048u);
+    *nblk = (int)nb;
+    if (nv + 1 <= 8) k_axpy_dots_acc<8><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+    else if (nv + 1 <= 16) k_axpy_dots_acc<16><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+    else k_axpy_dots_acc<32><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+  } else {
+    *nblk = (int)nb1(n);
+    k_axpy_dots<<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, partial);
+  }
   HIPCHECK(hipGetLastError());
 }
 

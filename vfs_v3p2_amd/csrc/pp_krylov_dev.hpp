@@ -40,6 +40,26 @@ __device__ inline void block_dots(bool in, size_t c, double a, const double* __r
   __syncthreads();  // lds is reused by the block's next tile
 }
 
+// Per-block sums of per-thread accumulators acc[0..ns) (ns <= S <= 64), written to
+// partial[l * nblk + b]: wave sums by shuffles, the four wave results of a
+// 256-thread block meet in LDS.  Every thread of the block must call it.
+template <int S>
+__device__ inline void block_sums(const double (&acc)[S], int ns, double* __restrict__ partial, size_t nblk,
+                                  size_t b, double (*lds)[64]) {
+  const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int l = 0; l < S; l++) {
+    if (l < ns) {
+      const double v = wave_sum(acc[l]);
+      if (lane == 0) lds[w][l] = v;
+    }
+  }
+  __syncthreads();
+  if (t < ns) partial[(size_t)t * nblk + b] = (lds[0][t] + lds[1][t]) + (lds[2][t] + lds[3][t]);
+  __syncthreads();  // lds is reused by the block's next call
+}
+
 // w[c] = (A z)[c] in flux form on a row cell, 0 elsewhere; returns w[c]
 template <typename TZ>
 __device__ __forceinline__ double apply_cell(const Dims& d, const double* __restrict__ g,
