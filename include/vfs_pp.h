@@ -78,10 +78,12 @@ typedef struct {
                         calls, poisson_hypre.c:565-574, 722) */
   double cheb_lo;    /* Chebyshev smoothing interval [cheb_lo, cheb_hi] x lambda_max(M^-1 A) */
   double cheb_hi;
-  int orthog;        /* Gram-Schmidt of the Krylov basis, both classical with
-                        re-orthogonalisation (CGS2): 0 fused into three passes over
-                        the basis (the first with the operator apply); 1 plain,
-                        one pass per step */
+  int orthog;        /* Gram-Schmidt of the Krylov basis (default 2): 0 classical with
+                        re-orthogonalisation (CGS2) fused into three passes over the
+                        basis (the first with the operator apply); 1 CGS2, one pass
+                        per step; 2 one classical pass on a lazily normalised basis
+                        (fused with the operator and with the next V-cycle's input),
+                        repeated only when it cancels |w| below 1e-4 |w| */
   int verbose;       /* 1: print the residual history of each solve to stderr */
 } vfs_pp_opts;
 

@@ -297,8 +297,12 @@ __global__ __launch_bounds__(256) void kb_axpy_dots(const VecItem<T>* __restrict
     const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
     const bool in = e < I.n;
     double* w = I.V + (size_t)(st.x + 1) * I.n;
-    const double w1 = in ? gs_sub(w, I.V, I.n, st.x + 1, I.red + 8, e) : 0.;
-    block_dots(in, e, w1, I.V, I.n, st.x + 1, partial, total, t, lds);
+    if (st.x + 1 < 16) {
+      gs_sub_block_dots<16>(in, w, I.V, I.n, st.x + 1, I.red + 8, e, partial, total, t, lds);
+    } else {
+      const double w1 = in ? gs_sub(w, I.V, I.n, st.x + 1, I.red + 8, e) : 0.;
+      block_dots(in, e, w1, I.V, I.n, st.x + 1, partial, total, t, lds);
+    }
   }
 }
 

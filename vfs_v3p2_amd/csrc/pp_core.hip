@@ -712,6 +712,37 @@ __global__ __launch_bounds__(256) void k_axpy_dots_acc(size_t n, double* __restr
   block_sums(acc, nv + 1, partial, gridDim.x, blockIdx.x, lds);
 }
 
+// Power of two near 1/||w|| from ||w||^2, the scale of the single-precision copy
+// of an unnormalised basis vector (the host computes the same value).
+__host__ __device__ inline double conv_scale(double a2) {
+  if (!(a2 > 0.)) return 1.;
+  const int e = ilogb(a2);
+  return ldexp(1.0, -(int)floor(0.5 * e));
+}
+
+// Lazily normalised Gram-Schmidt pass: w' = w - sum_l c_l U_l in place with
+// c_l = dots[l] * nuinv2[l] (the projection onto U_l / |U_l|), f = (TF)(w' s) with
+// s = conv_scale(*a2) for the next V-cycle, and per-block partial sums of |w'|^2.
+template <typename TF>
+__global__ __launch_bounds__(256) void k_axpy_norm(size_t n, double* __restrict__ w, const double* __restrict__ V,
+                                                   size_t ldv, int nv, const double* __restrict__ dots,
+                                                   const double* __restrict__ nuinv2, const double* __restrict__ a2,
+                                                   TF* __restrict__ f, double* __restrict__ partial) {
+  __shared__ double lds[4][64];
+  const double sc = conv_scale(a2[0]);
+  double acc[1] = {0.};
+  const size_t stride = (size_t)gridDim.x * 256;
+#pragma unroll 1
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) {
+    double v = w[t];
+    for (int l = 0; l < nv; l++) v -= (dots[l] * nuinv2[l]) * V[(size_t)l * ldv + t];
+    w[t] = v;
+    f[t] = (TF)(v * sc);
+    acc[0] += v * v;
+  }
+  block_sums(acc, 1, partial, gridDim.x, blockIdx.x, lds);
+}
+
 __global__ void k_gs_norm(double* __restrict__ red, int off, int nv, double* __restrict__ host) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   gs_norm_one(red, off, nv, host);
@@ -1007,6 +1038,21 @@ void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_
   }
   HIPCHECK(hipGetLastError());
 }
+
+template <typename TF>
+void launch_axpy_norm(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* dots,
+                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk) {
+  const unsigned nb = std::min<unsigned>(nb1(n), 2048u);
+  *nblk = (int)nb;
+  k_axpy_norm<TF><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, dots, nuinv2, a2, f, partial);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_axpy_norm<float>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
+                                      const double*, const double*, float*, double*, int*);
+template void launch_axpy_norm<double>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
+                                       const double*, const double*, double*, double*, int*);
+
+double host_conv_scale(double a2) { return conv_scale(a2); }
 
 void launch_gs_norm(hipStream_t s, double* red, int off, int nv, double* host) {
   k_gs_norm<<<1, 64, 0, s>>>(red, off, nv, host);

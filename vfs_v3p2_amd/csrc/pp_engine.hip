@@ -4,7 +4,8 @@
 //
 // Solve structure (one call of vfs_pp_project_device):
 //   VolumeFlux on a compact list of IB faces -> RHS b (FP64) -> FGMRES(m) in FP64
-//   with CGS2 orthogonalisation, right-preconditioned by one V-cycle of a
+//   with classical Gram-Schmidt on a lazily normalised basis (or CGS2),
+//   right-preconditioned by one V-cycle of a
 //   geometric multigrid in FP32 (mixed) or FP64 -> Remove_Nullspace -> scatter
 //   phi to the DMDA array and P += phi -> Projection -> Contra2Cart ->
 //   Divergence.
@@ -370,6 +371,9 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
   const double rtol = h->opts.rtol;
   TZ* Z = static_cast<TZ*>(h->Z);
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hcol(m + 2);
+  // lazily normalised basis (orthog 2): U_l = nu_l v_l, V-cycle inputs scaled so
+  // that the stored Z_l = sigma_l z_l
+  std::vector<double> nu(m + 1, 1.), sigma(m + 1, 1.);
   int it = 0;
   double rel = 1.;
   *conv = false;
@@ -388,6 +392,12 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
     if (it >= h->opts.max_iters) break;
     // v_0 = r / beta and its copy as the first V-cycle input
     launch_scale_conv<TZ>(h->s, n, h->r, h->red, h->V, static_cast<TZ*>(h->mg->f0()));
+    if (h->opts.orthog == 2) {
+      nu[0] = 1.;
+      sigma[0] = 1.;
+      h->hpin[RED_NUINV2] = 1.;
+      h2d(h->s, h->red + RED_NUINV2, h->hpin + RED_NUINV2, 1);
+    }
     std::fill(gv.begin(), gv.end(), 0.);
     gv[0] = beta;
     int j = 0;
@@ -398,7 +408,38 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
       h->mg->vcycle(h->s);
       (*vcyc)++;
       double sub = 0.;
-      if (h->opts.orthog == 1) {
+      if (h->opts.orthog == 2) {
+        // one classical Gram-Schmidt pass on the unnormalised basis, fused with the
+        // operator (dots) and with the next V-cycle's input (axpy + norm).  One pass
+        // leaves w' orthogonal to the basis to about eps |w| / |w'|, so a second pass
+        // runs only when the first cancelled |w'| below 1e-4 |w|.
+        const int nv = j + 1;
+        need_partials(h, (size_t)j + 2, std::max(h->stencil_blocks, (n + 255) / 256));
+        launch_apply_dots<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->V, n, nv, h->partial, &nb);
+        launch_finalize(h->s, h->partial, nv + 1, nb, h->red + 8, 0, h->hpin_dev + 8);
+        launch_axpy_norm<TZ>(h->s, n, w, h->V, n, nv, h->red + 8, h->red + RED_NUINV2, h->red + 8 + nv,
+                             static_cast<TZ*>(h->mg->f0()), h->partial, &nb);
+        launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+        HIPCHECK(hipStreamSynchronize(h->s));
+        const double a2 = h->hpin[8 + nv];
+        double b2 = h->hpin[0];
+        for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] / (nu[l] * sigma[j]);
+        if (b2 < 1e-8 * a2) {
+          launch_multidot(h->s, n, w, h->V, n, nv, h->partial, &nb);
+          launch_finalize(h->s, h->partial, nv, nb, h->red + 72, 0, h->hpin_dev + 72);
+          launch_axpy_norm<TZ>(h->s, n, w, h->V, n, nv, h->red + 72, h->red + RED_NUINV2, h->red + 8 + nv,
+                               static_cast<TZ*>(h->mg->f0()), h->partial, &nb);
+          launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+          HIPCHECK(hipStreamSynchronize(h->s));
+          for (int l = 0; l <= j; l++) H[(size_t)l * m + j] += h->hpin[72 + l] / (nu[l] * sigma[j]);
+          b2 = h->hpin[0];
+        }
+        nu[j + 1] = std::sqrt(b2);
+        sigma[j + 1] = host_conv_scale(a2) * nu[j + 1];
+        sub = nu[j + 1] / sigma[j];
+        h->hpin[RED_NUINV2 + j + 1] = b2 > 0. ? 1. / b2 : 0.;
+        h2d(h->s, h->red + RED_NUINV2 + j + 1, h->hpin + RED_NUINV2 + j + 1, 1);
+      } else if (h->opts.orthog == 1) {
         // two-pass classical Gram-Schmidt (CGS2)
         launch_apply64<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, false);
         for (int pass = 0; pass < 2; pass++) {
@@ -457,7 +498,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
       for (int q = l + 1; q < j; q++) s -= H[(size_t)l * m + q] * y[q];
       y[l] = s / H[(size_t)l * m + l];
     }
-    for (int l = 0; l < j; l++) h->hpin[l] = y[l];
+    for (int l = 0; l < j; l++) h->hpin[l] = h->opts.orthog == 2 ? y[l] / sigma[l] : y[l];
     h2d(h->s, h->red + 200, h->hpin, (size_t)j);
     launch_update_x<TZ>(h->s, n, h->x, Z, n, j, h->red + 200);
   }
@@ -713,7 +754,7 @@ void vfs_pp_default_opts(vfs_pp_opts* o) {
   o->zero_guess = 1;
   o->cheb_lo = 0.1;
   o->cheb_hi = 1.1;
-  o->orthog = 0;
+  o->orthog = 2;
   o->verbose = 0;
 }
 

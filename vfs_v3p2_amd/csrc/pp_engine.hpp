@@ -235,6 +235,7 @@ constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
 constexpr int RED_B2 = 320;    // ||b||^2
 constexpr int RED_DIV = 330;   // max |aj div u|
 constexpr int RED_DIVEST = 340;  // max |aj r| at the last true-residual check
+constexpr int RED_NUINV2 = 400;  // 1 / |U_l|^2 of the lazily normalised basis (up to 64)
 
 // engine functions shared with the batched engine (pp_engine.hip)
 double now_ms();

@@ -92,6 +92,10 @@ void launch_multidot(hipStream_t s, size_t n, const double* w, const double* V, 
 void launch_axpy_multi(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
                        double sign);
 void launch_norm2(hipStream_t s, size_t n, const double* a, double* partial, int* nblk);
+template <typename TF>
+void launch_axpy_norm(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* dots,
+                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk);
+double host_conv_scale(double a2);
 void launch_scale_to(hipStream_t s, size_t n, const double* a, const double* norm2, double* out);
 template <typename TZ>
 void launch_update_x(hipStream_t s, size_t n, double* x, const TZ* Z, size_t ldz, int nz, const double* y);

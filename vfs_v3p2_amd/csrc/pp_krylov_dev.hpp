@@ -60,6 +60,51 @@ __device__ inline void block_sums(const double (&acc)[S], int ns, double* __rest
   __syncthreads();  // lds is reused by the block's next call
 }
 
+// gs_sub followed by block_dots of the result at element t, with the basis
+// values V_l[t] (l < nv <= S) loaded once into registers for both; the same
+// arithmetic as the two calls.  Every thread of the block must call it.
+template <int S>
+__device__ inline void gs_sub_block_dots(bool in, double* __restrict__ w, const double* __restrict__ V, size_t ldv,
+                                         int nv, const double* __restrict__ h, size_t t,
+                                         double* __restrict__ partial, size_t nblk, size_t b, double (*lds)[64]) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int lane = tid & 63, wv = tid >> 6;
+  double vv[S];
+#pragma unroll
+  for (int l = 0; l < S; l++) vv[l] = (in && l < nv) ? V[(size_t)l * ldv + t] : 0.;
+  double w1 = 0.;
+  if (in) {
+    w1 = w[t];
+#pragma unroll
+    for (int l = 0; l < S; l++)
+      if (l < nv) w1 -= h[l] * vv[l];
+    w[t] = w1;
+  }
+#pragma unroll
+  for (int l0 = 0; l0 <= S; l0 += 8) {
+    if (l0 > nv) break;
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int l = l0 + q;
+      double v = 0.;
+      if (in && l < nv && l < S) v = w1 * vv[l < S ? l : 0];
+      else if (in && l == nv) v = w1 * w1;
+      p[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) p[q] = wave_sum(p[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (l0 + q <= nv) lds[wv][l0 + q] = p[q];
+    }
+  }
+  __syncthreads();
+  if (tid <= nv) partial[(size_t)tid * nblk + b] = (lds[0][tid] + lds[1][tid]) + (lds[2][tid] + lds[3][tid]);
+  __syncthreads();  // lds is reused by the block's next tile
+}
+
 // w[c] = (A z)[c] in flux form on a row cell, 0 elsewhere; returns w[c]
 template <typename TZ>
 __device__ __forceinline__ double apply_cell(const Dims& d, const double* __restrict__ g,
