@@ -120,6 +120,32 @@ int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta
                         const double *kaj, const double *aj, const double *csi,
                         const double *eta, const double *zet, const double *nvert);
 
+/* Pressure_Gradient (momentum.c:223-452), the step after the projection: the
+ * contravariant pressure-gradient flux dP (AoS, DMDA) on the i, j and k faces of
+ * every interior cell from P, with one-sided tangential differences next to
+ * nvert == 1 cells and non-periodic walls; zero on faces without a fluid pair and
+ * on boundary points.  All arrays on the device in the DMDA layout; P first gets
+ * its periodic ghost copies in place, as the reference does.  Levelset density
+ * and the -dpdz / -flux forcing terms (off by default) are not applied.
+ * Bit-exact with the reference's arithmetic. */
+int vfs_pp_pressure_gradient_device(const vfs_grid *grid, const double *icsi, const double *ieta,
+                                    const double *izet, const double *jcsi, const double *jeta,
+                                    const double *jzet, const double *kcsi, const double *keta,
+                                    const double *kzet, const double *iaj, const double *jaj,
+                                    const double *kaj, const double *nvert, double *p, double *dp);
+
+/* PETSc binary vector files, the form of the vfield / pfield / nvfield / ufield
+ * checkpoints (VecView / VecLoad on a binary viewer, main.c:291-335, 780-797,
+ * 865-935): big-endian int32 class id 1211214, big-endian int32 length, then
+ * big-endian float64 values; a one-rank DMDA vector is in the natural
+ * [k][j][i][component] order of the DMDA arrays here.  _info reads the length;
+ * _read fills a device array of exactly that length; _write stores a device
+ * array (and, for block_size > 1, the "-vecload_block_size" companion file
+ * PETSc writes next to it). */
+int vfs_pp_petsc_vec_info(const char *path, long long *n);
+int vfs_pp_read_petsc_vec_device(const char *path, double *v, size_t n);
+int vfs_pp_write_petsc_vec_device(const char *path, const double *v, size_t n, int block_size);
+
 /* Immersed-body surfaces (the ibmdataNN files of ibm_io.c:78-300): vertex and
  * triangle counts from the header. */
 int vfs_pp_ucd_info(const char *path, int *n_v, int *n_elmt);

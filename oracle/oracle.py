@@ -64,6 +64,7 @@ def lib():
         _lib.or_form_metrics.argtypes = [g, d, C.POINTER(d)]
         _lib.or_ib_search.restype = C.c_int
         _lib.or_ib_search.argtypes = [g, ip, d, C.c_int, d, d, d, C.c_int, ip, ip, ip, C.c_int, C.c_int, d]
+        _lib.or_pressure_gradient.argtypes = [g] + [d] * 15
         _lib.or_ray_s.restype = C.c_double
         _lib.or_ray_s.argtypes = [C.c_int]
         _lib.or_mg_create.restype = C.c_void_p
@@ -373,3 +374,14 @@ def ib_search(co, per, bctype, verts, tris, nvert, body=0, classify=True):
 
 def ray_s(seed):
     return lib().or_ray_s(int(seed))
+
+
+def pressure_gradient(g, f, p):
+    """or_pressure_gradient: (dP (mz,my,mx,3), p with periodic ghosts) from P and the
+    face metrics in f."""
+    pp = np.ascontiguousarray(p, dtype=np.float64).copy()
+    dp = np.zeros(pp.shape + (3,))
+    a = [np.ascontiguousarray(f[k]) for k in ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet",
+                                              "iaj", "jaj", "kaj", "nvert")]
+    lib().or_pressure_gradient(C.byref(g), *[_p(x) for x in a], _p(pp), _p(dp))
+    return dp, pp

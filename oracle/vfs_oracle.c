@@ -1676,3 +1676,83 @@ int or_ib_search(const or_grid *g, const int bctype[6], const double *co, int n_
   if (classify) or_nvert_classify(g, bctype, nvert); /* ibm.c:325-353, 540-568 */
   return status;
 }
+
+/* ---------------------------------------------------------- Pressure_Gradient */
+/* momentum.c:223-452 with levelset off, no -dpdz forcing and no -flux inlet term
+ * (their defaults, main.c:73 and 95): the contravariant pressure-gradient flux
+ * dP on the i, j, k faces of every interior cell, with the one-sided tangential
+ * differences next to nvert == 1 cells and non-periodic walls.  p gets its
+ * periodic ghost copies first, as the reference does to lP. */
+void or_pressure_gradient(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                          const double *jcsi, const double *jeta, const double *jzet, const double *kcsi,
+                          const double *keta, const double *kzet, const double *iaj, const double *jaj,
+                          const double *kaj, const double *nvert, double *p, double *dp) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  const int pi = g->per[0], pj = g->per[1], pk = g->per[2];
+  periodic_copy_scalar(g, p);
+  memset(dp, 0, 3 * (size_t)mx * my * mz * sizeof(double));
+#define P(k, j, i) p[IX(g, i, j, k)]
+#define N1(k, j, i) ((int)(nvert[IX(g, i, j, k)] + 0.5) == 1)
+#define DOT(a, b) (a[3 * c] * b[3 * c] + a[3 * c + 1] * b[3 * c + 1] + a[3 * c + 2] * b[3 * c + 2])
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        const size_t c = IX(g, i, j, k);
+        const double g11 = DOT(icsi, icsi), g12 = DOT(ieta, icsi), g13 = DOT(izet, icsi);
+        const double g21 = DOT(jcsi, jeta), g22 = DOT(jeta, jeta), g23 = DOT(jzet, jeta);
+        const double g31 = DOT(kcsi, kzet), g32 = DOT(keta, kzet), g33 = DOT(kzet, kzet);
+        double dpdc, dpde, dpdz;
+        /* i face */
+        if (i == mx - 2 && pi) dpdc = P(k, j, 1) - P(k, j, i);
+        else dpdc = P(k, j, i + 1) - P(k, j, i);
+        if (N1(k, j + 1, i) || N1(k, j + 1, i + 1) || (j == my - 2 && !pj))
+          dpde = (P(k, j, i) - P(k, j - 1, i) + P(k, j, i + 1) - P(k, j - 1, i + 1)) * 0.5;
+        else if (N1(k, j - 1, i) || N1(k, j - 1, i + 1) || (j == 1 && !pj))
+          dpde = (P(k, j + 1, i) - P(k, j, i) + P(k, j + 1, i + 1) - P(k, j, i + 1)) * 0.5;
+        else dpde = (P(k, j + 1, i) - P(k, j - 1, i) + P(k, j + 1, i + 1) - P(k, j - 1, i + 1)) * 0.25;
+        if (N1(k + 1, j, i) || N1(k + 1, j, i + 1) || (k == mz - 2 && !pk))
+          dpdz = (P(k, j, i) - P(k - 1, j, i) + P(k, j, i + 1) - P(k - 1, j, i + 1)) * 0.5;
+        else if (N1(k - 1, j, i) || N1(k - 1, j, i + 1) || (k == 1 && !pk))
+          dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j, i + 1) - P(k, j, i + 1)) * 0.5;
+        else dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j, i + 1) - P(k - 1, j, i + 1)) * 0.25;
+        double dx = (dpdc * g11 + dpde * g12 + dpdz * g13) * iaj[c];
+        /* j face */
+        if (N1(k, j, i + 1) || N1(k, j + 1, i + 1) || (i == mx - 2 && !pi))
+          dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k, j + 1, i) - P(k, j + 1, i - 1)) * 0.5;
+        else if (N1(k, j, i - 1) || N1(k, j + 1, i - 1) || (i == 1 && !pi))
+          dpdc = (P(k, j, i + 1) - P(k, j, i) + P(k, j + 1, i + 1) - P(k, j + 1, i)) * 0.5;
+        else dpdc = (P(k, j, i + 1) - P(k, j, i - 1) + P(k, j + 1, i + 1) - P(k, j + 1, i - 1)) * 0.25;
+        if (j == my - 2 && pj) dpde = P(k, 1, i) - P(k, j, i);
+        else dpde = P(k, j + 1, i) - P(k, j, i);
+        if (N1(k + 1, j, i) || N1(k + 1, j + 1, i) || (k == mz - 2 && !pk))
+          dpdz = (P(k, j, i) - P(k - 1, j, i) + P(k, j + 1, i) - P(k - 1, j + 1, i)) * 0.5;
+        else if (N1(k - 1, j, i) || N1(k - 1, j + 1, i) || (k == 1 && !pk))
+          dpdz = (P(k + 1, j, i) - P(k, j, i) + P(k + 1, j + 1, i) - P(k, j + 1, i)) * 0.5;
+        else dpdz = (P(k + 1, j, i) - P(k - 1, j, i) + P(k + 1, j + 1, i) - P(k - 1, j + 1, i)) * 0.25;
+        double dy = (dpdc * g21 + dpde * g22 + dpdz * g23) * jaj[c];
+        /* k face */
+        if (N1(k, j, i + 1) || N1(k + 1, j, i + 1) || (i == mx - 2 && !pi))
+          dpdc = (P(k, j, i) - P(k, j, i - 1) + P(k + 1, j, i) - P(k + 1, j, i - 1)) * 0.5;
+        else if (N1(k, j, i - 1) || N1(k + 1, j, i - 1) || (i == 1 && !pi))
+          dpdc = (P(k, j, i + 1) - P(k, j, i) + P(k + 1, j, i + 1) - P(k + 1, j, i)) * 0.5;
+        else dpdc = (P(k, j, i + 1) - P(k, j, i - 1) + P(k + 1, j, i + 1) - P(k + 1, j, i - 1)) * 0.25;
+        if (N1(k, j + 1, i) || N1(k + 1, j + 1, i) || (j == my - 2 && !pj))
+          dpde = (P(k, j, i) - P(k, j - 1, i) + P(k + 1, j, i) - P(k + 1, j - 1, i)) * 0.5;
+        else if (N1(k, j - 1, i) || N1(k + 1, j - 1, i) || (j == 1 && !pj))
+          dpde = (P(k, j + 1, i) - P(k, j, i) + P(k + 1, j + 1, i) - P(k + 1, j, i)) * 0.5;
+        else dpde = (P(k, j + 1, i) - P(k, j - 1, i) + P(k + 1, j + 1, i) - P(k + 1, j - 1, i)) * 0.25;
+        if (k == mz - 2 && pk) dpdz = P(1, j, i) - P(k, j, i);
+        else dpdz = P(k + 1, j, i) - P(k, j, i);
+        double dz = (dpdc * g31 + dpde * g32 + dpdz * g33) * kaj[c];
+        /* faces without a fluid pair or on a non-periodic wall carry none (momentum.c:426-436) */
+        if (nvert[c] + nvert[IX(g, i + 1, j, k)] > 0.1 || (!pi && i == mx - 2)) dx = 0;
+        if (nvert[c] + nvert[IX(g, i, j + 1, k)] > 0.1 || (!pj && j == my - 2)) dy = 0;
+        if (nvert[c] + nvert[IX(g, i, j, k + 1)] > 0.1 || (!pk && k == mz - 2)) dz = 0;
+        dp[3 * c] = dx;
+        dp[3 * c + 1] = dy;
+        dp[3 * c + 2] = dz;
+      }
+#undef P
+#undef N1
+#undef DOT
+}

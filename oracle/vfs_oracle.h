@@ -142,6 +142,13 @@ int or_ib_search(const or_grid *g, const int bctype[6], const double *co, int n_
                  int body, int classify, double *nvert);
 double or_ray_s(int seed); /* randomdirection's s for srand(seed), ibm.c:64-66 */
 
+/* Pressure_Gradient (momentum.c:223-452; levelset, -dpdz and -flux terms off):
+ * dp (AoS, DMDA) from P; p gets its periodic ghost copies in place. */
+void or_pressure_gradient(const or_grid *g, const double *icsi, const double *ieta, const double *izet,
+                          const double *jcsi, const double *jeta, const double *jzet, const double *kcsi,
+                          const double *keta, const double *kzet, const double *iaj, const double *jaj,
+                          const double *kaj, const double *nvert, double *p, double *dp);
+
 int or_num_threads(void);
 
 /* CPU port of the engine's solver (geometric multigrid + FGMRES/CGS2), on

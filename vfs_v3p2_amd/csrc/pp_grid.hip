@@ -29,7 +29,7 @@ namespace pp {
 namespace {
 
 [[noreturn]] void io_error(const std::string& path, const std::string& what) {
-  throw StatusError(VFS_PP_EIO, "grid file " + path + ": " + what);
+  throw StatusError(VFS_PP_EIO, "file " + path + ": " + what);
 }
 
 inline bool is_ws(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
@@ -308,6 +308,93 @@ int vfs_pp_read_grid_device(const char* path, int format, int block, double cl, 
   k_grid_layout<<<dim3((mx + 63) / 64, (my + 3) / 4, mz), dim3(64, 4, 1), 0, nullptr>>>(src, coor);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipDeviceSynchronize());
+  return VFS_PP_OK;
+  ABI_END
+}
+
+// PETSc binary vector files, the form of the vfield / pfield / nvfield / ufield
+// checkpoints (written with VecView and read with VecLoad on a binary viewer,
+// main.c:291-335, 780-797, 865-935): big-endian int32 class id 1211214,
+// big-endian int32 length, then the values as big-endian float64.  A DMDA vector
+// of one rank is stored in natural order, the [k][j][i][component] layout of the
+// engine's DMDA arrays.
+namespace {
+
+constexpr int32_t kVecClassId = 1211214;
+
+inline uint32_t be32(uint32_t v) { return __builtin_bswap32(v); }
+inline uint64_t be64(uint64_t v) { return __builtin_bswap64(v); }
+
+long long petsc_vec_header(const std::string& path, FILE* f) {
+  uint32_t h[2];
+  if (std::fread(h, 4, 2, f) != 2) pp::io_error(path, "short PETSc vector header");
+  if ((int32_t)be32(h[0]) != kVecClassId) pp::io_error(path, "not a PETSc binary vector (class id)");
+  const int32_t n = (int32_t)be32(h[1]);
+  if (n < 0) pp::io_error(path, "negative vector length");
+  return n;
+}
+
+}  // namespace
+
+int vfs_pp_petsc_vec_info(const char* path, long long* n) {
+  ABI_BEGIN
+  if (!path || !n) return VFS_PP_EINVAL;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) pp::io_error(path, "cannot open");
+  long long len = 0;
+  try {
+    len = petsc_vec_header(path, f);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  *n = len;
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_read_petsc_vec_device(const char* path, double* v, size_t n) {
+  ABI_BEGIN
+  if (!path || !v) return VFS_PP_EINVAL;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) pp::io_error(path, "cannot open");
+  std::vector<uint64_t> buf;
+  try {
+    const long long len = petsc_vec_header(path, f);
+    if ((size_t)len != n) throw pp::StatusError(VFS_PP_EINVAL, "vector length " + std::to_string(len) +
+                                                                   " differs from the array length " + std::to_string(n));
+    buf.resize(n);
+    if (n && std::fread(buf.data(), 8, n, f) != n) pp::io_error(path, "fewer values than the header announces");
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+  for (auto& x : buf) x = be64(x);
+  if (n) HIPCHECK(hipMemcpy(v, buf.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_write_petsc_vec_device(const char* path, const double* v, size_t n, int block_size) {
+  ABI_BEGIN
+  if (!path || (!v && n) || n > 0x7fffffffu || block_size < 1) return VFS_PP_EINVAL;
+  std::vector<uint64_t> buf(n);
+  if (n) HIPCHECK(hipMemcpy(buf.data(), v, n * sizeof(double), hipMemcpyDeviceToHost));
+  for (auto& x : buf) x = be64(x);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) pp::io_error(path, "cannot create");
+  const uint32_t h[2] = {be32((uint32_t)kVecClassId), be32((uint32_t)n)};
+  const bool ok = std::fwrite(h, 4, 2, f) == 2 && (!n || std::fwrite(buf.data(), 8, n, f) == n);
+  if (std::fclose(f) != 0 || !ok) pp::io_error(path, "write failed");
+  if (block_size > 1) {  // PETSc's companion file for VecLoad of a blocked vector
+    const std::string info = std::string(path) + ".info";
+    FILE* g = std::fopen(info.c_str(), "w");
+    if (!g) pp::io_error(info, "cannot create");
+    std::fprintf(g, "-vecload_block_size %d\n", block_size);
+    std::fclose(g);
+  }
   return VFS_PP_OK;
   ABI_END
 }

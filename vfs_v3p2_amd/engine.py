@@ -84,6 +84,10 @@ def lib():
         "vfs_pp_set_geometry": (C.c_int, [H] + [dp] * 17),
         "vfs_pp_form_metrics_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 17),
         "vfs_pp_set_grid_device": (C.c_int, [H, vp, vp]),
+        "vfs_pp_pressure_gradient_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 15),
+        "vfs_pp_petsc_vec_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_longlong)]),
+        "vfs_pp_read_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t]),
+        "vfs_pp_write_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t, C.c_int]),
         "vfs_pp_ucd_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "vfs_pp_read_ucd": (C.c_int, [C.c_char_p] + [C.c_double] * 4 + [dp] * 3 + [C.POINTER(C.c_int)] * 3),
         "vfs_pp_ib_search_device": (C.c_int, [C.POINTER(Grid), vp, C.c_int, dp, dp, dp, C.c_int] +
@@ -527,6 +531,46 @@ def form_metrics_device(grid: Grid, coor):
     _check(lib().vfs_pp_form_metrics_device(C.byref(grid), cp, *[C.c_void_p(out[k].data_ptr()) for k in METRIC_FIELDS]),
            "vfs_pp_form_metrics_device")
     return out
+
+
+def Pressure_Gradient(grid: Grid, metrics: dict, nvert, p):
+    """momentum.c:223 on the device: dP (mz, my, mx, 3) from P (periodic ghosts of P
+    are filled in place) and the face metrics (device tensors keyed icsi..kaj, as
+    form_metrics_device returns them)."""
+    import torch
+
+    V, S = dmda_shape(grid, vector=True), dmda_shape(grid)
+    names = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj")
+    ptrs = [_tptr(metrics[k], V if k[1:] in ("csi", "eta", "zet") else S, k) for k in names]
+    dp = torch.empty(V, dtype=torch.float64, device=p.device)
+    _check(lib().vfs_pp_pressure_gradient_device(C.byref(grid), *ptrs, _tptr(nvert, S, "nvert"), _tptr(p, S, "p"),
+                                                 _tptr(dp, V, "dp")), "vfs_pp_pressure_gradient_device")
+    return dp
+
+
+def petsc_vec_length(path) -> int:
+    n = C.c_longlong()
+    _check(lib().vfs_pp_petsc_vec_info(os.fsencode(path), C.byref(n)), "vfs_pp_petsc_vec_info")
+    return n.value
+
+
+def read_petsc_vec(path, shape, device="cuda"):
+    """A PETSc binary vector file (vfield/pfield/nvfield/ufield) into a new device
+    tensor of the given DMDA shape (mz, my, mx[, 3])."""
+    import torch
+
+    t = torch.empty(tuple(shape), dtype=torch.float64, device=device)
+    _check(lib().vfs_pp_read_petsc_vec_device(os.fsencode(path), _tptr(t, None, "v"), t.numel()),
+           "vfs_pp_read_petsc_vec_device")
+    return t
+
+
+def write_petsc_vec(path, t, block_size=None):
+    """Write a device tensor as a PETSc binary vector (block size 3 for AoS vector
+    fields by default)."""
+    bs = block_size if block_size is not None else (3 if t.dim() == 4 and t.shape[-1] == 3 else 1)
+    _check(lib().vfs_pp_write_petsc_vec_device(os.fsencode(path), _tptr(t, None, "v"), t.numel(), int(bs)),
+           "vfs_pp_write_petsc_vec_device")
 
 
 def read_ucd(path, cl=1.0, centre=(0.0, 0.0, 0.0)):
