@@ -639,6 +639,14 @@ __global__ __launch_bounds__(256) void k_convert(size_t n, const TA* __restrict_
   if (t < n) b[t] = (TB)a[t];
 }
 
+// b = (TB)(a * scale): FP16 storage of FP32-level coefficients (scale a power of two)
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_convert_scaled(size_t n, const TA* __restrict__ a, double scale,
+                                                        TB* __restrict__ b) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) b[t] = (TB)(a[t] * scale);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill(size_t n, T* a, T v) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -996,6 +1004,12 @@ void launch_convert(hipStream_t s, size_t n, const TA* a, TB* b) {
 template void launch_convert<double, float>(hipStream_t, size_t, const double*, float*);
 template void launch_convert<float, double>(hipStream_t, size_t, const float*, double*);
 template void launch_convert<double, double>(hipStream_t, size_t, const double*, double*);
+template <typename TA, typename TB>
+void launch_convert_scaled(hipStream_t s, size_t n, const TA* a, double scale, TB* b) {
+  k_convert_scaled<TA, TB><<<nb1(n), 256, 0, s>>>(n, a, scale, b);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_convert_scaled<double, _Float16>(hipStream_t, size_t, const double*, double, _Float16*);
 
 void launch_fill(hipStream_t s, size_t n, double* a, double v) {
   k_fill<double><<<nb1(n), 256, 0, s>>>(n, a, v);

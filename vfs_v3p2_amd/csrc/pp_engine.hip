@@ -277,11 +277,18 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     typename MG<T>::Lev L;
     L.d = H.d;
     const size_t n = H.d.n();
+    using TG = typename CoefOf<T>::type;
+    // FP16 coefficients of an FP32 level: scaled by a power of two so that the
+    // largest magnitude is below 2^14 (FP16 keeps normal numbers down to 2^-14)
+    double gmax = 0.;
+    for (size_t i = 0; i < 9 * n; i++) gmax = std::max(gmax, std::fabs(H.g[i]));
+    const double gs = (std::is_same<TG, T>::value || !(gmax > 0.)) ? 1. : std::ldexp(1., std::ilogb(gmax) + 1 - 14);
+    L.gs = T(gs);
     if (l == 0) {
-      if (std::is_same<T, double>::value) L.g = (T*)h->g64;
+      if (std::is_same<T, double>::value) L.g = (TG*)h->g64;
       else {
-        L.g = h->ar.alloc<T>(9 * n);
-        launch_convert<double, T>(h->s, 9 * n, h->g64, L.g);
+        L.g = h->ar.alloc<TG>(9 * n);
+        launch_convert_scaled<double, TG>(h->s, 9 * n, h->g64, 1. / gs, L.g);
       }
       L.code = h->code;
       uint8_t* cls = h->ar.alloc<uint8_t>(n);
@@ -291,9 +298,9 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
       L.cls = cls;
       HIPCHECK(hipStreamSynchronize(h->s));
     } else {
-      std::vector<T> gt(9 * n);
-      for (size_t i = 0; i < 9 * n; i++) gt[i] = T(H.g[i]);
-      L.g = h->ar.alloc<T>(9 * n);
+      std::vector<TG> gt(9 * n);
+      for (size_t i = 0; i < 9 * n; i++) gt[i] = TG(H.g[i] * (1. / gs));
+      L.g = h->ar.alloc<TG>(9 * n);
       h2d(h->s, L.g, gt.data(), 9 * n);
       uint32_t* code = h->ar.alloc<uint32_t>(n);
       h2d(h->s, code, H.code.data(), n);
@@ -1109,8 +1116,8 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   const double N = (double)n;
   switch (kernel) {
     case 0: *bytes = N * (8 + 72 + 1 + 8); break;  // phi, g11..g33, mask -> y           = 89
-    case 1:                                          // x, b, g, mask -> x   FP32 49 / FP64 97
-      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 36 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+    case 1:  // x, b, g, mask -> x: FP32 with FP16 coefficients 31 / FP64 97
+      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
       break;
     case 2: *bytes = N * (8 + 72 + 1 + 24 + 24); break;  // phi, g, mask, Ucont r/w   = 129
     case 3: *bytes = N * (24 + 72 + 1 + 24); break;      // Ucont, csi/eta/zet, mask -> Ucat = 121

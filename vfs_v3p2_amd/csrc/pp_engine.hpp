@@ -90,7 +90,8 @@ template <typename T>
 struct MG : MGBase {
   struct Lev {
     Dims d{};
-    T* g = nullptr;
+    typename CoefOf<T>::type* g = nullptr;  // face coefficients / gs
+    T gs = T(1);
     const uint32_t* code = nullptr;
     const uint8_t* cls = nullptr;
     T *f = nullptr, *xa = nullptr, *xb = nullptr, *dv = nullptr, *dinv = nullptr;
@@ -98,7 +99,7 @@ struct MG : MGBase {
     int line = 0;
     std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
     double lam = 0.;
-    LevelView<T> view() const { return LevelView<T>{d, g, code, cls}; }
+    LevelView<T> view() const { return LevelView<T>{d, g, code, cls, gs}; }
   };
   std::vector<Lev> lev;
   T* scr = nullptr;

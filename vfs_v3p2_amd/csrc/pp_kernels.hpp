@@ -101,6 +101,8 @@ template <typename TZ>
 void launch_update_x(hipStream_t s, size_t n, double* x, const TZ* Z, size_t ldz, int nz, const double* y);
 template <typename TA, typename TB>
 void launch_convert(hipStream_t s, size_t n, const TA* a, TB* b);
+template <typename TA, typename TB>
+void launch_convert_scaled(hipStream_t s, size_t n, const TA* a, double scale, TB* b);
 void launch_fill(hipStream_t s, size_t n, double* a, double v);
 void launch_copy(hipStream_t s, size_t n, const double* a, double* b);
 void launch_fillf(hipStream_t s, size_t n, float* a, float v);
@@ -123,12 +125,25 @@ template <typename TF>
 void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f);
 
 // ---- multigrid ---------------------------------------------------------------------
+// Storage type of a multigrid level's face coefficients: the FP32 levels of the
+// mixed-precision V-cycle keep them in FP16, scaled by a per-level power of two
+// (gs) into FP16's range; FP64 levels keep FP64 (gs = 1).
+template <typename T>
+struct CoefOf {
+  using type = T;
+};
+template <>
+struct CoefOf<float> {
+  using type = _Float16;
+};
+
 template <typename T>
 struct LevelView {
   Dims d;
-  const T* g;
+  const typename CoefOf<T>::type* g;
   const uint32_t* code;
   const uint8_t* cls;  // CLS_NONFLUID used by the transfers
+  T gs;                // the stored coefficients times gs are the level's coefficients
 };
 
 // Chebyshev smoothing launch modes (see k_smooth_point)

@@ -29,6 +29,8 @@ __device__ __forceinline__ void dinv_cell(const LevelView<T>& L, T* __restrict__
   for (int s = 0; s < 19; s++) xv[s] = T(0);
   T ax, diag;
   flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+  ax *= L.gs;
+  diag *= L.gs;
   dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
 }
 
@@ -64,6 +66,8 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
   }
   T ax, diag;
   flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+  ax *= L.gs;
+  diag *= L.gs;
   const T fc = f[c];
   const T dold = (MODE == SM_FIRST2) ? xv[CP] : (c1 == T(0) ? T(0) : dv[c]);
   if (!(cd & ROW_BIT)) { x_new[c] = T(0); dv[c] = T(0); return; }
@@ -90,9 +94,9 @@ __device__ __forceinline__ void line_factors_col(const LevelView<T>& L, T* __res
       neighbours(L.d, i, j, k, nb);
       T vol[19];
       build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
-      a = vol[SP];
-      b = vol[CP];
-      cc = vol[NP];
+      a = vol[SP] * L.gs;
+      b = vol[CP] * L.gs;
+      cc = vol[NP] * L.gs;
     }
     const T den = b - a * cprev;
     const T inv = den != T(0) ? T(1) / den : T(0);
@@ -114,6 +118,7 @@ __device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __res
   T xv[19], ax, dg;
   load_x(nb, x, xv);
   flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+  ax *= L.gs;
   r[c] = (cd & ROW_BIT) ? f[c] - ax : T(0);
 }
 
@@ -206,6 +211,7 @@ __device__ __forceinline__ void resid_restrict_cell(const LevelView<T>& F, const
         T xv[19], ax, dg;
         load_x(nb, x, xv);
         flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+        ax *= F.gs;
         sum += f[c] - ax;
       }
   fc[C] = sum;
@@ -231,6 +237,7 @@ __device__ __forceinline__ void resid_restrict_blk(const LevelView<T>& F, const 
     T xv[19], ax, dg;
     load_x(nb, x, xv);
     flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
+    ax *= F.gs;
     if (cd & ROW_BIT) r = f[c] - ax;
   }
   r += __shfl_xor(r, 1, 64);

@@ -149,13 +149,13 @@ HDI void face_term(T (&vol)[19], T gv, uint32_t code) {
   }
 }
 
-template <int D, int SGN, typename T>
-HDI void face_terms(T (&vol)[19], const T* __restrict__ g, size_t n, size_t owner, uint32_t code) {
+template <int D, int SGN, typename T, typename TG>
+HDI void face_terms(T (&vol)[19], const TG* __restrict__ g, size_t n, size_t owner, uint32_t code) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
   if (!((code >> F) & 1u)) return;
-  const T g0 = g[(size_t)(3 * D + 0) * n + owner];
-  const T g1 = g[(size_t)(3 * D + 1) * n + owner];
-  const T g2 = g[(size_t)(3 * D + 2) * n + owner];
+  const T g0 = (T)g[(size_t)(3 * D + 0) * n + owner];
+  const T g1 = (T)g[(size_t)(3 * D + 1) * n + owner];
+  const T g2 = (T)g[(size_t)(3 * D + 2) * n + owner];
   face_term<D, SGN, 0>(vol, g0, code);
   face_term<D, SGN, 1>(vol, g1, code);
   face_term<D, SGN, 2>(vol, g2, code);
@@ -163,8 +163,8 @@ HDI void face_terms(T (&vol)[19], const T* __restrict__ g, size_t n, size_t owne
 
 // 19 row coefficients of cell c (interior index); cw/cs/cb are the interior
 // indices of its i-1, j-1, k-1 neighbours (wrapped), which own the lower faces.
-template <typename T>
-HDI void build_row(const T* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
+template <typename T, typename TG>
+HDI void build_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
                          T (&vol)[19]) {
 #pragma unroll
   for (int s = 0; s < 19; s++) vol[s] = T(0);
@@ -209,15 +209,15 @@ HDI void flux_term(T& nrm, T& crs, T& diag, T gv, uint32_t code, const T (&xv)[1
   }
 }
 
-template <int D, int SGN, typename T>
-HDI void flux_face(T& ax, T& diag, const T* __restrict__ g, size_t n, size_t owner, uint32_t code,
+template <int D, int SGN, typename T, typename TG>
+HDI void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, size_t owner, uint32_t code,
                    const T (&xv)[19]) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
   // coefficients are loaded whether or not the face is active (all loads issue
   // together; the owner index is always a valid cell)
-  const T g0 = g[(size_t)(3 * D + 0) * n + owner];
-  const T g1 = g[(size_t)(3 * D + 1) * n + owner];
-  const T g2 = g[(size_t)(3 * D + 2) * n + owner];
+  const T g0 = (T)g[(size_t)(3 * D + 0) * n + owner];
+  const T g1 = (T)g[(size_t)(3 * D + 1) * n + owner];
+  const T g2 = (T)g[(size_t)(3 * D + 2) * n + owner];
   T nrm = T(0), crs = T(0), dg = T(0);
   flux_term<D, SGN, 0>(nrm, crs, dg, g0, code, xv);
   flux_term<D, SGN, 1>(nrm, crs, dg, g1, code, xv);
@@ -228,8 +228,8 @@ HDI void flux_face(T& ax, T& diag, const T* __restrict__ g, size_t n, size_t own
   }
 }
 
-template <typename T>
-HDI void flux_row(const T* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
+template <typename T, typename TG>
+HDI void flux_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
                   const T (&xv)[19], T& ax, T& diag) {
   ax = T(0);
   diag = T(0);
