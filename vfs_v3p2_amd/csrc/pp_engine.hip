@@ -277,45 +277,6 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     typename MG<T>::Lev L;
     L.d = H.d;
     const size_t n = H.d.n();
-    using TG = typename CoefOf<T>::type;
-    // FP16 coefficients of an FP32 level: scaled by a power of two so that the
-    // largest magnitude is below 2^14 (FP16 keeps normal numbers down to 2^-14)
-    double gmax = 0.;
-    for (size_t i = 0; i < 9 * n; i++) gmax = std::max(gmax, std::fabs(H.g[i]));
-    const double gs = (std::is_same<TG, T>::value || !(gmax > 0.)) ? 1. : std::ldexp(1., std::ilogb(gmax) + 1 - 14);
-    L.gs = T(gs);
-    if (l == 0) {
-      if (std::is_same<T, double>::value) L.g = (TG*)h->g64;
-      else {
-        L.g = h->ar.alloc<TG>(9 * n);
-        launch_convert_scaled<double, TG>(h->s, 9 * n, h->g64, 1. / gs, L.g);
-      }
-      L.code = h->code;
-      uint8_t* cls = h->ar.alloc<uint8_t>(n);
-      std::vector<uint8_t> c(n);
-      for (size_t i = 0; i < n; i++) c[i] = (H.code[i] & ROW_BIT) ? 0 : CLS_NONFLUID;
-      h2d(h->s, cls, c.data(), n);
-      L.cls = cls;
-      HIPCHECK(hipStreamSynchronize(h->s));
-    } else {
-      std::vector<TG> gt(9 * n);
-      for (size_t i = 0; i < 9 * n; i++) gt[i] = TG(H.g[i] * (1. / gs));
-      L.g = h->ar.alloc<TG>(9 * n);
-      h2d(h->s, L.g, gt.data(), 9 * n);
-      uint32_t* code = h->ar.alloc<uint32_t>(n);
-      h2d(h->s, code, H.code.data(), n);
-      uint8_t* cls = h->ar.alloc<uint8_t>(n);
-      h2d(h->s, cls, H.cls.data(), n);
-      L.code = code;
-      L.cls = cls;
-      HIPCHECK(hipStreamSynchronize(h->s));
-    }
-    L.f = h->ar.alloc<T>(n);
-    L.xa = h->ar.alloc<T>(n);
-    L.xb = h->ar.alloc<T>(n);
-    L.dv = h->ar.alloc<T>(n);
-    L.dinv = h->ar.alloc<T>(n);
-    launch_dinv<T>(h->s, L.view(), L.dinv);
     // smoother: y-line when the wall-normal coupling dominates (and j is not periodic)
     int line = 0;
     if (h->opts.smoother == 2) line = 1;
@@ -337,6 +298,60 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     }
     if (H.d.per[1]) line = 0;
     L.line = line;
+    // coefficients: FP16 on the y-line levels of the FP32 hierarchy, scaled by a
+    // power of two that puts the largest magnitude below 2^14 (FP16 keeps normal
+    // numbers down to 2^-14); the level precision everywhere else
+    const bool half = std::is_same<T, float>::value && line;
+    double gs = 1.;
+    if (half) {
+      double gmax = 0.;
+      for (size_t i = 0; i < 9 * n; i++) gmax = std::max(gmax, std::fabs(H.g[i]));
+      if (gmax > 0.) gs = std::ldexp(1., std::ilogb(gmax) + 1 - 14);
+    }
+    L.gs = T(gs);
+    if (half) {
+      L.gh = h->ar.alloc<_Float16>(9 * n);
+      if (l == 0) {
+        launch_convert_scaled<double, _Float16>(h->s, 9 * n, h->g64, 1. / gs, L.gh);
+      } else {
+        std::vector<_Float16> gt(9 * n);
+        for (size_t i = 0; i < 9 * n; i++) gt[i] = (_Float16)(H.g[i] * (1. / gs));
+        h2d(h->s, L.gh, gt.data(), 9 * n);
+      }
+    } else if (l == 0 && std::is_same<T, double>::value) {
+      L.g = (T*)h->g64;
+    } else {
+      L.g = h->ar.alloc<T>(9 * n);
+      if (l == 0) {
+        launch_convert<double, T>(h->s, 9 * n, h->g64, L.g);
+      } else {
+        std::vector<T> gt(9 * n);
+        for (size_t i = 0; i < 9 * n; i++) gt[i] = T(H.g[i]);
+        h2d(h->s, L.g, gt.data(), 9 * n);
+      }
+    }
+    if (l == 0) {
+      L.code = h->code;
+      uint8_t* cls = h->ar.alloc<uint8_t>(n);
+      std::vector<uint8_t> c(n);
+      for (size_t i = 0; i < n; i++) c[i] = (H.code[i] & ROW_BIT) ? 0 : CLS_NONFLUID;
+      h2d(h->s, cls, c.data(), n);
+      L.cls = cls;
+    } else {
+      uint32_t* code = h->ar.alloc<uint32_t>(n);
+      h2d(h->s, code, H.code.data(), n);
+      uint8_t* cls = h->ar.alloc<uint8_t>(n);
+      h2d(h->s, cls, H.cls.data(), n);
+      L.code = code;
+      L.cls = cls;
+    }
+    HIPCHECK(hipStreamSynchronize(h->s));
+    L.f = h->ar.alloc<T>(n);
+    L.xa = h->ar.alloc<T>(n);
+    L.xb = h->ar.alloc<T>(n);
+    L.dv = h->ar.alloc<T>(n);
+    L.dinv = h->ar.alloc<T>(n);
+    launch_dinv<T>(h->s, L.view(), L.dinv);
     if (line) {
       L.lfac = h->ar.alloc<T>(3 * n);
       launch_line_factors<T>(h->s, L.view(), L.lfac);
@@ -1116,8 +1131,12 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   const double N = (double)n;
   switch (kernel) {
     case 0: *bytes = N * (8 + 72 + 1 + 8); break;  // phi, g11..g33, mask -> y           = 89
-    case 1:  // x, b, g, mask -> x: FP32 with FP16 coefficients 31 / FP64 97
-      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+    case 1:
+      if (!h->mg->line0())  // x, b, g, mask -> x: FP32 49 / FP64 97
+        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 36 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+      else  // y-line step: x, b, g, mask -> r (FP16 g in FP32), then r, factors, x -> x: 55 / 145
+        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4)
+                                             : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8));
       break;
     case 2: *bytes = N * (8 + 72 + 1 + 24 + 24); break;  // phi, g, mask, Ucont r/w   = 129
     case 3: *bytes = N * (24 + 72 + 1 + 24); break;      // Ucont, csi/eta/zet, mask -> Ucat = 121

@@ -80,6 +80,7 @@ inline Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {
 struct MGBase {
   virtual ~MGBase() = default;
   virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
+  virtual bool line0() const = 0;           // the finest level uses the y-line smoother
   virtual void* f0() = 0;
   virtual void set_out(void* out) = 0;
   virtual std::string describe() const = 0;
@@ -90,7 +91,8 @@ template <typename T>
 struct MG : MGBase {
   struct Lev {
     Dims d{};
-    typename CoefOf<T>::type* g = nullptr;  // face coefficients / gs
+    T* g = nullptr;            // face coefficients, or
+    _Float16* gh = nullptr;    // FP16 coefficients / gs (y-line levels of the FP32 hierarchy)
     T gs = T(1);
     const uint32_t* code = nullptr;
     const uint8_t* cls = nullptr;
@@ -99,7 +101,7 @@ struct MG : MGBase {
     int line = 0;
     std::vector<T> c1, c2;  // Chebyshev coefficients per smoothing step
     double lam = 0.;
-    LevelView<T> view() const { return LevelView<T>{d, g, code, cls, gs}; }
+    LevelView<T> view() const { return LevelView<T>{d, g, gh, code, cls, gs}; }
   };
   std::vector<Lev> lev;
   T* scr = nullptr;
@@ -149,6 +151,7 @@ struct MG : MGBase {
   }
 
   void vcycle(hipStream_t s) override { cycle(s, 0, out); }
+  bool line0() const override { return !lev.empty() && lev[0].line; }
 
   // returns the buffer holding this level's result
   T* cycle(hipStream_t s, size_t l, T* final_out) {

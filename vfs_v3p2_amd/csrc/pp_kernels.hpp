@@ -125,25 +125,19 @@ template <typename TF>
 void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f);
 
 // ---- multigrid ---------------------------------------------------------------------
-// Storage type of a multigrid level's face coefficients: the FP32 levels of the
-// mixed-precision V-cycle keep them in FP16, scaled by a per-level power of two
-// (gs) into FP16's range; FP64 levels keep FP64 (gs = 1).
-template <typename T>
-struct CoefOf {
-  using type = T;
-};
-template <>
-struct CoefOf<float> {
-  using type = _Float16;
-};
-
+// A multigrid level as the kernels see it.  Face coefficients are in g (the
+// level precision), except on the y-line levels of the FP32 hierarchy, which keep
+// them in gh as FP16 scaled by a power of two (gh * gs are the coefficients): a
+// y-line step reads them twice (residual, then the line solve's factors), and
+// those levels are bandwidth-bound where the point levels are not.
 template <typename T>
 struct LevelView {
   Dims d;
-  const typename CoefOf<T>::type* g;
+  const T* g;
+  const _Float16* gh;
   const uint32_t* code;
   const uint8_t* cls;  // CLS_NONFLUID used by the transfers
-  T gs;                // the stored coefficients times gs are the level's coefficients
+  T gs;
 };
 
 // Chebyshev smoothing launch modes (see k_smooth_point)

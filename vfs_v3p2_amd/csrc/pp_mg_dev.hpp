@@ -12,10 +12,27 @@
 // coarse IB cells and at non-periodic walls, periodic axes wrap.
 #pragma once
 
+#include <type_traits>
+
 #include "pp_kernels.hpp"
 #include "pp_stencil.hpp"
 
 namespace pp {
+
+// (A x)_c and the diagonal on a level, from its coefficients in either storage
+template <typename T>
+__device__ __forceinline__ void level_flux(const LevelView<T>& L, uint32_t cd, size_t c, const Nbr& nb,
+                                           const T (&xv)[19], T& ax, T& diag) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.gh) {
+      flux_row(L.gh, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+      ax *= L.gs;
+      diag *= L.gs;
+      return;
+    }
+  }
+  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+}
 
 // D^-1 of the level operator, 0 on cells that carry no unknown.
 template <typename T>
@@ -28,9 +45,7 @@ __device__ __forceinline__ void dinv_cell(const LevelView<T>& L, T* __restrict__
 #pragma unroll
   for (int s = 0; s < 19; s++) xv[s] = T(0);
   T ax, diag;
-  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
-  ax *= L.gs;
-  diag *= L.gs;
+  level_flux(L, cd, c, nb, xv, ax, diag);
   dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
 }
 
@@ -65,9 +80,7 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
     load_x(nb, x_old, xv);
   }
   T ax, diag;
-  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
-  ax *= L.gs;
-  diag *= L.gs;
+  level_flux(L, cd, c, nb, xv, ax, diag);
   const T fc = f[c];
   const T dold = (MODE == SM_FIRST2) ? xv[CP] : (c1 == T(0) ? T(0) : dv[c]);
   if (!(cd & ROW_BIT)) { x_new[c] = T(0); dv[c] = T(0); return; }
@@ -93,10 +106,17 @@ __device__ __forceinline__ void line_factors_col(const LevelView<T>& L, T* __res
       Nbr nb;
       neighbours(L.d, i, j, k, nb);
       T vol[19];
-      build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
-      a = vol[SP] * L.gs;
-      b = vol[CP] * L.gs;
-      cc = vol[NP] * L.gs;
+      if (L.gh) {
+        build_row(L.gh, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+        a = vol[SP] * L.gs;
+        b = vol[CP] * L.gs;
+        cc = vol[NP] * L.gs;
+      } else {
+        build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+        a = vol[SP];
+        b = vol[CP];
+        cc = vol[NP];
+      }
     }
     const T den = b - a * cprev;
     const T inv = den != T(0) ? T(1) / den : T(0);
@@ -117,8 +137,7 @@ __device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __res
   neighbours(L.d, i, j, k, nb);
   T xv[19], ax, dg;
   load_x(nb, x, xv);
-  flux_row(L.g, L.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
-  ax *= L.gs;
+  level_flux(L, cd, c, nb, xv, ax, dg);
   r[c] = (cd & ROW_BIT) ? f[c] - ax : T(0);
 }
 
@@ -210,8 +229,7 @@ __device__ __forceinline__ void resid_restrict_cell(const LevelView<T>& F, const
         neighbours(F.d, i, j, k, nb);
         T xv[19], ax, dg;
         load_x(nb, x, xv);
-        flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
-        ax *= F.gs;
+        level_flux(F, cd, c, nb, xv, ax, dg);
         sum += f[c] - ax;
       }
   fc[C] = sum;
@@ -236,8 +254,7 @@ __device__ __forceinline__ void resid_restrict_blk(const LevelView<T>& F, const 
     neighbours(F.d, i, j, k, nb);
     T xv[19], ax, dg;
     load_x(nb, x, xv);
-    flux_row(F.g, F.d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
-    ax *= F.gs;
+    level_flux(F, cd, c, nb, xv, ax, dg);
     if (cd & ROW_BIT) r = f[c] - ax;
   }
   r += __shfl_xor(r, 1, 64);
