@@ -54,6 +54,29 @@ __device__ __forceinline__ int find_item(const uint32_t* __restrict__ start, int
 // slot state: x = Krylov step j, y = active
 using SlotDev = int2;
 
+// Cells of an item are walked flat, 256 consecutive cells per tile (i fastest),
+// so a level narrower than a wave still fills every lane.  Returns false past the
+// last cell.
+__device__ __forceinline__ bool flat_cell(const Dims& d, uint32_t lt, int& i, int& j, int& k) {
+  const uint32_t e = lt * 256u + threadIdx.x + 64u * threadIdx.y;
+  const uint32_t nxy = (uint32_t)d.nx * (uint32_t)d.ny;
+  if (e >= nxy * (uint32_t)d.nz) return false;
+  k = (int)(e / nxy);
+  const uint32_t r = e - (uint32_t)k * nxy;
+  j = (int)(r / (uint32_t)d.nx);
+  i = (int)(r - (uint32_t)j * (uint32_t)d.nx);
+  return true;
+}
+
+// y-line (i, k) of an item, 256 consecutive lines per tile
+__device__ __forceinline__ bool flat_line(const Dims& d, uint32_t lt, int& i, int& k) {
+  const uint32_t e = lt * 256u + threadIdx.x + 64u * threadIdx.y;
+  if (e >= (uint32_t)d.nx * (uint32_t)d.nz) return false;
+  k = (int)(e / (uint32_t)d.nx);
+  i = (int)(e - (uint32_t)k * (uint32_t)d.nx);
+  return true;
+}
+
 template <typename T>
 struct SmoothItem {
   LevelView<T> L;
@@ -145,11 +168,8 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
     const SmoothItem<T>& I = items[q];
     const SlotDev st = sd[I.slot];
     if (!st.y) continue;
-    const uint32_t lt = t - start[q];
-    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
-    const int k = (int)(lt / (I.tx * I.ty));
-    if (i >= I.L.d.nx || j >= I.L.d.ny) continue;
+    int i, j, k;
+    if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
     smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
   }
@@ -164,11 +184,8 @@ __global__ __launch_bounds__(256) void kb_resid(const SmoothItem<T>* __restrict_
     const int q = find_item(start, nitems, t);
     const SmoothItem<T>& I = items[q];
     if (!sd[I.slot].y) continue;
-    const uint32_t lt = t - start[q];
-    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
-    const int k = (int)(lt / (I.tx * I.ty));
-    if (i >= I.L.d.nx || j >= I.L.d.ny) continue;
+    int i, j, k;
+    if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
     resid_cell(I.L, I.f, I.xin, I.scr, i, j, k);
   }
 }
@@ -183,10 +200,8 @@ __global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__
     const SmoothItem<T>& I = items[q];
     const SlotDev st = sd[I.slot];
     if (!st.y) continue;
-    const uint32_t lt = t - start[q];
-    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int k = (int)(lt / I.tx) * 4 + (int)threadIdx.y;
-    if (i >= I.L.d.nx || k >= I.L.d.nz) continue;
+    int i, k;
+    if (!flat_line(I.L.d, t - start[q], i, k)) continue;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
     const size_t n = I.L.d.n();
     line_solve_col<T, ZG>(I.L.d, ZG ? I.f : I.scr, I.lfac, I.scr + n, I.xin, xout, I.dv, I.c1, I.c2, i, k);
@@ -216,11 +231,8 @@ __global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __
     const int q = find_item(start, nitems, t);
     const RestrictItem<T>& I = items[q];
     if (!sd[I.slot].y) continue;
-    const uint32_t lt = t - start[q];
-    const int ic = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int jc = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
-    const int kc = (int)(lt / (I.tx * I.ty));
-    if (ic >= I.dc.nx || jc >= I.dc.ny) continue;
+    int ic, jc, kc;
+    if (!flat_cell(I.dc, t - start[q], ic, jc, kc)) continue;
     resid_restrict_cell(I.F, I.f, I.x, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
   }
 }
@@ -233,11 +245,8 @@ __global__ __launch_bounds__(256) void kb_prolong(const ProlongItem<T>* __restri
     const int q = find_item(start, nitems, t);
     const ProlongItem<T>& I = items[q];
     if (!sd[I.slot].y) continue;
-    const uint32_t lt = t - start[q];
-    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
-    const int k = (int)(lt / (I.tx * I.ty));
-    if (i >= I.F.d.nx || j >= I.F.d.ny) continue;
+    int i, j, k;
+    if (!flat_cell(I.F.d, t - start[q], i, j, k)) continue;
     prolong_add_cell(I.F, I.dc, I.clsc, I.cmask, I.xc, I.xf, i, j, k);
   }
 }
@@ -270,11 +279,8 @@ __global__ __launch_bounds__(256) void kb_apply_dots(const ApplyItem<T>* __restr
     const ApplyItem<T>& I = items[q];
     const SlotDev st = sd[I.slot];
     if (!st.y) continue;
-    const uint32_t lt = t - start[q];
-    const int i = (int)(lt % I.tx) * 64 + (int)threadIdx.x;
-    const int j = (int)((lt / I.tx) % I.ty) * 4 + (int)threadIdx.y;
-    const int k = (int)(lt / (I.tx * I.ty));
-    const bool in = i < I.d.nx && j < I.d.ny;
+    int i = 0, j = 0, k = 0;
+    const bool in = flat_cell(I.d, t - start[q], i, j, k);
     const T* z = I.Z + (size_t)st.x * I.n;
     double* w = I.V + (size_t)(st.x + 1) * I.n;
     const size_t c = in ? I.d.idx(i, j, k) : 0;
@@ -497,8 +503,8 @@ struct BatchCore {
     seq.push_back(Launch{kind, mode, pr.first, pr.second, (int)items.size(), tot, 2048});
   }
 
-  static uint32_t tiles64x4(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz; }
-  static uint32_t line_tiles(const Dims& d) { return (uint32_t)((d.nx + 63) / 64) * ((d.nz + 3) / 4); }
+  static uint32_t cell_tiles(const Dims& d) { return (uint32_t)((d.n() + 255) / 256); }
+  static uint32_t line_tiles(const Dims& d) { return (uint32_t)(((size_t)d.nx * d.nz + 255) / 256); }
 
   // Launches of one smoothing step k (0-based) at one depth for a group of
   // members: point levels fuse the first two steps from a zero start; y-line
@@ -546,14 +552,14 @@ struct BatchCore {
         it.ty = (uint32_t)(L.d.ny + 3) / 4;
         pmode = mode;
         pt.push_back(it);
-        npt.push_back(tiles64x4(L.d));
+        npt.push_back(cell_tiles(L.d));
       } else {
         if (mode != SM_ZERO) {
           SmoothItem<T> r = it;
           r.tx = (uint32_t)(L.d.nx + 63) / 64;
           r.ty = (uint32_t)(L.d.ny + 3) / 4;
           rs.push_back(r);
-          nrs.push_back(tiles64x4(L.d));
+          nrs.push_back(cell_tiles(L.d));
         }
         it.tx = (uint32_t)(L.d.nx + 63) / 64;
         it.ty = 0;
@@ -607,7 +613,7 @@ struct BatchCore {
           it.fc = C.f;
           it.cmask = (C.d.nx != L.d.nx ? 1 : 0) | (C.d.ny != L.d.ny ? 2 : 0) | (C.d.nz != L.d.nz ? 4 : 0);
           it.slot = q;
-          if (it.cmask == 7) {
+          if (it.cmask == 7 && L.d.nx >= 64) {  // narrower levels: flat cells fill the lanes better
             it.tx = (uint32_t)(L.d.nx + 63) / 64;
             it.ty = (uint32_t)(L.d.ny + 1) / 2;
             rb.push_back(it);
@@ -616,7 +622,7 @@ struct BatchCore {
             it.tx = (uint32_t)(C.d.nx + 63) / 64;
             it.ty = (uint32_t)(C.d.ny + 3) / 4;
             rg.push_back(it);
-            ng.push_back(tiles64x4(C.d));
+            ng.push_back(cell_tiles(C.d));
           }
         }
         add_launch(K_RBLK, 0, rb, nb);
@@ -668,7 +674,7 @@ struct BatchCore {
         it.tx = (uint32_t)(L.d.nx + 63) / 64;
         it.ty = (uint32_t)(L.d.ny + 3) / 4;
         pi.push_back(it);
-        pn.push_back(tiles64x4(L.d));
+        pn.push_back(cell_tiles(L.d));
       }
       add_launch(K_PROLONG, 0, pi, pn);
       for (int k = 0; k < degree; k++) smoothing_step(d, k, false, true, up, cur);
@@ -694,7 +700,7 @@ struct BatchCore {
         a.slot = q;
         a.tx = (uint32_t)(h->d.nx + 63) / 64;
         a.ty = (uint32_t)(h->d.ny + 3) / 4;
-        const uint32_t na = tiles64x4(h->d);
+        const uint32_t na = cell_tiles(h->d);
         fa.push_back(FinItem{h->red, h->hpin_dev, ta, ta + na, q});
         ta += na;
         ai.push_back(a);
