@@ -51,13 +51,14 @@ def fields(torch, cs):
     return u, phi, p, ucat
 
 
-@pytest.mark.parametrize("precision,smoother", [(1, 1), (0, 1), (1, 2), (0, 0)])
-def test_batch_equals_alone_and_oracle(E, torch_cuda, precision, smoother):
+@pytest.mark.parametrize("precision,smoother,orthog", [(1, 1, 2), (0, 1, 2), (1, 2, 2), (0, 0, 2), (1, 0, 0)])
+def test_batch_equals_alone_and_oracle(E, torch_cuda, precision, smoother, orthog):
     """smoother 1: point Chebyshev on every level; 2: y-line on every level; 0:
-    chosen per level (y-line where the wall-normal coupling dominates)."""
+    chosen per level (y-line where the wall-normal coupling dominates).  orthog
+    2: one lazily normalised Gram-Schmidt pass per step; 0: CGS2."""
     torch = torch_cuda
     cs = get_cases()
-    kw = dict(precision=precision, rtol=1e-12, smoother=smoother)
+    kw = dict(precision=precision, rtol=1e-12, smoother=smoother, orthog=orthog)
     print(f"batch of {len(cs)} cases, {kw}", flush=True)
     # alone
     alone = []

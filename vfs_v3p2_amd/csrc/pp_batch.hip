@@ -9,8 +9,11 @@
 //   tiles of a case share that XCD's L2.
 // * Each case's Krylov step j and an active flag live in a small device array,
 //   so the launch sequence of one FGMRES iteration (the V-cycle of every case,
-//   then the three CGS2 passes) never changes while the membership stays the
-//   same: it is captured once as a HIP graph and replayed every iteration.
+//   the operator with its Gram-Schmidt dots, then one pass that orthogonalises
+//   the unnormalised new basis vector and writes the next V-cycle's input) never
+//   changes while the membership stays the same: it is captured once as a HIP
+//   graph and replayed every iteration.  Members that ask for CGS2 (orthog 0
+//   or 1) run it in three passes instead.
 // * The host keeps each case's Hessenberg column, Givens rotations and restart
 //   logic (the same algorithm as the single-case engine) and handles cases that
 //   start, restart or finish between iterations.
@@ -328,6 +331,44 @@ __global__ __launch_bounds__(256) void kb_axpy_scale(const VecItem<T>* __restric
   }
 }
 
+// Fixed-order sum of the tile partials pv[t0, t1) by one 256-thread block
+__device__ __forceinline__ double tile_sum(const double* __restrict__ pv, uint32_t t0, uint32_t t1, double* lds) {
+  double a0 = 0., a1 = 0.;
+  uint32_t b = t0 + threadIdx.x;
+  for (; b + 256 < t1; b += 512) { a0 += pv[b]; a1 += pv[b + 256]; }
+  for (; b < t1; b += 256) a0 += pv[b];
+  return block_sum(a0 + a1, lds);
+}
+
+// Lazily normalised Gram-Schmidt pass over U_{j+1} = w in place: w' = w - sum_l c_l U_l
+// with c_l = red[8 + l] red[RED_NUINV2 + l] (w.U_l / |U_l|^2), the next V-cycle's
+// input f0 = (T)(w' s) with s = conv_scale(|w|^2 = red[8 + j + 1]), and per-tile
+// partial sums of |w'|^2 at partial[t].  The same arithmetic as k_axpy_norm.
+template <typename T>
+__global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict__ items,
+                                                    const uint32_t* __restrict__ start, int nitems, uint32_t total,
+                                                    const SlotDev* __restrict__ sd, double* __restrict__ partial) {
+  __shared__ double lds[4][64];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const VecItem<T>& I = items[q];
+    const SlotDev st = sd[I.slot];
+    if (!st.y) continue;
+    const int nv = st.x + 1;
+    const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
+    double acc[1] = {0.};
+    if (e < I.n) {
+      double* w = I.V + (size_t)nv * I.n;
+      double v = w[e];
+      for (int l = 0; l < nv; l++) v -= (I.red[8 + l] * I.red[RED_NUINV2 + l]) * I.V[(size_t)l * I.n + e];
+      w[e] = v;
+      I.f0[e] = (T)(v * conv_scale(I.red[8 + nv]));
+      acc[0] = v * v;
+    }
+    block_sums(acc, 1, partial, total, t, lds);
+  }
+}
+
 // One block per (case, vector l < j + 1 + extra): fixed-order sum of the case's
 // tile partials into red[off + l], mirrored to pinned host memory.
 __global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ items, const double* __restrict__ partial,
@@ -338,15 +379,27 @@ __global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ i
   const SlotDev st = sd[I.slot];
   const int l = blockIdx.y;
   if (!st.y || l >= st.x + 1 + extra) return;
-  const double* pv = partial + (size_t)l * total;
-  double a0 = 0., a1 = 0.;
-  uint32_t b = I.t0 + threadIdx.x;
-  for (; b + 256 < I.t1; b += 512) { a0 += pv[b]; a1 += pv[b + 256]; }
-  for (; b < I.t1; b += 256) a0 += pv[b];
-  const double s = block_sum(a0 + a1, lds);
+  const double s = tile_sum(partial + (size_t)l * total, I.t0, I.t1, lds);
   if (threadIdx.x == 0) {
     I.red[off + l] = s;
     I.hpin[off + l] = s;
+  }
+}
+
+// One block per case: |U_{j+1}|^2 from the partials of kb_axpy_norm into red[0]
+// (mirrored) and its reciprocal into red[RED_NUINV2 + j + 1] for the next step.
+__global__ __launch_bounds__(256) void kb_finalize_norm(const FinItem* __restrict__ items,
+                                                        const double* __restrict__ partial,
+                                                        const SlotDev* __restrict__ sd) {
+  __shared__ double lds[4];
+  const FinItem& I = items[blockIdx.x];
+  const SlotDev st = sd[I.slot];
+  if (!st.y) return;
+  const double s = tile_sum(partial, I.t0, I.t1, lds);
+  if (threadIdx.x == 0) {
+    I.red[0] = s;
+    I.hpin[0] = s;
+    I.red[RED_NUINV2 + st.x + 1] = s > 0. ? 1. / s : 0.;
   }
 }
 
@@ -405,6 +458,7 @@ struct BatchCore {
     bool conv = false, pending_update = false, fresh = false;
     int pending_nv = 0;
     std::vector<double> H, cs, sn, gv, y;
+    std::vector<double> nu, sigma;  // lazily normalised basis: |U_l| and the scale of Z_l
   };
 
   hipStream_t s = nullptr;
@@ -430,14 +484,15 @@ struct BatchCore {
     unsigned cap = 2048;  // worker count limit (0: one tile per worker), set by tune()
   };
   std::vector<Launch> seq;  // V-cycle + Krylov passes of one iteration
-  Launch apply{}, axd{}, axs{};
+  Launch apply{}, axd{}, axs{}, axn{};
+  bool lazy = true;  // every member orthogonalises with the lazily normalised single pass
   const FinItem* fin = nullptr;
   int nfin = 0;
   uint32_t fin_total_apply = 0, fin_total_vec = 0;
   FinItem* fin_apply = nullptr;
   FinItem* fin_vec = nullptr;
 
-  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS };
+  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS, K_AXN };
   bool tuned_once = false;
 
   explicit BatchCore(int nslots) {
@@ -582,6 +637,8 @@ struct BatchCore {
     for (int q = 0; q < (int)slots.size(); q++)
       if (slots[q].h) mem.push_back(q);
     if (mem.empty()) { dirty = false; return; }
+    lazy = true;
+    for (int q : mem) lazy = lazy && slots[q].h->opts.orthog == 2;
     int dmax = 0;
     for (int q : mem) dmax = std::max(dmax, (int)slots[q].mg->lev.size());
     // current x buffer per (member, level)
@@ -732,6 +789,7 @@ struct BatchCore {
       apply = Launch{K_APPLY, 0, pa.first, pa.second, (int)ai.size(), ta, 2048};
       axd = Launch{K_AXD, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
       axs = Launch{K_AXS, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
+      axn = Launch{K_AXN, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
       fin_apply = plan.alloc<FinItem>(fa.size());
       fin_vec = plan.alloc<FinItem>(fv.size());
       HIPCHECK(hipMemcpyAsync(fin_apply, fa.data(), fa.size() * sizeof(FinItem), hipMemcpyHostToDevice, s));
@@ -795,14 +853,29 @@ struct BatchCore {
       case K_AXS:
         kb_axpy_scale<T><<<W, 256, 0, s>>>(static_cast<const VecItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
         break;
+      case K_AXN:
+        kb_axpy_norm<T><<<W, 256, 0, s>>>(static_cast<const VecItem<T>*>(L.items), L.start, L.nitems, L.total, sd,
+                                          partial);
+        break;
     }
     HIPCHECK(hipGetLastError());
   }
 
   void launch_seq() {
     for (const Launch& L : seq) launch_one(L);
-    // CGS2 in three passes over each case's basis
     launch_one(apply);
+    if (lazy) {
+      // one Gram-Schmidt pass on the unnormalised basis: w.U_l and |w|^2 from the
+      // operator pass, then w' and |w'|^2 (the host re-orthogonalises the rare
+      // case whose pass cancelled most of w)
+      kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 1, sd);
+      HIPCHECK(hipGetLastError());
+      launch_one(axn);
+      kb_finalize_norm<<<nfin, 256, 0, s>>>(fin_vec, partial, sd);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
+    // CGS2 in three passes over each case's basis
     kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 0, sd);
     HIPCHECK(hipGetLastError());
     launch_one(axd);
@@ -821,8 +894,12 @@ struct BatchCore {
     std::vector<Launch*> all;
     for (Launch& L : seq) all.push_back(&L);
     all.push_back(&apply);
-    all.push_back(&axd);
-    all.push_back(&axs);
+    if (lazy) {
+      all.push_back(&axn);
+    } else {
+      all.push_back(&axd);
+      all.push_back(&axs);
+    }
     auto& tc = tune_cache();
     std::lock_guard<std::mutex> lk(tc.mu);
     bool need = false;
@@ -947,7 +1024,7 @@ struct BatchCore {
       Slot& sl = slots[q];
       vfs_pp_handle* h = sl.h;
       if (sl.pending_update) {
-        for (int l = 0; l < sl.pending_nv; l++) h->hpin[200 + l] = sl.y[l];
+        for (int l = 0; l < sl.pending_nv; l++) h->hpin[200 + l] = lazy ? sl.y[l] / sl.sigma[l] : sl.y[l];
         HIPCHECK(hipMemcpyAsync(h->red + 200, h->hpin + 200, sl.pending_nv * sizeof(double), hipMemcpyHostToDevice, s));
         launch_update_x<T>(s, h->d.n(), h->x, static_cast<const T*>(h->Z), h->d.n(), sl.pending_nv, h->red + 200);
         sl.pending_update = false;
@@ -965,11 +1042,39 @@ struct BatchCore {
       if (residual_verdict(h, beta, sl.S.bnorm, sl.tgt)) { sl.conv = true; sl.state = FINISH; continue; }
       if (sl.it >= h->opts.max_iters) { sl.state = FINISH; continue; }
       launch_scale_conv<T>(s, h->d.n(), h->r, h->red, h->V, static_cast<T*>(sl.mg->f0()));
+      if (lazy) {
+        // U_0 = v_0, read by the first Gram-Schmidt pass as 1 / |U_0|^2 = 1
+        launch_fill(s, 1, h->red + RED_NUINV2, 1.);
+        sl.nu.assign(sl.m + 1, 1.);
+        sl.sigma.assign(sl.m + 1, 1.);
+      }
       std::fill(sl.gv.begin(), sl.gv.end(), 0.);
       sl.gv[0] = beta;
       sl.j = 0;
       sl.state = ITER;
     }
+  }
+
+  // A second Gram-Schmidt pass over U_nv of one case (its first pass cancelled
+  // |w'| below 1e-4 |w|): dots into red[72..], w'' and the V-cycle input with the
+  // same scale, |w''|^2 into red[0] and its reciprocal into red[RED_NUINV2 + nv].
+  // Returns |w''|^2.  Launches on the batch stream and waits for it.
+  double reorthogonalise(Slot& sl, int nv) {
+    vfs_pp_handle* h = sl.h;
+    const size_t n = h->d.n();
+    double* w = h->V + (size_t)nv * n;
+    int nb = 0;
+    need_partials(h, (size_t)nv, (size_t)partial_blocks(n));
+    launch_multidot(s, n, w, h->V, n, nv, h->partial, &nb);
+    launch_finalize(s, h->partial, nv, nb, h->red + 72, 0, h->hpin_dev + 72);
+    launch_axpy_norm<T>(s, n, w, h->V, n, nv, h->red + 72, h->red + RED_NUINV2, h->red + 8 + nv,
+                        static_cast<T*>(sl.mg->f0()), h->partial, &nb);
+    launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+    HIPCHECK(hipStreamSynchronize(s));
+    const double b2 = h->hpin[0];
+    h->hpin[RED_NUINV2 + nv] = b2 > 0. ? 1. / b2 : 0.;
+    h2d(s, h->red + RED_NUINV2 + nv, h->hpin + RED_NUINV2 + nv, 1);
+    return b2;
   }
 
   // after an iteration: Arnoldi column, Givens rotation, cycle end
@@ -980,8 +1085,23 @@ struct BatchCore {
       vfs_pp_handle* h = sl.h;
       const int m = sl.m, j = sl.j;
       auto H = [&](int r, int c) -> double& { return sl.H[(size_t)r * m + c]; };
-      for (int l = 0; l <= j; l++) H(l, j) = h->hpin[8 + l] + h->hpin[72 + l];
-      H(j + 1, j) = h->hpin[1];
+      if (lazy) {
+        // H(l, j) = v_l . A z_j with U_l = nu_l v_l and Z_j = sigma_j z_j
+        const int nv = j + 1;
+        const double a2 = h->hpin[8 + nv];
+        double b2 = h->hpin[0];
+        for (int l = 0; l <= j; l++) H(l, j) = h->hpin[8 + l] / (sl.nu[l] * sl.sigma[j]);
+        if (b2 < 1e-8 * a2) {
+          b2 = reorthogonalise(sl, nv);
+          for (int l = 0; l <= j; l++) H(l, j) += h->hpin[72 + l] / (sl.nu[l] * sl.sigma[j]);
+        }
+        sl.nu[j + 1] = std::sqrt(b2);
+        sl.sigma[j + 1] = host_conv_scale(a2) * sl.nu[j + 1];
+        H(j + 1, j) = sl.nu[j + 1] / sl.sigma[j];
+      } else {
+        for (int l = 0; l <= j; l++) H(l, j) = h->hpin[8 + l] + h->hpin[72 + l];
+        H(j + 1, j) = h->hpin[1];
+      }
       for (int l = 0; l < j; l++) {
         const double t1 = sl.cs[l] * H(l, j) + sl.sn[l] * H(l + 1, j);
         const double t2 = -sl.sn[l] * H(l, j) + sl.cs[l] * H(l + 1, j);

@@ -720,14 +720,6 @@ __global__ __launch_bounds__(256) void k_axpy_dots_acc(size_t n, double* __restr
   block_sums(acc, nv + 1, partial, gridDim.x, blockIdx.x, lds);
 }
 
-// Power of two near 1/||w|| from ||w||^2, the scale of the single-precision copy
-// of an unnormalised basis vector (the host computes the same value).
-__host__ __device__ inline double conv_scale(double a2) {
-  if (!(a2 > 0.)) return 1.;
-  const int e = ilogb(a2);
-  return ldexp(1.0, -(int)floor(0.5 * e));
-}
-
 // Lazily normalised Gram-Schmidt pass: w' = w - sum_l c_l U_l in place with
 // c_l = dots[l] * nuinv2[l] (the projection onto U_l / |U_l|), f = (TF)(w' s) with
 // s = conv_scale(*a2) for the next V-cycle, and per-block partial sums of |w'|^2.

@@ -105,6 +105,14 @@ __device__ inline void gs_sub_block_dots(bool in, double* __restrict__ w, const 
   __syncthreads();  // lds is reused by the block's next tile
 }
 
+// Power of two near 1/||w|| from ||w||^2, the scale of the single-precision copy
+// of an unnormalised basis vector (the host computes the same value).
+__host__ __device__ inline double conv_scale(double a2) {
+  if (!(a2 > 0.)) return 1.;
+  const int e = ilogb(a2);
+  return ldexp(1.0, -(int)floor(0.5 * e));
+}
+
 // w[c] = (A z)[c] in flux form on a row cell, 0 elsewhere; returns w[c]
 template <typename TZ>
 __device__ __forceinline__ double apply_cell(const Dims& d, const double* __restrict__ g,
