@@ -30,6 +30,7 @@
 #include "pp_engine.hpp"
 #include "pp_krylov_dev.hpp"
 #include "pp_mg_dev.hpp"
+#include "pp_path_dev.hpp"
 
 namespace pp {
 
@@ -412,6 +413,170 @@ __global__ void kb_gs_norm(const FinItem* __restrict__ items, int nitems, const 
   gs_norm_one(I.red, 72, st.x + 1, I.hpin);
 }
 
+// ------------------------------------------------------ batched per-case phases
+// The per-case work around the Krylov iterations (VolumeFlux and RHS when a case
+// starts, its initial guess, the solution update and true residual at each
+// cycle start, and the null space, Phi, P, Projection, Contra2Cart and
+// Divergence when it finishes) runs as one launch per step over every case that
+// takes the step, 256 elements of one case per tile, instead of one launch per
+// case and step.
+enum PhaseOp {
+  PH_IBSUM, PH_IBCORR, PH_RHS, PH_GATHER, PH_ZERO, PH_UPDATE_X, PH_RESID, PH_DIVEST, PH_SCALE_CONV,
+  PH_MASKED_SUM, PH_SUB_MEAN, PH_SCATTER, PH_PERIODIC, PH_ADD_PHI, PH_PROJECTION, PH_UCONT_GHOSTS,
+  PH_C2C_CORE, PH_VEC_GHOSTS, PH_C2C_BC, PH_C2C_SOLID, PH_DIVERGENCE
+};
+
+// One case's operands for one phase step.  Roles of the generic fields:
+//   IBSUM        w0 Ucont, faces, i0 zero_small          -> [flux, area]
+//   IBCORR       w0 Ucont, faces, i0 immersed, sc [flux, area]
+//   RHS          r0 Ucont, w0 b, s0 factor                -> |b|^2
+//   GATHER       r0 Phi, w0 x          ZERO  w0 x
+//   UPDATE_X     w0 x, z Z (ld, i0 vectors), sc y
+//   RESID        r0 b, r1 x, w0 r                         -> |r|^2
+//   DIVEST       r0 aj, r1 r                              -> max |aj r|
+//   SCALE_CONV   r0 r, w0 V_0, f f0, sc |r|^2, one (1 / |U_0|^2 of a lazy basis)
+//   MASKED_SUM   r0 x                                     -> sum over nvert < 0.1
+//   SUB_MEAN     w0 x, sc sum, s0 count
+//   SCATTER      r0 x, w0 Phi          ADD_PHI  r0 Phi, w0 P
+//   PERIODIC     w0 array, i0 zero_other
+//   PROJECTION   r0 Phi, w0 Ucont, s0 factor
+//   UCONT_GHOSTS w0 Ucont
+//   C2C_CORE     r0 cell metrics, r1 Ucont, w0 Ucat
+//   VEC_GHOSTS   w0 Ucat, w1 snapshot or null
+//   C2C_BC       r0 nvert, r1 snapshot, w0 Ucat, bc
+//   C2C_SOLID    w0 Ucat
+//   DIVERGENCE   r0 aj, r1 Ucont                          -> max |aj div u|
+struct PhaseItem {
+  Dmda m;
+  Dims d;
+  const double* g;
+  const uint32_t* code;
+  const uint8_t* cls;
+  const double* r0;
+  const double* r1;
+  double* w0;
+  double* w1;
+  const void* z;
+  void* f;
+  const IbFace* faces;
+  const double* sc;
+  double* one;
+  double s0;
+  size_t ld;
+  uint32_t n;  // elements: cells, vector entries, faces or boundary points
+  int i0;
+  int bc[6];
+};
+
+// reduction of one case's tile partials [t0, t1) into out[v] (and the pinned host mirror)
+struct PhaseFin {
+  double* out;
+  double* host;
+  uint32_t t0, t1;
+};
+
+__device__ __forceinline__ double tile_max(const double* __restrict__ pv, uint32_t t0, uint32_t t1, double* lds) {
+  double a0 = 0., a1 = 0.;
+  uint32_t b = t0 + threadIdx.x;
+  for (; b + 256 < t1; b += 512) { a0 = fmax(a0, pv[b]); a1 = fmax(a1, pv[b + 256]); }
+  for (; b < t1; b += 256) a0 = fmax(a0, pv[b]);
+  return block_max(fmax(a0, a1), lds);
+}
+
+template <int OP, typename T>
+__global__ __launch_bounds__(256) void kb_phase(const PhaseItem* __restrict__ items, const uint32_t* __restrict__ start,
+                                                int nitems, uint32_t total, double* __restrict__ partial) {
+  __shared__ double lds[4];
+  FOR_TILES(total, t) {
+    const int q = find_item(start, nitems, t);
+    const PhaseItem& I = items[q];
+    const uint32_t lt = t - start[q];
+    const uint32_t e = lt * 256u + threadIdx.x;
+    const bool in = e < I.n;
+    int i = 0, j = 0, k = 0;
+    if constexpr (OP == PH_IBSUM) {
+      double fl = 0., ar = 0.;
+      if (in) ibflux_sum_face(I.faces[e], I.w0, I.i0, fl, ar);
+      fl = block_sum(fl, lds);
+      ar = block_sum(ar, lds);
+      if (threadIdx.x == 0) { partial[t] = fl; partial[total + t] = ar; }
+    } else if constexpr (OP == PH_IBCORR) {
+      if (in) ibflux_correct_face(I.faces[e], I.w0, I.i0, I.sc);
+    } else if constexpr (OP == PH_RHS) {
+      double v = 0.;
+      if (flat_cell(I.d, lt, i, j, k)) v = rhs_cell(I.m, I.d, I.r0, I.cls, I.s0, I.w0, i, j, k);
+      v = block_sum(v, lds);
+      if (threadIdx.x == 0) partial[t] = v;
+    } else if constexpr (OP == PH_GATHER) {
+      if (flat_cell(I.d, lt, i, j, k)) gather_phi_cell(I.m, I.d, I.r0, I.cls, I.w0, i, j, k);
+    } else if constexpr (OP == PH_ZERO) {
+      if (in) I.w0[e] = 0.;
+    } else if constexpr (OP == PH_UPDATE_X) {
+      if (in) update_x_elem(I.w0, static_cast<const T*>(I.z), I.ld, I.i0, I.sc, e);
+    } else if constexpr (OP == PH_RESID) {
+      double v = 0.;
+      if (flat_cell(I.d, lt, i, j, k)) v = residual_cell<double>(I.d, I.g, I.code, I.r0, I.r1, I.w0, i, j, k);
+      v = block_sum(v, lds);
+      if (threadIdx.x == 0) partial[t] = v;
+    } else if constexpr (OP == PH_DIVEST) {
+      double v = in ? divest_elem(I.code, I.r0, I.r1, e) : 0.;
+      v = block_max(v, lds);
+      if (threadIdx.x == 0) partial[t] = v;
+    } else if constexpr (OP == PH_SCALE_CONV) {
+      if (in) scale_conv_elem(I.r0, I.sc[0], I.w0, static_cast<T*>(I.f), e);
+      if (I.one && e == 0) I.one[0] = 1.;
+    } else if constexpr (OP == PH_MASKED_SUM) {
+      double v = (in && (I.cls[e] & CLS_LT)) ? I.r0[e] : 0.;
+      v = block_sum(v, lds);
+      if (threadIdx.x == 0) partial[t] = v;
+    } else if constexpr (OP == PH_SUB_MEAN) {
+      if (in) {
+        const double val = -I.sc[0] / I.s0;  // poisson_hypre.c:256
+        if (I.cls[e] & CLS_LT) I.w0[e] += val;
+      }
+    } else if constexpr (OP == PH_SCATTER) {
+      if (flat_cell(I.d, lt, i, j, k)) scatter_cell(I.m, I.d, I.r0, I.cls, I.w0, nullptr, i, j, k);
+    } else if constexpr (OP == PH_PERIODIC) {
+      if (in && boundary_point(I.m, e, i, j, k)) periodic_point(I.m, I.w0, I.i0, i, j, k);
+    } else if constexpr (OP == PH_ADD_PHI) {
+      if (flat_cell(I.d, lt, i, j, k)) add_phi_cell(I.m, I.d, I.cls, I.r0, I.w0, i, j, k);
+    } else if constexpr (OP == PH_PROJECTION) {
+      if (flat_cell(I.d, lt, i, j, k)) projection_cell(I.m, I.d, I.g, I.code, I.cls, I.r0, I.w0, I.s0, i, j, k);
+    } else if constexpr (OP == PH_UCONT_GHOSTS) {
+      if (in && boundary_point(I.m, e, i, j, k)) ucont_ghost_point(I.m, I.w0, i, j, k);
+    } else if constexpr (OP == PH_C2C_CORE) {
+      if (flat_cell(I.d, lt, i, j, k)) c2c_core_cell(I.m, I.d, I.r0, I.cls, I.r1, I.w0, i, j, k);
+    } else if constexpr (OP == PH_VEC_GHOSTS) {
+      if (in && boundary_point(I.m, e, i, j, k)) vec_ghost_point(I.m, I.w0, I.w1, i, j, k);
+    } else if constexpr (OP == PH_C2C_BC) {
+      if (in && boundary_point(I.m, e, i, j, k)) c2c_bc_point(I.m, I.r0, I.r1, I.w0, I.bc, i, j, k);
+    } else if constexpr (OP == PH_C2C_SOLID) {
+      if (flat_cell(I.d, lt, i, j, k)) c2c_solid_cell(I.m, I.d, I.cls, I.w0, i, j, k);
+    } else if constexpr (OP == PH_DIVERGENCE) {
+      double v = 0.;
+      if (flat_cell(I.d, lt, i, j, k)) v = divergence_cell(I.m, I.d, I.code, I.r0, I.r1, i, j, k);
+      v = block_max(v, lds);
+      if (threadIdx.x == 0) partial[t] = v;
+    }
+  }
+}
+
+// One block per (case, value v): the case's tile partials summed (or maxed) in a
+// fixed order into out[v], mirrored to pinned host memory.
+template <bool MAX>
+__global__ __launch_bounds__(256) void kb_phase_fin(const PhaseFin* __restrict__ fins,
+                                                    const double* __restrict__ partial, uint32_t total) {
+  __shared__ double lds[4];
+  const PhaseFin& F = fins[blockIdx.x];
+  const int v = blockIdx.y;
+  const double* pv = partial + (size_t)v * total;
+  const double s = MAX ? tile_max(pv, F.t0, F.t1, lds) : tile_sum(pv, F.t0, F.t1, lds);
+  if (threadIdx.x == 0) {
+    F.out[v] = s;
+    if (F.host) F.host[v] = s;
+  }
+}
+
 // workers for a launch of `total` tiles: a multiple of 8, at most `cap` (0: one
 // tile per worker)
 inline unsigned workers(uint32_t total, unsigned cap) {
@@ -494,6 +659,15 @@ struct BatchCore {
 
   enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS, K_AXN };
   bool tuned_once = false;
+  // batched phase steps: pinned staging and device copy of their work lists, and
+  // the tile partials of their reductions
+  char* ph_host = nullptr;
+  char* ph_dev = nullptr;
+  size_t ph_cap = 0;
+  double* ph_partial = nullptr;
+  size_t ph_partial_cap = 0;
+  hipEvent_t ph_ev = nullptr;  // after the last upload from ph_host
+  bool ph_recorded = false;
 
   explicit BatchCore(int nslots) {
     HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamDefault));  // blocking: ordered with the legacy default stream
@@ -501,12 +675,18 @@ struct BatchCore {
     sd = fixed.alloc<SlotDev>(nslots);
     HIPCHECK(hipHostMalloc(&sd_host, sizeof(SlotDev) * nslots));
     std::memset(sd_host, 0, sizeof(SlotDev) * nslots);
+    HIPCHECK(hipEventCreateWithFlags(&ph_ev, hipEventDisableTiming));
   }
   ~BatchCore() {
     for (auto& sl : slots)
       if (sl.h) release(sl);
     drop_graph();
+    if (s) (void)hipStreamSynchronize(s);
     if (sd_host) (void)hipHostFree(sd_host);
+    if (ph_host) (void)hipHostFree(ph_host);
+    if (ph_dev) (void)hipFree(ph_dev);
+    if (ph_partial) (void)hipFree(ph_partial);
+    if (ph_ev) (void)hipEventDestroy(ph_ev);
     if (s) (void)hipStreamDestroy(s);
   }
 
@@ -994,44 +1174,302 @@ struct BatchCore {
     sl.sn.assign(sl.m, 0.);
     sl.gv.assign(sl.m + 1, 0.);
     sl.y.assign(sl.m, 0.);
-    solve_prologue(sl.h, ucont, dt, st, coeff);
+    solve_prologue_host(sl.h, dt, st, coeff);  // VolumeFlux and RHS run with the next start_cycles
     sl.fresh = true;
     sl.state = START;
   }
 
-  // cases whose cycle starts: x update from the finished cycle, true residual
+  // ------------------------------------------------------ batched phase steps
+  struct PhaseStep {
+    int op = 0;
+    int nval = 0;      // values reduced per case (0: none)
+    bool max = false;  // reduction by maximum instead of sum
+    std::vector<PhaseItem> items;
+    std::vector<PhaseFin> fins;  // destination of each item's reduced values
+    std::vector<int> data;       // per item: index of a Phase::data array patched into sc, or -1
+    void add(const PhaseItem& it, double* out = nullptr, double* host = nullptr, int d = -1) {
+      if (it.n == 0) return;
+      items.push_back(it);
+      fins.push_back(PhaseFin{out, host, 0, 0});
+      data.push_back(d);
+    }
+  };
+  struct Phase {
+    std::vector<PhaseStep> steps;
+    std::vector<std::vector<double>> data;
+    // the returned step is filled before the next one is added
+    PhaseStep& step(int op, int nval = 0, bool max = false) {
+      steps.emplace_back();
+      PhaseStep& S = steps.back();
+      S.op = op;
+      S.nval = nval;
+      S.max = max;
+      return S;
+    }
+  };
+
+  static PhaseItem case_item(const vfs_pp_handle* h) {
+    PhaseItem it{};
+    it.m = h->m;
+    it.d = h->d;
+    it.g = h->g64;
+    it.code = h->code;
+    it.cls = h->cls;
+    it.n = (uint32_t)h->d.n();
+    return it;
+  }
+  static PhaseItem boundary_item(const vfs_pp_handle* h) {
+    PhaseItem it = case_item(h);
+    it.n = boundary_count(h->m);
+    return it;
+  }
+
+  void ph_launch(int op, const PhaseItem* it, const uint32_t* st, int n, uint32_t total) {
+    const unsigned W = workers(total, 2048);
+    switch (op) {
+#define PH_CASE(OP) \
+  case OP: kb_phase<OP, T><<<W, 256, 0, s>>>(it, st, n, total, ph_partial); break;
+      PH_CASE(PH_IBSUM) PH_CASE(PH_IBCORR) PH_CASE(PH_RHS) PH_CASE(PH_GATHER) PH_CASE(PH_ZERO)
+      PH_CASE(PH_UPDATE_X) PH_CASE(PH_RESID) PH_CASE(PH_DIVEST) PH_CASE(PH_SCALE_CONV) PH_CASE(PH_MASKED_SUM)
+      PH_CASE(PH_SUB_MEAN) PH_CASE(PH_SCATTER) PH_CASE(PH_PERIODIC) PH_CASE(PH_ADD_PHI) PH_CASE(PH_PROJECTION)
+      PH_CASE(PH_UCONT_GHOSTS) PH_CASE(PH_C2C_CORE) PH_CASE(PH_VEC_GHOSTS) PH_CASE(PH_C2C_BC)
+      PH_CASE(PH_C2C_SOLID) PH_CASE(PH_DIVERGENCE)
+#undef PH_CASE
+      default: throw StatusError(VFS_PP_EINVAL, "unknown phase step");
+    }
+    HIPCHECK(hipGetLastError());
+  }
+
+  // Upload a phase's work lists (one copy from pinned staging) and launch its
+  // steps in order on the batch stream.  Launches only.
+  void ph_run(Phase& P) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t bytes = 0, need = 0;
+    std::vector<size_t> doff(P.data.size());
+    for (size_t q = 0; q < P.data.size(); q++) {
+      doff[q] = bytes;
+      bytes = al(bytes + P.data[q].size() * sizeof(double));
+    }
+    struct Off {
+      size_t it, st, fin;
+      uint32_t total;
+    };
+    std::vector<Off> off(P.steps.size());
+    for (size_t l = 0; l < P.steps.size(); l++) {
+      const PhaseStep& S = P.steps[l];
+      uint32_t tot = 0;
+      for (const PhaseItem& it : S.items) tot += (it.n + 255u) / 256u;
+      off[l].total = tot;
+      off[l].it = bytes;
+      bytes = al(bytes + S.items.size() * sizeof(PhaseItem));
+      off[l].st = bytes;
+      bytes = al(bytes + S.items.size() * sizeof(uint32_t));
+      off[l].fin = bytes;
+      bytes = al(bytes + S.items.size() * sizeof(PhaseFin));
+      need = std::max(need, (size_t)S.nval * tot);
+    }
+    if (bytes == 0) return;
+    if (ph_recorded) HIPCHECK(hipEventSynchronize(ph_ev));  // the staging buffer's previous upload was read
+    if (bytes > ph_cap) {
+      HIPCHECK(hipStreamSynchronize(s));
+      if (ph_host) (void)hipHostFree(ph_host);
+      if (ph_dev) (void)hipFree(ph_dev);
+      ph_host = nullptr;
+      ph_dev = nullptr;
+      ph_cap = 0;
+      const size_t cap = std::max<size_t>(2 * bytes, (size_t)1 << 16);
+      HIPCHECK(hipHostMalloc(&ph_host, cap));
+      HIPCHECK(hipMalloc(&ph_dev, cap));
+      ph_cap = cap;
+    }
+    if (need > ph_partial_cap) {
+      HIPCHECK(hipStreamSynchronize(s));
+      if (ph_partial) (void)hipFree(ph_partial);
+      ph_partial = nullptr;
+      ph_partial_cap = 0;
+      const size_t cap = std::max<size_t>(2 * need, 4096);
+      HIPCHECK(hipMalloc(&ph_partial, cap * sizeof(double)));
+      ph_partial_cap = cap;
+    }
+    // serialise, with the device addresses of the data arrays patched into the items
+    for (size_t q = 0; q < P.data.size(); q++)
+      std::memcpy(ph_host + doff[q], P.data[q].data(), P.data[q].size() * sizeof(double));
+    for (size_t l = 0; l < P.steps.size(); l++) {
+      const PhaseStep& S = P.steps[l];
+      auto* it = reinterpret_cast<PhaseItem*>(ph_host + off[l].it);
+      auto* st = reinterpret_cast<uint32_t*>(ph_host + off[l].st);
+      auto* fn = reinterpret_cast<PhaseFin*>(ph_host + off[l].fin);
+      uint32_t acc = 0;
+      for (size_t q = 0; q < S.items.size(); q++) {
+        it[q] = S.items[q];
+        if (S.data[q] >= 0) it[q].sc = reinterpret_cast<const double*>(ph_dev + doff[S.data[q]]);
+        const uint32_t nt = (S.items[q].n + 255u) / 256u;
+        st[q] = acc;
+        fn[q] = PhaseFin{S.fins[q].out, S.fins[q].host, acc, acc + nt};
+        acc += nt;
+      }
+    }
+    HIPCHECK(hipMemcpyAsync(ph_dev, ph_host, bytes, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipEventRecord(ph_ev, s));
+    ph_recorded = true;
+    for (size_t l = 0; l < P.steps.size(); l++) {
+      const PhaseStep& S = P.steps[l];
+      if (off[l].total == 0) continue;
+      ph_launch(S.op, reinterpret_cast<const PhaseItem*>(ph_dev + off[l].it),
+                reinterpret_cast<const uint32_t*>(ph_dev + off[l].st), (int)S.items.size(), off[l].total);
+      if (S.nval) {
+        const auto* fn = reinterpret_cast<const PhaseFin*>(ph_dev + off[l].fin);
+        const dim3 gr((unsigned)S.items.size(), (unsigned)S.nval);
+        if (S.max) kb_phase_fin<true><<<gr, 256, 0, s>>>(fn, ph_partial, off[l].total);
+        else kb_phase_fin<false><<<gr, 256, 0, s>>>(fn, ph_partial, off[l].total);
+        HIPCHECK(hipGetLastError());
+      }
+    }
+  }
+
+  // Cases whose cycle starts: for new cases VolumeFlux, RHS and the initial guess;
+  // for the others the x update from the finished cycle; then the true residual
+  // of all of them and, for those that go on, v_0 = r / ||r||.
   void start_cycles() {
-    std::vector<int> qs;
-    bool fresh = false;
+    std::vector<int> qs, fr;
     for (int q = 0; q < (int)slots.size(); q++)
       if (slots[q].state == START) {
         qs.push_back(q);
-        fresh = fresh || slots[q].fresh;
+        if (slots[q].fresh) fr.push_back(q);
       }
     if (qs.empty()) return;
-    if (fresh) {
-      // ||b|| and the VolumeFlux figures of new cases, then their initial guess
+    if (!fr.empty()) {
+      // VolumeFlux (poisson.c:2640-2980) and the RHS (poisson_hypre.c:346-376)
+      Phase P;
+      P.steps.reserve(4);
+      {
+        PhaseStep& S = P.step(PH_IBSUM, 2);
+        for (int q : fr) {
+          vfs_pp_handle* h = slots[q].h;
+          if (!h->nibf) continue;
+          PhaseItem it = case_item(h);
+          it.w0 = slots[q].ucont;
+          it.faces = h->ibf;
+          it.n = (uint32_t)h->nibf;
+          it.i0 = h->grid.immersed == 1 ? 1 : 0;
+          S.add(it, h->red + RED_IBF, h->hpin_dev + RED_IBF);
+        }
+      }
+      {
+        PhaseStep& S = P.step(PH_IBCORR);
+        for (int q : fr) {
+          vfs_pp_handle* h = slots[q].h;
+          if (!h->nibf) continue;
+          PhaseItem it = case_item(h);
+          it.w0 = slots[q].ucont;
+          it.faces = h->ibf;
+          it.n = (uint32_t)h->nibf;
+          it.i0 = h->grid.immersed;
+          it.sc = h->red + RED_IBF;
+          S.add(it);
+        }
+      }
+      {
+        PhaseStep& S = P.step(PH_IBSUM, 2);
+        for (int q : fr) {
+          vfs_pp_handle* h = slots[q].h;
+          if (!h->nibf) continue;
+          PhaseItem it = case_item(h);
+          it.w0 = slots[q].ucont;
+          it.faces = h->ibf;
+          it.n = (uint32_t)h->nibf;
+          it.i0 = 0;
+          S.add(it, h->red + RED_IBF + 2, h->hpin_dev + RED_IBF + 2);
+        }
+      }
+      {
+        PhaseStep& S = P.step(PH_RHS, 1);
+        for (int q : fr) {
+          const Slot& sl = slots[q];
+          vfs_pp_handle* h = sl.h;
+          PhaseItem it = case_item(h);
+          it.r0 = sl.ucont;
+          it.w0 = h->b;
+          it.s0 = -1.0 / sl.dt * sl.st * sl.coeff;  // poisson_hypre.c:376
+          S.add(it, h->red + RED_B2, h->hpin_dev + RED_B2);
+        }
+      }
+      ph_run(P);
       HIPCHECK(hipStreamSynchronize(s));
-      for (int q : qs) {
+      for (int q : fr) {
         Slot& sl = slots[q];
-        if (!sl.fresh) continue;
-        solve_start(sl.h, sl.phi, &sl.S);
+        solve_start_stats(sl.h, &sl.S);
         sl.tgt = sl.h->opts.rtol * sl.S.bnorm;
         sl.fresh = false;
       }
     }
-    for (int q : qs) {
-      Slot& sl = slots[q];
-      vfs_pp_handle* h = sl.h;
-      if (sl.pending_update) {
-        for (int l = 0; l < sl.pending_nv; l++) h->hpin[200 + l] = lazy ? sl.y[l] / sl.sigma[l] : sl.y[l];
-        HIPCHECK(hipMemcpyAsync(h->red + 200, h->hpin + 200, sl.pending_nv * sizeof(double), hipMemcpyHostToDevice, s));
-        launch_update_x<T>(s, h->d.n(), h->x, static_cast<const T*>(h->Z), h->d.n(), sl.pending_nv, h->red + 200);
+    Phase R;
+    R.steps.reserve(5);
+    {
+      PhaseStep& S = R.step(PH_ZERO);
+      for (int q : fr) {
+        vfs_pp_handle* h = slots[q].h;
+        if (!h->opts.zero_guess) continue;
+        PhaseItem it = case_item(h);
+        it.w0 = h->x;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = R.step(PH_GATHER);
+      for (int q : fr) {
+        vfs_pp_handle* h = slots[q].h;
+        if (h->opts.zero_guess) continue;
+        PhaseItem it = case_item(h);
+        it.r0 = slots[q].phi;
+        it.w0 = h->x;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = R.step(PH_UPDATE_X);
+      for (int q : qs) {
+        Slot& sl = slots[q];
+        if (!sl.pending_update) continue;
+        vfs_pp_handle* h = sl.h;
+        std::vector<double> y(sl.pending_nv);
+        for (int l = 0; l < sl.pending_nv; l++) y[l] = lazy ? sl.y[l] / sl.sigma[l] : sl.y[l];
+        R.data.push_back(std::move(y));
+        PhaseItem it = case_item(h);
+        it.w0 = h->x;
+        it.z = h->Z;
+        it.ld = h->d.n();
+        it.i0 = sl.pending_nv;
+        S.add(it, nullptr, nullptr, (int)R.data.size() - 1);
         sl.pending_update = false;
       }
-      launch_true_residual(h, s);
     }
+    {
+      PhaseStep& S = R.step(PH_RESID, 1);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        PhaseItem it = case_item(h);
+        it.r0 = h->b;
+        it.r1 = h->x;
+        it.w0 = h->r;
+        S.add(it, h->red, h->hpin_dev);
+      }
+    }
+    {
+      PhaseStep& S = R.step(PH_DIVEST, 1, true);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        if (!(h->div_fac > 0.)) continue;
+        PhaseItem it = case_item(h);
+        it.r0 = h->aj;
+        it.r1 = h->r;
+        S.add(it, h->red + RED_DIVEST, h->hpin_dev + RED_DIVEST);
+      }
+    }
+    ph_run(R);
     HIPCHECK(hipStreamSynchronize(s));
+    Phase C;
+    PhaseStep& SC = C.step(PH_SCALE_CONV);
     for (int q : qs) {
       Slot& sl = slots[q];
       vfs_pp_handle* h = sl.h;
@@ -1041,10 +1479,16 @@ struct BatchCore {
       sl.rel = beta / sl.S.bnorm;
       if (residual_verdict(h, beta, sl.S.bnorm, sl.tgt)) { sl.conv = true; sl.state = FINISH; continue; }
       if (sl.it >= h->opts.max_iters) { sl.state = FINISH; continue; }
-      launch_scale_conv<T>(s, h->d.n(), h->r, h->red, h->V, static_cast<T*>(sl.mg->f0()));
+      // v_0 = r / beta and its copy as the first V-cycle input; a lazily normalised
+      // basis reads 1 / |U_0|^2 = 1
+      PhaseItem it = case_item(h);
+      it.r0 = h->r;
+      it.w0 = h->V;
+      it.f = sl.mg->f0();
+      it.sc = h->red;
+      it.one = lazy ? h->red + RED_NUINV2 : nullptr;
+      SC.add(it);
       if (lazy) {
-        // U_0 = v_0, read by the first Gram-Schmidt pass as 1 / |U_0|^2 = 1
-        launch_fill(s, 1, h->red + RED_NUINV2, 1.);
         sl.nu.assign(sl.m + 1, 1.);
         sl.sigma.assign(sl.m + 1, 1.);
       }
@@ -1053,6 +1497,7 @@ struct BatchCore {
       sl.j = 0;
       sl.state = ITER;
     }
+    ph_run(C);
   }
 
   // A second Gram-Schmidt pass over U_nv of one case (its first pass cancelled
@@ -1136,19 +1581,157 @@ struct BatchCore {
     }
   }
 
-  // cases that are done iterating: null space, Phi, P, projection, divergence
+  // Cases that are done iterating: Remove_Nullspace and Convert_Phi2_Phi
+  // (poisson_hypre.c:207-270, poisson.c:97-185), UpdatePressure (poisson.c:2522-2614),
+  // Projection (poisson.c:2190-2441), Contra2Cart (rhs.c:151-260) and Divergence
+  // (main.c:1244-1391), each step one launch over all of them.
   void finish_cases(std::vector<int>* done) {
     std::vector<int> qs;
     for (int q = 0; q < (int)slots.size(); q++)
       if (slots[q].state == FINISH) qs.push_back(q);
     if (qs.empty()) return;
-    for (int q : qs) {
-      Slot& sl = slots[q];
-      solve_epilogue(sl.h, sl.phi, &sl.S);
-      update_pressure(sl.h, sl.p, sl.phi);
-      projection(sl.h, sl.ucont, sl.phi, sl.ucat, sl.dt, sl.st, sl.coeff);
-      divergence_launch(sl.h, sl.ucont);
+    auto periodic = [](const vfs_pp_handle* h) { return h->m.per[0] || h->m.per[1] || h->m.per[2]; };
+    Phase F;
+    F.steps.reserve(16);
+    {
+      PhaseStep& S = F.step(PH_MASKED_SUM, 1);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        PhaseItem it = case_item(h);
+        it.r0 = h->x;
+        S.add(it, h->red + RED_MEAN, h->hpin_dev + RED_MEAN);
+      }
     }
+    {
+      PhaseStep& S = F.step(PH_SUB_MEAN);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        if (!(h->n_lt > 0.)) continue;
+        PhaseItem it = case_item(h);
+        it.w0 = h->x;
+        it.sc = h->red + RED_MEAN;
+        it.s0 = h->n_lt;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_SCATTER);
+      for (int q : qs) {
+        PhaseItem it = case_item(slots[q].h);
+        it.r0 = slots[q].h->x;
+        it.w0 = slots[q].phi;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_PERIODIC);
+      for (int q : qs) {
+        PhaseItem it = boundary_item(slots[q].h);
+        it.w0 = slots[q].phi;
+        it.i0 = 1;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_ADD_PHI);
+      for (int q : qs) {
+        PhaseItem it = case_item(slots[q].h);
+        it.r0 = slots[q].phi;
+        it.w0 = slots[q].p;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_PERIODIC);
+      for (int q : qs) {
+        PhaseItem it = boundary_item(slots[q].h);
+        it.i0 = 0;
+        it.w0 = slots[q].p;
+        S.add(it);
+        it.w0 = slots[q].phi;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_PROJECTION);
+      for (int q : qs) {
+        const Slot& sl = slots[q];
+        PhaseItem it = case_item(sl.h);
+        it.r0 = sl.phi;
+        it.w0 = sl.ucont;
+        it.s0 = sl.dt * sl.st / sl.coeff;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_UCONT_GHOSTS);
+      for (int q : qs) {
+        if (!periodic(slots[q].h)) continue;
+        PhaseItem it = boundary_item(slots[q].h);
+        it.w0 = slots[q].ucont;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_C2C_CORE);
+      for (int q : qs) {
+        PhaseItem it = case_item(slots[q].h);
+        it.r0 = slots[q].h->met9;
+        it.r1 = slots[q].ucont;
+        it.w0 = slots[q].ucat;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_VEC_GHOSTS);
+      for (int q : qs) {
+        PhaseItem it = boundary_item(slots[q].h);
+        it.w0 = slots[q].ucat;
+        it.w1 = slots[q].h->snap;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_C2C_BC);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        PhaseItem it = boundary_item(h);
+        it.r0 = h->nvert;
+        it.r1 = h->snap;
+        it.w0 = slots[q].ucat;
+        for (int f = 0; f < 6; f++) it.bc[f] = h->grid.bctype[f];
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_C2C_SOLID);
+      for (int q : qs) {
+        PhaseItem it = case_item(slots[q].h);
+        it.w0 = slots[q].ucat;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_VEC_GHOSTS);
+      for (int q : qs) {
+        if (!periodic(slots[q].h)) continue;
+        PhaseItem it = boundary_item(slots[q].h);
+        it.w0 = slots[q].ucat;
+        it.w1 = nullptr;
+        S.add(it);
+      }
+    }
+    {
+      PhaseStep& S = F.step(PH_DIVERGENCE, 1, true);
+      for (int q : qs) {
+        vfs_pp_handle* h = slots[q].h;
+        PhaseItem it = case_item(h);
+        it.r0 = h->aj;
+        it.r1 = slots[q].ucont;
+        S.add(it, h->red + RED_DIV, h->hpin_dev + RED_DIV);
+      }
+    }
+    ph_run(F);
     HIPCHECK(hipStreamSynchronize(s));
     for (int q : qs) {
       Slot& sl = slots[q];

@@ -10,6 +10,7 @@
 #include "pp_kernels.hpp"
 #include "pp_stencil.hpp"
 #include "pp_krylov_dev.hpp"
+#include "pp_path_dev.hpp"
 
 namespace pp {
 
@@ -20,19 +21,6 @@ inline dim3 dmda_grid(const Dmda& m) { return dim3((m.mx + 63) / 64, (m.my + 3) 
 const dim3 kBlk(64, 4, 1);
 
 __device__ inline int blin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-
-// periodic image of DMDA point (poisson.c:153-174)
-__device__ inline bool image_of(const Dmda& m, int i, int j, int k, int& a, int& b, int& c, int fl[3]) {
-  a = i; b = j; c = k;
-  fl[0] = fl[1] = fl[2] = 0;
-  if (m.per[0] && i == 0) { a = m.mx - 2; fl[0] = 1; }
-  else if (m.per[0] && i == m.mx - 1) { a = 1; fl[0] = 1; }
-  if (m.per[1] && j == 0) { b = m.my - 2; fl[1] = 1; }
-  else if (m.per[1] && j == m.my - 1) { b = 1; fl[1] = 1; }
-  if (m.per[2] && k == 0) { c = m.mz - 2; fl[2] = 1; }
-  else if (m.per[2] && k == m.mz - 1) { c = 1; fl[2] = 1; }
-  return fl[0] | fl[1] | fl[2];
-}
 
 // ------------------------------------------------------------------------ set-up
 __device__ inline double dot3_exact(const double* a, const double* b) {
@@ -166,18 +154,7 @@ __global__ __launch_bounds__(256) void k_residual64(Dims d, const double* __rest
   __shared__ double lds[4];
   CELL_IJK(i, j, k);
   double rr = 0.;
-  if (i < d.nx && j < d.ny) {
-    const size_t c = d.idx(i, j, k);
-    const uint32_t cd = code[c];
-    Nbr nb;
-    neighbours(d, i, j, k, nb);
-    double xv[19], ax, dg;
-    load_x(nb, x, xv);
-    flux_row(g, d.n(), cd, c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, dg);
-    const double res = (cd & ROW_BIT) ? b[c] - ax : 0.;
-    r[c] = res;
-    rr = res * res;
-  }
+  if (i < d.nx && j < d.ny) rr = residual_cell(d, g, code, b, x, r, i, j, k);
   rr = block_sum(rr, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = rr;
 }
@@ -213,23 +190,7 @@ __global__ __launch_bounds__(256) void k_rhs(Dmda m, Dims d, const double* __res
   __shared__ double lds[4];
   CELL_IJK(i, j, k);
   double bb = 0.;
-  if (i < d.nx && j < d.ny) {
-    const size_t c = d.idx(i, j, k);
-    const int I = i + 1, J = j + 1, K = k + 1;
-    const size_t p = m.idx(I, J, K);
-    double val = 0.;
-    if (cls[c] & CLS_LT) {  // poisson_hypre.c:346-376
-      val -= u[3 * p + 0];
-      val += (I == 1 && m.per[0]) ? u[3 * m.idx(m.mx - 2, J, K) + 0] : u[3 * (p - 1) + 0];
-      val -= u[3 * p + 1];
-      val += (J == 1 && m.per[1]) ? u[3 * m.idx(I, m.my - 2, K) + 1] : u[3 * (p - m.mx) + 1];
-      val -= u[3 * p + 2];
-      val += (K == 1 && m.per[2]) ? u[3 * m.idx(I, J, m.mz - 2) + 2] : u[3 * (p - (size_t)m.mx * m.my) + 2];
-      val *= fac;
-    }
-    b[c] = val;
-    bb = val * val;
-  }
+  if (i < d.nx && j < d.ny) bb = rhs_cell(m, d, u, cls, fac, b, i, j, k);
   bb = block_sum(bb, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = bb;
 }
@@ -239,17 +200,8 @@ __global__ __launch_bounds__(256) void k_ibflux_sum(const IbFace* __restrict__ f
                                                     int zero_small, double* __restrict__ partial, int nblk) {
   __shared__ double lds[4];
   double fl = 0., ar = 0.;
-  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < nf; t += (size_t)nblk * 256) {
-    const IbFace F = f[t];
-    const double v = u[F.pos];
-    if (zero_small) {  // poisson.c:2938-2957: |u| <= 1e-10 faces are zeroed and skipped
-      if (fabs(v) > 1.e-10) { fl += F.sign * v; ar += F.area; }
-      else u[F.pos] = 0.;
-    } else {
-      fl += F.sign * v;
-      ar += F.area;
-    }
-  }
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < nf; t += (size_t)nblk * 256)
+    ibflux_sum_face(f[t], u, zero_small, fl, ar);
   fl = block_sum(fl, lds);
   ar = block_sum(ar, lds);
   if (threadIdx.x == 0) { partial[blockIdx.x] = fl; partial[nblk + blockIdx.x] = ar; }
@@ -260,15 +212,7 @@ __global__ __launch_bounds__(256) void k_ibflux_correct(const IbFace* __restrict
                                                         const double* __restrict__ sums) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= nf) return;
-  const double area = sums[1];
-  const double corr = area > 1.e-15 ? sums[0] / area : 0.;
-  const IbFace F = f[t];
-  if (immersed == 0) {  // poisson.c:2715-2750: corrected then set to zero
-    u[F.pos] = 0.;
-    return;
-  }
-  const double v = u[F.pos];
-  if (fabs(v) > 1.e-10) u[F.pos] = v - F.sign * (F.area * corr);
+  ibflux_correct_face(f[t], u, immersed, sums);
 }
 
 // ------------------------------------------------------------ null space, scatter
@@ -297,11 +241,7 @@ __global__ __launch_bounds__(256) void k_scatter_update(Dmda m, Dims d, const do
                                                         double* __restrict__ p) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= d.nx || j >= d.ny) return;
-  const size_t c = d.idx(i, j, k), q = m.idx(i + 1, j + 1, k + 1);
-  const uint8_t b = cls[c];
-  const double ph = (b & CLS_INTGT) ? 0. : x[c];  // Convert_Phi2_Phi, poisson.c:131-139
-  phi[q] = ph;
-  if (p && (b & CLS_LT)) p[q] += ph;  // UpdatePressure, poisson.c:2548
+  scatter_cell(m, d, x, cls, phi, p, i, j, k);
 }
 
 // cell metrics csi/eta/zet and aj of the interior cells, SoA (met9[3 v + q])
@@ -321,225 +261,94 @@ __global__ __launch_bounds__(256) void k_gather_cell_metrics(Dmda m, Dims d, con
   ajc[c] = aj[p];
 }
 
-// Initial guess from the caller's Phi: the inverse of Convert_Phi2_Phi.  The
-// reference keeps its HYPRE solution vector between time steps and GMRES starts
-// from it (poisson_hypre.c:565-574, 722); that vector equals Phi on every cell
-// that carries an unknown.
 __global__ __launch_bounds__(256) void k_gather_phi(Dmda m, Dims d, const double* __restrict__ phi,
                                                     const uint8_t* __restrict__ cls, double* __restrict__ x) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= d.nx || j >= d.ny) return;
-  const size_t c = d.idx(i, j, k);
-  x[c] = (cls[c] & CLS_INTGT) ? 0. : phi[m.idx(i + 1, j + 1, k + 1)];
+  gather_phi_cell(m, d, phi, cls, x, i, j, k);
 }
 
-// UpdatePressure: P += Phi on interior cells with nvert < 0.1 (poisson.c:2548)
 __global__ __launch_bounds__(256) void k_add_phi(Dmda m, Dims d, const uint8_t* __restrict__ cls,
                                                  const double* __restrict__ phi, double* __restrict__ p) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= d.nx || j >= d.ny) return;
-  if (!(cls[d.idx(i, j, k)] & CLS_LT)) return;
-  const size_t q = m.idx(i + 1, j + 1, k + 1);
-  p[q] += phi[q];
+  add_phi_cell(m, d, cls, phi, p, i, j, k);
 }
 
-// ghost planes of a DMDA scalar: periodic images; other boundary points are
-// zeroed when zero_other (Convert_Phi2_Phi starts from VecSet(Phi,0))
 __global__ __launch_bounds__(256) void k_periodic_scalar(Dmda m, double* __restrict__ a, int zero_other) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= m.mx || j >= m.my) return;
-  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
-  if (!bnd) return;
-  int aa, bb, cc, fl[3];
-  if (image_of(m, i, j, k, aa, bb, cc, fl)) {
-    // the image of a ghost point with one periodic flag may itself be a non-periodic
-    // boundary point; resolve it the same way
-    const bool ibnd = aa == 0 || bb == 0 || cc == 0 || aa == m.mx - 1 || bb == m.my - 1 || cc == m.mz - 1;
-    if (ibnd && zero_other) a[m.idx(i, j, k)] = 0.;
-    else a[m.idx(i, j, k)] = a[m.idx(aa, bb, cc)];
-  } else if (zero_other) {
-    a[m.idx(i, j, k)] = 0.;
-  }
+  if (on_boundary(m, i, j, k)) periodic_point(m, a, zero_other, i, j, k);
 }
 
 // ------------------------------------------------------------------ Projection
-// Cross difference of phi for the face (p, q) from its case code (poisson.c:2234-2257).
-__device__ inline double proj_cross(int cc, const double* __restrict__ phi, size_t p, size_t q, size_t se) {
-  switch (cc) {
-    case X_LOWER: return (phi[p] + phi[q] - phi[p - se] - phi[q - se]) * 0.5;
-    case X_UPPER: return (phi[p + se] + phi[q + se] - phi[p] - phi[q]) * 0.5;
-    case X_CENTRAL: return (phi[p + se] + phi[q + se] - phi[p - se] - phi[q - se]) * 0.25;
-    default: return 0.;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_projection(Dmda m, Dims d, const double* __restrict__ g,
                                                     const uint32_t* __restrict__ code,
                                                     const uint8_t* __restrict__ cls, const double* __restrict__ phi,
                                                     double* __restrict__ u, double fac) {
   CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
-  const size_t n = d.n(), c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
-  const uint8_t b = cls[c];
-  if (!(b & (CLS_PROJ_I | CLS_PROJ_J | CLS_PROJ_K))) return;
-  const uint32_t cd = code[c];
-  const size_t sx = 1, sy = m.mx, sz = (size_t)m.mx * m.my;
-  // the cell's three face fluxes are read, corrected and stored together, so each
-  // cache line of the AoS array is written once
-  double u0 = u[3 * p + 0], u1 = u[3 * p + 1], u2 = u[3 * p + 2];
-  if (b & CLS_PROJ_I) {  // poisson.c:2221-2265; face i+ is face 0, cross j then k
-    const size_t q = p + sx;
-    const double dc = phi[q] - phi[p];
-    const double de = proj_cross((cd >> 6) & 3u, phi, p, q, sy);
-    const double dz = proj_cross((cd >> 8) & 3u, phi, p, q, sz);
-    u0 -= (dc * g[0 * n + c] + de * g[1 * n + c] + dz * g[2 * n + c]) * fac;
-  }
-  if (b & CLS_PROJ_J) {  // poisson.c:2267-2314; face j+ is face 2, cross i then k
-    const size_t q = p + sy;
-    const double dc = proj_cross((cd >> 14) & 3u, phi, p, q, sx);
-    const double de = phi[q] - phi[p];
-    const double dz = proj_cross((cd >> 16) & 3u, phi, p, q, sz);
-    u1 -= (dc * g[3 * n + c] + de * g[4 * n + c] + dz * g[5 * n + c]) * fac;
-  }
-  if (b & CLS_PROJ_K) {  // poisson.c:2316-2363; face k+ is face 4, cross i then j
-    const size_t q = p + sz;
-    const double dc = proj_cross((cd >> 22) & 3u, phi, p, q, sx);
-    const double de = proj_cross((cd >> 24) & 3u, phi, p, q, sy);
-    const double dz = phi[q] - phi[p];
-    u2 -= (dc * g[6 * n + c] + de * g[7 * n + c] + dz * g[8 * n + c]) * fac;
-  }
-  u[3 * p + 0] = u0;
-  u[3 * p + 1] = u1;
-  u[3 * p + 2] = u2;
+  projection_cell(m, d, g, code, cls, phi, u, fac, i, j, k);
 }
 
-// per-component periodic ghost faces (poisson.c:2401-2441)
 __global__ __launch_bounds__(256) void k_ucont_ghosts(Dmda m, double* __restrict__ u) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= m.mx || j >= m.my) return;
-  int a, b, c, fl[3];
-  if (!image_of(m, i, j, k, a, b, c, fl)) return;
-  const size_t p = m.idx(i, j, k), s = m.idx(a, b, c);
-  if (fl[0]) u[3 * p + 0] = u[3 * s + 0];
-  if (fl[1]) u[3 * p + 1] = u[3 * s + 1];
-  if (fl[2]) u[3 * p + 2] = u[3 * s + 2];
+  ucont_ghost_point(m, u, i, j, k);
 }
 
 // ----------------------------------------------------------------- Contra2Cart
 __global__ __launch_bounds__(256) void k_c2c_core(Dmda m, Dims d, const double* __restrict__ met,
                                                   const uint8_t* __restrict__ cls, const double* __restrict__ u,
                                                   double* __restrict__ ucat) {
-#pragma clang fp contract(off)
   CELL_IJK(i, j, k);
   if (i >= d.nx || j >= d.ny) return;
-  const size_t n = d.n(), c = d.idx(i, j, k);
-  if (!(cls[c] & CLS_LT)) return;
-  const size_t p = m.idx(i + 1, j + 1, k + 1);
-  const double m00 = met[0 * n + c], m01 = met[1 * n + c], m02 = met[2 * n + c];
-  const double m10 = met[3 * n + c], m11 = met[4 * n + c], m12 = met[5 * n + c];
-  const double m20 = met[6 * n + c], m21 = met[7 * n + c], m22 = met[8 * n + c];
-  // rhs.c:166-189
-  const double q0 = 0.5 * (u[3 * (p - 1) + 0] + u[3 * p + 0]);
-  const double q1 = 0.5 * (u[3 * (p - m.mx) + 1] + u[3 * p + 1]);
-  const double q2 = 0.5 * (u[3 * (p - (size_t)m.mx * m.my) + 2] + u[3 * p + 2]);
-  const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) + m02 * (m10 * m21 - m11 * m20);
-  const double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) + q2 * (m01 * m12 - m02 * m11);
-  const double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) - q2 * (m00 * m12 - m02 * m10);
-  const double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) + q2 * (m00 * m11 - m01 * m10);
-  ucat[3 * p + 0] = det0 / det;
-  ucat[3 * p + 1] = det1 / det;
-  ucat[3 * p + 2] = det2 / det;
+  c2c_core_cell(m, d, met, cls, u, ucat, i, j, k);
 }
 
-// whole-vector periodic copy of ghost points (rhs.c:198-241, 645-673); when snap
-// is given the boundary points are also copied there (the snapshot the boundary
-// rules read)
 __global__ __launch_bounds__(256) void k_vec_ghosts(Dmda m, double* __restrict__ a, double* __restrict__ snap) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= m.mx || j >= m.my) return;
-  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
-  if (!bnd) return;
-  const size_t p = m.idx(i, j, k);
-  int aa, bb, cc, fl[3];
-  double v0 = a[3 * p], v1 = a[3 * p + 1], v2 = a[3 * p + 2];
-  if (image_of(m, i, j, k, aa, bb, cc, fl)) {
-    const size_t s = m.idx(aa, bb, cc);
-    v0 = a[3 * s]; v1 = a[3 * s + 1]; v2 = a[3 * s + 2];
-    a[3 * p] = v0; a[3 * p + 1] = v1; a[3 * p + 2] = v2;
-  }
-  if (snap) { snap[3 * p] = v0; snap[3 * p + 1] = v1; snap[3 * p + 2] = v2; }
+  if (on_boundary(m, i, j, k)) vec_ghost_point(m, a, snap, i, j, k);
 }
 
-// boundary rules of Contra2Cart on boundary points (rhs.c:257-260, 491-515);
-// sources on boundary points come from the snapshot, interior sources are
-// untouched until k_c2c_solid runs
 __global__ __launch_bounds__(256) void k_c2c_bc(Dmda m, const double* __restrict__ nv, const double* __restrict__ snap,
                                                 double* __restrict__ ucat, int b0, int b1, int b2, int b3, int b4,
                                                 int b5) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= m.mx || j >= m.my) return;
-  const bool bnd = i == 0 || j == 0 || k == 0 || i == m.mx - 1 || j == m.my - 1 || k == m.mz - 1;
-  if (!bnd) return;
-  const size_t p = m.idx(i, j, k);
-  if ((int)(nv[p] + 0.1) == 3) { ucat[3 * p] = ucat[3 * p + 1] = ucat[3 * p + 2] = 0.; return; }
-  auto src = [&](int a, int b, int c) -> const double* {
-    const bool sb = a == 0 || b == 0 || c == 0 || a == m.mx - 1 || b == m.my - 1 || c == m.mz - 1;
-    return (sb ? snap : ucat) + 3 * m.idx(a, b, c);
-  };
-  const double* s = nullptr;
-  if (i == 0 && b0 == 1 && (j != 0 && k != 0)) s = src(i + 1, j, k);
-  if (i == m.mx - 1 && b1 == 1 && (j != 0 && k != 0)) s = src(i - 1, j, k);
-  if (j == 0 && b2 == 1 && (i != 0 && k != 0)) s = src(i, j + 1, k);
-  if (j == m.my - 1 && b3 == 1 && (i != 0 && k != 0)) s = src(i, j - 1, k);
-  if (k == 0 && b4 == 1 && (i != 0 && j != 0)) s = src(i, j, k + 1);
-  if (k == m.mz - 1 && b5 == 1 && (i != 0 && j != 0)) s = src(i, j, k - 1);
-  if (s) {
-    const double v0 = -s[0], v1 = -s[1], v2 = -s[2];
-    ucat[3 * p] = v0; ucat[3 * p + 1] = v1; ucat[3 * p + 2] = v2;
-  }
+  if (!on_boundary(m, i, j, k)) return;
+  const int bc[6] = {b0, b1, b2, b3, b4, b5};
+  c2c_bc_point(m, nv, snap, ucat, bc, i, j, k);
 }
 
 __global__ __launch_bounds__(256) void k_c2c_solid(Dims d, Dmda m, const uint8_t* __restrict__ cls,
                                                    double* __restrict__ ucat) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= d.nx || j >= d.ny) return;
-  if (!(cls[d.idx(i, j, k)] & CLS_SOLID)) return;
-  const size_t p = m.idx(i + 1, j + 1, k + 1);
-  ucat[3 * p] = ucat[3 * p + 1] = ucat[3 * p + 2] = 0.;
+  c2c_solid_cell(m, d, cls, ucat, i, j, k);
 }
 
 // ------------------------------------------------------------------- Divergence
 __global__ __launch_bounds__(256) void k_divergence(Dmda m, Dims d, const uint32_t* __restrict__ code,
                                                     const double* __restrict__ aj, const double* __restrict__ u,
                                                     double* __restrict__ partial) {
-#pragma clang fp contract(off)
   __shared__ double lds[4];
   CELL_IJK(i, j, k);
   double mv = 0.;
-  if (i < d.nx && j < d.ny) {
-    const size_t c = d.idx(i, j, k);
-    if (code[c] & DIVOK_BIT) {
-      const size_t p = m.idx(i + 1, j + 1, k + 1);
-      // main.c:1283 operand order
-      mv = fabs((u[3 * p] - u[3 * (p - 1)] + u[3 * p + 1] - u[3 * (p - m.mx) + 1] + u[3 * p + 2] -
-                 u[3 * (p - (size_t)m.mx * m.my) + 2]) *
-                aj[c]);
-    }
-  }
+  if (i < d.nx && j < d.ny) mv = divergence_cell(m, d, code, aj, u, i, j, k);
   mv = block_max(mv, lds);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blin()] = mv;
 }
 
-// max over Divergence-counted cells of |aj r|: with r = b - A phi this is the
-// max |aj div u| the projection of phi leaves (div u_new = dt/coeff (b - A phi)
-// for st = 1), known before the projection runs
+// max over Divergence-counted cells of |aj r|
 __global__ __launch_bounds__(256) void k_divest(Dims d, const uint32_t* __restrict__ code,
                                                 const double* __restrict__ aj, const double* __restrict__ r,
                                                 double* __restrict__ partial) {
   __shared__ double lds[4];
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   double mv = 0.;
-  if (t < d.n() && (code[t] & DIVOK_BIT)) mv = fabs(aj[t] * r[t]);
+  if (t < d.n()) mv = divest_elem(code, aj, r, t);
   mv = block_max(mv, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = mv;
 }
@@ -628,9 +437,7 @@ __global__ __launch_bounds__(256) void k_update_x(size_t n, double* __restrict__
                                                   size_t ldz, int nz, const double* __restrict__ y) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  double acc = x[t];
-  for (int v = 0; v < nz; v++) acc += y[v] * (double)Z[(size_t)v * ldz + t];
-  x[t] = acc;
+  update_x_elem(x, Z, ldz, nz, y, t);
 }
 
 template <typename TA, typename TB>
@@ -765,10 +572,7 @@ __global__ __launch_bounds__(256) void k_scale_conv(size_t n, const double* __re
                                                     TF* __restrict__ f) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  const double s = nrm2[0] > 0. ? 1. / sqrt(nrm2[0]) : 0.;
-  const double x = a[t] * s;
-  v[t] = x;
-  if (f) f[t] = (TF)x;
+  scale_conv_elem(a, nrm2[0], v, f, t);
 }
 
 inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }

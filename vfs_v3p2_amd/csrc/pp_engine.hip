@@ -651,13 +651,17 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
 // ------------------------------------------------------------ pressure path
 // VolumeFlux and RHS, launches only; ||b||^2 lands in hpin[RED_B2] once the
 // stream is synchronised.
-void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff) {
+void solve_prologue_host(vfs_pp_handle* h, double dt, double st, double coeff) {
   ensure_geom(h);
-  hipStream_t s = h->s;
   // div u after projection = dt/coeff (b - A phi) holds for st = 1 (the RHS scales
   // by st, the projection by st as well); with another st the rule is off
   h->div_fac = (h->opts.div_tol > 0. && st == 1.) ? dt / coeff : 0.;
   h->div_goal = 0.5 * h->opts.div_tol;
+}
+
+void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff) {
+  solve_prologue_host(h, dt, st, coeff);
+  hipStream_t s = h->s;
   int nb = 0;
   if (h->nibf) {
     launch_ibflux_sum(s, h->ibf, h->nibf, ucont, h->grid.immersed, 1, h->partial, &nb);
@@ -672,12 +676,17 @@ void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, doubl
   launch_finalize(s, h->partial, 1, nb, h->red + RED_B2, 0, h->hpin_dev + RED_B2);
 }
 
-// After the prologue's stream sync: ||b||, VolumeFlux figures, initial guess.
-void solve_start(vfs_pp_handle* h, double* phi, vfs_pp_stats* S) {
+// After the prologue's stream sync: ||b|| and the VolumeFlux figures.
+void solve_start_stats(vfs_pp_handle* h, vfs_pp_stats* S) {
   if (h->nibf)
     for (int q = 0; q < 4; q++) S->ibm_flux[q] = h->hpin[RED_IBF + q];
   S->bnorm = std::sqrt(h->hpin[RED_B2]);
   if (!std::isfinite(S->bnorm)) throw StatusError(VFS_PP_ENONFINITE, "non-finite right-hand side");
+}
+
+// After the prologue's stream sync: ||b||, VolumeFlux figures, initial guess.
+void solve_start(vfs_pp_handle* h, double* phi, vfs_pp_stats* S) {
+  solve_start_stats(h, S);
   if (h->opts.zero_guess) launch_fill(h->s, h->d.n(), h->x, 0.);
   else launch_gather_phi(h->s, h->m, h->d, phi, h->cls, h->x);
 }

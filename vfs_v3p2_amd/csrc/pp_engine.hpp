@@ -245,7 +245,9 @@ constexpr int RED_NUINV2 = 400;  // 1 / |U_l|^2 of the lazily normalised basis (
 double now_ms();
 void ensure_geom(vfs_pp_handle* h);
 void need_partials(vfs_pp_handle* h, size_t nvec, size_t nblk);
+void solve_prologue_host(vfs_pp_handle* h, double dt, double st, double coeff);
 void solve_prologue(vfs_pp_handle* h, double* ucont, double dt, double st, double coeff);
+void solve_start_stats(vfs_pp_handle* h, vfs_pp_stats* S);
 void solve_start(vfs_pp_handle* h, double* phi, vfs_pp_stats* S);
 void solve_epilogue(vfs_pp_handle* h, double* phi, vfs_pp_stats* S);
 void update_pressure(vfs_pp_handle* h, double* p, double* phi);
