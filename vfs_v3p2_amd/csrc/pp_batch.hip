@@ -47,6 +47,10 @@ __device__ __forceinline__ int find_item(const uint32_t* __restrict__ start, int
   return lo;
 }
 
+// Work lists: `start` holds the first tile of each of the nitems items, followed
+// by one word per tile with the tile's item (high 16 bits) and that item's case
+// slot (low 16 bits), so a tile finds its item and slot with one load.
+//
 // Persistent workers (gridDim.x a multiple of 8): worker w runs on XCD w % 8 and
 // walks that XCD's contiguous eighth of the tile range [0, total).
 #define FOR_TILES(total, t)                                                               \
@@ -168,9 +172,10 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
                                                  const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                  const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const SmoothItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     int i, j, k;
     if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
@@ -185,9 +190,10 @@ __global__ __launch_bounds__(256) void kb_resid(const SmoothItem<T>* __restrict_
                                                 const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                 const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const SmoothItem<T>& I = items[q];
-    if (!sd[I.slot].y) continue;
+    if (!sd[tw_ & 0xffffu].y) continue;
     int i, j, k;
     if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
     resid_cell(I.L, I.f, I.xin, I.scr, i, j, k);
@@ -200,9 +206,10 @@ __global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__
                                                const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const SmoothItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     int i, k;
     if (!flat_line(I.L.d, t - start[q], i, k)) continue;
@@ -218,9 +225,10 @@ __global__ __launch_bounds__(256) void kb_restrict_blk(const RestrictItem<T>* __
                                                        const SlotDev* __restrict__ sd) {
   __shared__ T lds[2][2][32];
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const RestrictItem<T>& I = items[q];
-    if (!sd[I.slot].y) continue;
+    if (!sd[tw_ & 0xffffu].y) continue;
     const uint32_t lt = t - start[q];
     resid_restrict_blk(I.F, I.f, I.x, I.dc, I.clsc, I.fc, (int)(lt % I.tx), (int)((lt / I.tx) % I.ty),
                        (int)(lt / (I.tx * I.ty)), lds);
@@ -232,9 +240,10 @@ __global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __
                                                        const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                        const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const RestrictItem<T>& I = items[q];
-    if (!sd[I.slot].y) continue;
+    if (!sd[tw_ & 0xffffu].y) continue;
     int ic, jc, kc;
     if (!flat_cell(I.dc, t - start[q], ic, jc, kc)) continue;
     resid_restrict_cell(I.F, I.f, I.x, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
@@ -246,9 +255,10 @@ __global__ __launch_bounds__(256) void kb_prolong(const ProlongItem<T>* __restri
                                                   const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                   const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const ProlongItem<T>& I = items[q];
-    if (!sd[I.slot].y) continue;
+    if (!sd[tw_ & 0xffffu].y) continue;
     int i, j, k;
     if (!flat_cell(I.F.d, t - start[q], i, j, k)) continue;
     prolong_add_cell(I.F, I.dc, I.clsc, I.cmask, I.xc, I.xf, i, j, k);
@@ -262,9 +272,10 @@ __global__ __launch_bounds__(256) void kb_dense(const DenseItem<T>* __restrict__
   __shared__ T fs[1024];
   __shared__ T part[4][64];
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const DenseItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     T* x = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.x;
     dense_rows(I.n, I.cinv, I.f, x, (int)(t - start[q]), fs, part);
@@ -279,9 +290,10 @@ __global__ __launch_bounds__(256) void kb_apply_dots(const ApplyItem<T>* __restr
                                                      const SlotDev* __restrict__ sd, double* __restrict__ partial) {
   __shared__ double lds[4][64];
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const ApplyItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     int i = 0, j = 0, k = 0;
     const bool in = flat_cell(I.d, t - start[q], i, j, k);
@@ -300,9 +312,10 @@ __global__ __launch_bounds__(256) void kb_axpy_dots(const VecItem<T>* __restrict
                                                     const SlotDev* __restrict__ sd, double* __restrict__ partial) {
   __shared__ double lds[4][64];
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const VecItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
     const bool in = e < I.n;
@@ -322,9 +335,10 @@ __global__ __launch_bounds__(256) void kb_axpy_scale(const VecItem<T>* __restric
                                                      const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                      const SlotDev* __restrict__ sd) {
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const VecItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
     if (e >= I.n) continue;
@@ -351,9 +365,10 @@ __global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict
                                                     const SlotDev* __restrict__ sd, double* __restrict__ partial) {
   __shared__ double lds[4][64];
   FOR_TILES(total, t) {
-    const int q = find_item(start, nitems, t);
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
     const VecItem<T>& I = items[q];
-    const SlotDev st = sd[I.slot];
+    const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     const int nv = st.x + 1;
     const size_t e = (size_t)(t - start[q]) * 256 + threadIdx.x;
@@ -719,13 +734,30 @@ struct BatchCore {
   }
 
   // ---------------------------------------------------------------- the plan
+  // items and their tile starts + per-tile (item, slot) table on the device
   template <typename Item>
-  std::pair<const Item*, const uint32_t*> upload(const std::vector<Item>& items, const std::vector<uint32_t>& start) {
+  std::pair<const Item*, const uint32_t*> upload(const std::vector<Item>& items, const std::vector<uint32_t>& start,
+                                                 uint32_t total) {
+    if (items.size() > 0xffffu || slots.size() > 0xffffu) throw StatusError(VFS_PP_EINVAL, "work list too long");
+    std::vector<uint32_t> tab(start);
+    tab.reserve(start.size() + total);
+    for (size_t q = 0; q < items.size(); q++) {
+      const uint32_t end = q + 1 < items.size() ? start[q + 1] : total;
+      for (uint32_t t = start[q]; t < end; t++) tab.push_back(((uint32_t)q << 16) | (uint32_t)items[q].slot);
+    }
+    if (tab.size() != start.size() + total) throw StatusError(VFS_PP_EINVAL, "inconsistent work list");
+    return {upload_items(items), upload_table(tab)};
+  }
+  template <typename Item>
+  const Item* upload_items(const std::vector<Item>& items) {
     Item* di = plan.alloc<Item>(items.size());
-    uint32_t* ds = plan.alloc<uint32_t>(start.size());
     HIPCHECK(hipMemcpyAsync(di, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(ds, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    return {di, ds};
+    return di;
+  }
+  const uint32_t* upload_table(const std::vector<uint32_t>& tab) {
+    uint32_t* ds = plan.alloc<uint32_t>(tab.size());
+    HIPCHECK(hipMemcpyAsync(ds, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    return ds;
   }
   template <typename Item>
   void add_launch(int kind, int mode, const std::vector<Item>& items, const std::vector<uint32_t>& ntiles) {
@@ -734,7 +766,7 @@ struct BatchCore {
     uint32_t tot = 0;
     for (size_t q = 0; q < items.size(); q++) { start[q] = tot; tot += ntiles[q]; }
     if (tot == 0) return;
-    auto pr = upload(items, start);
+    auto pr = upload(items, start, tot);
     seq.push_back(Launch{kind, mode, pr.first, pr.second, (int)items.size(), tot, 2048});
   }
 
@@ -964,8 +996,8 @@ struct BatchCore {
       for (size_t q = 0; q < ai.size(); q++) { sa[q] = acc; acc += an[q]; }
       acc = 0;
       for (size_t q = 0; q < vi.size(); q++) { sv[q] = acc; acc += vn[q]; }
-      auto pa = upload(ai, sa);
-      auto pv = upload(vi, sv);
+      auto pa = upload(ai, sa, ta);
+      auto pv = upload(vi, sv, tv);
       apply = Launch{K_APPLY, 0, pa.first, pa.second, (int)ai.size(), ta, 2048};
       axd = Launch{K_AXD, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
       axs = Launch{K_AXS, 0, pv.first, pv.second, (int)vi.size(), tv, 2048};
