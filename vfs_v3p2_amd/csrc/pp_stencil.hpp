@@ -183,7 +183,11 @@ HDI void build_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, 
 // it differs from the assembled row only by rounding.  xv holds x at the 19
 // stencil points in slot order, loaded up front so all loads are in flight at
 // once.
-template <int D, int SGN, int E, typename T>
+//
+// KC < 0 decodes each cross-difference case from the code word; KC == X_CENTRAL
+// is the row of an interior cell (every face active, every cross difference
+// central), with the weights known at compile time.
+template <int D, int SGN, int E, int KC, typename T>
 HDI void flux_term(T& nrm, T& crs, T& diag, T gv, uint32_t code, const T (&xv)[19]) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
   constexpr int qd[3] = {D == 0 ? SGN : 0, D == 1 ? SGN : 0, D == 2 ? SGN : 0};
@@ -199,17 +203,23 @@ HDI void flux_term(T& nrm, T& crs, T& diag, T gv, uint32_t code, const T (&xv)[1
     constexpr int sQ_hi = slot_of(qd[0] + ee[0], qd[1] + ee[1], qd[2] + ee[2]);
     constexpr int sC_lo = slot_of(-ee[0], -ee[1], -ee[2]);
     constexpr int sQ_lo = slot_of(qd[0] - ee[0], qd[1] - ee[1], qd[2] - ee[2]);
-    const int cc = (int)((code >> (6 + 4 * F + 2 * XI)) & 3u);
-    const T whi = cc == X_CENTRAL ? T(0.25) : (cc == X_UPPER ? T(0.5) : T(0));
-    const T wlo = cc == X_CENTRAL ? T(0.25) : (cc == X_LOWER ? T(0.5) : T(0));
-    const T wmd = cc == X_LOWER ? T(0.5) : (cc == X_UPPER ? T(-0.5) : T(0));
-    // the cross part enters the row with the face's side sign (see face_term)
-    crs += gv * (whi * (xv[sC_hi] + xv[sQ_hi]) - wlo * (xv[sC_lo] + xv[sQ_lo]) + wmd * (xv[CP] + xv[sQ]));
-    diag += (SGN > 0 ? wmd : -wmd) * gv;
+    if constexpr (KC == X_CENTRAL) {
+      // the central case's addends: +0.25 g on the upper pair, -0.25 g on the
+      // lower pair, nothing on (c, q) or the diagonal
+      crs += gv * (T(0.25) * (xv[sC_hi] + xv[sQ_hi]) - T(0.25) * (xv[sC_lo] + xv[sQ_lo]));
+    } else {
+      const int cc = (int)((code >> (6 + 4 * F + 2 * XI)) & 3u);
+      const T whi = cc == X_CENTRAL ? T(0.25) : (cc == X_UPPER ? T(0.5) : T(0));
+      const T wlo = cc == X_CENTRAL ? T(0.25) : (cc == X_LOWER ? T(0.5) : T(0));
+      const T wmd = cc == X_LOWER ? T(0.5) : (cc == X_UPPER ? T(-0.5) : T(0));
+      // the cross part enters the row with the face's side sign (see face_term)
+      crs += gv * (whi * (xv[sC_hi] + xv[sQ_hi]) - wlo * (xv[sC_lo] + xv[sQ_lo]) + wmd * (xv[CP] + xv[sQ]));
+      diag += (SGN > 0 ? wmd : -wmd) * gv;
+    }
   }
 }
 
-template <int D, int SGN, typename T, typename TG>
+template <int D, int SGN, int KC, typename T, typename TG>
 HDI void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, size_t owner, uint32_t code,
                    const T (&xv)[19]) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
@@ -219,26 +229,41 @@ HDI void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, size_t ow
   const T g1 = (T)g[(size_t)(3 * D + 1) * n + owner];
   const T g2 = (T)g[(size_t)(3 * D + 2) * n + owner];
   T nrm = T(0), crs = T(0), dg = T(0);
-  flux_term<D, SGN, 0>(nrm, crs, dg, g0, code, xv);
-  flux_term<D, SGN, 1>(nrm, crs, dg, g1, code, xv);
-  flux_term<D, SGN, 2>(nrm, crs, dg, g2, code, xv);
-  if ((code >> F) & 1u) {
+  flux_term<D, SGN, 0, KC>(nrm, crs, dg, g0, code, xv);
+  flux_term<D, SGN, 1, KC>(nrm, crs, dg, g1, code, xv);
+  flux_term<D, SGN, 2, KC>(nrm, crs, dg, g2, code, xv);
+  if (KC == X_CENTRAL || ((code >> F) & 1u)) {
     ax += nrm + (SGN > 0 ? crs : -crs);
     diag += dg;
   }
 }
 
-template <typename T, typename TG>
-HDI void flux_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
-                  const T (&xv)[19], T& ax, T& diag) {
+template <int KC, typename T, typename TG>
+HDI void flux_row_k(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
+                    const T (&xv)[19], T& ax, T& diag) {
   ax = T(0);
   diag = T(0);
-  flux_face<0, +1>(ax, diag, g, n, c, code, xv);
-  flux_face<0, -1>(ax, diag, g, n, cw, code, xv);
-  flux_face<1, +1>(ax, diag, g, n, c, code, xv);
-  flux_face<1, -1>(ax, diag, g, n, cs, code, xv);
-  flux_face<2, +1>(ax, diag, g, n, c, code, xv);
-  flux_face<2, -1>(ax, diag, g, n, cb, code, xv);
+  flux_face<0, +1, KC>(ax, diag, g, n, c, code, xv);
+  flux_face<0, -1, KC>(ax, diag, g, n, cw, code, xv);
+  flux_face<1, +1, KC>(ax, diag, g, n, c, code, xv);
+  flux_face<1, -1, KC>(ax, diag, g, n, cs, code, xv);
+  flux_face<2, +1, KC>(ax, diag, g, n, c, code, xv);
+  flux_face<2, -1, KC>(ax, diag, g, n, cb, code, xv);
+}
+
+// code word of an interior cell: every face active, every cross difference
+// central, a row (the Divergence bit aside)
+constexpr uint32_t CODE_INTERIOR = 0x3Fu | (0x555555u << 6) | ROW_BIT;
+
+// (A x)_c and a_cc.  A wave whose cells are all interior cells takes the
+// compile-time-weight form; any other wave decodes every case.
+template <typename T, typename TG>
+__device__ __forceinline__ void flux_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw,
+                                         size_t cs, size_t cb, const T (&xv)[19], T& ax, T& diag) {
+  if (__all((code & ~DIVOK_BIT) == CODE_INTERIOR))
+    flux_row_k<X_CENTRAL>(g, n, code, c, cw, cs, cb, xv, ax, diag);
+  else
+    flux_row_k<-1>(g, n, code, c, cw, cs, cb, xv, ax, diag);
 }
 
 // Interior indices of the 19 stencil neighbours with periodic wrap.
