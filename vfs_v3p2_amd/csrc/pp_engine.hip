@@ -807,6 +807,8 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
   ABI_BEGIN
   if (!g || !out) return VFS_PP_EINVAL;
   if (g->nx < 2 || g->ny < 2 || g->nz < 2) return VFS_PP_EINVAL;
+  // the stencil kernels address interior arrays with 32-bit byte offsets
+  if ((long long)(g->nx + 2) * (g->ny + 2) * (g->nz + 2) >= (1LL << 29)) return VFS_PP_EINVAL;
   for (int a = 0; a < 3; a++)
     if (g->periodic[a] && (a == 0 ? g->nx : a == 1 ? g->ny : g->nz) < 3) return VFS_PP_EINVAL;
   auto h = std::make_unique<vfs_pp_handle>();

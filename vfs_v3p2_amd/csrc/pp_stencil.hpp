@@ -219,15 +219,28 @@ HDI void flux_term(T& nrm, T& crs, T& diag, T gv, uint32_t code, const T (&xv)[1
   }
 }
 
+// a[idx] as a global-memory load through a 32-bit byte offset from a base every
+// lane shares: the load takes its base from scalar registers and one offset
+// register serves every array read at the same index (interior arrays hold
+// fewer than 2^29 entries).  Pointers read from work-list structs would
+// otherwise be loaded through the flat address space.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ a, uint32_t idx) {
+  typedef const __attribute__((address_space(1))) char gchar;
+  typedef const __attribute__((address_space(1))) T gT;
+  return *(gT*)((gchar*)a + idx * (uint32_t)sizeof(T));
+}
+
 template <int D, int SGN, int KC, typename T, typename TG>
-HDI void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, size_t owner, uint32_t code,
+__device__ __forceinline__ void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, uint32_t owner, uint32_t code,
                    const T (&xv)[19]) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
   // coefficients are loaded whether or not the face is active (all loads issue
   // together; the owner index is always a valid cell)
-  const T g0 = (T)g[(size_t)(3 * D + 0) * n + owner];
-  const T g1 = (T)g[(size_t)(3 * D + 1) * n + owner];
-  const T g2 = (T)g[(size_t)(3 * D + 2) * n + owner];
+  const TG* gd = g + (size_t)(3 * D) * n;
+  const T g0 = (T)ld_at(gd, owner);
+  const T g1 = (T)ld_at(gd + n, owner);
+  const T g2 = (T)ld_at(gd + 2 * n, owner);
   T nrm = T(0), crs = T(0), dg = T(0);
   flux_term<D, SGN, 0, KC>(nrm, crs, dg, g0, code, xv);
   flux_term<D, SGN, 1, KC>(nrm, crs, dg, g1, code, xv);
@@ -239,8 +252,8 @@ HDI void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, size_t ow
 }
 
 template <int KC, typename T, typename TG>
-HDI void flux_row_k(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw, size_t cs, size_t cb,
-                    const T (&xv)[19], T& ax, T& diag) {
+__device__ __forceinline__ void flux_row_k(const TG* __restrict__ g, size_t n, uint32_t code, uint32_t c, uint32_t cw, uint32_t cs,
+                    uint32_t cb, const T (&xv)[19], T& ax, T& diag) {
   ax = T(0);
   diag = T(0);
   flux_face<0, +1, KC>(ax, diag, g, n, c, code, xv);
@@ -261,9 +274,9 @@ template <typename T, typename TG>
 __device__ __forceinline__ void flux_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw,
                                          size_t cs, size_t cb, const T (&xv)[19], T& ax, T& diag) {
   if (__all((code & ~DIVOK_BIT) == CODE_INTERIOR))
-    flux_row_k<X_CENTRAL>(g, n, code, c, cw, cs, cb, xv, ax, diag);
+    flux_row_k<X_CENTRAL>(g, n, code, (uint32_t)c, (uint32_t)cw, (uint32_t)cs, (uint32_t)cb, xv, ax, diag);
   else
-    flux_row_k<-1>(g, n, code, c, cw, cs, cb, xv, ax, diag);
+    flux_row_k<-1>(g, n, code, (uint32_t)c, (uint32_t)cw, (uint32_t)cs, (uint32_t)cb, xv, ax, diag);
 }
 
 // Interior indices of the 19 stencil neighbours with periodic wrap.
@@ -299,9 +312,9 @@ HDI void neighbours(const Dims& d, int i, int j, int k, Nbr& nb) {
 }
 
 template <typename T, typename TX>
-HDI void load_x(const Nbr& nb, const TX* __restrict__ x, T (&xv)[19]) {
+__device__ __forceinline__ void load_x(const Nbr& nb, const TX* __restrict__ x, T (&xv)[19]) {
 #pragma unroll
-  for (int s = 0; s < 19; s++) xv[s] = (T)x[nb.s[s]];
+  for (int s = 0; s < 19; s++) xv[s] = (T)ld_at(x, nb.s[s]);
 }
 
 }  // namespace pp
