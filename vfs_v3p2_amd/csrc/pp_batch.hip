@@ -47,6 +47,15 @@ __device__ __forceinline__ int find_item(const uint32_t* __restrict__ start, int
   return lo;
 }
 
+// Fills the per-tile table of a work list laid out as [starts (nitems) | table
+// (total) | slots (nitems)]: table[t] = item of tile t << 16 | that item's slot.
+__global__ __launch_bounds__(256) void k_tile_table(uint32_t* __restrict__ ws, int nitems, uint32_t total) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= total) return;
+  const int q = find_item(ws, nitems, t);
+  ws[nitems + t] = ((uint32_t)q << 16) | ws[(size_t)nitems + total + q];
+}
+
 // Work lists: `start` holds the first tile of each of the nitems items, followed
 // by one word per tile with the tile's item (high 16 bits) and that item's case
 // slot (low 16 bits), so a tile finds its item and slot with one load.
@@ -734,30 +743,27 @@ struct BatchCore {
   }
 
   // ---------------------------------------------------------------- the plan
-  // items and their tile starts + per-tile (item, slot) table on the device
+  // Items, their tile starts and the per-tile (item, slot) table on the device.
+  // The table (one word per tile) is written by a kernel from the starts and the
+  // items' slots, so a membership change costs no host work per tile.
   template <typename Item>
   std::pair<const Item*, const uint32_t*> upload(const std::vector<Item>& items, const std::vector<uint32_t>& start,
                                                  uint32_t total) {
-    if (items.size() > 0xffffu || slots.size() > 0xffffu) throw StatusError(VFS_PP_EINVAL, "work list too long");
-    std::vector<uint32_t> tab(start);
-    tab.reserve(start.size() + total);
-    for (size_t q = 0; q < items.size(); q++) {
-      const uint32_t end = q + 1 < items.size() ? start[q + 1] : total;
-      for (uint32_t t = start[q]; t < end; t++) tab.push_back(((uint32_t)q << 16) | (uint32_t)items[q].slot);
+    const size_t ni = items.size();
+    if (ni == 0 || ni > 0xffffu || slots.size() > 0xffffu) throw StatusError(VFS_PP_EINVAL, "work list too long");
+    Item* di = plan.alloc<Item>(ni);
+    HIPCHECK(hipMemcpyAsync(di, items.data(), ni * sizeof(Item), hipMemcpyHostToDevice, s));
+    // [starts (ni) | table (total) | slots (ni)]
+    std::vector<uint32_t> sl(ni);
+    for (size_t q = 0; q < ni; q++) sl[q] = (uint32_t)items[q].slot;
+    uint32_t* ds = plan.alloc<uint32_t>(2 * ni + total);
+    HIPCHECK(hipMemcpyAsync(ds, start.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(ds + ni + total, sl.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (total) {
+      k_tile_table<<<(total + 255) / 256, 256, 0, s>>>(ds, (int)ni, total);
+      HIPCHECK(hipGetLastError());
     }
-    if (tab.size() != start.size() + total) throw StatusError(VFS_PP_EINVAL, "inconsistent work list");
-    return {upload_items(items), upload_table(tab)};
-  }
-  template <typename Item>
-  const Item* upload_items(const std::vector<Item>& items) {
-    Item* di = plan.alloc<Item>(items.size());
-    HIPCHECK(hipMemcpyAsync(di, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, s));
-    return di;
-  }
-  const uint32_t* upload_table(const std::vector<uint32_t>& tab) {
-    uint32_t* ds = plan.alloc<uint32_t>(tab.size());
-    HIPCHECK(hipMemcpyAsync(ds, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    return ds;
+    return {di, ds};
   }
   template <typename Item>
   void add_launch(int kind, int mode, const std::vector<Item>& items, const std::vector<uint32_t>& ntiles) {

@@ -11,8 +11,8 @@ divergence kernel), and the per-solve breakdown by kernel for the first and the
 last group of solves (the benchmark runs the mixed form first, then FP64).
 
 Batch mode keeps the batched kernels (kb_*, in the multigrid precision given by
---precision), splits them into calls at gaps of more than 20 ms, and breaks
-down the last call by kernel.
+--precision), splits them into calls at each batched RHS step (the first launch
+of a call), and breaks down the last call by kernel.
 
 Traffic mode reads one FETCH_SIZE and one WRITE_SIZE pass (KiB per dispatch),
 scales each by what the same counter reads for the 1 GiB device copy kernel
@@ -63,7 +63,7 @@ def _counters(path):
 
 
 COPY = ("k_copy16", 1 << 26)  # 2^26 work-items x 16 B = 1 GiB read and 1 GiB written
-SMOOTHER = "k_smooth_point<float, false>"
+SMOOTHER = "k_smooth_point<float, 0>"
 
 
 def traffic(fetch_csv, write_csv, cells, top):
@@ -87,17 +87,20 @@ def traffic(fetch_csv, write_csv, cells, top):
         print(f"\ntraffic_bytes_per_launch: {t:.0f}")
 
 
+RHS_STEP = "kb_phase<2, "  # the batched RHS step: the first launch of every batched call
+
+
 def batch(rows, precision, top):
     kb = [r for r in rows if r["name"].startswith("kb_") and (precision in r["name"] or "<" not in r["name"])]
     calls, cur = [], []
     for r in kb:
-        if cur and r["s"] - cur[-1]["e"] > 20e6:
+        if cur and r["name"].startswith(RHS_STEP):
             calls.append(cur)
             cur = []
         cur.append(r)
     if cur:
         calls.append(cur)
-    print("## Batched calls (kb_* kernels, split at gaps > 20 ms)\n")
+    print("## Batched calls (kb_* kernels; a call starts at its batched RHS step)\n")
     print("| call | wall ms (profiled) | kernel ms | launches |")
     print("|---|---|---|---|")
     for q, c in enumerate(calls):
