@@ -103,7 +103,8 @@ void vfs_pp_default_opts(vfs_pp_opts *opts);
 
 /* Create a handle for one grid.  Replaces the ti==tistart set-up branch of
  * PoissonSolver_Hypre (setup_lidx2 + Create_Hypre_* + PoissonLHSNew,
- * poisson_hypre.c:609-627). */
+ * poisson_hypre.c:609-627).  VFS_PP_EINVAL if an axis has fewer than 2 cells (3
+ * if periodic) or the DMDA box (nx+2)(ny+2)(nz+2) has 2^29 points or more. */
 int vfs_pp_create(const vfs_grid *grid, const vfs_pp_opts *opts, vfs_pp_handle **out);
 int vfs_pp_destroy(vfs_pp_handle *h);
 int vfs_pp_set_opts(vfs_pp_handle *h, const vfs_pp_opts *opts);

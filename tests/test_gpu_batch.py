@@ -121,3 +121,61 @@ def test_batch_divergence_rule(E, torch_cuda):
         assert s.converged and s.max_div <= 1e-8
     for h in hs:
         h.close()
+
+
+def _topology_cases():
+    """The single-case parity cases (no periodic axis, immersed 0 and 1, towers)
+    plus an all-walls box whose Contra2Cart uses the no-slip rule on every face."""
+    from test_gpu_parity import CASES, affine_case, get_case
+
+    out = [get_case(n) for n in CASES]
+    c = affine_case(12, 10, 9, seed=3)
+    c.bctype = (1,) * 6
+    c.name = "affine-walls"
+    out.append((c, O.Assembled(c, drop=False), O.reference_projection(c, rtol=1e-13)))
+    return out
+
+
+def test_batched_steps_on_every_topology(E, torch_cuda):
+    """The batched per-case steps (VolumeFlux, RHS, initial guess from Phi, true
+    residual, null space, Phi, P, Projection, Contra2Cart, Divergence) against the
+    oracle: Phi and P to 1e-8, and Ucont, Ucat, the VolumeFlux figures and the
+    divergence equal to the oracle's processing of the batch's own Phi."""
+    torch = torch_cuda
+    cs = _topology_cases()
+    print("batch of " + ", ".join(c.name for c, _, _ in cs), flush=True)
+    hs = []
+    for c, _, _ in cs:
+        h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed),
+                     E.default_opts(rtol=1e-12, zero_guess=0))
+        h.set_geometry(c.fields)
+        hs.append(h)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    u = [dev(c.fields["ucont"]) for c, _, _ in cs]
+    phi = [dev(0.5 * ref["phi"]) for _, _, ref in cs]  # a warm start read by the batched gather step
+    p = [torch.zeros(c.shape, dtype=torch.float64, device="cuda") for c, _, _ in cs]
+    ucat = [torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda") for c, _, _ in cs]
+    B = E.Batch(hs)
+    sts = B.project(u, phi, p, ucat, [c.dt for c, _, _ in cs], [c.st for c, _, _ in cs],
+                    [c.coeff for c, _, _ in cs], strict=True)
+    B.close()
+    for h in hs:
+        h.close()
+    for q, (c, A, ref) in enumerate(cs):
+        st = sts[q]
+        fl = A.fluid_cells
+        ph = phi[q].cpu().numpy()
+        assert st.converged and st.max_div <= 1e-8, c.name
+        assert relL2(ph, ref["phi"], fl) <= 1e-8, c.name
+        assert relL2(p[q].cpu().numpy(), ref["p"], fl) <= 1e-8, c.name
+        uo = np.ascontiguousarray(c.fields["ucont"].copy())
+        vf = O.volume_flux(A.g, c.immersed, uo, c.fields)
+        np.testing.assert_allclose(st.ibm_flux, vf[:4], rtol=1e-12, atol=1e-15)
+        O.projection(A.g, c.fields, np.ascontiguousarray(ph), uo, c.dt, c.st, c.coeff)
+        uc = np.zeros_like(uo)
+        O.contra2cart(A.g, c.bctype, c.fields, uo, uc)
+        ug = u[q].cpu().numpy()
+        np.testing.assert_allclose(ug, uo, rtol=0, atol=1e-13 * np.abs(uo).max())
+        np.testing.assert_allclose(ucat[q].cpu().numpy(), uc, rtol=0, atol=1e-13 * np.abs(uc).max())
+        md = O.divergence(A.g, ug, np.ascontiguousarray(c.fields["aj"]), A.nvert)
+        assert abs(md - st.max_div) <= 1e-12 * max(md, 1e-300) + 1e-20, c.name
