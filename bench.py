@@ -501,7 +501,7 @@ def main():
         "mixed_over_fp64": round(value / fp64_value, 3),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_smooth_point<float> (finest-level Chebyshev-Jacobi sweep, 49 algorithmic B/cell)",
+            "kernel": "k_smooth_point<float> (finest-level Chebyshev-Jacobi sweep, FP16 coefficients, 31 algorithmic B/cell)",
             "achieved": sm["gbs"],
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

@@ -332,8 +332,8 @@ int vfs_pp_server_status(vfs_pp_server *sv, int *queued, int *active, size_t *re
  * the multigrid precision), 2 projection, 3 Contra2Cart, 4 RHS, 5 a plain
  * 1 GiB device copy (the bandwidth reference).  Writes the mean duration (ms)
  * and the algorithmic bytes of one launch: every distinct array once per cell,
- * with a 1-byte mask (apply 89, smoother 49 FP32 / 97 FP64, or for a y-line finest
- * level its residual + line-solve step 55 (FP16 coefficients) / 145, projection 129,
+ * with a 1-byte mask (apply 89, smoother 31 FP32 (FP16 coefficients) / 97 FP64, or for
+ * a y-line finest level its residual + line-solve step 55 / 145, projection 129,
  * Contra2Cart 121, RHS 33 bytes per cell; copy 16 bytes per element). */
 int vfs_pp_time_kernel(vfs_pp_handle *h, int kernel, int reps, double *ms_per_launch,
                        double *bytes_per_launch);

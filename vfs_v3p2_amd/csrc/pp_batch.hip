@@ -291,13 +291,12 @@ __global__ __launch_bounds__(256) void kb_dense(const DenseItem<T>* __restrict__
   }
 }
 
-// w = A z_j (FP64 flux form) into V_{j+1}; per-tile partial sums of w.V_l (l <= j)
-// and w.w at partial[l * total + t]
+// w = A z_j (FP64 flux form) into V_{j+1}; per-wave partial sums of w.V_l (l <= j)
+// and w.w at partial[l * 4 total + 4 t + wave]
 template <typename T>
 __global__ __launch_bounds__(256) void kb_apply_dots(const ApplyItem<T>* __restrict__ items,
                                                      const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                      const SlotDev* __restrict__ sd, double* __restrict__ partial) {
-  __shared__ double lds[4][64];
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
@@ -309,8 +308,10 @@ __global__ __launch_bounds__(256) void kb_apply_dots(const ApplyItem<T>* __restr
     const T* z = I.Z + (size_t)st.x * I.n;
     double* w = I.V + (size_t)(st.x + 1) * I.n;
     const size_t c = in ? I.d.idx(i, j, k) : 0;
+    double v8[8];
+    wave_dots_prefetch(in, c, I.V, I.n, st.x + 1, v8);
     const double wv = in ? apply_cell(I.d, I.g, I.code, z, w, i, j, k) : 0.;
-    block_dots(in, c, wv, I.V, I.n, st.x + 1, partial, total, t, lds);
+    wave_dots(in, c, wv, I.V, I.n, st.x + 1, v8, partial, 4 * (size_t)total, 4 * (size_t)t + threadIdx.y);
   }
 }
 
@@ -395,16 +396,18 @@ __global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict
 }
 
 // One block per (case, vector l < j + 1 + extra): fixed-order sum of the case's
-// tile partials into red[off + l], mirrored to pinned host memory.
+// partials (per tiles [t0, t1), `per` of them per tile) into red[off + l],
+// mirrored to pinned host memory.
 __global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ items, const double* __restrict__ partial,
-                                                   uint32_t total, int off, int extra,
+                                                   uint32_t total, int per, int off, int extra,
                                                    const SlotDev* __restrict__ sd) {
   __shared__ double lds[4];
   const FinItem& I = items[blockIdx.x];
   const SlotDev st = sd[I.slot];
   const int l = blockIdx.y;
   if (!st.y || l >= st.x + 1 + extra) return;
-  const double s = tile_sum(partial + (size_t)l * total, I.t0, I.t1, lds);
+  const uint32_t pt = (uint32_t)per * total;
+  const double s = tile_sum(partial + (size_t)l * pt, per * I.t0, per * I.t1, lds);
   if (threadIdx.x == 0) {
     I.red[off + l] = s;
     I.hpin[off + l] = s;
@@ -994,7 +997,7 @@ struct BatchCore {
         vn.push_back(nv);
         mmax = std::max(mmax, h->mrest);
       }
-      const size_t need = (size_t)(mmax + 2) * std::max(ta, tv);
+      const size_t need = (size_t)(mmax + 2) * std::max<size_t>(4 * (size_t)ta, tv);  // 4 partials per apply tile
       partial = plan.alloc<double>(need);
       partial_cap = need;
       std::vector<uint32_t> sa(ai.size()), sv(vi.size());
@@ -1086,7 +1089,7 @@ struct BatchCore {
       // one Gram-Schmidt pass on the unnormalised basis: w.U_l and |w|^2 from the
       // operator pass, then w' and |w'|^2 (the host re-orthogonalises the rare
       // case whose pass cancelled most of w)
-      kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 1, sd);
+      kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 1, sd);
       HIPCHECK(hipGetLastError());
       launch_one(axn);
       kb_finalize_norm<<<nfin, 256, 0, s>>>(fin_vec, partial, sd);
@@ -1094,10 +1097,10 @@ struct BatchCore {
       return;
     }
     // CGS2 in three passes over each case's basis
-    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 8, 0, sd);
+    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 0, sd);
     HIPCHECK(hipGetLastError());
     launch_one(axd);
-    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_vec, partial, fin_total_vec, 72, 1, sd);
+    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_vec, partial, fin_total_vec, 1, 72, 1, sd);
     HIPCHECK(hipGetLastError());
     kb_gs_norm<<<(nfin + 63) / 64, 64, 0, s>>>(fin_vec, nfin, sd);
     HIPCHECK(hipGetLastError());

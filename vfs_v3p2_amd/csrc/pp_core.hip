@@ -467,20 +467,22 @@ __global__ __launch_bounds__(256) void k_copy16(size_t n2, const double2* __rest
   if (t < n2) b[t] = a[t];
 }
 
-// w = A z (flux form) and per-block partial sums of w.V_l (l < nv) and w.w; the
-// first Krylov reduction of classical Gram-Schmidt fused into the operator pass.
+// w = A z (flux form) and per-wave partial sums of w.V_l (l < nv) and w.w (four
+// per 64x4 block); the first Krylov reduction of classical Gram-Schmidt fused
+// into the operator pass.
 template <typename TZ>
 __global__ __launch_bounds__(256) void k_apply_dots(Dims d, const double* __restrict__ g,
                                                     const uint32_t* __restrict__ code, const TZ* __restrict__ z,
                                                     double* __restrict__ w, const double* __restrict__ V, size_t ldv,
                                                     int nv, double* __restrict__ partial) {
-  __shared__ double lds[4][64];
   CELL_IJK(i, j, k);
-  const int nblk = gridDim.x * gridDim.y * gridDim.z, b = blin();
+  const size_t nblk = (size_t)gridDim.x * gridDim.y * gridDim.z, b = blin();
   const bool in = i < d.nx && j < d.ny;
   const size_t c = in ? d.idx(i, j, k) : 0;
+  double v8[8];
+  wave_dots_prefetch(in, c, V, ldv, nv, v8);
   const double wv = in ? apply_cell(d, g, code, z, w, i, j, k) : 0.;
-  block_dots(in, c, wv, V, ldv, nv, partial, nblk, b, lds);
+  wave_dots(in, c, wv, V, ldv, nv, v8, partial, 4 * nblk, 4 * b + threadIdx.y);
 }
 
 // CGS2 second pass, fused: w1 = w - sum_l h_l V_l (in place) and per-block
@@ -825,7 +827,7 @@ template <typename TZ>
 void launch_apply_dots(hipStream_t s, Dims d, const double* g, const uint32_t* code, const TZ* z, double* w,
                        const double* V, size_t ldv, int nv, double* partial, int* nblk) {
   const dim3 gr = cell_grid(d);
-  *nblk = (int)(gr.x * gr.y * gr.z);
+  *nblk = (int)(4 * gr.x * gr.y * gr.z);  // one partial per wave
   k_apply_dots<TZ><<<gr, kBlk, 0, s>>>(d, g, code, z, w, V, ldv, nv, partial);
   HIPCHECK(hipGetLastError());
 }

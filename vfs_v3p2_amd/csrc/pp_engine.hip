@@ -298,10 +298,10 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     }
     if (H.d.per[1]) line = 0;
     L.line = line;
-    // coefficients: FP16 on the y-line levels of the FP32 hierarchy, scaled by a
-    // power of two that puts the largest magnitude below 2^14 (FP16 keeps normal
-    // numbers down to 2^-14); the level precision everywhere else
-    const bool half = std::is_same<T, float>::value && line;
+    // coefficients: FP16 on every level of the FP32 hierarchy, scaled by a power
+    // of two that puts the largest magnitude below 2^14 (FP16 keeps normal numbers
+    // down to 2^-14); FP64 levels keep FP64
+    const bool half = std::is_same<T, float>::value;
     double gs = 1.;
     if (half) {
       double gmax = 0.;
@@ -436,7 +436,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
         // leaves w' orthogonal to the basis to about eps |w| / |w'|, so a second pass
         // runs only when the first cancelled |w'| below 1e-4 |w|.
         const int nv = j + 1;
-        need_partials(h, (size_t)j + 2, std::max(h->stencil_blocks, (n + 255) / 256));
+        need_partials(h, (size_t)j + 2, std::max(4 * h->stencil_blocks, (n + 255) / 256));
         launch_apply_dots<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->V, n, nv, h->partial, &nb);
         launch_finalize(h->s, h->partial, nv + 1, nb, h->red + 8, 0, h->hpin_dev + 8);
         launch_axpy_norm<TZ>(h->s, n, w, h->V, n, nv, h->red + 8, h->red + RED_NUINV2, h->red + 8 + nv,
@@ -482,7 +482,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
         // v_{j+1} = (w1 - V h2) / sqrt(|w1|^2 - |h2|^2) together with the next
         // V-cycle's input.  After the first pass w1 is nearly orthogonal to V,
         // so the last norm has no cancellation.
-        need_partials(h, (size_t)j + 2, std::max(h->stencil_blocks, (n + 255) / 256));
+        need_partials(h, (size_t)j + 2, std::max(4 * h->stencil_blocks, (n + 255) / 256));
         launch_apply_dots<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->V, n, j + 1, h->partial, &nb);
         launch_finalize(h->s, h->partial, j + 1, nb, h->red + 8, 0, h->hpin_dev + 8);
         launch_axpy_dots(h->s, n, w, h->V, n, j + 1, h->red + 8, h->partial, &nb);
@@ -556,8 +556,10 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
     // per-block partial sums: up to mrest+2 vectors per launch (fused Krylov
     // reductions over the cell grid), or 1024 blocks for grid-stride reductions
     h->stencil_blocks = (size_t)((d.nx + 63) / 64) * ((d.ny + 3) / 4) * d.nz;
+    // the operator pass with the basis dots writes one partial per wave (4 per block)
     h->partial_cap =
-        (size_t)(h->mrest + 4) * std::max<size_t>(std::max<size_t>(h->stencil_blocks, (n + 255) / 256), 1024) + 4096;
+        (size_t)(h->mrest + 4) * std::max<size_t>(std::max<size_t>(4 * h->stencil_blocks, (n + 255) / 256), 1024) +
+        4096;
     h->partial = h->ar.alloc<double>(h->partial_cap);
     h->red = h->ar.alloc<double>(512);
     h->snap = h->ar.alloc<double>(3 * M);
@@ -1143,8 +1145,8 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   switch (kernel) {
     case 0: *bytes = N * (8 + 72 + 1 + 8); break;  // phi, g11..g33, mask -> y           = 89
     case 1:
-      if (!h->mg->line0())  // x, b, g, mask -> x: FP32 49 / FP64 97
-        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 36 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+      if (!h->mg->line0())  // x, b, g, mask -> x: FP32 level (FP16 g) 31 / FP64 97
+        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
       else  // y-line step: x, b, g, mask -> r (FP16 g in FP32), then r, factors, x -> x: 55 / 145
         *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4)
                                              : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8));

@@ -125,11 +125,10 @@ template <typename TF>
 void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f);
 
 // ---- multigrid ---------------------------------------------------------------------
-// A multigrid level as the kernels see it.  Face coefficients are in g (the
-// level precision), except on the y-line levels of the FP32 hierarchy, which keep
-// them in gh as FP16 scaled by a power of two (gh * gs are the coefficients): a
-// y-line step reads them twice (residual, then the line solve's factors), and
-// those levels are bandwidth-bound where the point levels are not.
+// A multigrid level as the kernels see it.  The levels of the FP32 hierarchy keep
+// their face coefficients in gh as FP16 scaled by a power of two (gh * gs are the
+// coefficients); FP64 levels keep them in g.  FP16 halves the coefficient bytes,
+// the largest share of a smoothing sweep's traffic.
 template <typename T>
 struct LevelView {
   Dims d;

@@ -40,6 +40,70 @@ __device__ inline void block_dots(bool in, size_t c, double a, const double* __r
   __syncthreads();  // lds is reused by the block's next tile
 }
 
+// x from a lane DPP_CTRL selects (0 where the source lane is out of its row or
+// the lane is masked out by row_mask / bank_mask), a double moved as two dwords
+template <int DPP_CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  union {
+    double d;
+    int i[2];
+  } u, r;
+  u.d = x;
+  r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], DPP_CTRL, ROW_MASK, BANK_MASK, true);
+  r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], DPP_CTRL, ROW_MASK, BANK_MASK, true);
+  return r.d;
+}
+
+// Sum of v over the 64 lanes of a wave, left in lane 63 (other lanes hold partial
+// sums), without LDS: row shifts by 1, 2, 4 and 8 leave each row's sum in its
+// last lane, then row broadcasts 15 and 31 carry the row sums into lane 63.
+// Every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum_to63(double v) {
+  v += dpp_f64<0x111, 0xF, 0xF>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xF, 0xF>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xF, 0xE>(v);  // row_shr:4 into banks 1-3
+  v += dpp_f64<0x118, 0xF, 0xC>(v);  // row_shr:8 into banks 2-3
+  v += dpp_f64<0x142, 0xA, 0xF>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xC, 0xF>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// V_l[c] for l < min(nv, 8), loaded before the value they multiply is computed so
+// the loads overlap that computation
+__device__ __forceinline__ void wave_dots_prefetch(bool in, size_t c, const double* __restrict__ V, size_t ldv,
+                                                   int nv, double (&v8)[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; q++) v8[q] = (in && q < nv) ? V[(size_t)q * ldv + c] : 0.;
+}
+
+// Per-wave sums of a * V_l[c] for l < nv and of a * a (slot nv), written by the
+// wave's last lane to partial[l * nw + wv] (nw waves in the launch, wv this
+// wave's index).  v8 holds V_l[c] for l < 8 (wave_dots_prefetch); the rest are
+// loaded eight at a time.  Every lane of the wave must call it (cells outside the
+// grid pass in = false); nv < 64.
+__device__ inline void wave_dots(bool in, size_t c, double a, const double* __restrict__ V, size_t ldv, int nv,
+                                 const double (&v8)[8], double* __restrict__ partial, size_t nw, size_t wv) {
+  const int lane = (threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) & 63;
+  for (int l0 = 0; l0 <= nv; l0 += 8) {
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int l = l0 + q;
+      double v = 0.;
+      if (in && l < nv) v = a * (l0 == 0 ? v8[q] : V[(size_t)l * ldv + c]);
+      else if (in && l == nv) v = a * a;
+      p[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) p[q] = wave_sum_to63(p[q]);
+    if (lane == 63) {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (l0 + q <= nv) partial[(size_t)(l0 + q) * nw + wv] = p[q];
+    }
+  }
+}
+
 // Per-block sums of per-thread accumulators acc[0..ns) (ns <= S <= 64), written to
 // partial[l * nblk + b]: wave sums by shuffles, the four wave results of a
 // 256-thread block meet in LDS.  Every thread of the block must call it.
