@@ -281,27 +281,41 @@ __device__ __forceinline__ int axis_pick(int f, int n, int per, bool coarsened, 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // xf += P xc at fine cell (i, j, k): trilinear (27,9,9,3,9,3,3,1)/64, one-sided
-// next to coarse IB cells in the reference's order k, j, i (poisson.c:500-505).
+// next to coarse IB cells (poisson.c:500-505; the three neighbour tests are
+// independent).  Every operand is loaded at once - the fine mask, the three
+// coarse neighbours' masks and the coarse values at both choices on each axis
+// (cached) - and the one-sided choice is a register select, so the cell costs
+// one round trip to memory instead of three.
 template <typename T>
 __device__ __forceinline__ void prolong_add_cell(const LevelView<T>& F, const Dims& dc,
                                                  const uint8_t* __restrict__ clsc, int cmask,
                                                  const T* __restrict__ xc, T* __restrict__ xf, int i, int j, int k) {
   const size_t c = F.d.idx(i, j, k);
-  if (F.cls[c] & CLS_NONFLUID) return;
   int ic, jc, kc;
-  int ia = axis_pick(i, F.d.nx, F.d.per[0], cmask & 1, ic);
-  int ja = axis_pick(j, F.d.ny, F.d.per[1], cmask & 2, jc);
-  int ka = axis_pick(k, F.d.nz, F.d.per[2], cmask & 4, kc);
+  const int ia = axis_pick(i, F.d.nx, F.d.per[0], cmask & 1, ic);
+  const int ja = axis_pick(j, F.d.ny, F.d.per[1], cmask & 2, jc);
+  const int ka = axis_pick(k, F.d.nz, F.d.per[2], cmask & 4, kc);
   const int in = wrapc(ic + ia, dc.nx), jn = wrapc(jc + ja, dc.ny), kn = wrapc(kc + ka, dc.nz);
-  if (ka != 0 && (clsc[dc.idx(ic, jc, kn)] & CLS_NONFLUID)) ka = 0;
-  if (ja != 0 && (clsc[dc.idx(ic, jn, kc)] & CLS_NONFLUID)) ja = 0;
-  if (ia != 0 && (clsc[dc.idx(in, jc, kc)] & CLS_NONFLUID)) ia = 0;
-  const int i2 = ia ? in : ic, j2 = ja ? jn : jc, k2 = ka ? kn : kc;
-  const T v = (xc[dc.idx(ic, jc, kc)] * T(9) + xc[dc.idx(ic, j2, kc)] * T(3) + xc[dc.idx(i2, jc, kc)] * T(3) +
-               xc[dc.idx(i2, j2, kc)]) * T(3. / 64.) +
-              (xc[dc.idx(ic, jc, k2)] * T(9) + xc[dc.idx(ic, j2, k2)] * T(3) + xc[dc.idx(i2, jc, k2)] * T(3) +
-               xc[dc.idx(i2, j2, k2)]) * T(1. / 64.);
-  xf[c] += v;
+  const uint8_t fm = F.cls[c];
+  const uint8_t mk = clsc[dc.idx(ic, jc, kn)], mj = clsc[dc.idx(ic, jn, kc)], mi = clsc[dc.idx(in, jc, kc)];
+  const T x000 = xc[dc.idx(ic, jc, kc)], x010 = xc[dc.idx(ic, jn, kc)];
+  const T x100 = xc[dc.idx(in, jc, kc)], x110 = xc[dc.idx(in, jn, kc)];
+  const T x001 = xc[dc.idx(ic, jc, kn)], x011 = xc[dc.idx(ic, jn, kn)];
+  const T x101 = xc[dc.idx(in, jc, kn)], x111 = xc[dc.idx(in, jn, kn)];
+  const T xo = xf[c];
+  // neighbour used on each axis: the other coarse cell unless the axis is not
+  // coarsened, the cell sits on a non-periodic wall, or that coarse cell is IB
+  const bool ui = ia != 0 && !(mi & CLS_NONFLUID);
+  const bool uj = ja != 0 && !(mj & CLS_NONFLUID);
+  const bool uk = ka != 0 && !(mk & CLS_NONFLUID);
+  const T y0 = uj ? x010 : x000, y1 = uj ? x110 : x100;  // (ic, j2, kc), (in, j2, kc)
+  const T z0 = uj ? x011 : x001, z1 = uj ? x111 : x101;  // (ic, j2, kn), (in, j2, kn)
+  const T a_c = x000, a_j = y0, a_i = ui ? x100 : x000, a_ij = ui ? y1 : y0;
+  const T b_c = uk ? x001 : a_c, b_j = uk ? z0 : a_j;
+  const T b_i = uk ? (ui ? x101 : x001) : a_i, b_ij = uk ? (ui ? z1 : z0) : a_ij;
+  const T v = (a_c * T(9) + a_j * T(3) + a_i * T(3) + a_ij) * T(3. / 64.) +
+              (b_c * T(9) + b_j * T(3) + b_i * T(3) + b_ij) * T(1. / 64.);
+  if (!(fm & CLS_NONFLUID)) xf[c] = xo + v;
 }
 
 // x = Cinv f for rows 64 rb .. 64 rb + 63, Cinv column-major n x n (n <= 1024):
