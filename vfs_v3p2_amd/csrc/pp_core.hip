@@ -18,6 +18,7 @@ namespace {
 
 inline dim3 cell_grid(const Dims& d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
 inline dim3 dmda_grid(const Dmda& m) { return dim3((m.mx + 63) / 64, (m.my + 3) / 4, m.mz); }
+inline unsigned bnd_blocks(const Dmda& m) { return (boundary_count(m) + 255u) / 256u; }
 const dim3 kBlk(64, 4, 1);
 
 __device__ inline int blin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
@@ -275,10 +276,11 @@ __global__ __launch_bounds__(256) void k_add_phi(Dmda m, Dims d, const uint8_t* 
   add_phi_cell(m, d, cls, phi, p, i, j, k);
 }
 
+// Boundary-point kernels: one thread per boundary point of the DMDA box
+// (boundary_point), 256-thread blocks.
 __global__ __launch_bounds__(256) void k_periodic_scalar(Dmda m, double* __restrict__ a, int zero_other) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= m.mx || j >= m.my) return;
-  if (on_boundary(m, i, j, k)) periodic_point(m, a, zero_other, i, j, k);
+  int i, j, k;
+  if (boundary_point(m, blockIdx.x * 256u + threadIdx.x, i, j, k)) periodic_point(m, a, zero_other, i, j, k);
 }
 
 // ------------------------------------------------------------------ Projection
@@ -292,9 +294,8 @@ __global__ __launch_bounds__(256) void k_projection(Dmda m, Dims d, const double
 }
 
 __global__ __launch_bounds__(256) void k_ucont_ghosts(Dmda m, double* __restrict__ u) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= m.mx || j >= m.my) return;
-  ucont_ghost_point(m, u, i, j, k);
+  int i, j, k;
+  if (boundary_point(m, blockIdx.x * 256u + threadIdx.x, i, j, k)) ucont_ghost_point(m, u, i, j, k);
 }
 
 // ----------------------------------------------------------------- Contra2Cart
@@ -307,17 +308,15 @@ __global__ __launch_bounds__(256) void k_c2c_core(Dmda m, Dims d, const double* 
 }
 
 __global__ __launch_bounds__(256) void k_vec_ghosts(Dmda m, double* __restrict__ a, double* __restrict__ snap) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= m.mx || j >= m.my) return;
-  if (on_boundary(m, i, j, k)) vec_ghost_point(m, a, snap, i, j, k);
+  int i, j, k;
+  if (boundary_point(m, blockIdx.x * 256u + threadIdx.x, i, j, k)) vec_ghost_point(m, a, snap, i, j, k);
 }
 
 __global__ __launch_bounds__(256) void k_c2c_bc(Dmda m, const double* __restrict__ nv, const double* __restrict__ snap,
                                                 double* __restrict__ ucat, int b0, int b1, int b2, int b3, int b4,
                                                 int b5) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (i >= m.mx || j >= m.my) return;
-  if (!on_boundary(m, i, j, k)) return;
+  int i, j, k;
+  if (!boundary_point(m, blockIdx.x * 256u + threadIdx.x, i, j, k)) return;
   const int bc[6] = {b0, b1, b2, b3, b4, b5};
   c2c_bc_point(m, nv, snap, ucat, bc, i, j, k);
 }
@@ -697,7 +696,7 @@ void launch_add_phi(hipStream_t s, Dmda m, Dims d, const uint8_t* cls, const dou
 }
 
 void launch_periodic_scalar(hipStream_t s, Dmda m, double* a, int zero_other) {
-  k_periodic_scalar<<<dmda_grid(m), kBlk, 0, s>>>(m, a, zero_other);
+  k_periodic_scalar<<<bnd_blocks(m), 256, 0, s>>>(m, a, zero_other);
   HIPCHECK(hipGetLastError());
 }
 
@@ -709,7 +708,7 @@ void launch_projection(hipStream_t s, Dmda m, Dims d, const double* g, const uin
 
 void launch_ucont_ghosts(hipStream_t s, Dmda m, double* ucont) {
   if (!(m.per[0] || m.per[1] || m.per[2])) return;
-  k_ucont_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucont);
+  k_ucont_ghosts<<<bnd_blocks(m), 256, 0, s>>>(m, ucont);
   HIPCHECK(hipGetLastError());
 }
 
@@ -717,14 +716,14 @@ void launch_c2c(hipStream_t s, Dmda m, Dims d, const double* met9, const uint8_t
                 const double* ucont, double* ucat, double* scratch, const int bc[6]) {
   k_c2c_core<<<cell_grid(d), kBlk, 0, s>>>(m, d, met9, cls, ucont, ucat);
   HIPCHECK(hipGetLastError());
-  k_vec_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucat, scratch);
+  k_vec_ghosts<<<bnd_blocks(m), 256, 0, s>>>(m, ucat, scratch);
   HIPCHECK(hipGetLastError());
-  k_c2c_bc<<<dmda_grid(m), kBlk, 0, s>>>(m, nvert, scratch, ucat, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+  k_c2c_bc<<<bnd_blocks(m), 256, 0, s>>>(m, nvert, scratch, ucat, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
   HIPCHECK(hipGetLastError());
   k_c2c_solid<<<cell_grid(d), kBlk, 0, s>>>(d, m, cls, ucat);
   HIPCHECK(hipGetLastError());
   if (m.per[0] || m.per[1] || m.per[2]) {
-    k_vec_ghosts<<<dmda_grid(m), kBlk, 0, s>>>(m, ucat, nullptr);
+    k_vec_ghosts<<<bnd_blocks(m), 256, 0, s>>>(m, ucat, nullptr);
     HIPCHECK(hipGetLastError());
   }
 }
