@@ -648,6 +648,7 @@ struct BatchCore {
     int j = 0, it = 0, vc = 0, m = 0;
     double tgt = 0., rel = 1.;
     bool conv = false, pending_update = false, fresh = false;
+    bool x_zero = false;  // x is 0 (a zero-start case before its first cycle): r = b
     int pending_nv = 0;
     std::vector<double> H, cs, sn, gv, y;
     std::vector<double> nu, sigma;  // lazily normalised basis: |U_l| and the scale of Z_l
@@ -1444,6 +1445,8 @@ struct BatchCore {
         sl.fresh = false;
       }
     }
+    // zero-start cases need no operator pass for their first true residual (r = b)
+    for (int q : fr) slots[q].x_zero = slots[q].h->opts.zero_guess != 0;
     Phase R;
     R.steps.reserve(5);
     {
@@ -1489,6 +1492,7 @@ struct BatchCore {
       PhaseStep& S = R.step(PH_RESID, 1);
       for (int q : qs) {
         vfs_pp_handle* h = slots[q].h;
+        if (slots[q].x_zero) continue;
         PhaseItem it = case_item(h);
         it.r0 = h->b;
         it.r1 = h->x;
@@ -1503,7 +1507,7 @@ struct BatchCore {
         if (!(h->div_fac > 0.)) continue;
         PhaseItem it = case_item(h);
         it.r0 = h->aj;
-        it.r1 = h->r;
+        it.r1 = true_residual_src(h, slots[q].x_zero);
         S.add(it, h->red + RED_DIVEST, h->hpin_dev + RED_DIVEST);
       }
     }
@@ -1514,8 +1518,10 @@ struct BatchCore {
     for (int q : qs) {
       Slot& sl = slots[q];
       vfs_pp_handle* h = sl.h;
+      const bool xz = sl.x_zero;
+      sl.x_zero = false;
       if (sl.S.bnorm == 0.) { sl.conv = true; sl.rel = 0.; sl.state = FINISH; continue; }
-      const double beta = std::sqrt(h->hpin[0]);
+      const double beta = std::sqrt(true_residual_host_norm2(h, xz));
       if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
       sl.rel = beta / sl.S.bnorm;
       if (residual_verdict(h, beta, sl.S.bnorm, sl.tgt)) { sl.conv = true; sl.state = FINISH; continue; }
@@ -1523,10 +1529,10 @@ struct BatchCore {
       // v_0 = r / beta and its copy as the first V-cycle input; a lazily normalised
       // basis reads 1 / |U_0|^2 = 1
       PhaseItem it = case_item(h);
-      it.r0 = h->r;
+      it.r0 = true_residual_src(h, xz);
       it.w0 = h->V;
       it.f = sl.mg->f0();
-      it.sc = h->red;
+      it.sc = true_residual_norm2(h, xz);
       it.one = lazy ? h->red + RED_NUINV2 : nullptr;
       SC.add(it);
       if (lazy) {

@@ -400,11 +400,13 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
   double rel = 1.;
   *conv = false;
   double tgt = rtol * bnorm;
+  // the first check of a zero-start solve needs no operator pass (r = b)
+  bool x_zero = h->opts.zero_guess != 0;
   while (true) {
     int nb = 0;  // block count of the reductions below
-    launch_true_residual(h, h->s);
+    launch_true_residual(h, h->s, x_zero);
     HIPCHECK(hipStreamSynchronize(h->s));
-    const double beta = std::sqrt(h->hpin[0]);
+    const double beta = std::sqrt(true_residual_host_norm2(h, x_zero));
     rel = bnorm > 0. ? beta / bnorm : 0.;
     if (!std::isfinite(beta)) throw StatusError(VFS_PP_ENONFINITE, "non-finite residual");
     if (h->opts.verbose)
@@ -413,7 +415,9 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
     if (residual_verdict(h, beta, bnorm, tgt)) { *conv = true; break; }
     if (it >= h->opts.max_iters) break;
     // v_0 = r / beta and its copy as the first V-cycle input
-    launch_scale_conv<TZ>(h->s, n, h->r, h->red, h->V, static_cast<TZ*>(h->mg->f0()));
+    launch_scale_conv<TZ>(h->s, n, true_residual_src(h, x_zero), true_residual_norm2(h, x_zero), h->V,
+                          static_cast<TZ*>(h->mg->f0()));
+    x_zero = false;
     if (h->opts.orthog == 2) {
       nu[0] = 1.;
       sigma[0] = 1.;

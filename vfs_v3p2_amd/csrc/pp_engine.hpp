@@ -285,14 +285,27 @@ inline size_t workspace_bytes(const vfs_pp_handle* h) {
 namespace pp {
 // True residual r = b - A x in FP64 with ||r||^2 into red[0] / hpin[0] and, when
 // the divergence rule is on, max |aj r| into hpin[RED_DIVEST]; launches only.
-inline void launch_true_residual(vfs_pp_handle* h, hipStream_t s) {
+// With x_zero (x is known to be 0) r is b itself: no operator pass, ||b||^2 is
+// the RHS's own reduction (red / hpin[RED_B2]), and the divergence estimate reads
+// b.  true_residual_src / _norm2 name the vector and the device norm^2 to use.
+inline void launch_true_residual(vfs_pp_handle* h, hipStream_t s, bool x_zero = false) {
   int nb = 0;
-  launch_residual64<double>(s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
-  launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+  if (!x_zero) {
+    launch_residual64<double>(s, h->d, h->g64, h->code, h->b, h->x, h->r, h->partial, &nb);
+    launch_finalize(s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
+  }
   if (h->div_fac > 0.) {
-    launch_divest(s, h->d, h->code, h->aj, h->r, h->partial, &nb);
+    launch_divest(s, h->d, h->code, h->aj, x_zero ? h->b : h->r, h->partial, &nb);
     launch_finalize(s, h->partial, 1, nb, h->red + RED_DIVEST, 1, h->hpin_dev + RED_DIVEST);
   }
+}
+inline const double* true_residual_src(const vfs_pp_handle* h, bool x_zero) { return x_zero ? h->b : h->r; }
+inline const double* true_residual_norm2(const vfs_pp_handle* h, bool x_zero) {
+  return x_zero ? h->red + RED_B2 : h->red;
+}
+// ||r||^2 on the host after the stream sync
+inline double true_residual_host_norm2(const vfs_pp_handle* h, bool x_zero) {
+  return x_zero ? h->hpin[RED_B2] : h->hpin[0];
 }
 
 // Stopping rule at a true-residual check (after the stream sync): ||r|| <= tgt
