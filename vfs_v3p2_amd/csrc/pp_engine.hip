@@ -65,9 +65,12 @@ static void row_of(const HostLevel& L, int i, int j, int k, double (&vol)[19], N
 // isolated fluid cells), coarse face coefficients summed over the active fine
 // faces that make up each coarse face and divided by the coarsening factor of
 // the difference direction.
-static HostLevel coarsen(HostLevel& F) {
+// Coarse level of F: 2:1 on every axis with at least 3 cells, except j when keep_y
+// (semi-coarsening in i and k) while i or k can still be coarsened.
+static HostLevel coarsen(HostLevel& F, bool keep_y) {
   HostLevel C;
-  const int cm = (F.d.nx >= 3 ? 1 : 0) | (F.d.ny >= 3 ? 2 : 0) | (F.d.nz >= 3 ? 4 : 0);
+  int cm = (F.d.nx >= 3 ? 1 : 0) | (F.d.ny >= 3 ? 2 : 0) | (F.d.nz >= 3 ? 4 : 0);
+  if (keep_y && (cm & 5)) cm &= ~2;
   F.cmask = cm;
   C.d = F.d;
   if (cm & 1) C.d.nx = (F.d.nx + 1) / 2;
@@ -266,6 +269,41 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
   return best > 0. ? best : 2.;
 }
 
+// Coupling of each row along j against i and k: yc / oc with yc = |a_N| + |a_S| and
+// oc = |a_E| + |a_W| + |a_T| + |a_B|.  Returns the largest ratio and the fraction of
+// rows whose ratio is below 1/4 (rows with oc = 0 are skipped).
+static void level_coupling(const HostLevel& H, double* worst, double* weak_frac) {
+  double w = 0.;
+  size_t rows = 0, weak = 0;
+  for (int k = 0; k < H.d.nz; k++)
+    for (int j = 0; j < H.d.ny; j++)
+      for (int i = 0; i < H.d.nx; i++) {
+        const size_t c = H.d.idx(i, j, k);
+        if (!(H.code[c] & ROW_BIT)) continue;
+        double vol[19];
+        Nbr nb;
+        row_of(H, i, j, k, vol, nb);
+        const double yc = std::fabs(vol[NP]) + std::fabs(vol[SP]);
+        const double oc = std::fabs(vol[EP]) + std::fabs(vol[WP]) + std::fabs(vol[TP]) + std::fabs(vol[BP]);
+        if (!(oc > 0.)) continue;
+        w = std::max(w, yc / oc);
+        rows++;
+        if (yc < 0.25 * oc) weak++;
+      }
+  *worst = w;
+  *weak_frac = rows ? (double)weak / (double)rows : 0.;
+}
+
+// y-line smoothing on a level: always with smoother 2; with smoother 0 when the
+// wall-normal coupling dominates somewhere (|a_N| + |a_S| > 4 (|a_E| + |a_W| + |a_T|
+// + |a_B|) on some row); never when j is periodic
+static int line_rule(const HostLevel& H, const vfs_pp_opts& o, double worst) {
+  if (H.d.per[1]) return 0;
+  if (o.smoother == 2) return 1;
+  if (o.smoother != 0) return 0;
+  return worst > 4.0 ? 1 : 0;
+}
+
 template <typename T>
 static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   auto mg = std::make_unique<MG<T>>();
@@ -278,25 +316,7 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     L.d = H.d;
     const size_t n = H.d.n();
     // smoother: y-line when the wall-normal coupling dominates (and j is not periodic)
-    int line = 0;
-    if (h->opts.smoother == 2) line = 1;
-    else if (h->opts.smoother == 0 && !H.d.per[1]) {
-      double worst = 0.;
-      for (int k = 0; k < H.d.nz; k++)
-        for (int j = 0; j < H.d.ny; j++)
-          for (int i = 0; i < H.d.nx; i++) {
-            const size_t c = H.d.idx(i, j, k);
-            if (!(H.code[c] & ROW_BIT)) continue;
-            double vol[19];
-            Nbr nb;
-            row_of(H, i, j, k, vol, nb);
-            const double yc = std::fabs(vol[NP]) + std::fabs(vol[SP]);
-            const double oc = std::fabs(vol[EP]) + std::fabs(vol[WP]) + std::fabs(vol[TP]) + std::fabs(vol[BP]);
-            if (oc > 0.) worst = std::max(worst, yc / oc);
-          }
-      line = worst > 4.0 ? 1 : 0;
-    }
-    if (H.d.per[1]) line = 0;
+    const int line = H.line;
     L.line = line;
     // coefficients: FP16 on every level of the FP32 hierarchy, scaled by a power
     // of two that puts the largest magnitude below 2^14 (FP16 keeps normal numbers
@@ -638,8 +658,21 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
     HostLevel& F = h->hl.back();
     const bool can = F.d.nx >= 3 || F.d.ny >= 3 || F.d.nz >= 3;
     if ((long long)F.d.n() <= h->opts.coarse_max || !can || h->hl.size() >= 16) break;
-    HostLevel C = coarsen(F);
+    // j is kept on y-line levels where more than 1% of the rows couple weakly
+    // along j (below 1/4 of their i-k coupling): a y-line smoother barely reduces
+    // errors smooth in i-k but oscillating in j there, so the coarse level must
+    // still represent them (semi-coarsening in i and k)
+    double worst = 0., weak = 0.;
+    level_coupling(F, &worst, &weak);
+    F.line = line_rule(F, h->opts, worst);
+    HostLevel C = coarsen(F, F.line && weak > 0.01);
     h->hl.push_back(std::move(C));
+  }
+  {
+    HostLevel& Lc = h->hl.back();  // the coarsest level's smoother choice (it may be smoothed too)
+    double worst = 0., weak = 0.;
+    level_coupling(Lc, &worst, &weak);
+    Lc.line = line_rule(Lc, h->opts, worst);
   }
   if ((long long)h->hl.back().d.n() > 1024)
     throw StatusError(VFS_PP_EINVAL, "grid cannot be coarsened to a direct-solve size");

@@ -72,6 +72,7 @@ struct HostLevel {
   std::vector<uint32_t> code;  // n
   std::vector<uint8_t> cls;    // n, CLS_NONFLUID only on coarse levels
   int cmask = 0;               // axes coarsened towards the next level
+  int line = 0;                // the level is smoothed with y-lines
 };
 
 inline Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {d.per[0], d.per[1], d.per[2]}}; }
