@@ -271,7 +271,7 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
 
 // Coupling of each row along j against i and k: yc / oc with yc = |a_N| + |a_S| and
 // oc = |a_E| + |a_W| + |a_T| + |a_B|.  Returns the largest ratio and the fraction of
-// rows whose ratio is below 1/4 (rows with oc = 0 are skipped).
+// rows whose ratio is below 1/10 (rows with oc = 0 are skipped).
 static void level_coupling(const HostLevel& H, double* worst, double* weak_frac) {
   double w = 0.;
   size_t rows = 0, weak = 0;
@@ -288,7 +288,7 @@ static void level_coupling(const HostLevel& H, double* worst, double* weak_frac)
         if (!(oc > 0.)) continue;
         w = std::max(w, yc / oc);
         rows++;
-        if (yc < 0.25 * oc) weak++;
+        if (yc < 0.1 * oc) weak++;
       }
   *worst = w;
   *weak_frac = rows ? (double)weak / (double)rows : 0.;
@@ -659,7 +659,7 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
     const bool can = F.d.nx >= 3 || F.d.ny >= 3 || F.d.nz >= 3;
     if ((long long)F.d.n() <= h->opts.coarse_max || !can || h->hl.size() >= 16) break;
     // j is kept on y-line levels where more than 1% of the rows couple weakly
-    // along j (below 1/4 of their i-k coupling): a y-line smoother barely reduces
+    // along j (below 1/10 of their i-k coupling): a y-line smoother barely reduces
     // errors smooth in i-k but oscillating in j there, so the coarse level must
     // still represent them (semi-coarsening in i and k)
     double worst = 0., weak = 0.;
