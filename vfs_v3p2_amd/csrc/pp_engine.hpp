@@ -93,7 +93,7 @@ struct MG : MGBase {
   struct Lev {
     Dims d{};
     T* g = nullptr;            // face coefficients, or
-    _Float16* gh = nullptr;    // FP16 coefficients / gs (y-line levels of the FP32 hierarchy)
+    _Float16* gh = nullptr;    // FP16 coefficients / gs (the levels of the FP32 hierarchy)
     T gs = T(1);
     const uint32_t* code = nullptr;
     const uint8_t* cls = nullptr;
