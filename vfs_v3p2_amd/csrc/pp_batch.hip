@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void kb_restrict_blk(const RestrictItem<T>* __
     if (!sd[tw_ & 0xffffu].y) continue;
     const uint32_t lt = t - start[q];
     resid_restrict_blk(I.F, I.f, I.x, I.dc, I.clsc, I.fc, (int)(lt % I.tx), (int)((lt / I.tx) % I.ty),
-                       (int)(lt / (I.tx * I.ty)), lds);
+                       (int)(lt / (I.tx * I.ty)), I.cmask == 5, lds);
   }
 }
 
@@ -892,7 +892,7 @@ struct BatchCore {
           it.fc = C.f;
           it.cmask = (C.d.nx != L.d.nx ? 1 : 0) | (C.d.ny != L.d.ny ? 2 : 0) | (C.d.nz != L.d.nz ? 4 : 0);
           it.slot = q;
-          if (it.cmask == 7 && L.d.nx >= 64) {  // narrower levels: flat cells fill the lanes better
+          if ((it.cmask == 7 || it.cmask == 5) && L.d.nx >= 64) {  // narrower levels: flat cells fill the lanes better
             it.tx = (uint32_t)(L.d.nx + 63) / 64;
             it.ty = (uint32_t)(L.d.ny + 1) / 2;
             rb.push_back(it);

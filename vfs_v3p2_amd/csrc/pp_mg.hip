@@ -72,10 +72,11 @@ __global__ __launch_bounds__(256) void k_resid_restrict(LevelView<T> F, const T*
 template <typename T>
 __global__ __launch_bounds__(256) void k_resid_restrict_blk(LevelView<T> F, const T* __restrict__ f,
                                                             const T* __restrict__ x, Dims dc,
-                                                            const uint8_t* __restrict__ clsc, T* __restrict__ fc) {
+                                                            const uint8_t* __restrict__ clsc, T* __restrict__ fc,
+                                                            int keep_y) {
   __shared__ T lds[2][2][32];
   const uint3 xb = xcd_block();
-  resid_restrict_blk(F, f, x, dc, clsc, fc, (int)xb.x, (int)xb.y, (int)xb.z, lds);
+  resid_restrict_blk(F, f, x, dc, clsc, fc, (int)xb.x, (int)xb.y, (int)xb.z, keep_y != 0, lds);
 }
 
 template <typename T>
@@ -248,9 +249,9 @@ template <typename T>
 void launch_resid_restrict(hipStream_t s, LevelView<T> F, const T* f, const T* x, Dims dc, const uint8_t* cls_c,
                            T* fc) {
   const int cm = (dc.nx != F.d.nx ? 1 : 0) | (dc.ny != F.d.ny ? 2 : 0) | (dc.nz != F.d.nz ? 4 : 0);
-  if (cm == 7) {
+  if (cm == 7 || cm == 5) {  // 2x2x2 children, or 2x1x2 (i and k coarsened, j kept)
     const dim3 gr((F.d.nx + 63) / 64, (F.d.ny + 1) / 2, (F.d.nz + 1) / 2);
-    k_resid_restrict_blk<T><<<gr, dim3(64, 2, 2), 0, s>>>(F, f, x, dc, cls_c, fc);
+    k_resid_restrict_blk<T><<<gr, dim3(64, 2, 2), 0, s>>>(F, f, x, dc, cls_c, fc, cm == 5 ? 1 : 0);
   } else {
     k_resid_restrict<T><<<cell_grid(dc), kBlk, 0, s>>>(F, f, x, dc, cls_c, cm, fc);
   }

@@ -243,7 +243,7 @@ template <typename T>
 __device__ __forceinline__ void resid_restrict_blk(const LevelView<T>& F, const T* __restrict__ f,
                                                    const T* __restrict__ x, const Dims& dc,
                                                    const uint8_t* __restrict__ clsc, T* __restrict__ fc, int bx,
-                                                   int by, int bz, T (*lds)[2][32]) {
+                                                   int by, int bz, bool keep_y, T (*lds)[2][32]) {
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int i = bx * 64 + tx, j = by * 2 + ty, k = bz * 2 + tz;
   T r = T(0);
@@ -260,7 +260,17 @@ __device__ __forceinline__ void resid_restrict_blk(const LevelView<T>& F, const 
   r += __shfl_xor(r, 1, 64);
   if ((tx & 1) == 0) lds[tz][ty][tx >> 1] = r;
   __syncthreads();
-  if (ty == 0 && tz == 0 && tx < 32) {
+  if (keep_y) {
+    // i and k coarsened only: the two j rows of the tile are two coarse cells
+    if (tz == 0 && tx < 32) {
+      const int ic = bx * 32 + tx, jc = by * 2 + ty, kc = bz;
+      if (ic < dc.nx && jc < dc.ny && kc < dc.nz) {
+        const size_t C = dc.idx(ic, jc, kc);
+        const T v = lds[0][ty][tx] + lds[1][ty][tx];
+        fc[C] = (clsc[C] & CLS_NONFLUID) ? T(0) : v;
+      }
+    }
+  } else if (ty == 0 && tz == 0 && tx < 32) {
     const int ic = bx * 32 + tx, jc = by, kc = bz;
     if (ic < dc.nx && jc < dc.ny && kc < dc.nz) {
       const size_t C = dc.idx(ic, jc, kc);
