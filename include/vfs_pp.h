@@ -112,8 +112,11 @@ int vfs_pp_set_opts(vfs_pp_handle *h, const vfs_pp_opts *opts);
 /* Geometry: the UserCtx fields lICsi..lKZet, lIAj..lKAj, lAj, lCsi/lEta/lZet and
  * lNvert (variables.h:333-356, 378-385), host pointers in DMDA layout.  Builds the
  * face coefficients (poisson.c:1105-1141), the per-cell mask codes of
- * PoissonLHSNew's branches (poisson.c:1146-1558) and the multigrid hierarchy;
- * everything stays resident on the device. */
+ * PoissonLHSNew's branches (poisson.c:1146-1558), the VolumeFlux face list and the
+ * multigrid hierarchy, all on the device; everything stays resident there.  A
+ * later call replaces the geometry and frees the previous hierarchy: the
+ * reference's per-step rebuild for moving or rotating bodies, sediment and level
+ * sets (setup_lidx2 + PoissonLHSNew + a new solver, poisson_hypre.c:627-648). */
 int vfs_pp_set_geometry(vfs_pp_handle *h, const double *icsi, const double *ieta,
                         const double *izet, const double *jcsi, const double *jeta,
                         const double *jzet, const double *kcsi, const double *keta,
@@ -250,6 +253,27 @@ int vfs_pp_apply_device(vfs_pp_handle *h, const double *phi, double *y, int drop
  * per-face branch codes (18 bytes per point: faces i+,i-,j+,j-,k+,k- x {active,
  * cross1, cross2}).  Host outputs; for verification. */
 int vfs_pp_export_stencil(vfs_pp_handle *h, double *vol19, uint8_t *cases);
+
+/* Level `level` of the multigrid hierarchy (0 = the finest), for verification; no
+ * counterpart in the reference, whose preconditioner is BoomerAMG.  *nlevels (may
+ * be NULL) receives the number of levels, dims (may be NULL) the level's nx, ny,
+ * nz, *line (may be NULL) 1 when the level is smoothed by y-lines.  When vol19 and
+ * cases are both non-NULL they receive the level's rows as vfs_pp_export_stencil
+ * gives them, on the level's DMDA box (nx+2)(ny+2)(nz+2); the FP32 hierarchy holds
+ * FP16 coefficients times a power of two, so its rows carry that rounding.
+ * VFS_PP_EINVAL if level is out of range (after filling *nlevels). */
+int vfs_pp_export_level(vfs_pp_handle *h, int level, int *nlevels, int dims[3], int *line, double *vol19,
+                        uint8_t *cases);
+
+/* The VolumeFlux face list as the engine holds it, for verification: the faces
+ * between a fluid cell and an IB cell that VolumeFlux corrects (poisson.c:2670-2700
+ * for immersed == 1, 2936-3010 otherwise), in (k, j, i, axis) order.  *count
+ * receives their number; when pos, area and sign are non-NULL and cap >= *count,
+ * pos[f] = 3 * DMDA point + axis (the face's index in the AoS Ucont array),
+ * area[f] = |csi|, |eta| or |zet| of the cell, sign[f] = +1 from fluid to IB, -1
+ * from IB to fluid. */
+int vfs_pp_export_ib_faces(vfs_pp_handle *h, size_t cap, long long *pos, double *area, float *sign,
+                           size_t *count);
 
 /* IB classification (ibm.c:331-353, 540-568) on a device DMDA array holding 4
  * for inside cells and 0 elsewhere; rewritten in place: interior cells to

@@ -67,6 +67,9 @@ def lib():
         _lib.or_pressure_gradient.argtypes = [g] + [d] * 15
         _lib.or_ray_s.restype = C.c_double
         _lib.or_ray_s.argtypes = [C.c_int]
+        _lib.or_level_coupling.argtypes = [g, d, d, d, d]
+        _lib.or_level_coarsen.restype = C.c_int
+        _lib.or_level_coarsen.argtypes = [g, d, d, C.c_int, g, d, d]
         _lib.or_mg_create.restype = C.c_void_p
         _lib.or_mg_create.argtypes = [g, d, d, C.c_int, C.c_double, C.c_double, C.c_int]
         _lib.or_mg_destroy.argtypes = [C.c_void_p]
@@ -203,6 +206,25 @@ def mg_nvert_restrict(gf, gc, nvf):
     nvc = np.zeros((gc.mz, gc.my, gc.mx))
     lib().or_mg_nvert_restrict(C.byref(gf), C.byref(gc), _p(nvf), _p(nvc))
     return nvc
+
+
+def level_coupling(g, g9, nvert):
+    """(largest |a_N|+|a_S| over |a_E|+|a_W|+|a_T|+|a_B|, fraction of rows below 1/10)
+    of a multigrid level given by its DMDA face coefficients and nvert."""
+    w, f = C.c_double(), C.c_double()
+    lib().or_level_coupling(C.byref(g), _p(g9), _p(nvert), C.byref(w), C.byref(f))
+    return w.value, f.value
+
+
+def level_coarsen(g, g9, nvert, keep_y=False):
+    """One coarsening step of the CPU multigrid: (coarse grid, DMDA face
+    coefficients, nvert, coarsened-axes mask)."""
+    gc = Grid()
+    lib().or_level_coarsen(C.byref(g), _p(g9), _p(nvert), int(keep_y), C.byref(gc), None, None)
+    g9c = np.zeros((9, gc.mz, gc.my, gc.mx))
+    nvc = np.zeros((gc.mz, gc.my, gc.mx))
+    cm = lib().or_level_coarsen(C.byref(g), _p(g9), _p(nvert), int(keep_y), C.byref(gc), _p(g9c), _p(nvc))
+    return gc, g9c, nvc, cm
 
 
 def fully_blocked(g, nvert):

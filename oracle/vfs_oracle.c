@@ -927,8 +927,16 @@ static void lev_alloc(mg_lev *L) {
 
 static void dm_periodic(const or_grid *g, double *a) { periodic_copy_scalar(g, a); }
 
-static void coarsen_level(mg_lev *F, mg_lev *C) {
-  int cm = (F->nx >= 3 ? 1 : 0) | (F->ny >= 3 ? 2 : 0) | (F->nz >= 3 ? 4 : 0);
+/* axes coarsened from a level of nx x ny x nz cells: every axis with at least 3
+ * cells, except j when keep_y (semi-coarsening in i and k) while i or k can be */
+static int coarse_mask(int nx, int ny, int nz, int keep_y) {
+  int cm = (nx >= 3 ? 1 : 0) | (ny >= 3 ? 2 : 0) | (nz >= 3 ? 4 : 0);
+  if (keep_y && (cm & 5)) cm &= ~2;
+  return cm;
+}
+
+static void coarsen_level(mg_lev *F, mg_lev *C, int keep_y) {
+  int cm = coarse_mask(F->nx, F->ny, F->nz, keep_y);
   F->cmask = cm;
   C->nx = (cm & 1) ? (F->nx + 1) / 2 : F->nx;
   C->ny = (cm & 2) ? (F->ny + 1) / 2 : F->ny;
@@ -1128,6 +1136,64 @@ static double *bordered_inverse(mg_lev *L) {
   return out;
 }
 
+/* Coupling of each fluid row along j against i and k: yc = |a_N| + |a_S|,
+ * oc = |a_E| + |a_W| + |a_T| + |a_B| from the assembled row; the largest yc / oc and
+ * the fraction of rows with yc < oc / 10 (rows with oc = 0 are skipped). */
+void or_level_coupling(const or_grid *g, const double *g9, const double *nvert, double *worst, double *weak_frac) {
+  mg_lev L;
+  memset(&L, 0, sizeof L);
+  L.nx = g->mx - 2; L.ny = g->my - 2; L.nz = g->mz - 2;
+  L.g = *g;
+  L.g9 = (double *)g9;
+  L.nv = (double *)nvert;
+  lev_rows(&L);
+  double w = 0.;
+  size_t rows = 0, weak = 0;
+  for (int k = 0; k < L.nz; k++)
+    for (int j = 0; j < L.ny; j++)
+      for (int i = 0; i < L.nx; i++) {
+        size_t p = ldm(&L, i, j, k);
+        if (!L.row[p]) continue;
+        const double *v = L.vol + 19 * p;
+        double yc = fabs(v[3]) + fabs(v[4]);             /* (0,+1,0), (0,-1,0) */
+        double oc = fabs(v[1]) + fabs(v[2]) + fabs(v[5]) + fabs(v[6]);
+        if (!(oc > 0.)) continue;
+        if (yc / oc > w) w = yc / oc;
+        rows++;
+        if (yc < 0.1 * oc) weak++;
+      }
+  *worst = w;
+  *weak_frac = rows ? (double)weak / (double)rows : 0.;
+  free(L.vol); free(L.row); free(L.cases);
+}
+
+int or_level_coarsen(const or_grid *gf, const double *g9f, const double *nvf, int keep_y, or_grid *gc, double *g9c,
+                     double *nvc) {
+  const int nx = gf->mx - 2, ny = gf->my - 2, nz = gf->mz - 2;
+  const int cm = coarse_mask(nx, ny, nz, keep_y);
+  gc->mx = ((cm & 1) ? (nx + 1) / 2 : nx) + 2;
+  gc->my = ((cm & 2) ? (ny + 1) / 2 : ny) + 2;
+  gc->mz = ((cm & 4) ? (nz + 1) / 2 : nz) + 2;
+  for (int a = 0; a < 3; a++) gc->per[a] = gf->per[a];
+  if (!g9c || !nvc) return cm;
+  mg_lev F, C;
+  memset(&F, 0, sizeof F);
+  memset(&C, 0, sizeof C);
+  F.nx = nx; F.ny = ny; F.nz = nz;
+  for (int a = 0; a < 3; a++) F.per[a] = gf->per[a];
+  F.g = *gf;
+  F.g9 = (double *)g9f;
+  F.nv = (double *)nvf;
+  lev_rows(&F);
+  coarsen_level(&F, &C, keep_y);
+  const size_t M = (size_t)C.g.mx * C.g.my * C.g.mz;
+  memcpy(g9c, C.g9, 9 * M * sizeof(double));
+  memcpy(nvc, C.nv, M * sizeof(double));
+  free(F.vol); free(F.row); free(F.cases);
+  free(C.g9); free(C.nv); free(C.vol); free(C.row); free(C.cases);
+  return cm;
+}
+
 or_mg *or_mg_create(const or_grid *g, const double *g9, const double *nvert, int degree, double lo, double hi,
                     int coarse_max) {
   or_mg *mg = (or_mg *)calloc(1, sizeof(or_mg));
@@ -1149,7 +1215,7 @@ or_mg *or_mg_create(const or_grid *g, const double *g9, const double *nvert, int
     int can = F->nx >= 3 || F->ny >= 3 || F->nz >= 3;
     if ((long)F->n <= coarse_max || !can) break;
     mg_lev *C = &mg->L[mg->nl];
-    coarsen_level(F, C);
+    coarsen_level(F, C, 0);
     lev_alloc(C);
     mg->nl++;
   }

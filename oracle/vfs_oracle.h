@@ -158,6 +158,16 @@ typedef struct or_mg or_mg;
 or_mg *or_mg_create(const or_grid *g, const double *g9, const double *nvert, int degree, double lo, double hi,
                     int coarse_max);
 void or_mg_destroy(or_mg *mg);
+/* Set-up steps of the same hierarchy, one level at a time.  or_level_coupling:
+ * the largest |a_N| + |a_S| over |a_E| + |a_W| + |a_T| + |a_B| of the fluid rows
+ * and the fraction of rows below 1/10 (rows with no i-k coupling skipped).
+ * or_level_coarsen: one 2:1 step (j kept when keep_y while i or k can still be
+ * coarsened); fills gc and, when g9c and nvc are non-NULL, the coarse DMDA face
+ * coefficients and nvert; returns the coarsened axes (bit 0 i, 1 j, 2 k). */
+void or_level_coupling(const or_grid *g, const double *g9, const double *nvert, double *worst, double *weak_frac);
+int or_level_coarsen(const or_grid *gf, const double *g9f, const double *nvf, int keep_y, or_grid *gc, double *g9c,
+                     double *nvc);
+
 void or_mg_vcycle(or_mg *mg, const double *r, double *z);
 size_t or_mg_n(const or_mg *mg);
 int or_mg_levels(const or_mg *mg);

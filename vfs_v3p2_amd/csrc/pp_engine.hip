@@ -24,142 +24,23 @@
 
 namespace pp {
 
-static void periodic_fill_host(const Dmda& m, std::vector<double>& a) {
-  for (int k = 0; k < m.mz; k++)
-    for (int j = 0; j < m.my; j++)
-      for (int i = 0; i < m.mx; i++) {
-        int aa = i, bb = j, cc = k;
-        bool f = false;
-        if (m.per[0] && i == 0) aa = m.mx - 2, f = true;
-        else if (m.per[0] && i == m.mx - 1) aa = 1, f = true;
-        if (m.per[1] && j == 0) bb = m.my - 2, f = true;
-        else if (m.per[1] && j == m.my - 1) bb = 1, f = true;
-        if (m.per[2] && k == 0) cc = m.mz - 2, f = true;
-        else if (m.per[2] && k == m.mz - 1) cc = 1, f = true;
-        if (f) a[m.idx(i, j, k)] = a[m.idx(aa, bb, cc)];
-      }
-}
+// The coarsest level on the host, for its dense inverse.
+struct CoarseRows {
+  Dims d{};
+  std::vector<double> g;       // 9*n face coefficients
+  std::vector<uint32_t> code;  // n
+};
 
-static void codes_host(HostLevel& L) {
-  const Dmda m = dmda_of(L.d);
-  L.code.assign(L.d.n(), 0u);
-  L.cls.assign(L.d.n(), 0);
-  for (int k = 0; k < L.d.nz; k++)
-    for (int j = 0; j < L.d.ny; j++)
-      for (int i = 0; i < L.d.nx; i++) {
-        const size_t c = L.d.idx(i, j, k);
-        L.code[c] = make_code(L.nv.data(), m.mx, m.my, m.mz, m.per, i + 1, j + 1, k + 1);
-        if (L.nv[m.idx(i + 1, j + 1, k + 1)] > 0.1) L.cls[c] = CLS_NONFLUID;
-      }
-}
-
-static void row_of(const HostLevel& L, int i, int j, int k, double (&vol)[19], Nbr& nb) {
+static void row_of(const CoarseRows& L, int i, int j, int k, double (&vol)[19], Nbr& nb) {
   neighbours(L.d, i, j, k, nb);
   const size_t c = L.d.idx(i, j, k);
   build_row(L.g.data(), L.d.n(), L.code[c], c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
 }
 
-// Next coarser level: cell-centred 2:1 coarsening per axis with n >= 3 (odd
-// counts leave a one-child cell at the high end), coarse nvert by the
-// MyNvertRestriction rule (poisson.c:2042-2097: IB when every child is, then
-// isolated fluid cells), coarse face coefficients summed over the active fine
-// faces that make up each coarse face and divided by the coarsening factor of
-// the difference direction.
-// Coarse level of F: 2:1 on every axis with at least 3 cells, except j when keep_y
-// (semi-coarsening in i and k) while i or k can still be coarsened.
-static HostLevel coarsen(HostLevel& F, bool keep_y) {
-  HostLevel C;
-  int cm = (F.d.nx >= 3 ? 1 : 0) | (F.d.ny >= 3 ? 2 : 0) | (F.d.nz >= 3 ? 4 : 0);
-  if (keep_y && (cm & 5)) cm &= ~2;
-  F.cmask = cm;
-  C.d = F.d;
-  if (cm & 1) C.d.nx = (F.d.nx + 1) / 2;
-  if (cm & 2) C.d.ny = (F.d.ny + 1) / 2;
-  if (cm & 4) C.d.nz = (F.d.nz + 1) / 2;
-  const Dmda mf = dmda_of(F.d), mc = dmda_of(C.d);
-  const size_t nc = C.d.n(), nf = F.d.n();
-  auto ch = [&](int axis, int ci, int& lo, int& hi) {
-    const int n = axis == 0 ? F.d.nx : axis == 1 ? F.d.ny : F.d.nz;
-    if (cm & (1 << axis)) { lo = 2 * ci; hi = std::min(2 * ci + 1, n - 1); }
-    else { lo = hi = ci; }
-  };
-  // nvert
-  C.nv.assign(mc.n(), 0.);
-  for (int k = 0; k < C.d.nz; k++)
-    for (int j = 0; j < C.d.ny; j++)
-      for (int i = 0; i < C.d.nx; i++) {
-        int i0, i1, j0, j1, k0, k1;
-        ch(0, i, i0, i1); ch(1, j, j0, j1); ch(2, k, k0, k1);
-        double prod = 1.;
-        for (int kk = k0; kk <= k1; kk++)
-          for (int jj = j0; jj <= j1; jj++)
-            for (int ii = i0; ii <= i1; ii++) prod *= F.nv[mf.idx(ii + 1, jj + 1, kk + 1)];
-        if (prod > 0.1) C.nv[mc.idx(i + 1, j + 1, k + 1)] = 1.;
-      }
-  periodic_fill_host(mc, C.nv);
-  {
-    std::vector<double> pre = C.nv;
-    const size_t sy = mc.mx, sz = (size_t)mc.mx * mc.my;
-    for (int k = 1; k < mc.mz - 1; k++)
-      for (int j = 1; j < mc.my - 1; j++)
-        for (int i = 1; i < mc.mx - 1; i++) {
-          const size_t p = mc.idx(i, j, k);
-          if (pre[p] < 0.1 && pre[p + 1] + pre[p - 1] > 1.1 && pre[p + sy] + pre[p - sy] > 1.1 &&
-              pre[p + sz] + pre[p - sz] > 1.1)
-            C.nv[p] = 1.;
-        }
-    periodic_fill_host(mc, C.nv);
-  }
-  // face coefficients
-  C.g.assign(9 * nc, 0.);
-  const int fac[3] = {(cm & 1) ? 2 : 1, (cm & 2) ? 2 : 1, (cm & 4) ? 2 : 1};
-  for (int k = 0; k < C.d.nz; k++)
-    for (int j = 0; j < C.d.ny; j++)
-      for (int i = 0; i < C.d.nx; i++) {
-        const size_t cc = C.d.idx(i, j, k);
-        int r[3][2];
-        ch(0, i, r[0][0], r[0][1]);
-        ch(1, j, r[1][0], r[1][1]);
-        ch(2, k, r[2][0], r[2][1]);
-        for (int d = 0; d < 3; d++) {
-          int lo[3] = {r[0][0], r[1][0], r[2][0]}, hi[3] = {r[0][1], r[1][1], r[2][1]};
-          lo[d] = hi[d];  // the fine faces on the coarse upper face belong to the last child
-          double acc[3] = {0., 0., 0.};
-          for (int kk = lo[2]; kk <= hi[2]; kk++)
-            for (int jj = lo[1]; jj <= hi[1]; jj++)
-              for (int ii = lo[0]; ii <= hi[0]; ii++) {
-                const size_t fc = F.d.idx(ii, jj, kk);
-                if (!((F.code[fc] >> (2 * d)) & 1u)) continue;
-                for (int e = 0; e < 3; e++) acc[e] += F.g[(size_t)(3 * d + e) * nf + fc];
-              }
-          for (int e = 0; e < 3; e++) C.g[(size_t)(3 * d + e) * nc + cc] = acc[e] / fac[e];
-        }
-      }
-  codes_host(C);
-  // a coarse row whose faces all vanish cannot be smoothed: treat it as solid
-  for (int it = 0; it < 3; it++) {
-    bool changed = false;
-    for (int k = 0; k < C.d.nz; k++)
-      for (int j = 0; j < C.d.ny; j++)
-        for (int i = 0; i < C.d.nx; i++) {
-          const size_t c = C.d.idx(i, j, k);
-          if (!(C.code[c] & ROW_BIT)) continue;
-          double vol[19];
-          Nbr nb;
-          row_of(C, i, j, k, vol, nb);
-          if (vol[CP] == 0.) { C.nv[mc.idx(i + 1, j + 1, k + 1)] = 1.; changed = true; }
-        }
-    if (!changed) break;
-    periodic_fill_host(mc, C.nv);
-    codes_host(C);
-  }
-  return C;
-}
-
 // Bordered inverse of the coarsest operator: [A E; E^T 0]^-1, top-left block,
 // where E has one indicator column per connected fluid component (the
 // constants of each component span the null space of the Neumann problem).
-static std::vector<double> coarse_inverse(const HostLevel& L, int* ncomp_out) {
+static std::vector<double> coarse_inverse(const CoarseRows& L, int* ncomp_out) {
   const int n = (int)L.d.n();
   std::vector<int> comp(n, -1);
   std::vector<std::vector<std::pair<int, double>>> rows(n);
@@ -269,111 +150,128 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
   return best > 0. ? best : 2.;
 }
 
-// Coupling of each row along j against i and k: yc / oc with yc = |a_N| + |a_S| and
-// oc = |a_E| + |a_W| + |a_T| + |a_B|.  Returns the largest ratio and the fraction of
-// rows whose ratio is below 1/10 (rows with oc = 0 are skipped).
-static void level_coupling(const HostLevel& H, double* worst, double* weak_frac) {
-  double w = 0.;
-  size_t rows = 0, weak = 0;
-  for (int k = 0; k < H.d.nz; k++)
-    for (int j = 0; j < H.d.ny; j++)
-      for (int i = 0; i < H.d.nx; i++) {
-        const size_t c = H.d.idx(i, j, k);
-        if (!(H.code[c] & ROW_BIT)) continue;
-        double vol[19];
-        Nbr nb;
-        row_of(H, i, j, k, vol, nb);
-        const double yc = std::fabs(vol[NP]) + std::fabs(vol[SP]);
-        const double oc = std::fabs(vol[EP]) + std::fabs(vol[WP]) + std::fabs(vol[TP]) + std::fabs(vol[BP]);
-        if (!(oc > 0.)) continue;
-        w = std::max(w, yc / oc);
-        rows++;
-        if (yc < 0.1 * oc) weak++;
-      }
-  *worst = w;
-  *weak_frac = rows ? (double)weak / (double)rows : 0.;
-}
-
 // y-line smoothing on a level: always with smoother 2; with smoother 0 when the
 // wall-normal coupling dominates somewhere (|a_N| + |a_S| > 4 (|a_E| + |a_W| + |a_T|
 // + |a_B|) on some row); never when j is periodic
-static int line_rule(const HostLevel& H, const vfs_pp_opts& o, double worst) {
-  if (H.d.per[1]) return 0;
+static int line_rule(const Dims& d, const vfs_pp_opts& o, double worst) {
+  if (d.per[1]) return 0;
   if (o.smoother == 2) return 1;
   if (o.smoother != 0) return 0;
   return worst > 4.0 ? 1 : 0;
 }
 
+// The multigrid hierarchy, built on the device from the finest level's arrays
+// (h->g64, h->code, h->nvert).  Per level, the row statistics choose the smoother
+// and whether j is kept; then one coarsening step, until the level is small enough
+// for the dense solve.  Coarse FP64 coefficients stay (in h->geo) only for the FP64
+// hierarchy; nvert arrays and the FP32 hierarchy's FP64 coefficients go to tmp.
+static void build_hierarchy(vfs_pp_handle* h, Arena& tmp) {
+  hipStream_t s = h->s;
+  auto& H = h->setup;
+  H.clear();
+  need_partials(h, 1, level_stats_partials());
+  SetupLevel L0;
+  L0.d = h->d;
+  L0.g = h->g64;
+  L0.nv = h->nvert;
+  L0.code = h->code;
+  L0.cls = h->geo.alloc<uint8_t>(h->d.n());
+  launch_cls_from_code(s, h->d.n(), h->code, L0.cls);
+  H.push_back(L0);
+  Arena& gstore = h->opts.precision == 1 ? tmp : h->geo;
+  while (true) {
+    SetupLevel& F = H.back();
+    launch_level_stats(s, F.d, F.g, F.code, h->partial, h->hpin_dev + RED_SETUP);
+    HIPCHECK(hipStreamSynchronize(s));
+    const double worst = h->hpin[RED_SETUP], rows = h->hpin[RED_SETUP + 1];
+    const double weak = rows > 0. ? h->hpin[RED_SETUP + 2] / rows : 0.;
+    F.gmax = h->hpin[RED_SETUP + 3];
+    F.line = line_rule(F.d, h->opts, worst);
+    const bool can = F.d.nx >= 3 || F.d.ny >= 3 || F.d.nz >= 3;
+    if ((long long)F.d.n() <= h->opts.coarse_max || !can || H.size() >= 16) break;
+    // 2:1 on every axis with at least 3 cells (odd counts leave a one-child cell at
+    // the high end).  j is kept on y-line levels where more than 1% of the rows
+    // couple weakly along j (below 1/10 of their i-k coupling): a y-line smoother
+    // barely reduces errors smooth in i-k but oscillating in j there, so the coarse
+    // level must still represent them (semi-coarsening in i and k)
+    int cm = (F.d.nx >= 3 ? 1 : 0) | (F.d.ny >= 3 ? 2 : 0) | (F.d.nz >= 3 ? 4 : 0);
+    if (F.line && weak > 0.01 && (cm & 5)) cm &= ~2;
+    F.cmask = cm;
+    SetupLevel C;
+    C.d = F.d;
+    if (cm & 1) C.d.nx = (F.d.nx + 1) / 2;
+    if (cm & 2) C.d.ny = (F.d.ny + 1) / 2;
+    if (cm & 4) C.d.nz = (F.d.nz + 1) / 2;
+    const size_t nc = C.d.n(), Mc = dmda_of(C.d).n();
+    double* nv_tmp = tmp.alloc<double>(Mc);
+    double* nvc = tmp.alloc<double>(Mc);
+    double* gc = gstore.alloc<double>(9 * nc);
+    C.code = h->geo.alloc<uint32_t>(nc);
+    C.cls = h->geo.alloc<uint8_t>(nc);
+    launch_coarsen(s, F.d, F.nv, F.g, F.code, cm, C.d, nv_tmp, nvc, gc, C.code, C.cls);
+    C.nv = nvc;
+    C.g = gc;
+    H.push_back(C);
+  }
+  if ((long long)H.back().d.n() > 1024)
+    throw StatusError(VFS_PP_EINVAL, "grid cannot be coarsened to a direct-solve size");
+}
+
+// VolumeFlux face list (poisson.c:2670-2700 / 2936-3010) in (k, j, i, axis) order,
+// and the number of cells with nvert < 0.1
+static void build_ib_faces(vfs_pp_handle* h, const double* csi, const double* eta, const double* zet, Arena& tmp) {
+  hipStream_t s = h->s;
+  const size_t nb = ibface_blocks(h->d);
+  int* cnt = tmp.alloc<int>(3 * nb);
+  launch_ibface_count(s, h->m, h->d, h->nvert, h->grid.immersed, cnt, cnt + nb, cnt + 2 * nb,
+                      h->hpin_dev + RED_SETUP);
+  HIPCHECK(hipStreamSynchronize(s));
+  h->nibf = (size_t)h->hpin[RED_SETUP];
+  h->n_lt = h->hpin[RED_SETUP + 1];
+  h->ibf = nullptr;
+  if (h->nibf) {
+    h->ibf = h->geo.alloc<IbFace>(h->nibf);
+    launch_ibface_write(s, h->m, h->d, h->nvert, csi, eta, zet, h->grid.immersed, cnt + 2 * nb, h->ibf);
+  }
+}
+
 template <typename T>
 static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   auto mg = std::make_unique<MG<T>>();
+  Arena& A = h->geo;
   mg->degree = std::max(1, h->opts.cheb_degree);
   const size_t n0 = h->d.n();
-  mg->scr = h->ar.alloc<T>(2 * n0);
-  for (size_t l = 0; l < h->hl.size(); l++) {
-    HostLevel& H = h->hl[l];
+  mg->scr = A.alloc<T>(2 * n0);
+  for (size_t l = 0; l < h->setup.size(); l++) {
+    const SetupLevel& H = h->setup[l];
     typename MG<T>::Lev L;
     L.d = H.d;
     const size_t n = H.d.n();
-    // smoother: y-line when the wall-normal coupling dominates (and j is not periodic)
     const int line = H.line;
     L.line = line;
     // coefficients: FP16 on every level of the FP32 hierarchy, scaled by a power
     // of two that puts the largest magnitude below 2^14 (FP16 keeps normal numbers
-    // down to 2^-14); FP64 levels keep FP64
-    const bool half = std::is_same<T, float>::value;
-    double gs = 1.;
-    if (half) {
-      double gmax = 0.;
-      for (size_t i = 0; i < 9 * n; i++) gmax = std::max(gmax, std::fabs(H.g[i]));
-      if (gmax > 0.) gs = std::ldexp(1., std::ilogb(gmax) + 1 - 14);
-    }
-    L.gs = T(gs);
-    if (half) {
-      L.gh = h->ar.alloc<_Float16>(9 * n);
-      if (l == 0) {
-        launch_convert_scaled<double, _Float16>(h->s, 9 * n, h->g64, 1. / gs, L.gh);
-      } else {
-        std::vector<_Float16> gt(9 * n);
-        for (size_t i = 0; i < 9 * n; i++) gt[i] = (_Float16)(H.g[i] * (1. / gs));
-        h2d(h->s, L.gh, gt.data(), 9 * n);
-      }
-    } else if (l == 0 && std::is_same<T, double>::value) {
-      L.g = (T*)h->g64;
+    // down to 2^-14); the FP64 hierarchy uses the set-up's FP64 coefficients (g64 on
+    // the finest level)
+    if constexpr (std::is_same<T, float>::value) {
+      double gs = 1.;
+      if (H.gmax > 0.) gs = std::ldexp(1., std::ilogb(H.gmax) + 1 - 14);
+      L.gs = T(gs);
+      L.gh = A.alloc<_Float16>(9 * n);
+      launch_convert_scaled<double, _Float16>(h->s, 9 * n, H.g, 1. / gs, L.gh);
     } else {
-      L.g = h->ar.alloc<T>(9 * n);
-      if (l == 0) {
-        launch_convert<double, T>(h->s, 9 * n, h->g64, L.g);
-      } else {
-        std::vector<T> gt(9 * n);
-        for (size_t i = 0; i < 9 * n; i++) gt[i] = T(H.g[i]);
-        h2d(h->s, L.g, gt.data(), 9 * n);
-      }
+      L.g = const_cast<T*>(H.g);
     }
-    if (l == 0) {
-      L.code = h->code;
-      uint8_t* cls = h->ar.alloc<uint8_t>(n);
-      std::vector<uint8_t> c(n);
-      for (size_t i = 0; i < n; i++) c[i] = (H.code[i] & ROW_BIT) ? 0 : CLS_NONFLUID;
-      h2d(h->s, cls, c.data(), n);
-      L.cls = cls;
-    } else {
-      uint32_t* code = h->ar.alloc<uint32_t>(n);
-      h2d(h->s, code, H.code.data(), n);
-      uint8_t* cls = h->ar.alloc<uint8_t>(n);
-      h2d(h->s, cls, H.cls.data(), n);
-      L.code = code;
-      L.cls = cls;
-    }
-    HIPCHECK(hipStreamSynchronize(h->s));
-    L.f = h->ar.alloc<T>(n);
-    L.xa = h->ar.alloc<T>(n);
-    L.xb = h->ar.alloc<T>(n);
-    L.dv = h->ar.alloc<T>(n);
-    L.dinv = h->ar.alloc<T>(n);
+    L.code = H.code;
+    L.cls = H.cls;
+    L.f = A.alloc<T>(n);
+    L.xa = A.alloc<T>(n);
+    L.xb = A.alloc<T>(n);
+    L.dv = A.alloc<T>(n);
+    L.dinv = A.alloc<T>(n);
     launch_dinv<T>(h->s, L.view(), L.dinv);
     if (line) {
-      L.lfac = h->ar.alloc<T>(3 * n);
+      L.lfac = A.alloc<T>(3 * n);
       launch_line_factors<T>(h->s, L.view(), L.lfac);
     }
     mg->lev.push_back(L);
@@ -385,14 +283,22 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     MG<T>::cheb(L.lam, mg->degree, h->opts.cheb_lo, h->opts.cheb_hi, L.c1, L.c2);
   }
   // coarsest dense inverse
-  HostLevel& Hc = h->hl.back();
+  const SetupLevel& Hc = h->setup.back();
+  CoarseRows R;
+  R.d = Hc.d;
+  const size_t nc = Hc.d.n();
+  R.g.resize(9 * nc);
+  R.code.resize(nc);
+  d2h(h->s, R.g.data(), Hc.g, 9 * nc);
+  d2h(h->s, R.code.data(), Hc.code, nc);
+  HIPCHECK(hipStreamSynchronize(h->s));
   int ncomp = 0;
-  std::vector<double> ci = coarse_inverse(Hc, &ncomp);
+  std::vector<double> ci = coarse_inverse(R, &ncomp);
   std::vector<T> cit(ci.size());
   for (size_t i = 0; i < ci.size(); i++) cit[i] = T(ci[i]);
-  mg->ncoarse = (int)Hc.d.n();
+  mg->ncoarse = (int)nc;
   mg->ncomp = ncomp;
-  mg->cinv = h->ar.alloc<T>(cit.size());
+  mg->cinv = A.alloc<T>(cit.size());
   h2d(h->s, mg->cinv, cit.data(), cit.size());
   HIPCHECK(hipStreamSynchronize(h->s));
   return mg;
@@ -557,14 +463,21 @@ void ensure_geom(vfs_pp_handle* h) {
 }
 
 // Set-up from the 16 metric arrays on the device (dmet: icsi..kzet, iaj, jaj, kaj,
-// aj, csi, eta, zet) and nvert on the device, with host copies of csi/eta/zet and
-// nvert for the VolumeFlux face list and the host hierarchy.
-static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const double* dnvert, const double* csi,
-                              const double* eta, const double* zet, const double* nvert) {
+// aj, csi, eta, zet) and nvert on the device.  Everything is built on the device;
+// the host only reads back a few statistics per level and inverts the coarsest
+// level (at most 1024 cells).
+static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const double* dnvert) {
   const Dmda m = h->m;
   const Dims d = h->d;
   const size_t M = m.n(), n = d.n();
   hipStream_t s = h->s;
+  // a new geometry replaces the previous one's hierarchy and face list
+  HIPCHECK(hipStreamSynchronize(s));
+  h->have_geom = false;
+  h->mg.reset();
+  h->ibf = nullptr;
+  h->nibf = 0;
+  h->geo.release();
   // resident arrays
   if (!h->g64) {
     h->g64 = h->ar.alloc<double>(9 * n);
@@ -602,88 +515,14 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
                      dmet[10], dmet[11], h->g64);
   launch_codes(s, m, d, h->nvert, h->code, h->cls);
   launch_gather_cell_metrics(s, m, d, dmet[13], dmet[14], dmet[15], dmet[12], h->met9, h->aj);
-  HIPCHECK(hipStreamSynchronize(s));
-  // VolumeFlux face list (poisson.c:2670-2700 / 2936-3010)
-  {
-    std::vector<IbFace> faces;
-    const size_t st[3] = {1, (size_t)m.mx, (size_t)m.mx * m.my};
-    const int mm[3] = {m.mx, m.my, m.mz};
-    const double* met3[3] = {csi, eta, zet};
-    double cnt = 0.;
-    for (int k = 1; k < m.mz - 1; k++)
-      for (int j = 1; j < m.my - 1; j++)
-        for (int i = 1; i < m.mx - 1; i++) {
-          const size_t c = m.idx(i, j, k);
-          const int pos[3] = {i, j, k};
-          const double nc = nvert[c];
-          if (nc < 0.1) cnt += 1.;
-          for (int dd = 0; dd < 3; dd++) {
-            const double nq = nvert[c + st[dd]];
-            const double* mv = &met3[dd][3 * c];
-            const double area = std::sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
-            float sign = 0.f;
-            if (h->grid.immersed == 1) {
-              if (!(pos[dd] < mm[dd] - 2)) continue;
-              if (nc < 0.1 && nq > 0.1 && nq < 1.1) sign = 1.f;
-              else if (nc > 0.1 && nc < 1.1 && nq < 0.1) sign = -1.f;
-            } else {
-              if (pos[dd] > mm[dd] - 2) continue;
-              if (nc < TH && nq > TH) sign = 1.f;
-              else if (nc > TH && nq < TH) sign = -1.f;
-            }
-            if (sign != 0.f) faces.push_back(IbFace{(uint64_t)(3 * c + dd), area, sign, 0.f});
-          }
-        }
-    h->n_lt = cnt;
-    h->nibf = faces.size();
-    if (!faces.empty()) {
-      h->ibf = h->ar.alloc<IbFace>(faces.size());
-      h2d(s, h->ibf, faces.data(), faces.size());
-      HIPCHECK(hipStreamSynchronize(s));
-    }
-  }
-  // host hierarchy
-  h->hl.clear();
-  HostLevel L0;
-  L0.d = d;
-  L0.g.resize(9 * n);
-  L0.code.resize(n);
-  d2h(s, L0.g.data(), h->g64, 9 * n);
-  d2h(s, L0.code.data(), h->code, n);
-  L0.nv.assign(nvert, nvert + M);
-  HIPCHECK(hipStreamSynchronize(s));
-  L0.cls.assign(n, 0);
-  h->hl.push_back(std::move(L0));
-  while (true) {
-    HostLevel& F = h->hl.back();
-    const bool can = F.d.nx >= 3 || F.d.ny >= 3 || F.d.nz >= 3;
-    if ((long long)F.d.n() <= h->opts.coarse_max || !can || h->hl.size() >= 16) break;
-    // j is kept on y-line levels where more than 1% of the rows couple weakly
-    // along j (below 1/10 of their i-k coupling): a y-line smoother barely reduces
-    // errors smooth in i-k but oscillating in j there, so the coarse level must
-    // still represent them (semi-coarsening in i and k)
-    double worst = 0., weak = 0.;
-    level_coupling(F, &worst, &weak);
-    F.line = line_rule(F, h->opts, worst);
-    HostLevel C = coarsen(F, F.line && weak > 0.01);
-    h->hl.push_back(std::move(C));
-  }
-  {
-    HostLevel& Lc = h->hl.back();  // the coarsest level's smoother choice (it may be smoothed too)
-    double worst = 0., weak = 0.;
-    level_coupling(Lc, &worst, &weak);
-    Lc.line = line_rule(Lc, h->opts, worst);
-  }
-  if ((long long)h->hl.back().d.n() > 1024)
-    throw StatusError(VFS_PP_EINVAL, "grid cannot be coarsened to a direct-solve size");
+  Arena tmp;
+  build_ib_faces(h, dmet[13], dmet[14], dmet[15], tmp);
+  build_hierarchy(h, tmp);
   if (h->opts.precision == 1) h->mg = build_mg<float>(h);
   else h->mg = build_mg<double>(h);
   h->desc = h->mg->describe();
-  // free the host copies of the fine level (the device holds them)
-  for (auto& L : h->hl) {
-    L.g.clear(); L.g.shrink_to_fit();
-    L.nv.clear(); L.nv.shrink_to_fit();
-  }
+  h->setup.clear();
+  HIPCHECK(hipStreamSynchronize(s));
   h->have_geom = true;
 }
 
@@ -911,7 +750,7 @@ int vfs_pp_set_geometry(vfs_pp_handle* h, const double* icsi, const double* ieta
   }
   double* dnv = tmp.alloc<double>(M);
   h2d(h->s, dnv, nvert, M);
-  set_geometry_core(h, dmet, dnv, csi, eta, zet, nvert);
+  set_geometry_core(h, dmet, dnv);
   return VFS_PP_OK;
   ABI_END
 }
@@ -941,14 +780,7 @@ int vfs_pp_set_grid_device(vfs_pp_handle* h, const double* coor, const double* n
   double* dmet[16];
   for (int q = 0; q < 16; q++) dmet[q] = tmp.alloc<double>((q < 9 || q >= 13) ? 3 * M : M);
   launch_form_metrics(h->s, h->m, coor, dmet);
-  // host copies for the VolumeFlux face list and the host hierarchy
-  std::vector<double> hc(3 * M), he(3 * M), hz(3 * M), hn(M);
-  d2h(h->s, hc.data(), dmet[13], 3 * M);
-  d2h(h->s, he.data(), dmet[14], 3 * M);
-  d2h(h->s, hz.data(), dmet[15], 3 * M);
-  d2h(h->s, hn.data(), nvert, M);
-  HIPCHECK(hipStreamSynchronize(h->s));
-  set_geometry_core(h, dmet, nvert, hc.data(), he.data(), hz.data(), hn.data());
+  set_geometry_core(h, dmet, nvert);
   return VFS_PP_OK;
   ABI_END
 }
@@ -1047,35 +879,91 @@ int vfs_pp_apply_device(vfs_pp_handle* h, const double* phi, double* y, int drop
   ABI_END
 }
 
+// Rows of a level (device g and code) as 19 coefficients and 18 case bytes per point
+// of its DMDA box; zero outside the fluid rows.
+static void export_rows(vfs_pp_handle* h, const Dims& d, const double* g, const uint32_t* code, double* vol19,
+                        uint8_t* cases) {
+  const Dmda m = dmda_of(d);
+  const size_t n = d.n(), M = m.n();
+  Arena tmp;
+  double* dv = tmp.alloc<double>(19 * n);
+  launch_export_rows(h->s, d, g, code, dv);
+  std::vector<double> hv(19 * n);
+  std::vector<uint32_t> hc(n);
+  d2h(h->s, hv.data(), dv, 19 * n);
+  d2h(h->s, hc.data(), code, n);
+  HIPCHECK(hipStreamSynchronize(h->s));
+  std::memset(vol19, 0, 19 * M * sizeof(double));
+  std::memset(cases, 0, 18 * M);
+  for (int k = 0; k < d.nz; k++)
+    for (int j = 0; j < d.ny; j++)
+      for (int i = 0; i < d.nx; i++) {
+        const size_t c = d.idx(i, j, k), p = m.idx(i + 1, j + 1, k + 1);
+        if (!(hc[c] & ROW_BIT)) continue;
+        std::memcpy(vol19 + 19 * p, hv.data() + 19 * c, 19 * sizeof(double));
+        for (int f = 0; f < 6; f++) {
+          const bool act = (hc[c] >> f) & 1u;
+          cases[18 * p + 3 * f] = act ? 1 : 0;
+          if (!act) continue;
+          cases[18 * p + 3 * f + 1] = (uint8_t)((hc[c] >> (6 + 4 * f)) & 3u);
+          cases[18 * p + 3 * f + 2] = (uint8_t)((hc[c] >> (8 + 4 * f)) & 3u);
+        }
+      }
+}
+
 int vfs_pp_export_stencil(vfs_pp_handle* h, double* vol19, uint8_t* cases) {
   ABI_BEGIN
   if (!h || !vol19 || !cases) return VFS_PP_EINVAL;
   ensure_geom(h);
-  const size_t n = h->d.n(), M = h->m.n();
-  Arena tmp;
-  double* dv = tmp.alloc<double>(19 * n);
-  launch_export_rows(h->s, h->d, h->g64, h->code, dv);
-  std::vector<double> hv(19 * n);
-  std::vector<uint32_t> code(n);
-  d2h(h->s, hv.data(), dv, 19 * n);
-  d2h(h->s, code.data(), h->code, n);
-  HIPCHECK(hipStreamSynchronize(h->s));
-  std::memset(vol19, 0, 19 * M * sizeof(double));
-  std::memset(cases, 0, 18 * M);
-  for (int k = 0; k < h->d.nz; k++)
-    for (int j = 0; j < h->d.ny; j++)
-      for (int i = 0; i < h->d.nx; i++) {
-        const size_t c = h->d.idx(i, j, k), p = h->m.idx(i + 1, j + 1, k + 1);
-        if (!(code[c] & ROW_BIT)) continue;
-        std::memcpy(vol19 + 19 * p, hv.data() + 19 * c, 19 * sizeof(double));
-        for (int f = 0; f < 6; f++) {
-          const bool act = (code[c] >> f) & 1u;
-          cases[18 * p + 3 * f] = act ? 1 : 0;
-          if (!act) continue;
-          cases[18 * p + 3 * f + 1] = (uint8_t)((code[c] >> (6 + 4 * f)) & 3u);
-          cases[18 * p + 3 * f + 2] = (uint8_t)((code[c] >> (8 + 4 * f)) & 3u);
-        }
-      }
+  export_rows(h, h->d, h->g64, h->code, vol19, cases);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_export_level(vfs_pp_handle* h, int level, int* nlevels, int dims[3], int* line, double* vol19,
+                        uint8_t* cases) {
+  ABI_BEGIN
+  if (!h) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  const int nl = h->mg->nlev();
+  if (nlevels) *nlevels = nl;
+  if (level < 0 || level >= nl) return VFS_PP_EINVAL;
+  Dims d{};
+  int ln = 0;
+  const uint32_t* code = nullptr;
+  h->mg->level_export(h->s, level, &d, &ln, &code, nullptr);
+  if (dims) {
+    dims[0] = d.nx;
+    dims[1] = d.ny;
+    dims[2] = d.nz;
+  }
+  if (line) *line = ln;
+  if (vol19 && cases) {
+    Arena tmp;
+    double* g = tmp.alloc<double>(9 * d.n());
+    h->mg->level_export(h->s, level, &d, &ln, &code, g);
+    export_rows(h, d, g, code, vol19, cases);
+  }
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_export_ib_faces(vfs_pp_handle* h, size_t cap, long long* pos, double* area, float* sign, size_t* count) {
+  ABI_BEGIN
+  if (!h || !count) return VFS_PP_EINVAL;
+  ensure_geom(h);
+  *count = h->nibf;
+  if (!pos || !area || !sign || cap < h->nibf) return VFS_PP_OK;
+  std::vector<IbFace> f(h->nibf);
+  if (h->nibf) {
+    d2h(h->s, f.data(), h->ibf, h->nibf);
+    HIPCHECK(hipStreamSynchronize(h->s));
+  }
+  for (size_t i = 0; i < h->nibf; i++) {
+    pos[i] = (long long)f[i].pos;
+    area[i] = f[i].area;
+    sign[i] = f[i].sign;
+  }
   return VFS_PP_OK;
   ABI_END
 }

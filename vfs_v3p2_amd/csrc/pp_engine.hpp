@@ -65,12 +65,14 @@ inline void d2h(hipStream_t s, T* h, const T* d, size_t n) {
 }
 
 // ------------------------------------------------------------- host hierarchy
-struct HostLevel {
+// One level of the multigrid hierarchy while it is being built (device arrays).
+struct SetupLevel {
   Dims d{};
-  std::vector<double> g;       // 9*n face coefficients
-  std::vector<double> nv;      // DMDA nvert of this level (with ghost planes)
-  std::vector<uint32_t> code;  // n
-  std::vector<uint8_t> cls;    // n, CLS_NONFLUID only on coarse levels
+  const double* g = nullptr;   // 9*n FP64 face coefficients
+  const double* nv = nullptr;  // DMDA nvert of this level (with ghost planes)
+  uint32_t* code = nullptr;    // n
+  uint8_t* cls = nullptr;      // n, CLS_NONFLUID where nvert > 0.1
+  double gmax = 0.;            // largest |g|
   int cmask = 0;               // axes coarsened towards the next level
   int line = 0;                // the level is smoothed with y-lines
 };
@@ -86,6 +88,10 @@ struct MGBase {
   virtual void set_out(void* out) = 0;
   virtual std::string describe() const = 0;
   virtual void time_sweep(hipStream_t s) = 0;
+  virtual int nlev() const = 0;
+  // level l: dimensions, smoother, code words; its face coefficients as FP64 into
+  // g64 (device, 9 n)
+  virtual void level_export(hipStream_t s, int l, Dims* d, int* line, const uint32_t** code, double* g64) const = 0;
 };
 
 template <typename T>
@@ -183,6 +189,20 @@ struct MG : MGBase {
     return o.str();
   }
 
+  int nlev() const override { return (int)lev.size(); }
+  void level_export(hipStream_t s, int l, Dims* d, int* line, const uint32_t** code, double* g64) const override {
+    const Lev& L = lev[l];
+    *d = L.d;
+    *line = L.line;
+    *code = L.code;
+    if (!g64) return;
+    const size_t n9 = 9 * L.d.n();
+    if (L.gh) launch_half_to_double(s, n9, L.gh, (double)L.gs, g64);
+    else if constexpr (std::is_same<T, double>::value)
+      HIPCHECK(hipMemcpyAsync(g64, L.g, n9 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else launch_convert<T, double>(s, n9, L.g, g64);
+  }
+
   void time_sweep(hipStream_t s) override {
     Lev& L = lev[0];
     launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
@@ -198,7 +218,8 @@ struct vfs_pp_handle {
   pp::Dmda m{};
   pp::Dims d{};
   hipStream_t s = nullptr;
-  pp::Arena ar;
+  pp::Arena ar;   // buffers sized by the grid alone
+  pp::Arena geo;  // buffers of the current geometry (hierarchy, face list), renewed by each set-up
   bool have_geom = false;
   // fine-level resident data
   double* g64 = nullptr;
@@ -225,7 +246,7 @@ struct vfs_pp_handle {
   // after projection = div_fac * max |aj r|; stop when that is <= div_goal
   double div_fac = 0., div_goal = 0.;
   std::unique_ptr<pp::MGBase> mg;
-  std::vector<pp::HostLevel> hl;
+  std::vector<pp::SetupLevel> setup;  // the hierarchy while set-up runs
   double* hpin = nullptr;  // pinned host scratch
   double* hpin_dev = nullptr;  // the same memory as seen by kernels
   std::string desc;
@@ -240,6 +261,7 @@ constexpr int RED_MEAN = 310;  // Remove_Nullspace sum
 constexpr int RED_B2 = 320;    // ||b||^2
 constexpr int RED_DIV = 330;   // max |aj div u|
 constexpr int RED_DIVEST = 340;  // max |aj r| at the last true-residual check
+constexpr int RED_SETUP = 350;   // set-up statistics: 4 values
 constexpr int RED_NUINV2 = 400;  // 1 / |U_l|^2 of the lazily normalised basis (up to 64)
 
 // engine functions shared with the batched engine (pp_engine.hip)

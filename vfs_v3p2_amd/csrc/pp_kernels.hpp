@@ -41,6 +41,29 @@ void launch_codes(hipStream_t s, Dmda m, Dims d, const double* nvert, uint32_t* 
 void launch_export_rows(hipStream_t s, Dims d, const double* g, const uint32_t* code, double* vol19);
 void launch_classify_nvert(hipStream_t s, Dmda m, const int bctype[6], double* nv, uint8_t* tmp);
 
+// ---- multigrid hierarchy and VolumeFlux face list, built on the device ----------
+// out4 (written by one thread, may be mapped host memory): the largest
+// (|a_N| + |a_S|) / (|a_E| + |a_W| + |a_T| + |a_B|) over the rows of the level,
+// the rows with a nonzero denominator, those of them below 1/10, the largest |g|.
+// partial holds level_stats_partials() doubles.
+void launch_level_stats(hipStream_t s, Dims d, const double* g, const uint32_t* code, double* partial, double* out4);
+size_t level_stats_partials();
+// One coarsening step (axes in cm) from level (df, DMDA nvert nvf, face coefficients
+// gf, codes codef) to dc: coarse DMDA nvert nvc (nv_tmp: a second DMDA buffer),
+// face coefficients gc, codes and CLS_NONFLUID mask.
+void launch_coarsen(hipStream_t s, Dims df, const double* nvf, const double* gf, const uint32_t* codef, int cm,
+                    Dims dc, double* nv_tmp, double* nvc, double* gc, uint32_t* codec, uint8_t* clsc);
+void launch_cls_from_code(hipStream_t s, size_t n, const uint32_t* code, uint8_t* cls);
+void launch_half_to_double(hipStream_t s, size_t n, const _Float16* a, double gs, double* out);
+// VolumeFlux faces in (k, j, i, axis) order: counts per block of 256 interior cells
+// (cnt, lt: ibface_blocks(d) ints each), their exclusive prefix off, and
+// tot2 = {faces, cells with nvert < 0.1}; then the list itself.
+size_t ibface_blocks(Dims d);
+void launch_ibface_count(hipStream_t s, Dmda m, Dims d, const double* nv, int immersed, int* cnt, int* lt, int* off,
+                         double* tot2);
+void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* csi, const double* eta,
+                         const double* zet, int immersed, const int* off, IbFace* out);
+
 // ---- FP64 operator ----------------------------------------------------------------
 // y = A x on interior vectors; x may be float or double.
 template <typename TX>

@@ -104,6 +104,10 @@ def lib():
         "vfs_pp_project": (C.c_int, [H, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
         "vfs_pp_apply_device": (C.c_int, [H, vp, vp, C.c_int]),
         "vfs_pp_export_stencil": (C.c_int, [H, dp, u8]),
+        "vfs_pp_export_level": (C.c_int, [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          dp, u8]),
+        "vfs_pp_export_ib_faces": (C.c_int, [H, C.c_size_t, C.POINTER(C.c_longlong), dp, C.POINTER(C.c_float),
+                                             C.POINTER(C.c_size_t)]),
         "vfs_pp_classify_nvert_device": (C.c_int, [C.POINTER(Grid), vp]),
         "vfs_pp_mg_restrict_device": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
         "vfs_pp_mg_prolong_device": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
@@ -298,6 +302,46 @@ class Handle:
                                            cases.ctypes.data_as(C.POINTER(C.c_uint8))), "export stencil")
         shape = (self.grid.nz + 2, self.grid.ny + 2, self.grid.nx + 2)
         return vol.reshape(shape + (19,)), cases.reshape(shape + (18,))
+
+    def export_levels(self, rows=True):
+        """The multigrid hierarchy: one dict per level with dims (nx, ny, nz), line
+        (y-line smoothing) and, with rows, the level's row coefficients and case
+        bytes on its DMDA box as export_stencil gives them."""
+        nl = C.c_int()
+        rc = lib().vfs_pp_export_level(self._h, 0, C.byref(nl), None, None, None, None)
+        _check(rc, "export level")
+        out = []
+        for lv in range(nl.value):
+            dims = (C.c_int * 3)()
+            line = C.c_int()
+            _check(lib().vfs_pp_export_level(self._h, lv, None, dims, C.byref(line), None, None), "export level")
+            nx, ny, nz = dims[0], dims[1], dims[2]
+            item = {"dims": (nx, ny, nz), "line": bool(line.value)}
+            if rows:
+                shape = (nz + 2, ny + 2, nx + 2)
+                M = shape[0] * shape[1] * shape[2]
+                vol = np.zeros(19 * M)
+                cases = np.zeros(18 * M, np.uint8)
+                _check(lib().vfs_pp_export_level(self._h, lv, None, None, None,
+                                                 vol.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 cases.ctypes.data_as(C.POINTER(C.c_uint8))), "export level")
+                item["vol"] = vol.reshape(shape + (19,))
+                item["cases"] = cases.reshape(shape + (18,))
+            out.append(item)
+        return out
+
+    def export_ib_faces(self):
+        """VolumeFlux face list: (pos = 3 * DMDA point + axis, area, sign) arrays."""
+        n = C.c_size_t()
+        _check(lib().vfs_pp_export_ib_faces(self._h, 0, None, None, None, C.byref(n)), "export IB faces")
+        pos = np.zeros(n.value, np.int64)
+        area = np.zeros(n.value)
+        sign = np.zeros(n.value, np.float32)
+        _check(lib().vfs_pp_export_ib_faces(self._h, n.value, pos.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                            area.ctypes.data_as(C.POINTER(C.c_double)),
+                                            sign.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)),
+               "export IB faces")
+        return pos, area, sign
 
     def time_kernel(self, kernel: int, reps: int = 20):
         ms, by = C.c_double(), C.c_double()
