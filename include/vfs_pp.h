@@ -211,6 +211,14 @@ int vfs_pp_form_metrics_device(const vfs_grid *grid, const double *coor, double 
  * copy of the face metrics (init.c:900 FormMetrics, then the solver set-up). */
 int vfs_pp_set_grid_device(vfs_pp_handle *h, const double *coor, const double *nvert);
 
+/* A new IB mask on the same grid (device DMDA nvert): the reference's per-step
+ * rebuild for moving or rotating bodies, sediment and level sets (setup_lidx2 +
+ * PoissonLHSNew + a new solver, poisson_hypre.c:627-648).  Code words, VolumeFlux
+ * face list and multigrid hierarchy from the new nvert, on the device; the grid's
+ * metrics stay from the last vfs_pp_set_geometry or vfs_pp_set_grid_device
+ * (VFS_PP_ESTATE before either). */
+int vfs_pp_set_nvert_device(vfs_pp_handle *h, const double *nvert);
+
 /* PoissonSolver_Hypre (poisson_hypre.c:580-771) on device-resident arrays:
  * VolumeFlux (modifies ucont, poisson_hypre.c:662-667) -> PoissonRHS2_hypre ->
  * Krylov+MG solve -> Remove_Nullspace -> Convert_Phi2_Phi.  phi is the DMDA

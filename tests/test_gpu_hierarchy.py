@@ -7,8 +7,9 @@
   exactly where the oracle's rule keeps it.
 - The VolumeFlux face list equals a restatement of poisson.c:2670-2700 (immersed
   == 1) and 2936-3010 (immersed == 0) in (k, j, i, axis) order, areas bit for bit.
-- A second set-up on the same handle (a moved body) gives the same solve as a new
-  handle with that geometry, and repeated set-ups do not grow device memory.
+- A second set-up on the same handle (a moved body, through vfs_pp_set_geometry
+  or vfs_pp_set_nvert_device) gives the same solve as a new handle with that
+  geometry, and repeated set-ups do not grow device memory.
 """
 import numpy as np
 import pytest
@@ -186,6 +187,32 @@ def test_new_geometry_on_a_live_handle(E, torch_cuda, precision):
     np.testing.assert_array_equal(phi_b, phi_a)
 
 
+@pytest.mark.parametrize("precision", [1, 0])
+def test_new_mask_on_the_same_grid(E, torch_cuda, precision):
+    """vfs_pp_set_nvert_device: the moving-body step keeps the metrics and rebuilds
+    masks, face list and hierarchy; the solve equals a new handle's."""
+    torch = torch_cuda
+    c, _ = get_case("hill_sphere")
+    d = _moved(c, 3)
+    fresh = handle(E, d, precision=precision)
+    st_ref, phi_ref, u_ref = _solve(E, torch, fresh, d)
+    fresh.close()
+    h = handle(E, c, precision=precision)
+    _solve(E, torch, h, c)
+    h.set_nvert_device(torch.from_numpy(d.fields["nvert"]).cuda())
+    st, phi, u = _solve(E, torch, h, d)
+    h.close()
+    assert st.iterations == st_ref.iterations and st.max_div == st_ref.max_div
+    np.testing.assert_array_equal(phi, phi_ref)
+    np.testing.assert_array_equal(u, u_ref)
+    # before any geometry there are no metrics to keep
+    h2 = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed))
+    with pytest.raises(E.VfsPPError) as ei:
+        h2.set_nvert_device(torch.from_numpy(d.fields["nvert"]).cuda())
+    assert ei.value.code == E.VFS_PP_ESTATE
+    h2.close()
+
+
 def test_repeated_set_up_keeps_device_memory(E, torch_cuda):
     torch = torch_cuda
     c = cases.hill_case(96, 48, 64, h=0.3, sphere=(0.6, 0.75, 0.0, 0.3), seed=0)
@@ -196,8 +223,13 @@ def test_repeated_set_up_keeps_device_memory(E, torch_cuda):
     free0 = torch.cuda.mem_get_info()[0]
     for f in (c.fields, d.fields, c.fields, d.fields):
         h.set_geometry(f)
+    nvs = [torch.from_numpy(f["nvert"]).cuda() for f in (c.fields, d.fields)]
+    for q in range(4):
+        h.set_nvert_device(nvs[q % 2])
+    del nvs
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     free1 = torch.cuda.mem_get_info()[0]
     h.close()
-    # one hierarchy of this grid holds about 20 MB; four leaked ones would show
+    # one hierarchy of this grid holds about 20 MB; eight leaked ones would show
     assert free0 - free1 < 8 << 20, (free0, free1)

@@ -15,7 +15,6 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
-#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -122,14 +121,7 @@ double now_ms() {
 template <typename T>
 static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& L) {
   const size_t n = L.d.n();
-  std::vector<T> v0(n);
-  std::mt19937 rng(1234);
-  std::uniform_real_distribution<double> U(-1., 1.);
-  std::vector<uint32_t> code(n);
-  d2h(h->s, code.data(), L.code, n);
-  HIPCHECK(hipStreamSynchronize(h->s));
-  for (size_t i = 0; i < n; i++) v0[i] = (code[i] & ROW_BIT) ? T(U(rng)) : T(0);
-  h2d(h->s, L.xa, v0.data(), n);
+  launch_random_rows<T>(h->s, n, L.code, 1234u, L.xa);
   if (std::is_same<T, float>::value) launch_fillf(h->s, n, (float*)L.f, 0.f);
   else launch_fill(h->s, n, (double*)L.f, 0.);
   int nb = 0;
@@ -219,7 +211,7 @@ static void build_hierarchy(vfs_pp_handle* h, Arena& tmp) {
 
 // VolumeFlux face list (poisson.c:2670-2700 / 2936-3010) in (k, j, i, axis) order,
 // and the number of cells with nvert < 0.1
-static void build_ib_faces(vfs_pp_handle* h, const double* csi, const double* eta, const double* zet, Arena& tmp) {
+static void build_ib_faces(vfs_pp_handle* h, Arena& tmp) {
   hipStream_t s = h->s;
   const size_t nb = ibface_blocks(h->d);
   int* cnt = tmp.alloc<int>(3 * nb);
@@ -231,7 +223,7 @@ static void build_ib_faces(vfs_pp_handle* h, const double* csi, const double* et
   h->ibf = nullptr;
   if (h->nibf) {
     h->ibf = h->geo.alloc<IbFace>(h->nibf);
-    launch_ibface_write(s, h->m, h->d, h->nvert, csi, eta, zet, h->grid.immersed, cnt + 2 * nb, h->ibf);
+    launch_ibface_write(s, h->m, h->d, h->nvert, h->met9, h->grid.immersed, cnt + 2 * nb, h->ibf);
   }
 }
 
@@ -462,22 +454,41 @@ void ensure_geom(vfs_pp_handle* h) {
   if (!h->have_geom) throw StatusError(VFS_PP_ESTATE, "vfs_pp_set_geometry has not been called");
 }
 
-// Set-up from the 16 metric arrays on the device (dmet: icsi..kzet, iaj, jaj, kaj,
-// aj, csi, eta, zet) and nvert on the device.  Everything is built on the device;
-// the host only reads back a few statistics per level and inverts the coarsest
-// level (at most 1024 cells).
-static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const double* dnvert) {
-  const Dmda m = h->m;
-  const Dims d = h->d;
-  const size_t M = m.n(), n = d.n();
+// Everything that depends on nvert (device DMDA array): code words and masks, the
+// VolumeFlux face list and the multigrid hierarchy.  Built on the device; the host
+// only reads back a few statistics per level and inverts the coarsest level (at
+// most 1024 cells).  Replaces the previous mask's hierarchy and face list.
+static void set_mask_core(vfs_pp_handle* h, const double* dnvert) {
   hipStream_t s = h->s;
-  // a new geometry replaces the previous one's hierarchy and face list
   HIPCHECK(hipStreamSynchronize(s));
   h->have_geom = false;
   h->mg.reset();
   h->ibf = nullptr;
   h->nibf = 0;
   h->geo.release();
+  HIPCHECK(hipMemcpyAsync(h->nvert, dnvert, h->m.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
+  launch_codes(s, h->m, h->d, h->nvert, h->code, h->cls);
+  Arena tmp;
+  build_ib_faces(h, tmp);
+  build_hierarchy(h, tmp);
+  if (h->opts.precision == 1) h->mg = build_mg<float>(h);
+  else h->mg = build_mg<double>(h);
+  h->desc = h->mg->describe();
+  h->setup.clear();
+  HIPCHECK(hipStreamSynchronize(s));
+  h->have_geom = true;
+}
+
+// Set-up from the 16 metric arrays on the device (dmet: icsi..kzet, iaj, jaj, kaj,
+// aj, csi, eta, zet) and nvert on the device.
+static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const double* dnvert) {
+  const Dmda m = h->m;
+  const Dims d = h->d;
+  const size_t M = m.n(), n = d.n();
+  hipStream_t s = h->s;
+  HIPCHECK(hipStreamSynchronize(s));
+  h->have_geom = false;
+  h->have_metrics = false;
   // resident arrays
   if (!h->g64) {
     h->g64 = h->ar.alloc<double>(9 * n);
@@ -508,22 +519,13 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
     HIPCHECK(hipHostMalloc(&h->hpin, 4096 * sizeof(double)));
     HIPCHECK(hipHostGetDevicePointer((void**)&h->hpin_dev, h->hpin, 0));
   }
-  // face coefficients, branch codes, and the cell metrics and aj on interior cells
-  // (Contra2Cart, Divergence)
-  HIPCHECK(hipMemcpyAsync(h->nvert, dnvert, M * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // face coefficients, and the cell metrics and aj on interior cells (Contra2Cart,
+  // Divergence, VolumeFlux areas)
   launch_face_coeffs(s, m, d, dmet[0], dmet[1], dmet[2], dmet[3], dmet[4], dmet[5], dmet[6], dmet[7], dmet[8], dmet[9],
                      dmet[10], dmet[11], h->g64);
-  launch_codes(s, m, d, h->nvert, h->code, h->cls);
   launch_gather_cell_metrics(s, m, d, dmet[13], dmet[14], dmet[15], dmet[12], h->met9, h->aj);
-  Arena tmp;
-  build_ib_faces(h, dmet[13], dmet[14], dmet[15], tmp);
-  build_hierarchy(h, tmp);
-  if (h->opts.precision == 1) h->mg = build_mg<float>(h);
-  else h->mg = build_mg<double>(h);
-  h->desc = h->mg->describe();
-  h->setup.clear();
-  HIPCHECK(hipStreamSynchronize(s));
-  h->have_geom = true;
+  h->have_metrics = true;
+  set_mask_core(h, dnvert);
 }
 
 // ------------------------------------------------------------ pressure path
@@ -781,6 +783,15 @@ int vfs_pp_set_grid_device(vfs_pp_handle* h, const double* coor, const double* n
   for (int q = 0; q < 16; q++) dmet[q] = tmp.alloc<double>((q < 9 || q >= 13) ? 3 * M : M);
   launch_form_metrics(h->s, h->m, coor, dmet);
   set_geometry_core(h, dmet, nvert);
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_set_nvert_device(vfs_pp_handle* h, const double* nvert) {
+  ABI_BEGIN
+  if (!h || !nvert) return VFS_PP_EINVAL;
+  if (!h->have_metrics) return VFS_PP_ESTATE;
+  set_mask_core(h, nvert);
   return VFS_PP_OK;
   ABI_END
 }

@@ -220,7 +220,8 @@ struct vfs_pp_handle {
   hipStream_t s = nullptr;
   pp::Arena ar;   // buffers sized by the grid alone
   pp::Arena geo;  // buffers of the current geometry (hierarchy, face list), renewed by each set-up
-  bool have_geom = false;
+  bool have_geom = false;     // metrics, masks and hierarchy are in place
+  bool have_metrics = false;  // face coefficients and cell metrics are in place
   // fine-level resident data
   double* g64 = nullptr;
   uint32_t* code = nullptr;

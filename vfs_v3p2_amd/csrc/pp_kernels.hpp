@@ -55,14 +55,19 @@ void launch_coarsen(hipStream_t s, Dims df, const double* nvf, const double* gf,
                     Dims dc, double* nv_tmp, double* nvc, double* gc, uint32_t* codec, uint8_t* clsc);
 void launch_cls_from_code(hipStream_t s, size_t n, const uint32_t* code, uint8_t* cls);
 void launch_half_to_double(hipStream_t s, size_t n, const _Float16* a, double gs, double* out);
+// v = uniform [-1, 1) on the rows (code & ROW_BIT), 0 elsewhere; a counter-based
+// generator, so the vector depends on seed and n only
+template <typename T>
+void launch_random_rows(hipStream_t s, size_t n, const uint32_t* code, uint64_t seed, T* v);
 // VolumeFlux faces in (k, j, i, axis) order: counts per block of 256 interior cells
 // (cnt, lt: ibface_blocks(d) ints each), their exclusive prefix off, and
-// tot2 = {faces, cells with nvert < 0.1}; then the list itself.
+// tot2 = {faces, cells with nvert < 0.1}; then the list itself, with face areas
+// from the interior cells' csi, eta, zet (met9, planar: plane 3 v + q).
 size_t ibface_blocks(Dims d);
 void launch_ibface_count(hipStream_t s, Dmda m, Dims d, const double* nv, int immersed, int* cnt, int* lt, int* off,
                          double* tot2);
-void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* csi, const double* eta,
-                         const double* zet, int immersed, const int* off, IbFace* out);
+void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* met9, int immersed,
+                         const int* off, IbFace* out);
 
 // ---- FP64 operator ----------------------------------------------------------------
 // y = A x on interior vectors; x may be float or double.

@@ -213,6 +213,22 @@ __global__ __launch_bounds__(256) void k_half_to_double(size_t n, const _Float16
   if (i < n) out[i] = (double)a[i] * gs;
 }
 
+// Start vector of the power iteration for lambda_max(D^-1 A): uniform in [-1, 1) on
+// the rows, 0 elsewhere, from a counter-based generator (splitmix64 of seed + index,
+// Steele, Lea and Flood, OOPSLA 2014), so every cell draws independently.
+template <typename T>
+__global__ __launch_bounds__(256) void k_random_rows(size_t n, const uint32_t* __restrict__ code, uint64_t seed,
+                                                     T* __restrict__ v) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  uint64_t z = seed + (uint64_t)(c + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const double u = (double)(z >> 11) * 0x1.0p-53;
+  v[c] = (code[c] & ROW_BIT) ? T(2. * u - 1.) : T(0);
+}
+
 // ------------------------------------------------------------ VolumeFlux faces
 // The faces of DMDA point (I, J, K) that VolumeFlux corrects (poisson.c:2670-2700
 // for immersed == 1, 2936-3010 otherwise): bit dd of the result; bit dd of neg
@@ -299,9 +315,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(int nb, const int* __restr
   }
 }
 
+// met9: the cell metrics csi, eta, zet of the interior cells, plane 3 v + q holding
+// component q of metric v
 __global__ __launch_bounds__(256) void k_ibface_write(Dmda m, Dims d, const double* __restrict__ nv,
-                                                      const double* __restrict__ csi, const double* __restrict__ eta,
-                                                      const double* __restrict__ zet, int immersed,
+                                                      const double* __restrict__ met9, int immersed,
                                                       const int* __restrict__ off, IbFace* __restrict__ out) {
 #pragma clang fp contract(off)
   __shared__ int wsum[4];
@@ -324,14 +341,14 @@ __global__ __launch_bounds__(256) void k_ibface_write(Dmda m, Dims d, const doub
   int pos = off[blockIdx.x] + x - nf;
   for (int q = 0; q < w; q++) pos += wsum[q];
   if (!mask) return;
-  const size_t p = m.idx(I, J, K);
-  const double* met[3] = {csi, eta, zet};
+  const size_t p = m.idx(I, J, K), n = d.n();
   for (int dd = 0; dd < 3; dd++) {
     if (!((mask >> dd) & 1)) continue;
-    const double* mv = met[dd] + 3 * p;
+    const double m0 = met9[(size_t)(3 * dd) * n + c], m1 = met9[(size_t)(3 * dd + 1) * n + c],
+                 m2 = met9[(size_t)(3 * dd + 2) * n + c];
     IbFace f;
     f.pos = (uint64_t)(3 * p + dd);
-    f.area = sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+    f.area = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
     f.sign = ((neg >> dd) & 1) ? -1.f : 1.f;
     f.pad = 0.f;
     out[pos++] = f;
@@ -387,6 +404,14 @@ void launch_half_to_double(hipStream_t s, size_t n, const _Float16* a, double gs
   HIPCHECK(hipGetLastError());
 }
 
+template <typename T>
+void launch_random_rows(hipStream_t s, size_t n, const uint32_t* code, uint64_t seed, T* v) {
+  k_random_rows<T><<<lin_blocks(n), 256, 0, s>>>(n, code, seed, v);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_random_rows<float>(hipStream_t, size_t, const uint32_t*, uint64_t, float*);
+template void launch_random_rows<double>(hipStream_t, size_t, const uint32_t*, uint64_t, double*);
+
 size_t ibface_blocks(Dims d) { return lin_blocks(d.n()); }
 
 void launch_ibface_count(hipStream_t s, Dmda m, Dims d, const double* nv, int immersed, int* cnt, int* lt, int* off,
@@ -398,9 +423,9 @@ void launch_ibface_count(hipStream_t s, Dmda m, Dims d, const double* nv, int im
   HIPCHECK(hipGetLastError());
 }
 
-void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* csi, const double* eta,
-                         const double* zet, int immersed, const int* off, IbFace* out) {
-  k_ibface_write<<<lin_blocks(d.n()), 256, 0, s>>>(m, d, nv, csi, eta, zet, immersed, off, out);
+void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* met9, int immersed,
+                         const int* off, IbFace* out) {
+  k_ibface_write<<<lin_blocks(d.n()), 256, 0, s>>>(m, d, nv, met9, immersed, off, out);
   HIPCHECK(hipGetLastError());
 }
 

@@ -106,6 +106,7 @@ def lib():
         "vfs_pp_export_stencil": (C.c_int, [H, dp, u8]),
         "vfs_pp_export_level": (C.c_int, [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           dp, u8]),
+        "vfs_pp_set_nvert_device": (C.c_int, [H, vp]),
         "vfs_pp_export_ib_faces": (C.c_int, [H, C.c_size_t, C.POINTER(C.c_longlong), dp, C.POINTER(C.c_float),
                                              C.POINTER(C.c_size_t)]),
         "vfs_pp_classify_nvert_device": (C.c_int, [C.POINTER(Grid), vp]),
@@ -239,6 +240,11 @@ class Handle:
         set-up of vfs_pp_set_geometry, from device tensors."""
         _check(lib().vfs_pp_set_grid_device(self._h, self._v(coor, "coor"), self._s(nvert, "nvert")),
                "vfs_pp_set_grid_device")
+
+    def set_nvert_device(self, nvert):
+        """A new IB mask (device tensor, DMDA shape) on the same grid: codes, face
+        list and multigrid hierarchy are rebuilt, the metrics kept."""
+        _check(lib().vfs_pp_set_nvert_device(self._h, self._s(nvert, "nvert")), "vfs_pp_set_nvert_device")
 
     def _s(self, t, what):
         return _tptr(t, dmda_shape(self.grid), what)
