@@ -117,9 +117,11 @@ double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// power iteration for lambda_max of M^-1 A on one level
+// Power iteration for lambda_max(M^-1 A) on one level: 12 steps from a random
+// start, the norm after step `it` going to host_norms2[it] (mapped host memory);
+// the caller reads them after synchronising the stream.
 template <typename T>
-static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& L) {
+static void queue_power_iteration(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& L, double* host_norms2) {
   const size_t n = L.d.n();
   launch_random_rows<T>(h->s, n, L.code, 1234u, L.xa);
   if (std::is_same<T, float>::value) launch_fillf(h->s, n, (float*)L.f, 0.f);
@@ -128,17 +130,19 @@ static double estimate_lambda(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::Lev& 
   launch_normsq<T>(h->s, n, L.xa, h->partial, &nb);
   launch_finalize(h->s, h->partial, 1, nb, h->red, 0);
   launch_scale_by<T>(h->s, n, L.xa, h->red);
-  double lam = 0., best = 0.;
   for (int it = 0; it < 12; it++) {
     launch_smooth<T>(h->s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, mg.scr, T(0), T(1), T(0));
     launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
-    launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
-    HIPCHECK(hipStreamSynchronize(h->s));
-    lam = std::sqrt(h->hpin[0]);
-    if (it >= 6) best = std::max(best, lam);
+    launch_finalize(h->s, h->partial, 1, nb, h->red, 0, host_norms2 + it);
     HIPCHECK(hipMemcpyAsync(L.xa, L.dv, n * sizeof(T), hipMemcpyDeviceToDevice, h->s));
     launch_scale_by<T>(h->s, n, L.xa, h->red);
   }
+}
+
+// the estimate from the norms of steps 6..11
+static double power_estimate(const double* norms2) {
+  double best = 0.;
+  for (int it = 6; it < 12; it++) best = std::max(best, std::sqrt(norms2[it]));
   return best > 0. ? best : 2.;
 }
 
@@ -268,10 +272,14 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     }
     mg->lev.push_back(L);
   }
-  // Chebyshev parameters per smoothed level
+  // Chebyshev parameters per smoothed level: the power iterations of all levels are
+  // queued together and read back once
+  for (size_t l = 0; l + 1 < mg->lev.size(); l++)
+    queue_power_iteration<T>(h, *mg, mg->lev[l], h->hpin_dev + RED_LAMBDA + 12 * l);
+  HIPCHECK(hipStreamSynchronize(h->s));
   for (size_t l = 0; l + 1 < mg->lev.size(); l++) {
     auto& L = mg->lev[l];
-    L.lam = estimate_lambda<T>(h, *mg, L);
+    L.lam = power_estimate(h->hpin + RED_LAMBDA + 12 * l);
     MG<T>::cheb(L.lam, mg->degree, h->opts.cheb_lo, h->opts.cheb_hi, L.c1, L.c2);
   }
   // coarsest dense inverse
@@ -457,18 +465,21 @@ void ensure_geom(vfs_pp_handle* h) {
 // Everything that depends on nvert (device DMDA array): code words and masks, the
 // VolumeFlux face list and the multigrid hierarchy.  Built on the device; the host
 // only reads back a few statistics per level and inverts the coarsest level (at
-// most 1024 cells).  Replaces the previous mask's hierarchy and face list.
-static void set_mask_core(vfs_pp_handle* h, const double* dnvert) {
+// most 1024 cells).  Replaces the previous mask's hierarchy and face list, reusing
+// their buffers where the sizes repeat; keep_scratch keeps the set-up's scratch
+// buffers for the next call (a body that moves every step).
+static void set_mask_core(vfs_pp_handle* h, const double* dnvert, bool keep_scratch) {
   hipStream_t s = h->s;
   HIPCHECK(hipStreamSynchronize(s));
   h->have_geom = false;
   h->mg.reset();
   h->ibf = nullptr;
   h->nibf = 0;
-  h->geo.release();
+  h->geo.recycle();
+  Arena& tmp = h->setup_tmp;
+  tmp.recycle();
   HIPCHECK(hipMemcpyAsync(h->nvert, dnvert, h->m.n() * sizeof(double), hipMemcpyDeviceToDevice, s));
   launch_codes(s, h->m, h->d, h->nvert, h->code, h->cls);
-  Arena tmp;
   build_ib_faces(h, tmp);
   build_hierarchy(h, tmp);
   if (h->opts.precision == 1) h->mg = build_mg<float>(h);
@@ -476,6 +487,13 @@ static void set_mask_core(vfs_pp_handle* h, const double* dnvert) {
   h->desc = h->mg->describe();
   h->setup.clear();
   HIPCHECK(hipStreamSynchronize(s));
+  h->geo.trim();
+  if (keep_scratch) {
+    tmp.trim();
+    tmp.recycle();
+  } else {
+    tmp.release();
+  }
   h->have_geom = true;
 }
 
@@ -525,7 +543,7 @@ static void set_geometry_core(vfs_pp_handle* h, const double* const* dmet, const
                      dmet[10], dmet[11], h->g64);
   launch_gather_cell_metrics(s, m, d, dmet[13], dmet[14], dmet[15], dmet[12], h->met9, h->aj);
   h->have_metrics = true;
-  set_mask_core(h, dnvert);
+  set_mask_core(h, dnvert, false);
 }
 
 // ------------------------------------------------------------ pressure path
@@ -791,7 +809,7 @@ int vfs_pp_set_nvert_device(vfs_pp_handle* h, const double* nvert) {
   ABI_BEGIN
   if (!h || !nvert) return VFS_PP_EINVAL;
   if (!h->have_metrics) return VFS_PP_ESTATE;
-  set_mask_core(h, nvert);
+  set_mask_core(h, nvert, true);
   return VFS_PP_OK;
   ABI_END
 }

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -26,32 +27,56 @@ class Arena {
   T* alloc(size_t n) {
     void* p = nullptr;
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    auto it = pool_.find(bytes);
+    if (it != pool_.end()) {
+      p = it->second;
+      pool_.erase(it);
+    } else {
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw StatusError(VFS_PP_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+      }
     }
     // every buffer starts at zero so no result can depend on earlier contents of
     // the device heap; the memset goes to the null stream and is waited for here
-    e = hipMemsetAsync(p, 0, bytes, nullptr);
+    hipError_t e = hipMemsetAsync(p, 0, bytes, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
       (void)hipFree(p);
       throw StatusError(VFS_PP_EHIP, "zeroing a new buffer failed");
     }
-    ptrs_.push_back(p);
+    ptrs_.push_back({p, bytes});
     total_ += bytes;
     return static_cast<T*>(p);
   }
+  // hand every buffer back for reuse by later allocations of the same size (the
+  // caller no longer uses them)
+  void recycle() {
+    for (const Buf& b : ptrs_) pool_.emplace(b.bytes, b.p);
+    ptrs_.clear();
+    total_ = 0;
+  }
+  // free the recycled buffers that no allocation has taken back
+  void trim() {
+    for (auto& e : pool_) (void)hipFree(e.second);
+    pool_.clear();
+  }
   void release() {
-    for (void* p : ptrs_) (void)hipFree(p);
+    trim();
+    for (const Buf& b : ptrs_) (void)hipFree(b.p);
     ptrs_.clear();
     total_ = 0;
   }
   size_t total() const { return total_; }
 
  private:
-  std::vector<void*> ptrs_;
+  struct Buf {
+    void* p;
+    size_t bytes;
+  };
+  std::vector<Buf> ptrs_;
+  std::multimap<size_t, void*> pool_;
   size_t total_ = 0;
 };
 
@@ -220,6 +245,7 @@ struct vfs_pp_handle {
   hipStream_t s = nullptr;
   pp::Arena ar;   // buffers sized by the grid alone
   pp::Arena geo;  // buffers of the current geometry (hierarchy, face list), renewed by each set-up
+  pp::Arena setup_tmp;  // set-up scratch, kept between set-ups of a moving body
   bool have_geom = false;     // metrics, masks and hierarchy are in place
   bool have_metrics = false;  // face coefficients and cell metrics are in place
   // fine-level resident data
@@ -263,6 +289,7 @@ constexpr int RED_B2 = 320;    // ||b||^2
 constexpr int RED_DIV = 330;   // max |aj div u|
 constexpr int RED_DIVEST = 340;  // max |aj r| at the last true-residual check
 constexpr int RED_SETUP = 350;   // set-up statistics: 4 values
+constexpr int RED_LAMBDA = 1024;  // set-up power iterations: 12 norms per level (hpin only)
 constexpr int RED_NUINV2 = 400;  // 1 / |U_l|^2 of the lazily normalised basis (up to 64)
 
 // engine functions shared with the batched engine (pp_engine.hip)
