@@ -340,3 +340,135 @@ def test_linear_potential_is_harmonic_on_sheared_grid():
     y2 = O.stencil_apply(A.g, A.vol, A.rowflag, phi2)[3:-3, 3:-3, 3:-3]
     vol = 1.0 / A.case.fields["aj"][3:-3, 3:-3, 3:-3]
     np.testing.assert_allclose(y2, 6.0 * vol, rtol=1e-11)
+
+
+# ------------------------------------------ multigrid set-up of the CPU port
+# (the engine's hierarchy is checked level by level against these functions)
+
+def _rectilinear(x, y, z, per=(0, 0, 0)):
+    """(grid, DMDA face coefficients, case fields) of the grid with nodes x_i, y_j, z_k."""
+    N = np.stack(np.meshgrid(x, y, z, indexing="ij"), -1).transpose(2, 1, 0, 3).copy()
+    return _from_nodes(N, per)
+
+
+def _from_nodes(N, per=(0, 0, 0)):
+    co = cases.dmda_coords(N)
+    f = cases.form_metrics(co, per)
+    g = O.grid(f["aj"].shape, per)
+    return g, O.face_coeffs(g, f), f
+
+
+def _sheared_nodes(nx, ny, nz):
+    M = np.array([[1.0, 0.35, -0.2], [0.15, 0.8, 0.1], [-0.25, 0.3, 1.2]])
+    P = np.stack(np.meshgrid(np.arange(nx + 1) / nx, np.arange(ny + 1) / ny, np.arange(nz + 1) / nz,
+                             indexing="ij"), -1).transpose(2, 1, 0, 3)
+    return P @ M.T
+
+
+@pytest.mark.parametrize("keep_y", [False, True])
+def test_level_coarsening_rediscretises_a_sheared_grid(keep_y):
+    """On an affine grid every metric is constant, so the coarse face coefficients
+    must equal the ones FormMetrics gives on the coarse grid itself (every other
+    node; j kept under semi-coarsening): pins which fine faces each of the nine
+    coarse coefficients sums and its division by the coarsening factor of its
+    difference direction."""
+    N = _sheared_nodes(16, 12, 8)
+    g, g9, _ = _from_nodes(N)
+    gc, g9c, nvc, cm = O.level_coarsen(g, g9, np.zeros((g.mz, g.my, g.mx)), keep_y=keep_y)
+    assert cm == (5 if keep_y else 7)
+    Nc = N[::2, :, ::2] if keep_y else N[::2, ::2, ::2]
+    gr, g9r, _ = _from_nodes(np.ascontiguousarray(Nc))
+    assert (gc.mx, gc.my, gc.mz) == (gr.mx, gr.my, gr.mz)
+    assert np.abs(g9r[[1, 2, 3, 5, 6, 7]][:, 1:-1, 1:-1, 1:-1]).min() > 1e-3
+    # every interior cell's three upper faces except those on the far boundary planes
+    a = g9c[:, 1:-2, 1:-2, 1:-2]
+    b = g9r[:, 1:-2, 1:-2, 1:-2]
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max())
+    assert np.all(nvc == 0.0)
+
+
+def test_level_coarsening_is_the_scaled_galerkin_operator():
+    """Rectilinear stretched grid (cross terms vanish) with odd sizes and an IB
+    block: for piecewise-constant transfers R (sum over children) and P = R^T, the
+    coupling of coarse cells I and I + e_d in the dense product R A P equals the
+    coarse face coefficient times the coarsening factor of d.  Pins the children of
+    every coarse cell, the one-child cells at odd ends, the fine faces that make up
+    each coarse face (the last child's upper faces) and the active-face mask."""
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 9, 8, 7
+    x, y, z = (np.concatenate([[0.0], np.cumsum(0.5 + rng.random(n))]) for n in (nx, ny, nz))
+    g, g9, _ = _rectilinear(x, y, z)
+    nv = np.zeros((g.mz, g.my, g.mx))
+    nv[2:5, 3:6, 4:8] = 1.0
+    nv[6, 2, 2] = 1.0
+    vol, rowflag, _ = O.stencil(g, g9, nv)
+    n = nx * ny * nz
+    A = np.zeros((n, n))
+    off = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if not rowflag[k + 1, j + 1, i + 1]:
+                    continue
+                p = (k * ny + j) * nx + i
+                for t, (di, dj, dk) in enumerate(off):
+                    v = vol[k + 1, j + 1, i + 1, t]
+                    if v != 0.0:
+                        A[p, ((k + dk) * ny + (j + dj)) * nx + (i + di)] += v
+    for keep_y in (False, True):
+        gc, g9c, _, cm = O.level_coarsen(g, g9, nv, keep_y=keep_y)
+        ncx, ncy, ncz = gc.mx - 2, gc.my - 2, gc.mz - 2
+        R = np.zeros((ncx * ncy * ncz, n))
+        for k in range(nz):
+            for j in range(ny):
+                for i in range(nx):
+                    I, J, K = (i // 2 if cm & 1 else i), (j // 2 if cm & 2 else j), (k // 2 if cm & 4 else k)
+                    R[(K * ncy + J) * ncx + I, (k * ny + j) * nx + i] = 1.0
+        Ac = R @ A @ R.T
+        checked = 0
+        for d, (di, dj, dk) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+            fac = 2.0 if cm & (1 << d) else 1.0
+            for K in range(ncz - dk):
+                for J in range(ncy - dj):
+                    for I in range(ncx - di):
+                        want = Ac[(K * ncy + J) * ncx + I, ((K + dk) * ncy + J + dj) * ncx + I + di]
+                        got = fac * g9c[4 * d, K + 1, J + 1, I + 1]
+                        assert abs(got - want) <= 1e-13 * abs(want) + 1e-300, (keep_y, d, I, J, K, got, want)
+                        checked += want != 0.0
+        assert checked > 100
+
+
+def test_level_coarsening_nvert_rules():
+    """Coarse nvert: blocked when every child is (one fluid child keeps it fluid),
+    a fluid cell blocked on both sides along every axis becomes blocked, and a cell
+    left without any active face (here against a wall) becomes blocked."""
+    x = np.arange(9) * 1.0
+    g, g9, _ = _rectilinear(x, x, x)
+
+    def block(nv, K, J, I):  # every child of coarse interior cell (I, J, K)
+        nv[2 * K + 1:2 * K + 3, 2 * J + 1:2 * J + 3, 2 * I + 1:2 * I + 3] = 1.0
+
+    nv = np.zeros((g.mz, g.my, g.mx))
+    block(nv, 1, 1, 1)
+    block(nv, 1, 1, 2)
+    nv[3, 3, 5] = 0.0  # one fluid child of coarse (2, 1, 1)
+    _, _, nvc, _ = O.level_coarsen(g, g9, nv)
+    assert nvc[2, 2, 2] == 1.0 and nvc[2, 2, 3] == 0.0
+    # isolated interior cell (2, 2, 2)
+    nv = np.zeros((g.mz, g.my, g.mx))
+    for K, J, I in [(1, 2, 2), (3, 2, 2), (2, 1, 2), (2, 3, 2), (2, 2, 1), (2, 2, 3)]:
+        block(nv, K, J, I)
+    _, _, nvc, _ = O.level_coarsen(g, g9, nv)
+    assert nvc[3, 3, 3] == 1.0 and nvc[3, 3, 2] == 1.0
+    # cell (0, 1, 1) against the i = 0 wall: blocked on its other five sides; the
+    # pair rule does not see it (the wall side is not blocked), the row rule does
+    nv = np.zeros((g.mz, g.my, g.mx))
+    for K, J, I in [(0, 1, 0), (2, 1, 0), (1, 0, 0), (1, 2, 0), (1, 1, 1)]:
+        block(nv, K, J, I)
+    assert np.all(nv[3:5, 3:5, 1:3] == 0.0)  # its own children are fluid
+    _, _, nvc, _ = O.level_coarsen(g, g9, nv)
+    assert nvc[2, 2, 1] == 1.0
+    # the same cell with its +i neighbour fluid keeps an active face and stays fluid
+    nv[3:5, 3:5, 3:5] = 0.0
+    _, _, nvc, _ = O.level_coarsen(g, g9, nv)
+    assert nvc[2, 2, 1] == 0.0 and nvc[2, 2, 2] == 0.0
