@@ -101,7 +101,8 @@ struct SmoothItem {
   const T* dinv;
   const T* xin;
   T* xout;
-  T* dv;
+  T* xprev;  // point levels: the previous iterate (read and/or written; may be xout)
+  T* dv;     // y-line levels: the Chebyshev direction
   T* out_base;  // non-null: the output is out_base + j * out_stride (the case's Z_j)
   size_t out_stride;
   const T* lfac;  // y-line levels: Thomas factors; scr holds the residual and the forward sweep
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
     int i, j, k;
     if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
-    smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
+    smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, I.xprev, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
   }
 }
 
@@ -814,6 +815,7 @@ struct BatchCore {
       it.dinv = L.dinv;
       it.xin = xin;
       it.xout = xout;
+      it.xprev = (mode == SM_NEXT) ? xout : xin;
       it.dv = L.dv;
       it.out_base = fin ? static_cast<T*>(slots[q].h->Z) : nullptr;
       it.out_stride = L.d.n();

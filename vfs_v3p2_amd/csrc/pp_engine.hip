@@ -131,7 +131,8 @@ static void queue_power_iteration(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::L
   launch_finalize(h->s, h->partial, 1, nb, h->red, 0);
   launch_scale_by<T>(h->s, n, L.xa, h->red);
   for (int it = 0; it < 12; it++) {
-    launch_smooth<T>(h->s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, mg.scr, T(0), T(1), T(0));
+    launch_smooth<T>(h->s, L.view(), L.lfac, SM_POWER, L.f, L.dinv, L.xa, L.xb, L.xb, L.dv, mg.scr, T(0), T(1),
+                     T(0));
     launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
     launch_finalize(h->s, h->partial, 1, nb, h->red, 0, host_norms2 + it);
     HIPCHECK(hipMemcpyAsync(L.xa, L.dv, n * sizeof(T), hipMemcpyDeviceToDevice, h->s));

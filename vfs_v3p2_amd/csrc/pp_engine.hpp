@@ -173,10 +173,15 @@ struct MG : MGBase {
       }
       const int last = k + adv - 1;  // the last step this launch performs
       T* xin = x;
-      T* xout = (x == L.xa) ? L.xb : L.xa;
+      T* other = (x == L.xa) ? L.xb : L.xa;
+      T* xout = other;
       if (last == degree - 1 && final_out) xout = final_out;
+      // point levels: the previous iterate lives in the other buffer (which the next
+      // iterate overwrites); a step from zero leaves its x_prev where x was
+      T* xprev = (mode == SM_NEXT) ? other : xin;
       const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
-      launch_smooth<T>(s, L.view(), L.lfac, mode, L.f, L.dinv, xin, xout, L.dv, scr, c1, L.c2[last], L.c2[0]);
+      launch_smooth<T>(s, L.view(), L.lfac, mode, L.f, L.dinv, xin, xprev, xout, L.dv, scr, c1, L.c2[last],
+                       L.c2[0]);
       x = xout;
       k += adv;
     }
@@ -230,7 +235,7 @@ struct MG : MGBase {
 
   void time_sweep(hipStream_t s) override {
     Lev& L = lev[0];
-    launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.dv, scr,
+    launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.xb, L.dv, scr,
                      L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
   }
 };

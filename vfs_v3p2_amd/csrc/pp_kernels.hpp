@@ -15,6 +15,7 @@ constexpr uint8_t CLS_PROJ_I = 8;     // Projection corrects this cell's i+1/2 f
 constexpr uint8_t CLS_PROJ_J = 16;    // ... j+1/2 face
 constexpr uint8_t CLS_PROJ_K = 32;    // ... k+1/2 face
 constexpr uint8_t CLS_NONFLUID = 64;  // nvert > 0.1 (multigrid masks: MyInterpolation zeroes, restriction weight)
+constexpr uint8_t CLS_INTERIOR = 128;  // multigrid level masks only: the cell's code word is the interior one
 
 // DMDA description used by kernels touching caller arrays.
 struct Dmda {
@@ -50,9 +51,10 @@ void launch_level_stats(hipStream_t s, Dims d, const double* g, const uint32_t* 
 size_t level_stats_partials();
 // One coarsening step (axes in cm) from level (df, DMDA nvert nvf, face coefficients
 // gf, codes codef) to dc: coarse DMDA nvert nvc (nv_tmp: a second DMDA buffer),
-// face coefficients gc, codes and CLS_NONFLUID mask.
+// face coefficients gc, codes and class bytes (CLS_NONFLUID, CLS_INTERIOR).
 void launch_coarsen(hipStream_t s, Dims df, const double* nvf, const double* gf, const uint32_t* codef, int cm,
                     Dims dc, double* nv_tmp, double* nvc, double* gc, uint32_t* codec, uint8_t* clsc);
+// the finest level's multigrid class bytes (CLS_NONFLUID, CLS_INTERIOR) from its codes
 void launch_cls_from_code(hipStream_t s, size_t n, const uint32_t* code, uint8_t* cls);
 void launch_half_to_double(hipStream_t s, size_t n, const _Float16* a, double gs, double* out);
 // v = uniform [-1, 1) on the rows (code & ROW_BIT), 0 elsewhere; a counter-based
@@ -163,16 +165,20 @@ struct LevelView {
   const T* g;
   const _Float16* gh;
   const uint32_t* code;
-  const uint8_t* cls;  // CLS_NONFLUID used by the transfers
+  const uint8_t* cls;  // CLS_NONFLUID (transfers) and CLS_INTERIOR (point smoother)
   T gs;
 };
 
-// Chebyshev smoothing launch modes (see k_smooth_point)
-enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2 };
+// Chebyshev smoothing launch modes (see smooth_point_cell); SM_POWER is one step
+// with c1 = 0, c2 = 1 from f = 0 that stores D^-1 (f - A x) in dv (power iteration)
+enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2, SM_POWER = 3 };
 
+// Point levels keep the previous iterate in x_prev (x_new may be the same buffer:
+// each cell reads its own x_prev before writing x_new); y-line levels keep the
+// Chebyshev direction in dv and ignore x_prev.
 template <typename T>
 void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const T* f, const T* dinv,
-                   const T* x_old, T* x_new, T* dv, T* scr, T c1, T c2, T c0);
+                   const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0);
 template <typename T>
 void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac);
 template <typename T>

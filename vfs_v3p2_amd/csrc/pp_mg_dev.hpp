@@ -49,27 +49,53 @@ __device__ __forceinline__ void dinv_cell(const LevelView<T>& L, T* __restrict__
   dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
 }
 
-// Chebyshev smoothing preconditioned by the diagonal.  One step is
-//   z = D^-1 (f - A x_old);  d = c1 d + c2 z;  x_new = x_old + d
-// (c1 == 0 starts a new sequence and does not read d).  Modes:
+// (A x)_c and the diagonal with the cross-difference cases fixed at compile time
+// (KC == X_CENTRAL: an interior row) or decoded from cd (KC < 0)
+template <int KC, typename T>
+__device__ __forceinline__ void level_flux_k(const LevelView<T>& L, uint32_t cd, size_t c, const Nbr& nb,
+                                             const T (&xv)[19], T& ax, T& diag) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.gh) {
+      flux_row_k<KC>(L.gh, L.d.n(), cd, (uint32_t)c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+      ax *= L.gs;
+      diag *= L.gs;
+      return;
+    }
+  }
+  flux_row_k<KC>(L.g, L.d.n(), cd, (uint32_t)c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
+}
+
+// Chebyshev smoothing preconditioned by the diagonal, in three-term form: one step is
+//   z = D^-1 (f - A x);  x_new = x + c1 (x - x_prev) + c2 z
+// (the direction d = x - x_prev is not stored; c1 == 0 starts a new sequence and
+// does not read x_prev).  x_new may be x_prev's buffer.  Modes:
 //   SM_NEXT    one step from x_old;
-//   SM_ZERO    the first step from x = 0:  x_new = d = c2 D^-1 f;
+//   SM_ZERO    the first step from x = 0:  x_new = c2 D^-1 f, x_prev = 0;
 //   SM_FIRST2  the first two steps from x = 0 in one pass: the first step's
-//              x1 = d1 = c0 D^-1 f is formed on the fly at the 19 neighbours from
-//              f and D^-1, then one step from x1 with (c1, c2).
+//              x1 = c0 D^-1 f is formed on the fly at the 19 neighbours from f and
+//              D^-1, then one step from x1 with (c1, c2); x_new = x2, x_prev = x1;
+//   SM_POWER   dv = D^-1 (f - A x_old) only.
+// The level's class bytes pick the row form: a wave whose cells are all interior
+// takes the compile-time-weight form and reads no code word.
 template <typename T, int MODE>
 __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T* __restrict__ f,
                                                   const T* __restrict__ dinv, const T* __restrict__ x_old,
-                                                  T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
+                                                  T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
                                                   int j, int k) {
   const size_t c = L.d.idx(i, j, k);
   if constexpr (MODE == SM_ZERO) {
-    const T v = c2 * (f[c] * dinv[c]);
-    x_new[c] = v;
-    dv[c] = v;
+    x_new[c] = c2 * (f[c] * dinv[c]);
+    x_prev[c] = T(0);
     return;
   }
-  const uint32_t cd = L.code[c];
+  const bool interior = L.cls[c] & CLS_INTERIOR;
+  // the cell's own operands are loaded up front with the stencil, so their latency
+  // hides behind the stencil's
+  const T fc = f[c];
+  T xp = T(0);
+  if constexpr (MODE == SM_NEXT) {
+    if (c1 != T(0)) xp = x_prev[c];
+  }
   Nbr nb;
   neighbours(L.d, i, j, k, nb);
   T xv[19];
@@ -80,14 +106,26 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
     load_x(nb, x_old, xv);
   }
   T ax, diag;
-  level_flux(L, cd, c, nb, xv, ax, diag);
-  const T fc = f[c];
-  const T dold = (MODE == SM_FIRST2) ? xv[CP] : (c1 == T(0) ? T(0) : dv[c]);
-  if (!(cd & ROW_BIT)) { x_new[c] = T(0); dv[c] = T(0); return; }
-  const T z = diag != T(0) ? (fc - ax) / diag : T(0);
-  const T dn = c1 * dold + c2 * z;
-  dv[c] = dn;
-  x_new[c] = xv[CP] + dn;
+  bool row = true;
+  if (__all(interior)) {
+    level_flux_k<X_CENTRAL>(L, CODE_INTERIOR, c, nb, xv, ax, diag);
+  } else {
+    const uint32_t cd = L.code[c];
+    level_flux_k<-1>(L, cd, c, nb, xv, ax, diag);
+    row = (cd & ROW_BIT) != 0;
+  }
+  // straight-line division with a safe divisor: a branch around it would let the
+  // compiler move the load of f into the branch, after the stencil
+  const bool live = row && diag != T(0);
+  const T z = live ? (fc - ax) / (live ? diag : T(1)) : T(0);
+  if constexpr (MODE == SM_POWER) {
+    dv[c] = z;
+    return;
+  }
+  const T xc = xv[CP];
+  const T dold = (MODE == SM_FIRST2) ? xc : (c1 == T(0) ? T(0) : xc - xp);
+  if constexpr (MODE == SM_FIRST2) x_prev[c] = row ? xc : T(0);
+  x_new[c] = row ? xc + (c1 * dold + c2 * z) : T(0);
 }
 
 // Thomas factors of the y-line tridiagonal part of A on the line (i, k), from the

@@ -175,14 +175,20 @@ __global__ __launch_bounds__(256) void k_coarse_faces(Dims df, Dims dc, int cm, 
   }
 }
 
-// code words of a coarse level; cls marks nvert > 0.1 only
+// the multigrid class byte of a cell: CLS_INTERIOR for the interior code word
+__device__ inline uint8_t interior_bit(uint32_t cd) {
+  return (cd & ~DIVOK_BIT) == CODE_INTERIOR ? CLS_INTERIOR : 0;
+}
+
+// code words of a coarse level; cls marks nvert > 0.1 and interior cells
 __global__ __launch_bounds__(256) void k_level_codes(Dmda m, Dims d, const double* __restrict__ nv,
                                                      uint32_t* __restrict__ code, uint8_t* __restrict__ cls) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= d.nx || j >= d.ny) return;
   const size_t c = d.idx(i, j, k);
-  code[c] = make_code(nv, m.mx, m.my, m.mz, m.per, i + 1, j + 1, k + 1);
-  cls[c] = nv[m.idx(i + 1, j + 1, k + 1)] > 0.1 ? CLS_NONFLUID : 0;
+  const uint32_t cd = make_code(nv, m.mx, m.my, m.mz, m.per, i + 1, j + 1, k + 1);
+  code[c] = cd;
+  cls[c] = (nv[m.idx(i + 1, j + 1, k + 1)] > 0.1 ? CLS_NONFLUID : 0) | interior_bit(cd);
 }
 
 // rows whose diagonal vanishes (no active face left) become solid
@@ -204,7 +210,9 @@ __global__ __launch_bounds__(256) void k_dead_rows(Dmda m, Dims d, const double*
 __global__ __launch_bounds__(256) void k_cls_from_code(size_t n, const uint32_t* __restrict__ code,
                                                        uint8_t* __restrict__ cls) {
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c < n) cls[c] = (code[c] & ROW_BIT) ? 0 : CLS_NONFLUID;
+  if (c >= n) return;
+  const uint32_t cd = code[c];
+  cls[c] = ((cd & ROW_BIT) ? 0 : CLS_NONFLUID) | interior_bit(cd);
 }
 
 __global__ __launch_bounds__(256) void k_half_to_double(size_t n, const _Float16* __restrict__ a, double gs,
