@@ -205,8 +205,10 @@ HDI void flux_term(T& nrm, T& crs, T& diag, T gv, uint32_t code, const T (&xv)[1
     constexpr int sQ_lo = slot_of(qd[0] - ee[0], qd[1] - ee[1], qd[2] - ee[2]);
     if constexpr (KC == X_CENTRAL) {
       // the central case's addends: +0.25 g on the upper pair, -0.25 g on the
-      // lower pair, nothing on (c, q) or the diagonal
-      crs += gv * (T(0.25) * (xv[sC_hi] + xv[sQ_hi]) - T(0.25) * (xv[sC_lo] + xv[sQ_lo]));
+      // lower pair, nothing on (c, q) or the diagonal.  The 0.25 is applied once
+      // per face (flux_face); scaling by a power of two commutes with every rounding,
+      // so the result is the same
+      crs += gv * ((xv[sC_hi] + xv[sQ_hi]) - (xv[sC_lo] + xv[sQ_lo]));
     } else {
       const int cc = (int)((code >> (6 + 4 * F + 2 * XI)) & 3u);
       const T whi = cc == X_CENTRAL ? T(0.25) : (cc == X_UPPER ? T(0.5) : T(0));
@@ -245,6 +247,7 @@ __device__ __forceinline__ void flux_face(T& ax, T& diag, const TG* __restrict__
   flux_term<D, SGN, 0, KC>(nrm, crs, dg, g0, code, xv);
   flux_term<D, SGN, 1, KC>(nrm, crs, dg, g1, code, xv);
   flux_term<D, SGN, 2, KC>(nrm, crs, dg, g2, code, xv);
+  if constexpr (KC == X_CENTRAL) crs *= T(0.25);
   if (KC == X_CENTRAL || ((code >> F) & 1u)) {
     ax += nrm + (SGN > 0 ? crs : -crs);
     diag += dg;
