@@ -472,3 +472,40 @@ def test_level_coarsening_nvert_rules():
     nv[3:5, 3:5, 3:5] = 0.0
     _, _, nvc, _ = O.level_coarsen(g, g9, nv)
     assert nvc[2, 2, 1] == 0.0 and nvc[2, 2, 2] == 0.0
+
+
+def test_linear_potential_next_to_an_ib_block():
+    """With an IB block inside the sheared affine grid, every difference the rows use
+    is exact for phi = a.x: the one-sided cross differences next to the block equal
+    the central ones.  A fluid row then misses exactly the fluxes of its inactive
+    faces, each F_d = sum_e g_de (a . J_e) (J_e the constant step between cell
+    centres along e): (A phi)_c = sum_d F_d (act(d+) - act(d-)).  Every value off the
+    fluid rows is noise, so no row may reach one.  Pins the weight, side and sign of
+    every one-sided case and the face masks."""
+    c, cent = affine_case(12, 11, 10)
+    nv = c.fields["nvert"]
+    nv[4:8, 3:7, 5:9] = 1.0
+    A = O.Assembled(c, drop=False)
+    cs = A.cases
+    rows = A.rowflag.astype(bool)
+    # one-sided cases occur, and no active face has lost a cross difference altogether
+    assert ((cs[..., 1::3] >= 2) & rows[..., None]).sum() > 50
+    a = np.array([0.7, -1.3, 0.4])
+    phi = np.zeros(c.shape)
+    phi[1:-1, 1:-1, 1:-1] = cent[1:-1, 1:-1, 1:-1] @ a
+    # values that no row may touch: a row that reached an IB cell or a ghost (a
+    # one-sided difference on the wrong side) would pick them up
+    rng = np.random.default_rng(4)
+    phi[~rows] = 1e3 * rng.standard_normal(int((~rows).sum()))
+    y = O.stencil_apply(A.g, A.vol, A.rowflag, phi)
+    J = [cent[5, 5, 6] - cent[5, 5, 5], cent[5, 6, 5] - cent[5, 5, 5], cent[6, 5, 5] - cent[5, 5, 5]]
+    g9 = A.g9[:, 5, 5, 5]
+    F = [sum(g9[3 * d + e] * (a @ J[e]) for e in range(3)) for d in range(3)]
+    act = cs[..., 0::3].astype(float)  # faces i+, i-, j+, j-, k+, k-
+    zero_cross = ((cs[..., 1::3] == 0) | (cs[..., 2::3] == 0)) & (cs[..., 0::3] == 1)
+    ok = rows & ~zero_cross.any(-1)
+    want = sum(F[d] * (act[..., 2 * d] - act[..., 2 * d + 1]) for d in range(3))
+    assert ok.sum() > 0.9 * rows.sum()
+    near = ok & (act.sum(-1) < 6)
+    assert near.sum() > 100
+    np.testing.assert_allclose(y[ok], want[ok], rtol=0, atol=1e-11 * max(abs(f) for f in F))
