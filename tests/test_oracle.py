@@ -167,6 +167,43 @@ def test_volume_flux_zeroes_net_ib_flux():
     assert abs(b.sum()) < 1e-11 * np.abs(b).sum()
 
 
+def test_volume_flux_hand_built_faces():
+    """Unit cube cells (every face area 1).  One near-boundary IB cell A (nvert 1) and
+    one solid cell B (nvert 3), apart; one of A's faces carries |u| <= 1e-10.  The
+    immersed == 1 variant (poisson.c:2898-3240) counts only A's faces (B's nvert is
+    not in (0.1, 1.1)) and drops and zeroes the tiny one; the immersed == 0 variant
+    (poisson.c:2634-2828) counts all twelve faces, fluid-to-IB with +u, IB-to-fluid
+    with -u, and zeroes every face it corrects."""
+    x = np.arange(7) * 1.0
+    g, _, f = _rectilinear(x, x, x)
+    f["nvert"] = np.zeros((8, 8, 8))
+    f["nvert"][3, 3, 3] = 1.0  # A
+    f["nvert"][3, 3, 5] = 3.0  # B
+    rng = np.random.default_rng(9)
+    u0 = rng.standard_normal((8, 8, 8, 3))
+    u0[3, 3, 3, 2] = 1e-12  # A's k+ face
+    # A's faces: i-, j-, k- belong to the fluid neighbours (fluid to IB, +u); i+, j+,
+    # k+ are A's own (IB to fluid, -u)
+    a_in = [u0[3, 3, 2, 0], u0[3, 2, 3, 1], u0[2, 3, 3, 2]]
+    a_out = [u0[3, 3, 3, 0], u0[3, 3, 3, 1], u0[3, 3, 3, 2]]
+    b_in = [u0[3, 3, 4, 0], u0[3, 2, 5, 1], u0[2, 3, 5, 2]]
+    b_out = [u0[3, 3, 5, 0], u0[3, 3, 5, 1], u0[3, 3, 5, 2]]
+    u = u0.copy()
+    out = O.volume_flux(g, 1, u, f)
+    assert out[1] == 5.0
+    assert abs(out[0] - (sum(a_in) - a_out[0] - a_out[1])) < 1e-14
+    assert u[3, 3, 3, 2] == 0.0  # the tiny face is zeroed
+    corr = out[0] / out[1]
+    np.testing.assert_allclose(u[3, 3, 2, 0], a_in[0] - corr, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(u[3, 3, 3, 0], a_out[0] + corr, rtol=0, atol=1e-14)
+    assert np.all(u[3, 3, 4] == u0[3, 3, 4]) and np.all(u[3, 3, 5] == u0[3, 3, 5])  # B untouched
+    u = u0.copy()
+    out = O.volume_flux(g, 0, u, f)
+    assert out[1] == 12.0
+    assert abs(out[0] - (sum(a_in) + sum(b_in) - sum(a_out) - sum(b_out))) < 1e-13
+    assert u[3, 3, 2, 0] == 0.0 and u[3, 3, 5, 1] == 0.0 and out[3] == 12.0 and out[2] == 0.0
+
+
 def test_contra2cart_uniform_flow():
     c = uniform_case(8, 6, 5)
     f = c.fields
