@@ -233,3 +233,24 @@ def test_repeated_set_up_keeps_device_memory(E, torch_cuda):
     h.close()
     # one hierarchy of this grid holds about 20 MB; eight leaked ones would show
     assert free0 - free1 < 8 << 20, (free0, free1)
+
+
+def test_new_mask_after_device_grid_set_up(E, torch_cuda):
+    """Metrics formed on the device (vfs_pp_set_grid_device), then a moved body
+    through vfs_pp_set_nvert_device: the same solve as a handle set up from host
+    metrics with the moved mask."""
+    torch = torch_cuda
+    c = cases.hill_case(24, 16, 16, h=0.3, sphere=(0.6, 0.75, 0.0, 0.3), seed=0, keep_coords=True)
+    d = _moved(c, 2)
+    fresh = handle(E, d)
+    st_ref, phi_ref, u_ref = _solve(E, torch, fresh, d)
+    fresh.close()
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), E.default_opts())
+    h.set_grid_device(torch.from_numpy(c.coor).cuda(), torch.from_numpy(c.fields["nvert"]).cuda())
+    _solve(E, torch, h, c)
+    h.set_nvert_device(torch.from_numpy(d.fields["nvert"]).cuda())
+    st, phi, u = _solve(E, torch, h, d)
+    h.close()
+    assert st.iterations == st_ref.iterations
+    np.testing.assert_array_equal(phi, phi_ref)
+    np.testing.assert_array_equal(u, u_ref)
