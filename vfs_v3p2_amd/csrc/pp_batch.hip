@@ -1916,6 +1916,7 @@ struct vfs_pp_server {
   struct Case {
     vfs_pp_handle* h = nullptr;
     size_t ws = 0;
+    size_t foot = 0;  // device bytes the case holds while registered
     bool busy = false;
   };
   struct Req {
@@ -2070,7 +2071,7 @@ int vfs_pp_server_add_case(vfs_pp_server* sv, const vfs_grid* g, const double* i
     vfs_pp_destroy(h);
     return rc;
   }
-  const size_t foot = h->ar.total();
+  const size_t foot = h->resident_bytes();
   const size_t ws = workspace_bytes(h);
   // every registered case must still be able to run alone
   if (sv->resident + foot + std::max(ws, sv->ws_max) > sv->o.hbm_budget) {
@@ -2080,7 +2081,7 @@ int vfs_pp_server_add_case(vfs_pp_server* sv, const vfs_grid* g, const double* i
   sv->resident += foot;
   sv->ws_max = std::max(sv->ws_max, ws);
   const int id = sv->next_case++;
-  sv->cases[id] = vfs_pp_server::Case{h, ws, false};
+  sv->cases[id] = vfs_pp_server::Case{h, ws, foot, false};
   *case_id = id;
   return VFS_PP_OK;
   ABI_END
@@ -2094,7 +2095,7 @@ int vfs_pp_server_remove_case(vfs_pp_server* sv, int case_id) {
   if (it->second.busy) return VFS_PP_ESTATE;
   for (const auto& r : sv->queue)
     if (r.cid == case_id) return VFS_PP_ESTATE;
-  sv->resident -= it->second.h->ar.total();
+  sv->resident -= it->second.foot;
   vfs_pp_destroy(it->second.h);
   sv->cases.erase(it);
   return VFS_PP_OK;

@@ -251,6 +251,8 @@ struct vfs_pp_handle {
   pp::Arena ar;   // buffers sized by the grid alone
   pp::Arena geo;  // buffers of the current geometry (hierarchy, face list), renewed by each set-up
   pp::Arena setup_tmp;  // set-up scratch, kept between set-ups of a moving body
+  // device bytes the handle holds: grid-sized buffers, the geometry's and set-up scratch
+  size_t resident_bytes() const { return ar.total() + geo.total() + setup_tmp.total(); }
   bool have_geom = false;     // metrics, masks and hierarchy are in place
   bool have_metrics = false;  // face coefficients and cell metrics are in place
   // fine-level resident data
