@@ -54,7 +54,11 @@ typedef struct {
   int nx, ny, nz;   /* interior cells; DMDA extents are nx+2 etc. (init.c:156-157) */
   int periodic[3];  /* i_periodic|ii_periodic, j..., k... (main.c:2150-2160) */
   int bctype[6];    /* user->bctype (init.c:502); Contra2Cart honours 1 (no-slip) */
-  int immersed;     /* 0: VolumeFlux(user,Ucont,..) poisson.c:2634; 1: flg variant poisson.c:2898 */
+  int immersed;     /* the reference's `immersed` switch (poisson_hypre.c:662-667):
+                       0 or 3: VolumeFlux(user,Ucont,..) poisson.c:2634, corrected faces
+                       set to zero (poisson.c:2728); 1: flg variant poisson.c:2898;
+                       2 (IB flux added to the outlet, Add_IBMFlux_to_Outlet) is not
+                       supported: vfs_pp_create returns VFS_PP_EINVAL */
 } vfs_grid;
 
 /* Solver options.  Defaults (vfs_pp_default_opts) give the mixed-precision form. */
@@ -107,6 +111,10 @@ void vfs_pp_default_opts(vfs_pp_opts *opts);
  * if periodic) or the DMDA box (nx+2)(ny+2)(nz+2) has 2^29 points or more. */
 int vfs_pp_create(const vfs_grid *grid, const vfs_pp_opts *opts, vfs_pp_handle **out);
 int vfs_pp_destroy(vfs_pp_handle *h);
+/* New options.  precision and restart size the Krylov storage the first
+ * vfs_pp_set_geometry allocates: changing them afterwards returns VFS_PP_ESTATE
+ * (also after a set-up that failed); smoother, cheb_degree and coarse_max shape
+ * the hierarchy and can change only while no hierarchy is in place. */
 int vfs_pp_set_opts(vfs_pp_handle *h, const vfs_pp_opts *opts);
 
 /* Geometry: the UserCtx fields lICsi..lKZet, lIAj..lKAj, lAj, lCsi/lEta/lZet and
@@ -274,8 +282,8 @@ int vfs_pp_export_level(vfs_pp_handle *h, int level, int *nlevels, int dims[3], 
                         uint8_t *cases);
 
 /* The VolumeFlux face list as the engine holds it, for verification: the faces
- * between a fluid cell and an IB cell that VolumeFlux corrects (poisson.c:2670-2700
- * for immersed == 1, 2936-3010 otherwise), in (k, j, i, axis) order.  *count
+ * between a fluid cell and an IB cell that VolumeFlux corrects (poisson.c:2936-3010
+ * for immersed == 1, 2670-2700 otherwise), in (k, j, i, axis) order.  *count
  * receives their number; when pos, area and sign are non-NULL and cap >= *count,
  * pos[f] = 3 * DMDA point + axis (the face's index in the AoS Ucont array),
  * area[f] = |csi|, |eta| or |zet| of the cell, sign[f] = +1 from fluid to IB, -1
@@ -307,8 +315,11 @@ int vfs_pp_mg_nvert_restrict_device(const vfs_grid *fine, const double *nvert_f,
  * XCD-contiguous tile order), and one FGMRES iteration of all members is one
  * HIP graph launch.  Members keep their own geometry, hierarchy and Krylov
  * storage (and their own point or y-line smoother per level); they must share
- * opts.precision and opts.cheb_degree.  A member must not be used on its own
- * while the batch exists. */
+ * opts.precision and opts.cheb_degree.  While the batch exists its members
+ * belong to it: the single-case entry points (set-up, solves, exports,
+ * vfs_pp_set_opts, vfs_pp_destroy) return VFS_PP_ESTATE for them, and a handle
+ * can be a member of one batch at a time (vfs_pp_batch_create returns
+ * VFS_PP_ESTATE otherwise). */
 typedef struct vfs_pp_batch vfs_pp_batch;
 int vfs_pp_batch_create(vfs_pp_handle *const *members, int count, vfs_pp_batch **out);
 int vfs_pp_batch_destroy(vfs_pp_batch *b);

@@ -56,8 +56,6 @@ def lib():
         _lib.or_mg_restrict.argtypes = [g, g, d, d, d, d]
         _lib.or_mg_prolong.argtypes = [g, g, d, d, d, d]
         _lib.or_mg_nvert_restrict.argtypes = [g, g, d, d]
-        _lib.or_fully_blocked.restype = C.c_int
-        _lib.or_fully_blocked.argtypes = [g, d, ip]
         _lib.or_gmres.restype = C.c_int
         _lib.or_gmres.argtypes = [C.c_int, ip, ip, d, d, d, C.c_int, C.c_double, C.c_int, C.c_int, d]
         _lib.or_num_threads.restype = C.c_int
@@ -225,12 +223,6 @@ def level_coarsen(g, g9, nvert, keep_y=False):
     nvc = np.zeros((gc.mz, gc.my, gc.mx))
     cm = lib().or_level_coarsen(C.byref(g), _p(g9), _p(nvert), int(keep_y), C.byref(gc), _p(g9c), _p(nvc))
     return gc, g9c, nvc, cm
-
-
-def fully_blocked(g, nvert):
-    kske = np.zeros(2 * g.mx * g.my, np.int32)
-    multi = lib().or_fully_blocked(C.byref(g), _p(nvert), _p(kske, C.c_int))
-    return bool(multi), kske
 
 
 def gmres(rowptr, col, val, b, x0=None, m=51, rtol=1e-13, maxit=100000, precond=1):

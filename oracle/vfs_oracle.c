@@ -298,8 +298,9 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
   const size_t st[3] = {1, (size_t)mx, (size_t)mx * my};
   const double *met[3] = {csi, eta, zet};
   const int mm[3] = {mx, my, mz};
-  if (immersed == 0) {
-    /* poisson.c:2634-2828 */
+  if (immersed == 0 || immersed == 3) {
+    /* poisson.c:2634-2828 (poisson_hypre.c:667 takes it for every value but 1 and
+       2; the corrected faces are zeroed for 0 and 3, poisson.c:2728) */
     for (int pass = 0; pass < 2; pass++) {
       double flux = 0, area = 0;
       for (int k = 1; k < mz - 1; k++)
@@ -764,22 +765,6 @@ void or_mg_nvert_restrict(const or_grid *gf, const or_grid *gc, const double *nv
             nvc[c + sz] + nvc[c - sz] > 1.1)
           nvc[c] = 1.;
       }
-}
-
-int or_fully_blocked(const or_grid *g, const double *nvert, int *kske) {
-  const int mx = g->mx, my = g->my, mz = g->mz;
-  int multi = 1;
-  for (int j = 1; j < my - 1; j++)
-    for (int i = 1; i < mx - 1; i++) {
-      int blocked = 0;
-      for (int k = 0; k < mz; k++)
-        if (nvert[IX(g, i, j, k)] > 0.1) {
-          if (!blocked) { kske[2 * (j * mx + i)] = k; blocked = 1; }
-          else if (k < kske[2 * (j * mx + i)]) kske[2 * (j * mx + i)] = k;
-        }
-      if (!blocked) multi = 0;
-    }
-  return multi;
 }
 
 /* ---------------------------------------------------------------- GMRES(m) */

@@ -75,8 +75,8 @@ void or_stencil_apply(const or_grid *g, const double *vol19, const uint8_t *rowf
 void or_rhs(const or_grid *g, const double *ucont, const double *nvert, double dt, double st,
             double coeff, double *b);
 
-/* VolumeFlux (poisson.c:2634-2828 when immersed==0; poisson.c:2898-3240 flg=0 when
- * immersed==1).  out[0..3] = flux, area before correction, flux, area after. */
+/* VolumeFlux (poisson.c:2634-2828 when immersed is 0 or 3; poisson.c:2898-3240 flg=0
+ * when immersed==1).  out[0..3] = flux, area before correction, flux, area after. */
 void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double *csi,
                     const double *eta, const double *zet, const double *nvert, double out[4]);
 
@@ -116,7 +116,6 @@ void or_mg_prolong(const or_grid *gf, const or_grid *gc, const double *nvert_f,
                    const double *nvert_c, const double *xc, double *ff);
 void or_mg_nvert_restrict(const or_grid *gf, const or_grid *gc, const double *nvert_f,
                           double *nvert_c);
-int or_fully_blocked(const or_grid *g, const double *nvert, int *kske);
 
 /* Right-preconditioned restarted GMRES(m) on the compact CSR system, FP64.
  * precond: 0 none, 1 Jacobi.  Stops at ||b-Ax||/||b|| <= rtol.  Returns iterations,

@@ -128,3 +128,14 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_create_refuses_unsupported_immersed_values():
+    """immersed = 2 (VolumeFlux plus Add_IBMFlux_to_Outlet, poisson_hypre.c:662-665)
+    and values the reference never defines are refused before any device work, so
+    this runs without a GPU; 0, 1 and 3 pass the argument checks."""
+    E = _ensure_built()
+    for bad in (2, 4, -1):
+        h = C.c_void_p()
+        rc = E.lib().vfs_pp_create(C.byref(E.make_grid(8, 6, 4, immersed=bad)), None, C.byref(h))
+        assert rc == E.VFS_PP_EINVAL and not h.value, bad

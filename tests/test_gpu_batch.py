@@ -179,3 +179,30 @@ def test_batched_steps_on_every_topology(E, torch_cuda):
         np.testing.assert_allclose(ucat[q].cpu().numpy(), uc, rtol=0, atol=1e-13 * np.abs(uc).max())
         md = O.divergence(A.g, ug, np.ascontiguousarray(c.fields["aj"]), A.nvert)
         assert abs(md - st.max_div) <= 1e-12 * max(md, 1e-300) + 1e-20, c.name
+
+
+def test_batch_members_are_refused_while_bound(E, torch_cuda):
+    """While a batch holds a handle, its work lists and captured graph refer to the
+    handle's hierarchy and buffers: the single-case entry points, a second batch
+    and vfs_pp_destroy return VFS_PP_ESTATE, and after the batch is destroyed the
+    handle works alone again."""
+    torch = torch_cuda
+    cs = get_cases()[:2]
+    hs = handles(E, cs, rtol=1e-12)
+    B = E.Batch(hs)
+    c, _ = cs[0]
+    u, phi, p, ucat = fields(torch, cs[:1])
+    nv = torch.from_numpy(np.ascontiguousarray(c.fields["nvert"])).cuda()
+    calls = [lambda: hs[0].set_geometry(c.fields), lambda: hs[0].set_nvert_device(nv),
+             lambda: hs[0].project(u[0], phi[0], p[0], ucat[0], c.dt, c.st, c.coeff),
+             lambda: hs[0].poisson_solve(u[0], phi[0], c.dt, c.st, c.coeff), lambda: hs[0].set_opts(rtol=1e-11),
+             lambda: hs[0].export_stencil(), lambda: hs[0].close(), lambda: E.Batch(hs[:1])]
+    for q, call in enumerate(calls):
+        with pytest.raises(E.VfsPPError) as ei:
+            call()
+        assert ei.value.code == E.VFS_PP_ESTATE, q
+    B.close()
+    st = hs[0].project(u[0], phi[0], p[0], ucat[0], c.dt, c.st, c.coeff, strict=True)
+    assert st.converged
+    for h in hs:
+        h.close()

@@ -5,8 +5,8 @@
   in FP64; within FP16 rounding in the FP32 hierarchy, whose structure is
   bit-exact), the smoother choice follows the oracle's row coupling, and j is kept
   exactly where the oracle's rule keeps it.
-- The VolumeFlux face list equals a restatement of poisson.c:2670-2700 (immersed
-  == 1) and 2936-3010 (immersed == 0) in (k, j, i, axis) order, areas bit for bit.
+- The VolumeFlux face list equals a restatement of poisson.c:2936-3010 (immersed
+  == 1) and 2670-2700 (immersed == 0) in (k, j, i, axis) order, areas bit for bit.
 - A second set-up on the same handle (a moved body, through vfs_pp_set_geometry
   or vfs_pp_set_nvert_device) gives the same solve as a new handle with that
   geometry, and repeated set-ups do not grow device memory.
@@ -92,7 +92,7 @@ def test_hierarchy_matches_oracle_level_by_level(E, name, precision):
 
 
 def ib_faces_ref(c):
-    """(pos, area, sign) of the VolumeFlux faces, poisson.c:2670-2700 / 2936-3010."""
+    """(pos, area, sign) of the VolumeFlux faces, poisson.c:2936-3010 / 2670-2700."""
     nv = c.fields["nvert"]
     mz, my, mx = nv.shape
     out = []

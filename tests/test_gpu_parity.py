@@ -395,3 +395,79 @@ def test_orthogonalisation_variants(E, torch_cuda, name, precision, orthog):
         assert relL2(phi.cpu().numpy(), ref["phi"], A.fluid_cells) <= PHI_TOL
         assert st.max_div <= DIV_TOL and st.rel_residual <= 1e-12
     assert abs(its[orthog] - its[0]) <= 1
+
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("dt,st,coeff", [(0.01, 0.5, 1.5), (0.01, 1.0, 1.5)])
+def test_projection_parity_time_factors(E, dt, st, coeff, precision):
+    """coeff = 1.5 is time_coeff() under rans after the first step (poisson.c:22-30);
+    st (user->st) and dt enter the RHS as st coeff/dt (poisson_hypre.c:376) and the
+    projection as dt st/coeff (poisson.c:2263).  The engine matches the oracle with
+    these values.  Only st = 1 makes the projected field divergence-free (the RHS
+    and the projection both scale by st), so max|div u| is checked there."""
+    import dataclasses
+
+    c0, A, _ = get_case("hill_sphere")
+    c = dataclasses.replace(c0, dt=dt, st=st, coeff=coeff)
+    ref = O.reference_projection(c, rtol=1e-13)
+    print(f"hill_sphere dt {dt} st {st} coeff {coeff} precision {precision}", flush=True)
+    user = E.UserCtx(c, E.default_opts(precision=precision, rtol=1e-12))
+    stt = E.Flow_Solver_pressure_step(user, strict=True)
+    fl = A.fluid_cells
+    assert relL2(user.Phi.cpu().numpy(), ref["phi"], fl) <= PHI_TOL
+    assert relL2(user.P.cpu().numpy(), ref["p"], fl) <= PHI_TOL
+    # the oracle's projection of the engine's own phi (the engine folds dt st/coeff
+    # into one factor, the reference applies them in turn: rounding-level agreement)
+    u = np.ascontiguousarray(c.fields["ucont"].copy())
+    O.volume_flux(A.g, c.immersed, u, c.fields)
+    O.projection(A.g, c.fields, np.ascontiguousarray(user.Phi.cpu().numpy()), u, dt, st, coeff)
+    np.testing.assert_allclose(user.Ucont.cpu().numpy(), u, rtol=0, atol=1e-13 * np.abs(u).max())
+    ucat = np.zeros_like(u)
+    O.contra2cart(A.g, c.bctype, c.fields, u, ucat)
+    np.testing.assert_allclose(user.Ucat.cpu().numpy(), ucat, rtol=0, atol=1e-13 * np.abs(ucat).max())
+    md = O.divergence(A.g, user.Ucont.cpu().numpy(), np.ascontiguousarray(c.fields["aj"]), A.nvert)
+    assert abs(md - stt.max_div) <= 1e-12 * max(md, 1e-300) + 1e-20
+    if st == 1.0:
+        assert stt.max_div <= DIV_TOL and ref["maxdiv"] <= DIV_TOL
+    else:
+        assert stt.max_div > 1e-3 and abs(stt.max_div - ref["maxdiv"]) <= 1e-8 * ref["maxdiv"]
+    user.handle.close()
+
+
+def test_immersed_3_is_the_four_argument_volume_flux(E, torch_cuda):
+    """immersed = 3 takes VolumeFlux(user, Ucont, ..) and zeroes the corrected faces
+    like immersed = 0 (poisson_hypre.c:667, poisson.c:2728): identical results."""
+    torch = torch_cuda
+    outs = []
+    for imm in (0, 3):
+        c = _sphere_immersed0()
+        h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, imm), E.default_opts(rtol=1e-12))
+        h.set_geometry(c.fields)
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+        outs.append((phi.cpu().numpy(), u.cpu().numpy(), list(st.ibm_flux)))
+        h.close()
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_set_opts_after_a_failed_set_up(E):
+    """A set-up that fails after the Krylov storage was sized (here: a coarse_max
+    above the 1024-cell direct-solve limit leaves a 1728-cell coarsest level) still
+    fixes precision and restart; hierarchy options may change, and the next set-up
+    succeeds."""
+    c = cases.channel_case(12, 12, 12, beta=2.0, seed=0)
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed), E.default_opts(coarse_max=2000))
+    with pytest.raises(E.VfsPPError) as ei:
+        h.set_geometry(c.fields)
+    assert ei.value.code == E.VFS_PP_EINVAL
+    for kw in ({"precision": 0}, {"restart": 12}):
+        o = E.default_opts(coarse_max=2000, **kw)
+        assert E.lib().vfs_pp_set_opts(h._h, E.C.byref(o)) == E.VFS_PP_ESTATE
+    h.set_opts(coarse_max=512)
+    h.set_geometry(c.fields)
+    h.close()

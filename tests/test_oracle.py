@@ -63,14 +63,40 @@ def test_operator_eigenfunction_uniform_grid():
     np.testing.assert_allclose(y, lam * f, atol=1e-12)
 
 
-def test_rhs_is_scaled_divergence():
+@pytest.mark.parametrize("dt,st,coeff", [(0.04, 1.0, 1.0), (0.01, 0.5, 1.5)])
+def test_rhs_is_scaled_divergence(dt, st, coeff):
+    """b = st coeff / dt * div U (poisson_hypre.c:376, val *= -1/dt*st*coeff of -div)."""
     c = cases.hill_case(12, 8, 10, h=0.3, sphere=None, seed=4)
     A = O.Assembled(c)
     u = c.fields["ucont"]
-    b = O.rhs(A.g, np.ascontiguousarray(u), A.nvert, 0.04, 1.0, 1.0)
+    b = O.rhs(A.g, np.ascontiguousarray(u), A.nvert, dt, st, coeff)
     div = (u[1:-1, 1:-1, 1:-1, 0] - u[1:-1, 1:-1, :-2, 0] + u[1:-1, 1:-1, 1:-1, 1] - u[1:-1, :-2, 1:-1, 1] +
            u[1:-1, 1:-1, 1:-1, 2] - u[:-2, 1:-1, 1:-1, 2])
-    np.testing.assert_allclose(b[1:-1, 1:-1, 1:-1], div / 0.04, rtol=1e-12, atol=1e-12)
+    scale = {(0.04, 1.0, 1.0): 25.0, (0.01, 0.5, 1.5): 75.0}[(dt, st, coeff)]  # st * coeff / dt
+    np.testing.assert_allclose(b[1:-1, 1:-1, 1:-1], div * scale, rtol=1e-12, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("dt,st,coeff", [(0.01, 0.5, 1.5), (0.02, 2.0, 1.5)])
+def test_projection_scales_with_dt_st_over_coeff(dt, st, coeff):
+    """The face correction is F(phi) dt st / coeff (poisson.c:2263, 2312, 2361):
+    against dt = st = coeff = 1 every corrected face moves by exactly that factor
+    (0.01 * 0.5 / 1.5 = 1/300 and 0.02 * 2 / 1.5 = 2/75), which a coeff that
+    multiplies or an st that divides would not give."""
+    c = cases.hill_case(16, 10, 12, h=0.3, sphere=(0.5, 0.8, 0.0, 0.4), seed=5)
+    A = O.Assembled(c, drop=False)
+    f = c.fields
+    rng = np.random.default_rng(1)
+    phi = np.where(A.nvert < 0.1, rng.standard_normal(A.nvert.shape), 0.0)
+    cases.periodic_fill(phi, c.per)
+    u0 = np.ascontiguousarray(f["ucont"].copy())
+    u1, u2 = u0.copy(), u0.copy()
+    O.projection(A.g, f, phi, u1, 1.0, 1.0, 1.0)
+    O.projection(A.g, f, phi, u2, dt, st, coeff)
+    ratio = {(0.01, 0.5, 1.5): 1.0 / 300.0, (0.02, 2.0, 1.5): 2.0 / 75.0}[(dt, st, coeff)]
+    # faces of the interior cells (ghost faces are copies of whole values, not of the change)
+    d1, d2 = (u1 - u0)[1:-1, 1:-1, 1:-1], (u2 - u0)[1:-1, 1:-1, 1:-1]
+    assert np.abs(d1).max() > 0.1
+    np.testing.assert_allclose(d2, ratio * d1, rtol=0, atol=1e-13 * np.abs(d1).max())
 
 
 def test_generator_flux_is_divergence_free():
@@ -131,15 +157,20 @@ def test_drop_rule():
     assert np.all(diff.row != diff.col), "the diagonal is never dropped"
 
 
-def test_projection_identity_random_phi():
-    """div(U - dt/coeff F(phi)) = dt/coeff (b - A phi) on fluid cells, for any phi:
-    the projection and the operator use the same masked face fluxes."""
+@pytest.mark.parametrize("dt,st,coeff", [(0.04, 1.0, 1.0), (0.01, 1.0, 1.5), (0.01, 0.5, 1.5)])
+def test_projection_identity_random_phi(dt, st, coeff):
+    """div(U - dt st/coeff F(phi)) = dt/coeff (b/st - st A phi) on fluid cells, for
+    any phi: the projection and the operator use the same masked face fluxes, the
+    RHS scales div U by st coeff/dt (poisson_hypre.c:376) and the projection
+    scales F by dt st/coeff (poisson.c:2263).  With coeff = 1.5 (time_coeff under
+    rans, poisson.c:22-30) and st = 0.5 a coeff or st on the wrong side of either
+    factor fails; at st = 1 this is div u_new = dt/coeff (b - A phi)."""
     c = cases.hill_case(16, 10, 12, h=0.3, sphere=(0.5, 0.8, 0.0, 0.4), seed=5)
     A = O.Assembled(c, drop=False)
     f = c.fields
     u = np.ascontiguousarray(f["ucont"].copy())
     O.volume_flux(A.g, c.immersed, u, f)
-    b = O.rhs(A.g, u, A.nvert, c.dt, c.st, c.coeff)
+    b = O.rhs(A.g, u, A.nvert, dt, st, coeff)
     rng = np.random.default_rng(0)
     phi = np.where(A.nvert < 0.1, rng.standard_normal(A.nvert.shape), 0.0)
     phi[0] = phi[-1] = 0
@@ -147,12 +178,12 @@ def test_projection_identity_random_phi():
     phi[:, :, 0] = phi[:, :, -1] = 0
     cases.periodic_fill(phi, c.per)
     Aphi = O.stencil_apply(A.g, A.vol, A.rowflag, phi)
-    O.projection(A.g, f, phi, u, c.dt, c.st, c.coeff)
+    O.projection(A.g, f, phi, u, dt, st, coeff)
     _, div = O.divergence(A.g, u, np.ones(A.nvert.shape), A.nvert, want_field=True)
     lt = A.nvert < 0.1
     d = (u[1:-1, 1:-1, 1:-1, 0] - u[1:-1, 1:-1, :-2, 0] + u[1:-1, 1:-1, 1:-1, 1] - u[1:-1, :-2, 1:-1, 1] +
          u[1:-1, 1:-1, 1:-1, 2] - u[:-2, 1:-1, 1:-1, 2])
-    expect = c.dt / c.coeff * (b - Aphi)[1:-1, 1:-1, 1:-1]
+    expect = dt / coeff * (b / st - st * Aphi)[1:-1, 1:-1, 1:-1]
     m = lt[1:-1, 1:-1, 1:-1]
     np.testing.assert_allclose(d[m], expect[m], atol=1e-12 * np.abs(expect).max())
 
@@ -197,11 +228,12 @@ def test_volume_flux_hand_built_faces():
     np.testing.assert_allclose(u[3, 3, 2, 0], a_in[0] - corr, rtol=0, atol=1e-14)
     np.testing.assert_allclose(u[3, 3, 3, 0], a_out[0] + corr, rtol=0, atol=1e-14)
     assert np.all(u[3, 3, 4] == u0[3, 3, 4]) and np.all(u[3, 3, 5] == u0[3, 3, 5])  # B untouched
-    u = u0.copy()
-    out = O.volume_flux(g, 0, u, f)
-    assert out[1] == 12.0
-    assert abs(out[0] - (sum(a_in) + sum(b_in) - sum(a_out) - sum(b_out))) < 1e-13
-    assert u[3, 3, 2, 0] == 0.0 and u[3, 3, 5, 1] == 0.0 and out[3] == 12.0 and out[2] == 0.0
+    for immersed in (0, 3):  # poisson_hypre.c:667 and poisson.c:2728 treat 3 as 0
+        u = u0.copy()
+        out = O.volume_flux(g, immersed, u, f)
+        assert out[1] == 12.0
+        assert abs(out[0] - (sum(a_in) + sum(b_in) - sum(a_out) - sum(b_out))) < 1e-13
+        assert u[3, 3, 2, 0] == 0.0 and u[3, 3, 5, 1] == 0.0 and out[3] == 12.0 and out[2] == 0.0
 
 
 def test_contra2cart_uniform_flow():
@@ -217,6 +249,31 @@ def test_contra2cart_uniform_flow():
     O.contra2cart(A.g, (0, 0, 0, 0, 0, 0), f, np.ascontiguousarray(u), ucat)
     np.testing.assert_allclose(ucat[1:-1, 1:-1, 1:-1], np.broadcast_to(U, ucat[1:-1, 1:-1, 1:-1].shape),
                                rtol=1e-12)
+
+
+def test_contra2cart_uniform_flow_sheared_grid():
+    """Under a full affine map the rows csi, eta, zet of Contra2Cart's 3x3 system
+    (rhs.c:144-189) form a non-symmetric, non-diagonal matrix M, and the face
+    fluxes of a uniform Cartesian velocity U are q = M U at every face, so the
+    Cramer solve returns U to rounding; a transposed system, a swapped
+    determinant column or a wrong face average does not."""
+    c, _ = affine_case(9, 8, 7)
+    f = c.fields
+    U = np.array([0.7, -0.3, 0.45])
+    M = np.stack([f["csi"][4, 4, 4], f["eta"][4, 4, 4], f["zet"][4, 4, 4]])
+    s = np.abs(M).max()
+    assert np.abs(M - M.T).max() > 0.1 * s
+    assert np.abs(M[~np.eye(3, dtype=bool)]).min() > 1e-3 * s
+    assert np.abs(np.linalg.solve(M.T, M @ U) - U).max() > 0.05  # a transposed solve would differ
+    u = np.zeros(c.shape + (3,))
+    u[..., 0] = f["icsi"] @ U
+    u[..., 1] = f["jeta"] @ U
+    u[..., 2] = f["kzet"] @ U
+    A = O.Assembled(c)
+    ucat = np.zeros_like(u)
+    O.contra2cart(A.g, (0, 0, 0, 0, 0, 0), f, np.ascontiguousarray(u), ucat)
+    np.testing.assert_allclose(ucat[1:-1, 1:-1, 1:-1], np.broadcast_to(U, ucat[1:-1, 1:-1, 1:-1].shape),
+                               rtol=0, atol=1e-13)
 
 
 def test_contra2cart_noslip_ghosts_and_solid():
@@ -273,6 +330,91 @@ def test_mg_transfers_constants():
     fc = O.mg_restrict(gf, gc, nvf, nvc, xf)
     np.testing.assert_allclose(fc[1:-1, 1:-1, 1:-1], 1.5, rtol=1e-15)  # 0.125 * 8 children
     assert np.all(fc[0] == 0) and np.all(ff[0] == 0)
+
+
+def test_mg_prolongation_of_a_linear_field():
+    """MyInterpolation (poisson.c:473-506, 526-628): the weights
+    (27, 9, 9, 3, 9, 3, 3, 1)/64 reproduce a field linear in the cell-centre
+    coordinates at every fine cell whose three offsets are two-sided; along an
+    axis where the offset is zeroed (the first and last fine cells, or a coarse IB
+    neighbour) the field is taken at the coarse centre on that axis.  Distinct
+    slopes on the three axes make any swap of the 9/3 weights between axes, or
+    a wrong offset sign, fail; fine IB cells and boundary points are zero."""
+    gf = O.grid((12, 10, 14), (0, 0, 0))  # fine interior 12 x 8 x 10 (i, j, k)
+    gc = _coarse_grid(gf)
+    a = np.array([0.37, -1.21, 2.03])  # slopes along i, j, k (fine cell units)
+    d0 = 0.77
+    kc, jc, ic = np.meshgrid(np.arange(gc.mz), np.arange(gc.my), np.arange(gc.mx), indexing="ij")
+    X = lambda q: 2.0 * q - 1.0  # coarse centre in fine units (fine centre of cell q is q - 0.5)
+    xc = a[0] * X(ic) + a[1] * X(jc) + a[2] * X(kc) + d0
+    nvf = np.zeros((gf.mz, gf.my, gf.mx))
+    nvc = np.zeros((gc.mz, gc.my, gc.mx))
+    nvc[3, 2, 4] = 1.0  # a coarse IB cell: its fine neighbours interpolate one-sided towards it
+    nvf[5:7, 3:5, 7:9] = 1.0  # its children are IB too
+    ff = O.mg_prolong(gf, gc, nvf, nvc, np.ascontiguousarray(xc))
+    expect = np.zeros_like(ff)
+    m = (gf.mz, gf.my, gf.mx)
+    dropped = 0
+    for k in range(1, m[0] - 1):
+        for j in range(1, m[1] - 1):
+            for i in range(1, m[2] - 1):
+                if nvf[k, j, i] > 0.1:
+                    continue
+                cc = [(i + 1) // 2, (j + 1) // 2, (k + 1) // 2]
+                off = []
+                for ax, (q, mq) in enumerate(((i, m[2]), (j, m[1]), (k, m[0]))):
+                    o = 1 if q == 2 * cc[ax] else -1
+                    if q == 1 or q == mq - 2:
+                        o = 0
+                    off.append(o)
+                # an offset towards a coarse IB neighbour is dropped
+                nb = [(cc[2] + off[2], cc[1], cc[0]), (cc[2], cc[1] + off[1], cc[0]), (cc[2], cc[1], cc[0] + off[0])]
+                for ax, p in zip((2, 1, 0), nb):
+                    if off[ax] and nvc[p] > 0.1:
+                        off[ax] = 0
+                        dropped += 1
+                pos = [(q - 0.5) if off[ax] else X(cc[ax]) for ax, q in enumerate((i, j, k))]
+                expect[k, j, i] = a @ np.array(pos) + d0
+    assert dropped >= 8  # fine cells beside the coarse IB cell
+    np.testing.assert_allclose(ff, expect, rtol=0, atol=1e-13)
+    assert np.any(ff[1:-1, 1:-1, 1:-1] != 0) and np.all(ff[nvf > 0.1] == 0)
+    assert np.all(ff[0] == 0) and np.all(ff[:, :, -1] == 0)
+
+
+def test_divergence_hand_built():
+    """Divergence (main.c:1244-1391) on a hand-built 5x5x5 box: |aj * (flux out -
+    flux in)| per cell, zero where any of the 7 points (the cell and its six face
+    neighbours) has nvert > 0.1, so an IB cell silences itself and its six face
+    neighbours but not its diagonal neighbours, and an IB-flagged boundary plane
+    silences the interior layer next to it."""
+    g = O.grid((7, 7, 7), (0, 0, 0))
+    aj = np.full((7, 7, 7), 2.0)
+    aj[1, 1, 1] = 3.0
+    nv = np.zeros((7, 7, 7))
+    nv[3, 3, 3] = 1.0
+    u = np.zeros((7, 7, 7, 3))
+    u[1, 1, 1, 0] = 0.5    # out of (1,1,1) [aj 3]: 1.5; into (1,1,2): 1.0
+    u[3, 3, 2, 0] = 4.0    # between (3,3,2) and the IB cell (3,3,3): both silenced
+    u[3, 2, 4, 1] = 0.25   # out of (3,2,4): 0.5; into (3,3,4), a face neighbour of the IB cell: silenced
+    u[4, 4, 4, 0] = 0.1    # (4,4,4) and (4,4,5) are diagonal neighbours of the IB cell: 0.2 each
+    u[5, 5, 5, 2] = -2.0   # out of (5,5,5) through its top face into the boundary plane: 4.0
+    u[3, 3, 3, 2] = 9.0    # out of the IB cell into (4,3,3), its face neighbour: silenced
+    md, div = O.divergence(g, u, aj, nv, want_field=True)
+    expect = np.zeros((7, 7, 7))
+    expect[1, 1, 1] = 1.5
+    expect[1, 1, 2] = 1.0
+    expect[3, 2, 4] = 0.5
+    expect[4, 4, 4] = 0.2
+    expect[4, 4, 5] = 0.2
+    expect[5, 5, 5] = 4.0
+    np.testing.assert_allclose(div, expect, rtol=1e-15, atol=0)
+    assert md == 4.0
+    nv[:, 0, :] = 1.0  # a flagged j = 0 plane silences the j = 1 layer
+    expect[1, 1, 1] = expect[1, 1, 2] = 0.0
+    md, div = O.divergence(g, u, aj, nv, want_field=True)
+    np.testing.assert_allclose(div, expect, rtol=1e-15, atol=0)
+    u[5, 5, 5, 2] = 0.0
+    assert O.divergence(g, u, aj, nv) == 0.5
 
 
 def test_mg_nvert_restriction_rules():

@@ -730,6 +730,7 @@ struct BatchCore {
   void bind(int q, vfs_pp_handle* h) {
     Slot& sl = slots[q];
     ensure_geom(h);
+    ensure_unbound(h);
     auto* mg = dynamic_cast<MG<T>*>(h->mg.get());
     if (!mg) throw StatusError(VFS_PP_EINVAL, "batch members must share the multigrid precision");
     if (degree == 0) degree = mg->degree;
@@ -738,11 +739,15 @@ struct BatchCore {
     sl.mg = mg;
     sl.own_stream = h->s;
     h->s = s;
+    h->batch_refs++;
     sl.state = DONE;
     dirty = true;
   }
   void release(Slot& sl) {
-    if (sl.h) sl.h->s = sl.own_stream;
+    if (sl.h) {
+      sl.h->s = sl.own_stream;
+      sl.h->batch_refs--;
+    }
     sl = Slot{};
     dirty = true;
   }

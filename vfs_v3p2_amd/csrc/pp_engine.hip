@@ -214,7 +214,7 @@ static void build_hierarchy(vfs_pp_handle* h, Arena& tmp) {
     throw StatusError(VFS_PP_EINVAL, "grid cannot be coarsened to a direct-solve size");
 }
 
-// VolumeFlux face list (poisson.c:2670-2700 / 2936-3010) in (k, j, i, axis) order,
+// VolumeFlux face list (poisson.c:2936-3010 / 2670-2700) in (k, j, i, axis) order,
 // and the number of cells with nvert < 0.1
 static void build_ib_faces(vfs_pp_handle* h, Arena& tmp) {
   hipStream_t s = h->s;
@@ -710,8 +710,14 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
   if ((long long)(g->nx + 2) * (g->ny + 2) * (g->nz + 2) >= (1LL << 29)) return VFS_PP_EINVAL;
   for (int a = 0; a < 3; a++)
     if (g->periodic[a] && (a == 0 ? g->nx : a == 1 ? g->ny : g->nz) < 3) return VFS_PP_EINVAL;
+  // VolumeFlux variants (poisson_hypre.c:662-667): 1 takes the flg form; 0 and 3
+  // take the four-argument form, which zeroes the corrected faces for both
+  // (poisson.c:2728); 2 adds the IB flux to the outlet (Add_IBMFlux_to_Outlet),
+  // an inflow/outflow boundary this engine does not model
+  if (g->immersed != 0 && g->immersed != 1 && g->immersed != 3) return VFS_PP_EINVAL;
   auto h = std::make_unique<vfs_pp_handle>();
   h->grid = *g;
+  h->grid.immersed = g->immersed == 1 ? 1 : 0;
   if (o) h->opts = *o;
   else vfs_pp_default_opts(&h->opts);
   if (!(h->opts.rtol > 0.) || h->opts.max_iters < 0 || h->opts.restart < 1) return VFS_PP_EINVAL;
@@ -729,6 +735,7 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
 
 int vfs_pp_destroy(vfs_pp_handle* h) {
   if (!h) return VFS_PP_OK;
+  if (h->batch_refs > 0) return VFS_PP_ESTATE;  // still bound to a batch
   if (h->s) (void)hipStreamSynchronize(h->s);
   h->mg.reset();
   h->ar.release();
@@ -740,9 +747,13 @@ int vfs_pp_destroy(vfs_pp_handle* h) {
 
 int vfs_pp_set_opts(vfs_pp_handle* h, const vfs_pp_opts* o) {
   if (!h || !o) return VFS_PP_EINVAL;
-  // options that change the hierarchy require a new handle
-  if (h->have_geom && (o->precision != h->opts.precision || o->restart != h->opts.restart ||
-                       o->coarse_max != h->opts.coarse_max || o->smoother != h->opts.smoother ||
+  if (h->batch_refs > 0) return VFS_PP_ESTATE;
+  // precision and restart size the Krylov basis and partial-sum buffers, which
+  // exist from the first set-up on (also after a set-up that failed); the
+  // hierarchy options take effect at the next set-up, so they are fixed only
+  // while a hierarchy is in place
+  if (h->g64 && (o->precision != h->opts.precision || o->restart != h->opts.restart)) return VFS_PP_ESTATE;
+  if (h->have_geom && (o->coarse_max != h->opts.coarse_max || o->smoother != h->opts.smoother ||
                        o->cheb_degree != h->opts.cheb_degree))
     return VFS_PP_ESTATE;
   if (!(o->rtol > 0.)) return VFS_PP_EINVAL;
@@ -757,6 +768,7 @@ int vfs_pp_set_geometry(vfs_pp_handle* h, const double* icsi, const double* ieta
                         const double* nvert) {
   ABI_BEGIN
   if (!h) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   const double* hmet[16] = {icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, aj, csi, eta, zet};
   for (auto p : hmet)
     if (!p) return VFS_PP_EINVAL;
@@ -796,6 +808,7 @@ int vfs_pp_form_metrics_device(const vfs_grid* g, const double* coor, double* ic
 int vfs_pp_set_grid_device(vfs_pp_handle* h, const double* coor, const double* nvert) {
   ABI_BEGIN
   if (!h || !coor || !nvert) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   const size_t M = h->m.n();
   Arena tmp;
   double* dmet[16];
@@ -809,6 +822,7 @@ int vfs_pp_set_grid_device(vfs_pp_handle* h, const double* coor, const double* n
 int vfs_pp_set_nvert_device(vfs_pp_handle* h, const double* nvert) {
   ABI_BEGIN
   if (!h || !nvert) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   if (!h->have_metrics) return VFS_PP_ESTATE;
   set_mask_core(h, nvert, true);
   return VFS_PP_OK;
@@ -819,6 +833,7 @@ int vfs_pp_poisson_solve_device(vfs_pp_handle* h, double* ucont, double* phi, do
                                 vfs_pp_stats* stats) {
   ABI_BEGIN
   if (!h || !ucont || !phi || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   vfs_pp_stats S{};
   poisson_solve(h, ucont, phi, dt, st, coeff, &S);
   if (stats) *stats = S;
@@ -829,6 +844,7 @@ int vfs_pp_poisson_solve_device(vfs_pp_handle* h, double* ucont, double* phi, do
 int vfs_pp_update_pressure_device(vfs_pp_handle* h, double* p, double* phi) {
   ABI_BEGIN
   if (!h || !p || !phi) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   update_pressure(h, p, phi);
   HIPCHECK(hipStreamSynchronize(h->s));
   return VFS_PP_OK;
@@ -839,6 +855,7 @@ int vfs_pp_projection_device(vfs_pp_handle* h, double* ucont, const double* phi,
                              double coeff) {
   ABI_BEGIN
   if (!h || !ucont || !phi || !ucat || coeff == 0.) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   projection(h, ucont, phi, ucat, dt, st, coeff);
   HIPCHECK(hipStreamSynchronize(h->s));
   return VFS_PP_OK;
@@ -848,6 +865,7 @@ int vfs_pp_projection_device(vfs_pp_handle* h, double* ucont, const double* phi,
 int vfs_pp_divergence_device(vfs_pp_handle* h, const double* ucont, double* maxdiv) {
   ABI_BEGIN
   if (!h || !ucont || !maxdiv) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   *maxdiv = divergence(h, ucont);
   return VFS_PP_OK;
   ABI_END
@@ -857,6 +875,7 @@ int vfs_pp_project_device(vfs_pp_handle* h, double* ucont, double* phi, double* 
                           double st, double coeff, vfs_pp_stats* stats) {
   ABI_BEGIN
   if (!h || !ucont || !phi || !p || !ucat || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   const double t0 = now_ms();
   vfs_pp_stats S{};
   poisson_solve(h, ucont, phi, dt, st, coeff, &S);
@@ -876,6 +895,7 @@ int vfs_pp_project(vfs_pp_handle* h, double* ucont, double* phi, double* p, doub
                    double coeff, vfs_pp_stats* stats) {
   ABI_BEGIN
   if (!h || !ucont || !phi || !p || !ucat) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   const size_t M = h->m.n();
   Arena tmp;
@@ -902,6 +922,7 @@ int vfs_pp_project(vfs_pp_handle* h, double* ucont, double* phi, double* p, doub
 int vfs_pp_apply_device(vfs_pp_handle* h, const double* phi, double* y, int drop_small) {
   ABI_BEGIN
   if (!h || !phi || !y) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   launch_apply_dmda(h->s, h->m, h->d, h->g64, h->code, phi, y, drop_small != 0);
   HIPCHECK(hipStreamSynchronize(h->s));
@@ -944,6 +965,7 @@ static void export_rows(vfs_pp_handle* h, const Dims& d, const double* g, const 
 int vfs_pp_export_stencil(vfs_pp_handle* h, double* vol19, uint8_t* cases) {
   ABI_BEGIN
   if (!h || !vol19 || !cases) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   export_rows(h, h->d, h->g64, h->code, vol19, cases);
   return VFS_PP_OK;
@@ -954,6 +976,7 @@ int vfs_pp_export_level(vfs_pp_handle* h, int level, int* nlevels, int dims[3], 
                         uint8_t* cases) {
   ABI_BEGIN
   if (!h) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   const int nl = h->mg->nlev();
   if (nlevels) *nlevels = nl;
@@ -981,6 +1004,7 @@ int vfs_pp_export_level(vfs_pp_handle* h, int level, int* nlevels, int dims[3], 
 int vfs_pp_export_ib_faces(vfs_pp_handle* h, size_t cap, long long* pos, double* area, float* sign, size_t* count) {
   ABI_BEGIN
   if (!h || !count) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   *count = h->nibf;
   if (!pos || !area || !sign || cap < h->nibf) return VFS_PP_OK;
@@ -1050,6 +1074,7 @@ int vfs_pp_mg_nvert_restrict_device(const vfs_grid* g, const double* nvf, double
 int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, double* bytes) {
   ABI_BEGIN
   if (!h || reps < 1 || !ms || !bytes) return VFS_PP_EINVAL;
+  ensure_unbound(h);
   ensure_geom(h);
   const size_t n = h->d.n(), M = h->m.n();
   Arena tmp;

@@ -253,6 +253,7 @@ struct vfs_pp_handle {
   pp::Arena setup_tmp;  // set-up scratch, kept between set-ups of a moving body
   // device bytes the handle holds: grid-sized buffers, the geometry's and set-up scratch
   size_t resident_bytes() const { return ar.total() + geo.total() + setup_tmp.total(); }
+  int batch_refs = 0;         // batches / server slots the handle is bound to (0 or 1)
   bool have_geom = false;     // metrics, masks and hierarchy are in place
   bool have_metrics = false;  // face coefficients and cell metrics are in place
   // fine-level resident data
@@ -312,6 +313,12 @@ void update_pressure(vfs_pp_handle* h, double* p, double* phi);
 void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
                 double coeff);
 void divergence_launch(vfs_pp_handle* h, const double* ucont);
+// A handle bound to a batch or a server slot runs on the batch's stream and its
+// buffers are referenced by the batch's work lists and captured graph: the
+// single-case entry points refuse it until the batch lets it go.
+inline void ensure_unbound(const vfs_pp_handle* h) {
+  if (h->batch_refs > 0) throw StatusError(VFS_PP_ESTATE, "the handle belongs to a batch; destroy the batch first");
+}
 }  // namespace pp
 
 // C-ABI wrappers: exceptions become status codes

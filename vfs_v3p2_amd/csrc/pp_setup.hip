@@ -238,8 +238,8 @@ __global__ __launch_bounds__(256) void k_random_rows(size_t n, const uint32_t* _
 }
 
 // ------------------------------------------------------------ VolumeFlux faces
-// The faces of DMDA point (I, J, K) that VolumeFlux corrects (poisson.c:2670-2700
-// for immersed == 1, 2936-3010 otherwise): bit dd of the result; bit dd of neg
+// The faces of DMDA point (I, J, K) that VolumeFlux corrects (poisson.c:2936-3010
+// for immersed == 1, 2670-2700 otherwise): bit dd of the result; bit dd of neg
 // when the face goes from IB to fluid.
 __device__ inline int ib_face_mask(const Dmda& m, const double* __restrict__ nv, int immersed, int I, int J, int K,
                                    int& neg) {

@@ -208,7 +208,8 @@ class Handle:
 
     def close(self):
         if self._h:
-            lib().vfs_pp_destroy(self._h)
+            # VFS_PP_ESTATE while a batch still holds the handle: it stays open
+            _check(lib().vfs_pp_destroy(self._h), "vfs_pp_destroy")
             self._h = C.c_void_p()
 
     def __del__(self):
