@@ -248,6 +248,20 @@ int vfs_pp_projection_device(vfs_pp_handle *h, double *ucont, const double *phi,
  * neighbourhood is fluid. */
 int vfs_pp_divergence_device(vfs_pp_handle *h, const double *ucont, double *maxdiv);
 
+/* Divergence without a handle and without any device allocation.  No
+ * counterpart in the reference: it serves callers that cannot allocate device
+ * memory (work recorded into a HIP graph, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as vfs_pp_divergence_device, bit for bit, from
+ * the reference's own DMDA arrays: ucont (AoS), aj and nvert (main.c:1244-1391),
+ * all device pointers; *maxdiv is one device double.  The block maxima go to a
+ * fixed 32 KiB device array of the library (4096 partials, one copy per GPU,
+ * held until the process ends), reduced with a grid-stride loop, so any grid up
+ * to the 2^29-point DMDA limit fits (larger ones: VFS_PP_EINVAL).  Runs on the
+ * legacy default stream and returns when *maxdiv is written.  Calls that overlap
+ * in time on one GPU share that array and race; sequential calls do not. */
+int vfs_pp_divergence_noalloc(const vfs_grid *grid, const double *ucont, const double *aj,
+                              const double *nvert, double *maxdiv);
+
 /* The whole per-step pressure path, as solvers.c:874-895 runs it:
  * poisson_solve -> update_pressure -> projection, plus the divergence check. */
 int vfs_pp_project_device(vfs_pp_handle *h, double *ucont, double *phi, double *p, double *ucat,

@@ -99,6 +99,7 @@ def lib():
         "vfs_pp_update_pressure_device": (C.c_int, [H, vp, vp]),
         "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
         "vfs_pp_divergence_device": (C.c_int, [H, vp, dp]),
+        "vfs_pp_divergence_noalloc": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
         "vfs_pp_project_device": (C.c_int, [H, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(Stats)]),
         "vfs_pp_project": (C.c_int, [H, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
@@ -525,6 +526,17 @@ def Projection(user: UserCtx):
 def Divergence(user: UserCtx) -> float:
     """Divergence (main.c:1244): max |aj div u| over cells not next to the IB."""
     return user.handle.divergence(user.Ucont)
+
+
+def divergence_noalloc(grid: Grid, ucont, aj, nvert, out) -> float:
+    """Divergence (main.c:1244) from the DMDA arrays themselves, without a handle
+    and without any device allocation: max |aj div u| is written to the one-element
+    float64 device tensor `out` (and returned)."""
+    S = dmda_shape(grid)
+    _check(lib().vfs_pp_divergence_noalloc(C.byref(grid), _tptr(ucont, dmda_shape(grid, True), "ucont"),
+                                           _tptr(aj, S, "aj"), _tptr(nvert, S, "nvert"), _tptr(out, (1,), "out")),
+           "vfs_pp_divergence_noalloc")
+    return float(out.item())
 
 
 def Flow_Solver_pressure_step(user: UserCtx, strict=False):
