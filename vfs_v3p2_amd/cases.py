@@ -482,13 +482,16 @@ def divergent_ucont(nx, ny, nz, f, per, seed, noise=0.05, mean_profile=None, flu
 # ------------------------------------------------------------------------- configs
 
 def build_case(name, nx, ny, nz, x, eta, z, *, hill_h=0.0, hill_s=0.5, top=2.0,
-               ib=None, seed=0, noise=0.05, mean_profile_fn=None, fluct=1.0, keep_coords=False):
+               ib=None, seed=0, noise=0.05, mean_profile_fn=None, fluct=1.0, keep_coords=False, metrics_fn=None):
+    """metrics_fn(co, per) -> the 16 metric arrays; form_metrics by default (any
+    function bit-identical to it, such as FormMetrics on the device, gives the same
+    case)."""
     per = (1, 0, 1)
     bct = (0, 0, 1, 1, 0, 0)  # no-slip walls in j; periodic i and k
     N = make_nodes(nx, ny, nz, x, eta, z, hill_h=hill_h, hill_s=hill_s, top=top)
     co = dmda_coords(N)
     del N
-    f = form_metrics(co, per)
+    f = (metrics_fn or form_metrics)(co, per)
     cent = cell_centres(co)
     if ib is not None:
         inside = interior_only(ib(cent))
@@ -510,17 +513,17 @@ def build_case(name, nx, ny, nz, x, eta, z, *, hill_h=0.0, hill_s=0.5, top=2.0,
                 coor=co if keep_coords else None)
 
 
-def channel_case(nx=64, ny=64, nz=64, beta=2.0, seed=0, noise=0.05, keep_coords=False):
+def channel_case(nx=64, ny=64, nz=64, beta=2.0, seed=0, noise=0.05, keep_coords=False, metrics_fn=None):
     """Periodic channel 2pi x 2 x pi, tanh wall stretching in y, no IB."""
     x = np.arange(nx + 1) * (2 * np.pi / nx)
     z = np.arange(nz + 1) * (np.pi / nz)
     eta = 0.5 * (tanh_wall_nodes(ny, beta) + 1.0)
     return build_case(f"channel{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, top=2.0, seed=seed, noise=noise,
-                      keep_coords=keep_coords)
+                      keep_coords=keep_coords, metrics_fn=metrics_fn)
 
 
 def hill_case(nx=192, ny=96, nz=128, h=0.3, s=0.5, beta=0.0, sphere=(0.6, 0.75, 0.0, 0.15), seed=0,
-              noise=0.05, keep_coords=False):
+              noise=0.05, keep_coords=False, metrics_fn=None):
     """Gaussian hill under algebraic grid-line blending, optional sphere IB."""
     x = -3.0 + np.arange(nx + 1) * (6.0 / nx)
     z = -2.0 + np.arange(nz + 1) * (4.0 / nz)
@@ -530,11 +533,11 @@ def hill_case(nx=192, ny=96, nz=128, h=0.3, s=0.5, beta=0.0, sphere=(0.6, 0.75, 
         cx, cy, cz, r = sphere
         ib = lambda cent: inside_sphere(cent, (cx, cy, cz), r)
     return build_case(f"hill{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, hill_h=h, hill_s=s, top=2.0, ib=ib,
-                      seed=seed, noise=noise, keep_coords=keep_coords)
+                      seed=seed, noise=noise, keep_coords=keep_coords, metrics_fn=metrics_fn)
 
 
 def towers_case(nx=512, ny=256, nz=256, ratio=1.02, rows=3, cols=4, radius_cells=4, height=0.6, seed=0,
-                noise=0.05, keep_coords=False):
+                noise=0.05, keep_coords=False, metrics_fn=None):
     """Wind-farm-like box: uniform in x and z, geometric stretching in y, an array of
     cylindrical towers, log-law mean flow with 10% fluctuations."""
     Lz = 1.0
@@ -551,7 +554,7 @@ def towers_case(nx=512, ny=256, nz=256, ratio=1.02, rows=3, cols=4, radius_cells
     norm = math.log((1.0 + y0) / y0)
     prof = lambda yc: np.log((np.maximum(yc, 0.0) + y0) / y0) / norm
     return build_case(f"towers{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, top=1.0, ib=ib, seed=seed,
-                      noise=noise, mean_profile_fn=prof, fluct=0.1, keep_coords=keep_coords)
+                      noise=noise, mean_profile_fn=prof, fluct=0.1, keep_coords=keep_coords, metrics_fn=metrics_fn)
 
 
 def batch_sizes(count=64, lo=32, hi=128, jitter=0.25, seed=3):
@@ -566,7 +569,7 @@ def batch_sizes(count=64, lo=32, hi=128, jitter=0.25, seed=3):
     return out
 
 
-def ensemble_case(dims, beta, h, with_sphere, seed):
+def ensemble_case(dims, beta, h, with_sphere, seed, metrics_fn=None):
     nx, ny, nz = dims
     x = -3.0 + np.arange(nx + 1) * (6.0 / nx)
     z = -2.0 + np.arange(nz + 1) * (4.0 / nz)
@@ -576,28 +579,29 @@ def ensemble_case(dims, beta, h, with_sphere, seed):
         rng = np.random.default_rng(seed + 7)
         c = (rng.uniform(-2.0, 2.0), rng.uniform(0.8, 1.4), rng.uniform(-1.2, 1.2))
         ib = lambda cent: inside_sphere(cent, c, 0.15)
-    return build_case(f"ens{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, hill_h=h, top=2.0, ib=ib, seed=seed)
+    return build_case(f"ens{nx}x{ny}x{nz}", nx, ny, nz, x, eta, z, hill_h=h, top=2.0, ib=ib, seed=seed,
+                      metrics_fn=metrics_fn)
 
 
 FIELD_ORDER_VEC = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet")
 FIELD_ORDER_SCA = ("iaj", "jaj", "kaj", "aj")
 
 
-def served_case(kind, dims, seed):
+def served_case(kind, dims, seed, metrics_fn=None):
     """One case of the served stream: 'channel' (tanh-stretched periodic channel,
     no IB), 'hill' (terrain + IB sphere) or 'towers' (stretched box with an IB
     tower array), at the given interior extents."""
     nx, ny, nz = dims
     rng = np.random.default_rng(seed)
     if kind == "channel":
-        c = channel_case(nx, ny, nz, beta=float(rng.uniform(1.0, 3.0)), seed=seed)
+        c = channel_case(nx, ny, nz, beta=float(rng.uniform(1.0, 3.0)), seed=seed, metrics_fn=metrics_fn)
     elif kind == "hill":
         c = hill_case(nx, ny, nz, h=float(rng.uniform(0.1, 0.4)), s=0.5, beta=float(rng.uniform(0.0, 2.0)),
                       sphere=(float(rng.uniform(-1.5, 1.5)), float(rng.uniform(0.7, 1.3)), float(rng.uniform(-1.0, 1.0)),
-                              0.25), seed=seed)
+                              0.25), seed=seed, metrics_fn=metrics_fn)
     elif kind == "towers":
         c = towers_case(nx, ny, nz, ratio=1.0 + 4.0 / ny, rows=2, cols=3, radius_cells=max(2, nx // 48), height=0.6,
-                        seed=seed)
+                        seed=seed, metrics_fn=metrics_fn)
     else:
         raise ValueError(kind)
     c.name = f"{kind}{nx}x{ny}x{nz}"

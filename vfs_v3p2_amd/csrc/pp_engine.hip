@@ -1097,8 +1097,36 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
     HIPCHECK(hipMemsetAsync(uc, 0, 3 * M * sizeof(double), h->s));
   }
   int nb = 0;
+  // kernel 6: the finest sweep over rotating copies of its operands (HBM-resident)
+  if (kernel == 6) {
+    const int copies = 6;
+    *ms = h->mg->time_sweep_rotating(h->s, std::max(reps, copies), copies);
+    const double N = (double)n;
+    if (!h->mg->line0())
+      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
+    else
+      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4)
+                                           : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return VFS_PP_OK;
+  }
+  // kernel 7: the Krylov operator pass w = A z with the dots of w against kDots
+  // basis vectors and itself (k_apply_dots, z in the multigrid precision)
+  const int kDots = 4;
+  if (kernel == 7 && (!h->V || !h->Z)) return VFS_PP_EINVAL;
+  double* wv = kernel == 7 ? tmp.alloc<double>(n) : nullptr;
+  if (kernel == 7) need_partials(h, kDots + 1, std::max(4 * h->stencil_blocks, (n + 255) / 256));
   auto launch = [&]() {
     switch (kernel) {
+      case 7:
+        if (h->opts.precision == 1)
+          launch_apply_dots<float>(h->s, h->d, h->g64, h->code, static_cast<const float*>(h->Z), wv, h->V, n, kDots,
+                                   h->partial, &nb);
+        else
+          launch_apply_dots<double>(h->s, h->d, h->g64, h->code, static_cast<const double*>(h->Z), wv, h->V, n,
+                                    kDots, h->partial, &nb);
+        break;
       case 0: launch_apply64<double>(h->s, h->d, h->g64, h->code, h->x, y, false); break;
       case 1: h->mg->time_sweep(h->s); break;
       case 2: launch_projection(h->s, h->m, h->d, h->g64, h->code, h->cls, phi, u, 1.0); break;
@@ -1135,6 +1163,9 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
     case 3: *bytes = N * (24 + 72 + 1 + 24); break;      // Ucont, csi/eta/zet, mask -> Ucat = 121
     case 4: *bytes = N * (24 + 1 + 8); break;            // Ucont, mask -> b        = 33
     case 5: *bytes = (double)ncopy * 16.; break;         // read + write
+    case 7:  // z, g, mask, kDots basis vectors -> w
+      *bytes = N * ((h->opts.precision == 1 ? 4 : 8) + 72 + 1 + 8 * kDots + 8);
+      break;
   }
   return VFS_PP_OK;
   ABI_END

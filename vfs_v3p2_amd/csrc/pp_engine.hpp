@@ -11,6 +11,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pp_kernels.hpp"
@@ -113,6 +114,10 @@ struct MGBase {
   virtual void set_out(void* out) = 0;
   virtual std::string describe() const = 0;
   virtual void time_sweep(hipStream_t s) = 0;
+  // ms per finest-level sweep when `copies` copies of the level's operands are
+  // swept in turn (together larger than the 256 MB last-level cache, so every
+  // sweep streams its operands from HBM)
+  virtual double time_sweep_rotating(hipStream_t s, int reps, int copies) = 0;
   virtual int nlev() const = 0;
   // level l: dimensions, smoother, code words; its face coefficients as FP64 into
   // g64 (device, 9 n)
@@ -237,6 +242,52 @@ struct MG : MGBase {
     Lev& L = lev[0];
     launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.xb, L.dv, scr,
                      L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
+  }
+
+  double time_sweep_rotating(hipStream_t s, int reps, int copies) override {
+    const Lev& L0 = lev[0];
+    const size_t n = L0.d.n();
+    Arena tmp;
+    auto dup = [&](auto* src, size_t len) {
+      using E = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
+      if (!src) return static_cast<E*>(nullptr);
+      E* d = tmp.alloc<E>(len);
+      HIPCHECK(hipMemcpyAsync(d, src, len * sizeof(E), hipMemcpyDeviceToDevice, s));
+      return d;
+    };
+    std::vector<Lev> cp(copies, L0);
+    std::vector<T*> scrs(copies);
+    for (auto& C : cp) {
+      C.g = dup(L0.g, 9 * n);
+      C.gh = dup(L0.gh, 9 * n);
+      C.code = dup(L0.code, n);
+      C.cls = dup(L0.cls, n);
+      C.f = dup(L0.f, n);
+      C.xa = dup(L0.xa, n);
+      C.xb = dup(L0.xb, n);
+      C.dv = dup(L0.dv, n);
+      C.dinv = dup(L0.dinv, n);
+      C.lfac = dup(L0.lfac, 3 * n);
+    }
+    for (auto& sc : scrs) sc = tmp.alloc<T>(2 * n);
+    auto sweep = [&](int q) {
+      Lev& C = cp[q];
+      launch_smooth<T>(s, C.view(), C.lfac, SM_NEXT, C.f, C.dinv, C.xa, C.xb, C.xb, C.dv, scrs[q],
+                       C.c1.size() > 1 ? C.c1[1] : T(0), C.c2[0], T(0));
+    };
+    for (int q = 0; q < copies; q++) sweep(q);
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) sweep(r % copies);
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
   }
 };
 

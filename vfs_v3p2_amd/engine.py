@@ -596,6 +596,27 @@ def form_metrics_device(grid: Grid, coor):
     return out
 
 
+def device_metrics_fn(device="cuda"):
+    """A metrics function for cases.build_case that runs FormMetrics on the device
+    (bit-identical to cases.form_metrics) and returns host arrays: generating a
+    large synthetic case then costs one PCIe round trip instead of the numpy pass."""
+    import torch
+
+    def fn(co, per):
+        mz, my, mx, _ = co.shape
+        g = make_grid(mx - 2, my - 2, mz - 2, per)
+        coor = torch.from_numpy(np.ascontiguousarray(co, dtype=np.float64)).to(device)
+        out = form_metrics_device(g, coor)
+        del coor
+        host = {}
+        for k in METRIC_FIELDS:
+            host[k] = out.pop(k).cpu().numpy()
+        torch.cuda.empty_cache()
+        return host
+
+    return fn
+
+
 def Pressure_Gradient(grid: Grid, metrics: dict, nvert, p):
     """momentum.c:223 on the device: dP (mz, my, mx, 3) from P (periodic ghosts of P
     are filled in place) and the face metrics (device tensors keyed icsi..kaj, as
