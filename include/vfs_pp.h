@@ -52,7 +52,14 @@ typedef struct vfs_pp_handle vfs_pp_handle;
 /* Grid description: extents of interior cells and the reference's flags. */
 typedef struct {
   int nx, ny, nz;   /* interior cells; DMDA extents are nx+2 etc. (init.c:156-157) */
-  int periodic[3];  /* i_periodic|ii_periodic, j..., k... (main.c:2150-2160) */
+  int periodic[3];  /* i_periodic || ii_periodic, j..., k... (main.c:2150-2160): both
+                       wrap the same way on the dense [mz][my][mx] arrays every entry
+                       point takes (ii_periodic's ghost index -2 is the point mx-2 that
+                       i_periodic reads, poisson_hypre.c:358-359), so one flag serves
+                       both.  Under ii_periodic the reference's ghosted local arrays are
+                       3 points wider on that axis (DMDA_BOUNDARY_PERIODIC, stencil
+                       width 3, init.c:147-158): pass the global vectors' arrays, dense
+                       on one rank, or repack the local ones with vfs_pp_repack. */
   int bctype[6];    /* user->bctype (init.c:502); Contra2Cart honours 1 (no-slip) */
   int immersed;     /* the reference's `immersed` switch (poisson_hypre.c:662-667):
                        0 or 3: VolumeFlux(user,Ucont,..) poisson.c:2634, corrected faces
@@ -157,6 +164,17 @@ int vfs_pp_pressure_gradient_device(const vfs_grid *grid, const double *icsi, co
 int vfs_pp_petsc_vec_info(const char *path, long long *n);
 int vfs_pp_read_petsc_vec_device(const char *path, double *v, size_t n);
 int vfs_pp_write_petsc_vec_device(const char *path, const double *v, size_t n, int block_size);
+
+/* Host-memory repack between a ghosted local DMDA array and the dense block the
+ * entry points take.  The ghosted array covers, per axis a (0 = i, fastest),
+ * indices ghost_start[a] .. ghost_start[a] + ghost_dims[a] - 1 (DMDAGetGhostCorners:
+ * ghost_start = -3 and ghost_dims = m + 6 on an axis with DMDA_BOUNDARY_PERIODIC
+ * and stencil width 3, init.c:147-158; 0 and m otherwise), ncomp doubles per
+ * point (3 for Cmpnts).  to_dense = 1 copies its [0, dims) window into the dense
+ * dst; 0 copies the dense src into that window of dst, leaving its ghost layers.
+ * VFS_PP_EINVAL unless the window lies inside the ghosted range.  No device work. */
+int vfs_pp_repack(const double *src, double *dst, const int ghost_start[3], const int ghost_dims[3],
+                  const int dims[3], int ncomp, int to_dense);
 
 /* Immersed-body surfaces (the ibmdataNN files of ibm_io.c:78-300): vertex and
  * triangle counts from the header. */

@@ -139,3 +139,35 @@ def test_create_refuses_unsupported_immersed_values():
         h = C.c_void_p()
         rc = E.lib().vfs_pp_create(C.byref(E.make_grid(8, 6, 4, immersed=bad)), None, C.byref(h))
         assert rc == E.VFS_PP_EINVAL and not h.value, bad
+
+
+def test_repack_of_a_periodic_ghosted_array():
+    """Under ii_periodic (or jj/kk) the reference's local arrays are ghosted by 3
+    points on that axis (DMDA_BOUNDARY_PERIODIC, stencil width 3, init.c:147-158):
+    index -2 holds the image of m-2.  vfs_pp_repack takes such an array (here
+    built by periodic padding of a dense block) back to the dense block, and the
+    dense block back into the window of a ghosted one; misaligned windows are
+    refused."""
+    E = _ensure_built()
+    rng = np.random.default_rng(0)
+    mz, my, mx = 5, 6, 7
+    dense = rng.standard_normal((mz, my, mx, 3))
+    # ii_periodic and kk_periodic: 3 ghost points on i and k, none on j
+    ghosted = np.pad(dense, ((3, 3), (0, 0), (3, 3), (0, 0)), mode="wrap")
+    assert np.array_equal(ghosted[3:-3, :, 3 - 2], dense[:, :, mx - 2])  # index -2 is the image of mx-2
+    got = E.repack_to_dense(ghosted, (-3, 0, -3), (mx, my, mz))
+    np.testing.assert_array_equal(got, dense)
+    scalar = rng.standard_normal((mz, my, mx))
+    np.testing.assert_array_equal(
+        E.repack_to_dense(np.pad(scalar, ((0, 0), (3, 3), (0, 0)), mode="wrap"), (0, -3, 0), (mx, my, mz)), scalar)
+    # dense -> ghosted window, ghost layers untouched
+    back = np.full(ghosted.shape, 7.0)
+    gd = (C.c_int * 3)(ghosted.shape[2], ghosted.shape[1], ghosted.shape[0])
+    rc = E.lib().vfs_pp_repack(dense.ctypes.data_as(C.c_void_p), back.ctypes.data_as(C.c_void_p),
+                               (C.c_int * 3)(-3, 0, -3), gd, (C.c_int * 3)(mx, my, mz), 3, 0)
+    assert rc == 0
+    np.testing.assert_array_equal(back[3:-3, :, 3:-3], dense)
+    assert np.all(back[:3] == 7.0) and np.all(back[:, :, :3] == 7.0)
+    for start, dims in (((1, 0, -3), (mx, my, mz)), ((-3, 0, -3), (mx + 4, my, mz))):
+        with pytest.raises(E.VfsPPError):  # the window does not lie inside the ghosted range
+            E.repack_to_dense(ghosted, start, dims)

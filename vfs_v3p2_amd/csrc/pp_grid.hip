@@ -398,3 +398,26 @@ int vfs_pp_write_petsc_vec_device(const char* path, const double* v, size_t n, i
   return VFS_PP_OK;
   ABI_END
 }
+
+// The [0, m) window of a ghosted local DMDA array and the dense [mz][my][mx]
+// block the engine takes (host memory, ncomp values per point, i fastest).
+int vfs_pp_repack(const double* src, double* dst, const int ghost_start[3], const int ghost_dims[3],
+                  const int dims[3], int ncomp, int to_dense) {
+  if (!src || !dst || !ghost_start || !ghost_dims || !dims || ncomp < 1) return VFS_PP_EINVAL;
+  for (int a = 0; a < 3; a++) {
+    if (dims[a] < 1 || ghost_dims[a] < 1) return VFS_PP_EINVAL;
+    // the window [0, dims) must lie inside [ghost_start, ghost_start + ghost_dims)
+    if (ghost_start[a] > 0 || ghost_start[a] + ghost_dims[a] < dims[a]) return VFS_PP_EINVAL;
+  }
+  const size_t gx = (size_t)ghost_dims[0], gy = (size_t)ghost_dims[1];
+  const size_t row = (size_t)dims[0] * ncomp;
+  for (int k = 0; k < dims[2]; k++)
+    for (int j = 0; j < dims[1]; j++) {
+      const size_t gofs = ((((size_t)(k - ghost_start[2]) * gy) + (size_t)(j - ghost_start[1])) * gx +
+                           (size_t)(-ghost_start[0])) * ncomp;
+      const size_t dofs = (((size_t)k * dims[1]) + j) * row;
+      if (to_dense) std::memcpy(dst + dofs, src + gofs, row * sizeof(double));
+      else std::memcpy(dst + gofs, src + dofs, row * sizeof(double));
+    }
+  return VFS_PP_OK;
+}

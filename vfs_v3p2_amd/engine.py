@@ -100,6 +100,8 @@ def lib():
         "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
         "vfs_pp_divergence_device": (C.c_int, [H, vp, dp]),
         "vfs_pp_divergence_noalloc": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
+        "vfs_pp_repack": (C.c_int, [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                    C.c_int]),
         "vfs_pp_project_device": (C.c_int, [H, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(Stats)]),
         "vfs_pp_project": (C.c_int, [H, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Stats)]),
@@ -526,6 +528,21 @@ def Projection(user: UserCtx):
 def Divergence(user: UserCtx) -> float:
     """Divergence (main.c:1244): max |aj div u| over cells not next to the IB."""
     return user.handle.divergence(user.Ucont)
+
+
+def repack_to_dense(ghosted: np.ndarray, ghost_start, dims) -> np.ndarray:
+    """The [0, m) window of a ghosted local DMDA array (numpy, shape (gz, gy, gx[, c])
+    with ghost corners ghost_start = (gxs, gys, gzs)) as the dense block
+    (mz, my, mx[, c]) the engine takes (vfs_pp_repack)."""
+    g = np.ascontiguousarray(ghosted, dtype=np.float64)
+    nc = g.shape[3] if g.ndim == 4 else 1
+    gd = (C.c_int * 3)(g.shape[2], g.shape[1], g.shape[0])
+    gs = (C.c_int * 3)(*ghost_start)
+    dm = (C.c_int * 3)(*dims)
+    out = np.zeros((dims[2], dims[1], dims[0]) + ((nc,) if g.ndim == 4 else ()))
+    _check(lib().vfs_pp_repack(g.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), gs, gd, dm, nc, 1),
+           "vfs_pp_repack")
+    return out
 
 
 def divergence_noalloc(grid: Grid, ucont, aj, nvert, out) -> float:
