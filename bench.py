@@ -66,9 +66,9 @@ def parse():
     ap.add_argument("--rates", type=str, default="10,50,100,200,500", help="serve form: arrival rates (1/s)")
     ap.add_argument("--cases", type=int, default=64, help="batch form: number of cases")
     ap.add_argument("--workers", type=int, default=16, help="host processes generating the batched and served cases")
-    ap.add_argument("--time-kernel", type=int, default=None,
-                    help="profiling aid: set up the 192x96x128 case and only time engine kernel ID "
-                         "(vfs_pp_time_kernel), printing its ms and GB/s")
+    ap.add_argument("--time-kernel", type=str, default=None,
+                    help="profiling aid: set up the 192x96x128 case and only time the engine kernels with "
+                         "these comma-separated ids (vfs_pp_time_kernel), printing their ms and GB/s")
     ap.add_argument("--single-only", action="store_true",
                     help="default run: only the 192x96x128 projection (no towers, batch, serve or host-array blocks)")
     ap.add_argument("--block-steps", type=int, default=8,
@@ -545,10 +545,11 @@ def main():
         h = E.Handle(E.make_grid(case.nx, case.ny, case.nz, case.per, case.bctype, case.immersed),
                      E.default_opts(precision=1, rtol=RTOL, div_tol=DIV_TOL))
         h.set_geometry(case.fields)
-        ms, by = h.time_kernel(args.time_kernel, 100)
+        for kid in [int(x) for x in args.time_kernel.split(",") if x]:
+            ms, by = h.time_kernel(kid, 100 if kid != 5 else 20)
+            print(json.dumps({"kernel": KERNELS.get(kid, kid), "ms_per_launch": ms, "algorithmic_bytes": by,
+                              "gbs": round(by / (ms * 1e-3) / 1e9, 1)}), flush=True)
         h.close()
-        print(json.dumps({"kernel": KERNELS.get(args.time_kernel, args.time_kernel), "ms_per_launch": ms,
-                          "algorithmic_bytes": by, "gbs": round(by / (ms * 1e-3) / 1e9, 1)}), flush=True)
         return
 
     h_mx, mx = run_form(E, torch, case, 1, args.steps, args.warmup)

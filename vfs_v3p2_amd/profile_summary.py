@@ -14,6 +14,9 @@ Batch mode keeps the batched kernels (kb_*, in the multigrid precision given by
 --precision), splits them into calls at each batched RHS step (the first launch
 of a call), and breaks down the last call by kernel.
 
+Counter mode (--counters a.csv b.csv --kernel NAME --cells N) prints the mean per
+dispatch (and per wave, given SQ_WAVES) of every counter those runs collected.
+
 Traffic mode reads one FETCH_SIZE and one WRITE_SIZE pass (KiB per dispatch),
 scales each by what the same counter reads for the 1 GiB device copy kernel
 (k_copy16: 1 GiB read, 1 GiB written), and prints HBM-side bytes per launch and
@@ -87,6 +90,27 @@ def traffic(fetch_csv, write_csv, cells, top):
         print(f"\ntraffic_bytes_per_launch: {t:.0f}")
 
 
+def counters(paths, kernel, cells):
+    """Mean per dispatch of every counter in the given --pmc runs, for the
+    dispatches of `kernel` (short name) covering `cells` work-items, and per wave
+    when SQ_WAVES is among them."""
+    vals = {}
+    for p in paths:
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(p)):
+            if _short(r["Kernel_Name"]) == kernel and int(r["Grid_Size"]) == cells:
+                acc[r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for k, v in acc.items():
+            vals[k] = (sum(v) / len(v), len(v))
+    waves = vals.get("SQ_WAVES", (0.0, 0))[0]
+    print(f"| counter ({kernel}, {cells} work-items) | dispatches | mean per dispatch | per wave |")
+    print("|---|---|---|---|")
+    for k in sorted(vals):
+        m, n = vals[k]
+        pw = f"{m / waves:.1f}" if waves else "-"
+        print(f"| {k} | {n} | {m:.0f} | {pw} |")
+
+
 RHS_STEP = "kb_phase<2, "  # the batched RHS step: the first launch of every batched call
 
 
@@ -128,7 +152,12 @@ def main():
     ap.add_argument("--write", default=None)
     ap.add_argument("--cells", type=int, default=192 * 96 * 128)
     ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--counters", nargs="*", default=None, help="--pmc counter CSVs (with --kernel)")
+    ap.add_argument("--kernel", default=SMOOTHER)
     a = ap.parse_args()
+    if a.counters:
+        counters(a.counters, a.kernel, a.cells)
+        return
     if a.fetch:
         traffic(a.fetch, a.write, a.cells, a.top)
         return
