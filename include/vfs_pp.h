@@ -153,6 +153,41 @@ int vfs_pp_pressure_gradient_device(const vfs_grid *grid, const double *icsi, co
                                     const double *kzet, const double *iaj, const double *jaj,
                                     const double *kaj, const double *nvert, double *p, double *dp);
 
+/* Formfunction_2 (momentum.c:529-2034), the momentum right-hand side of the
+ * step before the projection, on device DMDA arrays: convective flux by
+ * fourth-order central interpolation (one-sided upwind beside IB points when
+ * grid->immersed != 0; zero IB-face convection when it is 3), laminar viscous
+ * flux with nu = 1/ren, in contravariant form, times scale; zero on IB faces
+ * and boundary points.  Ucont is read with whole-vector periodic ghosts and
+ * not modified.  The reference's default branch: no LES/RANS, level set, skew
+ * form, second-order or inviscid switch, no bctype 11 force sums.  Allocates
+ * 8 vector-field scratch arrays per call; bit-exact with the reference's
+ * arithmetic.  Runs on the legacy default stream and returns when done. */
+int vfs_pp_formfunction2_device(const vfs_grid *grid, const double *icsi, const double *ieta,
+                                const double *izet, const double *jcsi, const double *jeta,
+                                const double *jzet, const double *kcsi, const double *keta,
+                                const double *kzet, const double *iaj, const double *jaj,
+                                const double *kaj, const double *csi, const double *eta,
+                                const double *zet, const double *nvert, const double *ucont,
+                                const double *ucat, double ren, double scale, double *rhs);
+
+/* RungeKutta (rhs.c:1511-1631): one explicit four-stage step, stage s setting
+ * Ucont = pUcont + alfa_s dt st (Formfunction_2(Ucont, Ucat) - dP) with
+ * alfa = 1/4, 1/3, 1/2, 1 and Contra2Cart before and after, on the handle's
+ * grid, nvert and cell metrics (set up as for the projection) with the face and
+ * cell metrics passed here.  ucont and ucat (device, DMDA) are updated in
+ * place; dp is Pressure_Gradient's output (vfs_pp_pressure_gradient_device).
+ * The inflow/outflow boundary (InflowFlux, FormBCS) and the IB velocity
+ * interpolation (ibm_interpolation_advanced) are the caller's: Contra2Cart
+ * leaves the Ucat of IB-adjacent cells as passed.  Allocates 10 vector-field
+ * scratch arrays per call. */
+int vfs_pp_runge_kutta_device(vfs_pp_handle *h, const double *icsi, const double *ieta, const double *izet,
+                              const double *jcsi, const double *jeta, const double *jzet,
+                              const double *kcsi, const double *keta, const double *kzet,
+                              const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                              const double *eta, const double *zet, const double *dp, double *ucont,
+                              double *ucat, double ren, double dt, double st);
+
 /* PETSc binary vector files, the form of the vfield / pfield / nvfield / ufield
  * checkpoints (VecView / VecLoad on a binary viewer, main.c:291-335, 780-797,
  * 865-935): big-endian int32 class id 1211214, big-endian int32 length, then

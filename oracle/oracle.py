@@ -63,6 +63,8 @@ def lib():
         _lib.or_ib_search.restype = C.c_int
         _lib.or_ib_search.argtypes = [g, ip, d, C.c_int, d, d, d, C.c_int, ip, ip, ip, C.c_int, C.c_int, d]
         _lib.or_pressure_gradient.argtypes = [g] + [d] * 15
+        _lib.or_formfunction2.argtypes = [g, ip, C.c_int] + [d] * 18 + [C.c_double, C.c_double, d]
+        _lib.or_runge_kutta.argtypes = [g, ip, C.c_int] + [d] * 19 + [C.c_double, C.c_double, C.c_double]
         _lib.or_ray_s.restype = C.c_double
         _lib.or_ray_s.argtypes = [C.c_int]
         _lib.or_level_coupling.argtypes = [g, d, d, d, d]
@@ -399,3 +401,33 @@ def pressure_gradient(g, f, p):
                                               "iaj", "jaj", "kaj", "nvert")]
     lib().or_pressure_gradient(C.byref(g), *[_p(x) for x in a], _p(pp), _p(dp))
     return dp, pp
+
+
+MOMENTUM_METRICS = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj",
+                    "csi", "eta", "zet", "nvert")
+
+
+def formfunction2(g, bctype, immersed, f, ucont, ucat, ren, scale=1.0):
+    """or_formfunction2: the momentum right-hand side (mz, my, mx, 3) from Ucont,
+    Ucat and the metrics in f (Formfunction_2, momentum.c:529-2034)."""
+    a = [np.ascontiguousarray(f[k], dtype=np.float64) for k in MOMENTUM_METRICS]
+    u = np.ascontiguousarray(ucont, dtype=np.float64)
+    uc = np.ascontiguousarray(ucat, dtype=np.float64)
+    rhs = np.zeros(u.shape)
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    lib().or_formfunction2(C.byref(g), _p(bc, C.c_int), int(immersed), *[_p(x) for x in a], _p(u), _p(uc),
+                           float(ren), float(scale), _p(rhs))
+    return rhs
+
+
+def runge_kutta(g, bctype, immersed, f, dp, ucont, ucat, ren, dt, st):
+    """or_runge_kutta: one explicit four-stage step (rhs.c:1511-1631); returns
+    (Ucont, Ucat) after it, inputs untouched."""
+    a = [np.ascontiguousarray(f[k], dtype=np.float64) for k in MOMENTUM_METRICS]
+    u = np.ascontiguousarray(ucont, dtype=np.float64).copy()
+    uc = np.ascontiguousarray(ucat, dtype=np.float64).copy()
+    d = np.ascontiguousarray(dp, dtype=np.float64)
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    lib().or_runge_kutta(C.byref(g), _p(bc, C.c_int), int(immersed), *[_p(x) for x in a], _p(d), _p(u), _p(uc),
+                         float(ren), float(dt), float(st))
+    return u, uc

@@ -14,7 +14,8 @@
  * per[d] != 0 means the axis is periodic (i_periodic or ii_periodic: plane 0
  * aliases plane m-2 and plane m-1 aliases plane 1).
  *
- * Parity status of each function is listed in oracle/README_ORACLE.md.
+ * Each function's comment names the reference lines it restates; tests/test_oracle.py
+ * and tests/test_momentum.py pin them by closed forms, invariants and brute force.
  */
 #ifndef VFS_ORACLE_H
 #define VFS_ORACLE_H
@@ -171,6 +172,26 @@ void or_mg_vcycle(or_mg *mg, const double *r, double *z);
 size_t or_mg_n(const or_mg *mg);
 int or_mg_levels(const or_mg *mg);
 int or_mg_fgmres(or_mg *mg, const double *b, double *x, int m, double rtol, int maxit, double *rel, int *vcyc);
+
+/* Formfunction_2 (momentum.c:529-2034), default branch (no LES/RANS/level set,
+ * no skew form, fourth-order central convection): the momentum right-hand side
+ * Rhs (AoS, DMDA) from Ucont (read with whole-vector periodic ghosts, not
+ * modified) and Ucat, with nu = 1/ren, times `scale`. */
+void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
+                      const double *ieta, const double *izet, const double *jcsi, const double *jeta,
+                      const double *jzet, const double *kcsi, const double *keta, const double *kzet,
+                      const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                      const double *eta, const double *zet, const double *nvert, const double *ucont,
+                      const double *ucat, double ren, double scale, double *rhs);
+
+/* RungeKutta (rhs.c:1511-1631): four stages Ucont = pUcont + alfa dt st
+ * (Formfunction_2 - dP), each between Contra2Cart calls; ucont and ucat in/out. */
+void or_runge_kutta(const or_grid *g, const int bctype[6], int immersed, const double *icsi, const double *ieta,
+                    const double *izet, const double *jcsi, const double *jeta, const double *jzet,
+                    const double *kcsi, const double *keta, const double *kzet, const double *iaj,
+                    const double *jaj, const double *kaj, const double *csi, const double *eta, const double *zet,
+                    const double *nvert, const double *dp, double *ucont, double *ucat, double ren, double dt,
+                    double st);
 
 #ifdef __cplusplus
 }

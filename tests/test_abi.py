@@ -57,6 +57,25 @@ def test_python_mirror_binds_every_symbol():
         assert getattr(L, n).argtypes is not None or n == "vfs_pp_error_string", n
 
 
+def declared_arities():
+    """Number of parameters of every function the header declares."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(vfs_pp_\w+)\s*\(([^)]*)\)\s*;", text):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_python_mirror_arities_match_header():
+    E = _ensure_built()
+    L = E.lib()
+    ar = declared_arities()
+    assert set(ar) == set(declared_functions())
+    for n, k in ar.items():
+        assert len(getattr(L, n).argtypes) == k, n
+
+
 PROBE = r"""
 #include <stdio.h>
 #include <stddef.h>

@@ -1,0 +1,150 @@
+"""Pins of the oracle's momentum predictor (Formfunction_2, momentum.c:529-2034;
+RungeKutta, rhs.c:1511-1631) by closed forms on sheared affine grids, where every
+metric is constant and every cross metric is non-zero:
+
+- a uniform flow has no momentum right-hand side;
+- with no convecting flux, a quadratic velocity gets exactly nu times its
+  Laplacian (the scheme's second-order viscous differences, cross derivatives
+  and transpose terms are exact for quadratics);
+- with a uniform convecting flux and (nearly) no viscosity, a cubic velocity
+  gets exactly -(U0 . grad) u: the (-1, 9, 9, -1)/16 face interpolation and the
+  (9/8, -1/8) combination of the two cell differences are exact for cubics, and
+  any other weight, sign or swap is not;
+- beside an IB block, the one-sided face values of a linear field that the
+  upwind branch takes and the shortened stencils stay exact;
+- four Runge-Kutta stages of a uniform flow under a uniform pressure-gradient
+  flux end at Ucont - dt st dP.
+The GPU kernels are compared with these functions bit for bit in
+tests/test_gpu_momentum.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vfs_v3p2_amd import cases
+
+M = np.array([[1.0, 0.35, -0.2], [0.15, 0.8, 0.1], [-0.25, 0.3, 1.2]])
+
+
+def sheared(nx, ny, nz, per=(0, 0, 0)):
+    """(grid, metrics, cell centres) of the affine map M on [0,1]^3; the centres of
+    every DMDA point, ghost planes included, are the mapped index centres, so a
+    polynomial velocity is the same polynomial along every grid line."""
+    P = np.stack(np.meshgrid(np.arange(nx + 1) / nx, np.arange(ny + 1) / ny, np.arange(nz + 1) / nz,
+                             indexing="ij"), -1).transpose(2, 1, 0, 3)
+    co = cases.dmda_coords(P @ M.T)
+    f = cases.form_metrics(co, per)
+    f["nvert"] = np.zeros(f["aj"].shape)
+    Q = np.stack(np.meshgrid((np.arange(nx + 2) - 0.5) / nx, (np.arange(ny + 2) - 0.5) / ny,
+                             (np.arange(nz + 2) - 0.5) / nz, indexing="ij"), -1).transpose(2, 1, 0, 3)
+    return O.grid(f["aj"].shape, per), f, Q @ M.T
+
+
+def uniform_flux(f, U):
+    u = np.zeros(f["aj"].shape + (3,))
+    u[..., 0] = f["icsi"] @ U
+    u[..., 1] = f["jeta"] @ U
+    u[..., 2] = f["kzet"] @ U
+    return u
+
+
+def expected_rhs(f, fp):
+    """rhs_d = 0.5 (a_d . fp_p + a_d . fp_{p+e_d}) / vol with a_d = csi, eta, zet."""
+    out = np.zeros(fp.shape)
+    for d, nm in enumerate(("csi", "eta", "zet")):
+        a = f[nm]
+        nxt = np.roll(fp, -1, axis=2 - d)
+        out[..., d] = 0.5 * ((a * fp).sum(-1) + (a * nxt).sum(-1)) * f["aj"]
+    return out
+
+
+# cells whose own and upper neighbours' fourth-order stencils stay off the walls
+DEEP = (slice(3, -5), slice(3, -5), slice(3, -5))
+
+
+def test_uniform_flow_has_no_rhs():
+    g, f, cent = sheared(11, 10, 12)
+    U0 = np.array([0.8, -0.4, 0.3])
+    rhs = O.formfunction2(g, (0,) * 6, 0, f, uniform_flux(f, U0), np.broadcast_to(U0, f["aj"].shape + (3,)),
+                          ren=100.0)
+    scale = np.abs(uniform_flux(f, U0)).max() * np.abs(U0).max()
+    assert np.abs(rhs).max() <= 1e-13 * scale
+
+
+def test_viscous_term_of_a_quadratic_velocity():
+    g, f, cent = sheared(12, 11, 13)
+    y, z = cent[..., 1], cent[..., 2]
+    ucat = np.zeros(cent.shape)
+    ucat[..., 0] = y * y + 0.5 * y * z + z * z  # div u = 0, (u . grad) u = 0, lap u = (4, 0, 0)
+    ren = 50.0
+    rhs = O.formfunction2(g, (0,) * 6, 0, f, np.zeros(cent.shape), ucat, ren=ren)
+    lap = np.zeros(cent.shape)
+    lap[..., 0] = 4.0 / ren
+    fp = lap / f["aj"][..., None]
+    want = expected_rhs(f, fp)
+    np.testing.assert_allclose(rhs[DEEP], want[DEEP], rtol=0, atol=1e-11 * np.abs(want).max())
+    # a transposed gradient term would add grad(div u)-like parts in y and z: they stay zero
+    assert np.abs(rhs[DEEP][..., 0]).max() > 0
+
+
+def test_convective_term_of_a_cubic_velocity():
+    g, f, cent = sheared(14, 13, 12)
+    U0 = np.array([0.7, -0.5, 0.4])
+    w = np.array([0.9, -0.6, 0.5])
+    L = np.array([[0.3, -0.2, 0.5], [0.1, 0.4, -0.3], [-0.6, 0.2, 0.25]])
+    s = cent @ w
+    ucat = cent @ L.T + np.array([1.0, -2.0, 0.5]) * (s ** 3)[..., None]
+    # (U0 . grad) u = L U0 + 3 s^2 (w . U0) b
+    b = np.array([1.0, -2.0, 0.5])
+    conv = (L @ U0) + 3.0 * (s ** 2)[..., None] * (w @ U0) * b
+    fp = -conv / f["aj"][..., None]
+    rhs = O.formfunction2(g, (0,) * 6, 0, f, uniform_flux(f, U0), ucat, ren=1e300)
+    want = expected_rhs(f, fp)
+    np.testing.assert_allclose(rhs[DEEP], want[DEEP], rtol=0, atol=1e-11 * np.abs(want[DEEP]).max())
+
+
+def test_upwind_faces_beside_an_ib_block():
+    g, f, cent = sheared(14, 12, 13)
+    nv = f["nvert"]
+    nv[6:9, 5:8, 7:10] = 1.0  # an IB block (k, j, i)
+    U0 = np.array([0.6, 0.5, 0.45])  # positive fluxes on every face family
+    L = np.array([[0.3, -0.2, 0.5], [0.1, 0.4, -0.3], [-0.6, 0.2, 0.25]])
+    ucat = cent @ L.T
+    rhs = O.formfunction2(g, (0,) * 6, 1, f, uniform_flux(f, U0), ucat, ren=1e300)
+    fp = np.broadcast_to(-(L @ U0), cent.shape) / f["aj"][..., None]
+    want = expected_rhs(f, fp)
+    # exact cells: fluid, with no IB point just below them on any axis (the faces
+    # there would take the other, first-order upwind form), away from the walls
+    fl = nv < 0.1
+    ok = fl.copy()
+    for ax in range(3):
+        ok &= np.roll(fl, 1, axis=ax)
+    ok_face = np.zeros(cent.shape, bool)
+    for d in range(3):
+        nb = np.roll(ok, -1, axis=2 - d) & np.roll(fl, -1, axis=2 - d)
+        ok_face[..., d] = ok & nb
+    inner = np.zeros(cent.shape, bool)
+    inner[DEEP] = True
+    m = ok_face & inner
+    beside = np.zeros(cent.shape[:3], bool)
+    beside[5:8, 4:7, 6:9] = True  # fluid cells whose upper neighbours include the block
+    assert (m & beside[..., None]).sum() > 10
+    np.testing.assert_allclose(rhs[m], want[m], rtol=0, atol=1e-11 * np.abs(want[m]).max())
+    # IB faces (either side IB) carry no right-hand side
+    for d in range(3):
+        blk = (nv + np.roll(nv, -1, axis=2 - d)) > 0.1
+        assert np.all(rhs[..., d][blk & inner[..., d]] == 0)
+
+
+def test_runge_kutta_of_a_uniform_flow_under_a_uniform_gradient_flux():
+    g, f, cent = sheared(10, 9, 11, per=(1, 1, 1))
+    U0 = np.array([0.8, -0.4, 0.3])
+    dU = np.array([0.3, 0.2, -0.5])
+    u0 = uniform_flux(f, U0)
+    dp = uniform_flux(f, dU)
+    ucat = np.broadcast_to(U0, cent.shape).copy()
+    dt, st = 0.05, 0.8
+    u1, uc1 = O.runge_kutta(g, (0,) * 6, 0, f, dp, u0, ucat, ren=100.0, dt=dt, st=st)
+    np.testing.assert_allclose(u1, u0 - dt * st * dp, rtol=0, atol=1e-13 * np.abs(u0).max())
+    np.testing.assert_allclose(uc1[1:-1, 1:-1, 1:-1], np.broadcast_to(U0 - dt * st * dU, uc1[1:-1, 1:-1, 1:-1].shape),
+                               rtol=0, atol=1e-12)

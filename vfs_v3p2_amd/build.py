@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvfs_pp.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["pp_core.hip", "pp_mg.hip", "pp_setup.hip", "pp_engine.hip", "pp_batch.hip", "pp_metrics.hip", "pp_grid.hip", "pp_ibsearch.hip", "pp_gradient.hip", "pp_noalloc.hip"]
+SOURCES = ["pp_core.hip", "pp_mg.hip", "pp_setup.hip", "pp_engine.hip", "pp_batch.hip", "pp_metrics.hip", "pp_grid.hip", "pp_ibsearch.hip", "pp_gradient.hip", "pp_noalloc.hip", "pp_momentum.hip"]
 
 ARCH = os.environ.get("VFS_PP_ARCH", "gfx950")
 FLAGS = [

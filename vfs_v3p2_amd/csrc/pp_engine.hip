@@ -718,6 +718,7 @@ int vfs_pp_create(const vfs_grid* g, const vfs_pp_opts* o, vfs_pp_handle** out) 
   auto h = std::make_unique<vfs_pp_handle>();
   h->grid = *g;
   h->grid.immersed = g->immersed == 1 ? 1 : 0;
+  h->immersed_ref = g->immersed;
   if (o) h->opts = *o;
   else vfs_pp_default_opts(&h->opts);
   if (!(h->opts.rtol > 0.) || h->opts.max_iters < 0 || h->opts.restart < 1) return VFS_PP_EINVAL;

@@ -305,6 +305,7 @@ struct vfs_pp_handle {
   // device bytes the handle holds: grid-sized buffers, the geometry's and set-up scratch
   size_t resident_bytes() const { return ar.total() + geo.total() + setup_tmp.total(); }
   int batch_refs = 0;         // batches / server slots the handle is bound to (0 or 1)
+  int immersed_ref = 0;       // the caller's `immersed` as given (grid.immersed holds the VolumeFlux variant)
   bool have_geom = false;     // metrics, masks and hierarchy are in place
   bool have_metrics = false;  // face coefficients and cell metrics are in place
   // fine-level resident data

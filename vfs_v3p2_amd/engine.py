@@ -85,6 +85,8 @@ def lib():
         "vfs_pp_form_metrics_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 17),
         "vfs_pp_set_grid_device": (C.c_int, [H, vp, vp]),
         "vfs_pp_pressure_gradient_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 15),
+        "vfs_pp_formfunction2_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 18 + [C.c_double, C.c_double, vp]),
+        "vfs_pp_runge_kutta_device": (C.c_int, [H] + [vp] * 18 + [C.c_double, C.c_double, C.c_double]),
         "vfs_pp_petsc_vec_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_longlong)]),
         "vfs_pp_read_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t]),
         "vfs_pp_write_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t, C.c_int]),
@@ -647,6 +649,39 @@ def Pressure_Gradient(grid: Grid, metrics: dict, nvert, p):
     _check(lib().vfs_pp_pressure_gradient_device(C.byref(grid), *ptrs, _tptr(nvert, S, "nvert"), _tptr(p, S, "p"),
                                                  _tptr(dp, V, "dp")), "vfs_pp_pressure_gradient_device")
     return dp
+
+
+MOMENTUM_METRICS = ("icsi", "ieta", "izet", "jcsi", "jeta", "jzet", "kcsi", "keta", "kzet", "iaj", "jaj", "kaj",
+                    "csi", "eta", "zet")
+
+
+def _momentum_ptrs(grid: Grid, metrics: dict):
+    V, S = dmda_shape(grid, vector=True), dmda_shape(grid)
+    return [_tptr(metrics[k], S if k.endswith("aj") else V, k) for k in MOMENTUM_METRICS]
+
+
+def Formfunction_2(grid: Grid, metrics: dict, nvert, ucont, ucat, ren: float, scale: float = 1.0):
+    """momentum.c:529 on the device: the momentum right-hand side (mz, my, mx, 3)
+    from Ucont and Ucat and the 15 metric tensors (icsi..kzet, iaj, jaj, kaj, csi,
+    eta, zet); grid.immersed is the reference's `immersed` switch."""
+    import torch
+
+    V, S = dmda_shape(grid, vector=True), dmda_shape(grid)
+    rhs = torch.empty(V, dtype=torch.float64, device=ucont.device)
+    _check(lib().vfs_pp_formfunction2_device(C.byref(grid), *_momentum_ptrs(grid, metrics), _tptr(nvert, S, "nvert"),
+                                             _tptr(ucont, V, "ucont"), _tptr(ucat, V, "ucat"), float(ren),
+                                             float(scale), _tptr(rhs, V, "rhs")), "vfs_pp_formfunction2_device")
+    return rhs
+
+
+def RungeKutta(handle: "Handle", metrics: dict, dp, ucont, ucat, ren: float, dt: float, st: float):
+    """rhs.c:1511 on the device: one explicit four-stage step of Ucont (and Ucat)
+    in place, with the handle's grid, masks and cell metrics."""
+    g = handle.grid
+    V = dmda_shape(g, vector=True)
+    _check(lib().vfs_pp_runge_kutta_device(handle._h, *_momentum_ptrs(g, metrics), _tptr(dp, V, "dp"),
+                                           _tptr(ucont, V, "ucont"), _tptr(ucat, V, "ucat"), float(ren), float(dt),
+                                           float(st)), "vfs_pp_runge_kutta_device")
 
 
 def petsc_vec_length(path) -> int:
