@@ -355,7 +355,8 @@ def batch_block(E, torch, generated, steps):
         # all together
         B = E.Batch(hs)
         u, phi, p, ucat = fields()
-        B.project(u, phi, p, ucat, dts, sts, cos, strict=True)  # warm-up, graph capture
+        B.project(u, phi, p, ucat, dts, sts, cos, strict=True)  # warm-up, tuning, graph capture
+        shapes = B.tile_shapes()
         bat = []
         for rep in range(steps):
             u, phi, p, ucat = fields()
@@ -378,6 +379,7 @@ def batch_block(E, torch, generated, steps):
             iterations_max=int(max(s.iterations for s in sts_b)),
             max_div=float(max(s.max_div for s in sts_b)),
             setup_s=round(setup_s, 2),
+            tile_shapes={"flat256": shapes[0], "64x4": shapes[1], "32x8": shapes[2], "16x16": shapes[3]},
         )
     del cs
     torch.cuda.empty_cache()

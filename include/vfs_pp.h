@@ -392,6 +392,16 @@ int vfs_pp_batch_create(vfs_pp_handle *const *members, int count, vfs_pp_batch *
 int vfs_pp_batch_destroy(vfs_pp_batch *b);
 /* 1 (default): replay each iteration as a captured HIP graph; 0: plain launches */
 int vfs_pp_batch_set_graphs(vfs_pp_batch *b, int on);
+/* Tile shape of the batched launches that walk cells (smoothing sweeps, y-line
+ * residuals, restriction to coarse cells, prolongation): -1 (default) lets the
+ * autotuner pick, per launch kind and size, among 0 (256 consecutive cells, i
+ * fastest), 1 (64 x 4 cells in i x j), 2 (32 x 8) and 3 (16 x 16), together with
+ * the worker count; 0..3 force one shape (the worker counts are still tuned).
+ * Results do not depend on the shape.  Takes effect at the next project call. */
+int vfs_pp_batch_set_tile_shape(vfs_pp_batch *b, int shape);
+/* Number of cell-tiled launches of the current plan that walk each of the four
+ * shapes (after at least one project call). */
+int vfs_pp_batch_tile_shapes(vfs_pp_batch *b, int counts[4]);
 /* vfs_pp_project_device for every member at once; per-member device arrays and
  * scalars; stats (may be NULL) receives one entry per member.  Returns
  * VFS_PP_ENOCONV if any member missed rtol or div_tol (see its stats). */

@@ -206,3 +206,32 @@ def test_batch_members_are_refused_while_bound(E, torch_cuda):
     assert st.converged
     for h in hs:
         h.close()
+
+
+def test_tile_shapes_give_identical_results(E, torch_cuda):
+    """The cell-tiled batched launches walk 256 consecutive cells or 64x4, 32x8,
+    16x16 (i x j) tiles, chosen by the autotuner; every shape, forced, gives the
+    same bits, and the tuned plan reports the shapes it uses."""
+    torch = torch_cuda
+    cs = get_cases()
+    hs = handles(E, cs, rtol=1e-12)
+    B = E.Batch(hs)
+    outs = {}
+    for shape in (-1, 0, 1, 2, 3):
+        B.set_tile_shape(shape)
+        u, phi, p, ucat = fields(torch, cs)
+        sts = B.project(u, phi, p, ucat, [c.dt for c, _ in cs], [c.st for c, _ in cs], [c.coeff for c, _ in cs],
+                        strict=True)
+        counts = B.tile_shapes()
+        print(f"tile shape {shape}: launches per shape {counts}", flush=True)
+        if shape >= 0:
+            assert counts[shape] == sum(counts) > 0
+        outs[shape] = [(ph.cpu().numpy(), uu.cpu().numpy(), s.iterations) for ph, uu, s in zip(phi, u, sts)]
+    B.close()
+    for h in hs:
+        h.close()
+    for shape in (0, 1, 2, 3):
+        for (a, b, ia), (c, d, ic) in zip(outs[shape], outs[-1]):
+            np.testing.assert_array_equal(a, c)
+            np.testing.assert_array_equal(b, d)
+            assert ia == ic

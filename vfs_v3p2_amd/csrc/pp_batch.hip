@@ -85,6 +85,29 @@ __device__ __forceinline__ bool flat_cell(const Dims& d, uint32_t lt, int& i, in
   return true;
 }
 
+// Tile shapes of the cell walk, chosen per launch by the autotuner: 0 walks 256
+// consecutive cells (i fastest), so levels narrower than a wave fill every lane;
+// 1, 2, 3 walk 64 x 4, 32 x 8 and 16 x 16 (i x j) cell tiles of one k plane, whose
+// waves hold one, two or four i-rows and share more stencil neighbours in cache.
+constexpr int kTileShapes = 4;
+HDI uint32_t shape_log2_tx(int shape) { return shape == 1 ? 6u : (shape == 2 ? 5u : 4u); }
+HDI uint32_t shape_tiles(int shape, const Dims& d) {
+  if (shape == 0) return (uint32_t)((d.n() + 255) / 256);
+  const uint32_t lw = shape_log2_tx(shape), tx = 1u << lw, ty = 256u >> lw;
+  return (((uint32_t)d.nx + tx - 1) >> lw) * (((uint32_t)d.ny + ty - 1) / ty) * (uint32_t)d.nz;
+}
+__device__ __forceinline__ bool shaped_cell(int shape, const Dims& d, uint32_t lt, int& i, int& j, int& k) {
+  if (shape == 0) return flat_cell(d, lt, i, j, k);
+  const uint32_t lw = shape_log2_tx(shape), tx = 1u << lw, ty = 256u >> lw;
+  const uint32_t e = threadIdx.x + 64u * threadIdx.y;
+  const uint32_t nti = ((uint32_t)d.nx + tx - 1) >> lw, ntj = ((uint32_t)d.ny + ty - 1) / ty;
+  const uint32_t ti = lt % nti, r = lt / nti, tj = r % ntj;
+  i = (int)(ti * tx + (e & (tx - 1)));
+  j = (int)(tj * ty + (e >> lw));
+  k = (int)(r / ntj);
+  return i < d.nx && j < d.ny && k < d.nz;
+}
+
 // y-line (i, k) of an item, 256 consecutive lines per tile
 __device__ __forceinline__ bool flat_line(const Dims& d, uint32_t lt, int& i, int& k) {
   const uint32_t e = lt * 256u + threadIdx.x + 64u * threadIdx.y;
@@ -180,7 +203,7 @@ struct FinItem {
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict__ items,
                                                  const uint32_t* __restrict__ start, int nitems, uint32_t total,
-                                                 const SlotDev* __restrict__ sd) {
+                                                 const SlotDev* __restrict__ sd, int shape) {
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
@@ -188,7 +211,7 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
     const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;
     int i, j, k;
-    if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
+    if (!shaped_cell(shape, I.L.d, t - start[q], i, j, k)) continue;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
     smooth_point_cell<T, MODE>(I.L, I.f, I.dinv, I.xin, I.xprev, xout, I.dv, I.c1, I.c2, I.c0, i, j, k);
   }
@@ -198,14 +221,14 @@ __global__ __launch_bounds__(256) void kb_smooth(const SmoothItem<T>* __restrict
 template <typename T>
 __global__ __launch_bounds__(256) void kb_resid(const SmoothItem<T>* __restrict__ items,
                                                 const uint32_t* __restrict__ start, int nitems, uint32_t total,
-                                                const SlotDev* __restrict__ sd) {
+                                                const SlotDev* __restrict__ sd, int shape) {
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
     const SmoothItem<T>& I = items[q];
     if (!sd[tw_ & 0xffffu].y) continue;
     int i, j, k;
-    if (!flat_cell(I.L.d, t - start[q], i, j, k)) continue;
+    if (!shaped_cell(shape, I.L.d, t - start[q], i, j, k)) continue;
     resid_cell(I.L, I.f, I.xin, I.scr, i, j, k);
   }
 }
@@ -248,14 +271,14 @@ __global__ __launch_bounds__(256) void kb_restrict_blk(const RestrictItem<T>* __
 template <typename T>
 __global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __restrict__ items,
                                                        const uint32_t* __restrict__ start, int nitems, uint32_t total,
-                                                       const SlotDev* __restrict__ sd) {
+                                                       const SlotDev* __restrict__ sd, int shape) {
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
     const RestrictItem<T>& I = items[q];
     if (!sd[tw_ & 0xffffu].y) continue;
     int ic, jc, kc;
-    if (!flat_cell(I.dc, t - start[q], ic, jc, kc)) continue;
+    if (!shaped_cell(shape, I.dc, t - start[q], ic, jc, kc)) continue;
     resid_restrict_cell(I.F, I.f, I.x, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
   }
 }
@@ -263,14 +286,14 @@ __global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __
 template <typename T>
 __global__ __launch_bounds__(256) void kb_prolong(const ProlongItem<T>* __restrict__ items,
                                                   const uint32_t* __restrict__ start, int nitems, uint32_t total,
-                                                  const SlotDev* __restrict__ sd) {
+                                                  const SlotDev* __restrict__ sd, int shape) {
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
     const ProlongItem<T>& I = items[q];
     if (!sd[tw_ & 0xffffu].y) continue;
     int i, j, k;
-    if (!flat_cell(I.F.d, t - start[q], i, j, k)) continue;
+    if (!shaped_cell(shape, I.F.d, t - start[q], i, j, k)) continue;
     prolong_add_cell(I.F, I.dc, I.clsc, I.cmask, I.xc, I.xf, i, j, k);
   }
 }
@@ -614,11 +637,12 @@ inline unsigned workers(uint32_t total, unsigned cap) {
   return w;
 }
 
-// Worker counts chosen by measurement, per (launch kind and precision, mode,
-// log2 of the tile count); shared by every batch and server of the process.
+// Worker counts and cell-tile shapes chosen by measurement, per (launch kind and
+// precision, mode, log2 of the flat tile count); shared by every batch and server
+// of the process.
 struct TuneCache {
   std::mutex mu;
-  std::map<std::tuple<int, int, int>, unsigned> best;
+  std::map<std::tuple<int, int, int>, std::pair<unsigned, int>> best;  // (worker cap, tile shape)
 };
 TuneCache& tune_cache() {
   static TuneCache c;
@@ -676,6 +700,13 @@ struct BatchCore {
     int nitems;
     uint32_t total;
     unsigned cap = 2048;  // worker count limit (0: one tile per worker), set by tune()
+    // cell-tiled launches: the tile shape (see shaped_cell), each item's tile count
+    // per shape and its slot, from which retile() rebuilds the work list
+    int shape = 0;
+    bool shaped = false;
+    uint32_t total0 = 0;  // tiles of the flat walk (the tuner's size bucket)
+    std::vector<uint32_t> tiles[kTileShapes];
+    std::vector<uint32_t> item_slot;
   };
   std::vector<Launch> seq;  // V-cycle + Krylov passes of one iteration
   Launch apply{}, axd{}, axs{}, axn{};
@@ -688,6 +719,7 @@ struct BatchCore {
 
   enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS, K_AXN };
   bool tuned_once = false;
+  int forced_shape = -1;  // >= 0: every cell-tiled launch walks this tile shape (no shape tuning)
   // batched phase steps: pinned staging and device copy of their work lists, and
   // the tile partials of their reductions
   char* ph_host = nullptr;
@@ -851,8 +883,8 @@ struct BatchCore {
         else { lnx.push_back(it); nlnx.push_back(line_tiles(L.d)); }
       }
     }
-    add_launch(K_SMOOTH, pmode, pt, npt);
-    add_launch(K_RESID, 0, rs, nrs);
+    add_launch_cells(K_SMOOTH, pmode, pt);
+    add_launch_cells(K_RESID, 0, rs);
     add_launch(K_LINE, SM_ZERO, lz, nlz);
     add_launch(K_LINE, SM_NEXT, lnx, nlnx);
   }
@@ -912,7 +944,7 @@ struct BatchCore {
           }
         }
         add_launch(K_RBLK, 0, rb, nb);
-        add_launch(K_RGEN, 0, rg, ng);
+        add_launch_cells(K_RGEN, 0, rg);
       }
       if (!dn.empty()) {
         std::vector<DenseItem<T>> items;
@@ -962,7 +994,7 @@ struct BatchCore {
         pi.push_back(it);
         pn.push_back(cell_tiles(L.d));
       }
-      add_launch(K_PROLONG, 0, pi, pn);
+      add_launch_cells(K_PROLONG, 0, pi);
       for (int k = 0; k < degree; k++) smoothing_step(d, k, false, true, up, cur);
     }
     // Krylov passes
@@ -1031,22 +1063,81 @@ struct BatchCore {
     dirty = false;
   }
 
+  // A cell-tiled launch: items over the cells of dims[q] (the level each item
+  // walks); starts with the flat walk, the tuner may pick another shape.
+  static Dims cells_of(const SmoothItem<T>& I) { return I.L.d; }
+  static Dims cells_of(const RestrictItem<T>& I) { return I.dc; }
+  static Dims cells_of(const ProlongItem<T>& I) { return I.F.d; }
+  template <typename Item>
+  void add_launch_cells(int kind, int mode, const std::vector<Item>& items) {
+    if (items.empty()) return;
+    std::vector<Dims> dims(items.size());
+    for (size_t q = 0; q < items.size(); q++) dims[q] = cells_of(items[q]);
+    std::vector<uint32_t> tiles[kTileShapes];
+    uint32_t flat = 0;
+    for (int sh = 0; sh < kTileShapes; sh++) {
+      tiles[sh].resize(items.size());
+      for (size_t q = 0; q < items.size(); q++) tiles[sh][q] = shape_tiles(sh, dims[q]);
+    }
+    for (uint32_t v : tiles[0]) flat += v;
+    // the shape a forced setting or an earlier tuning chose for this kind and size
+    int sh = 0;
+    if (forced_shape >= 0) {
+      sh = forced_shape;
+    } else {
+      auto& tc = tune_cache();
+      std::lock_guard<std::mutex> lk(tc.mu);
+      auto it = tc.best.find(std::make_tuple(kind + 100 * (int)sizeof(T), mode, log2_bucket(flat)));
+      if (it != tc.best.end()) sh = it->second.second;
+    }
+    const size_t before = seq.size();
+    add_launch(kind, mode, items, tiles[sh]);
+    if (seq.size() == before) return;
+    Launch& L = seq.back();
+    L.shaped = true;
+    L.shape = sh;
+    L.total0 = flat;
+    for (int q = 0; q < kTileShapes; q++) L.tiles[q] = std::move(tiles[q]);
+    L.item_slot.resize(items.size());
+    for (size_t q = 0; q < items.size(); q++) L.item_slot[q] = (uint32_t)items[q].slot;
+  }
+
+  // rebuild a cell-tiled launch's starts and per-tile table for tile shape sh
+  void retile(Launch& L, int sh) {
+    if (!L.shaped || L.shape == sh) return;
+    const size_t ni = L.item_slot.size();
+    std::vector<uint32_t> start(ni);
+    uint32_t tot = 0;
+    for (size_t q = 0; q < ni; q++) { start[q] = tot; tot += L.tiles[sh][q]; }
+    uint32_t* ds = plan.alloc<uint32_t>(2 * ni + tot);
+    HIPCHECK(hipMemcpyAsync(ds, start.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(ds + ni + tot, L.item_slot.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (tot) {
+      k_tile_table<<<(tot + 255) / 256, 256, 0, s>>>(ds, (int)ni, tot);
+      HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    L.start = ds;
+    L.total = tot;
+    L.shape = sh;
+  }
+
   void launch_one(const Launch& L) {
     const unsigned W = workers(L.total, L.cap);
     switch (L.kind) {
       case K_SMOOTH: {
         auto* it = static_cast<const SmoothItem<T>*>(L.items);
         if (L.mode == SM_FIRST2)
-          kb_smooth<T, SM_FIRST2><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          kb_smooth<T, SM_FIRST2><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd, L.shape);
         else if (L.mode == SM_ZERO)
-          kb_smooth<T, SM_ZERO><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          kb_smooth<T, SM_ZERO><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd, L.shape);
         else
-          kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd);
+          kb_smooth<T, SM_NEXT><<<W, dim3(64, 4, 1), 0, s>>>(it, L.start, L.nitems, L.total, sd, L.shape);
         break;
       }
       case K_RESID:
         kb_resid<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start, L.nitems,
-                                                 L.total, sd);
+                                                 L.total, sd, L.shape);
         break;
       case K_LINE:
         if (L.mode == SM_ZERO)
@@ -1062,11 +1153,11 @@ struct BatchCore {
         break;
       case K_RGEN:
         kb_restrict_gen<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,
-                                                        L.nitems, L.total, sd);
+                                                        L.nitems, L.total, sd, L.shape);
         break;
       case K_PROLONG:
         kb_prolong<T><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const ProlongItem<T>*>(L.items), L.start, L.nitems,
-                                                   L.total, sd);
+                                                   L.total, sd, L.shape);
         break;
       case K_DENSE:
         kb_dense<T><<<W, 256, 0, s>>>(static_cast<const DenseItem<T>*>(L.items), L.start, L.nitems, L.total, sd);
@@ -1115,10 +1206,12 @@ struct BatchCore {
     launch_one(axs);
   }
 
-  // Autotuner: each launch's worker count is picked by timing its candidates
-  // (HIP events, best of 3 after a warm-up) and remembered per (kind, mode,
-  // size bucket).  The timed launches run on the cases' scratch state, so this
-  // only happens while no case is in the middle of an iteration.
+  // Autotuner: each launch's worker count and, for the cell-tiled launches, its
+  // tile shape (flat 256 cells, 64x4, 32x8 or 16x16) are picked by timing the
+  // candidates (HIP events, best of 3 after a warm-up) and remembered per (kind,
+  // mode, size bucket).  The timed launches run on the cases' scratch state, so
+  // this only happens while no case is in the middle of an iteration.
+  static int bucket_of(const Launch& L) { return log2_bucket(L.shaped ? L.total0 : L.total); }
   void tune(bool allowed) {
     std::vector<Launch*> all;
     for (Launch& L : seq) all.push_back(&L);
@@ -1133,11 +1226,15 @@ struct BatchCore {
     std::lock_guard<std::mutex> lk(tc.mu);
     bool need = false;
     for (Launch* L : all) {
-      auto it = tc.best.find(std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, log2_bucket(L->total)));
-      if (it != tc.best.end()) L->cap = it->second;
-      else need = true;
+      auto it = tc.best.find(std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, bucket_of(*L)));
+      if (it != tc.best.end()) {
+        L->cap = it->second.first;
+        retile(*L, forced_shape >= 0 ? forced_shape : it->second.second);
+      } else {
+        need = true;
+      }
     }
-    if (!need || !allowed) return;
+    if (!need || !allowed || forced_shape >= 0) return;
     // every slot active at step 0 while timing
     for (size_t q = 0; q < slots.size(); q++) sd_host[q] = SlotDev{0, slots[q].h ? 1 : 0};
     HIPCHECK(hipMemcpyAsync(sd, sd_host, sizeof(SlotDev) * slots.size(), hipMemcpyHostToDevice, s));
@@ -1146,36 +1243,54 @@ struct BatchCore {
     HIPCHECK(hipEventCreate(&e1));
     const unsigned cands[] = {0u, 16384u, 8192u, 4096u, 2048u, 1024u};
     for (Launch* L : all) {
-      const auto key = std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, log2_bucket(L->total));
+      const auto key = std::make_tuple(L->kind + 100 * (int)sizeof(T), L->mode, bucket_of(*L));
       auto it = tc.best.find(key);
-      if (it != tc.best.end()) { L->cap = it->second; continue; }
+      if (it != tc.best.end()) {
+        L->cap = it->second.first;
+        retile(*L, it->second.second);
+        continue;
+      }
       float best_ms = 1e30f;
       unsigned best = 2048u;
-      unsigned last_w = 0;
-      for (unsigned cap : cands) {
-        const unsigned w = workers(L->total, cap);
-        if (w == last_w) continue;  // same grid as the previous candidate
-        last_w = w;
-        L->cap = cap;
-        launch_one(*L);
-        float tmin = 1e30f;
-        for (int r = 0; r < 3; r++) {
-          HIPCHECK(hipEventRecord(e0, s));
+      int best_shape = L->shape;
+      const int nsh = L->shaped ? kTileShapes : 1;
+      for (int shi = 0; shi < nsh; shi++) {
+        const int sh = L->shaped ? shi : L->shape;
+        retile(*L, sh);
+        unsigned last_w = 0;
+        for (unsigned cap : cands) {
+          const unsigned w = workers(L->total, cap);
+          if (w == last_w) continue;  // same grid as the previous candidate
+          last_w = w;
+          L->cap = cap;
           launch_one(*L);
-          HIPCHECK(hipEventRecord(e1, s));
-          HIPCHECK(hipEventSynchronize(e1));
-          float ms = 0.f;
-          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-          tmin = std::min(tmin, ms);
+          float tmin = 1e30f;
+          for (int r = 0; r < 3; r++) {
+            HIPCHECK(hipEventRecord(e0, s));
+            launch_one(*L);
+            HIPCHECK(hipEventRecord(e1, s));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            tmin = std::min(tmin, ms);
+          }
+          if (tmin < best_ms) { best_ms = tmin; best = cap; best_shape = sh; }
         }
-        if (tmin < best_ms) { best_ms = tmin; best = cap; }
       }
+      retile(*L, best_shape);
       L->cap = best;
-      tc.best[key] = best;
+      tc.best[key] = {best, best_shape};
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPCHECK(hipStreamSynchronize(s));
+  }
+
+  // tile shapes the current plan's cell-tiled launches use: counts per shape
+  void shape_counts(int* counts) const {
+    for (int sh = 0; sh < kTileShapes; sh++) counts[sh] = 0;
+    for (const Launch& L : seq)
+      if (L.shaped) counts[L.shape]++;
   }
 
   void run_iteration() {
@@ -1893,6 +2008,20 @@ int vfs_pp_batch_set_graphs(vfs_pp_batch* b, int on) {
   if (!b) return VFS_PP_EINVAL;
   if (b->cf) { b->cf->use_graph = on != 0; b->cf->drop_graph(); }
   if (b->cd) { b->cd->use_graph = on != 0; b->cd->drop_graph(); }
+  return VFS_PP_OK;
+}
+
+int vfs_pp_batch_set_tile_shape(vfs_pp_batch* b, int shape) {
+  if (!b || shape < -1 || shape >= kTileShapes) return VFS_PP_EINVAL;
+  if (b->cf) { b->cf->forced_shape = shape; b->cf->dirty = true; }
+  if (b->cd) { b->cd->forced_shape = shape; b->cd->dirty = true; }
+  return VFS_PP_OK;
+}
+
+int vfs_pp_batch_tile_shapes(vfs_pp_batch* b, int counts[4]) {
+  if (!b || !counts) return VFS_PP_EINVAL;
+  if (b->cf) b->cf->shape_counts(counts);
+  else b->cd->shape_counts(counts);
   return VFS_PP_OK;
 }
 

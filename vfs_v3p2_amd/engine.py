@@ -102,6 +102,8 @@ def lib():
         "vfs_pp_projection_device": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double]),
         "vfs_pp_divergence_device": (C.c_int, [H, vp, dp]),
         "vfs_pp_divergence_noalloc": (C.c_int, [C.POINTER(Grid), vp, vp, vp, vp]),
+        "vfs_pp_batch_set_tile_shape": (C.c_int, [H, C.c_int]),
+        "vfs_pp_batch_tile_shapes": (C.c_int, [H, C.POINTER(C.c_int)]),
         "vfs_pp_repack": (C.c_int, [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                     C.c_int]),
         "vfs_pp_project_device": (C.c_int, [H, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
@@ -385,6 +387,16 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_tile_shape(self, shape: int):
+        """-1: tuned (default); 0 flat 256 cells, 1 64x4, 2 32x8, 3 16x16 forced."""
+        _check(lib().vfs_pp_batch_set_tile_shape(self._b, int(shape)), "vfs_pp_batch_set_tile_shape")
+
+    def tile_shapes(self):
+        """Cell-tiled launches of the current plan per tile shape (flat, 64x4, 32x8, 16x16)."""
+        c = (C.c_int * 4)()
+        _check(lib().vfs_pp_batch_tile_shapes(self._b, c), "vfs_pp_batch_tile_shapes")
+        return list(c)
 
     def project(self, ucont, phi, p, ucat, dt, st, coeff, strict=False):
         """Per-member lists of device tensors and scalars; returns a list of Stats."""
