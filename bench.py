@@ -12,7 +12,9 @@ Forms: mixed (FP32 multigrid preconditioner inside FP64 FGMRES; the headline
 
 Prints one JSON line: solves/s of the mixed form as `value`, p50/p99 latency,
 the FP64 form beside it, the per-kernel HBM figures (`roofline`) and a CPU
-baseline timed on this host's cores (`cpu_baseline`).
+baseline timed on this host's cores (`cpu_baseline`: the headline leg is this
+engine's solver ported to the CPU; `legs.reference_method` is the reference's
+GMRES(51) with a diagonal preconditioner standing in for BoomerAMG).
 """
 from __future__ import annotations
 
@@ -57,7 +59,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU work for the baseline sample")
+    ap.add_argument("--cpu-budget", type=float, default=25.0,
+                    help="seconds of CPU work for the port leg of the baseline (half is spent timing)")
+    ap.add_argument("--cpu-ref-cap", type=float, default=150.0,
+                    help="cap in seconds on the reference-method leg's one GMRES(51) solve; 0 skips that leg")
     ap.add_argument("--form", choices=("single", "batch", "serve", "towers"), default="single",
                     help="single: the 192x96x128 hill projection (default); batch: 64 uneven cases solved "
                          "together; serve: a Poisson stream of requests over resident cases; towers: the "
@@ -637,7 +642,9 @@ def main():
         try:
             from oracle import cpu_baseline
 
-            out["cpu_baseline"] = cpu_baseline.measure(case, budget_s=args.cpu_budget)
+            out["cpu_baseline"] = cpu_baseline.measure(case, budget_s=args.cpu_budget,
+                                                       reference_leg=args.cpu_ref_cap > 0,
+                                                       reference_cap_s=args.cpu_ref_cap)
         except Exception as e:  # the GPU figures stand on their own
             out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.single_only:

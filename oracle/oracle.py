@@ -46,6 +46,7 @@ def lib():
         _lib.or_rhs.argtypes = [g, d, d, C.c_double, C.c_double, C.c_double, d]
         _lib.or_volume_flux.argtypes = [g, C.c_int, d, d, d, d, d, d]
         _lib.or_remove_nullspace.argtypes = [g, d, d, d]
+        _lib.or_interior_to_phi.argtypes = [g, d, d, d]
         _lib.or_convert_phi.argtypes = [g, d, d, d]
         _lib.or_update_pressure.argtypes = [g, d, d, d]
         _lib.or_projection.argtypes = [g] + [d] * 12 + [d, d, d, C.c_double, C.c_double, C.c_double]
@@ -155,6 +156,14 @@ def volume_flux(g, immersed, ucont, f):
 
 def remove_nullspace(g, nvert, gid, x):
     lib().or_remove_nullspace(C.byref(g), _p(nvert), _p(gid), _p(x))
+
+
+def interior_to_phi(g, nvert, x):
+    """or_interior_to_phi: removes the fluid mean from the interior array x (in
+    place) and returns phi on the DMDA with periodic ghost images."""
+    phi = np.zeros((g.mz, g.my, g.mx))
+    lib().or_interior_to_phi(C.byref(g), _p(nvert), _p(x), _p(phi))
+    return phi
 
 
 def convert_phi(g, nvert, phi2):
@@ -337,13 +346,7 @@ def cpu_projection(case, A: "Assembled", mg: CpuMG, rtol=1e-10, ucont=None):
     vf = volume_flux(g, case.immersed, u, f)
     b = rhs(g, u, A.nvert, case.dt, case.st, case.coeff)
     x, it, rel = mg.solve(b[1:-1, 1:-1, 1:-1], rtol=rtol)
-    lt = A.nvert[1:-1, 1:-1, 1:-1] < 0.1
-    x = x - np.where(lt, x, 0.0).sum() / lt.sum() * lt
-    phi = np.zeros(A.nvert.shape)
-    inner = phi[1:-1, 1:-1, 1:-1]
-    inner[:] = np.where(A.nvert[1:-1, 1:-1, 1:-1].astype(np.int64) > 0.1, 0.0, x)
-    from vfs_v3p2_amd.cases import periodic_fill  # ghost images, poisson.c:153-174
-    periodic_fill(phi, case.per)
+    phi = interior_to_phi(g, A.nvert, x)  # Remove_Nullspace + Convert_Phi2_Phi + ghost images
     p = np.zeros_like(phi)
     update_pressure(g, A.nvert, p, phi)
     projection(g, f, phi, u, case.dt, case.st, case.coeff)

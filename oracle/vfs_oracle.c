@@ -47,6 +47,8 @@ static int periodic_image(const or_grid *g, int i, int j, int k, int *a, int *b,
 
 static void periodic_copy_scalar(const or_grid *g, double *f) {
   if (!(g->per[0] || g->per[1] || g->per[2])) return;
+  /* an image has every periodic index interior, so no image is itself written */
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < g->mz; k++)
     for (int j = 0; j < g->my; j++)
       for (int i = 0; i < g->mx; i++) {
@@ -269,6 +271,7 @@ void or_rhs(const or_grid *g, const double *ucont, const double *nvert, double d
   size_t n = (size_t)g->mx * g->my * g->mz;
   memset(b, 0, n * sizeof(double));
   const int mx = g->mx, my = g->my, mz = g->mz;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
@@ -292,18 +295,29 @@ void or_rhs(const or_grid *g, const double *ucont, const double *nvert, double d
 /* ---------------------------------------------------------------- VolumeFlux */
 static double norm3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 
+/* Sums over the interior are taken per k plane (in i, j order) and the plane
+ * partials are added in k order, so the result does not depend on the thread
+ * count; the planes themselves run in parallel (each writes only its own faces). */
+static double sum_planes(const double *part, int mz) {
+  double s = 0;
+  for (int k = 1; k < mz - 1; k++) s += part[k];
+  return s;
+}
+
 void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double *csi,
                     const double *eta, const double *zet, const double *nvert, double out[4]) {
   const int mx = g->mx, my = g->my, mz = g->mz;
   const size_t st[3] = {1, (size_t)mx, (size_t)mx * my};
   const double *met[3] = {csi, eta, zet};
   const int mm[3] = {mx, my, mz};
+  double *pf = (double *)calloc(2 * (size_t)mz, sizeof(double)), *pa = pf + mz;
   if (immersed == 0 || immersed == 3) {
     /* poisson.c:2634-2828 (poisson_hypre.c:667 takes it for every value but 1 and
        2; the corrected faces are zeroed for 0 and 3, poisson.c:2728) */
     for (int pass = 0; pass < 2; pass++) {
-      double flux = 0, area = 0;
-      for (int k = 1; k < mz - 1; k++)
+#pragma omp parallel for schedule(static)
+      for (int k = 1; k < mz - 1; k++) {
+        double flux = 0, area = 0;
         for (int j = 1; j < my - 1; j++)
           for (int i = 1; i < mx - 1; i++) {
             size_t c = IX(g, i, j, k);
@@ -315,10 +329,15 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
               if (nvert[c] > TH && nq < TH) { flux -= ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
             }
           }
+        pf[k] = flux;
+        pa[k] = area;
+      }
+      double flux = sum_planes(pf, mz), area = sum_planes(pa, mz);
       out[2 * pass] = flux;
       out[2 * pass + 1] = area;
       if (pass == 1) break;
       double corr = (area > 1.e-15) ? flux / area : 0.;
+#pragma omp parallel for collapse(2) schedule(static)
       for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
           for (int i = 1; i < mx - 1; i++) {
@@ -332,12 +351,14 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
             }
           }
     }
+    free(pf);
     return;
   }
   /* immersed == 1: poisson.c:2898-3240 with flg = 0 */
   const double eps = 1.e-10, ibmval = 1.1;
-  double flux = 0, area = 0;
-  for (int k = 1; k < mz - 1; k++)
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < mz - 1; k++) {
+    double flux = 0, area = 0;
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
         size_t c = IX(g, i, j, k);
@@ -360,9 +381,14 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
             }
           }
       }
+    pf[k] = flux;
+    pa[k] = area;
+  }
+  double flux = sum_planes(pf, mz), area = sum_planes(pa, mz);
   out[0] = flux;
   out[1] = area;
   double corr = (area > 1.e-15) ? flux / area : 0.;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
@@ -381,8 +407,9 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
             if (nq < 0.1 && fabs(ucont[3 * c + d]) > eps) ucont[3 * c + d] += norm3(&met[d][3 * c]) * corr;
           }
       }
-  flux = 0; area = 0;
-  for (int k = 1; k < mz - 1; k++)
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k < mz - 1; k++) {
+    double fl = 0, ar = 0;
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
         size_t c = IX(g, i, j, k);
@@ -390,36 +417,77 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
         for (int d = 0; d < 3; d++) {
           double nq = nvert[c + st[d]];
           if (!(pos[d] < mm[d] - 2)) continue;
-          if (nvert[c] < 0.1 && nq > 0.1 && nq < ibmval) { flux += ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+          if (nvert[c] < 0.1 && nq > 0.1 && nq < ibmval) { fl += ucont[3 * c + d]; ar += norm3(&met[d][3 * c]); }
         }
         if (nvert[c] > 0.1 && nvert[c] < ibmval)
           for (int d = 0; d < 3; d++) {
             double nq = nvert[c + st[d]];
             if (!(pos[d] < mm[d] - 2)) continue;
-            if (nq < 0.1) { flux -= ucont[3 * c + d]; area += norm3(&met[d][3 * c]); }
+            if (nq < 0.1) { fl -= ucont[3 * c + d]; ar += norm3(&met[d][3 * c]); }
           }
       }
-  out[2] = flux;
-  out[3] = area;
+    pf[k] = fl;
+    pa[k] = ar;
+  }
+  out[2] = sum_planes(pf, mz);
+  out[3] = sum_planes(pa, mz);
+  free(pf);
 }
 
 /* ------------------------------------------------------ null space, scatter */
 void or_remove_nullspace(const or_grid *g, const double *nvert, const double *gid, double *x) {
-  double sum = 0;
+  const int mz = g->mz;
+  double *ps = (double *)calloc((size_t)mz, sizeof(double));
   int cnt = 0;
-  for (int k = 1; k < g->mz - 1; k++)
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+  for (int k = 1; k < mz - 1; k++) {
+    double sum = 0;
     for (int j = 1; j < g->my - 1; j++)
       for (int i = 1; i < g->mx - 1; i++) {
         size_t c = IX(g, i, j, k);
         if (nvert[c] < 0.1) { sum += x[(int)gid[c]]; cnt++; }
       }
-  double val = -sum / (double)cnt;
+    ps[k] = sum;
+  }
+  double val = -sum_planes(ps, mz) / (double)cnt;
+  free(ps);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < g->mz - 1; k++)
     for (int j = 1; j < g->my - 1; j++)
       for (int i = 1; i < g->mx - 1; i++) {
         size_t c = IX(g, i, j, k);
         if (nvert[c] < 0.1) x[(int)gid[c]] += val;
       }
+}
+
+/* The same mean removal and scatter on an interior array (nz, ny, nx) as the CPU
+ * port of the engine's solver holds it: x -= mean over fluid cells, then phi =
+ * x on cells whose int(nvert) is 0 (Convert_Phi2_Phi, poisson.c:1086-1125),
+ * zero elsewhere, with periodic ghost images. */
+void or_interior_to_phi(const or_grid *g, const double *nvert, double *x, double *phi) {
+  const int mx = g->mx, my = g->my, mz = g->mz, nx = mx - 2, ny = my - 2;
+  double *ps = (double *)calloc((size_t)mz, sizeof(double));
+  int cnt = 0;
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+  for (int k = 1; k < mz - 1; k++) {
+    double sum = 0;
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++)
+        if (nvert[IX(g, i, j, k)] < 0.1) { sum += x[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)]; cnt++; }
+    ps[k] = sum;
+  }
+  double mean = cnt ? sum_planes(ps, mz) / (double)cnt : 0.;
+  free(ps);
+  memset(phi, 0, (size_t)mx * my * mz * sizeof(double));
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k < mz - 1; k++)
+    for (int j = 1; j < my - 1; j++)
+      for (int i = 1; i < mx - 1; i++) {
+        size_t c = IX(g, i, j, k), q = ((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1);
+        if (nvert[c] < 0.1) x[q] -= mean;
+        phi[c] = (int)nvert[c] > TH ? 0. : x[q];
+      }
+  periodic_copy_scalar(g, phi);
 }
 
 void or_convert_phi(const or_grid *g, const double *nvert, const double *phi2, double *phi) {
@@ -437,6 +505,7 @@ void or_convert_phi(const or_grid *g, const double *nvert, const double *phi2, d
 }
 
 void or_update_pressure(const or_grid *g, const double *nvert, double *p, double *phi) {
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < g->mz - 1; k++)
     for (int j = 1; j < g->my - 1; j++)
       for (int i = 1; i < g->mx - 1; i++) {
@@ -469,6 +538,7 @@ void or_projection(const or_grid *g, const double *icsi, const double *ieta, con
   const int mx = g->mx, my = g->my, mz = g->mz;
   const size_t sx = 1, sy = (size_t)mx, sz = (size_t)mx * my;
   const int pi = g->per[0], pj = g->per[1], pk = g->per[2];
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < mz; k++)
     for (int j = 0; j < my; j++)
       for (int i = 0; i < mx; i++) {
@@ -526,6 +596,7 @@ void or_projection(const or_grid *g, const double *icsi, const double *ieta, con
       }
   /* periodic ghost faces, poisson.c:2401-2441 */
   if (pi || pj || pk)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
@@ -548,6 +619,7 @@ void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, co
   double *lu = (double *)malloc(3 * n * sizeof(double));
   memcpy(lu, ucont_in, 3 * n * sizeof(double));
   if (anyper)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
@@ -557,6 +629,7 @@ void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, co
           lu[3 * p + 0] = lu[3 * s + 0]; lu[3 * p + 1] = lu[3 * s + 1]; lu[3 * p + 2] = lu[3 * s + 2];
         }
   /* core Cramer solve, rhs.c:144-189 */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
@@ -580,6 +653,7 @@ void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, co
       }
   /* periodic copy, rhs.c:198-241 */
   if (anyper)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
@@ -591,6 +665,7 @@ void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, co
   /* boundary values: solid cells and no-slip walls, rhs.c:257-260, 491-515 */
   double *lc = lu; /* reuse the buffer as the snapshot of Ucat */
   memcpy(lc, ucat, 3 * n * sizeof(double));
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < mz; k++)
     for (int j = 0; j < my; j++)
       for (int i = 0; i < mx; i++) {
@@ -614,6 +689,7 @@ void or_contra2cart(const or_grid *g, const int bctype[6], const double *csi, co
       }
   /* final periodic copy, rhs.c:645-673 */
   if (anyper)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 0; k < mz; k++)
       for (int j = 0; j < my; j++)
         for (int i = 0; i < mx; i++) {
@@ -632,6 +708,7 @@ double or_divergence(const or_grid *g, const double *u, const double *aj, const 
   const size_t sy = (size_t)mx, sz = (size_t)mx * my;
   double mx_div = 0.;
   if (div) memset(div, 0, (size_t)mx * my * mz * sizeof(double));
+#pragma omp parallel for collapse(2) reduction(max : mx_div) schedule(static)
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
       for (int i = 1; i < mx - 1; i++) {
@@ -768,12 +845,25 @@ void or_mg_nvert_restrict(const or_grid *gf, const or_grid *gc, const double *nv
 }
 
 /* ---------------------------------------------------------------- GMRES(m) */
-static double vdot(int n, const double *a, const double *b) {
+/* Dot product over a fixed split into DOT_BLOCKS contiguous blocks, each summed
+ * in order by one thread, the block sums then added in block order: the result
+ * is the same for every thread count and every run. */
+#define DOT_BLOCKS 256
+static double det_dot(size_t n, const double *a, const double *b) {
+  double part[DOT_BLOCKS];
+#pragma omp parallel for schedule(static)
+  for (int blk = 0; blk < DOT_BLOCKS; blk++) {
+    size_t lo = n * (size_t)blk / DOT_BLOCKS, hi = n * (size_t)(blk + 1) / DOT_BLOCKS;
+    double s = 0.;
+    for (size_t i = lo; i < hi; i++) s += a[i] * b[i];
+    part[blk] = s;
+  }
   double s = 0.;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  for (int blk = 0; blk < DOT_BLOCKS; blk++) s += part[blk];
   return s;
 }
+
+static double vdot(int n, const double *a, const double *b) { return det_dot((size_t)n, a, b); }
 
 int or_gmres(int n, const int *rowptr, const int *col, const double *val, const double *b,
              double *x, int m, double rtol, int maxit, int precond, double *rel) {
@@ -784,6 +874,7 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
   double *gv = (double *)malloc((m + 1) * sizeof(double)), *y = (double *)malloc(m * sizeof(double));
   double *dinv = (double *)malloc(n * sizeof(double));
   double *w = (double *)malloc(n * sizeof(double));
+#pragma omp parallel for schedule(static)
   for (int r = 0; r < n; r++) {
     double d = 1.;
     if (precond == 1)
@@ -797,27 +888,34 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
   double res = 0.;
   for (;;) {
     or_csr_matvec(n, rowptr, col, val, x, w);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) V[i] = b[i] - w[i];
     double beta = sqrt(vdot(n, V, V));
     res = beta / bn;
     if (res <= rtol || it >= maxit) break;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) V[i] /= beta;
     for (int i = 0; i <= m; i++) gv[i] = 0.;
     gv[0] = beta;
     int j;
     for (j = 0; j < m && it < maxit; j++, it++) {
       double *zj = Z + (size_t)j * n, *vj = V + (size_t)j * n, *vn = V + (size_t)(j + 1) * n;
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; i++) zj[i] = dinv[i] * vj[i];
       or_csr_matvec(n, rowptr, col, val, zj, vn);
       for (int l = 0; l <= j; l++) { /* modified Gram-Schmidt */
         double h = vdot(n, vn, V + (size_t)l * n);
         H[l * m + j] = h;
         const double *vl = V + (size_t)l * n;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; i++) vn[i] -= h * vl[i];
       }
       double hn = sqrt(vdot(n, vn, vn));
       H[(j + 1) * m + j] = hn;
-      if (hn != 0.) for (int i = 0; i < n; i++) vn[i] /= hn;
+      if (hn != 0.) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++) vn[i] /= hn;
+      }
       for (int l = 0; l < j; l++) {
         double t1 = cs[l] * H[l * m + j] + sn[l] * H[(l + 1) * m + j];
         double t2 = -sn[l] * H[l * m + j] + cs[l] * H[(l + 1) * m + j];
@@ -834,9 +932,11 @@ int or_gmres(int n, const int *rowptr, const int *col, const double *val, const 
       for (int q = l + 1; q < j; q++) s -= H[l * m + q] * y[q];
       y[l] = s / H[l * m + l];
     }
-    for (int l = 0; l < j; l++) {
-      const double *zl = Z + (size_t)l * n;
-      for (int i = 0; i < n; i++) x[i] += y[l] * zl[i];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+      double xi = x[i];
+      for (int l = 0; l < j; l++) xi += y[l] * Z[(size_t)l * n + i];
+      x[i] = xi;
     }
   }
   if (rel) *rel = res;
@@ -1014,12 +1114,7 @@ static void cheb_step(mg_lev *L, const double *xin, double *xout, double c1, dou
       }
 }
 
-static double vnorm(size_t n, const double *a) {
-  double s = 0.;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-  for (size_t i = 0; i < n; i++) s += a[i] * a[i];
-  return sqrt(s);
-}
+static double vnorm(size_t n, const double *a) { return sqrt(det_dot(n, a, a)); }
 
 static double power_lambda(mg_lev *L) {
   unsigned s = 1234u;
@@ -1347,9 +1442,7 @@ int or_mg_fgmres(or_mg *mg, const double *b, double *x, int m, double rtol, int 
       for (int pass = 0; pass < 2; pass++)
         for (int l = 0; l <= j; l++) {
           const double *vl = V + (size_t)l * n;
-          double h = 0.;
-#pragma omp parallel for reduction(+ : h) schedule(static)
-          for (size_t i = 0; i < n; i++) h += w[i] * vl[i];
+          double h = det_dot(n, w, vl);
 #pragma omp parallel for schedule(static)
           for (size_t i = 0; i < n; i++) w[i] -= h * vl[i];
           H[l * m + j] += h;

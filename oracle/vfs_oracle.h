@@ -83,6 +83,7 @@ void or_volume_flux(const or_grid *g, int immersed, double *ucont, const double 
 
 /* Remove_Nullspace (poisson_hypre.c:207-270) on the compact vector. */
 void or_remove_nullspace(const or_grid *g, const double *nvert, const double *gid, double *x);
+void or_interior_to_phi(const or_grid *g, const double *nvert, double *x, double *phi);
 
 /* Convert_Phi2_Phi (poisson.c:97-185). */
 void or_convert_phi(const or_grid *g, const double *nvert, const double *phi2, double *phi);
