@@ -69,6 +69,9 @@ def parse():
                          "512x256x256 wind-farm box on one GPU")
     ap.add_argument("--requests", type=int, default=150, help="serve form: requests per arrival rate")
     ap.add_argument("--rates", type=str, default="10,50,100,200,500", help="serve form: arrival rates (1/s)")
+    ap.add_argument("--per-case", type=int, default=4, help="serve form: requests of one case in flight at once")
+    ap.add_argument("--queue-cap", type=int, default=0,
+                    help="serve form: refuse submissions while this many wait (0: unbounded)")
     ap.add_argument("--cases", type=int, default=64, help="batch form: number of cases")
     ap.add_argument("--workers", type=int, default=16, help="host processes generating the batched and served cases")
     ap.add_argument("--time-kernel", type=str, default=None,
@@ -175,13 +178,16 @@ def _gen_served(spec):
     return cases.served_case(*spec)
 
 
-def serve_block(E, torch, generated, rates, nreq):
+def serve_block(E, torch, generated, rates, nreq, max_per_case=4, max_queue=0):
     """Poisson stream of projection requests over 16 resident cases (sizes seeded
     log-uniform 24^3-256^3; 60% channel, 30% hill + sphere, 10% towers), served by
-    continuous batching; p50/p99 latency (arrival to completion) and solves/s per
-    arrival rate.  `generated` yields the cases (served_case)."""
+    continuous batching with up to `max_per_case` requests of one case in flight;
+    p50/p99 latency (arrival to completion) of the served requests, solves/s and
+    (with a queue cap) refused requests per arrival rate.  `generated` yields the
+    cases (served_case)."""
     t0 = time.perf_counter()
-    sv = E.Server(max_active=64, precision=1, rtol=RTOL, div_tol=DIV_TOL)
+    sv = E.Server(max_active=64, max_per_case=max_per_case, max_queue=max_queue, precision=1, rtol=RTOL,
+                  div_tol=DIV_TOL)
     res = []
     for c in generated:
         g = E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed)
@@ -207,8 +213,9 @@ def serve_block(E, torch, generated, rates, nreq):
         arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
         picks = rng.integers(len(res), size=nreq)
         mix = rng.uniform(-1.0, 1.0, size=nreq)
-        lat = np.zeros(nreq)
+        lat = np.full(nreq, np.nan)
         divs, its, failed = [], [], []
+        refused = 0
         t2r = {}
         i = done = 0
         torch.cuda.synchronize()
@@ -222,7 +229,11 @@ def serve_block(E, torch, generated, rates, nreq):
                 t = sv.submit(cid, u, torch.zeros(sh, dtype=torch.float64, device="cuda"),
                               torch.zeros(sh, dtype=torch.float64, device="cuda"),
                               torch.zeros(sh + (3,), dtype=torch.float64, device="cuda"), r[2], r[3], r[4])
-                t2r[t] = i
+                if t is None:  # the full queue refused it
+                    refused += 1
+                    done += 1
+                else:
+                    t2r[t] = i
                 i += 1
             st = sv.status()
             if st["queued"] == 0 and st["active"] == 0:
@@ -241,10 +252,12 @@ def serve_block(E, torch, generated, rates, nreq):
                           f"max_div {s.max_div:.3e} bnorm {s.bnorm:.3e}", file=sys.stderr, flush=True)
                 done += 1
         span = time.perf_counter() - start
-        per_rate.append(dict(rate=lam, requests=nreq, solves_per_s=round(nreq / span, 2),
-                             p50_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
-                             p99_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
-                             max_div=float(max(divs)), iterations_max=int(max(its)), failed=len(failed)))
+        served = lat[~np.isnan(lat)]
+        per_rate.append(dict(rate=lam, requests=nreq, solves_per_s=round(len(served) / span, 2),
+                             p50_ms=round(1e3 * float(np.percentile(served, 50)), 2),
+                             p99_ms=round(1e3 * float(np.percentile(served, 99)), 2),
+                             max_div=float(max(divs)), iterations_max=int(max(its)), failed=len(failed),
+                             refused=refused))
     st = sv.status()
     sv.close()
     del res
@@ -253,7 +266,8 @@ def serve_block(E, torch, generated, rates, nreq):
     return dict(config={"workload": "served stream: Poisson arrivals over 16 resident cases (sizes seeded "
                                     "log-uniform 24^3-256^3; 60% stretched channel, 30% hill + IB sphere, 10% IB "
                                     "towers), continuous batching, one step = one request",
-                        "cells_resident": ncell, "max_active": 64},
+                        "cells_resident": ncell, "max_active": 64, "max_per_case": max_per_case,
+                        "max_queue": max_queue},
                 solves_per_s=top["solves_per_s"], p50_ms=top["p50_ms"], p99_ms=top["p99_ms"], rates=per_rate,
                 resident_gb=round(st["resident_bytes"] / 1e9, 2), workspace_gb=round(st["workspace_bytes"] / 1e9, 2),
                 setup_s=round(setup_s, 1), failed=sum(p["failed"] for p in per_rate))
@@ -276,7 +290,7 @@ def bench_serve(args):
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     rates = [float(x) for x in args.rates.split(",") if x]
-    blk = serve_block(E, torch, pending, rates, args.requests)
+    blk = serve_block(E, torch, pending, rates, args.requests, args.per_case, args.queue_cap)
     pool.close()
     per_rate = blk["rates"]
     top = per_rate[-1]
@@ -657,7 +671,7 @@ def main():
         del gen_batch
         progress("serve block")
         rates = [float(x) for x in args.block_rates.split(",") if x]
-        out["serve"] = serve_block(E, torch, gen_serve, rates, args.block_requests)
+        out["serve"] = serve_block(E, torch, gen_serve, rates, args.block_requests, args.per_case, args.queue_cap)
         del gen_serve
         out["batch_over_plain_fp64"] = out["batch"]["mixed_batched_over_fp64_sequential"]
         out["pregenerate_s"] = round(pre_s, 1)

@@ -46,6 +46,7 @@ extern "C" {
 #define VFS_PP_ESTATE 5      /* call order violated (e.g. solve before set_geometry) */
 #define VFS_PP_ENONFINITE 6  /* NaN/Inf in the inputs or the iterate */
 #define VFS_PP_EIO 7         /* grid file missing, truncated or malformed */
+#define VFS_PP_EBUSY 8       /* served form: the request queue is full, request refused */
 
 typedef struct vfs_pp_handle vfs_pp_handle;
 
@@ -415,15 +416,30 @@ int vfs_pp_batch_project_device(vfs_pp_batch *b, double *const *ucont, double *c
  * multigrid hierarchy) are registered once and stay resident.  Requests
  * (one projection of a registered case on caller device arrays) queue FIFO;
  * each vfs_pp_server_progress call admits queued requests into free slots
- * (at most max_active at once, one per case at a time) when the Krylov
- * workspace of the case fits the HBM budget, runs one batched FGMRES
- * iteration of everything in flight (as vfs_pp_batch), and reports the
- * requests that finished.  Results are the ones vfs_pp_project_device gives. */
+ * (at most max_active at once, and at most max_per_case of one case) when the
+ * Krylov workspace fits the HBM budget, runs one batched FGMRES iteration of
+ * everything in flight (as vfs_pp_batch), and reports the requests that
+ * finished.  Requests of one case in flight together share the case's resident
+ * geometry and hierarchy coefficients; each has its own solve vectors and
+ * multigrid work arrays (a per-case solver kept for reuse once made, about
+ * 80 bytes per cell in the mixed form besides the Krylov workspace).  Results
+ * are the ones vfs_pp_project_device gives, to rounding (the batched kernels
+ * order some sums differently), and the same bit for bit however requests are
+ * interleaved or batched together.
+ *
+ * Back-pressure: with max_queue > 0, vfs_pp_server_submit refuses a request
+ * (VFS_PP_EBUSY, no ticket) while max_queue requests wait; with deadline_ms >
+ * 0, a request still waiting for admission deadline_ms after its submission is
+ * dropped at the next vfs_pp_server_progress, untouched (its arrays are never
+ * read or written), and its ticket is reported by vfs_pp_server_expired. */
 typedef struct vfs_pp_server vfs_pp_server;
 typedef struct {
   size_t hbm_budget;   /* device bytes for resident cases + workspaces; 0: 80% of free memory */
   int max_active;      /* requests in flight at once */
   vfs_pp_opts solver;  /* options of every case (precision and cheb_degree shared) */
+  int max_per_case;    /* requests of one case in flight at once (default 4, >= 1) */
+  int max_queue;       /* waiting requests at most; 0: unbounded (default) */
+  double deadline_ms;  /* admission deadline after submission; 0: none (default) */
 } vfs_pp_server_opts;
 void vfs_pp_server_default_opts(vfs_pp_server_opts *opts);
 int vfs_pp_server_create(const vfs_pp_server_opts *opts, vfs_pp_server **out);
@@ -437,13 +453,19 @@ int vfs_pp_server_add_case(vfs_pp_server *sv, const vfs_grid *grid, const double
                            const double *jaj, const double *kaj, const double *aj,
                            const double *csi, const double *eta, const double *zet,
                            const double *nvert, int *case_id);
+/* VFS_PP_ESTATE while a request of the case is queued or in flight */
 int vfs_pp_server_remove_case(vfs_pp_server *sv, int case_id);
-/* queue one projection; the device arrays must stay valid until it finishes */
+/* queue one projection; the device arrays must stay valid until it finishes or
+ * expires.  VFS_PP_EBUSY (no ticket) when max_queue requests already wait. */
 int vfs_pp_server_submit(vfs_pp_server *sv, int case_id, double *ucont, double *phi, double *p,
                          double *ucat, double dt, double st, double coeff, int *ticket);
-/* admit, iterate once, retire: *ndone finished tickets (and their stats) */
+/* drop expired requests, admit, iterate once, retire: *ndone finished tickets
+ * (and their stats) */
 int vfs_pp_server_progress(vfs_pp_server *sv, int *tickets, vfs_pp_stats *stats, int cap,
                            int *ndone);
+/* tickets dropped by the admission deadline since the last call: up to cap of
+ * them into tickets (oldest first), their number in *n; the rest stay listed */
+int vfs_pp_server_expired(vfs_pp_server *sv, int *tickets, int cap, int *n);
 int vfs_pp_server_status(vfs_pp_server *sv, int *queued, int *active, size_t *resident_bytes,
                          size_t *workspace_bytes);
 

@@ -156,3 +156,115 @@ def test_inputs_written_just_before_submit_are_seen(E, torch_cuda):
     sv.close()
     for ph in outs:
         assert relL2(ph, phi_a) <= 1e-9
+
+
+def _arrays(torch, c, u):
+    ud = torch.from_numpy(u.copy()).cuda()
+    phi = torch.full(c.shape, -7.0, dtype=torch.float64, device="cuda")
+    p = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+    ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+    return ud, phi, p, ucat
+
+
+def _drain(sv):
+    done = {}
+    rounds = 0
+    while sv.status()["queued"] or sv.status()["active"]:
+        for t, s in sv.progress():
+            done[t] = s
+        rounds += 1
+        assert rounds < 10000
+    return done
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_two_tickets_of_one_case_in_flight_together(E, torch_cuda, precision):
+    """Three requests of one case run at once on the case's resident hierarchy
+    (the first on the case's own handle, the others on solve clones of it) and
+    give, bit for bit, what each gives when the server runs it by itself; a
+    second round reuses the clones."""
+    torch = torch_cuda
+    c = get_cases()[1]
+    opts = dict(precision=precision, rtol=1e-12)
+    reqs = [u for _, u in requests(torch, [c], 3, seed=40 + precision)]
+    print(f"3 requests of one {c.nx}x{c.ny}x{c.nz} case, precision {precision}", flush=True)
+    grid = E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed)
+    # one at a time
+    sv1 = E.Server(max_active=4, max_per_case=1, **opts)
+    cid = sv1.add_case(grid, c.fields)
+    alone = []
+    for u in reqs:
+        a = _arrays(torch, c, u)
+        t = sv1.submit(cid, *a, c.dt, c.st, c.coeff)
+        st = _drain(sv1)[t]
+        assert st.converged
+        alone.append([x.cpu().numpy() for x in a])
+    res1 = sv1.status()["resident_bytes"]
+    sv1.close()
+    # together
+    sv = E.Server(max_active=4, max_per_case=3, **opts)
+    cid = sv.add_case(grid, c.fields)
+    res0 = sv.status()["resident_bytes"]
+    assert res0 == res1
+    for rnd in range(2):
+        arrs, tick = [], []
+        for u in reqs:
+            a = _arrays(torch, c, u)
+            arrs.append(a)
+            tick.append(sv.submit(cid, *a, c.dt, c.st, c.coeff))
+        first = sv.progress()
+        assert sv.status()["active"] + len(first) == 3  # all three admitted at once
+        done = dict(first)
+        done.update(_drain(sv))
+        for q in range(3):
+            assert done[tick[q]].converged
+            for got, want in zip(arrs[q], alone[q]):
+                np.testing.assert_array_equal(got.cpu().numpy(), want)
+        if rnd == 0:
+            grown = sv.status()["resident_bytes"]
+            assert grown > res0  # two solve clones
+        else:
+            assert sv.status()["resident_bytes"] == grown  # kept and reused
+    clone = (grown - res0) / 2 / (c.nx * c.ny * c.nz)
+    print(f"solve clone: {clone:.1f} bytes per cell", flush=True)
+    assert clone < 200
+    sv.close()
+
+
+def test_queue_cap_and_admission_deadline(E, torch_cuda):
+    """max_queue refuses submissions beyond the cap (no ticket); a request not
+    admitted within deadline_ms is dropped untouched and listed by expired();
+    the admitted one finishes as usual."""
+    import time
+
+    torch = torch_cuda
+    c = get_cases()[0]
+    opts = dict(precision=1, rtol=1e-12)
+    grid = E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed)
+    u = np.ascontiguousarray(c.fields["ucont"])
+    sv = E.Server(max_active=1, max_per_case=1, max_queue=2, **opts)
+    cid = sv.add_case(grid, c.fields)
+    arrs = [_arrays(torch, c, u) for _ in range(3)]
+    t0 = sv.submit(cid, *arrs[0], c.dt, c.st, c.coeff)
+    t1 = sv.submit(cid, *arrs[1], c.dt, c.st, c.coeff)
+    assert t0 is not None and t1 is not None
+    assert sv.submit(cid, *arrs[2], c.dt, c.st, c.coeff) is None  # queue full
+    done = _drain(sv)
+    assert set(done) == {t0, t1} and all(s.converged for s in done.values())
+    sv.close()
+
+    sv = E.Server(max_active=1, max_per_case=1, deadline_ms=250.0, **opts)
+    cid = sv.add_case(grid, c.fields)
+    a0, a1 = _arrays(torch, c, u), _arrays(torch, c, u)
+    t0 = sv.submit(cid, *a0, c.dt, c.st, c.coeff)
+    got = sv.progress()  # admits t0 (well inside its deadline)
+    t1 = sv.submit(cid, *a1, c.dt, c.st, c.coeff)  # waits: the only slot is taken
+    time.sleep(0.5)
+    done = dict(got)
+    done.update(_drain(sv))
+    assert set(done) == {t0} and done[t0].converged
+    assert sv.expired() == [t1]
+    assert sv.expired() == []
+    assert bool((a1[1] == -7.0).all())  # the dropped request's phi was never written
+    np.testing.assert_array_equal(a1[0].cpu().numpy(), u)
+    sv.close()

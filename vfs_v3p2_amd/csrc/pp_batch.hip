@@ -2041,27 +2041,39 @@ int vfs_pp_batch_project_device(vfs_pp_batch* b, double* const* ucont, double* c
 }  // extern "C"
 
 // ====================================================================== server
-// Continuous batching: requests queue per resident case; between batched
-// iterations the server admits queued requests into free slots as long as a
-// Krylov workspace for the case fits the HBM budget (resident cases plus the
-// workspace pool), and retires finished ones, whose workspaces go back to the
-// pool for the next request of the same size.
+// Continuous batching: requests queue FIFO; between batched iterations the
+// server drops requests whose admission deadline has passed, admits queued
+// requests into free slots as long as a Krylov workspace fits the HBM budget
+// (resident cases plus the workspace pool), and retires finished ones, whose
+// workspaces go back to the pool for the next request of the same size.  A case
+// runs up to max_per_case requests at once: the first on the handle that holds
+// its geometry, the others on solve clones of it, made on demand and kept.
 struct vfs_pp_server {
-  struct Case {
+  struct Solver {
     vfs_pp_handle* h = nullptr;
-    size_t ws = 0;
-    size_t foot = 0;  // device bytes the case holds while registered
     bool busy = false;
+  };
+  struct Case {
+    std::vector<Solver> solvers;  // [0] owns the geometry, the rest are clones of it
+    size_t ws = 0;
+    size_t foot = 0;  // device bytes the case holds while registered (clones included)
+    int busy() const {
+      int b = 0;
+      for (const Solver& v : solvers) b += v.busy ? 1 : 0;
+      return b;
+    }
   };
   struct Req {
     int ticket, cid;
     double *ucont, *phi, *p, *ucat;
     double dt, st, coeff;
+    double t_submit;
   };
   vfs_pp_server_opts o{};
   std::map<int, Case> cases;
   std::deque<Req> queue;
-  std::vector<int> slot_ticket, slot_case;
+  std::deque<int> expired;
+  std::vector<int> slot_ticket, slot_case, slot_solver;
   std::vector<void*> slot_ws;
   std::multimap<size_t, void*> pool;  // free workspaces by size
   size_t resident = 0, ws_held = 0, ws_max = 0;
@@ -2075,7 +2087,14 @@ struct vfs_pp_server {
     for (auto& kv : pool) (void)hipFree(kv.second);
     for (void* w : slot_ws)
       if (w) (void)hipFree(w);
-    for (auto& kv : cases) vfs_pp_destroy(kv.second.h);
+    for (auto& kv : cases) destroy_case(kv.second);
+  }
+
+  static void destroy_case(Case& c) {
+    // clones first: they read the owner's arrays
+    for (size_t v = c.solvers.size(); v-- > 1;) vfs_pp_destroy(c.solvers[v].h);
+    vfs_pp_destroy(c.solvers[0].h);
+    c.solvers.clear();
   }
 
   void* take_ws(size_t bytes) {
@@ -2102,26 +2121,73 @@ struct vfs_pp_server {
     return w;
   }
 
+  // a free solver of the case, making a clone when all are busy and the case
+  // may run one more request; -1 when none can be had now
+  int take_solver(Case& c) {
+    for (size_t v = 0; v < c.solvers.size(); v++)
+      if (!c.solvers[v].busy) return (int)v;
+    if ((int)c.solvers.size() >= o.max_per_case) return -1;
+    vfs_pp_handle* h = nullptr;
+    try {
+      h = clone_for_solves(c.solvers[0].h);
+    } catch (const StatusError& e) {
+      if (e.code == VFS_PP_ENOMEM) return -1;
+      throw;
+    }
+    // the clone and a workspace must fit beside everything resident: pooled
+    // workspaces of other sizes go first
+    const size_t foot = h->resident_bytes();
+    while (!pool.empty() && resident + foot + ws_held + c.ws > o.hbm_budget) {
+      auto big = std::prev(pool.end());
+      ws_held -= big->first;
+      (void)hipFree(big->second);
+      pool.erase(big);
+    }
+    if (resident + foot + ws_held + c.ws > o.hbm_budget) {
+      vfs_pp_destroy(h);
+      return -1;
+    }
+    resident += foot;
+    c.foot += foot;
+    c.solvers.push_back(Solver{h, false});
+    return (int)c.solvers.size() - 1;
+  }
+
+  void expire(double now) {
+    if (!(o.deadline_ms > 0.)) return;
+    for (auto it = queue.begin(); it != queue.end();) {
+      if (now - it->t_submit > o.deadline_ms) {
+        expired.push_back(it->ticket);
+        it = queue.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+
   template <typename T>
   int progress(BatchCore<T>& core, int* tickets, vfs_pp_stats* stats, int cap) {
-    // admit: FIFO, skipping requests whose case is already in flight
+    expire(now_ms());
+    // admit: FIFO, skipping requests whose case has no solver free now
     for (auto it = queue.begin(); it != queue.end();) {
       int q = -1;
       for (int r = 0; r < (int)slot_ticket.size(); r++)
         if (slot_ticket[r] < 0) { q = r; break; }
       if (q < 0) break;
       Case& c = cases.at(it->cid);
-      if (c.busy) { ++it; continue; }
+      const int v = take_solver(c);
+      if (v < 0) { ++it; continue; }
       void* w = take_ws(c.ws);
       if (!w) break;  // wait for memory to free up
-      vfs_pp_handle* h = c.h;
+      vfs_pp_handle* h = c.solvers[v].h;
       h->V = static_cast<double*>(w);
       h->Z = static_cast<char*>(w) + (size_t)(h->mrest + 1) * h->d.n() * sizeof(double);
       core.bind(q, h);
       core.begin(q, it->ucont, it->phi, it->p, it->ucat, it->dt, it->st, it->coeff);
-      c.busy = true;
+      c.solvers[v].busy = true;
       slot_ticket[q] = it->ticket;
       slot_case[q] = it->cid;
+      slot_solver[q] = v;
       slot_ws[q] = w;
       it = queue.erase(it);
     }
@@ -2136,14 +2202,16 @@ struct vfs_pp_server {
       }
       nd++;
       Case& c = cases.at(slot_case[q]);
-      c.busy = false;
+      Solver& v = c.solvers[slot_solver[q]];
+      v.busy = false;
       core.release(core.slots[q]);
-      c.h->V = nullptr;
-      c.h->Z = nullptr;
+      v.h->V = nullptr;
+      v.h->Z = nullptr;
       pool.emplace(c.ws, slot_ws[q]);
       slot_ws[q] = nullptr;
       slot_ticket[q] = -1;
       slot_case[q] = -1;
+      slot_solver[q] = -1;
     }
     if (nd > cap) throw StatusError(VFS_PP_EINVAL, "more finished requests than the output arrays hold");
     return nd;
@@ -2157,6 +2225,9 @@ void vfs_pp_server_default_opts(vfs_pp_server_opts* o) {
   o->hbm_budget = 0;
   o->max_active = 64;
   vfs_pp_default_opts(&o->solver);
+  o->max_per_case = 4;
+  o->max_queue = 0;
+  o->deadline_ms = 0.;
 }
 
 int vfs_pp_server_create(const vfs_pp_server_opts* o, vfs_pp_server** out) {
@@ -2166,6 +2237,7 @@ int vfs_pp_server_create(const vfs_pp_server_opts* o, vfs_pp_server** out) {
   if (o) sv->o = *o;
   else vfs_pp_server_default_opts(&sv->o);
   if (sv->o.max_active < 1 || sv->o.max_active > 4096) return VFS_PP_EINVAL;
+  if (sv->o.max_per_case < 1 || sv->o.max_queue < 0 || !(sv->o.deadline_ms >= 0.)) return VFS_PP_EINVAL;
   if (sv->o.hbm_budget == 0) {
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
@@ -2174,6 +2246,7 @@ int vfs_pp_server_create(const vfs_pp_server_opts* o, vfs_pp_server** out) {
   const int n = sv->o.max_active;
   sv->slot_ticket.assign(n, -1);
   sv->slot_case.assign(n, -1);
+  sv->slot_solver.assign(n, -1);
   sv->slot_ws.assign(n, nullptr);
   if (sv->o.solver.precision == 1) sv->cf = std::make_unique<BatchCore<float>>(n);
   else sv->cd = std::make_unique<BatchCore<double>>(n);
@@ -2215,7 +2288,11 @@ int vfs_pp_server_add_case(vfs_pp_server* sv, const vfs_grid* g, const double* i
   sv->resident += foot;
   sv->ws_max = std::max(sv->ws_max, ws);
   const int id = sv->next_case++;
-  sv->cases[id] = vfs_pp_server::Case{h, ws, foot, false};
+  vfs_pp_server::Case c;
+  c.solvers.push_back(vfs_pp_server::Solver{h, false});
+  c.ws = ws;
+  c.foot = foot;
+  sv->cases[id] = std::move(c);
   *case_id = id;
   return VFS_PP_OK;
   ABI_END
@@ -2226,11 +2303,11 @@ int vfs_pp_server_remove_case(vfs_pp_server* sv, int case_id) {
   if (!sv) return VFS_PP_EINVAL;
   auto it = sv->cases.find(case_id);
   if (it == sv->cases.end()) return VFS_PP_EINVAL;
-  if (it->second.busy) return VFS_PP_ESTATE;
+  if (it->second.busy()) return VFS_PP_ESTATE;
   for (const auto& r : sv->queue)
     if (r.cid == case_id) return VFS_PP_ESTATE;
   sv->resident -= it->second.foot;
-  vfs_pp_destroy(it->second.h);
+  vfs_pp_server::destroy_case(it->second);
   sv->cases.erase(it);
   return VFS_PP_OK;
   ABI_END
@@ -2241,8 +2318,9 @@ int vfs_pp_server_submit(vfs_pp_server* sv, int case_id, double* ucont, double* 
   ABI_BEGIN
   if (!sv || !ucont || !phi || !p || !ucat || !ticket || !(dt > 0.) || coeff == 0.) return VFS_PP_EINVAL;
   if (!sv->cases.count(case_id)) return VFS_PP_EINVAL;
+  if (sv->o.max_queue > 0 && (int)sv->queue.size() >= sv->o.max_queue) return VFS_PP_EBUSY;
   const int t = sv->next_ticket++;
-  sv->queue.push_back(vfs_pp_server::Req{t, case_id, ucont, phi, p, ucat, dt, st, coeff});
+  sv->queue.push_back(vfs_pp_server::Req{t, case_id, ucont, phi, p, ucat, dt, st, coeff, now_ms()});
   *ticket = t;
   return VFS_PP_OK;
   ABI_END
@@ -2254,6 +2332,17 @@ int vfs_pp_server_progress(vfs_pp_server* sv, int* tickets, vfs_pp_stats* stats,
   *ndone = sv->cf ? sv->progress(*sv->cf, tickets, stats, cap) : sv->progress(*sv->cd, tickets, stats, cap);
   return VFS_PP_OK;
   ABI_END
+}
+
+int vfs_pp_server_expired(vfs_pp_server* sv, int* tickets, int cap, int* n) {
+  if (!sv || !n || cap < 0 || (cap > 0 && !tickets)) return VFS_PP_EINVAL;
+  int k = 0;
+  while (k < cap && !sv->expired.empty()) {
+    tickets[k++] = sv->expired.front();
+    sv->expired.pop_front();
+  }
+  *n = k;
+  return VFS_PP_OK;
 }
 
 int vfs_pp_server_status(vfs_pp_server* sv, int* queued, int* active, size_t* resident_bytes,

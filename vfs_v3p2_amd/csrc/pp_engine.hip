@@ -663,6 +663,52 @@ static double divergence(vfs_pp_handle* h, const double* ucont) {
   return h->hpin[RED_DIV];
 }
 
+vfs_pp_handle* clone_for_solves(const vfs_pp_handle* src) {
+  if (!src->have_geom || !src->mg) throw StatusError(VFS_PP_ESTATE, "the case has no geometry to share");
+  auto h = std::make_unique<vfs_pp_handle>();
+  h->grid = src->grid;
+  h->opts = src->opts;
+  h->m = src->m;
+  h->d = src->d;
+  h->immersed_ref = src->immersed_ref;
+  HIPCHECK(hipStreamCreateWithFlags(&h->s, hipStreamDefault));
+  try {
+    // read-only resident data of the owner
+    h->g64 = src->g64;
+    h->code = src->code;
+    h->cls = src->cls;
+    h->met9 = src->met9;
+    h->aj = src->aj;
+    h->nvert = src->nvert;
+    h->ibf = src->ibf;
+    h->nibf = src->nibf;
+    h->n_lt = src->n_lt;
+    h->desc = src->desc;
+    // per-solve state of its own, sized as the owner's
+    const size_t n = h->d.n(), M = h->m.n();
+    h->mrest = src->mrest;
+    h->stencil_blocks = src->stencil_blocks;
+    h->partial_cap = src->partial_cap;
+    h->x = h->ar.alloc<double>(n);
+    h->b = h->ar.alloc<double>(n);
+    h->r = h->ar.alloc<double>(n);
+    h->partial = h->ar.alloc<double>(h->partial_cap);
+    h->red = h->ar.alloc<double>(512);
+    h->snap = h->ar.alloc<double>(3 * M);
+    h->external_ws = true;
+    HIPCHECK(hipHostMalloc(&h->hpin, 4096 * sizeof(double)));
+    HIPCHECK(hipHostGetDevicePointer((void**)&h->hpin_dev, h->hpin, 0));
+    h->mg = src->mg->clone_work(h->geo);
+    h->geometry_of = src;
+    h->have_metrics = true;
+    h->have_geom = true;
+  } catch (...) {
+    vfs_pp_destroy(h.release());
+    throw;
+  }
+  return h.release();
+}
+
 }  // namespace pp
 
 // ======================================================================= C ABI
@@ -698,6 +744,7 @@ const char* vfs_pp_error_string(int code) {
     case VFS_PP_ESTATE: return "call order violated";
     case VFS_PP_ENONFINITE: return "non-finite values";
     case VFS_PP_EIO: return "grid file unreadable";
+    case VFS_PP_EBUSY: return "request refused: server queue full";
     default: return "unknown";
   }
 }

@@ -122,6 +122,9 @@ struct MGBase {
   // level l: dimensions, smoother, code words; its face coefficients as FP64 into
   // g64 (device, 9 n)
   virtual void level_export(hipStream_t s, int l, Dims* d, int* line, const uint32_t** code, double* g64) const = 0;
+  // a hierarchy that shares this one's coefficients, masks, Chebyshev parameters
+  // and coarsest inverse and owns fresh work vectors (allocated from A)
+  virtual std::unique_ptr<MGBase> clone_work(Arena& A) const = 0;
 };
 
 template <typename T>
@@ -238,6 +241,20 @@ struct MG : MGBase {
     else launch_convert<T, double>(s, n9, L.g, g64);
   }
 
+  std::unique_ptr<MGBase> clone_work(Arena& A) const override {
+    auto c = std::make_unique<MG<T>>(*this);
+    c->scr = A.alloc<T>(2 * (lev.empty() ? 1 : lev[0].d.n()));
+    for (auto& L : c->lev) {
+      const size_t n = L.d.n();
+      L.f = A.alloc<T>(n);
+      L.xa = A.alloc<T>(n);
+      L.xb = A.alloc<T>(n);
+      L.dv = A.alloc<T>(n);
+    }
+    c->out = nullptr;
+    return c;
+  }
+
   void time_sweep(hipStream_t s) override {
     Lev& L = lev[0];
     launch_smooth<T>(s, L.view(), L.lfac, SM_NEXT, L.f, L.dinv, L.xa, L.xb, L.xb, L.dv, scr,
@@ -337,6 +354,10 @@ struct vfs_pp_handle {
   double* hpin = nullptr;  // pinned host scratch
   double* hpin_dev = nullptr;  // the same memory as seen by kernels
   std::string desc;
+  // A solve clone (made by clone_for_solves) reads this handle's geometry, masks,
+  // IB face list and hierarchy coefficients and owns only its solve vectors; the
+  // owner must outlive it and keep its geometry unchanged meanwhile.
+  const vfs_pp_handle* geometry_of = nullptr;
 };
 
 namespace pp {
@@ -365,6 +386,10 @@ void update_pressure(vfs_pp_handle* h, double* p, double* phi);
 void projection(vfs_pp_handle* h, double* ucont, const double* phi, double* ucat, double dt, double st,
                 double coeff);
 void divergence_launch(vfs_pp_handle* h, const double* ucont);
+// A handle that solves on src's resident geometry with its own vectors, reduction
+// scratch, pinned scalars, stream and multigrid work arrays (no Krylov basis: it
+// is attached per solve, as for a server case).  Internal to the served form.
+vfs_pp_handle* clone_for_solves(const vfs_pp_handle* src);
 // A handle bound to a batch or a server slot runs on the batch's stream and its
 // buffers are referenced by the batch's work lists and captured graph: the
 // single-case entry points refuse it until the batch lets it go.

@@ -28,7 +28,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvfs_pp.so")
 
-VFS_PP_OK, VFS_PP_EINVAL, VFS_PP_ENOMEM, VFS_PP_EHIP, VFS_PP_ENOCONV, VFS_PP_ESTATE, VFS_PP_ENONFINITE = range(7)
+(VFS_PP_OK, VFS_PP_EINVAL, VFS_PP_ENOMEM, VFS_PP_EHIP, VFS_PP_ENOCONV, VFS_PP_ESTATE, VFS_PP_ENONFINITE, VFS_PP_EIO,
+ VFS_PP_EBUSY) = range(9)
 
 
 class VfsPPError(RuntimeError):
@@ -50,7 +51,8 @@ class Opts(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("solver", Opts)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("solver", Opts), ("max_per_case", C.c_int),
+                ("max_queue", C.c_int), ("deadline_ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -137,6 +139,7 @@ def lib():
                                            C.POINTER(C.c_int)]),
         "vfs_pp_server_progress": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(Stats), C.c_int,
                                              C.POINTER(C.c_int)]),
+        "vfs_pp_server_expired": (C.c_int, [H, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
         "vfs_pp_server_status": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_size_t),
                                            C.POINTER(C.c_size_t)]),
     }
@@ -424,13 +427,22 @@ class Server:
     resident; `submit` queues a projection of a case on device tensors and
     `progress` admits queued requests into free slots as the HBM budget allows,
     runs one batched iteration of everything in flight, and returns the
-    requests that finished as (ticket, Stats) pairs."""
+    requests that finished as (ticket, Stats) pairs.  Up to `max_per_case`
+    requests of one case run at once; `max_queue` (> 0) refuses submissions
+    while that many wait, and `deadline_ms` (> 0) drops requests not admitted in
+    time (their tickets come back from `expired`)."""
 
-    def __init__(self, max_active=64, hbm_budget=0, **solver):
+    def __init__(self, max_active=64, hbm_budget=0, max_per_case=None, max_queue=None, deadline_ms=None, **solver):
         o = ServerOpts()
         lib().vfs_pp_server_default_opts(C.byref(o))
         o.max_active = int(max_active)
         o.hbm_budget = int(hbm_budget)
+        if max_per_case is not None:
+            o.max_per_case = int(max_per_case)
+        if max_queue is not None:
+            o.max_queue = int(max_queue)
+        if deadline_ms is not None:
+            o.deadline_ms = float(deadline_ms)
         for k, v in solver.items():
             setattr(o.solver, k, v)
         self.max_active = o.max_active
@@ -468,15 +480,31 @@ class Server:
         _check(lib().vfs_pp_server_remove_case(self._s, cid), "vfs_pp_server_remove_case")
         self.grids.pop(cid, None)
 
-    def submit(self, cid, ucont, phi, p, ucat, dt, st, coeff) -> int:
+    def submit(self, cid, ucont, phi, p, ucat, dt, st, coeff) -> Optional[int]:
+        """The request's ticket, or None when the full queue refused it."""
         g = self.grids[cid]
         t = C.c_int()
-        _check(lib().vfs_pp_server_submit(self._s, cid, _tptr(ucont, dmda_shape(g, True), "ucont"),
-                                          _tptr(phi, dmda_shape(g), "phi"), _tptr(p, dmda_shape(g), "p"),
-                                          _tptr(ucat, dmda_shape(g, True), "ucat"), dt, st, coeff, C.byref(t)),
-               "vfs_pp_server_submit")
-        self._keep[t.value] = (ucont, phi, p, ucat)  # the arrays stay alive until the request finishes
+        rc = _check(lib().vfs_pp_server_submit(self._s, cid, _tptr(ucont, dmda_shape(g, True), "ucont"),
+                                               _tptr(phi, dmda_shape(g), "phi"), _tptr(p, dmda_shape(g), "p"),
+                                               _tptr(ucat, dmda_shape(g, True), "ucat"), dt, st, coeff, C.byref(t)),
+                    "vfs_pp_server_submit", allow=(VFS_PP_EBUSY,))
+        if rc == VFS_PP_EBUSY:
+            return None
+        self._keep[t.value] = (ucont, phi, p, ucat)  # the arrays stay alive until the request finishes or expires
         return t.value
+
+    def expired(self):
+        """Tickets dropped by the admission deadline since the last call."""
+        out = []
+        buf = (C.c_int * 256)()
+        n = C.c_int()
+        while True:
+            _check(lib().vfs_pp_server_expired(self._s, buf, 256, C.byref(n)), "vfs_pp_server_expired")
+            for q in range(n.value):
+                self._keep.pop(buf[q], None)
+                out.append(buf[q])
+            if n.value < 256:
+                return out
 
     def progress(self):
         cap = self.max_active
