@@ -49,20 +49,24 @@ __device__ __forceinline__ void dinv_cell(const LevelView<T>& L, T* __restrict__
   dinv[c] = ((cd & ROW_BIT) && diag != T(0)) ? T(1) / diag : T(0);
 }
 
-// (A x)_c and the diagonal with the cross-difference cases fixed at compile time
-// (KC == X_CENTRAL: an interior row) or decoded from cd (KC < 0)
-template <int KC, typename T>
-__device__ __forceinline__ void level_flux_k(const LevelView<T>& L, uint32_t cd, size_t c, const Nbr& nb,
-                                             const T (&xv)[19], T& ax, T& diag) {
-  if constexpr (std::is_same<T, float>::value) {
-    if (L.gh) {
-      flux_row_k<KC>(L.gh, L.d.n(), cd, (uint32_t)c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
-      ax *= L.gs;
-      diag *= L.gs;
-      return;
-    }
+// (A x)_c, the diagonal and whether the cell has a row, with the level's class
+// bytes picking the row form: a wave whose cells are all interior (class flag)
+// takes the compile-time-weight form and reads no code word.  The coefficient
+// loads are issued before the class test, together with the caller's stencil
+// loads.
+template <typename T, typename TG>
+__device__ __forceinline__ void flux_by_class(const TG* __restrict__ g, size_t n, const uint32_t* __restrict__ code,
+                                              size_t c, const Nbr& nb, const T (&xv)[19], bool interior, T& ax,
+                                              T& diag, bool& row) {
+  RowCoef<TG> rc;
+  load_coef(g, n, (uint32_t)c, nb.s[WP], nb.s[SP], nb.s[BP], rc);
+  if (__all(interior)) {
+    flux_row_v<X_CENTRAL>(rc, CODE_INTERIOR, xv, ax, diag);
+  } else {
+    const uint32_t cd = code[c];
+    flux_row_v<-1>(rc, cd, xv, ax, diag);
+    row = (cd & ROW_BIT) != 0;
   }
-  flux_row_k<KC>(L.g, L.d.n(), cd, (uint32_t)c, nb.s[WP], nb.s[SP], nb.s[BP], xv, ax, diag);
 }
 
 // Chebyshev smoothing preconditioned by the diagonal, in three-term form: one step is
@@ -76,21 +80,17 @@ __device__ __forceinline__ void level_flux_k(const LevelView<T>& L, uint32_t cd,
 //              D^-1, then one step from x1 with (c1, c2); x_new = x2, x_prev = x1;
 //   SM_POWER   dv = D^-1 (f - A x_old) only.
 // The level's class bytes pick the row form: a wave whose cells are all interior
-// takes the compile-time-weight form and reads no code word.
-template <typename T, int MODE>
-__device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T* __restrict__ f,
-                                                  const T* __restrict__ dinv, const T* __restrict__ x_old,
-                                                  T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
-                                                  int j, int k) {
+// takes the compile-time-weight form and reads no code word.  g holds the
+// level's coefficients in their storage type (FP16 scaled by gs, or T); every
+// load of the cell -- class, f, x_prev, the 19 stencil values, the 18
+// coefficients -- is issued before any of them is used.
+template <typename T, int MODE, typename TG>
+__device__ __forceinline__ void smooth_point_body(const LevelView<T>& L, const TG* __restrict__ g, bool scaled,
+                                                  const T* __restrict__ f, const T* __restrict__ dinv,
+                                                  const T* __restrict__ x_old, T* x_prev, T* x_new,
+                                                  T* __restrict__ dv, T c1, T c2, T c0, int i, int j, int k) {
   const size_t c = L.d.idx(i, j, k);
-  if constexpr (MODE == SM_ZERO) {
-    x_new[c] = c2 * (f[c] * dinv[c]);
-    x_prev[c] = T(0);
-    return;
-  }
-  const bool interior = L.cls[c] & CLS_INTERIOR;
-  // the cell's own operands are loaded up front with the stencil, so their latency
-  // hides behind the stencil's
+  const uint8_t cl = L.cls[c];
   const T fc = f[c];
   T xp = T(0);
   if constexpr (MODE == SM_NEXT) {
@@ -107,12 +107,10 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
   }
   T ax, diag;
   bool row = true;
-  if (__all(interior)) {
-    level_flux_k<X_CENTRAL>(L, CODE_INTERIOR, c, nb, xv, ax, diag);
-  } else {
-    const uint32_t cd = L.code[c];
-    level_flux_k<-1>(L, cd, c, nb, xv, ax, diag);
-    row = (cd & ROW_BIT) != 0;
+  flux_by_class(g, L.d.n(), L.code, c, nb, xv, (cl & CLS_INTERIOR) != 0, ax, diag, row);
+  if (scaled) {
+    ax *= L.gs;
+    diag *= L.gs;
   }
   // straight-line division with a safe divisor: a branch around it would let the
   // compiler move the load of f into the branch, after the stencil
@@ -126,6 +124,27 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
   const T dold = (MODE == SM_FIRST2) ? xc : (c1 == T(0) ? T(0) : xc - xp);
   if constexpr (MODE == SM_FIRST2) x_prev[c] = row ? xc : T(0);
   x_new[c] = row ? xc + (c1 * dold + c2 * z) : T(0);
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T* __restrict__ f,
+                                                  const T* __restrict__ dinv, const T* __restrict__ x_old,
+                                                  T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
+                                                  int j, int k) {
+  if constexpr (MODE == SM_ZERO) {
+    const size_t c = L.d.idx(i, j, k);
+    x_new[c] = c2 * (f[c] * dinv[c]);
+    x_prev[c] = T(0);
+    return;
+  } else {
+    if constexpr (std::is_same<T, float>::value) {
+      if (L.gh) {
+        smooth_point_body<T, MODE>(L, L.gh, true, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k);
+        return;
+      }
+    }
+    smooth_point_body<T, MODE>(L, L.g, false, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k);
+  }
 }
 
 // Thomas factors of the y-line tridiagonal part of A on the line (i, k), from the

@@ -233,16 +233,34 @@ __device__ __forceinline__ T ld_at(const T* __restrict__ a, uint32_t idx) {
   return *(gT*)((gchar*)a + idx * (uint32_t)sizeof(T));
 }
 
+// The 18 face coefficients one row reads, faces in the order i+, i-, j+, j-, k+,
+// k- (owners c, cw, c, cs, c, cb), three per face.  They are loaded in one group,
+// before any test of the row's form, so that every load of the row -- stencil
+// values and coefficients -- is in flight at once: the kernels that use them are
+// bound by memory latency, and a second round of loads after a branch would
+// double it.  Every owner index is a valid cell, whether or not its face is
+// active.
+template <typename TG>
+struct RowCoef {
+  TG v[6][3];
+};
+
+template <typename TG>
+__device__ __forceinline__ void load_coef(const TG* __restrict__ g, size_t n, uint32_t c, uint32_t cw, uint32_t cs,
+                                          uint32_t cb, RowCoef<TG>& rc) {
+  const uint32_t own[6] = {c, cw, c, cs, c, cb};
+#pragma unroll
+  for (int f = 0; f < 6; f++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) rc.v[f][e] = ld_at(g + (size_t)(3 * (f / 2) + e) * n, own[f]);
+}
+
 template <int D, int SGN, int KC, typename T, typename TG>
-__device__ __forceinline__ void flux_face(T& ax, T& diag, const TG* __restrict__ g, size_t n, uint32_t owner, uint32_t code,
-                   const T (&xv)[19]) {
+__device__ __forceinline__ void flux_face(T& ax, T& diag, const TG (&gf)[3], uint32_t code, const T (&xv)[19]) {
   constexpr int F = 2 * D + (SGN > 0 ? 0 : 1);
-  // coefficients are loaded whether or not the face is active (all loads issue
-  // together; the owner index is always a valid cell)
-  const TG* gd = g + (size_t)(3 * D) * n;
-  const T g0 = (T)ld_at(gd, owner);
-  const T g1 = (T)ld_at(gd + n, owner);
-  const T g2 = (T)ld_at(gd + 2 * n, owner);
+  const T g0 = (T)gf[0];
+  const T g1 = (T)gf[1];
+  const T g2 = (T)gf[2];
   T nrm = T(0), crs = T(0), dg = T(0);
   flux_term<D, SGN, 0, KC>(nrm, crs, dg, g0, code, xv);
   flux_term<D, SGN, 1, KC>(nrm, crs, dg, g1, code, xv);
@@ -254,17 +272,18 @@ __device__ __forceinline__ void flux_face(T& ax, T& diag, const TG* __restrict__
   }
 }
 
+// (A x)_c and a_cc from loaded coefficients, the cross-difference cases fixed at
+// compile time (KC == X_CENTRAL: an interior row) or decoded from code (KC < 0)
 template <int KC, typename T, typename TG>
-__device__ __forceinline__ void flux_row_k(const TG* __restrict__ g, size_t n, uint32_t code, uint32_t c, uint32_t cw, uint32_t cs,
-                    uint32_t cb, const T (&xv)[19], T& ax, T& diag) {
+__device__ __forceinline__ void flux_row_v(const RowCoef<TG>& rc, uint32_t code, const T (&xv)[19], T& ax, T& diag) {
   ax = T(0);
   diag = T(0);
-  flux_face<0, +1, KC>(ax, diag, g, n, c, code, xv);
-  flux_face<0, -1, KC>(ax, diag, g, n, cw, code, xv);
-  flux_face<1, +1, KC>(ax, diag, g, n, c, code, xv);
-  flux_face<1, -1, KC>(ax, diag, g, n, cs, code, xv);
-  flux_face<2, +1, KC>(ax, diag, g, n, c, code, xv);
-  flux_face<2, -1, KC>(ax, diag, g, n, cb, code, xv);
+  flux_face<0, +1, KC>(ax, diag, rc.v[0], code, xv);
+  flux_face<0, -1, KC>(ax, diag, rc.v[1], code, xv);
+  flux_face<1, +1, KC>(ax, diag, rc.v[2], code, xv);
+  flux_face<1, -1, KC>(ax, diag, rc.v[3], code, xv);
+  flux_face<2, +1, KC>(ax, diag, rc.v[4], code, xv);
+  flux_face<2, -1, KC>(ax, diag, rc.v[5], code, xv);
 }
 
 // code word of an interior cell: every face active, every cross difference
@@ -276,10 +295,12 @@ constexpr uint32_t CODE_INTERIOR = 0x3Fu | (0x555555u << 6) | ROW_BIT;
 template <typename T, typename TG>
 __device__ __forceinline__ void flux_row(const TG* __restrict__ g, size_t n, uint32_t code, size_t c, size_t cw,
                                          size_t cs, size_t cb, const T (&xv)[19], T& ax, T& diag) {
+  RowCoef<TG> rc;
+  load_coef(g, n, (uint32_t)c, (uint32_t)cw, (uint32_t)cs, (uint32_t)cb, rc);
   if (__all((code & ~DIVOK_BIT) == CODE_INTERIOR))
-    flux_row_k<X_CENTRAL>(g, n, code, (uint32_t)c, (uint32_t)cw, (uint32_t)cs, (uint32_t)cb, xv, ax, diag);
+    flux_row_v<X_CENTRAL>(rc, code, xv, ax, diag);
   else
-    flux_row_k<-1>(g, n, code, (uint32_t)c, (uint32_t)cw, (uint32_t)cs, (uint32_t)cb, xv, ax, diag);
+    flux_row_v<-1>(rc, code, xv, ax, diag);
 }
 
 // Interior indices of the 19 stencil neighbours with periodic wrap.
