@@ -252,6 +252,30 @@ __global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__
   }
 }
 
+// the same step with each line split over the waves of a (64, LINE_SEGS) block,
+// 64 consecutive lines (i fastest, then k) per tile; every thread of the block
+// takes part in the barriers of line_solve_seg, lanes past the last line too
+template <typename T, bool ZG>
+__global__ __launch_bounds__(64 * LINE_SEGS) void kb_line_seg(const SmoothItem<T>* __restrict__ items,
+                                                              const uint32_t* __restrict__ start, int nitems,
+                                                              uint32_t total, const SlotDev* __restrict__ sd) {
+  __shared__ LineSegLds<T> sh;
+  FOR_TILES(total, t) {
+    const uint32_t tw_ = start[nitems + t];
+    const int q = (int)(tw_ >> 16);
+    const SmoothItem<T>& I = items[q];
+    const SlotDev st = sd[tw_ & 0xffffu];
+    if (!st.y) continue;  // the whole block skips the tile
+    const uint32_t nx = (uint32_t)I.L.d.nx;
+    const uint32_t e = (t - start[q]) * 64u + threadIdx.x;
+    const bool live = e < nx * (uint32_t)I.L.d.nz;
+    const int k = live ? (int)(e / nx) : 0;
+    const int i = live ? (int)(e - (uint32_t)k * nx) : 0;
+    T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
+    line_solve_seg<T, ZG>(I.L.d, live, i, k, ZG ? I.f : I.scr, I.lfac, I.xin, xout, I.dv, I.c1, I.c2, sh);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void kb_restrict_blk(const RestrictItem<T>* __restrict__ items,
                                                        const uint32_t* __restrict__ start, int nitems, uint32_t total,
@@ -717,7 +741,7 @@ struct BatchCore {
   FinItem* fin_apply = nullptr;
   FinItem* fin_vec = nullptr;
 
-  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS, K_AXN };
+  enum { K_SMOOTH, K_RBLK, K_RGEN, K_PROLONG, K_DENSE, K_RESID, K_LINE, K_APPLY, K_AXD, K_AXS, K_AXN, K_LSEG };
   bool tuned_once = false;
   int forced_shape = -1;  // >= 0: every cell-tiled launch walks this tile shape (no shape tuning)
   // batched phase steps: pinned staging and device copy of their work lists, and
@@ -820,6 +844,8 @@ struct BatchCore {
 
   static uint32_t cell_tiles(const Dims& d) { return (uint32_t)((d.n() + 255) / 256); }
   static uint32_t line_tiles(const Dims& d) { return (uint32_t)(((size_t)d.nx * d.nz + 255) / 256); }
+  static uint32_t seg_line_tiles(const Dims& d) { return (uint32_t)(((size_t)d.nx * d.nz + 63) / 64); }
+  static bool seg_lines(const Dims& d) { return line_seg_fits<T>(d); }
 
   // Launches of one smoothing step k (0-based) at one depth for a group of
   // members: point levels fuse the first two steps from a zero start; y-line
@@ -827,8 +853,8 @@ struct BatchCore {
   // step needs no residual).  cur[] tracks each member's current x buffer.
   void smoothing_step(int d, int k, bool zero_start, bool final_level0, const std::vector<int>& grp,
                       std::vector<std::vector<T*>>& cur) {
-    std::vector<SmoothItem<T>> pt, rs, lz, lnx;
-    std::vector<uint32_t> npt, nrs, nlz, nlnx;
+    std::vector<SmoothItem<T>> pt, rs, lz, lnx, sz, snx;
+    std::vector<uint32_t> npt, nrs, nlz, nlnx, nsz, nsnx;
     int pmode = -1;
     for (int q : grp) {
       auto& L = slots[q].mg->lev[d];
@@ -879,14 +905,24 @@ struct BatchCore {
         }
         it.tx = (uint32_t)(L.d.nx + 63) / 64;
         it.ty = 0;
-        if (mode == SM_ZERO) { lz.push_back(it); nlz.push_back(line_tiles(L.d)); }
-        else { lnx.push_back(it); nlnx.push_back(line_tiles(L.d)); }
+        if (seg_lines(L.d)) {
+          if (mode == SM_ZERO) { sz.push_back(it); nsz.push_back(seg_line_tiles(L.d)); }
+          else { snx.push_back(it); nsnx.push_back(seg_line_tiles(L.d)); }
+        } else if (mode == SM_ZERO) {
+          lz.push_back(it);
+          nlz.push_back(line_tiles(L.d));
+        } else {
+          lnx.push_back(it);
+          nlnx.push_back(line_tiles(L.d));
+        }
       }
     }
     add_launch_cells(K_SMOOTH, pmode, pt);
     add_launch_cells(K_RESID, 0, rs);
     add_launch(K_LINE, SM_ZERO, lz, nlz);
     add_launch(K_LINE, SM_NEXT, lnx, nlnx);
+    add_launch(K_LSEG, SM_ZERO, sz, nsz);
+    add_launch(K_LSEG, SM_NEXT, snx, nsnx);
   }
 
   void build_plan() {
@@ -1146,6 +1182,14 @@ struct BatchCore {
         else
           kb_line<T, false><<<W, dim3(64, 4, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items), L.start,
                                                          L.nitems, L.total, sd);
+        break;
+      case K_LSEG:
+        if (L.mode == SM_ZERO)
+          kb_line_seg<T, true><<<W, dim3(64, LINE_SEGS, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
+                                                                    L.start, L.nitems, L.total, sd);
+        else
+          kb_line_seg<T, false><<<W, dim3(64, LINE_SEGS, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
+                                                                     L.start, L.nitems, L.total, sd);
         break;
       case K_RBLK:
         kb_restrict_blk<T><<<W, dim3(64, 2, 2), 0, s>>>(static_cast<const RestrictItem<T>*>(L.items), L.start,

@@ -263,6 +263,137 @@ __device__ __forceinline__ void line_solve_col(const Dims& d, const T* __restric
   }
 }
 
+// The same Chebyshev step with each line split into LINE_SEGS segments of at most
+// line_seg_rows<T>() rows, one per wave of a (64, LINE_SEGS) block that holds 64 lines
+// (one per lane): lines of up to 256 rows keep LINE_SEGS times more lanes busy than
+// one thread per line, and the forward sweep stays in registers instead of going
+// through memory.  Both sweeps are affine recurrences, d_j = (r_j - a_j d_{j-1})
+// inv_j and z_j = d_j - cp_j z_{j+1}; each segment first runs its sweep from a zero
+// carry, keeping the product of its multipliers, the carries into every segment
+// follow from one pass over the segments of a line (LDS), and each segment then
+// reruns the recurrence exactly from its carry.  Every thread of the block must
+// call it (two block barriers per sweep); lanes without a line pass live = false.
+constexpr int LINE_SEGS = 16;
+// rows per segment held in registers: 16 in FP32 (lines up to 256 rows), 8 in FP64
+// (up to 128 rows), within the 128 registers a 1024-thread block allows
+template <typename T>
+constexpr int line_seg_rows() {
+  return sizeof(T) == 4 ? 16 : 8;
+}
+template <typename T>
+HDI bool line_seg_fits(const Dims& d) {
+  return d.ny <= LINE_SEGS * line_seg_rows<T>();
+}
+template <typename T>
+struct LineSegLds {
+  T part[2][LINE_SEGS][64];  // a segment's zero-carry end value and multiplier product
+  T carry[LINE_SEGS][64];    // carry into each segment
+};
+
+template <typename T, bool ZG>
+__device__ __forceinline__ void line_solve_seg(const Dims& d, bool live, int i, int k, const T* __restrict__ r,
+                                               const T* __restrict__ lfac, const T* __restrict__ x_old,
+                                               T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2,
+                                               LineSegLds<T>& sh) {
+  constexpr int LM = line_seg_rows<T>();
+  const int lane = threadIdx.x, s = threadIdx.y;
+  const int ny = d.ny;
+  const int L = (ny + LINE_SEGS - 1) / LINE_SEGS;
+  const int j0 = s * L;
+  const int nr = live ? max(0, min(L, ny - j0)) : 0;
+  const size_t n = d.n();
+  const size_t sy = (size_t)d.nx;
+  const size_t c0 = live ? d.idx(i, j0 < ny ? j0 : 0, k) : 0;
+  const T* __restrict__ la = lfac;
+  const T* __restrict__ lcp = lfac + n;
+  const T* __restrict__ linv = lfac + 2 * n;
+  T rr[LM], aa[LM], vv[LM];
+#pragma unroll
+  for (int q = 0; q < LM; q++) {
+    rr[q] = aa[q] = vv[q] = T(0);
+    if (q < nr) {
+      const size_t c = c0 + (size_t)q * sy;
+      rr[q] = r[c];
+      aa[q] = la[c];
+      vv[q] = linv[c];
+    }
+  }
+  // forward sweep from a zero carry: end value and product of the multipliers
+  T dl = T(0), prod = T(1);
+#pragma unroll
+  for (int q = 0; q < LM; q++)
+    if (q < nr) {
+      dl = (rr[q] - aa[q] * dl) * vv[q];
+      prod = prod * -(aa[q] * vv[q]);
+    }
+  sh.part[0][s][lane] = dl;
+  sh.part[1][s][lane] = prod;
+  __syncthreads();
+  if (s == 0) {
+    T c = T(0);
+    for (int t = 0; t < LINE_SEGS; t++) {
+      sh.carry[t][lane] = c;
+      c = sh.part[0][t][lane] + sh.part[1][t][lane] * c;
+    }
+  }
+  __syncthreads();
+  T dd[LM];
+  T dprev = sh.carry[s][lane];
+#pragma unroll
+  for (int q = 0; q < LM; q++) {
+    dd[q] = T(0);
+    if (q < nr) {
+      dprev = (rr[q] - aa[q] * dprev) * vv[q];
+      dd[q] = dprev;
+    }
+  }
+  // backward sweep
+  const bool rd = !ZG && c1 != T(0);
+  T pp[LM], dvo[LM], xo[LM];
+#pragma unroll
+  for (int q = 0; q < LM; q++) {
+    pp[q] = dvo[q] = xo[q] = T(0);
+    if (q < nr) {
+      const size_t c = c0 + (size_t)q * sy;
+      pp[q] = lcp[c];
+      if (rd) dvo[q] = dv[c];
+      if (!ZG) xo[q] = x_old[c];
+    }
+  }
+  T zl = T(0);
+  prod = T(1);
+#pragma unroll
+  for (int q = LM - 1; q >= 0; q--)
+    if (q < nr) {
+      zl = dd[q] - pp[q] * zl;
+      prod = prod * -pp[q];
+    }
+  __syncthreads();  // every thread has read its forward carry
+  sh.part[0][s][lane] = zl;
+  sh.part[1][s][lane] = prod;
+  __syncthreads();
+  if (s == 0) {
+    T c = T(0);
+    for (int t = LINE_SEGS - 1; t >= 0; t--) {
+      sh.carry[t][lane] = c;
+      c = sh.part[0][t][lane] + sh.part[1][t][lane] * c;
+    }
+  }
+  __syncthreads();
+  T znext = sh.carry[s][lane];
+#pragma unroll
+  for (int q = LM - 1; q >= 0; q--)
+    if (q < nr) {
+      const size_t c = c0 + (size_t)q * sy;
+      const T z = dd[q] - pp[q] * znext;
+      znext = z;
+      const T dn = c1 * dvo[q] + c2 * z;
+      dv[c] = dn;
+      x_new[c] = xo[q] + dn;
+    }
+  __syncthreads();  // the LDS is free for the block's next tile
+}
+
 // fc[C] = sum over fluid children of (f - A x); zero on coarse IB cells.  Any
 // coarsening mask; one thread per coarse cell.
 template <typename T>
