@@ -253,8 +253,9 @@ __global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__
 }
 
 // the same step with each line split over the waves of a (64, LINE_SEGS) block,
-// 64 consecutive lines (i fastest, then k) per tile; every thread of the block
-// takes part in the barriers of line_solve_seg, lanes past the last line too
+// 64 consecutive lines (i fastest, then k) per tile, the residual of a step from a
+// non-zero x computed in the same pass; every thread of the block takes part in
+// the barriers of the segmented sweeps, lanes past the last line too
 template <typename T, bool ZG>
 __global__ __launch_bounds__(64 * LINE_SEGS) void kb_line_seg(const SmoothItem<T>* __restrict__ items,
                                                               const uint32_t* __restrict__ start, int nitems,
@@ -272,7 +273,10 @@ __global__ __launch_bounds__(64 * LINE_SEGS) void kb_line_seg(const SmoothItem<T
     const int k = live ? (int)(e / nx) : 0;
     const int i = live ? (int)(e - (uint32_t)k * nx) : 0;
     T* xout = I.out_base ? I.out_base + (size_t)st.x * I.out_stride : I.xout;
-    line_solve_seg<T, ZG>(I.L.d, live, i, k, ZG ? I.f : I.scr, I.lfac, I.xin, xout, I.dv, I.c1, I.c2, sh);
+    if constexpr (ZG)
+      line_solve_seg<T, true>(I.L.d, live, i, k, I.f, I.lfac, I.xin, xout, I.dv, I.c1, I.c2, sh);
+    else
+      line_step_cell<T>(I.L, live, i, k, I.f, I.lfac, I.xin, xout, I.dv, I.c1, I.c2, sh);
   }
 }
 
@@ -896,7 +900,9 @@ struct BatchCore {
         pt.push_back(it);
         npt.push_back(cell_tiles(L.d));
       } else {
-        if (mode != SM_ZERO) {
+        // long lines: a residual launch, then one thread per line; lines that fit
+        // the segmented layout take their residual in the line-step pass itself
+        if (mode != SM_ZERO && !seg_lines(L.d)) {
           SmoothItem<T> r = it;
           r.tx = (uint32_t)(L.d.nx + 63) / 64;
           r.ty = (uint32_t)(L.d.ny + 3) / 4;

@@ -70,6 +70,17 @@ __global__ __launch_bounds__(64 * LINE_SEGS) void k_line_seg(Dims d, const T* __
   line_solve_seg<T, ZG>(d, i < d.nx, i, k, r, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
+// residual and line step in one pass, same block layout as k_line_seg
+template <typename T>
+__global__ __launch_bounds__(64 * LINE_SEGS) void k_line_fused(LevelView<T> L, const T* __restrict__ f,
+                                                               const T* __restrict__ lfac,
+                                                               const T* __restrict__ x_old, T* __restrict__ x_new,
+                                                               T* __restrict__ dv, T c1, T c2) {
+  __shared__ LineSegLds<T> sh;
+  const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y;
+  line_step_cell<T>(L, i < L.d.nx, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_resid_restrict(LevelView<T> F, const T* __restrict__ f,
                                                         const T* __restrict__ x, Dims dc,
@@ -228,12 +239,12 @@ void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const
     if (mode == SM_ZERO) {
       if (seg) k_line_seg<T, true><<<sgrid, sblk, 0, s>>>(L.d, f, lfac, x_old, x_new, dv, c1, c2);
       else k_line_solve<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L.d, f, lfac, scr + n, x_old, x_new, dv, c1, c2);
+    } else if (seg) {
+      k_line_fused<T><<<sgrid, sblk, 0, s>>>(L, f, lfac, x_old, x_new, dv, c1, c2);
     } else {
       k_resid<T><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, x_old, scr);
       HIPCHECK(hipGetLastError());
-      if (seg) k_line_seg<T, false><<<sgrid, sblk, 0, s>>>(L.d, scr, lfac, x_old, x_new, dv, c1, c2);
-      else
-        k_line_solve<T, false><<<line_grid(L.d), kBlk, 0, s>>>(L.d, scr, lfac, scr + n, x_old, x_new, dv, c1, c2);
+      k_line_solve<T, false><<<line_grid(L.d), kBlk, 0, s>>>(L.d, scr, lfac, scr + n, x_old, x_new, dv, c1, c2);
     }
   } else {
     switch (mode) {

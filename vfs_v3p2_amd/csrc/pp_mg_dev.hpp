@@ -184,18 +184,36 @@ __device__ __forceinline__ void line_factors_col(const LevelView<T>& L, T* __res
   }
 }
 
-// r = f - A x on cells with an unknown, 0 elsewhere
-template <typename T>
-__device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __restrict__ f, const T* __restrict__ x,
-                                           T* __restrict__ r, int i, int j, int k) {
+// r = f - A x on cells with an unknown, 0 elsewhere.  The class byte picks the
+// row form (interior waves read no code word); every load is issued before the
+// vote.
+template <typename T, typename TG>
+__device__ __forceinline__ void resid_body(const LevelView<T>& L, const TG* __restrict__ g, bool scaled,
+                                           const T* __restrict__ f, const T* __restrict__ x, T* __restrict__ r,
+                                           int i, int j, int k) {
   const size_t c = L.d.idx(i, j, k);
-  const uint32_t cd = L.code[c];
+  const uint8_t cl = L.cls[c];
+  const T fc = f[c];
   Nbr nb;
   neighbours(L.d, i, j, k, nb);
   T xv[19], ax, dg;
   load_x(nb, x, xv);
-  level_flux(L, cd, c, nb, xv, ax, dg);
-  r[c] = (cd & ROW_BIT) ? f[c] - ax : T(0);
+  bool row = true;
+  flux_by_class(g, L.d.n(), L.code, c, nb, xv, (cl & CLS_INTERIOR) != 0, ax, dg, row);
+  if (scaled) ax *= L.gs;
+  r[c] = row ? fc - ax : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ void resid_cell(const LevelView<T>& L, const T* __restrict__ f, const T* __restrict__ x,
+                                           T* __restrict__ r, int i, int j, int k) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.gh) {
+      resid_body(L, L.gh, true, f, x, r, i, j, k);
+      return;
+    }
+  }
+  resid_body(L, L.g, false, f, x, r, i, j, k);
 }
 
 // One Chebyshev step preconditioned by the y-line tridiagonal part T of A on the
@@ -290,30 +308,26 @@ struct LineSegLds {
   T carry[LINE_SEGS][64];    // carry into each segment
 };
 
+// The two segmented sweeps of one Chebyshev step: rr holds the segment's nr rows
+// of r (rows j0 .. j0 + nr - 1 of the line whose row j0 is cell c0).
 template <typename T, bool ZG>
-__device__ __forceinline__ void line_solve_seg(const Dims& d, bool live, int i, int k, const T* __restrict__ r,
-                                               const T* __restrict__ lfac, const T* __restrict__ x_old,
-                                               T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2,
-                                               LineSegLds<T>& sh) {
+__device__ __forceinline__ void line_seg_sweeps(const Dims& d, size_t c0, int nr,
+                                                const T (&rr)[line_seg_rows<T>()], const T* __restrict__ lfac,
+                                                const T* __restrict__ x_old, T* __restrict__ x_new,
+                                                T* __restrict__ dv, T c1, T c2, LineSegLds<T>& sh) {
   constexpr int LM = line_seg_rows<T>();
   const int lane = threadIdx.x, s = threadIdx.y;
-  const int ny = d.ny;
-  const int L = (ny + LINE_SEGS - 1) / LINE_SEGS;
-  const int j0 = s * L;
-  const int nr = live ? max(0, min(L, ny - j0)) : 0;
   const size_t n = d.n();
   const size_t sy = (size_t)d.nx;
-  const size_t c0 = live ? d.idx(i, j0 < ny ? j0 : 0, k) : 0;
   const T* __restrict__ la = lfac;
   const T* __restrict__ lcp = lfac + n;
   const T* __restrict__ linv = lfac + 2 * n;
-  T rr[LM], aa[LM], vv[LM];
+  T aa[LM], vv[LM];
 #pragma unroll
   for (int q = 0; q < LM; q++) {
-    rr[q] = aa[q] = vv[q] = T(0);
+    aa[q] = vv[q] = T(0);
     if (q < nr) {
       const size_t c = c0 + (size_t)q * sy;
-      rr[q] = r[c];
       aa[q] = la[c];
       vv[q] = linv[c];
     }
@@ -392,6 +406,140 @@ __device__ __forceinline__ void line_solve_seg(const Dims& d, bool live, int i, 
       x_new[c] = xo[q] + dn;
     }
   __syncthreads();  // the LDS is free for the block's next tile
+}
+
+// rows of this thread's segment: first row j0, count nr (0 for a lane without a line)
+template <typename T>
+__device__ __forceinline__ void line_seg_rows_of(const Dims& d, bool live, int& j0, int& nr) {
+  const int L = (d.ny + LINE_SEGS - 1) / LINE_SEGS;
+  j0 = (int)threadIdx.y * L;
+  nr = live ? max(0, min(L, d.ny - j0)) : 0;
+}
+
+template <typename T, bool ZG>
+__device__ __forceinline__ void line_solve_seg(const Dims& d, bool live, int i, int k, const T* __restrict__ r,
+                                               const T* __restrict__ lfac, const T* __restrict__ x_old,
+                                               T* __restrict__ x_new, T* __restrict__ dv, T c1, T c2,
+                                               LineSegLds<T>& sh) {
+  constexpr int LM = line_seg_rows<T>();
+  int j0, nr;
+  line_seg_rows_of<T>(d, live, j0, nr);
+  const size_t sy = (size_t)d.nx;
+  const size_t c0 = live ? d.idx(i, j0 < d.ny ? j0 : 0, k) : 0;
+  T rr[LM];
+#pragma unroll
+  for (int q = 0; q < LM; q++) rr[q] = q < nr ? r[c0 + (size_t)q * sy] : T(0);
+  line_seg_sweeps<T, ZG>(d, c0, nr, rr, lfac, x_old, x_new, dv, c1, c2, sh);
+}
+
+// One Chebyshev step (from a non-zero x) with the y-line preconditioner and the
+// residual r = f - A x computed in the same pass: each thread walks its segment's
+// rows, keeping the stencil's values of x in the five columns (i, k), (i +- 1, k),
+// (i, k +- 1) for the rows below, at and above the current one, so a row loads
+// five values of x for the row above plus the four (i +- 1, k +- 1) values of its
+// own row (9 instead of 19), and its lower j-face coefficients are the row below's
+// upper ones.  The row's stencil values and coefficients are the ones the residual
+// kernel reads, and its form is voted over the wave's 64 lines at that row.
+template <typename T, typename TG>
+__device__ __forceinline__ void line_step_fused(const LevelView<T>& L, const TG* __restrict__ g, bool scaled, bool live,
+                                                int i, int k, const T* __restrict__ f, const T* __restrict__ lfac,
+                                                const T* __restrict__ x_old, T* __restrict__ x_new,
+                                                T* __restrict__ dv, T c1, T c2, LineSegLds<T>& sh) {
+  constexpr int LM = line_seg_rows<T>();
+  const Dims& d = L.d;
+  int j0, nr;
+  line_seg_rows_of<T>(d, live, j0, nr);
+  const uint32_t nx = (uint32_t)d.nx, plane = (uint32_t)d.nx * (uint32_t)d.ny;
+  const size_t n = d.n();
+  const int ii = live ? i : 0, kk = live ? k : 0;
+  const uint32_t ip = (uint32_t)wrap_p(ii, d.nx, d.per[0]), im = (uint32_t)wrap_m(ii, d.nx, d.per[0]);
+  const uint32_t kp = (uint32_t)wrap_p(kk, d.nz, d.per[2]) * plane, km = (uint32_t)wrap_m(kk, d.nz, d.per[2]) * plane;
+  const uint32_t ic = (uint32_t)ii, kc = (uint32_t)kk * plane;
+  // column bases (cell index of row 0) of the marched columns C, E, W, T, B and of
+  // the four corner columns read at the row itself
+  const uint32_t col[5] = {ic + kc, ip + kc, im + kc, ic + kp, ic + km};
+  const uint32_t cor[4] = {ip + kp, ip + km, im + kp, im + km};  // TE, BE, TW, BW
+  const int jr = j0 < d.ny ? j0 : 0;
+  const uint32_t rowm0 = (uint32_t)wrap_m(jr, d.ny, d.per[1]) * nx;
+  T xm[5], x0[5];
+#pragma unroll
+  for (int a = 0; a < 5; a++) {
+    xm[a] = nr > 0 ? (T)ld_at(x_old, col[a] + rowm0) : T(0);
+    x0[a] = nr > 0 ? (T)ld_at(x_old, col[a] + (uint32_t)jr * nx) : T(0);
+  }
+  // upper j-face coefficients of the row below the segment
+  TG gs_prev[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++) gs_prev[e] = nr > 0 ? ld_at(g + (size_t)(3 + e) * n, ic + kc + rowm0) : TG(0);
+  T rr[LM];
+#pragma unroll
+  for (int q = 0; q < LM; q++) {
+    rr[q] = T(0);
+    if (q < nr) {
+      const int j = j0 + q;
+      const uint32_t rj = (uint32_t)j * nx, rp = (uint32_t)wrap_p(j, d.ny, d.per[1]) * nx;
+      const uint32_t c = ic + kc + rj;
+      const uint8_t cl = L.cls[c];
+      const T fc = f[c];
+      T xp_[5];
+#pragma unroll
+      for (int a = 0; a < 5; a++) xp_[a] = (T)ld_at(x_old, col[a] + rp);
+      T xv[19];
+      xv[CP] = x0[0]; xv[NP] = xp_[0]; xv[SP] = xm[0];
+      xv[EP] = x0[1]; xv[NE] = xp_[1]; xv[SE] = xm[1];
+      xv[WP] = x0[2]; xv[NW] = xp_[2]; xv[SW] = xm[2];
+      xv[TP] = x0[3]; xv[TN] = xp_[3]; xv[TS] = xm[3];
+      xv[BP] = x0[4]; xv[BN] = xp_[4]; xv[BS] = xm[4];
+      xv[TE] = (T)ld_at(x_old, cor[0] + rj);
+      xv[BE] = (T)ld_at(x_old, cor[1] + rj);
+      xv[TW] = (T)ld_at(x_old, cor[2] + rj);
+      xv[BW] = (T)ld_at(x_old, cor[3] + rj);
+      RowCoef<TG> rc;
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        rc.v[0][e] = ld_at(g + (size_t)e * n, c);
+        rc.v[1][e] = ld_at(g + (size_t)e * n, im + kc + rj);
+        rc.v[2][e] = ld_at(g + (size_t)(3 + e) * n, c);
+        rc.v[3][e] = gs_prev[e];
+        rc.v[4][e] = ld_at(g + (size_t)(6 + e) * n, c);
+        rc.v[5][e] = ld_at(g + (size_t)(6 + e) * n, ic + km + rj);
+      }
+      T ax, dg;
+      bool row = true;
+      if (__all((cl & CLS_INTERIOR) != 0)) {
+        flux_row_v<X_CENTRAL>(rc, CODE_INTERIOR, xv, ax, dg);
+      } else {
+        const uint32_t cd = L.code[c];
+        flux_row_v<-1>(rc, cd, xv, ax, dg);
+        row = (cd & ROW_BIT) != 0;
+      }
+      if (scaled) ax *= L.gs;
+      rr[q] = row ? fc - ax : T(0);
+#pragma unroll
+      for (int a = 0; a < 5; a++) {
+        xm[a] = x0[a];
+        x0[a] = xp_[a];
+      }
+#pragma unroll
+      for (int e = 0; e < 3; e++) gs_prev[e] = rc.v[2][e];
+    }
+  }
+  const size_t c0 = nr > 0 ? d.idx(ii, j0, kk) : 0;
+  line_seg_sweeps<T, false>(d, c0, nr, rr, lfac, x_old, x_new, dv, c1, c2, sh);
+}
+
+template <typename T>
+__device__ __forceinline__ void line_step_cell(const LevelView<T>& L, bool live, int i, int k,
+                                               const T* __restrict__ f, const T* __restrict__ lfac,
+                                               const T* __restrict__ x_old, T* __restrict__ x_new,
+                                               T* __restrict__ dv, T c1, T c2, LineSegLds<T>& sh) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.gh) {
+      line_step_fused(L, L.gh, true, live, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
+      return;
+    }
+  }
+  line_step_fused(L, L.g, false, live, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
 // fc[C] = sum over fluid children of (f - A x); zero on coarse IB cells.  Any
