@@ -252,12 +252,12 @@ __global__ __launch_bounds__(256) void kb_line(const SmoothItem<T>* __restrict__
   }
 }
 
-// the same step with each line split over the waves of a (64, LINE_SEGS) block,
-// 64 consecutive lines (i fastest, then k) per tile, the residual of a step from a
+// the same step with each line split over the waves of a (64, 16) block,
+// line_lanes<T>() consecutive lines (i fastest, then k) per tile, the residual of a step from a
 // non-zero x computed in the same pass; every thread of the block takes part in
 // the barriers of the segmented sweeps, lanes past the last line too
 template <typename T, bool ZG>
-__global__ __launch_bounds__(64 * LINE_SEGS) void kb_line_seg(const SmoothItem<T>* __restrict__ items,
+__global__ __launch_bounds__(1024) void kb_line_seg(const SmoothItem<T>* __restrict__ items,
                                                               const uint32_t* __restrict__ start, int nitems,
                                                               uint32_t total, const SlotDev* __restrict__ sd) {
   __shared__ LineSegLds<T> sh;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64 * LINE_SEGS) void kb_line_seg(const SmoothItem<T
     const SlotDev st = sd[tw_ & 0xffffu];
     if (!st.y) continue;  // the whole block skips the tile
     const uint32_t nx = (uint32_t)I.L.d.nx;
-    const uint32_t e = (t - start[q]) * 64u + threadIdx.x;
+    const uint32_t e = (t - start[q]) * (uint32_t)line_lanes<T>() + (uint32_t)line_lane<T>();
     const bool live = e < nx * (uint32_t)I.L.d.nz;
     const int k = live ? (int)(e / nx) : 0;
     const int i = live ? (int)(e - (uint32_t)k * nx) : 0;
@@ -848,7 +848,9 @@ struct BatchCore {
 
   static uint32_t cell_tiles(const Dims& d) { return (uint32_t)((d.n() + 255) / 256); }
   static uint32_t line_tiles(const Dims& d) { return (uint32_t)(((size_t)d.nx * d.nz + 255) / 256); }
-  static uint32_t seg_line_tiles(const Dims& d) { return (uint32_t)(((size_t)d.nx * d.nz + 63) / 64); }
+  static uint32_t seg_line_tiles(const Dims& d) {
+    return (uint32_t)(((size_t)d.nx * d.nz + line_lanes<T>() - 1) / line_lanes<T>());
+  }
   static bool seg_lines(const Dims& d) { return line_seg_fits<T>(d); }
 
   // Launches of one smoothing step k (0-based) at one depth for a group of
@@ -1191,10 +1193,10 @@ struct BatchCore {
         break;
       case K_LSEG:
         if (L.mode == SM_ZERO)
-          kb_line_seg<T, true><<<W, dim3(64, LINE_SEGS, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
+          kb_line_seg<T, true><<<W, dim3(64, 16, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
                                                                     L.start, L.nitems, L.total, sd);
         else
-          kb_line_seg<T, false><<<W, dim3(64, LINE_SEGS, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
+          kb_line_seg<T, false><<<W, dim3(64, 16, 1), 0, s>>>(static_cast<const SmoothItem<T>*>(L.items),
                                                                      L.start, L.nitems, L.total, sd);
         break;
       case K_RBLK:

@@ -59,25 +59,26 @@ __global__ __launch_bounds__(256) void k_line_solve(Dims d, const T* __restrict_
   line_solve_col<T, ZG>(d, r, lfac, tmp, x_old, x_new, dv, c1, c2, i, k);
 }
 
-// 64 lines (i = 64 blockIdx.x + lane, k = blockIdx.y) per (64, LINE_SEGS) block
+// line_lanes<T>() lines (i = line_lanes<T>() blockIdx.x + line_lane<T>(), k =
+// blockIdx.y) per (64, 16) block
 template <typename T, bool ZG>
-__global__ __launch_bounds__(64 * LINE_SEGS) void k_line_seg(Dims d, const T* __restrict__ r,
+__global__ __launch_bounds__(1024) void k_line_seg(Dims d, const T* __restrict__ r,
                                                              const T* __restrict__ lfac,
                                                              const T* __restrict__ x_old, T* __restrict__ x_new,
                                                              T* __restrict__ dv, T c1, T c2) {
   __shared__ LineSegLds<T> sh;
-  const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y;
+  const int i = blockIdx.x * line_lanes<T>() + line_lane<T>(), k = blockIdx.y;
   line_solve_seg<T, ZG>(d, i < d.nx, i, k, r, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
 // residual and line step in one pass, same block layout as k_line_seg
 template <typename T>
-__global__ __launch_bounds__(64 * LINE_SEGS) void k_line_fused(LevelView<T> L, const T* __restrict__ f,
+__global__ __launch_bounds__(1024) void k_line_fused(LevelView<T> L, const T* __restrict__ f,
                                                                const T* __restrict__ lfac,
                                                                const T* __restrict__ x_old, T* __restrict__ x_new,
                                                                T* __restrict__ dv, T c1, T c2) {
   __shared__ LineSegLds<T> sh;
-  const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y;
+  const int i = blockIdx.x * line_lanes<T>() + line_lane<T>(), k = blockIdx.y;
   line_step_cell<T>(L, i < L.d.nx, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
@@ -233,9 +234,9 @@ void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const
     // y-line smoothing: residual over all cells, then one thread per line
     const size_t n = L.d.n();
     if (mode == SM_FIRST2) throw StatusError(1, "the line smoother has no fused first-two-steps mode");
-    // lines of up to LINE_SEGS x line_seg_rows<T>() rows are split over the waves of a block
+    // lines of up to 256 rows are split over the waves of a block
     const bool seg = line_seg_fits<T>(L.d);
-    const dim3 sgrid((L.d.nx + 63) / 64, L.d.nz), sblk(64, LINE_SEGS);
+    const dim3 sgrid((L.d.nx + line_lanes<T>() - 1) / line_lanes<T>(), L.d.nz), sblk(64, 16);
     if (mode == SM_ZERO) {
       if (seg) k_line_seg<T, true><<<sgrid, sblk, 0, s>>>(L.d, f, lfac, x_old, x_new, dv, c1, c2);
       else k_line_solve<T, true><<<line_grid(L.d), kBlk, 0, s>>>(L.d, f, lfac, scr + n, x_old, x_new, dv, c1, c2);

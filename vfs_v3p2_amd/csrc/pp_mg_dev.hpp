@@ -281,31 +281,49 @@ __device__ __forceinline__ void line_solve_col(const Dims& d, const T* __restric
   }
 }
 
-// The same Chebyshev step with each line split into LINE_SEGS segments of at most
-// line_seg_rows<T>() rows, one per wave of a (64, LINE_SEGS) block that holds 64 lines
-// (one per lane): lines of up to 256 rows keep LINE_SEGS times more lanes busy than
-// one thread per line, and the forward sweep stays in registers instead of going
-// through memory.  Both sweeps are affine recurrences, d_j = (r_j - a_j d_{j-1})
-// inv_j and z_j = d_j - cp_j z_{j+1}; each segment first runs its sweep from a zero
-// carry, keeping the product of its multipliers, the carries into every segment
-// follow from one pass over the segments of a line (LDS), and each segment then
-// reruns the recurrence exactly from its carry.  Every thread of the block must
-// call it (two block barriers per sweep); lanes without a line pass live = false.
-constexpr int LINE_SEGS = 16;
-// rows per segment held in registers: 16 in FP32 (lines up to 256 rows), 8 in FP64
-// (up to 128 rows), within the 128 registers a 1024-thread block allows
+// The same Chebyshev step with each line split into segments of a few rows held
+// in registers.  A (64, 16) block of 1024 threads holds line_lanes<T>() lines:
+// in FP32 64 lines of 16 segments of up to 16 rows (wave w takes segment w of
+// every line), in FP64 32 lines of 32 segments of up to 8 rows (lanes 0-31 of
+// wave w take segment 2w, lanes 32-63 segment 2w + 1), so lines of up to 256 rows
+// fit either way within the registers a 1024-thread block allows.  Lines of up
+// to 256 rows keep 16 or 32 times more lanes busy than one thread per line, and
+// the forward sweep stays in registers instead of going through memory.  Both
+// sweeps are affine recurrences, d_j = (r_j - a_j d_{j-1}) inv_j and
+// z_j = d_j - cp_j z_{j+1}; each segment first runs its sweep from a zero carry,
+// keeping the product of its multipliers, the carries into every segment follow
+// from one pass over the segments of a line (LDS), and each segment then reruns
+// the recurrence exactly from its carry.  Every thread of the block must call it
+// (block barriers); lanes without a line pass live = false.
+template <typename T>
+constexpr int line_lanes() {
+  return sizeof(T) == 4 ? 64 : 32;
+}
+template <typename T>
+constexpr int line_segs() {
+  return 1024 / line_lanes<T>();
+}
 template <typename T>
 constexpr int line_seg_rows() {
   return sizeof(T) == 4 ? 16 : 8;
 }
 template <typename T>
 HDI bool line_seg_fits(const Dims& d) {
-  return d.ny <= LINE_SEGS * line_seg_rows<T>();
+  return d.ny <= line_segs<T>() * line_seg_rows<T>();
+}
+// this thread's line within the block and its segment
+template <typename T>
+__device__ __forceinline__ int line_lane() {
+  return (int)(threadIdx.x & (line_lanes<T>() - 1));
+}
+template <typename T>
+__device__ __forceinline__ int line_seg() {
+  return (int)(threadIdx.y * (64 / line_lanes<T>()) + threadIdx.x / line_lanes<T>());
 }
 template <typename T>
 struct LineSegLds {
-  T part[2][LINE_SEGS][64];  // a segment's zero-carry end value and multiplier product
-  T carry[LINE_SEGS][64];    // carry into each segment
+  T part[2][line_segs<T>()][line_lanes<T>()];  // a segment's zero-carry end value and multiplier product
+  T carry[line_segs<T>()][line_lanes<T>()];    // carry into each segment
 };
 
 // The two segmented sweeps of one Chebyshev step: rr holds the segment's nr rows
@@ -316,7 +334,7 @@ __device__ __forceinline__ void line_seg_sweeps(const Dims& d, size_t c0, int nr
                                                 const T* __restrict__ x_old, T* __restrict__ x_new,
                                                 T* __restrict__ dv, T c1, T c2, LineSegLds<T>& sh) {
   constexpr int LM = line_seg_rows<T>();
-  const int lane = threadIdx.x, s = threadIdx.y;
+  const int lane = line_lane<T>(), s = line_seg<T>();
   const size_t n = d.n();
   const size_t sy = (size_t)d.nx;
   const T* __restrict__ la = lfac;
@@ -345,7 +363,7 @@ __device__ __forceinline__ void line_seg_sweeps(const Dims& d, size_t c0, int nr
   __syncthreads();
   if (s == 0) {
     T c = T(0);
-    for (int t = 0; t < LINE_SEGS; t++) {
+    for (int t = 0; t < line_segs<T>(); t++) {
       sh.carry[t][lane] = c;
       c = sh.part[0][t][lane] + sh.part[1][t][lane] * c;
     }
@@ -388,7 +406,7 @@ __device__ __forceinline__ void line_seg_sweeps(const Dims& d, size_t c0, int nr
   __syncthreads();
   if (s == 0) {
     T c = T(0);
-    for (int t = LINE_SEGS - 1; t >= 0; t--) {
+    for (int t = line_segs<T>() - 1; t >= 0; t--) {
       sh.carry[t][lane] = c;
       c = sh.part[0][t][lane] + sh.part[1][t][lane] * c;
     }
@@ -411,8 +429,8 @@ __device__ __forceinline__ void line_seg_sweeps(const Dims& d, size_t c0, int nr
 // rows of this thread's segment: first row j0, count nr (0 for a lane without a line)
 template <typename T>
 __device__ __forceinline__ void line_seg_rows_of(const Dims& d, bool live, int& j0, int& nr) {
-  const int L = (d.ny + LINE_SEGS - 1) / LINE_SEGS;
-  j0 = (int)threadIdx.y * L;
+  const int L = (d.ny + line_segs<T>() - 1) / line_segs<T>();
+  j0 = line_seg<T>() * L;
   nr = live ? max(0, min(L, d.ny - j0)) : 0;
 }
 
