@@ -37,7 +37,7 @@ RTOL = 2e-11  # flux-form relative residual; leaves margin under DIV_TOL
 
 # vfs_pp_time_kernel ids
 KERNELS = {0: "apply_fp64", 1: "smoother_fp32", 2: "projection", 3: "contra2cart", 4: "rhs", 5: "copy_1GiB",
-           6: "smoother_fp32_hbm", 7: "krylov_apply_dots"}
+           6: "smoother_fp32_hbm", 7: "krylov_apply_dots", 8: "smoother_fp32_stream_floor"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "smoother_fp32_hbm_hill192x96x128_hbm_traffic.txt")
 
 
@@ -642,6 +642,10 @@ def main():
             "ms_per_launch": sm["ms_per_launch"],
             "algorithmic_bytes_per_launch": sm["algorithmic_bytes"],
             "copy_gbs": kern["copy_1GiB"]["gbs"],
+            # the same operands streamed once per cell with no stencil, over the same
+            # rotating copies: the rate this array set allows at all
+            "stream_floor_ms": kern["smoother_fp32_stream_floor"]["ms_per_launch"],
+            "over_stream_floor": round(sm["ms_per_launch"] / kern["smoother_fp32_stream_floor"]["ms_per_launch"], 3),
             "kernels": kern,
         },
         "setup_s": round(mx["setup_s"], 3),

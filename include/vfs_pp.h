@@ -475,13 +475,16 @@ int vfs_pp_server_status(vfs_pp_server *sv, int *queued, int *active, size_t *re
  * 1 GiB device copy (the bandwidth reference), 6 the finest smoother sweep over
  * six rotating copies of its operands (together beyond the 256 MB last-level
  * cache, so each sweep streams from HBM), 7 the Krylov operator pass
- * (w = A z with the dots of w against 4 basis vectors and itself).  Writes the
- * mean duration (ms) and the algorithmic bytes of one launch: every distinct
- * array once per cell, with a 1-byte mask (apply 89, smoother 31 FP32 (FP16
- * coefficients) / 97 FP64, or for a y-line finest level its residual +
- * line-solve step 55 / 145, projection 129, Contra2Cart 121, RHS 33 bytes per
- * cell, Krylov pass z + 72 + 1 + 8 + 4 x 8 = 117 (FP32 z) / 121; copy 16 bytes
- * per element). */
+ * (w = A z with the dots of w against 4 basis vectors and itself), 8 the
+ * streaming floor of a point sweep: every array the sweep reads streamed once
+ * per cell over the same rotating copies, with no stencil (VFS_PP_EINVAL when
+ * the finest level is smoothed by y-lines).  Writes the mean duration (ms) and
+ * the algorithmic bytes of one launch: every distinct array once per cell, with
+ * a 1-byte mask (apply 89, smoother 31 FP32 (FP16 coefficients) / 97 FP64, or
+ * for a y-line finest level its fused residual + line step 51 / 137 (lines of
+ * up to 256 rows) or separate residual + line solve 55 / 145, projection 129,
+ * Contra2Cart 121, RHS 33 bytes per cell, Krylov pass z + 72 + 1 + 8 + 4 x 8 =
+ * 117 (FP32 z) / 121, streaming floor 35 / 105; copy 16 bytes per element). */
 int vfs_pp_time_kernel(vfs_pp_handle *h, int kernel, int reps, double *ms_per_launch,
                        double *bytes_per_launch);
 

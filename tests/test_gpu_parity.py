@@ -471,3 +471,26 @@ def test_set_opts_after_a_failed_set_up(E):
     h.set_opts(coarse_max=512)
     h.set_geometry(c.fields)
     h.close()
+
+
+def test_kernel_timer_ids(E, torch_cuda):
+    """vfs_pp_time_kernel: every id times its kernel and states its algorithmic
+    bytes (point levels: sweep 31 B/cell in FP32, streaming floor 35); a y-line
+    finest level has no point-sweep streaming floor (EINVAL) and prices its fused
+    residual + line step at 51 B/cell."""
+    c = get_case("hill_sphere")[0]
+    h = handle_for(E, c, precision=1, smoother=1)
+    n = c.nx * c.ny * c.nz
+    print(f"time_kernel ids on {c.nx}x{c.ny}x{c.nz}", flush=True)
+    for kid in (0, 1, 2, 3, 4, 6, 7, 8):
+        ms, by = h.time_kernel(kid, 3)
+        assert ms > 0 and by > 0, kid
+    assert h.time_kernel(1, 3)[1] == 31 * n and h.time_kernel(8, 3)[1] == 35 * n
+    h.close()
+    ch = cases.channel_case(16, 64, 12, beta=3.0, seed=2)
+    h = handle_for(E, ch, precision=1, smoother=2)
+    assert h.time_kernel(1, 3)[1] == 51 * ch.nx * ch.ny * ch.nz
+    with pytest.raises(E.VfsPPError) as ei:
+        h.time_kernel(8, 3)
+    assert ei.value.code == E.VFS_PP_EINVAL
+    h.close()

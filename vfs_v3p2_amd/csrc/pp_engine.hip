@@ -1119,6 +1119,20 @@ int vfs_pp_mg_nvert_restrict_device(const vfs_grid* g, const double* nvf, double
   ABI_END
 }
 
+// Algorithmic bytes per cell of one finest-level Chebyshev sweep: a point sweep
+// reads x, f, the coefficients and the mask and writes x (FP32 level with FP16
+// coefficients 31, FP64 97); a y-line step whose lines fit the segmented layout
+// reads x, f, coefficients, mask, the three Thomas factors and the direction and
+// writes the direction and x (51 / 137); on longer lines the residual goes through
+// memory (x, f, g, mask -> r, then r, factors, x -> x: 55 / 145).
+static double sweep_bytes_per_cell(const vfs_pp_handle* h) {
+  const bool f32 = h->opts.precision == 1;
+  if (!h->mg->line0()) return f32 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8);
+  const bool seg = f32 ? line_seg_fits<float>(h->d) : line_seg_fits<double>(h->d);
+  if (seg) return f32 ? (4 + 4 + 18 + 1 + 12 + 8 + 4) : (8 + 8 + 72 + 1 + 24 + 16 + 8);
+  return f32 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4) : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8);
+}
+
 int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, double* bytes) {
   ABI_BEGIN
   if (!h || reps < 1 || !ms || !bytes) return VFS_PP_EINVAL;
@@ -1145,18 +1159,17 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
     HIPCHECK(hipMemsetAsync(uc, 0, 3 * M * sizeof(double), h->s));
   }
   int nb = 0;
-  // kernel 6: the finest sweep over rotating copies of its operands (HBM-resident)
-  if (kernel == 6) {
-    const int copies = 6;
-    *ms = h->mg->time_sweep_rotating(h->s, std::max(reps, copies), copies);
-    const double N = (double)n;
-    if (!h->mg->line0())
-      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
-    else
-      *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4)
-                                           : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8));
+  // kernel 6: the finest sweep over rotating copies of its operands (HBM-resident);
+  // kernel 8: the streaming floor of a point sweep's operands over the same copies
+  if (kernel == 6 || kernel == 8) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (kernel == 8 && h->mg->line0()) return VFS_PP_EINVAL;
+    const int copies = 6;
+    *ms = h->mg->time_sweep_rotating(h->s, std::max(reps, copies), copies, kernel == 8);
+    const double N = (double)n;
+    // the floor kernel reads the previous iterate too
+    *bytes = N * (kernel == 6 ? sweep_bytes_per_cell(h) : (h->opts.precision == 1 ? 35 : 105));
     return VFS_PP_OK;
   }
   // kernel 7: the Krylov operator pass w = A z with the dots of w against kDots
@@ -1200,13 +1213,7 @@ int vfs_pp_time_kernel(vfs_pp_handle* h, int kernel, int reps, double* ms, doubl
   const double N = (double)n;
   switch (kernel) {
     case 0: *bytes = N * (8 + 72 + 1 + 8); break;  // phi, g11..g33, mask -> y           = 89
-    case 1:
-      if (!h->mg->line0())  // x, b, g, mask -> x: FP32 level (FP16 g) 31 / FP64 97
-        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) : (8 + 8 + 72 + 1 + 8));
-      else  // y-line step: x, b, g, mask -> r (FP16 g in FP32), then r, factors, x -> x: 55 / 145
-        *bytes = N * (h->opts.precision == 1 ? (4 + 4 + 18 + 1 + 4) + (4 + 12 + 4 + 4)
-                                             : (8 + 8 + 72 + 1 + 8) + (8 + 24 + 8 + 8));
-      break;
+    case 1: *bytes = N * sweep_bytes_per_cell(h); break;
     case 2: *bytes = N * (8 + 72 + 1 + 24 + 24); break;  // phi, g, mask, Ucont r/w   = 129
     case 3: *bytes = N * (24 + 72 + 1 + 24); break;      // Ucont, csi/eta/zet, mask -> Ucat = 121
     case 4: *bytes = N * (24 + 1 + 8); break;            // Ucont, mask -> b        = 33

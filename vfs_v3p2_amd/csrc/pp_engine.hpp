@@ -117,7 +117,7 @@ struct MGBase {
   // ms per finest-level sweep when `copies` copies of the level's operands are
   // swept in turn (together larger than the 256 MB last-level cache, so every
   // sweep streams its operands from HBM)
-  virtual double time_sweep_rotating(hipStream_t s, int reps, int copies) = 0;
+  virtual double time_sweep_rotating(hipStream_t s, int reps, int copies, bool floor = false) = 0;
   virtual int nlev() const = 0;
   // level l: dimensions, smoother, code words; its face coefficients as FP64 into
   // g64 (device, 9 n)
@@ -261,7 +261,7 @@ struct MG : MGBase {
                      L.c1.size() > 1 ? L.c1[1] : T(0), L.c2[0], T(0));
   }
 
-  double time_sweep_rotating(hipStream_t s, int reps, int copies) override {
+  double time_sweep_rotating(hipStream_t s, int reps, int copies, bool floor = false) override {
     const Lev& L0 = lev[0];
     const size_t n = L0.d.n();
     Arena tmp;
@@ -289,8 +289,11 @@ struct MG : MGBase {
     for (auto& sc : scrs) sc = tmp.alloc<T>(2 * n);
     auto sweep = [&](int q) {
       Lev& C = cp[q];
-      launch_smooth<T>(s, C.view(), C.lfac, SM_NEXT, C.f, C.dinv, C.xa, C.xb, C.xb, C.dv, scrs[q],
-                       C.c1.size() > 1 ? C.c1[1] : T(0), C.c2[0], T(0));
+      if (floor)
+        launch_stream_floor<T>(s, C.view(), C.f, C.xa, C.xb, scrs[q]);
+      else
+        launch_smooth<T>(s, C.view(), C.lfac, SM_NEXT, C.f, C.dinv, C.xa, C.xb, C.xb, C.dv, scrs[q],
+                         C.c1.size() > 1 ? C.c1[1] : T(0), C.c2[0], T(0));
     };
     for (int q = 0; q < copies; q++) sweep(q);
     hipEvent_t e0, e1;

@@ -169,6 +169,26 @@ struct LevelView {
   T gs;
 };
 
+// Layout of the segmented y-line step (see line_solve_seg): lines per 1024-thread
+// block, segments per line and rows per segment by precision; lines of up to
+// line_segs * line_seg_rows rows take it, longer ones one thread per line.
+template <typename T>
+constexpr int line_lanes() {
+  return sizeof(T) == 4 ? 64 : 32;
+}
+template <typename T>
+constexpr int line_segs() {
+  return 1024 / line_lanes<T>();
+}
+template <typename T>
+constexpr int line_seg_rows() {
+  return sizeof(T) == 4 ? 16 : 8;
+}
+template <typename T>
+HD inline bool line_seg_fits(const Dims& d) {
+  return d.ny <= line_segs<T>() * line_seg_rows<T>();
+}
+
 // Chebyshev smoothing launch modes (see smooth_point_cell); SM_POWER is one step
 // with c1 = 0, c2 = 1 from f = 0 that stores D^-1 (f - A x) in dv (power iteration)
 enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2, SM_POWER = 3 };
@@ -181,6 +201,11 @@ void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const
                    const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0);
 template <typename T>
 void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac);
+// streaming floor of a point sweep: reads every array the sweep reads once per
+// cell (class byte, nine coefficient arrays, f, x, the previous iterate) and
+// writes one, with no stencil
+template <typename T>
+void launch_stream_floor(hipStream_t s, LevelView<T> L, const T* f, const T* x, const T* x_prev, T* out);
 template <typename T>
 void launch_dinv(hipStream_t s, LevelView<T> L, T* dinv);
 template <typename T>

@@ -82,6 +82,19 @@ __global__ __launch_bounds__(1024) void k_line_fused(LevelView<T> L, const T* __
   line_step_cell<T>(L, i < L.d.nx, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
+template <typename T, typename TG>
+__global__ __launch_bounds__(256) void k_stream_floor(size_t n, const TG* __restrict__ g,
+                                                      const uint8_t* __restrict__ cls, const T* __restrict__ f,
+                                                      const T* __restrict__ x, const T* __restrict__ xp,
+                                                      T* __restrict__ out) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T acc = (T)cls[c] + f[c] + x[c] + xp[c];
+#pragma unroll
+  for (int q = 0; q < 9; q++) acc += (T)g[(size_t)q * n + c];
+  out[c] = acc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_resid_restrict(LevelView<T> F, const T* __restrict__ f,
                                                         const T* __restrict__ x, Dims dc,
@@ -267,6 +280,20 @@ void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const
 }
 
 template <typename T>
+void launch_stream_floor(hipStream_t s, LevelView<T> L, const T* f, const T* x, const T* x_prev, T* out) {
+  const size_t n = L.d.n();
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.gh) {
+      k_stream_floor<T, _Float16><<<nb1(n), 256, 0, s>>>(n, L.gh, L.cls, f, x, x_prev, out);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
+  }
+  k_stream_floor<T, T><<<nb1(n), 256, 0, s>>>(n, L.g, L.cls, f, x, x_prev, out);
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
 void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac) {
   k_line_factors<T><<<line_grid(L.d), kBlk, 0, s>>>(L, lfac);
   HIPCHECK(hipGetLastError());
@@ -323,6 +350,7 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
   template void launch_smooth<T>(hipStream_t, LevelView<T>, const T*, int, const T*, const T*, const T*, T*, T*, T*, \
                                  T*, T, T, T);                                                                    \
   template void launch_line_factors<T>(hipStream_t, LevelView<T>, T*);                                            \
+  template void launch_stream_floor<T>(hipStream_t, LevelView<T>, const T*, const T*, const T*, T*);             \
   template void launch_dinv<T>(hipStream_t, LevelView<T>, T*);                                                     \
   template void launch_resid_restrict<T>(hipStream_t, LevelView<T>, const T*, const T*, Dims, const uint8_t*, T*); \
   template void launch_prolong_add<T>(hipStream_t, LevelView<T>, Dims, const uint8_t*, const T*, T*);              \

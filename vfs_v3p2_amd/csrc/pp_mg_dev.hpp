@@ -295,22 +295,7 @@ __device__ __forceinline__ void line_solve_col(const Dims& d, const T* __restric
 // from one pass over the segments of a line (LDS), and each segment then reruns
 // the recurrence exactly from its carry.  Every thread of the block must call it
 // (block barriers); lanes without a line pass live = false.
-template <typename T>
-constexpr int line_lanes() {
-  return sizeof(T) == 4 ? 64 : 32;
-}
-template <typename T>
-constexpr int line_segs() {
-  return 1024 / line_lanes<T>();
-}
-template <typename T>
-constexpr int line_seg_rows() {
-  return sizeof(T) == 4 ? 16 : 8;
-}
-template <typename T>
-HDI bool line_seg_fits(const Dims& d) {
-  return d.ny <= line_segs<T>() * line_seg_rows<T>();
-}
+// (line_lanes, line_segs, line_seg_rows and line_seg_fits: pp_kernels.hpp)
 // this thread's line within the block and its segment
 template <typename T>
 __device__ __forceinline__ int line_lane() {
