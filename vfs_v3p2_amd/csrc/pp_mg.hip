@@ -82,8 +82,9 @@ __global__ __launch_bounds__(1024) void k_line_fused(LevelView<T> L, const T* __
   line_step_cell<T>(L, i < L.d.nx, i, k, f, lfac, x_old, x_new, dv, c1, c2, sh);
 }
 
-template <typename T, typename TG>
-__global__ __launch_bounds__(256) void k_stream_floor(size_t n, const TG* __restrict__ g,
+// HALF: the coefficients are FP16 (gh), else T (g)
+template <typename T, bool HALF>
+__global__ __launch_bounds__(256) void k_stream_floor(size_t n, const void* __restrict__ gv,
                                                       const uint8_t* __restrict__ cls, const T* __restrict__ f,
                                                       const T* __restrict__ x, const T* __restrict__ xp,
                                                       T* __restrict__ out) {
@@ -91,7 +92,10 @@ __global__ __launch_bounds__(256) void k_stream_floor(size_t n, const TG* __rest
   if (c >= n) return;
   T acc = (T)cls[c] + f[c] + x[c] + xp[c];
 #pragma unroll
-  for (int q = 0; q < 9; q++) acc += (T)g[(size_t)q * n + c];
+  for (int q = 0; q < 9; q++) {
+    if constexpr (HALF) acc += (T) static_cast<const _Float16*>(gv)[(size_t)q * n + c];
+    else acc += static_cast<const T*>(gv)[(size_t)q * n + c];
+  }
   out[c] = acc;
 }
 
@@ -284,12 +288,12 @@ void launch_stream_floor(hipStream_t s, LevelView<T> L, const T* f, const T* x, 
   const size_t n = L.d.n();
   if constexpr (std::is_same<T, float>::value) {
     if (L.gh) {
-      k_stream_floor<T, _Float16><<<nb1(n), 256, 0, s>>>(n, L.gh, L.cls, f, x, x_prev, out);
+      k_stream_floor<T, true><<<nb1(n), 256, 0, s>>>(n, L.gh, L.cls, f, x, x_prev, out);
       HIPCHECK(hipGetLastError());
       return;
     }
   }
-  k_stream_floor<T, T><<<nb1(n), 256, 0, s>>>(n, L.g, L.cls, f, x, x_prev, out);
+  k_stream_floor<T, false><<<nb1(n), 256, 0, s>>>(n, L.g, L.cls, f, x, x_prev, out);
   HIPCHECK(hipGetLastError());
 }
 

@@ -49,12 +49,17 @@ def load_trace(path):
 
 
 def solves(rows):
-    starts = [r for r in rows if r["name"] == "k_rhs"]
-    ends = [r for r in rows if r["name"] == "k_divergence"]
-    out = []
-    for a, b in zip(starts, ends):
-        seg = [r for r in rows if r["s"] >= a["s"] and r["e"] <= b["e"]]
-        out.append((a, b, seg))
+    """(first, last, launches) of every solve: from the last RHS launch before a
+    Divergence launch to that Divergence (RHS launches of a timing loop in between
+    solves start no solve)."""
+    out, last_rhs = [], None
+    for r in rows:
+        if r["name"] == "k_rhs":
+            last_rhs = r
+        elif r["name"] == "k_divergence" and last_rhs is not None:
+            a, b = last_rhs, r
+            out.append((a, b, [x for x in rows if x["s"] >= a["s"] and x["e"] <= b["e"]]))
+            last_rhs = None
     return out
 
 
