@@ -189,6 +189,59 @@ int vfs_pp_runge_kutta_device(vfs_pp_handle *h, const double *icsi, const double
                               const double *eta, const double *zet, const double *dp, double *ucont,
                               double *ucat, double ren, double dt, double st);
 
+/* The implicit momentum step (Implicit_MatrixFree, implicitsolver.c:281-470) for
+ * immersed == 0.  Its residual (FormFunction_SNES, momentum.c:2285-2440) is
+ *   Uz = U with the wall components zeroed (bctype 1 and -1 on every face, 2 and
+ *        12 on the j faces, 10 on the i and j faces away from the boundary rows),
+ *   F  = -(Uz - Uo)/dt + 0.5 Formfunction_2(Uz, Contra2Cart(Uz)) + 0.5 RHS_o - dP
+ *        for coeff in (0.9, 1.1), otherwise
+ *        -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt + Formfunction_2(...) - dP,
+ * summed in the reference's order (bit-exact with a CPU restatement).
+ * ucont_rm1 is needed only for coeff outside (0.9, 1.1), rhs_o only for coeff in
+ * it (NULL: zero).  VFS_PP_EINVAL for immersed != 0 (the IB velocity boundary,
+ * IB_BC, is the caller's) or a bctype outside {0, 1, -1, 2, 10, -10, 12}.
+ * All arrays are device DMDA vector fields on the handle's grid, stream-ordered
+ * on the handle's stream; the call returns when done. */
+int vfs_pp_momentum_residual_device(vfs_pp_handle *h, const double *icsi, const double *ieta,
+                                    const double *izet, const double *jcsi, const double *jeta,
+                                    const double *jzet, const double *kcsi, const double *keta,
+                                    const double *kzet, const double *iaj, const double *jaj,
+                                    const double *kaj, const double *csi, const double *eta,
+                                    const double *zet, const double *dp, const double *ucont_o,
+                                    const double *ucont_rm1, const double *rhs_o, const double *ucont,
+                                    double ren, double dt, double coeff, double *ucat, double *f);
+
+typedef struct {
+  int newton_iterations;   /* Newton steps taken */
+  int krylov_iterations;   /* GMRES iterations over all steps */
+  int residual_evals;      /* evaluations of F (one per Jacobian-vector product) */
+  int converged;           /* 1 if ||F|| <= rtol ||F0|| or the step fell below 1e-8 ||U|| */
+  double fnorm0;           /* ||F|| at the initial guess */
+  double fnorm;            /* ||F|| at the solution */
+  double ms_total;         /* wall time of the call (ms) */
+} vfs_pp_newton_stats;
+
+/* Solve F(U) = 0 for U, starting from ucont (updated in place): Newton steps
+ * with Jacobian-free products (J v = (F(U + h v) - F(U)) / h, h = sqrt(eps)
+ * sqrt(1 + |U|) / |v|), unpreconditioned restarted GMRES(30) with an
+ * Eisenstat-Walker forcing term (version 3, as SNESKSPSetParametersEW(snes, 3)),
+ * at most max_newton steps, relative tolerance rtol on ||F|| (the reference's
+ * imp_free_tol, 1e-4 by default, main.c:86).  The reference globalises with
+ * PETSc's trust region; this backtracks along the Newton step, so iterates
+ * differ while the solution meets the same test.  Afterwards the solution's wall
+ * faces are zeroed as Implicit_MatrixFree does (implicitsolver.c:408-415) and
+ * ucat receives its Contra2Cart.  VFS_PP_ENOCONV if the tolerance was not met
+ * (ucont then holds the last iterate); stats may be NULL. */
+int vfs_pp_implicit_momentum_device(vfs_pp_handle *h, const double *icsi, const double *ieta,
+                                    const double *izet, const double *jcsi, const double *jeta,
+                                    const double *jzet, const double *kcsi, const double *keta,
+                                    const double *kzet, const double *iaj, const double *jaj,
+                                    const double *kaj, const double *csi, const double *eta,
+                                    const double *zet, const double *dp, const double *ucont_o,
+                                    const double *ucont_rm1, const double *rhs_o, double *ucont,
+                                    double *ucat, double ren, double dt, double coeff, double rtol,
+                                    int max_newton, vfs_pp_newton_stats *stats);
+
 /* PETSc binary vector files, the form of the vfield / pfield / nvfield / ufield
  * checkpoints (VecView / VecLoad on a binary viewer, main.c:291-335, 780-797,
  * 865-935): big-endian int32 class id 1211214, big-endian int32 length, then

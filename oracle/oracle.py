@@ -66,6 +66,8 @@ def lib():
         _lib.or_pressure_gradient.argtypes = [g] + [d] * 15
         _lib.or_formfunction2.argtypes = [g, ip, C.c_int] + [d] * 18 + [C.c_double, C.c_double, d]
         _lib.or_runge_kutta.argtypes = [g, ip, C.c_int] + [d] * 19 + [C.c_double, C.c_double, C.c_double]
+        _lib.or_momentum_bc_zero.argtypes = [g, ip, d]
+        _lib.or_momentum_residual.argtypes = [g, ip] + [d] * 16 + [d] * 5 + [C.c_double] * 3 + [d] * 3
         _lib.or_ray_s.restype = C.c_double
         _lib.or_ray_s.argtypes = [C.c_int]
         _lib.or_level_coupling.argtypes = [g, d, d, d, d]
@@ -434,3 +436,29 @@ def runge_kutta(g, bctype, immersed, f, dp, ucont, ucat, ren, dt, st):
     lib().or_runge_kutta(C.byref(g), _p(bc, C.c_int), int(immersed), *[_p(x) for x in a], _p(d), _p(u), _p(uc),
                          float(ren), float(dt), float(st))
     return u, uc
+
+
+def momentum_bc_zero(g, bctype, u):
+    """or_momentum_bc_zero: U with the wall components FormFunction_SNES zeroes
+    (momentum.c:2323-2356) set to zero (a copy)."""
+    uz = np.ascontiguousarray(u, dtype=np.float64).copy()
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    lib().or_momentum_bc_zero(C.byref(g), _p(bc, C.c_int), _p(uz))
+    return uz
+
+
+def momentum_residual(g, bctype, f, dp, uo, u, ren, dt, coeff=1.0, urm1=None, rhs_o=None):
+    """or_momentum_residual: (F, Uz, Ucat) of the implicit momentum equation
+    (FormFunction_SNES, momentum.c:2285-2440) at U, immersed == 0."""
+    a = [np.ascontiguousarray(f[k], dtype=np.float64) for k in MOMENTUM_METRICS]
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    z = np.zeros(u.shape)
+    uo = np.ascontiguousarray(uo, dtype=np.float64)
+    urm1 = z if urm1 is None else np.ascontiguousarray(urm1, dtype=np.float64)
+    rhs_o = z if rhs_o is None else np.ascontiguousarray(rhs_o, dtype=np.float64)
+    dp = np.ascontiguousarray(dp, dtype=np.float64)
+    uz, ucat, out = np.zeros(u.shape), np.zeros(u.shape), np.zeros(u.shape)
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    lib().or_momentum_residual(C.byref(g), _p(bc, C.c_int), *[_p(x) for x in a], _p(dp), _p(uo), _p(urm1),
+                               _p(rhs_o), _p(u), float(ren), float(dt), float(coeff), _p(uz), _p(ucat), _p(out))
+    return out, uz, ucat

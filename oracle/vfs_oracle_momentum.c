@@ -181,12 +181,14 @@ static double mm_cell_ends(int per, int pos, int m, const double *nv, size_t p, 
   return 1. / denom;
 }
 
-void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
-                      const double *ieta, const double *izet, const double *jcsi, const double *jeta,
-                      const double *jzet, const double *kcsi, const double *keta, const double *kzet,
-                      const double *iaj, const double *jaj, const double *kaj, const double *csi,
-                      const double *eta, const double *zet, const double *nvert, const double *ucont,
-                      const double *ucat, double ren, double scale, double *rhs) {
+/* acc = 0: Rhs starts at zero (the caller's VecSet, rhs.c:1552); acc = 1: the
+   terms add to what Rhs holds (FormFunction_SNES, momentum.c:2410-2413) */
+static void mm_formfunction2(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
+                             const double *ieta, const double *izet, const double *jcsi, const double *jeta,
+                             const double *jzet, const double *kcsi, const double *keta, const double *kzet,
+                             const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                             const double *eta, const double *zet, const double *nvert, const double *ucont,
+                             const double *ucat, double ren, double scale, double *rhs, int acc) {
   const int mx = g->mx, my = g->my, mz = g->mz;
   const size_t n = (size_t)mx * my * mz;
   const size_t st[3] = {1, (size_t)mx, (size_t)mx * my};
@@ -235,8 +237,8 @@ void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const
                            rr;
       }
   mm_periodic_vec(g, fp);
-  /* Rhs (momentum.c:1750-1866): set to 0 by the caller's VecSet, then += */
-  memset(rhs, 0, 3 * n * sizeof(double));
+  /* Rhs (momentum.c:1750-1866): += on what the caller left there */
+  if (!acc) memset(rhs, 0, 3 * n * sizeof(double));
   const double *cm[3] = {csi, eta, zet}, *caj[3] = {iaj, jaj, kaj};
   for (int k = 1; k < mz - 1; k++)
     for (int j = 1; j < my - 1; j++)
@@ -272,6 +274,16 @@ void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const
   free(fp);
 }
 
+void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
+                      const double *ieta, const double *izet, const double *jcsi, const double *jeta,
+                      const double *jzet, const double *kcsi, const double *keta, const double *kzet,
+                      const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                      const double *eta, const double *zet, const double *nvert, const double *ucont,
+                      const double *ucat, double ren, double scale, double *rhs) {
+  mm_formfunction2(g, bctype, immersed, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, csi,
+                   eta, zet, nvert, ucont, ucat, ren, scale, rhs, 0);
+}
+
 void or_runge_kutta(const or_grid *g, const int bctype[6], int immersed, const double *icsi, const double *ieta,
                     const double *izet, const double *jcsi, const double *jeta, const double *jzet,
                     const double *kcsi, const double *keta, const double *kzet, const double *iaj,
@@ -294,4 +306,68 @@ void or_runge_kutta(const or_grid *g, const int bctype[6], int immersed, const d
   }
   free(pu);
   free(rhs);
+}
+
+/* ------------------------------------------------------- implicit momentum
+ * The nonlinear residual the reference's matrix-free SNES drives to zero
+ * (FormFunction_SNES, momentum.c:2285-2440, called by Implicit_MatrixFree,
+ * implicitsolver.c:281), for immersed == 0 (no IB_BC), no wall model, forcing or
+ * level set:
+ *   Uz  = U with the wall components zeroed (momentum.c:2323-2356: bctype 1 and
+ *         -1 on every face, cavity 2 and couette 12 on j, slip 10 on i and j),
+ *   Ucat = Contra2Cart(Uz),
+ *   F   = -(Uz - Uo)/dt (coeff 1) or -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt (otherwise),
+ *         summed as the VecAXPY sequence does,
+ *   F  += Formfunction_2(Uz, Ucat) scaled by 0.5 (coeff 1; plus 0.5 RHS_o) or 1,
+ *   F  += -1 dP.
+ * uz and ucat receive Uz and its Ucat. */
+void or_momentum_bc_zero(const or_grid *g, const int bc[6], double *u) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        const int ik = j != 0 && j != my - 1 && k != 0 && k != mz - 1;  /* slip-rule interior in j, k */
+        const int jk = i != 0 && i != mx - 1 && k != 0 && k != mz - 1;  /* slip-rule interior in i, k */
+        if (i == 0 && (bc[0] == 1 || bc[0] == -1)) u[3 * IXM(g, i, j, k)] = 0;
+        if (i == mx - 1 && (bc[1] == 1 || bc[1] == -1)) u[3 * IXM(g, i - 1, j, k)] = 0;
+        if (j == 0 && (bc[2] == 1 || bc[2] == -1 || bc[2] == 12)) u[3 * IXM(g, i, j, k) + 1] = 0;
+        if (j == my - 1 && (bc[3] == 1 || bc[3] == -1 || bc[3] == 2 || bc[3] == 12)) u[3 * IXM(g, i, j - 1, k) + 1] = 0;
+        if (k == 0 && (bc[4] == 1 || bc[4] == -1)) u[3 * IXM(g, i, j, k) + 2] = 0;
+        if (k == mz - 1 && (bc[5] == 1 || bc[5] == -1)) u[3 * IXM(g, i, j, k - 1) + 2] = 0;
+        if (bc[0] == 10 && i == 0 && ik) u[3 * IXM(g, i, j, k)] = 0;
+        if (bc[1] == 10 && i == mx - 1 && ik) u[3 * IXM(g, i - 1, j, k)] = 0;
+        if (bc[2] == 10 && j == 0 && jk) u[3 * IXM(g, i, j, k) + 1] = 0;
+        if ((bc[3] == 10 || bc[3] == -10) && j == my - 1 && jk) u[3 * IXM(g, i, j - 1, k) + 1] = 0;
+      }
+}
+
+void or_momentum_residual(const or_grid *g, const int bctype[6], const double *icsi, const double *ieta,
+                          const double *izet, const double *jcsi, const double *jeta, const double *jzet,
+                          const double *kcsi, const double *keta, const double *kzet, const double *iaj,
+                          const double *jaj, const double *kaj, const double *csi, const double *eta,
+                          const double *zet, const double *nvert, const double *dp, const double *uo,
+                          const double *urm1, const double *rhs_o, const double *u, double ren, double dt,
+                          double coeff, double *uz, double *ucat, double *f) {
+  const size_t n3 = 3 * (size_t)g->mx * g->my * g->mz;
+  memcpy(uz, u, n3 * sizeof(double));
+  or_momentum_bc_zero(g, bctype, uz);
+  or_contra2cart(g, bctype, csi, eta, zet, nvert, uz, ucat);
+  const int bdf1 = coeff > 0.9 && coeff < 1.1;
+  for (size_t q = 0; q < n3; q++) {
+    double r = 0.;
+    if (bdf1) {
+      r = r + -1. / dt * uz[q];
+      r = r + 1. / dt * uo[q];
+    } else {
+      r = r + -1.5 / dt * uz[q];
+      r = r + 2. / dt * uo[q];
+      r = r + -0.5 / dt * urm1[q];
+    }
+    f[q] = r;
+  }
+  mm_formfunction2(g, bctype, 0, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, csi, eta,
+                   zet, nvert, uz, ucat, ren, bdf1 ? 0.5 : 1.0, f, 1);
+  if (bdf1)
+    for (size_t q = 0; q < n3; q++) f[q] = f[q] + 0.5 * rhs_o[q];
+  for (size_t q = 0; q < n3; q++) f[q] = f[q] + -1. * dp[q];
 }

@@ -152,3 +152,154 @@ def test_one_time_step_resident(E, torch_cuda):
     assert rel(pd.cpu().numpy(), p1) <= 1e-8
     d1 = dp1d.cpu().numpy()
     assert float(np.linalg.norm(d1 - dp1) / np.linalg.norm(dp1)) <= 1e-8
+
+
+# ------------------------------------------------------------ implicit form
+
+def _case_imm0(name):
+    c = CASES[name]()
+    return dataclasses.replace(c, immersed=0)
+
+
+def _handle0(E, c):
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, 0))
+    h.set_geometry(c.fields)
+    return h
+
+
+@pytest.mark.parametrize("coeff", [1.0, 1.5])
+@pytest.mark.parametrize("name", ["channel", "affine_ib", "hill_sphere"])
+def test_momentum_residual_bit_exact(E, torch_cuda, name, coeff):
+    """FormFunction_SNES's residual on the device equals the oracle's bit for bit
+    (time terms, wall zeroing, Contra2Cart, accumulated Formfunction_2, RHS_o, dP)."""
+    torch = torch_cuda
+    c = _case_imm0(name)
+    print(f"momentum residual {name} coeff {coeff}", flush=True)
+    g = O.grid(c.shape, c.per)
+    rng = np.random.default_rng(7)
+    u = np.ascontiguousarray(c.fields["ucont"])
+    uo = u * (1.0 + 0.1 * rng.standard_normal(u.shape))
+    urm1 = u * (1.0 + 0.1 * rng.standard_normal(u.shape))
+    rhs_o = 0.01 * rng.standard_normal(u.shape)
+    dp, _ = O.pressure_gradient(g, c.fields, _pressure(c))
+    ren, dt = 300.0, 0.01
+    F_ref, uz_ref, ucat_ref = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u, ren, dt, coeff=coeff, urm1=urm1,
+                                                  rhs_o=rhs_o)
+    h = _handle0(E, c)
+    met = {k: dev(torch, c.fields[k]) for k in METRICS}
+    F, ucat = E.momentum_residual(h, met, dev(torch, dp), dev(torch, uo), dev(torch, u), ren, dt, coeff,
+                                  ucont_rm1=dev(torch, urm1), rhs_o=dev(torch, rhs_o))
+    h.close()
+    np.testing.assert_array_equal(F.cpu().numpy(), F_ref)
+    np.testing.assert_array_equal(ucat.cpu().numpy(), ucat_ref)
+    assert np.abs(F_ref).max() > 0
+
+
+def _periodic_affine(nx=10, ny=9, nz=11):
+    M = np.array([[1.0, 0.35, -0.2], [0.15, 0.8, 0.1], [-0.25, 0.3, 1.2]])
+    P = np.stack(np.meshgrid(np.arange(nx + 1) / nx, np.arange(ny + 1) / ny, np.arange(nz + 1) / nz,
+                             indexing="ij"), -1).transpose(2, 1, 0, 3)
+    f = cases.form_metrics(cases.dmda_coords(P @ M.T), (1, 1, 1))
+    f["nvert"] = np.zeros(f["aj"].shape)
+    f["ucont"] = np.zeros(f["aj"].shape + (3,))
+    return cases.Case(name="affine_periodic", nx=nx, ny=ny, nz=nz, per=(1, 1, 1), bctype=(0,) * 6, immersed=0,
+                      fields=f)
+
+
+def _uniform_flux(f, U):
+    u = np.zeros(f["aj"].shape + (3,))
+    u[..., 0] = f["icsi"] @ U
+    u[..., 1] = f["jeta"] @ U
+    u[..., 2] = f["kzet"] @ U
+    return u
+
+
+def test_implicit_step_of_a_uniform_flow(E, torch_cuda):
+    """A uniform flow on a periodic sheared grid has no Formfunction_2 part, so
+    F(U) = -(U - Uo)/dt - dP is linear and its root is the uniform flux of
+    Uo - dt dU: the Newton-Krylov solve lands on it."""
+    torch = torch_cuda
+    c = _periodic_affine()
+    f = c.fields
+    U0, Uo, dU = np.array([0.8, -0.4, 0.3]), np.array([0.7, -0.3, 0.2]), np.array([0.2, 0.1, -0.3])
+    u0, uo, dp = _uniform_flux(f, U0), _uniform_flux(f, Uo), _uniform_flux(f, dU)
+    # Pressure_Gradient leaves the boundary planes at zero, where Formfunction_2
+    # zeroes the residual's other terms too
+    for a in range(3):
+        dp[(slice(None),) * a + (0,)] = 0.0
+        dp[(slice(None),) * a + (-1,)] = 0.0
+    dt = 0.05
+    ustar = _uniform_flux(f, Uo - dt * dU)
+    h = _handle0(E, c)
+    met = {k: dev(torch, f[k]) for k in METRICS}
+    ud, ucd = dev(torch, u0), torch.zeros(u0.shape, dtype=torch.float64, device="cuda")
+    st = E.Implicit_MatrixFree(h, met, dev(torch, dp), dev(torch, uo), ud, ucd, 100.0, dt, 1.0,
+                               rhs_o=dev(torch, np.zeros(u0.shape)), rtol=1e-10)
+    h.close()
+    print(f"uniform flow: {st.newton_iterations} Newton, {st.krylov_iterations} GMRES", flush=True)
+    assert st.converged and st.newton_iterations >= 1 and st.fnorm <= 1e-10 * st.fnorm0
+    got = ud.cpu().numpy()
+    inner = (slice(1, -1),) * 3  # the residual does not see the ghost planes of U
+    assert np.abs(got - ustar)[inner].max() <= 1e-8 * np.abs(u0 - ustar)[inner].max()
+
+
+@pytest.mark.parametrize("coeff", [1.0, 1.5])
+def test_implicit_step_meets_the_reference_test(E, torch_cuda, coeff):
+    """A walled channel from a perturbed start: the device's Ucont satisfies the
+    reference's SNES test ||F|| <= 1e-4 ||F(U0)|| with F evaluated by the oracle,
+    its ||F(U0)|| is the oracle's, and its Ucat is the oracle's Contra2Cart of it."""
+    torch = torch_cuda
+    c = _case_imm0("channel")
+    g = O.grid(c.shape, c.per)
+    rng = np.random.default_rng(11)
+    uo = np.ascontiguousarray(c.fields["ucont"])
+    ucat_o = ucat_of(c, uo)
+    ren, dt = 400.0, 0.02
+    rhs_o = O.formfunction2(g, c.bctype, 0, c.fields, uo, ucat_o, ren)
+    u0 = uo * (1.0 + 0.05 * rng.standard_normal(uo.shape))
+    urm1 = uo * (1.0 + 0.05 * rng.standard_normal(uo.shape))
+    dp, _ = O.pressure_gradient(g, c.fields, _pressure(c))
+    F0, _, _ = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u0, ren, dt, coeff=coeff, urm1=urm1, rhs_o=rhs_o)
+    h = _handle0(E, c)
+    met = {k: dev(torch, c.fields[k]) for k in METRICS}
+    ud, ucd = dev(torch, u0), torch.zeros(u0.shape, dtype=torch.float64, device="cuda")
+    st = E.Implicit_MatrixFree(h, met, dev(torch, dp), dev(torch, uo), ud, ucd, ren, dt, coeff,
+                               ucont_rm1=dev(torch, urm1), rhs_o=dev(torch, rhs_o), rtol=1e-4)
+    h.close()
+    print(f"channel coeff {coeff}: {st.newton_iterations} Newton, {st.krylov_iterations} GMRES, "
+          f"{st.residual_evals} F, |F| {st.fnorm0:.3e} -> {st.fnorm:.3e}", flush=True)
+    u = ud.cpu().numpy()
+    F1, _, ucat_ref = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u, ren, dt, coeff=coeff, urm1=urm1,
+                                          rhs_o=rhs_o)
+    assert st.converged
+    assert abs(st.fnorm0 - np.linalg.norm(F0)) <= 1e-12 * np.linalg.norm(F0)
+    assert np.linalg.norm(F1) <= 1e-4 * np.linalg.norm(F0) * (1 + 1e-6)
+    assert np.linalg.norm(u - u0) > 1e-3 * np.linalg.norm(u0)
+    np.testing.assert_allclose(ucd.cpu().numpy(), ucat_ref, rtol=0, atol=1e-14 * np.abs(ucat_ref).max())
+
+
+def test_implicit_form_refusals(E, torch_cuda):
+    """immersed != 0 (the IB velocity boundary is the caller's), a bctype outside
+    the wall rules, or coeff 1.5 without the older field: VFS_PP_EINVAL."""
+    torch = torch_cuda
+    c = CASES["affine_ib"]()
+    met = {k: dev(torch, c.fields[k]) for k in METRICS}
+    u = dev(torch, c.fields["ucont"])
+    z = torch.zeros_like(u)
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, 1))
+    h.set_geometry(c.fields)
+    with pytest.raises(E.VfsPPError) as ei:
+        E.momentum_residual(h, met, z, u, u, 100.0, 0.01)
+    assert ei.value.code == E.VFS_PP_EINVAL
+    h.close()
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, (4, 1, 1, 1, 1, 1), 0))
+    h.set_geometry(c.fields)
+    with pytest.raises(E.VfsPPError) as ei:
+        E.Implicit_MatrixFree(h, met, z, u, u.clone(), z.clone(), 100.0, 0.01)
+    assert ei.value.code == E.VFS_PP_EINVAL
+    h.close()
+    h = _handle0(E, dataclasses.replace(c, immersed=0))
+    with pytest.raises(E.VfsPPError) as ei:
+        E.momentum_residual(h, met, z, u, u, 100.0, 0.01, coeff=1.5)
+    assert ei.value.code == E.VFS_PP_EINVAL
+    h.close()

@@ -148,3 +148,95 @@ def test_runge_kutta_of_a_uniform_flow_under_a_uniform_gradient_flux():
     np.testing.assert_allclose(u1, u0 - dt * st * dp, rtol=0, atol=1e-13 * np.abs(u0).max())
     np.testing.assert_allclose(uc1[1:-1, 1:-1, 1:-1], np.broadcast_to(U0 - dt * st * dU, uc1[1:-1, 1:-1, 1:-1].shape),
                                rtol=0, atol=1e-12)
+
+
+# ----------------------------------------------------------- implicit momentum
+# FormFunction_SNES (momentum.c:2285-2440): F(U) = -(Uz - Uo)/dt + 0.5 Ff2(Uz)
+# + 0.5 RHS_o - dP for coeff 1, -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt + Ff2(Uz) - dP
+# otherwise, Uz = U with its wall components zeroed.
+
+
+def test_momentum_residual_of_uniform_flows():
+    """On a periodic sheared grid a uniform flow has no Formfunction_2 part, so
+    F is the time terms and -dP alone: exact for both time formulas, and zero at
+    U = Uo - dt dP (coeff 1), the solution the implicit step must find."""
+    g, f, cent = sheared(10, 9, 11, per=(1, 1, 1))
+    U, Uo, Urm1, dU = (np.array(v) for v in ([0.8, -0.4, 0.3], [0.7, -0.3, 0.2], [0.5, -0.1, 0.4], [0.2, 0.1, -0.3]))
+    u, uo, urm1, dp = (uniform_flux(f, v) for v in (U, Uo, Urm1, dU))
+    dt = 0.05
+    scale = np.abs(u).max() / dt
+    F, uz, _ = O.momentum_residual(g, (0,) * 6, f, dp, uo, u, ren=100.0, dt=dt, coeff=1.0)
+    np.testing.assert_array_equal(uz, u)  # no walls: nothing zeroed
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_allclose(F[inner], (-(u - uo) / dt - dp)[inner], rtol=0, atol=1e-12 * scale)
+    F2, _, _ = O.momentum_residual(g, (0,) * 6, f, dp, uo, u, ren=100.0, dt=dt, coeff=1.5, urm1=urm1)
+    np.testing.assert_allclose(F2[inner], (-(1.5 * u - 2 * uo + 0.5 * urm1) / dt - dp)[inner], rtol=0,
+                               atol=1e-12 * scale)
+    ustar = uniform_flux(f, Uo - dt * dU)
+    F3, _, _ = O.momentum_residual(g, (0,) * 6, f, dp, uo, ustar, ren=100.0, dt=dt, coeff=1.0)
+    assert np.abs(F3[inner]).max() <= 1e-12 * scale
+
+
+def test_momentum_residual_time_and_ff2_parts_and_rhs_o():
+    """coeff 1 weights Formfunction_2 by 0.5 and adds 0.5 RHS_o; coeff 1.5 takes it
+    whole.  F minus its time terms and -dP equals the scaled Formfunction_2 of Uz
+    (to rounding: the sums associate differently), except on faces that
+    Formfunction_2 zeroes, where F is zero too."""
+    g, f, cent = sheared(12, 11, 13)
+    L = np.array([[0.3, -0.2, 0.5], [0.1, 0.4, -0.3], [-0.6, 0.2, 0.25]])
+    rng = np.random.default_rng(2)
+    u = 0.3 * rng.standard_normal(cent.shape)
+    uo = 0.3 * rng.standard_normal(cent.shape)
+    rhs_o = rng.standard_normal(cent.shape)
+    dp = 0.1 * rng.standard_normal(cent.shape)
+    bc = (0, 0, 1, 1, 0, 0)
+    dt, ren = 0.02, 150.0
+    for coeff, w in ((1.0, 0.5), (1.5, 1.0)):
+        F, uz, ucat = O.momentum_residual(g, bc, f, dp, uo, u, ren=ren, dt=dt, coeff=coeff, urm1=uo,
+                                          rhs_o=rhs_o)
+        ff2 = O.formfunction2(g, bc, 0, f, uz, ucat, ren, scale=w)
+        time = (-(uz - uo) / dt) if coeff == 1.0 else (-(1.5 * uz - 2 * uo + 0.5 * uo) / dt)
+        want = time + ff2 + (0.5 * rhs_o if coeff == 1.0 else 0.0) - dp
+        live = ff2 != 0
+        assert live.sum() > 0.8 * live[1:-1, 1:-1, 1:-1].size
+        np.testing.assert_allclose(F[live], want[live], rtol=1e-12, atol=1e-12 * np.abs(want).max())
+        # faces Formfunction_2 zeroes carry only what is added after it
+        tail = (0.5 * rhs_o if coeff == 1.0 else 0.0) - dp
+        np.testing.assert_allclose(F[~live], np.broadcast_to(tail, F.shape)[~live], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("bc, comp, where", [
+    ((1, 1, 0, 0, 0, 0), 0, "i"), ((-1, 0, 0, 0, 0, 0), 0, "i0"), ((0, 0, 1, 1, 0, 0), 1, "j"),
+    ((0, 0, 12, 2, 0, 0), 1, "j"), ((0, 0, 0, 0, 1, 1), 2, "k"), ((10, 10, 0, 0, 0, 0), 0, "islip"),
+    ((0, 0, 10, -10, 0, 0), 1, "jslip")])
+def test_momentum_wall_zeroing_hand_built(bc, comp, where):
+    """The wall components FormFunction_SNES zeroes, from its rules written out
+    by hand: no-slip / wall-model walls zero the normal flux on the wall face
+    (i = 0 and i = mx - 2 for the two i walls, likewise j and k), cavity and
+    couette walls the j ones, slip walls the i or j faces away from the
+    boundary rows of the other two axes; nothing else changes."""
+    g = O.grid((6, 7, 8), (0, 0, 0))  # mz, my, mx
+    mz, my, mx = 6, 7, 8
+    u = np.random.default_rng(1).uniform(1.0, 2.0, (mz, my, mx, 3))
+    uz = O.momentum_bc_zero(g, bc, u)
+    want = u.copy()
+    K, J, I = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    if where in ("i", "i0"):
+        want[:, :, 0, 0] = 0
+        if where == "i":
+            want[:, :, mx - 2, 0] = 0
+    elif where == "j":
+        want[:, 0, :, 1] = 0 if bc[2] in (1, 12) else want[:, 0, :, 1]
+        want[:, my - 2, :, 1] = 0
+    elif where == "k":
+        want[0, :, :, 2] = 0
+        want[mz - 2, :, :, 2] = 0
+    elif where == "islip":
+        rows = (J > 0) & (J < my - 1) & (K > 0) & (K < mz - 1)
+        for i in (0, mx - 2):
+            want[..., i, 0][rows[..., i]] = 0
+    elif where == "jslip":
+        rows = (I > 0) & (I < mx - 1) & (K > 0) & (K < mz - 1)
+        for j in (0, my - 2):
+            want[:, j, :, 1][rows[:, j, :]] = 0
+    np.testing.assert_array_equal(uz, want)

@@ -55,6 +55,14 @@ class ServerOpts(C.Structure):
                 ("max_queue", C.c_int), ("deadline_ms", C.c_double)]
 
 
+class NewtonStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int), ("krylov_iterations", C.c_int), ("residual_evals", C.c_int),
+                ("converged", C.c_int), ("fnorm0", C.c_double), ("fnorm", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("vcycles", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
                 ("bnorm", C.c_double), ("max_div", C.c_double), ("ibm_flux", C.c_double * 4),
@@ -89,6 +97,9 @@ def lib():
         "vfs_pp_pressure_gradient_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 15),
         "vfs_pp_formfunction2_device": (C.c_int, [C.POINTER(Grid)] + [vp] * 18 + [C.c_double, C.c_double, vp]),
         "vfs_pp_runge_kutta_device": (C.c_int, [H] + [vp] * 18 + [C.c_double, C.c_double, C.c_double]),
+        "vfs_pp_momentum_residual_device": (C.c_int, [H] + [vp] * 20 + [C.c_double] * 3 + [vp, vp]),
+        "vfs_pp_implicit_momentum_device": (C.c_int, [H] + [vp] * 21 + [C.c_double] * 4 +
+                                            [C.c_int, C.POINTER(NewtonStats)]),
         "vfs_pp_petsc_vec_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_longlong)]),
         "vfs_pp_read_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t]),
         "vfs_pp_write_petsc_vec_device": (C.c_int, [C.c_char_p, vp, C.c_size_t, C.c_int]),
@@ -722,6 +733,44 @@ def RungeKutta(handle: "Handle", metrics: dict, dp, ucont, ucat, ren: float, dt:
     _check(lib().vfs_pp_runge_kutta_device(handle._h, *_momentum_ptrs(g, metrics), _tptr(dp, V, "dp"),
                                            _tptr(ucont, V, "ucont"), _tptr(ucat, V, "ucat"), float(ren), float(dt),
                                            float(st)), "vfs_pp_runge_kutta_device")
+
+
+def _opt_ptr(t, shape, what):
+    return None if t is None else _tptr(t, shape, what)
+
+
+def momentum_residual(handle: "Handle", metrics: dict, dp, ucont_o, ucont, ren: float, dt: float, coeff: float = 1.0,
+                      ucont_rm1=None, rhs_o=None):
+    """FormFunction_SNES (momentum.c:2285) on the device: (F, Ucat) of the implicit
+    momentum equation at Ucont, immersed == 0 (new device tensors)."""
+    import torch
+
+    g = handle.grid
+    V = dmda_shape(g, vector=True)
+    F = torch.zeros(V, dtype=torch.float64, device=ucont.device)
+    ucat = torch.zeros(V, dtype=torch.float64, device=ucont.device)
+    _check(lib().vfs_pp_momentum_residual_device(
+        handle._h, *_momentum_ptrs(g, metrics), _tptr(dp, V, "dp"), _tptr(ucont_o, V, "ucont_o"),
+        _opt_ptr(ucont_rm1, V, "ucont_rm1"), _opt_ptr(rhs_o, V, "rhs_o"), _tptr(ucont, V, "ucont"), float(ren),
+        float(dt), float(coeff), _tptr(ucat, V, "ucat"), _tptr(F, V, "F")), "vfs_pp_momentum_residual_device")
+    return F, ucat
+
+
+def Implicit_MatrixFree(handle: "Handle", metrics: dict, dp, ucont_o, ucont, ucat, ren: float, dt: float,
+                        coeff: float = 1.0, ucont_rm1=None, rhs_o=None, rtol: float = 1e-4, max_newton: int = 50,
+                        strict: bool = True) -> NewtonStats:
+    """implicitsolver.c:281 on the device: Ucont (in place) solving the implicit
+    momentum equation by Jacobian-free Newton-Krylov, Ucat its Contra2Cart.
+    strict=False returns the stats of a solve that missed rtol instead of raising."""
+    g = handle.grid
+    V = dmda_shape(g, vector=True)
+    st = NewtonStats()
+    rc = lib().vfs_pp_implicit_momentum_device(
+        handle._h, *_momentum_ptrs(g, metrics), _tptr(dp, V, "dp"), _tptr(ucont_o, V, "ucont_o"),
+        _opt_ptr(ucont_rm1, V, "ucont_rm1"), _opt_ptr(rhs_o, V, "rhs_o"), _tptr(ucont, V, "ucont"),
+        _tptr(ucat, V, "ucat"), float(ren), float(dt), float(coeff), float(rtol), int(max_newton), C.byref(st))
+    _check(rc, "vfs_pp_implicit_momentum_device", allow=() if strict else (VFS_PP_ENOCONV,))
+    return st
 
 
 def petsc_vec_length(path) -> int:
