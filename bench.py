@@ -478,6 +478,72 @@ def host_array_block(E, torch, case, steps):
                 note="vfs_pp_project on host arrays: Ucont, Phi, P, Ucat copied in and out over PCIe each step")
 
 
+def time_step_block(E, torch, case, steps):
+    """Whole time steps kept on the device (Pressure_Gradient of the previous
+    pressure, the momentum predictor, the pressure projection), timed per step:
+    the explicit RungeKutta predictor on the projection's own case, and the
+    Jacobian-free Newton-Krylov implicit predictor (Implicit_MatrixFree,
+    immersed = 0) on a walled stretched channel."""
+    from vfs_v3p2_amd import cases
+
+    def setup(c):
+        g = E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed)
+        h = E.Handle(g, E.default_opts(precision=1, rtol=RTOL, div_tol=DIV_TOL))
+        h.set_geometry(c.fields)
+        met = {k: torch.from_numpy(np.ascontiguousarray(c.fields[k])).cuda() for k in E.MOMENTUM_METRICS}
+        nv = torch.from_numpy(np.ascontiguousarray(c.fields["nvert"])).cuda()
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)  # a divergence-free start
+        return g, h, met, nv, u, phi, p, ucat
+
+    out = {}
+    ren, dt = 1000.0, 0.005
+    g, h, met, nv, u, phi, p, ucat = setup(case)
+    lat, its = [], []
+    for k in range(steps + 1):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        dp = E.Pressure_Gradient(g, met, nv, p)
+        E.RungeKutta(h, met, dp, u, ucat, ren, dt, 1.0)
+        st = h.project(u, phi, p, ucat, dt, 1.0, 1.0, strict=True)
+        torch.cuda.synchronize()
+        if k:
+            lat.append(time.perf_counter() - a)
+            its.append(st.iterations)
+    h.close()
+    out["explicit"] = dict(case=f"hill {case.nx}x{case.ny}x{case.nz} + IB sphere, ren {ren:g}, dt {dt:g}",
+                           ms_per_step=round(1e3 * float(np.median(lat)), 3), steps=steps,
+                           projection_iterations=int(max(its)), max_div=float(st.max_div))
+    c2 = cases.channel_case(128, 96, 96, beta=2.0, seed=5)
+    g, h, met, nv, u, phi, p, ucat = setup(c2)
+    lat, newton, gm = [], [], []
+    for k in range(steps + 1):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        dp = E.Pressure_Gradient(g, met, nv, p)
+        rhs_o = E.Formfunction_2(g, met, nv, u, ucat, ren)
+        un = u.clone()
+        ns = E.Implicit_MatrixFree(h, met, dp, u, un, ucat, ren, dt, 1.0, rhs_o=rhs_o, rtol=1e-4)
+        st = h.project(un, phi, p, ucat, dt, 1.0, 1.0, strict=True)
+        u = un
+        torch.cuda.synchronize()
+        if k:
+            lat.append(time.perf_counter() - a)
+            newton.append(ns.newton_iterations)
+            gm.append(ns.krylov_iterations)
+    h.close()
+    torch.cuda.empty_cache()
+    out["implicit"] = dict(case=f"channel {c2.nx}x{c2.ny}x{c2.nz}, walls in y, ren {ren:g}, dt {dt:g}, "
+                                f"rtol 1e-4 on ||F||",
+                           ms_per_step=round(1e3 * float(np.median(lat)), 3), steps=steps,
+                           newton_iterations_max=int(max(newton)), gmres_iterations_max=int(max(gm)),
+                           max_div=float(st.max_div))
+    return out
+
+
 def run_form(E, torch, case, precision, steps, warmup):
     g = E.make_grid(case.nx, case.ny, case.nz, case.per, case.bctype, case.immersed)
     opts = E.default_opts(precision=precision, rtol=RTOL, div_tol=DIV_TOL)
@@ -656,6 +722,7 @@ def main():
         out["error"] = "max|div u| above 1e-8"
     if not args.single_only:
         out["host_array"] = host_array_block(E, torch, case, max(3, min(args.steps, 10)))
+        out["time_step"] = time_step_block(E, torch, case, 5)
     if not args.no_cpu:
         try:
             from oracle import cpu_baseline
