@@ -494,3 +494,48 @@ def test_kernel_timer_ids(E, torch_cuda):
         h.time_kernel(8, 3)
     assert ei.value.code == E.VFS_PP_EINVAL
     h.close()
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("ny", [136, 300])
+def test_y_line_layouts_against_the_oracle(E, torch_cuda, precision, ny):
+    """Every y-line level through the line kernels that its length selects: lines
+    of up to 256 rows take the segmented, fused step (FP32: 16 segments of up to
+    16 rows; FP64: 32 of up to 8), longer ones the residual kernel and one thread
+    per line; nx = 20 leaves partial blocks.  Single-case and batched solves agree;
+    the oracle's Divergence of the projected Ucont meets 1e-8 and the oracle's
+    Projection of the engine's phi reproduces that Ucont; at ny = 136, where the
+    CPU multigrid port converges, phi matches its solve."""
+    torch = torch_cuda
+    c = cases.channel_case(20, ny, 6, beta=2.5, seed=4)
+    print(f"y-line layouts: channel 20x{ny}x6, precision {precision}", flush=True)
+    A = O.Assembled(c, drop=False)
+    outs = []
+    for batched in (False, True):
+        h = handle_for(E, c, precision=precision, smoother=2, rtol=1e-12)
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        if batched:
+            B = E.Batch([h])
+            st = B.project([u], [phi], [p], [ucat], [c.dt], [c.st], [c.coeff], strict=True)[0]
+            B.close()
+        else:
+            st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+        assert "y-line" in h.describe()
+        h.close()
+        assert st.converged and st.max_div <= 1e-8
+        outs.append((phi.cpu().numpy(), u.cpu().numpy()))
+    assert relL2(outs[1][0], outs[0][0], A.fluid_cells) <= 1e-9
+    phi, ug = outs[0]
+    assert O.divergence(A.g, ug, np.ascontiguousarray(c.fields["aj"]), A.nvert) <= 1e-8
+    u = np.ascontiguousarray(c.fields["ucont"].copy())
+    O.volume_flux(A.g, c.immersed, u, c.fields)
+    O.projection(A.g, c.fields, np.ascontiguousarray(phi), u, c.dt, c.st, c.coeff)
+    np.testing.assert_allclose(ug, u, rtol=0, atol=1e-13 * np.abs(u).max())
+    if ny == 136:
+        mg = O.CpuMG(A)
+        r = O.cpu_projection(c, A, mg, rtol=1e-12)
+        mg.close()
+        assert relL2(phi, r["phi"], A.fluid_cells) <= 1e-8
