@@ -192,6 +192,39 @@ def test_projection_parity(E, name, precision):
     user.handle.close()
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_coarse_tail_is_bit_identical(E, torch_cuda, name):
+    """The FP32 V-cycle's bottom (the last point level of at most 1024 cells and the
+    dense solve in one workgroup, coarse_tail=1, the default) gives the same bits
+    as one launch per step (coarse_tail=0): phi, P, Ucont, Ucat and the iteration
+    count.  The hill cases (periodic in i and k) put the tail on level 1 (12x8x8
+    above a 6x4x4 dense level), the affine box on the finest level itself (10x9x8,
+    so its final x is the V-cycle's output); the other cases end in y-line or
+    larger levels and must not take it."""
+    torch = torch_cuda
+    c, _, _ = get_case(name)
+    outs = []
+    for tail in (1, 0):
+        h = handle_for(E, c, rtol=1e-12)
+        h.set_opts(coarse_tail=tail)
+        desc = h.describe()
+        print(f"case {name} coarse_tail {tail}:\n{desc}", flush=True)
+        if tail == 0 or name not in ("hill_sphere", "hill_sphere_immersed0", "affine"):
+            assert "coarse tail" not in desc
+        else:
+            assert "coarse tail" in desc
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+        outs.append((st.iterations, phi.cpu().numpy(), p.cpu().numpy(), u.cpu().numpy(), ucat.cpu().numpy()))
+        h.close()
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_drop_rule_solve(E):
     """drop_small=1 solves the assembled HYPRE matrix (|a| <= 1e-10 dropped)."""
     c = cases.hill_case(16, 10, 12, h=0.3, sphere=(0.5, 0.8, 0.0, 0.4), seed=6)

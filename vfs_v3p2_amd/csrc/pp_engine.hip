@@ -358,6 +358,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
       double* w = h->V + (size_t)(j + 1) * n;
       // z_j = M^-1 v_j (the V-cycle reads v_j from its level-0 right-hand side)
       h->mg->set_out(Z + (size_t)j * n);
+      h->mg->tail = h->opts.coarse_tail != 0;
       h->mg->vcycle(h->s);
       (*vcyc)++;
       double sub = 0.;
@@ -485,6 +486,7 @@ static void set_mask_core(vfs_pp_handle* h, const double* dnvert, bool keep_scra
   build_hierarchy(h, tmp);
   if (h->opts.precision == 1) h->mg = build_mg<float>(h);
   else h->mg = build_mg<double>(h);
+  h->mg->tail = h->opts.coarse_tail != 0;
   h->desc = h->mg->describe();
   h->setup.clear();
   HIPCHECK(hipStreamSynchronize(s));
@@ -732,6 +734,7 @@ void vfs_pp_default_opts(vfs_pp_opts* o) {
   o->cheb_hi = 1.1;
   o->orthog = 2;
   o->verbose = 0;
+  o->coarse_tail = 1;
 }
 
 const char* vfs_pp_error_string(int code) {
@@ -806,6 +809,10 @@ int vfs_pp_set_opts(vfs_pp_handle* h, const vfs_pp_opts* o) {
     return VFS_PP_ESTATE;
   if (!(o->rtol > 0.)) return VFS_PP_EINVAL;
   h->opts = *o;
+  if (h->mg) {
+    h->mg->tail = o->coarse_tail != 0;
+    h->desc = h->mg->describe();
+  }
   return VFS_PP_OK;
 }
 

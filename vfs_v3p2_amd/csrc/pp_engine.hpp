@@ -109,6 +109,7 @@ inline Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {
 struct MGBase {
   virtual ~MGBase() = default;
   virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
+  bool tail = true;  // run the bottom point level and the dense solve as one workgroup (k_coarse_tail)
   virtual bool line0() const = 0;           // the finest level uses the y-line smoother
   virtual void* f0() = 0;
   virtual void set_out(void* out) = 0;
@@ -198,6 +199,13 @@ struct MG : MGBase {
   void vcycle(hipStream_t s) override { cycle(s, 0, out); }
   bool line0() const override { return !lev.empty() && lev[0].line; }
 
+  // the level just above the dense one, when it can run as the coarse tail
+  bool tail_level(size_t l) const {
+    const Lev& L = lev[l];
+    return std::is_same<T, float>::value && tail && l + 2 == lev.size() && !L.line && !L.lfac && L.d.n() <= (size_t)TAIL_MAX_CELLS &&
+           ncoarse <= TAIL_MAX_CELLS && degree <= TAIL_MAX_DEGREE && (int)L.c1.size() >= degree;
+  }
+
   // returns the buffer holding this level's result
   T* cycle(hipStream_t s, size_t l, T* final_out) {
     Lev& L = lev[l];
@@ -205,6 +213,24 @@ struct MG : MGBase {
       T* x = final_out ? final_out : L.xa;
       launch_dense_solve<T>(s, ncoarse, cinv, L.f, x);
       return x;
+    }
+    if (tail_level(l)) {
+      TailArgs<T> a{};
+      a.L = L.view();
+      a.f = L.f;
+      a.dinv = L.dinv;
+      a.xout = final_out ? final_out : L.xa;
+      a.dc = lev[l + 1].d;
+      a.clsc = lev[l + 1].cls;
+      a.cinv = cinv;
+      a.degree = degree;
+      a.first2 = degree >= 2 ? 1 : 0;
+      for (int k = 0; k < degree; k++) {
+        a.c1[k] = L.c1[k];
+        a.c2[k] = L.c2[k];
+      }
+      launch_coarse_tail<T>(s, a);
+      return a.xout;
     }
     T* x = L.xa;
     smooth(s, L, true, x, nullptr);
@@ -222,6 +248,7 @@ struct MG : MGBase {
       o << "level " << l << ": " << lev[l].d.nx << "x" << lev[l].d.ny << "x" << lev[l].d.nz;
       if (l + 1 == lev.size()) o << " dense(" << ncoarse << ", components " << ncomp << ")";
       else o << (lev[l].line ? " y-line" : " point") << " chebyshev deg " << degree << " lambda " << lev[l].lam;
+      if (l + 1 < lev.size() && tail_level(l)) o << " (coarse tail: with the dense solve in one workgroup)";
       o << "\n";
     }
     return o.str();

@@ -215,6 +215,31 @@ template <typename T>
 void launch_prolong_add(hipStream_t s, LevelView<T> F, Dims dc, const uint8_t* cls_c, const T* xc, T* xf);
 template <typename T>
 void launch_dense_solve(hipStream_t s, int n, const T* cinv, const T* f, T* x);
+
+// The bottom of a V-cycle in one 1024-thread workgroup (k_coarse_tail): a point
+// level of at most TAIL_MAX_CELLS cells whose next level is the dense one --
+// pre-smoothing from zero (degree Chebyshev steps, the first two fused when
+// first2), residual and restriction, dense solve, prolongation, post-smoothing --
+// with the level's iterates in LDS; the final x goes to xout.  Same results, bit
+// for bit, as the separate launches.  FP32 levels only.
+constexpr int TAIL_MAX_CELLS = 1024;
+constexpr int TAIL_MAX_DEGREE = 8;
+template <typename T>
+struct TailArgs {
+  LevelView<T> L;
+  const T* f;
+  const T* dinv;
+  T* xout;
+  Dims dc;              // the dense level
+  const uint8_t* clsc;  // its class bytes
+  const T* cinv;        // its inverse, column-major dc.n() x dc.n()
+  int degree;
+  int first2;
+  T c1[TAIL_MAX_DEGREE], c2[TAIL_MAX_DEGREE];
+};
+template <typename T>
+void launch_coarse_tail(hipStream_t s, const TailArgs<T>& a);
+
 template <typename T>
 void launch_normsq(hipStream_t s, size_t n, const T* a, double* partial, int* nblk);
 template <typename T>

@@ -135,6 +135,290 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const T* __restrict_
   dense_rows(n, cinv, f, x, (int)blockIdx.x, fs, part);
 }
 
+// ------------------------------------------------------------- coarse tail
+// The bottom of a V-cycle in one workgroup: the last smoothed level (point
+// Chebyshev, at most 1024 cells, one per thread) with its iterates in LDS, the
+// residual restricted to the dense level, the dense solve, the prolongation and
+// the post-smoothing.  Each thread loads its cell's stencil indices, code word,
+// class byte, f, D^-1 and its 18 face coefficients once into registers, so the
+// phases between barriers touch only LDS (and the dense inverse), instead of one
+// launch with cold caches per phase.  Every step is the arithmetic of the
+// separate kernels (smooth_point_body, resid_restrict_blk / resid_restrict_cell,
+// dense_rows, prolong_add_cell) in the same order, so the result is the same bits.
+// dense inverses up to this many entries (64 KB in FP32) are staged in LDS
+constexpr int TAIL_CINV_LDS = 16384;
+template <typename T, typename TG>
+struct TailLds {
+  T cinv[TAIL_CINV_LDS];                                          // the dense inverse, when it fits
+  TG g[9][TAIL_MAX_CELLS];                                        // the level's face coefficients
+  T xa[TAIL_MAX_CELLS], xb[TAIL_MAX_CELLS], fd[TAIL_MAX_CELLS];  // iterates; f D^-1, then the residual
+  T fc[TAIL_MAX_CELLS], xc[TAIL_MAX_CELLS];                        // dense level: right-hand side and solution
+  T part[4][4][64];                                               // dense_rows partial sums, 4 row blocks
+  T c1[TAIL_MAX_DEGREE], c2[TAIL_MAX_DEGREE];                     // Chebyshev coefficients per step
+  uint8_t clsc[TAIL_MAX_CELLS];                                   // dense level's class bytes
+};
+
+// (A x)_c, the diagonal and whether the cell has a row, as flux_by_class and
+// smooth_point_body form them (coefficients already in registers)
+template <typename T, typename TG>
+__device__ __forceinline__ void tail_flux(const RowCoef<TG>& rc, uint32_t code, bool interior, bool scaled, T gs,
+                                          const T (&xv)[19], T& ax, T& diag, bool& row) {
+  row = true;
+  if (__all(interior)) {
+    flux_row_v<X_CENTRAL>(rc, CODE_INTERIOR, xv, ax, diag);
+  } else {
+    flux_row_v<-1>(rc, code, xv, ax, diag);
+    row = (code & ROW_BIT) != 0;
+  }
+  if (scaled) {
+    ax *= gs;
+    diag *= gs;
+  }
+}
+
+// the cell's stencil indices and its row's 18 coefficients (load_coef's faces) from LDS
+template <typename T, typename TG>
+__device__ __forceinline__ void tail_row(const TailLds<T, TG>& sh, const Dims& d, int i, int j, int k, Nbr& nb,
+                                         RowCoef<TG>& rc) {
+  neighbours(d, i, j, k, nb);
+  const uint32_t own[6] = {nb.s[CP], nb.s[WP], nb.s[CP], nb.s[SP], nb.s[CP], nb.s[BP]};
+#pragma unroll
+  for (int f = 0; f < 6; f++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) rc.v[f][e] = sh.g[3 * (f / 2) + e][own[f]];
+}
+
+// one three-term Chebyshev step from cur (neighbours) and prev (own cell): x_new
+template <typename T, typename TG>
+__device__ __forceinline__ T tail_step(const TailLds<T, TG>& sh, const Dims& d, int i, int j, int k, uint32_t code,
+                                       bool interior, bool scaled, T gs, T fv, const T* cur, const T* prev,
+                                       uint32_t c, T c1, T c2) {
+  const T xp = (c1 != T(0)) ? prev[c] : T(0);
+  Nbr nb;
+  RowCoef<TG> rc;
+  tail_row(sh, d, i, j, k, nb, rc);
+  T xv[19];
+#pragma unroll
+  for (int s = 0; s < 19; s++) xv[s] = cur[nb.s[s]];
+  T ax, dg;
+  bool row;
+  tail_flux(rc, code, interior, scaled, gs, xv, ax, dg, row);
+  const bool live = row && dg != T(0);
+  const T z = live ? (fv - ax) / (live ? dg : T(1)) : T(0);
+  const T xc = xv[CP];
+  const T dold = c1 == T(0) ? T(0) : xc - xp;
+  return row ? xc + (c1 * dold + c2 * z) : T(0);
+}
+
+template <typename T, typename TG>
+__device__ __forceinline__ void coarse_tail_body(const TailArgs<T>& A, const TG* __restrict__ g, bool scaled,
+                                                 TailLds<T, TG>& sh) {
+  const LevelView<T>& L = A.L;
+  const Dims& d = L.d;
+  const int n = (int)d.n(), nc = (int)A.dc.n();
+  const int t = (int)threadIdx.x;
+  const bool mine = t < n;
+  const uint32_t c = (uint32_t)t;
+  const int i = mine ? t % d.nx : 0, j = mine ? (t / d.nx) % d.ny : 0, k = mine ? t / (d.nx * d.ny) : 0;
+  uint32_t code = 0;
+  uint8_t cl = 0;
+  T fv = T(0), dv = T(0);
+  if (mine) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) sh.g[q][c] = ld_at(g + (size_t)q * d.n(), c);
+    code = L.code[c];
+    cl = L.cls[c];
+    fv = A.f[c];
+    dv = A.dinv[c];
+    sh.fd[c] = fv * dv;
+  }
+  for (int q = t; q < nc; q += 1024) sh.clsc[q] = A.clsc[q];
+  // the dense inverse's loads overlap the smoothing that precedes its use
+  const bool cinv_lds = nc * nc <= TAIL_CINV_LDS;
+  if (cinv_lds)
+    for (int q = t; q < nc * nc; q += 1024) sh.cinv[q] = A.cinv[q];
+  if (t < TAIL_MAX_DEGREE) {
+    sh.c1[t] = A.c1[t];
+    sh.c2[t] = A.c2[t];
+  }
+  const bool interior = (cl & CLS_INTERIOR) != 0;
+  const T gs = L.gs;
+  __syncthreads();
+
+  // pre-smoothing from zero: the fused first two steps (SM_FIRST2), or one step
+  T* cur = sh.xb;
+  T* prev = sh.xa;
+  int k0;
+  if (A.first2) {
+    if (mine) {
+      const T c0 = sh.c2[0], c1 = sh.c1[1], c2 = sh.c2[1];
+      Nbr nb;
+      RowCoef<TG> rc;
+      tail_row(sh, d, i, j, k, nb, rc);
+      T xv[19];
+#pragma unroll
+      for (int s = 0; s < 19; s++) xv[s] = c0 * sh.fd[nb.s[s]];
+      T ax, dg;
+      bool row;
+      tail_flux(rc, code, interior, scaled, gs, xv, ax, dg, row);
+      const bool live = row && dg != T(0);
+      const T z = live ? (fv - ax) / (live ? dg : T(1)) : T(0);
+      const T xc = xv[CP];
+      sh.xa[c] = row ? xc : T(0);
+      sh.xb[c] = row ? xc + (c1 * xc + c2 * z) : T(0);
+    }
+    k0 = 2;
+  } else {
+    if (mine) {
+      sh.xb[c] = sh.c2[0] * sh.fd[c];
+      sh.xa[c] = T(0);
+    }
+    k0 = 1;
+  }
+  __syncthreads();
+  for (int s = k0; s < A.degree; s++) {
+    const T xn = mine ? tail_step(sh, d, i, j, k, code, interior, scaled, gs, fv, cur, prev, c, sh.c1[s], sh.c2[s])
+                      : T(0);
+    if (mine) prev[c] = xn;  // the step's x_prev slot takes x_new
+    T* tmp = cur;
+    cur = prev;
+    prev = tmp;
+    __syncthreads();
+  }
+
+  // residual of the fine rows, then the restriction (the form the launch path takes)
+  if (mine) {
+    Nbr nb;
+    RowCoef<TG> rc;
+    tail_row(sh, d, i, j, k, nb, rc);
+    T xv[19];
+#pragma unroll
+    for (int s = 0; s < 19; s++) xv[s] = cur[nb.s[s]];
+    T ax, dg;
+    if (__all((code & ~DIVOK_BIT) == CODE_INTERIOR))
+      flux_row_v<X_CENTRAL>(rc, code, xv, ax, dg);
+    else
+      flux_row_v<-1>(rc, code, xv, ax, dg);
+    if (scaled) ax *= gs;
+    sh.fd[c] = (code & ROW_BIT) ? fv - ax : T(0);
+  }
+  __syncthreads();
+  const Dims& dc = A.dc;
+  const int cm = (dc.nx != d.nx ? 1 : 0) | (dc.ny != d.ny ? 2 : 0) | (dc.nz != d.nz ? 4 : 0);
+  for (int q = t; q < nc; q += 1024) {
+    const int ic = q % dc.nx, jc = (q / dc.nx) % dc.ny, kc = q / (dc.nx * dc.ny);
+    T v = T(0);
+    if (cm == 7 || cm == 5) {
+      // resid_restrict_blk: pairs in i, then j and k pairs (j kept when cm == 5)
+      auto r_at = [&](int ii, int jj, int kk) -> T {
+        return (ii < d.nx && jj < d.ny && kk < d.nz) ? sh.fd[d.idx(ii, jj, kk)] : T(0);
+      };
+      auto pair = [&](int jj, int kk) -> T { return r_at(2 * ic, jj, kk) + r_at(2 * ic + 1, jj, kk); };
+      if (cm == 7)
+        v = (pair(2 * jc, 2 * kc) + pair(2 * jc + 1, 2 * kc)) + (pair(2 * jc, 2 * kc + 1) + pair(2 * jc + 1, 2 * kc + 1));
+      else
+        v = pair(jc, 2 * kc) + pair(jc, 2 * kc + 1);
+    } else {
+      // resid_restrict_cell: children in (k, j, i) order, rows only
+      const int i0 = (cm & 1) ? 2 * ic : ic, i1 = (cm & 1) ? min(2 * ic + 1, d.nx - 1) : ic;
+      const int j0 = (cm & 2) ? 2 * jc : jc, j1 = (cm & 2) ? min(2 * jc + 1, d.ny - 1) : jc;
+      const int kb = (cm & 4) ? 2 * kc : kc, k1 = (cm & 4) ? min(2 * kc + 1, d.nz - 1) : kc;
+      for (int kk = kb; kk <= k1; kk++)
+        for (int jj = j0; jj <= j1; jj++)
+          for (int ii = i0; ii <= i1; ii++) {
+            const size_t cf = d.idx(ii, jj, kk);
+            if (L.code[cf] & ROW_BIT) v += sh.fd[cf];
+          }
+    }
+    sh.fc[q] = (sh.clsc[q] & CLS_NONFLUID) ? T(0) : v;
+  }
+  __syncthreads();
+
+  // dense solve (dense_rows): 16 waves = 4 row blocks of 64 rows x 4 column quarters
+  {
+    const int lane = t & 63, w = t >> 6, rb = w >> 2, wq = w & 3;
+    for (int r0 = 0; r0 < nc; r0 += 256) {
+      const int r = r0 + rb * 64 + lane;
+      T a0 = T(0), a1 = T(0);
+      if (r < nc && cinv_lds) {
+        int cc = wq;
+        for (; cc + 4 < nc; cc += 8) {
+          a0 += sh.cinv[cc * nc + r] * sh.fc[cc];
+          a1 += sh.cinv[(cc + 4) * nc + r] * sh.fc[cc + 4];
+        }
+        for (; cc < nc; cc += 4) a0 += sh.cinv[cc * nc + r] * sh.fc[cc];
+      } else if (r < nc) {
+        int cc = wq;
+        for (; cc + 4 < nc; cc += 8) {
+          a0 += A.cinv[(size_t)cc * nc + r] * sh.fc[cc];
+          a1 += A.cinv[(size_t)(cc + 4) * nc + r] * sh.fc[cc + 4];
+        }
+        for (; cc < nc; cc += 4) a0 += A.cinv[(size_t)cc * nc + r] * sh.fc[cc];
+      }
+      sh.part[rb][wq][lane] = a0 + a1;
+      __syncthreads();
+      if (wq == 0 && r < nc)
+        sh.xc[r] = (sh.part[rb][0][lane] + sh.part[rb][1][lane]) + (sh.part[rb][2][lane] + sh.part[rb][3][lane]);
+      __syncthreads();
+    }
+  }
+
+  // prolongation into cur (prolong_add_cell)
+  if (mine) {
+    int ic, jc, kc;
+    const int ia = axis_pick(i, d.nx, d.per[0], cm & 1, ic);
+    const int ja = axis_pick(j, d.ny, d.per[1], cm & 2, jc);
+    const int ka = axis_pick(k, d.nz, d.per[2], cm & 4, kc);
+    const int in = wrapc(ic + ia, dc.nx), jn = wrapc(jc + ja, dc.ny), kn = wrapc(kc + ka, dc.nz);
+    const uint8_t mk = sh.clsc[dc.idx(ic, jc, kn)], mj = sh.clsc[dc.idx(ic, jn, kc)],
+                  mi = sh.clsc[dc.idx(in, jc, kc)];
+    const T x000 = sh.xc[dc.idx(ic, jc, kc)], x010 = sh.xc[dc.idx(ic, jn, kc)];
+    const T x100 = sh.xc[dc.idx(in, jc, kc)], x110 = sh.xc[dc.idx(in, jn, kc)];
+    const T x001 = sh.xc[dc.idx(ic, jc, kn)], x011 = sh.xc[dc.idx(ic, jn, kn)];
+    const T x101 = sh.xc[dc.idx(in, jc, kn)], x111 = sh.xc[dc.idx(in, jn, kn)];
+    const T xo = cur[c];
+    const bool ui = ia != 0 && !(mi & CLS_NONFLUID);
+    const bool uj = ja != 0 && !(mj & CLS_NONFLUID);
+    const bool uk = ka != 0 && !(mk & CLS_NONFLUID);
+    const T y0 = uj ? x010 : x000, y1 = uj ? x110 : x100;
+    const T z0 = uj ? x011 : x001, z1 = uj ? x111 : x101;
+    const T a_c = x000, a_j = y0, a_i = ui ? x100 : x000, a_ij = ui ? y1 : y0;
+    const T b_c = uk ? x001 : a_c, b_j = uk ? z0 : a_j;
+    const T b_i = uk ? (ui ? x101 : x001) : a_i, b_ij = uk ? (ui ? z1 : z0) : a_ij;
+    const T v = (a_c * T(9) + a_j * T(3) + a_i * T(3) + a_ij) * T(3. / 64.) +
+                (b_c * T(9) + b_j * T(3) + b_i * T(3) + b_ij) * T(1. / 64.);
+    if (!(cl & CLS_NONFLUID)) cur[c] = xo + v;
+  }
+  __syncthreads();
+
+  // post-smoothing; the last step goes to the level's x in memory
+  for (int s = 0; s < A.degree; s++) {
+    const T c1 = s == 0 ? T(0) : sh.c1[s];
+    const T xn = mine ? tail_step(sh, d, i, j, k, code, interior, scaled, gs, fv, cur, prev, c, c1, sh.c2[s]) : T(0);
+    if (s + 1 == A.degree) {
+      if (mine) A.xout[c] = xn;
+    } else {
+      if (mine) prev[c] = xn;
+      T* tmp = cur;
+      cur = prev;
+      prev = tmp;
+      __syncthreads();
+    }
+  }
+}
+
+// HALF: the level's coefficients are FP16 (gh), else T (g)
+template <typename T, bool HALF>
+__global__ __launch_bounds__(1024) void k_coarse_tail(TailArgs<T> A) {
+  if constexpr (HALF) {
+    __shared__ TailLds<T, _Float16> sh;
+    coarse_tail_body(A, A.L.gh, true, sh);
+  } else {
+    __shared__ TailLds<T, T> sh;
+    coarse_tail_body(A, A.L.g, false, sh);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_normsq(size_t n, const T* __restrict__ a, double* __restrict__ partial,
                                                 int nblk) {
@@ -337,6 +621,21 @@ void launch_dense_solve(hipStream_t s, int n, const T* cinv, const T* f, T* x) {
 }
 
 template <typename T>
+void launch_coarse_tail(hipStream_t s, const TailArgs<T>& a) {
+  if (a.L.d.n() > (size_t)TAIL_MAX_CELLS || a.dc.n() > (size_t)TAIL_MAX_CELLS || a.degree < 1 ||
+      a.degree > TAIL_MAX_DEGREE)
+    throw StatusError(1, "coarse tail: level or degree out of range");
+  if constexpr (std::is_same<T, float>::value) {
+    if (a.L.gh) k_coarse_tail<T, true><<<1, 1024, 0, s>>>(a);
+    else k_coarse_tail<T, false><<<1, 1024, 0, s>>>(a);
+  } else {
+    // an FP64 step needs more than the 128 registers a 1024-thread workgroup allows
+    throw StatusError(1, "coarse tail: FP32 levels only");
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
 void launch_normsq(hipStream_t s, size_t n, const T* a, double* partial, int* nblk) {
   const int nb = partial_blocks(n);
   *nblk = nb;
@@ -359,6 +658,7 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
   template void launch_resid_restrict<T>(hipStream_t, LevelView<T>, const T*, const T*, Dims, const uint8_t*, T*); \
   template void launch_prolong_add<T>(hipStream_t, LevelView<T>, Dims, const uint8_t*, const T*, T*);              \
   template void launch_dense_solve<T>(hipStream_t, int, const T*, const T*, T*);                                   \
+  template void launch_coarse_tail<T>(hipStream_t, const TailArgs<T>&);                                            \
   template void launch_normsq<T>(hipStream_t, size_t, const T*, double*, int*);                                    \
   template void launch_scale_by<T>(hipStream_t, size_t, T*, const double*);
 INST(float)
