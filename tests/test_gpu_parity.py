@@ -225,6 +225,30 @@ def test_coarse_tail_is_bit_identical(E, torch_cuda, name):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("degree", [1, 2, 4])
+def test_coarse_tail_chebyshev_degrees(E, torch_cuda, degree):
+    """The coarse tail's step sequence for other Chebyshev degrees: one step from zero
+    (degree 1, no fused pair), the fused pair alone (degree 2) and the fused pair
+    with two further steps (degree 4), bit for bit against one launch per step."""
+    torch = torch_cuda
+    c, _, _ = get_case("hill_sphere")
+    outs = []
+    for tail in (1, 0):
+        h = handle_for(E, c, rtol=1e-12, cheb_degree=degree, coarse_tail=tail)
+        assert ("coarse tail" in h.describe()) == (tail == 1)
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+        print(f"degree {degree} coarse_tail {tail}: {st.iterations} iterations", flush=True)
+        outs.append((st.iterations, phi.cpu().numpy(), u.cpu().numpy()))
+        h.close()
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_drop_rule_solve(E):
     """drop_small=1 solves the assembled HYPRE matrix (|a| <= 1e-10 dropped)."""
     c = cases.hill_case(16, 10, 12, h=0.3, sphere=(0.5, 0.8, 0.0, 0.4), seed=6)
