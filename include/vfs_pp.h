@@ -97,10 +97,14 @@ typedef struct {
                         (fused with the operator and with the next V-cycle's input),
                         repeated only when it cancels |w| below 1e-4 |w| */
   int verbose;       /* 1: print the residual history of each solve to stderr */
-  int coarse_tail;   /* 1 (default): in the FP32 multigrid, the V-cycle's last point-smoothed
-                        level (at most 1024 cells) and the dense solve run as one workgroup
-                        with the level's iterates in LDS; 0: one launch per step.  Same
-                        results bit for bit */
+  int fused;         /* fused launches, each giving the same results bit for bit as the
+                        separate ones (default 3):
+                        bit 0: in the FP32 multigrid, the V-cycle's last point-smoothed level
+                               (at most 1024 cells) and the dense solve run as one workgroup
+                               with the level's iterates in LDS;
+                        bit 1: the writers of the V-cycle input also store f D^-1 of the
+                               finest level, which its fused first two Chebyshev steps then
+                               read at the 19 neighbours instead of f and D^-1 */
 } vfs_pp_opts;
 
 typedef struct {

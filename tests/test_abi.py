@@ -89,7 +89,7 @@ int main(void) {
   F(vfs_pp_opts, rtol) F(vfs_pp_opts, div_tol) F(vfs_pp_opts, max_iters) F(vfs_pp_opts, restart)
   F(vfs_pp_opts, smoother) F(vfs_pp_opts, cheb_degree) F(vfs_pp_opts, coarse_max) F(vfs_pp_opts, drop_small)
   F(vfs_pp_opts, zero_guess) F(vfs_pp_opts, cheb_lo) F(vfs_pp_opts, cheb_hi) F(vfs_pp_opts, orthog)
-  F(vfs_pp_opts, verbose) F(vfs_pp_opts, coarse_tail)
+  F(vfs_pp_opts, verbose) F(vfs_pp_opts, fused)
   F(vfs_pp_stats, iterations) F(vfs_pp_stats, vcycles) F(vfs_pp_stats, converged) F(vfs_pp_stats, rel_residual)
   F(vfs_pp_stats, bnorm) F(vfs_pp_stats, max_div) F(vfs_pp_stats, ibm_flux) F(vfs_pp_stats, phi_mean)
   F(vfs_pp_stats, ms_total)

@@ -192,56 +192,58 @@ def test_projection_parity(E, name, precision):
     user.handle.close()
 
 
+@pytest.mark.parametrize("precision", [1, 0])
 @pytest.mark.parametrize("name", list(CASES))
-def test_coarse_tail_is_bit_identical(E, torch_cuda, name):
-    """The FP32 V-cycle's bottom (the last point level of at most 1024 cells and the
-    dense solve in one workgroup, coarse_tail=1, the default) gives the same bits
-    as one launch per step (coarse_tail=0): phi, P, Ucont, Ucat and the iteration
-    count.  The hill cases (periodic in i and k) put the tail on level 1 (12x8x8
-    above a 6x4x4 dense level), the affine box on the finest level itself (10x9x8,
-    so its final x is the V-cycle's output); the other cases end in y-line or
-    larger levels and must not take it."""
+def test_fused_launches_are_bit_identical(E, torch_cuda, name, precision):
+    """The fused launches (option `fused`) give the same bits as the separate ones:
+    bit 0, the FP32 V-cycle's bottom (the last point level of at most 1024 cells and
+    the dense solve in one workgroup); bit 1, the finest level's f D^-1 stored by
+    the writers of the V-cycle input and read by its fused first Chebyshev steps.
+    phi, P, Ucont, Ucat and the iteration count are compared for every combination
+    against fused=0.  The hill cases (periodic in i and k) put the tail on level 1
+    (12x8x8 above a 6x4x4 dense level), the affine box on the finest level itself
+    (10x9x8, so its final x is the V-cycle's output); the other cases end in y-line
+    or larger levels and must not take it, nor does the FP64 hierarchy."""
     torch = torch_cuda
     c, _, _ = get_case(name)
-    outs = []
-    for tail in (1, 0):
-        h = handle_for(E, c, rtol=1e-12)
-        h.set_opts(coarse_tail=tail)
+    tail_expected = precision == 1 and name in ("hill_sphere", "hill_sphere_immersed0", "affine")
+    outs = {}
+    for fused in (0, 1, 2, 3):
+        h = handle_for(E, c, rtol=1e-12, precision=precision, fused=fused)
         desc = h.describe()
-        print(f"case {name} coarse_tail {tail}:\n{desc}", flush=True)
-        if tail == 0 or name not in ("hill_sphere", "hill_sphere_immersed0", "affine"):
-            assert "coarse tail" not in desc
-        else:
-            assert "coarse tail" in desc
+        print(f"case {name} precision {precision} fused {fused}:\n{desc}", flush=True)
+        assert ("coarse tail" in desc) == (tail_expected and (fused & 1) == 1)
         u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
         phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
         p = torch.zeros_like(phi)
         ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
         st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
-        outs.append((st.iterations, phi.cpu().numpy(), p.cpu().numpy(), u.cpu().numpy(), ucat.cpu().numpy()))
+        outs[fused] = (st.iterations, phi.cpu().numpy(), p.cpu().numpy(), u.cpu().numpy(), ucat.cpu().numpy())
         h.close()
-    assert outs[0][0] == outs[1][0]
-    for a, b in zip(outs[0][1:], outs[1][1:]):
-        np.testing.assert_array_equal(a, b)
+    for fused in (1, 2, 3):
+        assert outs[fused][0] == outs[0][0]
+        for x, y in zip(outs[fused][1:], outs[0][1:]):
+            np.testing.assert_array_equal(x, y)
 
 
 @pytest.mark.parametrize("degree", [1, 2, 4])
 def test_coarse_tail_chebyshev_degrees(E, torch_cuda, degree):
     """The coarse tail's step sequence for other Chebyshev degrees: one step from zero
     (degree 1, no fused pair), the fused pair alone (degree 2) and the fused pair
-    with two further steps (degree 4), bit for bit against one launch per step."""
+    with two further steps (degree 4), bit for bit against one launch per step
+    (fused bit 0 clear)."""
     torch = torch_cuda
     c, _, _ = get_case("hill_sphere")
     outs = []
     for tail in (1, 0):
-        h = handle_for(E, c, rtol=1e-12, cheb_degree=degree, coarse_tail=tail)
+        h = handle_for(E, c, rtol=1e-12, cheb_degree=degree, fused=3 if tail else 2)
         assert ("coarse tail" in h.describe()) == (tail == 1)
         u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
         phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
         p = torch.zeros_like(phi)
         ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
         st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
-        print(f"degree {degree} coarse_tail {tail}: {st.iterations} iterations", flush=True)
+        print(f"degree {degree} coarse tail {tail}: {st.iterations} iterations", flush=True)
         outs.append((st.iterations, phi.cpu().numpy(), u.cpu().numpy()))
         h.close()
     assert outs[0][0] == outs[1][0]

@@ -535,7 +535,8 @@ template <typename TF>
 __global__ __launch_bounds__(256) void k_axpy_norm(size_t n, double* __restrict__ w, const double* __restrict__ V,
                                                    size_t ldv, int nv, const double* __restrict__ dots,
                                                    const double* __restrict__ nuinv2, const double* __restrict__ a2,
-                                                   TF* __restrict__ f, double* __restrict__ partial) {
+                                                   TF* __restrict__ f, double* __restrict__ partial,
+                                                   const TF* __restrict__ dinv, TF* __restrict__ fd) {
   __shared__ double lds[4][64];
   const double sc = conv_scale(a2[0]);
   double acc[1] = {0.};
@@ -545,7 +546,9 @@ __global__ __launch_bounds__(256) void k_axpy_norm(size_t n, double* __restrict_
     double v = w[t];
     for (int l = 0; l < nv; l++) v -= (dots[l] * nuinv2[l]) * V[(size_t)l * ldv + t];
     w[t] = v;
-    f[t] = (TF)(v * sc);
+    const TF fv = (TF)(v * sc);
+    f[t] = fv;
+    if (fd) fd[t] = fv * dinv[t];
     acc[0] += v * v;
   }
   block_sums(acc, 1, partial, gridDim.x, blockIdx.x, lds);
@@ -560,20 +563,24 @@ __global__ void k_gs_norm(double* __restrict__ red, int off, int nv, double* __r
 template <typename TF>
 __global__ __launch_bounds__(256) void k_axpy_scale(size_t n, double* __restrict__ w, const double* __restrict__ V,
                                                     size_t ldv, int nv, const double* __restrict__ h,
-                                                    const double* __restrict__ inv, TF* __restrict__ f) {
+                                                    const double* __restrict__ inv, TF* __restrict__ f,
+                                                    const TF* __restrict__ dinv, TF* __restrict__ fd) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   axpy_scale_one(w, V, ldv, nv, h, inv[0], f, t);
+  if (fd) fd[t] = f[t] * dinv[t];
 }
 
 // v = a / sqrt(norm2); f = (TF)v
 template <typename TF>
 __global__ __launch_bounds__(256) void k_scale_conv(size_t n, const double* __restrict__ a,
                                                     const double* __restrict__ nrm2, double* __restrict__ v,
-                                                    TF* __restrict__ f) {
+                                                    TF* __restrict__ f, const TF* __restrict__ dinv,
+                                                    TF* __restrict__ fd) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   scale_conv_elem(a, nrm2[0], v, f, t);
+  if (fd) fd[t] = f[t] * dinv[t];
 }
 
 inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -852,16 +859,18 @@ void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_
 
 template <typename TF>
 void launch_axpy_norm(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* dots,
-                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk) {
+                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk, const TF* dinv,
+                      TF* fd) {
   const unsigned nb = std::min<unsigned>(nb1(n), 2048u);
   *nblk = (int)nb;
-  k_axpy_norm<TF><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, dots, nuinv2, a2, f, partial);
+  k_axpy_norm<TF><<<nb, 256, 0, s>>>(n, w, V, ldv, nv, dots, nuinv2, a2, f, partial, dinv, fd);
   HIPCHECK(hipGetLastError());
 }
 template void launch_axpy_norm<float>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
-                                      const double*, const double*, float*, double*, int*);
+                                      const double*, const double*, float*, double*, int*, const float*, float*);
 template void launch_axpy_norm<double>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
-                                       const double*, const double*, double*, double*, int*);
+                                       const double*, const double*, double*, double*, int*, const double*,
+                                       double*);
 
 double host_conv_scale(double a2) { return conv_scale(a2); }
 
@@ -872,21 +881,24 @@ void launch_gs_norm(hipStream_t s, double* red, int off, int nv, double* host) {
 
 template <typename TF>
 void launch_axpy_scale(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
-                       const double* inv, TF* f) {
-  k_axpy_scale<TF><<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, inv, f);
+                       const double* inv, TF* f, const TF* dinv, TF* fd) {
+  k_axpy_scale<TF><<<nb1(n), 256, 0, s>>>(n, w, V, ldv, nv, h, inv, f, dinv, fd);
   HIPCHECK(hipGetLastError());
 }
 template void launch_axpy_scale<float>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
-                                       const double*, float*);
+                                       const double*, float*, const float*, float*);
 template void launch_axpy_scale<double>(hipStream_t, size_t, double*, const double*, size_t, int, const double*,
-                                        const double*, double*);
+                                        const double*, double*, const double*, double*);
 
 template <typename TF>
-void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f) {
-  k_scale_conv<TF><<<nb1(n), 256, 0, s>>>(n, a, nrm2, v, f);
+void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f,
+                       const TF* dinv, TF* fd) {
+  k_scale_conv<TF><<<nb1(n), 256, 0, s>>>(n, a, nrm2, v, f, dinv, fd);
   HIPCHECK(hipGetLastError());
 }
-template void launch_scale_conv<float>(hipStream_t, size_t, const double*, const double*, double*, float*);
-template void launch_scale_conv<double>(hipStream_t, size_t, const double*, const double*, double*, double*);
+template void launch_scale_conv<float>(hipStream_t, size_t, const double*, const double*, double*, float*,
+                                       const float*, float*);
+template void launch_scale_conv<double>(hipStream_t, size_t, const double*, const double*, double*, double*,
+                                        const double*, double*);
 
 }  // namespace pp

@@ -273,6 +273,8 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
     }
     mg->lev.push_back(L);
   }
+  // the finest level's f D^-1, stored by the writers of the V-cycle input
+  if (mg->lev.size() > 1 && !mg->lev[0].lfac) mg->fd = A.alloc<T>(n0);
   // Chebyshev parameters per smoothed level: the power iterations of all levels are
   // queued together and read back once
   for (size_t l = 0; l + 1 < mg->lev.size(); l++)
@@ -319,6 +321,9 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
   const int m = h->mrest;
   const double rtol = h->opts.rtol;
   TZ* Z = static_cast<TZ*>(h->Z);
+  // the V-cycle input's writers also store f D^-1 for the finest level's first steps
+  TZ* fd0 = (h->opts.fused & 2) ? static_cast<TZ*>(h->mg->fd0()) : nullptr;
+  const TZ* dinv0 = fd0 ? static_cast<const TZ*>(h->mg->dinv0()) : nullptr;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m), hcol(m + 2);
   // lazily normalised basis (orthog 2): U_l = nu_l v_l, V-cycle inputs scaled so
   // that the stored Z_l = sigma_l z_l
@@ -343,7 +348,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
     if (it >= h->opts.max_iters) break;
     // v_0 = r / beta and its copy as the first V-cycle input
     launch_scale_conv<TZ>(h->s, n, true_residual_src(h, x_zero), true_residual_norm2(h, x_zero), h->V,
-                          static_cast<TZ*>(h->mg->f0()));
+                          static_cast<TZ*>(h->mg->f0()), dinv0, fd0);
     x_zero = false;
     if (h->opts.orthog == 2) {
       nu[0] = 1.;
@@ -358,7 +363,8 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
       double* w = h->V + (size_t)(j + 1) * n;
       // z_j = M^-1 v_j (the V-cycle reads v_j from its level-0 right-hand side)
       h->mg->set_out(Z + (size_t)j * n);
-      h->mg->tail = h->opts.coarse_tail != 0;
+      h->mg->tail = (h->opts.fused & 1) != 0;
+      h->mg->use_fd = fd0 != nullptr;
       h->mg->vcycle(h->s);
       (*vcyc)++;
       double sub = 0.;
@@ -372,7 +378,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
         launch_apply_dots<TZ>(h->s, h->d, h->g64, h->code, Z + (size_t)j * n, w, h->V, n, nv, h->partial, &nb);
         launch_finalize(h->s, h->partial, nv + 1, nb, h->red + 8, 0, h->hpin_dev + 8);
         launch_axpy_norm<TZ>(h->s, n, w, h->V, n, nv, h->red + 8, h->red + RED_NUINV2, h->red + 8 + nv,
-                             static_cast<TZ*>(h->mg->f0()), h->partial, &nb);
+                             static_cast<TZ*>(h->mg->f0()), h->partial, &nb, dinv0, fd0);
         launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
         HIPCHECK(hipStreamSynchronize(h->s));
         const double a2 = h->hpin[8 + nv];
@@ -382,7 +388,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
           launch_multidot(h->s, n, w, h->V, n, nv, h->partial, &nb);
           launch_finalize(h->s, h->partial, nv, nb, h->red + 72, 0, h->hpin_dev + 72);
           launch_axpy_norm<TZ>(h->s, n, w, h->V, n, nv, h->red + 72, h->red + RED_NUINV2, h->red + 8 + nv,
-                               static_cast<TZ*>(h->mg->f0()), h->partial, &nb);
+                               static_cast<TZ*>(h->mg->f0()), h->partial, &nb, dinv0, fd0);
           launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
           HIPCHECK(hipStreamSynchronize(h->s));
           for (int l = 0; l <= j; l++) H[(size_t)l * m + j] += h->hpin[72 + l] / (nu[l] * sigma[j]);
@@ -404,7 +410,7 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
         }
         launch_norm2(h->s, n, w, h->partial, &nb);
         launch_finalize(h->s, h->partial, 1, nb, h->red, 0, h->hpin_dev);
-        launch_scale_conv<TZ>(h->s, n, w, h->red, w, static_cast<TZ*>(h->mg->f0()));
+        launch_scale_conv<TZ>(h->s, n, w, h->red, w, static_cast<TZ*>(h->mg->f0()), dinv0, fd0);
         HIPCHECK(hipStreamSynchronize(h->s));
         for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[8 + 64 + l];
         sub = std::sqrt(h->hpin[0]);
@@ -420,7 +426,8 @@ static int fgmres(vfs_pp_handle* h, double bnorm, double* rel_out, int* vcyc, bo
         launch_axpy_dots(h->s, n, w, h->V, n, j + 1, h->red + 8, h->partial, &nb);
         launch_finalize(h->s, h->partial, j + 2, nb, h->red + 72, 0, h->hpin_dev + 72);
         launch_gs_norm(h->s, h->red, 72, j + 1, h->hpin_dev);
-        launch_axpy_scale<TZ>(h->s, n, w, h->V, n, j + 1, h->red + 72, h->red + 2, static_cast<TZ*>(h->mg->f0()));
+        launch_axpy_scale<TZ>(h->s, n, w, h->V, n, j + 1, h->red + 72, h->red + 2, static_cast<TZ*>(h->mg->f0()),
+                              dinv0, fd0);
         HIPCHECK(hipStreamSynchronize(h->s));
         for (int l = 0; l <= j; l++) H[(size_t)l * m + j] = h->hpin[8 + l] + h->hpin[72 + l];
         sub = h->hpin[1];
@@ -486,7 +493,7 @@ static void set_mask_core(vfs_pp_handle* h, const double* dnvert, bool keep_scra
   build_hierarchy(h, tmp);
   if (h->opts.precision == 1) h->mg = build_mg<float>(h);
   else h->mg = build_mg<double>(h);
-  h->mg->tail = h->opts.coarse_tail != 0;
+  h->mg->tail = (h->opts.fused & 1) != 0;
   h->desc = h->mg->describe();
   h->setup.clear();
   HIPCHECK(hipStreamSynchronize(s));
@@ -734,7 +741,7 @@ void vfs_pp_default_opts(vfs_pp_opts* o) {
   o->cheb_hi = 1.1;
   o->orthog = 2;
   o->verbose = 0;
-  o->coarse_tail = 1;
+  o->fused = 3;
 }
 
 const char* vfs_pp_error_string(int code) {
@@ -810,7 +817,7 @@ int vfs_pp_set_opts(vfs_pp_handle* h, const vfs_pp_opts* o) {
   if (!(o->rtol > 0.)) return VFS_PP_EINVAL;
   h->opts = *o;
   if (h->mg) {
-    h->mg->tail = o->coarse_tail != 0;
+    h->mg->tail = (o->fused & 1) != 0;
     h->desc = h->mg->describe();
   }
   return VFS_PP_OK;

@@ -109,9 +109,15 @@ inline Dmda dmda_of(const Dims& d) { return Dmda{d.nx + 2, d.ny + 2, d.nz + 2, {
 struct MGBase {
   virtual ~MGBase() = default;
   virtual void vcycle(hipStream_t s) = 0;  // f0 -> out
-  bool tail = true;  // run the bottom point level and the dense solve as one workgroup (k_coarse_tail)
+  bool tail = true;    // run the bottom point level and the dense solve as one workgroup (k_coarse_tail)
+  bool use_fd = true;  // the finest level's fused first steps read fd0() (its writers stored it)
   virtual bool line0() const = 0;           // the finest level uses the y-line smoother
   virtual void* f0() = 0;
+  // f0 * D^-1 of the finest level, stored by the writers of f0 for the fused first
+  // Chebyshev steps (nullptr when the finest level is smoothed with y-lines), and
+  // that level's D^-1
+  virtual void* fd0() = 0;
+  virtual const void* dinv0() const = 0;
   virtual void set_out(void* out) = 0;
   virtual std::string describe() const = 0;
   virtual void time_sweep(hipStream_t s) = 0;
@@ -146,12 +152,15 @@ struct MG : MGBase {
   };
   std::vector<Lev> lev;
   T* scr = nullptr;
+  T* fd = nullptr;  // finest level's f D^-1 (point smoothing only)
   T* cinv = nullptr;
   int ncoarse = 0, ncomp = 0;
   T* out = nullptr;
   int degree = 2;
 
   void* f0() override { return lev[0].f; }
+  void* fd0() override { return fd; }
+  const void* dinv0() const override { return lev.empty() ? nullptr : lev[0].dinv; }
   void set_out(void* o) override { out = static_cast<T*>(o); }
 
   // Chebyshev coefficients on [lo*lam, hi*lam]
@@ -172,7 +181,7 @@ struct MG : MGBase {
 
   // `degree` Chebyshev steps; from a zero start the point smoother fuses the
   // first two steps into one pass
-  void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out) {
+  void smooth(hipStream_t s, Lev& L, bool zero_start, T*& x, T* final_out, const T* fdl = nullptr) {
     int k = 0;
     while (k < degree) {
       int mode = SM_NEXT, adv = 1;
@@ -190,7 +199,7 @@ struct MG : MGBase {
       T* xprev = (mode == SM_NEXT) ? other : xin;
       const T c1 = (mode == SM_ZERO || last == 0) ? T(0) : L.c1[last];
       launch_smooth<T>(s, L.view(), L.lfac, mode, L.f, L.dinv, xin, xprev, xout, L.dv, scr, c1, L.c2[last],
-                       L.c2[0]);
+                       L.c2[0], mode == SM_FIRST2 ? fdl : nullptr);
       x = xout;
       k += adv;
     }
@@ -233,7 +242,7 @@ struct MG : MGBase {
       return a.xout;
     }
     T* x = L.xa;
-    smooth(s, L, true, x, nullptr);
+    smooth(s, L, true, x, nullptr, l == 0 && use_fd ? fd : nullptr);
     Lev& C = lev[l + 1];
     launch_resid_restrict<T>(s, L.view(), L.f, x, C.d, C.cls, C.f);
     T* xc = cycle(s, l + 1, nullptr);
@@ -271,6 +280,7 @@ struct MG : MGBase {
   std::unique_ptr<MGBase> clone_work(Arena& A) const override {
     auto c = std::make_unique<MG<T>>(*this);
     c->scr = A.alloc<T>(2 * (lev.empty() ? 1 : lev[0].d.n()));
+    if (fd) c->fd = A.alloc<T>(lev[0].d.n());
     for (auto& L : c->lev) {
       const size_t n = L.d.n();
       L.f = A.alloc<T>(n);

@@ -122,9 +122,14 @@ void launch_multidot(hipStream_t s, size_t n, const double* w, const double* V, 
 void launch_axpy_multi(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
                        double sign);
 void launch_norm2(hipStream_t s, size_t n, const double* a, double* partial, int* nblk);
+// The writers of the V-cycle's input f (launch_axpy_norm, launch_axpy_scale,
+// launch_scale_conv) also store fd = f * dinv when fd is given: the finest level's
+// fused first two Chebyshev steps then read f D^-1 at the 19 neighbours from one
+// array instead of forming it from two.
 template <typename TF>
 void launch_axpy_norm(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* dots,
-                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk);
+                      const double* nuinv2, const double* a2, TF* f, double* partial, int* nblk,
+                      const TF* dinv = nullptr, TF* fd = nullptr);
 double host_conv_scale(double a2);
 void launch_scale_to(hipStream_t s, size_t n, const double* a, const double* norm2, double* out);
 template <typename TZ>
@@ -150,9 +155,10 @@ void launch_axpy_dots(hipStream_t s, size_t n, double* w, const double* V, size_
 void launch_gs_norm(hipStream_t s, double* red, int off, int nv, double* host);
 template <typename TF>
 void launch_axpy_scale(hipStream_t s, size_t n, double* w, const double* V, size_t ldv, int nv, const double* h,
-                       const double* inv, TF* f);
+                       const double* inv, TF* f, const TF* dinv = nullptr, TF* fd = nullptr);
 template <typename TF>
-void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f);
+void launch_scale_conv(hipStream_t s, size_t n, const double* a, const double* nrm2, double* v, TF* f,
+                       const TF* dinv = nullptr, TF* fd = nullptr);
 
 // ---- multigrid ---------------------------------------------------------------------
 // A multigrid level as the kernels see it.  The levels of the FP32 hierarchy keep
@@ -195,10 +201,11 @@ enum { SM_NEXT = 0, SM_ZERO = 1, SM_FIRST2 = 2, SM_POWER = 3 };
 
 // Point levels keep the previous iterate in x_prev (x_new may be the same buffer:
 // each cell reads its own x_prev before writing x_new); y-line levels keep the
-// Chebyshev direction in dv and ignore x_prev.
+// Chebyshev direction in dv and ignore x_prev.  fd (optional, point levels in
+// SM_FIRST2): f * dinv as stored by the writer of f.
 template <typename T>
 void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const T* f, const T* dinv,
-                   const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0);
+                   const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0, const T* fd = nullptr);
 template <typename T>
 void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac);
 // streaming floor of a point sweep: reads every array the sweep reads once per

@@ -29,10 +29,11 @@ __global__ __launch_bounds__(256) void k_dinv(LevelView<T> L, T* __restrict__ di
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_smooth_point(LevelView<T> L, const T* __restrict__ f,
                                                       const T* __restrict__ dinv, const T* __restrict__ x_old,
-                                                      T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0) {
+                                                      T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0,
+                                                      const T* __restrict__ fd) {
   CELL_IJK(i, j, k);
   if (i >= L.d.nx || j >= L.d.ny) return;
-  smooth_point_cell<T, MODE>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k);
+  smooth_point_cell<T, MODE>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k, fd);
 }
 
 template <typename T>
@@ -530,7 +531,7 @@ inline unsigned nb1(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <typename T>
 void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const T* f, const T* dinv,
-                   const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0) {
+                   const T* x_old, T* x_prev, T* x_new, T* dv, T* scr, T c1, T c2, T c0, const T* fd) {
   if (lfac) {
     // y-line smoothing: residual over all cells, then one thread per line
     const size_t n = L.d.n();
@@ -551,17 +552,17 @@ void launch_smooth(hipStream_t s, LevelView<T> L, const T* lfac, int mode, const
   } else {
     switch (mode) {
       case SM_ZERO:
-        k_smooth_point<T, SM_ZERO><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0);
+        k_smooth_point<T, SM_ZERO><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, fd);
         break;
       case SM_FIRST2:
         k_smooth_point<T, SM_FIRST2>
-            <<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0);
+            <<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, fd);
         break;
       case SM_POWER:
-        k_smooth_point<T, SM_POWER><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0);
+        k_smooth_point<T, SM_POWER><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, fd);
         break;
       default:
-        k_smooth_point<T, SM_NEXT><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0);
+        k_smooth_point<T, SM_NEXT><<<cell_grid(L.d), kBlk, 0, s>>>(L, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, fd);
     }
   }
   HIPCHECK(hipGetLastError());
@@ -651,7 +652,7 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
 
 #define INST(T)                                                                                                     \
   template void launch_smooth<T>(hipStream_t, LevelView<T>, const T*, int, const T*, const T*, const T*, T*, T*, T*, \
-                                 T*, T, T, T);                                                                    \
+                                 T*, T, T, T, const T*);                                                          \
   template void launch_line_factors<T>(hipStream_t, LevelView<T>, T*);                                            \
   template void launch_stream_floor<T>(hipStream_t, LevelView<T>, const T*, const T*, const T*, T*);             \
   template void launch_dinv<T>(hipStream_t, LevelView<T>, T*);                                                     \

@@ -77,7 +77,8 @@ __device__ __forceinline__ void flux_by_class(const TG* __restrict__ g, size_t n
 //   SM_ZERO    the first step from x = 0:  x_new = c2 D^-1 f, x_prev = 0;
 //   SM_FIRST2  the first two steps from x = 0 in one pass: the first step's
 //              x1 = c0 D^-1 f is formed on the fly at the 19 neighbours from f and
-//              D^-1, then one step from x1 with (c1, c2); x_new = x2, x_prev = x1;
+//              D^-1 (or from fd = f D^-1 when the writer of f stored it), then one
+//              step from x1 with (c1, c2); x_new = x2, x_prev = x1;
 //   SM_POWER   dv = D^-1 (f - A x_old) only.
 // The level's class bytes pick the row form: a wave whose cells are all interior
 // takes the compile-time-weight form and reads no code word.  g holds the
@@ -88,7 +89,8 @@ template <typename T, int MODE, typename TG>
 __device__ __forceinline__ void smooth_point_body(const LevelView<T>& L, const TG* __restrict__ g, bool scaled,
                                                   const T* __restrict__ f, const T* __restrict__ dinv,
                                                   const T* __restrict__ x_old, T* x_prev, T* x_new,
-                                                  T* __restrict__ dv, T c1, T c2, T c0, int i, int j, int k) {
+                                                  T* __restrict__ dv, T c1, T c2, T c0, int i, int j, int k,
+                                                  const T* __restrict__ fd) {
   const size_t c = L.d.idx(i, j, k);
   const uint8_t cl = L.cls[c];
   const T fc = f[c];
@@ -100,8 +102,13 @@ __device__ __forceinline__ void smooth_point_body(const LevelView<T>& L, const T
   neighbours(L.d, i, j, k, nb);
   T xv[19];
   if constexpr (MODE == SM_FIRST2) {
+    if (fd) {
 #pragma unroll
-    for (int s = 0; s < 19; s++) xv[s] = c0 * (f[nb.s[s]] * dinv[nb.s[s]]);
+      for (int s = 0; s < 19; s++) xv[s] = c0 * fd[nb.s[s]];
+    } else {
+#pragma unroll
+      for (int s = 0; s < 19; s++) xv[s] = c0 * (f[nb.s[s]] * dinv[nb.s[s]]);
+    }
   } else {
     load_x(nb, x_old, xv);
   }
@@ -130,7 +137,7 @@ template <typename T, int MODE>
 __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T* __restrict__ f,
                                                   const T* __restrict__ dinv, const T* __restrict__ x_old,
                                                   T* x_prev, T* x_new, T* __restrict__ dv, T c1, T c2, T c0, int i,
-                                                  int j, int k) {
+                                                  int j, int k, const T* __restrict__ fd = nullptr) {
   if constexpr (MODE == SM_ZERO) {
     const size_t c = L.d.idx(i, j, k);
     x_new[c] = c2 * (f[c] * dinv[c]);
@@ -139,11 +146,11 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
   } else {
     if constexpr (std::is_same<T, float>::value) {
       if (L.gh) {
-        smooth_point_body<T, MODE>(L, L.gh, true, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k);
+        smooth_point_body<T, MODE>(L, L.gh, true, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k, fd);
         return;
       }
     }
-    smooth_point_body<T, MODE>(L, L.g, false, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k);
+    smooth_point_body<T, MODE>(L, L.g, false, f, dinv, x_old, x_prev, x_new, dv, c1, c2, c0, i, j, k, fd);
   }
 }
 

@@ -47,7 +47,7 @@ class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("rtol", C.c_double), ("div_tol", C.c_double), ("max_iters", C.c_int),
                 ("restart", C.c_int), ("smoother", C.c_int), ("cheb_degree", C.c_int), ("coarse_max", C.c_int),
                 ("drop_small", C.c_int), ("zero_guess", C.c_int), ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
-                ("orthog", C.c_int), ("verbose", C.c_int), ("coarse_tail", C.c_int)]
+                ("orthog", C.c_int), ("verbose", C.c_int), ("fused", C.c_int)]
 
 
 class ServerOpts(C.Structure):
