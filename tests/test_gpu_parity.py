@@ -226,6 +226,33 @@ def test_fused_launches_are_bit_identical(E, torch_cuda, name, precision):
             np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("dims", [(32, 2, 16), (24, 20, 2)])
+def test_coarse_tail_thin_axes(E, torch_cuda, dims):
+    """The coarse tail's restriction when an axis of the tail level has fewer than 3
+    cells and is not coarsened: j kept (32x2x16 -> 16x2x8, pairs in i and k) and k
+    kept (24x20x2 -> 12x10x2, the per-child sum of the general restriction), on the
+    finest level of a sheared non-periodic box, bit for bit against one launch per
+    step (point smoothing forced, so the finest level is the tail level)."""
+    torch = torch_cuda
+    c = affine_case(*dims, seed=5)
+    outs = []
+    for fused in (3, 2):
+        h = handle_for(E, c, rtol=1e-12, fused=fused, smoother=1)
+        desc = h.describe()
+        print(f"dims {dims} fused {fused}:\n{desc}", flush=True)
+        assert ("coarse tail" in desc) == (fused == 3)
+        u = torch.from_numpy(np.ascontiguousarray(c.fields["ucont"])).cuda()
+        phi = torch.zeros(c.shape, dtype=torch.float64, device="cuda")
+        p = torch.zeros_like(phi)
+        ucat = torch.zeros(c.shape + (3,), dtype=torch.float64, device="cuda")
+        st = h.project(u, phi, p, ucat, c.dt, c.st, c.coeff, strict=True)
+        outs.append((st.iterations, phi.cpu().numpy(), u.cpu().numpy()))
+        h.close()
+    assert outs[0][0] == outs[1][0]
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("degree", [1, 2, 4])
 def test_coarse_tail_chebyshev_degrees(E, torch_cuda, degree):
     """The coarse tail's step sequence for other Chebyshev degrees: one step from zero
