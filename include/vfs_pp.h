@@ -198,16 +198,22 @@ int vfs_pp_runge_kutta_device(vfs_pp_handle *h, const double *icsi, const double
                               double *ucat, double ren, double dt, double st);
 
 /* The implicit momentum step (Implicit_MatrixFree, implicitsolver.c:281-470) for
- * immersed == 0.  Its residual (FormFunction_SNES, momentum.c:2285-2440) is
+ * a body at rest.  Its residual (FormFunction_SNES, momentum.c:2285-2440) is
  *   Uz = U with the wall components zeroed (bctype 1 and -1 on every face, 2 and
  *        12 on the j faces, 10 on the i and j faces away from the boundary rows),
- *   F  = -(Uz - Uo)/dt + 0.5 Formfunction_2(Uz, Contra2Cart(Uz)) + 0.5 RHS_o - dP
+ *   Ucat = Contra2Cart(Uz), IB and solid cells keeping the ucat passed in (the
+ *        caller's IB velocity interpolation, done before the solve as the
+ *        reference does),
+ *   Ul = Uz after IB_BC (momentum.c:2037-2220): with immersed == 3 the faces
+ *        between cells whose nvert sum exceeds 1.1 zeroed, and the slip faces of
+ *        every row, ghost rows included,
+ *   F  = -(Uz - Uo)/dt + 0.5 Formfunction_2(Ul, Ucat) + 0.5 RHS_o - dP
  *        for coeff in (0.9, 1.1), otherwise
  *        -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt + Formfunction_2(...) - dP,
- * summed in the reference's order (bit-exact with a CPU restatement).
- * ucont_rm1 is needed only for coeff outside (0.9, 1.1), rhs_o only for coeff in
- * it (NULL: zero).  VFS_PP_EINVAL for immersed != 0 (the IB velocity boundary,
- * IB_BC, is the caller's) or a bctype outside {0, 1, -1, 2, 10, -10, 12}.
+ * summed in the reference's order (bit-exact with a CPU restatement), with the
+ * handle's immersed switch in Formfunction_2.  ucont_rm1 is needed only for coeff
+ * outside (0.9, 1.1), rhs_o only for coeff in it (NULL: zero).  VFS_PP_EINVAL for
+ * a bctype outside {0, 1, -1, 2, 10, -10, 12} or a slip face on a periodic axis.
  * All arrays are device DMDA vector fields on the handle's grid, stream-ordered
  * on the handle's stream; the call returns when done. */
 int vfs_pp_momentum_residual_device(vfs_pp_handle *h, const double *icsi, const double *ieta,

@@ -67,7 +67,8 @@ def lib():
         _lib.or_formfunction2.argtypes = [g, ip, C.c_int] + [d] * 18 + [C.c_double, C.c_double, d]
         _lib.or_runge_kutta.argtypes = [g, ip, C.c_int] + [d] * 19 + [C.c_double, C.c_double, C.c_double]
         _lib.or_momentum_bc_zero.argtypes = [g, ip, d]
-        _lib.or_momentum_residual.argtypes = [g, ip] + [d] * 16 + [d] * 5 + [C.c_double] * 3 + [d] * 3
+        _lib.or_momentum_ib_bc.argtypes = [g, ip, C.c_int, d, d]
+        _lib.or_momentum_residual.argtypes = [g, ip, C.c_int] + [d] * 16 + [d] * 5 + [C.c_double] * 3 + [d] * 3
         _lib.or_ray_s.restype = C.c_double
         _lib.or_ray_s.argtypes = [C.c_int]
         _lib.or_level_coupling.argtypes = [g, d, d, d, d]
@@ -447,9 +448,21 @@ def momentum_bc_zero(g, bctype, u):
     return uz
 
 
-def momentum_residual(g, bctype, f, dp, uo, u, ren, dt, coeff=1.0, urm1=None, rhs_o=None):
+def momentum_ib_bc(g, bctype, immersed, nvert, u):
+    """or_momentum_ib_bc: IB_BC (momentum.c:2037-2220) applied to a copy of the
+    Ucont field u, for a body at rest."""
+    ul = np.ascontiguousarray(u, dtype=np.float64).copy()
+    bc = np.ascontiguousarray(bctype, dtype=np.int32)
+    nv = np.ascontiguousarray(nvert, dtype=np.float64)
+    lib().or_momentum_ib_bc(C.byref(g), _p(bc, C.c_int), int(immersed), _p(nv), _p(ul))
+    return ul
+
+
+def momentum_residual(g, bctype, f, dp, uo, u, ren, dt, coeff=1.0, urm1=None, rhs_o=None, immersed=0,
+                      ucat=None):
     """or_momentum_residual: (F, Uz, Ucat) of the implicit momentum equation
-    (FormFunction_SNES, momentum.c:2285-2440) at U, immersed == 0."""
+    (FormFunction_SNES, momentum.c:2285-2440) at U; ucat (default zero) holds the
+    Ucat that IB and solid cells keep."""
     a = [np.ascontiguousarray(f[k], dtype=np.float64) for k in MOMENTUM_METRICS]
     u = np.ascontiguousarray(u, dtype=np.float64)
     z = np.zeros(u.shape)
@@ -457,8 +470,10 @@ def momentum_residual(g, bctype, f, dp, uo, u, ren, dt, coeff=1.0, urm1=None, rh
     urm1 = z if urm1 is None else np.ascontiguousarray(urm1, dtype=np.float64)
     rhs_o = z if rhs_o is None else np.ascontiguousarray(rhs_o, dtype=np.float64)
     dp = np.ascontiguousarray(dp, dtype=np.float64)
-    uz, ucat, out = np.zeros(u.shape), np.zeros(u.shape), np.zeros(u.shape)
+    uz, out = np.zeros(u.shape), np.zeros(u.shape)
+    ucat = np.zeros(u.shape) if ucat is None else np.ascontiguousarray(ucat, dtype=np.float64).copy()
     bc = np.ascontiguousarray(bctype, dtype=np.int32)
-    lib().or_momentum_residual(C.byref(g), _p(bc, C.c_int), *[_p(x) for x in a], _p(dp), _p(uo), _p(urm1),
-                               _p(rhs_o), _p(u), float(ren), float(dt), float(coeff), _p(uz), _p(ucat), _p(out))
+    lib().or_momentum_residual(C.byref(g), _p(bc, C.c_int), int(immersed), *[_p(x) for x in a], _p(dp), _p(uo),
+                               _p(urm1), _p(rhs_o), _p(u), float(ren), float(dt), float(coeff), _p(uz), _p(ucat),
+                               _p(out))
     return out, uz, ucat

@@ -188,13 +188,15 @@ void or_formfunction2(const or_grid *g, const int bctype[6], int immersed, const
 /* RungeKutta (rhs.c:1511-1631): four stages Ucont = pUcont + alfa dt st
  * (Formfunction_2 - dP), each between Contra2Cart calls; ucont and ucat in/out. */
 void or_momentum_bc_zero(const or_grid *g, const int bc[6], double *u);
-void or_momentum_residual(const or_grid *g, const int bctype[6], const double *icsi, const double *ieta,
-                          const double *izet, const double *jcsi, const double *jeta, const double *jzet,
-                          const double *kcsi, const double *keta, const double *kzet, const double *iaj,
-                          const double *jaj, const double *kaj, const double *csi, const double *eta,
-                          const double *zet, const double *nvert, const double *dp, const double *uo,
-                          const double *urm1, const double *rhs_o, const double *u, double ren, double dt,
-                          double coeff, double *uz, double *ucat, double *f);
+void or_momentum_ib_bc(const or_grid *g, const int bc[6], int immersed, const double *nvert, double *u);
+/* ucat in/out: IB and solid cells keep the values passed in */
+void or_momentum_residual(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
+                          const double *ieta, const double *izet, const double *jcsi, const double *jeta,
+                          const double *jzet, const double *kcsi, const double *keta, const double *kzet,
+                          const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                          const double *eta, const double *zet, const double *nvert, const double *dp,
+                          const double *uo, const double *urm1, const double *rhs_o, const double *u, double ren,
+                          double dt, double coeff, double *uz, double *ucat, double *f);
 void or_runge_kutta(const or_grid *g, const int bctype[6], int immersed, const double *icsi, const double *ieta,
                     const double *izet, const double *jcsi, const double *jeta, const double *jzet,
                     const double *kcsi, const double *keta, const double *kzet, const double *iaj,

@@ -311,16 +311,54 @@ void or_runge_kutta(const or_grid *g, const int bctype[6], int immersed, const d
 /* ------------------------------------------------------- implicit momentum
  * The nonlinear residual the reference's matrix-free SNES drives to zero
  * (FormFunction_SNES, momentum.c:2285-2440, called by Implicit_MatrixFree,
- * implicitsolver.c:281), for immersed == 0 (no IB_BC), no wall model, forcing or
- * level set:
+ * implicitsolver.c:281), for a body at rest (movefsi = rotatefsi = 0), no wall
+ * model, forcing or level set:
  *   Uz  = U with the wall components zeroed (momentum.c:2323-2356: bctype 1 and
  *         -1 on every face, cavity 2 and couette 12 on j, slip 10 on i and j),
- *   Ucat = Contra2Cart(Uz),
+ *   Ucat = Contra2Cart(Uz) (IB and solid cells keep the Ucat passed in),
  *   F   = -(Uz - Uo)/dt (coeff 1) or -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt (otherwise),
  *         summed as the VecAXPY sequence does,
- *   F  += Formfunction_2(Uz, Ucat) scaled by 0.5 (coeff 1; plus 0.5 RHS_o) or 1,
+ *   Ul  = IB_BC applied to Uz (or_momentum_ib_bc; the local Ucont that
+ *         Formfunction_2 reads, while the time terms use the global Uz),
+ *   F  += Formfunction_2(Ul, Ucat) scaled by 0.5 (coeff 1; plus 0.5 RHS_o) or 1,
  *   F  += -1 dP.
  * uz and ucat receive Uz and its Ucat. */
+
+/* IB_BC (momentum.c:2037-2220) on a Ucont vector field, for a body at rest:
+ * with immersed == 3, the faces between interior cells whose nvert sum exceeds
+ * 1.1 are zeroed (momentum.c:2158-2162); the slip (bctype 10) faces next to the
+ * first and last interior cells are zeroed on every row, ghost rows included
+ * (momentum.c:2190-2196); the periodic ghosts of each axis's normal component
+ * copy their images (momentum.c:2198-2213), in the reference's point order.  Its
+ * periodic Ucat copy (momentum.c:2075-2103) repeats what Contra2Cart leaves. */
+void or_momentum_ib_bc(const or_grid *g, const int bc[6], int immersed, const double *nvert, double *u) {
+  const int mx = g->mx, my = g->my, mz = g->mz;
+  if (immersed == 3)
+    for (int k = 1; k < mz - 1; k++)
+      for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+          const size_t p = IXM(g, i, j, k);
+          if (nvert[IXM(g, i + 1, j, k)] + nvert[p] > 1.1) u[3 * p] = 0;
+          if (nvert[IXM(g, i, j + 1, k)] + nvert[p] > 1.1) u[3 * p + 1] = 0;
+          if (nvert[IXM(g, i, j, k + 1)] + nvert[p] > 1.1) u[3 * p + 2] = 0;
+        }
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        if (bc[0] == 10 && i == 1) u[3 * IXM(g, 0, j, k)] = 0;
+        if (bc[1] == 10 && i == mx - 2) u[3 * IXM(g, mx - 2, j, k)] = 0;
+        if (bc[2] == 10 && j == 1) u[3 * IXM(g, i, 0, k) + 1] = 0;
+        if (bc[3] == 10 && j == my - 2) u[3 * IXM(g, i, my - 2, k) + 1] = 0;
+        if (bc[4] == 10 && k == 1) u[3 * IXM(g, i, j, 0) + 2] = 0;
+        if (bc[5] == 10 && k == mz - 2) u[3 * IXM(g, i, j, mz - 2) + 2] = 0;
+        if (g->per[0] && i == 0) u[3 * IXM(g, 0, j, k)] = u[3 * IXM(g, mx - 2, j, k)];
+        if (g->per[0] && i == mx - 1) u[3 * IXM(g, mx - 1, j, k)] = u[3 * IXM(g, 1, j, k)];
+        if (g->per[1] && j == 0) u[3 * IXM(g, i, 0, k) + 1] = u[3 * IXM(g, i, my - 2, k) + 1];
+        if (g->per[1] && j == my - 1) u[3 * IXM(g, i, my - 1, k) + 1] = u[3 * IXM(g, i, 1, k) + 1];
+        if (g->per[2] && k == 0) u[3 * IXM(g, i, j, 0) + 2] = u[3 * IXM(g, i, j, mz - 2) + 2];
+        if (g->per[2] && k == mz - 1) u[3 * IXM(g, i, j, mz - 1) + 2] = u[3 * IXM(g, i, j, 1) + 2];
+      }
+}
 void or_momentum_bc_zero(const or_grid *g, const int bc[6], double *u) {
   const int mx = g->mx, my = g->my, mz = g->mz;
   for (int k = 0; k < mz; k++)
@@ -341,13 +379,13 @@ void or_momentum_bc_zero(const or_grid *g, const int bc[6], double *u) {
       }
 }
 
-void or_momentum_residual(const or_grid *g, const int bctype[6], const double *icsi, const double *ieta,
-                          const double *izet, const double *jcsi, const double *jeta, const double *jzet,
-                          const double *kcsi, const double *keta, const double *kzet, const double *iaj,
-                          const double *jaj, const double *kaj, const double *csi, const double *eta,
-                          const double *zet, const double *nvert, const double *dp, const double *uo,
-                          const double *urm1, const double *rhs_o, const double *u, double ren, double dt,
-                          double coeff, double *uz, double *ucat, double *f) {
+void or_momentum_residual(const or_grid *g, const int bctype[6], int immersed, const double *icsi,
+                          const double *ieta, const double *izet, const double *jcsi, const double *jeta,
+                          const double *jzet, const double *kcsi, const double *keta, const double *kzet,
+                          const double *iaj, const double *jaj, const double *kaj, const double *csi,
+                          const double *eta, const double *zet, const double *nvert, const double *dp,
+                          const double *uo, const double *urm1, const double *rhs_o, const double *u, double ren,
+                          double dt, double coeff, double *uz, double *ucat, double *f) {
   const size_t n3 = 3 * (size_t)g->mx * g->my * g->mz;
   memcpy(uz, u, n3 * sizeof(double));
   or_momentum_bc_zero(g, bctype, uz);
@@ -365,8 +403,12 @@ void or_momentum_residual(const or_grid *g, const int bctype[6], const double *i
     }
     f[q] = r;
   }
-  mm_formfunction2(g, bctype, 0, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, csi, eta,
-                   zet, nvert, uz, ucat, ren, bdf1 ? 0.5 : 1.0, f, 1);
+  double *ul = (double *)malloc(n3 * sizeof(double));
+  memcpy(ul, uz, n3 * sizeof(double));
+  or_momentum_ib_bc(g, bctype, immersed, nvert, ul);
+  mm_formfunction2(g, bctype, immersed, icsi, ieta, izet, jcsi, jeta, jzet, kcsi, keta, kzet, iaj, jaj, kaj, csi,
+                   eta, zet, nvert, ul, ucat, ren, bdf1 ? 0.5 : 1.0, f, 1);
+  free(ul);
   if (bdf1)
     for (size_t q = 0; q < n3; q++) f[q] = f[q] + 0.5 * rhs_o[q];
   for (size_t q = 0; q < n3; q++) f[q] = f[q] + -1. * dp[q];

@@ -278,18 +278,100 @@ def test_implicit_step_meets_the_reference_test(E, torch_cuda, coeff):
     np.testing.assert_allclose(ucd.cpu().numpy(), ucat_ref, rtol=0, atol=1e-14 * np.abs(ucat_ref).max())
 
 
+def _ib_inputs(c, seed):
+    """uo, urm1, rhs_o, dp and a Ucat whose IB and solid cells hold arbitrary
+    values (the caller's IB interpolation), for the residual at c's Ucont."""
+    g = O.grid(c.shape, c.per)
+    rng = np.random.default_rng(seed)
+    u = np.ascontiguousarray(c.fields["ucont"])
+    uo = u * (1.0 + 0.1 * rng.standard_normal(u.shape))
+    urm1 = u * (1.0 + 0.1 * rng.standard_normal(u.shape))
+    rhs_o = 0.01 * rng.standard_normal(u.shape)
+    dp, _ = O.pressure_gradient(g, c.fields, _pressure(c))
+    ucat = 0.2 * rng.standard_normal(u.shape)
+    return g, u, uo, urm1, rhs_o, dp, ucat
+
+
+@pytest.mark.parametrize("coeff", [1.0, 1.5])
+@pytest.mark.parametrize("immersed", [1, 3])
+@pytest.mark.parametrize("name", ["affine_ib", "hill_sphere"])
+def test_momentum_residual_with_a_body_bit_exact(E, torch_cuda, name, immersed, coeff):
+    """The residual with an immersed body: IB-aware Formfunction_2, IB_BC (zeroed
+    faces between IB and solid cells for immersed 3), the IB cells' Ucat kept as
+    passed and the solid cells' zeroed, bit for bit against the oracle; immersed 3
+    differs from 1 (the zeroed faces reach the residual)."""
+    torch = torch_cuda
+    c = dataclasses.replace(CASES[name](), immersed=immersed)
+    print(f"momentum residual {name} immersed {immersed} coeff {coeff}", flush=True)
+    g, u, uo, urm1, rhs_o, dp, ucat0 = _ib_inputs(c, 17)
+    ren, dt = 300.0, 0.01
+    F_ref, _, ucat_ref = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u, ren, dt, coeff=coeff, urm1=urm1,
+                                             rhs_o=rhs_o, immersed=immersed, ucat=ucat0)
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, immersed))
+    h.set_geometry(c.fields)
+    met = {k: dev(torch, c.fields[k]) for k in METRICS}
+    F, ucat = E.momentum_residual(h, met, dev(torch, dp), dev(torch, uo), dev(torch, u), ren, dt, coeff,
+                                  ucont_rm1=dev(torch, urm1), rhs_o=dev(torch, rhs_o), ucat=dev(torch, ucat0))
+    h.close()
+    np.testing.assert_array_equal(F.cpu().numpy(), F_ref)
+    np.testing.assert_array_equal(ucat.cpu().numpy(), ucat_ref)
+    nv = c.fields["nvert"]
+    ibm, solid = (nv > 0.1) & (nv < 2.9), nv > 2.9
+    assert ibm.any()
+    np.testing.assert_array_equal(ucat_ref[ibm], ucat0[ibm])  # IB cells keep their Ucat
+    assert (ucat_ref[solid] == 0).all()  # solid cells are zeroed (rhs.c:257)
+    if immersed == 3:
+        F1, _, _ = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u, ren, dt, coeff=coeff, urm1=urm1,
+                                       rhs_o=rhs_o, immersed=1, ucat=ucat0)
+        assert not np.array_equal(F1, F_ref)
+
+
+def test_implicit_step_with_a_body_meets_the_reference_test(E, torch_cuda):
+    """The implicit step on the hill with an IB sphere (immersed 1, the IB cells'
+    Ucat as the caller's interpolation left it): the device's Ucont satisfies
+    ||F|| <= 1e-4 ||F(U0)|| with F evaluated by the oracle."""
+    torch = torch_cuda
+    c = CASES["hill_sphere"]()
+    g, uo, _, _, _, dp, _ = _ib_inputs(c, 23)
+    rng = np.random.default_rng(29)
+    ucat0 = ucat_of(c, uo)
+    ren, dt = 400.0, 0.02
+    rhs_o = O.formfunction2(g, c.bctype, c.immersed, c.fields, uo, ucat0, ren)
+    u0 = uo * (1.0 + 0.05 * rng.standard_normal(uo.shape))
+    F0, _, _ = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u0, ren, dt, rhs_o=rhs_o, immersed=c.immersed,
+                                   ucat=ucat0)
+    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, c.immersed))
+    h.set_geometry(c.fields)
+    met = {k: dev(torch, c.fields[k]) for k in METRICS}
+    ud, ucd = dev(torch, u0), dev(torch, ucat0)
+    st = E.Implicit_MatrixFree(h, met, dev(torch, dp), dev(torch, uo), ud, ucd, ren, dt, 1.0,
+                               rhs_o=dev(torch, rhs_o), rtol=1e-4)
+    h.close()
+    print(f"hill + sphere: {st.newton_iterations} Newton, {st.krylov_iterations} GMRES, "
+          f"|F| {st.fnorm0:.3e} -> {st.fnorm:.3e}", flush=True)
+    u = ud.cpu().numpy()
+    F1, _, _ = O.momentum_residual(g, c.bctype, c.fields, dp, uo, u, ren, dt, rhs_o=rhs_o, immersed=c.immersed,
+                                   ucat=ucat0)
+    assert st.converged
+    assert abs(st.fnorm0 - np.linalg.norm(F0)) <= 1e-12 * np.linalg.norm(F0)
+    assert np.linalg.norm(F1) <= 1e-4 * np.linalg.norm(F0) * (1 + 1e-6)
+
+
 def test_implicit_form_refusals(E, torch_cuda):
-    """immersed != 0 (the IB velocity boundary is the caller's), a bctype outside
-    the wall rules, or coeff 1.5 without the older field: VFS_PP_EINVAL."""
+    """A bctype outside the wall rules, a slip face on a periodic axis, or coeff 1.5
+    without the older field: VFS_PP_EINVAL."""
     torch = torch_cuda
     c = CASES["affine_ib"]()
     met = {k: dev(torch, c.fields[k]) for k in METRICS}
     u = dev(torch, c.fields["ucont"])
     z = torch.zeros_like(u)
-    h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, c.bctype, 1))
-    h.set_geometry(c.fields)
+    cp = _periodic_affine()
+    metp = {k: dev(torch, cp.fields[k]) for k in METRICS}
+    up = dev(torch, cp.fields["ucont"])
+    h = E.Handle(E.make_grid(cp.nx, cp.ny, cp.nz, cp.per, (10, 10, 0, 0, 0, 0), 0))
+    h.set_geometry(cp.fields)
     with pytest.raises(E.VfsPPError) as ei:
-        E.momentum_residual(h, met, z, u, u, 100.0, 0.01)
+        E.momentum_residual(h, metp, torch.zeros_like(up), up, up, 100.0, 0.01)
     assert ei.value.code == E.VFS_PP_EINVAL
     h.close()
     h = E.Handle(E.make_grid(c.nx, c.ny, c.nz, c.per, (4, 1, 1, 1, 1, 1), 0))

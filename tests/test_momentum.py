@@ -240,3 +240,53 @@ def test_momentum_wall_zeroing_hand_built(bc, comp, where):
         for j in (0, my - 2):
             want[:, j, :, 1][rows[:, j, :]] = 0
     np.testing.assert_array_equal(uz, want)
+
+
+def _ib_bc_grid(per=(0, 0, 0)):
+    """6x6x6 DMDA (4^3 interior), an IB cell (nvert 1) at (i, j, k) = (2, 2, 2)
+    and a solid cell (nvert 3) above it at (2, 2, 3); a random Ucont."""
+    g = O.grid((6, 6, 6), per)
+    nv = np.zeros((6, 6, 6))
+    nv[2, 2, 2] = 1.0
+    nv[3, 2, 2] = 3.0
+    u = np.random.default_rng(5).standard_normal((6, 6, 6, 3))
+    return g, nv, u
+
+
+def test_ib_bc_zeroes_the_faces_of_the_solid_cell():
+    """IB_BC (momentum.c:2158-2162) with immersed 3 zeroes exactly the six faces of
+    the solid cell, the one it shares with the IB cell included; the IB cell's faces
+    to fluid cells (nvert sum 1) stay.  With immersed 1 nothing changes."""
+    g, nv, u = _ib_bc_grid()
+    ul = O.momentum_ib_bc(g, (0,) * 6, 3, nv, u)
+    # (component, k, j, i) of the zeroed faces: a face belongs to the cell on its low side
+    zeroed = {(0, 3, 2, 1), (0, 3, 2, 2), (1, 3, 1, 2), (1, 3, 2, 2), (2, 2, 2, 2), (2, 3, 2, 2)}
+    diff = {(c, k, j, i) for k, j, i, c in zip(*np.nonzero(ul != u))}
+    assert diff == zeroed
+    for c, k, j, i in zeroed:
+        assert ul[k, j, i, c] == 0.0
+    np.testing.assert_array_equal(O.momentum_ib_bc(g, (0,) * 6, 1, nv, u), u)
+
+
+def test_ib_bc_slip_faces_on_every_row():
+    """A slip (bctype 10) i-min wall: IB_BC zeroes the first face of every row of
+    the DMDA, ghost rows and planes included (momentum.c:2190), and nothing else;
+    the wall zeroing before it leaves the ghost rows."""
+    g, nv, u = _ib_bc_grid()
+    ul = O.momentum_ib_bc(g, (10, 0, 0, 0, 0, 0), 0, nv, u)
+    exp = u.copy()
+    exp[:, :, 0, 0] = 0.0
+    np.testing.assert_array_equal(ul, exp)
+    uz = O.momentum_bc_zero(g, (10, 0, 0, 0, 0, 0), u)
+    assert (uz[1:-1, 1:-1, 0, 0] == 0).all() and (uz[0, :, 0, 0] == u[0, :, 0, 0]).all()
+
+
+def test_ib_bc_periodic_normal_ghosts():
+    """On a periodic i axis IB_BC copies the normal component's ghost faces from
+    their images (momentum.c:2198-2199) and leaves the other components."""
+    g, nv, u = _ib_bc_grid(per=(1, 0, 0))
+    ul = O.momentum_ib_bc(g, (0,) * 6, 0, nv, u)
+    exp = u.copy()
+    exp[:, :, 0, 0] = u[:, :, 4, 0]
+    exp[:, :, 5, 0] = u[:, :, 1, 0]
+    np.testing.assert_array_equal(ul, exp)

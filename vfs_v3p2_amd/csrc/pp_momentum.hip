@@ -344,6 +344,31 @@ __global__ __launch_bounds__(256) void k_mm_bc_zero(Dmda m, Bc6 bc, const double
   uz[3 * p + 2] = zz ? 0. : u[3 * p + 2];
 }
 
+// IB_BC (momentum.c:2037-2220) on the Ucont that Formfunction_2 reads, for a body
+// at rest: with immersed == 3 the faces between interior cells whose nvert sum
+// exceeds 1.1 are zeroed (momentum.c:2158-2162), and the slip (bctype 10) faces
+// next to the first and last interior cells on every row, ghost rows included
+// (momentum.c:2190-2196).  The normal components' periodic ghost copies
+// (momentum.c:2198-2213) are left out: Formfunction_2 reads Ucont through its own
+// whole-vector periodic ghosts, which overwrite them.  A slip face on a periodic
+// axis (where the reference's point order would matter) is refused by the caller.
+__global__ __launch_bounds__(256) void k_mm_ib_bc(Dmda m, Bc6 bc, int immersed, const double* __restrict__ nv,
+                                                  const double* __restrict__ u, double* __restrict__ ul) {
+  int i, j, k;
+  if (!dmda_point(m, i, j, k)) return;
+  const size_t p = m.idx(i, j, k);
+  const size_t sy = (size_t)m.mx, sz = (size_t)m.mx * m.my;
+  const int* b = bc.b;
+  const bool in = i > 0 && i < m.mx - 1 && j > 0 && j < m.my - 1 && k > 0 && k < m.mz - 1;
+  const bool ib = immersed == 3 && in;
+  const bool zx = (ib && nv[p + 1] + nv[p] > 1.1) || (b[0] == 10 && i == 0) || (b[1] == 10 && i == m.mx - 2);
+  const bool zy = (ib && nv[p + sy] + nv[p] > 1.1) || (b[2] == 10 && j == 0) || (b[3] == 10 && j == m.my - 2);
+  const bool zz = (ib && nv[p + sz] + nv[p] > 1.1) || (b[4] == 10 && k == 0) || (b[5] == 10 && k == m.mz - 2);
+  ul[3 * p] = zx ? 0. : u[3 * p];
+  ul[3 * p + 1] = zy ? 0. : u[3 * p + 1];
+  ul[3 * p + 2] = zz ? 0. : u[3 * p + 2];
+}
+
 // the time terms, summed as the VecAXPY sequence from a zero vector does:
 // -(Uz - Uo)/dt (coeff 1) or -(1.5 Uz - 2 Uo + 0.5 Urm1)/dt
 __global__ __launch_bounds__(256) void k_mm_time(size_t n3, const double* __restrict__ uz,
@@ -464,7 +489,8 @@ struct ImplicitMomentum {
   double ren, dt, coeff;
   size_t n3;
   Arena ar;
-  double *ws = nullptr, *uz = nullptr, *partial = nullptr, *red = nullptr;
+  double *ws = nullptr, *uz = nullptr, *ul = nullptr, *partial = nullptr, *red = nullptr;
+  bool ib_bc = false;  // IB_BC changes the Ucont Formfunction_2 reads (immersed 3 or slip faces)
   double* hpin = nullptr;  // pinned host mirror of red
   double* hpin_dev = nullptr;
   int evals = 0;
@@ -475,6 +501,9 @@ struct ImplicitMomentum {
     n3 = 3 * m.n();
     ws = ar.alloc<double>(8 * n3);
     uz = ar.alloc<double>(n3);
+    for (int f = 0; f < 6; f++) ib_bc = ib_bc || bc.b[f] == 10;
+    ib_bc = ib_bc || in.immersed == 3;
+    ul = ib_bc ? ar.alloc<double>(n3) : uz;
     partial = ar.alloc<double>((size_t)64 * 1024);
     red = ar.alloc<double>(128);
     HIPCHECK(hipHostMalloc(&hpin, 128 * sizeof(double)));
@@ -494,7 +523,11 @@ struct ImplicitMomentum {
     const bool bdf1 = coeff > 0.9 && coeff < 1.1;
     k_mm_time<<<nb(), 256, 0, s>>>(n3, uz, uo, urm1 ? urm1 : uo, bdf1 ? 1 : 0, dt, f);
     HIPCHECK(hipGetLastError());
-    formfunction2(s, m, in, uz, ucat, ren, bdf1 ? 0.5 : 1.0, ws, f, true);
+    if (ib_bc) {
+      k_mm_ib_bc<<<grd, blk, 0, s>>>(m, bc, in.immersed, in.nv, uz, ul);
+      HIPCHECK(hipGetLastError());
+    }
+    formfunction2(s, m, in, ul, ucat, ren, bdf1 ? 0.5 : 1.0, ws, f, true);
     k_mm_tail<<<nb(), 256, 0, s>>>(n3, bdf1 ? rhs_o : nullptr, dp, f);
     HIPCHECK(hipGetLastError());
     evals++;
@@ -662,12 +695,12 @@ void implicit_setup(vfs_pp_handle* h, const double* const met[15], const double*
                     const double* urm1, double ren, double dt, double coeff, ImplicitMomentum& P) {
   ensure_unbound(h);
   ensure_geom(h);
-  if (h->immersed_ref != 0)
-    throw StatusError(VFS_PP_EINVAL, "the implicit momentum form takes immersed == 0 (no IB velocity boundary)");
   for (int f = 0; f < 6; f++) {
     const int b = h->grid.bctype[f];
     if (!(b == 0 || b == 1 || b == -1 || b == 2 || b == 10 || b == -10 || b == 12))
       throw StatusError(VFS_PP_EINVAL, "unsupported bctype for the implicit momentum form");
+    if ((b == 10 || b == -10) && h->grid.periodic[f / 2])
+      throw StatusError(VFS_PP_EINVAL, "a slip boundary on a periodic axis");
   }
   if (!(coeff > 0.9 && coeff < 1.1) && !urm1)
     throw StatusError(VFS_PP_EINVAL, "coeff 1.5 needs Ucont of the step before the previous one");
@@ -675,7 +708,7 @@ void implicit_setup(vfs_pp_handle* h, const double* const met[15], const double*
   P.m = h->m;
   P.d = h->d;
   vfs_grid gr = h->grid;
-  gr.immersed = 0;
+  gr.immersed = h->immersed_ref;
   P.in = make_in(&gr, met, h->nvert);
   P.h = h;
   for (int f = 0; f < 6; f++) P.bc.b[f] = h->grid.bctype[f];

@@ -740,15 +740,17 @@ def _opt_ptr(t, shape, what):
 
 
 def momentum_residual(handle: "Handle", metrics: dict, dp, ucont_o, ucont, ren: float, dt: float, coeff: float = 1.0,
-                      ucont_rm1=None, rhs_o=None):
+                      ucont_rm1=None, rhs_o=None, ucat=None):
     """FormFunction_SNES (momentum.c:2285) on the device: (F, Ucat) of the implicit
-    momentum equation at Ucont, immersed == 0 (new device tensors)."""
+    momentum equation at Ucont (new device tensors).  ucat (default zero; copied)
+    holds the Ucat that IB and solid cells keep."""
     import torch
 
     g = handle.grid
     V = dmda_shape(g, vector=True)
     F = torch.zeros(V, dtype=torch.float64, device=ucont.device)
-    ucat = torch.zeros(V, dtype=torch.float64, device=ucont.device)
+    ucat = (torch.zeros(V, dtype=torch.float64, device=ucont.device) if ucat is None
+            else ucat.detach().clone().contiguous())
     _check(lib().vfs_pp_momentum_residual_device(
         handle._h, *_momentum_ptrs(g, metrics), _tptr(dp, V, "dp"), _tptr(ucont_o, V, "ucont_o"),
         _opt_ptr(ucont_rm1, V, "ucont_rm1"), _opt_ptr(rhs_o, V, "rhs_o"), _tptr(ucont, V, "ucont"), float(ren),
