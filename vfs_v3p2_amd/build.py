@@ -2,7 +2,9 @@
 
 Each .hip translation unit is compiled in parallel to an object under
 vfs_v3p2_amd/_build/, then linked into vfs_v3p2_amd/libvfs_pp.so.  Every HIP
-unit is compiled with packed FP32 instructions disabled.
+unit is compiled with packed FP32 instructions disabled.  The C++ host side
+(host/vfs_userctx.cpp) is linked against it into host/libvfs_host.so, and the
+one-block pressure-step driver host/pressure_step against both.
 """
 from __future__ import annotations
 
@@ -58,9 +60,33 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_host()
     if verbose:
         print(f"built {OUT}")
     return OUT
+
+
+HOST = os.path.join(HERE, "host")
+
+
+def build_host() -> None:
+    """libvfs_host.so (the C++ UserCtx and per-step calls over the C ABI) and the
+    pressure_step driver; host code only, linked against libvfs_pp.so."""
+    base = [hipcc(), "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops", "-Wall", f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST}"]
+    lib = os.path.join(HOST, "libvfs_host.so")
+    steps = [
+        base + ["-fPIC", "-shared", os.path.join(HOST, "vfs_userctx.cpp"), f"-L{HERE}", "-lvfs_pp",
+                "-Wl,-rpath,$ORIGIN/..", "-o", lib + ".tmp"],
+        base + [os.path.join(HOST, "pressure_step.cpp"), f"-L{HOST}", "-lvfs_host", f"-L{HERE}", "-lvfs_pp",
+                "-Wl,-rpath,$ORIGIN:$ORIGIN/..", "-o", os.path.join(HOST, "pressure_step")],
+    ]
+    for i, cmd in enumerate(steps):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed:\n{r.stdout}\n{r.stderr}")
+        if i == 0:
+            os.replace(lib + ".tmp", lib)
 
 
 if __name__ == "__main__":
