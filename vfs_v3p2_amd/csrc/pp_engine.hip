@@ -303,6 +303,20 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   mg->ncomp = ncomp;
   mg->cinv = A.alloc<T>(cit.size());
   h2d(h->s, mg->cinv, cit.data(), cit.size());
+  // the coarse tail's thread-to-cell order, from the codes of the level above the
+  // dense one (FP32 point levels that fit one workgroup)
+  std::vector<uint16_t> ord;
+  if (std::is_same<T, float>::value && h->setup.size() >= 2) {
+    const SetupLevel& Ht = h->setup[h->setup.size() - 2];
+    if (!Ht.line && Ht.d.n() <= (size_t)TAIL_MAX_CELLS) {
+      std::vector<uint32_t> tcode(Ht.d.n());
+      d2h(h->s, tcode.data(), Ht.code, tcode.size());
+      HIPCHECK(hipStreamSynchronize(h->s));
+      ord = tail_order(tcode.size(), tcode.data());
+      mg->tord = A.alloc<uint16_t>(ord.size());
+      h2d(h->s, mg->tord, ord.data(), ord.size());
+    }
+  }
   HIPCHECK(hipStreamSynchronize(h->s));
   return mg;
 }

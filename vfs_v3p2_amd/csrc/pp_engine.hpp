@@ -154,6 +154,7 @@ struct MG : MGBase {
   T* scr = nullptr;
   T* fd = nullptr;  // finest level's f D^-1 (point smoothing only)
   T* cinv = nullptr;
+  uint16_t* tord = nullptr;  // thread-to-cell order of the coarse tail's level (tail_order)
   int ncoarse = 0, ncomp = 0;
   T* out = nullptr;
   int degree = 2;
@@ -232,6 +233,7 @@ struct MG : MGBase {
       a.dc = lev[l + 1].d;
       a.clsc = lev[l + 1].cls;
       a.cinv = cinv;
+      a.order = tord;
       a.degree = degree;
       a.first2 = degree >= 2 ? 1 : 0;
       for (int k = 0; k < degree; k++) {

@@ -2,6 +2,8 @@
 // stream passed in; nothing here synchronises.
 #pragma once
 
+#include <vector>
+
 #include "pp_common.hpp"
 
 namespace pp {
@@ -231,6 +233,13 @@ void launch_dense_solve(hipStream_t s, int n, const T* cinv, const T* f, T* x);
 // for bit, as the separate launches.  FP32 levels only.
 constexpr int TAIL_MAX_CELLS = 1024;
 constexpr int TAIL_MAX_DEGREE = 8;
+constexpr uint16_t TAIL_NO_CELL = 0xFFFF;
+// Which cell each of the tail's 1024 threads smooths: cells whose code word is the
+// interior one first (waves made only of them take the compile-time-weight flux),
+// then the others, and the resulting 64-cell waves placed on the workgroup's waves
+// (wave w runs on SIMD w % 4) so that every SIMD carries about the same
+// arithmetic.  code holds the level's n code words; the result has 1024 entries.
+std::vector<uint16_t> tail_order(size_t n, const uint32_t* code);
 template <typename T>
 struct TailArgs {
   LevelView<T> L;
@@ -240,6 +249,7 @@ struct TailArgs {
   Dims dc;              // the dense level
   const uint8_t* clsc;  // its class bytes
   const T* cinv;        // its inverse, column-major dc.n() x dc.n()
+  const uint16_t* order;  // the cell thread t smooths (TAIL_NO_CELL: none); null: cell t
   int degree;
   int first2;
   T c1[TAIL_MAX_DEGREE], c2[TAIL_MAX_DEGREE];

@@ -9,6 +9,8 @@
 // integrated flux form as the fine one, so coarse residuals add up), zero on
 // coarse IB cells; prolongation uses the (27,9,9,3,9,3,3,1)/64 weights, one-sided
 // next to coarse IB cells and at non-periodic walls, periodic axes wrap.
+#include <algorithm>
+
 #include "pp_mg_dev.hpp"
 
 namespace pp {
@@ -140,10 +142,11 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const T* __restrict_
 // The bottom of a V-cycle in one workgroup: the last smoothed level (point
 // Chebyshev, at most 1024 cells, one per thread) with its iterates in LDS, the
 // residual restricted to the dense level, the dense solve, the prolongation and
-// the post-smoothing.  Each thread loads its cell's stencil indices, code word,
-// class byte, f, D^-1 and its 18 face coefficients once into registers, so the
-// phases between barriers touch only LDS (and the dense inverse), instead of one
-// launch with cold caches per phase.  Every step is the arithmetic of the
+// the post-smoothing.  The launch loads the level's face coefficients, code words,
+// class bytes, f and f D^-1 into LDS once, so the phases between barriers touch
+// only LDS, instead of one launch with cold caches per phase.  Thread t smooths
+// cell order[t] (tail_order): interior cells grouped into waves of their own, the
+// waves spread evenly over the SIMDs.  Every step is the arithmetic of the
 // separate kernels (smooth_point_body, resid_restrict_blk / resid_restrict_cell,
 // dense_rows, prolong_add_cell) in the same order, so the result is the same bits.
 // dense inverses up to this many entries (64 KB in FP32) are staged in LDS
@@ -154,6 +157,9 @@ struct TailLds {
   TG g[9][TAIL_MAX_CELLS];                                        // the level's face coefficients
   T xa[TAIL_MAX_CELLS], xb[TAIL_MAX_CELLS], fd[TAIL_MAX_CELLS];  // iterates; f D^-1, then the residual
   T fc[TAIL_MAX_CELLS], xc[TAIL_MAX_CELLS];                        // dense level: right-hand side and solution
+  T fv[TAIL_MAX_CELLS];                                            // the level's f
+  uint32_t code[TAIL_MAX_CELLS];                                   // its code words
+  uint8_t cls[TAIL_MAX_CELLS];                                     // its class bytes
   T part[4][4][64];                                               // dense_rows partial sums, 4 row blocks
   T c1[TAIL_MAX_DEGREE], c2[TAIL_MAX_DEGREE];                     // Chebyshev coefficients per step
   uint8_t clsc[TAIL_MAX_CELLS];                                   // dense level's class bytes
@@ -218,33 +224,66 @@ __device__ __forceinline__ void coarse_tail_body(const TailArgs<T>& A, const TG*
   const Dims& d = L.d;
   const int n = (int)d.n(), nc = (int)A.dc.n();
   const int t = (int)threadIdx.x;
-  const bool mine = t < n;
-  const uint32_t c = (uint32_t)t;
-  const int i = mine ? t % d.nx : 0, j = mine ? (t / d.nx) % d.ny : 0, k = mine ? t / (d.nx * d.ny) : 0;
-  uint32_t code = 0;
-  uint8_t cl = 0;
-  T fv = T(0), dv = T(0);
-  if (mine) {
+  // Phase 0 loads cell t's data into LDS; from then on thread t owns cell A.order[t]
+  // (issued with the other loads, so it adds no round trip to memory).
+  const bool own_t = t < n;
+  const uint16_t ord = A.order ? A.order[t] : (own_t ? (uint16_t)t : TAIL_NO_CELL);
+  // Every global load of the launch is issued before the first LDS store, so the
+  // launch waits for memory once (a load-store loop over the dense inverse would
+  // wait once per round).
+  constexpr int CINV_PER_THREAD = TAIL_CINV_LDS / 1024;
+  const int ncc = nc * nc;
+  const bool cinv_lds = ncc <= TAIL_CINV_LDS;
+  T cv[CINV_PER_THREAD];
 #pragma unroll
-    for (int q = 0; q < 9; q++) sh.g[q][c] = ld_at(g + (size_t)q * d.n(), c);
-    code = L.code[c];
-    cl = L.cls[c];
-    fv = A.f[c];
-    dv = A.dinv[c];
-    sh.fd[c] = fv * dv;
+  for (int u = 0; u < CINV_PER_THREAD; u++) {
+    const int q = t + 1024 * u;
+    cv[u] = (cinv_lds && q < ncc) ? A.cinv[q] : T(0);
   }
-  for (int q = t; q < nc; q += 1024) sh.clsc[q] = A.clsc[q];
-  // the dense inverse's loads overlap the smoothing that precedes its use
-  const bool cinv_lds = nc * nc <= TAIL_CINV_LDS;
-  if (cinv_lds)
-    for (int q = t; q < nc * nc; q += 1024) sh.cinv[q] = A.cinv[q];
+  TG gv[9];
+  uint32_t code_t = 0;
+  uint8_t cl_t = 0, clc = 0;
+  T f_t = T(0), dv = T(0), cf1 = T(0), cf2 = T(0);
+  if (own_t) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) gv[q] = ld_at(g + (size_t)q * d.n(), (uint32_t)t);
+    code_t = L.code[t];
+    cl_t = L.cls[t];
+    f_t = A.f[t];
+    dv = A.dinv[t];
+  }
+  if (t < nc) clc = A.clsc[t];  // nc <= 1024: one dense cell per thread at most
   if (t < TAIL_MAX_DEGREE) {
-    sh.c1[t] = A.c1[t];
-    sh.c2[t] = A.c2[t];
+    cf1 = A.c1[t];
+    cf2 = A.c2[t];
   }
+  if (own_t) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) sh.g[q][t] = gv[q];
+    sh.fd[t] = f_t * dv;
+    sh.code[t] = code_t;
+    sh.cls[t] = cl_t;
+    sh.fv[t] = f_t;
+  }
+  if (t < nc) sh.clsc[t] = clc;
+#pragma unroll
+  for (int u = 0; u < CINV_PER_THREAD; u++) {
+    const int q = t + 1024 * u;
+    if (cinv_lds && q < ncc) sh.cinv[q] = cv[u];
+  }
+  if (t < TAIL_MAX_DEGREE) {
+    sh.c1[t] = cf1;
+    sh.c2[t] = cf2;
+  }
+  __syncthreads();
+  const bool mine = ord < n;
+  const uint32_t c = mine ? (uint32_t)ord : 0u;
+  const int i = (int)c % d.nx, j = ((int)c / d.nx) % d.ny, k = (int)c / (d.nx * d.ny);
+  const uint32_t code = mine ? sh.code[c] : 0u;
+  const uint8_t cl = mine ? sh.cls[c] : (uint8_t)0;
+  const T fv = mine ? sh.fv[c] : T(0);
   const bool interior = (cl & CLS_INTERIOR) != 0;
   const T gs = L.gs;
-  __syncthreads();
 
   // pre-smoothing from zero: the fused first two steps (SM_FIRST2), or one step
   T* cur = sh.xb;
@@ -619,6 +658,57 @@ void launch_dense_solve(hipStream_t s, int n, const T* cinv, const T* f, T* x) {
   if (n > 1024) throw StatusError(1, "coarsest level larger than 1024 cells");
   k_dense_solve<T><<<(n + 63) / 64, 256, 0, s>>>(n, cinv, f, x);
   HIPCHECK(hipGetLastError());
+}
+
+std::vector<uint16_t> tail_order(size_t n, const uint32_t* code) {
+  if (n > (size_t)TAIL_MAX_CELLS) throw StatusError(1, "coarse tail: level larger than 1024 cells");
+  // cells in two groups, each in index order: interior code words, then the rest
+  std::vector<uint16_t> cells;
+  cells.reserve(n);
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t c = 0; c < n; c++)
+      if (((code[c] & ~DIVOK_BIT) == CODE_INTERIOR) == (pass == 0)) cells.push_back((uint16_t)c);
+  size_t ni = 0;
+  while (ni < n && (code[cells[ni]] & ~DIVOK_BIT) == CODE_INTERIOR) ni++;
+  // 64-cell waves, an all-interior one costed at 3 and any other at 5 (their
+  // instruction counts relative to each other); the counts of each kind per SIMD
+  // (at most four waves each) minimise the largest SIMD cost
+  const int nw = (int)((n + 63) / 64), nf = (int)(ni / 64), ns = nw - nf;
+  int pairs[15][2], np = 0;  // (slow, fast) waves one SIMD can hold
+  for (int sl = 0; sl <= 4; sl++)
+    for (int fa = 0; sl + fa <= 4; fa++) {
+      pairs[np][0] = sl;
+      pairs[np++][1] = fa;
+    }
+  int best_cost = 1 << 30, best[4] = {0, 0, 0, 0};
+  for (int p0 = 0; p0 < np; p0++)
+    for (int p1 = 0; p1 < np; p1++)
+      for (int p2 = 0; p2 < np; p2++)
+        for (int p3 = 0; p3 < np; p3++) {
+          const int pk[4] = {p0, p1, p2, p3};
+          int cs = 0, cf = 0, worst = 0;
+          for (int q = 0; q < 4; q++) {
+            cs += pairs[pk[q]][0];
+            cf += pairs[pk[q]][1];
+            worst = std::max(worst, 5 * pairs[pk[q]][0] + 3 * pairs[pk[q]][1]);
+          }
+          if (cs == ns && cf == nf && worst < best_cost) {
+            best_cost = worst;
+            for (int q = 0; q < 4; q++) best[q] = pk[q];
+          }
+        }
+  // wave slots q, q + 4, q + 8, q + 12 run on SIMD q: the interior waves (the first
+  // nf) take the fast places, the rest the slow ones
+  std::vector<int> slot_of(nw, -1);
+  int wf = 0, wsl = nf;
+  for (int q = 0; q < 4; q++) {
+    int u = 0;
+    for (int e = 0; e < pairs[best[q]][1]; e++) slot_of[wf++] = q + 4 * u++;
+    for (int e = 0; e < pairs[best[q]][0]; e++) slot_of[wsl++] = q + 4 * u++;
+  }
+  std::vector<uint16_t> order(1024, TAIL_NO_CELL);
+  for (size_t p = 0; p < n; p++) order[(size_t)slot_of[p / 64] * 64 + p % 64] = cells[p];
+  return order;
 }
 
 template <typename T>
