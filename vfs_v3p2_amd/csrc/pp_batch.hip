@@ -419,13 +419,13 @@ __device__ __forceinline__ double tile_sum(const double* __restrict__ pv, uint32
 
 // Lazily normalised Gram-Schmidt pass over U_{j+1} = w in place: w' = w - sum_l c_l U_l
 // with c_l = red[8 + l] red[RED_NUINV2 + l] (w.U_l / |U_l|^2), the next V-cycle's
-// input f0 = (T)(w' s) with s = conv_scale(|w|^2 = red[8 + j + 1]), and per-tile
-// partial sums of |w'|^2 at partial[t].  The same arithmetic as k_axpy_norm.
+// input f0 = (T)(w' s) with s = conv_scale(|w|^2 = red[8 + j + 1]), and per-wave
+// partial sums of |w'|^2 at partial[4 t + wave] (no LDS and no barrier per tile).
+// The same per-element arithmetic as k_axpy_norm.
 template <typename T>
 __global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict__ items,
                                                     const uint32_t* __restrict__ start, int nitems, uint32_t total,
                                                     const SlotDev* __restrict__ sd, double* __restrict__ partial) {
-  __shared__ double lds[4][64];
   FOR_TILES(total, t) {
     const uint32_t tw_ = start[nitems + t];
     const int q = (int)(tw_ >> 16);
@@ -443,7 +443,8 @@ __global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict
       I.f0[e] = (T)(v * conv_scale(I.red[8 + nv]));
       acc[0] = v * v;
     }
-    block_sums(acc, 1, partial, total, t, lds);
+    const double ws = wave_sum_to63(acc[0]);
+    if ((threadIdx.x & 63) == 63) partial[4 * (size_t)t + (threadIdx.x >> 6)] = ws;
   }
 }
 
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(256) void kb_finalize_norm(const FinItem* __restric
   const FinItem& I = items[blockIdx.x];
   const SlotDev st = sd[I.slot];
   if (!st.y) return;
-  const double s = tile_sum(partial, I.t0, I.t1, lds);
+  const double s = tile_sum(partial, 4 * I.t0, 4 * I.t1, lds);  // kb_axpy_norm's per-wave partials
   if (threadIdx.x == 0) {
     I.red[0] = s;
     I.hpin[0] = s;
@@ -1081,7 +1082,7 @@ struct BatchCore {
         vn.push_back(nv);
         mmax = std::max(mmax, h->mrest);
       }
-      const size_t need = (size_t)(mmax + 2) * std::max<size_t>(4 * (size_t)ta, tv);  // 4 partials per apply tile
+      const size_t need = (size_t)(mmax + 2) * 4 * std::max<size_t>(ta, tv);  // up to 4 partials per tile
       partial = plan.alloc<double>(need);
       partial_cap = need;
       std::vector<uint32_t> sa(ai.size()), sv(vi.size());
