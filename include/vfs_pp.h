@@ -416,6 +416,15 @@ int vfs_pp_export_stencil(vfs_pp_handle *h, double *vol19, uint8_t *cases);
 int vfs_pp_export_level(vfs_pp_handle *h, int level, int *nlevels, int dims[3], int *line, double *vol19,
                         uint8_t *cases);
 
+/* The dense inverse of the coarsest multigrid level, for verification; no
+ * counterpart in the reference (BoomerAMG solves its coarsest level itself).  *n
+ * receives that level's cell count; when cinv is non-NULL and cap >= n * n, cinv
+ * receives the top-left n x n block of the inverse of [A E; E^T 0] (A: the level's
+ * rows, identity rows where a cell has none; E: one indicator column per connected
+ * fluid component), column-major (cinv[c * n + r]), in the precision the hierarchy
+ * holds it (FP32 values widened to double in the mixed form). */
+int vfs_pp_export_coarse_inverse(vfs_pp_handle *h, size_t cap, double *cinv, int *n);
+
 /* The VolumeFlux face list as the engine holds it, for verification: the faces
  * between a fluid cell and an IB cell that VolumeFlux corrects (poisson.c:2936-3010
  * for immersed == 1, 2670-2700 otherwise), in (k, j, i, axis) order.  *count

@@ -254,3 +254,99 @@ def test_new_mask_after_device_grid_set_up(E, torch_cuda):
     assert st.iterations == st_ref.iterations
     np.testing.assert_array_equal(phi, phi_ref)
     np.testing.assert_array_equal(u, u_ref)
+
+
+def _oracle_coarsest(c, A, levels):
+    """The oracle's coarsest level (g, g9, nvert), coarsened along the engine's
+    level sequence (j kept where the engine keeps it)."""
+    g, g9, nv = A.g, A.g9, A.nvert
+    for L in levels[:-1]:
+        _, weak = O.level_coupling(g, g9, nv)
+        g, g9, nv, _ = O.level_coarsen(g, g9, nv, keep_y=L["line"] and weak > 0.01)
+    return g, g9, nv
+
+
+# stencil slot offsets (di, dj, dk) in the slot order of poisson.c:956-974
+SLOT_OFFSETS = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (1, -1, 0),
+                (-1, 1, 0), (-1, -1, 0), (0, 1, 1), (0, 1, -1), (0, -1, 1), (0, -1, -1), (1, 0, 1), (1, 0, -1),
+                (-1, 0, 1), (-1, 0, -1)]
+
+
+def _bordered_inverse(g, g9, nv, per):
+    """Top-left block of [A E; E^T 0]^-1 from the oracle's 19-slot rows of the level:
+    A on the interior cells in (k, j, i) order, each slot added at its neighbour
+    (periodic axes wrap, so on an axis of two cells both images add up, as in the
+    matrix-free operator), identity rows for cells without a row; E one indicator
+    column per fluid component (reached along each row's nonzero entries, cells
+    taken in index order).  Inverted by LAPACK."""
+    vol, rowflag, _ = O.stencil(g, g9, nv)
+    nx, ny, nz = g.mx - 2, g.my - 2, g.mz - 2
+    dims = (nx, ny, nz)
+    n = nx * ny * nz
+    has_row = rowflag[1:-1, 1:-1, 1:-1].ravel() != 0
+    Afull = np.zeros((n, n))
+    nbrs = [[] for _ in range(n)]
+    for c in np.nonzero(has_row)[0]:
+        k, j, i = c // (nx * ny), (c // nx) % ny, c % nx
+        for s, off in enumerate(SLOT_OFFSETS):
+            v = vol[k + 1, j + 1, i + 1, s]
+            if v == 0.0:
+                continue
+            q = [i + off[0], j + off[1], k + off[2]]
+            for a in range(3):
+                if per[a]:
+                    q[a] %= dims[a]
+                assert 0 <= q[a] < dims[a], (c, s)  # a wall face carries no coefficient
+            m = (q[2] * ny + q[1]) * nx + q[0]
+            Afull[c, m] += v
+            nbrs[c].append(m)
+    comp = np.full(n, -1)
+    ncomp = 0
+    for c0 in range(n):
+        if not has_row[c0] or comp[c0] >= 0:
+            continue
+        comp[c0] = ncomp
+        stack = [c0]
+        while stack:
+            u = stack.pop()
+            for v in nbrs[u]:
+                if v != u and has_row[v] and comp[v] < 0:
+                    comp[v] = ncomp
+                    stack.append(v)
+        ncomp += 1
+    N = n + ncomp
+    M = np.zeros((N, N))
+    M[:n, :n] = Afull
+    for c in range(n):
+        if not has_row[c]:
+            M[c, c] = 1.0
+        else:
+            M[c, n + comp[c]] = 1.0
+            M[n + comp[c], c] = 1.0
+    return np.linalg.inv(M)[:n, :n], N
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,coarse_max", [("hill_sphere", 512), ("hill_sphere", 64), ("towers", 512),
+                                             ("stretched_towers", 512)])
+def test_coarse_inverse_matches_lapack(E, name, coarse_max, precision):
+    """The dense inverse of the coarsest level -- Gauss-Jordan on the host for
+    bordered systems of up to 96 rows, one launch per column on the device above --
+    against LAPACK's inverse of the same bordered system assembled from the oracle's
+    rows of that level: to 1e-9 of its largest entry in FP64, to FP32 rounding in the
+    mixed form.  coarse_max 64 takes the hill case down to a 3x2x2 level, periodic
+    on an axis of two cells."""
+    c, A = get_case(name)
+    h = handle(E, c, precision=precision, coarse_max=coarse_max)
+    levels = h.export_levels(rows=False)
+    ce = h.export_coarse_inverse()
+    h.close()
+    ref, N = _bordered_inverse(*_oracle_coarsest(c, A, levels), c.per)
+    print(f"coarse inverse {name} coarse_max {coarse_max} precision {precision}: n {ce.shape[0]} "
+          f"bordered {N} ({'host' if N <= 96 else 'device'})", flush=True)
+    assert ce.shape == ref.shape
+    scale = np.abs(ref).max()
+    tol = 1e-9 if precision == 0 else 2.0 ** -23
+    assert np.abs(ce - ref).max() <= tol * scale
+    # the system is not trivial: off-diagonal coupling is present
+    assert np.abs(ref - np.diag(np.diag(ref))).max() > 1e-3 * scale

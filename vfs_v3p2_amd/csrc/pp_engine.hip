@@ -36,10 +36,11 @@ static void row_of(const CoarseRows& L, int i, int j, int k, double (&vol)[19], 
   build_row(L.g.data(), L.d.n(), L.code[c], c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
 }
 
-// Bordered inverse of the coarsest operator: [A E; E^T 0]^-1, top-left block,
-// where E has one indicator column per connected fluid component (the
-// constants of each component span the null space of the Neumann problem).
-static std::vector<double> coarse_inverse(const CoarseRows& L, int* ncomp_out) {
+// The bordered coarsest operator [A E; E^T 0] (N = n + ncomp, row-major), where E
+// has one indicator column per connected fluid component (the constants of each
+// component span the null space of the Neumann problem); rows without a row of A
+// hold 1 on the diagonal.  Its inverse's top-left n x n block is the dense solve.
+static std::vector<double> coarse_bordered(const CoarseRows& L, int* ncomp_out) {
   const int n = (int)L.d.n();
   std::vector<int> comp(n, -1);
   std::vector<std::vector<std::pair<int, double>>> rows(n);
@@ -71,15 +72,26 @@ static std::vector<double> coarse_inverse(const CoarseRows& L, int* ncomp_out) {
     ncomp++;
   }
   const int N = n + ncomp;
-  std::vector<double> B((size_t)N * N, 0.), I((size_t)N * N, 0.);
+  std::vector<double> B((size_t)N * N, 0.);
   for (int c = 0; c < n; c++) {
     if (!(L.code[c] & ROW_BIT)) { B[(size_t)c * N + c] = 1.; continue; }
     for (auto& e : rows[c]) B[(size_t)c * N + e.first] += e.second;
     B[(size_t)c * N + n + comp[c]] = 1.;
     B[(size_t)(n + comp[c]) * N + c] = 1.;
   }
+  *ncomp_out = ncomp;
+  return B;
+}
+
+// The same elimination as launch_gj_step (partial pivoting on the first largest
+// |B[r][col]|, the pivot row scaled by inv = 1 / pivot, every other row with f != 0
+// reduced by f times the scaled pivot row), on the host: it returns the top-left
+// n x n block of B^-1, column-major.  Plain products and differences (contraction
+// into fused multiply-adds is off), so it agrees bit for bit with the device kernel.
+static std::vector<double> coarse_inverse_host(std::vector<double> B, int N, int n) {
+#pragma clang fp contract(off)
+  std::vector<double> I((size_t)N * N, 0.);
   for (int r = 0; r < N; r++) I[(size_t)r * N + r] = 1.;
-  // Gauss-Jordan with partial pivoting
   for (int col = 0; col < N; col++) {
     int piv = col;
     double best = std::fabs(B[(size_t)col * N + col]);
@@ -87,29 +99,70 @@ static std::vector<double> coarse_inverse(const CoarseRows& L, int* ncomp_out) {
       const double v = std::fabs(B[(size_t)r * N + col]);
       if (v > best) { best = v; piv = r; }
     }
-    if (best == 0.) throw StatusError(VFS_PP_EINVAL, "coarsest operator is singular beyond its null space");
+    if (!(best > 0.)) throw StatusError(VFS_PP_EINVAL, "coarsest operator is singular beyond its null space");
     if (piv != col)
       for (int t = 0; t < N; t++) {
         std::swap(B[(size_t)piv * N + t], B[(size_t)col * N + t]);
         std::swap(I[(size_t)piv * N + t], I[(size_t)col * N + t]);
       }
     const double inv = 1. / B[(size_t)col * N + col];
-    for (int t = 0; t < N; t++) { B[(size_t)col * N + t] *= inv; I[(size_t)col * N + t] *= inv; }
+    for (int t = col + 1; t < N; t++) B[(size_t)col * N + t] *= inv;
+    for (int t = 0; t < N; t++) I[(size_t)col * N + t] *= inv;
     for (int r = 0; r < N; r++) {
       if (r == col) continue;
       const double f = B[(size_t)r * N + col];
       if (f == 0.) continue;
-      for (int t = 0; t < N; t++) {
-        B[(size_t)r * N + t] -= f * B[(size_t)col * N + t];
-        I[(size_t)r * N + t] -= f * I[(size_t)col * N + t];
-      }
+      for (int t = col + 1; t < N; t++) B[(size_t)r * N + t] -= f * B[(size_t)col * N + t];
+      for (int t = 0; t < N; t++) I[(size_t)r * N + t] -= f * I[(size_t)col * N + t];
     }
   }
   std::vector<double> out((size_t)n * n);
   for (int r = 0; r < n; r++)
-    for (int c = 0; c < n; c++) out[(size_t)c * n + r] = I[(size_t)r * N + c];  // column-major
-  *ncomp_out = ncomp;
+    for (int c = 0; c < n; c++) out[(size_t)c * n + r] = I[(size_t)r * N + c];
   return out;
+}
+
+// Bordered systems up to this many rows are inverted on the host: below it the
+// elimination costs less there than one device launch per column.
+constexpr int COARSE_HOST_MAX = 96;
+
+// cinv (n x n, column-major, precision T) = the top-left block of B^-1, by
+// Gauss-Jordan with partial pivoting on the device (launch_gj_step; on the host
+// for small systems); the working matrices are allocated for the call and freed
+// before it returns.
+template <typename T>
+static void coarse_inverse_device(hipStream_t s, const std::vector<double>& B, int n, T* cinv) {
+  const int N = (int)std::lround(std::sqrt((double)B.size()));
+  if (N <= COARSE_HOST_MAX) {
+    const std::vector<double> ci = coarse_inverse_host(B, N, n);
+    std::vector<T> ct(ci.size());
+    for (size_t i = 0; i < ci.size(); i++) ct[i] = T(ci[i]);
+    h2d(s, cinv, ct.data(), ct.size());
+    HIPCHECK(hipStreamSynchronize(s));
+    return;
+  }
+  const size_t nn = (size_t)N * N;
+  struct Buf {
+    void* p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } buf;
+  if (hipMalloc(&buf.p, (4 * nn + 1) * sizeof(double)) != hipSuccess)
+    throw StatusError(VFS_PP_ENOMEM, "coarsest inverse: no device memory for the elimination");
+  double* m = static_cast<double*>(buf.p);
+  double *b[2] = {m, m + nn}, *id[2] = {m + 2 * nn, m + 3 * nn};
+  int* singular = reinterpret_cast<int*>(m + 4 * nn);
+  h2d(s, b[0], B.data(), nn);
+  launch_gj_identity(s, N, id[0]);
+  HIPCHECK(hipMemsetAsync(singular, 0, sizeof(int), s));
+  for (int col = 0; col < N; col++)
+    launch_gj_step(s, N, col, b[col & 1], id[col & 1], b[(col + 1) & 1], id[(col + 1) & 1], singular);
+  launch_gj_out<T>(s, N, n, id[N & 1], cinv);
+  int bad = 0;
+  d2h(s, &bad, singular, 1);
+  HIPCHECK(hipStreamSynchronize(s));
+  if (bad) throw StatusError(VFS_PP_EINVAL, "coarsest operator is singular beyond its null space");
 }
 
 // --------------------------------------------------------------------- engine
@@ -296,13 +349,11 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   d2h(h->s, R.code.data(), Hc.code, nc);
   HIPCHECK(hipStreamSynchronize(h->s));
   int ncomp = 0;
-  std::vector<double> ci = coarse_inverse(R, &ncomp);
-  std::vector<T> cit(ci.size());
-  for (size_t i = 0; i < ci.size(); i++) cit[i] = T(ci[i]);
+  const std::vector<double> Bc = coarse_bordered(R, &ncomp);
   mg->ncoarse = (int)nc;
   mg->ncomp = ncomp;
-  mg->cinv = A.alloc<T>(cit.size());
-  h2d(h->s, mg->cinv, cit.data(), cit.size());
+  mg->cinv = A.alloc<T>(nc * nc);
+  coarse_inverse_device<T>(h->s, Bc, (int)nc, mg->cinv);
   // the coarse tail's thread-to-cell order, from the codes of the level above the
   // dense one (FP32 point levels that fit one workgroup)
   std::vector<uint16_t> ord;
@@ -1073,6 +1124,20 @@ int vfs_pp_export_level(vfs_pp_handle* h, int level, int* nlevels, int dims[3], 
     h->mg->level_export(h->s, level, &d, &ln, &code, g);
     export_rows(h, d, g, code, vol19, cases);
   }
+  return VFS_PP_OK;
+  ABI_END
+}
+
+int vfs_pp_export_coarse_inverse(vfs_pp_handle* h, size_t cap, double* cinv, int* n) {
+  ABI_BEGIN
+  if (!h || !n) return VFS_PP_EINVAL;
+  ensure_unbound(h);
+  ensure_geom(h);
+  const int nc = h->mg->coarse_cells();
+  *n = nc;
+  if (!cinv || cap < (size_t)nc * nc) return VFS_PP_OK;
+  const std::vector<double> c = h->mg->coarse_inverse_export(h->s);
+  std::copy(c.begin(), c.end(), cinv);
   return VFS_PP_OK;
   ABI_END
 }

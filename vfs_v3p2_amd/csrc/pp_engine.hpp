@@ -129,6 +129,9 @@ struct MGBase {
   // level l: dimensions, smoother, code words; its face coefficients as FP64 into
   // g64 (device, 9 n)
   virtual void level_export(hipStream_t s, int l, Dims* d, int* line, const uint32_t** code, double* g64) const = 0;
+  // the coarsest level's cell count and its dense inverse (column-major) as doubles
+  virtual int coarse_cells() const = 0;
+  virtual std::vector<double> coarse_inverse_export(hipStream_t s) const = 0;
   // a hierarchy that shares this one's coefficients, masks, Chebyshev parameters
   // and coarsest inverse and owns fresh work vectors (allocated from A)
   virtual std::unique_ptr<MGBase> clone_work(Arena& A) const = 0;
@@ -277,6 +280,14 @@ struct MG : MGBase {
     else if constexpr (std::is_same<T, double>::value)
       HIPCHECK(hipMemcpyAsync(g64, L.g, n9 * sizeof(double), hipMemcpyDeviceToDevice, s));
     else launch_convert<T, double>(s, n9, L.g, g64);
+  }
+
+  int coarse_cells() const override { return ncoarse; }
+  std::vector<double> coarse_inverse_export(hipStream_t s) const override {
+    std::vector<T> c((size_t)ncoarse * ncoarse);
+    d2h(s, c.data(), cinv, c.size());
+    HIPCHECK(hipStreamSynchronize(s));
+    return std::vector<double>(c.begin(), c.end());
   }
 
   std::unique_ptr<MGBase> clone_work(Arena& A) const override {

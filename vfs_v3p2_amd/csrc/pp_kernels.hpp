@@ -73,6 +73,16 @@ void launch_ibface_count(hipStream_t s, Dmda m, Dims d, const double* nv, int im
 void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const double* met9, int immersed,
                          const int* off, IbFace* out);
 
+// Dense inverse of the coarsest level on the device: Gauss-Jordan with partial
+// pivoting on [B | I] (N x N each, row-major), one launch per column col = 0 .. N-1
+// from (bi, ii) to (bo, io) (two buffer pairs in turn); launch_gj_out writes the
+// column-major top-left n x n block of the final I as T.
+void launch_gj_identity(hipStream_t s, int N, double* I);
+void launch_gj_step(hipStream_t s, int N, int col, const double* bi, const double* ii, double* bo, double* io,
+                    int* singular);
+template <typename T>
+void launch_gj_out(hipStream_t s, int N, int n, const double* I, T* cinv);
+
 // ---- FP64 operator ----------------------------------------------------------------
 // y = A x on interior vectors; x may be float or double.
 template <typename TX>

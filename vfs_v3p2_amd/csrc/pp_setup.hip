@@ -437,4 +437,114 @@ void launch_ibface_write(hipStream_t s, Dmda m, Dims d, const double* nv, const 
   HIPCHECK(hipGetLastError());
 }
 
+// ---- dense inverse of the coarsest level: Gauss-Jordan on [B | I], one launch per column
+
+__global__ __launch_bounds__(256) void k_gj_identity(int N, double* __restrict__ I) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  I[e] = (e / N == e % N) ? 1. : 0.;
+}
+
+// Column col: the pivot p is the first row r >= col with the largest |B[r][col]|
+// (every block finds it itself); row p, scaled by inv = 1 / B[p][col], becomes row
+// col, row col's old content moves to row p, and every other row (after that swap)
+// loses f = B[r][col] times the scaled pivot row, where f != 0.  Each product and
+// difference is rounded on its own (contraction into fused multiply-adds is off in
+// this kernel), in the order of the classic elimination, so the inverse is the same
+// to the last bit as that elimination run on the host.  B's columns up to col are not written: no later step reads them.
+// Blocks take rows blockIdx.x, blockIdx.x + gridDim.x, ...; a zero (or NaN) pivot
+// column sets *singular = col + 1 and the step writes nothing.
+__global__ __launch_bounds__(256) void k_gj_step(int N, int col, const double* __restrict__ bi,
+                                                 const double* __restrict__ ii, double* __restrict__ bo,
+                                                 double* __restrict__ io, int* __restrict__ singular) {
+#pragma clang fp contract(off)
+  __shared__ double sv[4];
+  __shared__ int sr[4];
+  double best = -1.;
+  int piv = N;
+  for (int r = col + (int)threadIdx.x; r < N; r += 256) {
+    const double v = fabs(bi[(size_t)r * N + col]);
+    if (v > best) {  // a thread's rows ascend: a tie keeps the first
+      best = v;
+      piv = r;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(best, off, 64);
+    const int orow = __shfl_xor(piv, off, 64);
+    if (ov > best || (ov == best && orow < piv)) {
+      best = ov;
+      piv = orow;
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sv[threadIdx.x >> 6] = best;
+    sr[threadIdx.x >> 6] = piv;
+  }
+  __syncthreads();
+  best = sv[0];
+  piv = sr[0];
+  for (int q = 1; q < 4; q++)
+    if (sv[q] > best || (sv[q] == best && sr[q] < piv)) {
+      best = sv[q];
+      piv = sr[q];
+    }
+  if (!(best > 0.)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *singular == 0) *singular = col + 1;
+    return;
+  }
+  const double inv = 1. / bi[(size_t)piv * N + col];
+  const double* __restrict__ pb = bi + (size_t)piv * N;
+  const double* __restrict__ pi = ii + (size_t)piv * N;
+  for (int r = blockIdx.x; r < N; r += gridDim.x) {
+    double* __restrict__ ob = bo + (size_t)r * N;
+    double* __restrict__ oi = io + (size_t)r * N;
+    if (r == col) {
+      for (int t = col + 1 + (int)threadIdx.x; t < N; t += 256) ob[t] = pb[t] * inv;
+      for (int t = (int)threadIdx.x; t < N; t += 256) oi[t] = pi[t] * inv;
+      continue;
+    }
+    const int src = (r == piv) ? col : r;
+    const double* __restrict__ sb = bi + (size_t)src * N;
+    const double* __restrict__ si = ii + (size_t)src * N;
+    const double f = sb[col];
+    if (f == 0.) {
+      for (int t = col + 1 + (int)threadIdx.x; t < N; t += 256) ob[t] = sb[t];
+      for (int t = (int)threadIdx.x; t < N; t += 256) oi[t] = si[t];
+    } else {
+      for (int t = col + 1 + (int)threadIdx.x; t < N; t += 256)
+        ob[t] = sb[t] - f * (pb[t] * inv);
+      for (int t = (int)threadIdx.x; t < N; t += 256) oi[t] = si[t] - f * (pi[t] * inv);
+    }
+  }
+}
+
+// cinv[c n + r] = (T) I[r N + c] for r, c < n (column-major top-left block)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gj_out(int N, int n, const double* __restrict__ I, T* __restrict__ cinv) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const size_t c = e / n, r = e % n;
+  cinv[e] = (T)I[r * N + c];
+}
+
+void launch_gj_identity(hipStream_t s, int N, double* I) {
+  k_gj_identity<<<lin_blocks((size_t)N * N), 256, 0, s>>>(N, I);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_gj_step(hipStream_t s, int N, int col, const double* bi, const double* ii, double* bo, double* io,
+                    int* singular) {
+  k_gj_step<<<(unsigned)std::min(N, 256), 256, 0, s>>>(N, col, bi, ii, bo, io, singular);
+  HIPCHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_gj_out(hipStream_t s, int N, int n, const double* I, T* cinv) {
+  k_gj_out<T><<<lin_blocks((size_t)n * n), 256, 0, s>>>(N, n, I, cinv);
+  HIPCHECK(hipGetLastError());
+}
+template void launch_gj_out<float>(hipStream_t, int, int, const double*, float*);
+template void launch_gj_out<double>(hipStream_t, int, int, const double*, double*);
+
 }  // namespace pp

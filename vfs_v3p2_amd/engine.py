@@ -127,6 +127,7 @@ def lib():
         "vfs_pp_export_level": (C.c_int, [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           dp, u8]),
         "vfs_pp_set_nvert_device": (C.c_int, [H, vp]),
+        "vfs_pp_export_coarse_inverse": (C.c_int, [H, C.c_size_t, dp, C.POINTER(C.c_int)]),
         "vfs_pp_export_ib_faces": (C.c_int, [H, C.c_size_t, C.POINTER(C.c_longlong), dp, C.POINTER(C.c_float),
                                              C.POINTER(C.c_size_t)]),
         "vfs_pp_classify_nvert_device": (C.c_int, [C.POINTER(Grid), vp]),
@@ -357,6 +358,17 @@ class Handle:
                 item["cases"] = cases.reshape(shape + (18,))
             out.append(item)
         return out
+
+    def export_coarse_inverse(self):
+        """The coarsest level's dense inverse as an (n, n) array C[r, c] (the top-left
+        block of the bordered inverse; FP32 values widened in the mixed form)."""
+        n = C.c_int()
+        _check(lib().vfs_pp_export_coarse_inverse(self._h, 0, None, C.byref(n)), "export coarse inverse")
+        m = n.value
+        out = np.zeros(m * m)
+        _check(lib().vfs_pp_export_coarse_inverse(self._h, out.size, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  C.byref(n)), "export coarse inverse")
+        return out.reshape(m, m).T.copy()
 
     def export_ib_faces(self):
         """VolumeFlux face list: (pos = 3 * DMDA point + axis, area, sign) arrays."""
