@@ -417,6 +417,28 @@ __device__ __forceinline__ double tile_sum(const double* __restrict__ pv, uint32
   return block_sum(a0 + a1, lds);
 }
 
+// The same for one 1024-thread block (the per-case finalizes, whose largest case
+// sets the launch's length): wave sums, then the 16 wave results in a fixed tree
+__device__ __forceinline__ double tile_sum_wide(const double* __restrict__ pv, uint32_t t0, uint32_t t1,
+                                                double* lds /* >= 16 */) {
+  double a0 = 0., a1 = 0.;
+  uint32_t b = t0 + threadIdx.x;
+  for (; b + 1024 < t1; b += 2048) { a0 += pv[b]; a1 += pv[b + 1024]; }
+  for (; b < t1; b += 1024) a0 += pv[b];
+  const double v = wave_sum(a0 + a1);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.;
+  if (threadIdx.x == 0) {
+    double q[4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) q[g] = (lds[4 * g] + lds[4 * g + 1]) + (lds[4 * g + 2] + lds[4 * g + 3]);
+    r = (q[0] + q[1]) + (q[2] + q[3]);
+  }
+  __syncthreads();
+  return r;
+}
+
 // Lazily normalised Gram-Schmidt pass over U_{j+1} = w in place: w' = w - sum_l c_l U_l
 // with c_l = red[8 + l] red[RED_NUINV2 + l] (w.U_l / |U_l|^2), the next V-cycle's
 // input f0 = (T)(w' s) with s = conv_scale(|w|^2 = red[8 + j + 1]), and per-wave
@@ -451,16 +473,16 @@ __global__ __launch_bounds__(256) void kb_axpy_norm(const VecItem<T>* __restrict
 // One block per (case, vector l < j + 1 + extra): fixed-order sum of the case's
 // partials (per tiles [t0, t1), `per` of them per tile) into red[off + l],
 // mirrored to pinned host memory.
-__global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ items, const double* __restrict__ partial,
-                                                   uint32_t total, int per, int off, int extra,
-                                                   const SlotDev* __restrict__ sd) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(1024) void kb_finalize(const FinItem* __restrict__ items, const double* __restrict__ partial,
+                                                    uint32_t total, int per, int off, int extra,
+                                                    const SlotDev* __restrict__ sd) {
+  __shared__ double lds[16];
   const FinItem& I = items[blockIdx.x];
   const SlotDev st = sd[I.slot];
   const int l = blockIdx.y;
   if (!st.y || l >= st.x + 1 + extra) return;
   const uint32_t pt = (uint32_t)per * total;
-  const double s = tile_sum(partial + (size_t)l * pt, per * I.t0, per * I.t1, lds);
+  const double s = tile_sum_wide(partial + (size_t)l * pt, per * I.t0, per * I.t1, lds);
   if (threadIdx.x == 0) {
     I.red[off + l] = s;
     I.hpin[off + l] = s;
@@ -469,14 +491,14 @@ __global__ __launch_bounds__(256) void kb_finalize(const FinItem* __restrict__ i
 
 // One block per case: |U_{j+1}|^2 from the partials of kb_axpy_norm into red[0]
 // (mirrored) and its reciprocal into red[RED_NUINV2 + j + 1] for the next step.
-__global__ __launch_bounds__(256) void kb_finalize_norm(const FinItem* __restrict__ items,
-                                                        const double* __restrict__ partial,
-                                                        const SlotDev* __restrict__ sd) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(1024) void kb_finalize_norm(const FinItem* __restrict__ items,
+                                                         const double* __restrict__ partial,
+                                                         const SlotDev* __restrict__ sd) {
+  __shared__ double lds[16];
   const FinItem& I = items[blockIdx.x];
   const SlotDev st = sd[I.slot];
   if (!st.y) return;
-  const double s = tile_sum(partial, 4 * I.t0, 4 * I.t1, lds);  // kb_axpy_norm's per-wave partials
+  const double s = tile_sum_wide(partial, 4 * I.t0, 4 * I.t1, lds);  // kb_axpy_norm's per-wave partials
   if (threadIdx.x == 0) {
     I.red[0] = s;
     I.hpin[0] = s;
@@ -1241,18 +1263,18 @@ struct BatchCore {
       // one Gram-Schmidt pass on the unnormalised basis: w.U_l and |w|^2 from the
       // operator pass, then w' and |w'|^2 (the host re-orthogonalises the rare
       // case whose pass cancelled most of w)
-      kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 1, sd);
+      kb_finalize<<<dim3(nfin, 64), 1024, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 1, sd);
       HIPCHECK(hipGetLastError());
       launch_one(axn);
-      kb_finalize_norm<<<nfin, 256, 0, s>>>(fin_vec, partial, sd);
+      kb_finalize_norm<<<nfin, 1024, 0, s>>>(fin_vec, partial, sd);
       HIPCHECK(hipGetLastError());
       return;
     }
     // CGS2 in three passes over each case's basis
-    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 0, sd);
+    kb_finalize<<<dim3(nfin, 64), 1024, 0, s>>>(fin_apply, partial, fin_total_apply, 4, 8, 0, sd);
     HIPCHECK(hipGetLastError());
     launch_one(axd);
-    kb_finalize<<<dim3(nfin, 64), 256, 0, s>>>(fin_vec, partial, fin_total_vec, 1, 72, 1, sd);
+    kb_finalize<<<dim3(nfin, 64), 1024, 0, s>>>(fin_vec, partial, fin_total_vec, 1, 72, 1, sd);
     HIPCHECK(hipGetLastError());
     kb_gs_norm<<<(nfin + 63) / 64, 64, 0, s>>>(fin_vec, nfin, sd);
     HIPCHECK(hipGetLastError());
