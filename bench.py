@@ -460,10 +460,14 @@ def host_array_block(E, torch, case, steps):
     h.set_geometry(case.fields)
     sh = case.shape
     base = np.ascontiguousarray(case.fields["ucont"])
+    # the caller's arrays persist from step to step, as the reference's Vecs do
+    u = np.empty_like(base)
+    phi, p, ucat = np.zeros(sh), np.zeros(sh), np.zeros(sh + (3,))
     lat = []
     for k in range(steps + 1):
-        u = base.copy()
-        phi, p, ucat = np.zeros(sh), np.zeros(sh), np.zeros(sh + (3,))
+        u[...] = base
+        for a_ in (phi, p, ucat):
+            a_.fill(0.0)
         a = time.perf_counter()
         st = h.project_host(u, phi, p, ucat, case.dt, case.st, case.coeff)
         if k:  # the first call is the warm-up

@@ -1025,16 +1025,18 @@ int vfs_pp_project(vfs_pp_handle* h, double* ucont, double* phi, double* p, doub
   ensure_unbound(h);
   ensure_geom(h);
   const size_t M = h->m.n();
-  Arena tmp;
-  double* du = tmp.alloc<double>(3 * M);
-  double* dphi = tmp.alloc<double>(M);
-  double* dp = tmp.alloc<double>(M);
-  double* dc = tmp.alloc<double>(3 * M);
+  // device copies of the four arrays, kept by the handle for its next call (the
+  // grid fixes their size); the copies and the projection follow each other on
+  // the handle's stream
+  if (!h->hstage) h->hstage = h->ar.alloc<double>(8 * M);
+  double* du = h->hstage;
+  double* dphi = du + 3 * M;
+  double* dp = dphi + M;
+  double* dc = dp + M;
   h2d(h->s, du, ucont, 3 * M);
   h2d(h->s, dphi, phi, M);
   h2d(h->s, dp, p, M);
   h2d(h->s, dc, ucat, 3 * M);
-  HIPCHECK(hipStreamSynchronize(h->s));
   const int rc = vfs_pp_project_device(h, du, dphi, dp, dc, dt, st, coeff, stats);
   if (rc != VFS_PP_OK && rc != VFS_PP_ENOCONV) return rc;
   d2h(h->s, ucont, du, 3 * M);

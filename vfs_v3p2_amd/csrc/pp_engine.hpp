@@ -374,6 +374,7 @@ struct vfs_pp_handle {
   pp::Arena setup_tmp;  // set-up scratch, kept between set-ups of a moving body
   // device bytes the handle holds: grid-sized buffers, the geometry's and set-up scratch
   size_t resident_bytes() const { return ar.total() + geo.total() + setup_tmp.total(); }
+  double* hstage = nullptr;   // device copies of vfs_pp_project's host arrays (8 M doubles, from ar)
   int batch_refs = 0;         // batches / server slots the handle is bound to (0 or 1)
   int immersed_ref = 0;       // the caller's `immersed` as given (grid.immersed holds the VolumeFlux variant)
   bool have_geom = false;     // metrics, masks and hierarchy are in place
