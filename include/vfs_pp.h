@@ -388,7 +388,9 @@ int vfs_pp_divergence_noalloc(const vfs_grid *grid, const double *ucont, const d
 int vfs_pp_project_device(vfs_pp_handle *h, double *ucont, double *phi, double *p, double *ucat,
                           double dt, double st, double coeff, vfs_pp_stats *stats);
 
-/* Same as vfs_pp_project_device with host arrays (copies over PCIe). */
+/* Same as vfs_pp_project_device with host arrays (copies over PCIe).  The first
+ * call allocates device copies of the four arrays (8 doubles per DMDA point) that
+ * the handle keeps for later calls until vfs_pp_destroy. */
 int vfs_pp_project(vfs_pp_handle *h, double *ucont, double *phi, double *p, double *ucat,
                    double dt, double st, double coeff, vfs_pp_stats *stats);
 
