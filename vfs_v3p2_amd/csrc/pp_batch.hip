@@ -143,6 +143,7 @@ struct RestrictItem {
   Dims dc;
   const uint8_t* clsc;
   T* fc;
+  const T* r;  // general: the fine residual f - A x (written by a kb_resid launch before)
   int cmask;
   int slot;
   uint32_t tx, ty;  // blk: 64x2x2 fine tiles; general: 64x4x1 coarse tiles
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void kb_restrict_gen(const RestrictItem<T>* __
     if (!sd[tw_ & 0xffffu].y) continue;
     int ic, jc, kc;
     if (!shaped_cell(shape, I.dc, t - start[q], ic, jc, kc)) continue;
-    resid_restrict_cell(I.F, I.f, I.x, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
+    restrict_sum_cell(I.F, I.r, I.dc, I.clsc, I.cmask, I.fc, ic, jc, kc);
   }
 }
 
@@ -985,6 +986,7 @@ struct BatchCore {
         for (int k = 0; k < degree; k++) smoothing_step(d, k, true, false, sm, cur);
         // residual + restriction to level d+1
         std::vector<RestrictItem<T>> rb, rg;
+        std::vector<SmoothItem<T>> rr;  // fine residuals of the general restriction
         std::vector<uint32_t> nb, ng;
         for (int q : sm) {
           auto& L = slots[q].mg->lev[d];
@@ -1004,13 +1006,27 @@ struct BatchCore {
             rb.push_back(it);
             nb.push_back(it.tx * it.ty * (uint32_t)((L.d.nz + 1) / 2));
           } else {
+            // the fine residual goes to the level's direction buffer, which no step
+            // reads between the end of the pre-smoothing and the first
+            // post-smoothing step (a new Chebyshev sequence, c1 = 0)
             it.tx = (uint32_t)(C.d.nx + 63) / 64;
             it.ty = (uint32_t)(C.d.ny + 3) / 4;
+            it.r = L.dv;
             rg.push_back(it);
             ng.push_back(cell_tiles(C.d));
+            SmoothItem<T> ri{};
+            ri.L = L.view();
+            ri.f = L.f;
+            ri.xin = cur[q][d];
+            ri.scr = L.dv;
+            ri.slot = q;
+            ri.tx = (uint32_t)(L.d.nx + 63) / 64;
+            ri.ty = (uint32_t)(L.d.ny + 3) / 4;
+            rr.push_back(ri);
           }
         }
         add_launch(K_RBLK, 0, rb, nb);
+        add_launch_cells(K_RESID, 0, rr);
         add_launch_cells(K_RGEN, 0, rg);
       }
       if (!dn.empty()) {

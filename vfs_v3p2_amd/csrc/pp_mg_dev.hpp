@@ -581,6 +581,29 @@ __device__ __forceinline__ void resid_restrict_cell(const LevelView<T>& F, const
   fc[C] = sum;
 }
 
+// fc[C] from the fine residual r (f - A x on the rows, as resid_cell writes it): the
+// sum resid_restrict_cell forms, over the same children in the same (k, j, i) order,
+// rows only, zero on coarse IB cells -- the same value to the last bit, with the
+// children's residuals computed one thread per fine cell beforehand.
+template <typename T>
+__device__ __forceinline__ void restrict_sum_cell(const LevelView<T>& F, const T* __restrict__ r, const Dims& dc,
+                                                  const uint8_t* __restrict__ clsc, int cmask, T* __restrict__ fc,
+                                                  int ic, int jc, int kc) {
+  const size_t C = dc.idx(ic, jc, kc);
+  if (clsc[C] & CLS_NONFLUID) { fc[C] = T(0); return; }
+  const int i0 = (cmask & 1) ? 2 * ic : ic, i1 = (cmask & 1) ? min(2 * ic + 1, F.d.nx - 1) : ic;
+  const int j0 = (cmask & 2) ? 2 * jc : jc, j1 = (cmask & 2) ? min(2 * jc + 1, F.d.ny - 1) : jc;
+  const int k0 = (cmask & 4) ? 2 * kc : kc, k1 = (cmask & 4) ? min(2 * kc + 1, F.d.nz - 1) : kc;
+  T sum = T(0);
+  for (int k = k0; k <= k1; k++)
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        const size_t c = F.d.idx(i, j, k);
+        if (F.code[c] & ROW_BIT) sum += r[c];
+      }
+  fc[C] = sum;
+}
+
 // Residual + restriction when all three axes are coarsened, for the 64x2x2 fine
 // tile (bx, by, bz) of a (64,2,2) block: one thread per fine cell, pair sums in i
 // by lane shuffle, the j and k pairs through LDS; the tile covers coarse cells
