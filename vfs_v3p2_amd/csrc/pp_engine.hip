@@ -188,8 +188,7 @@ static void queue_power_iteration(vfs_pp_handle* h, MG<T>& mg, typename MG<T>::L
                      T(0));
     launch_normsq<T>(h->s, n, L.dv, h->partial, &nb);
     launch_finalize(h->s, h->partial, 1, nb, h->red, 0, host_norms2 + it);
-    HIPCHECK(hipMemcpyAsync(L.xa, L.dv, n * sizeof(T), hipMemcpyDeviceToDevice, h->s));
-    launch_scale_by<T>(h->s, n, L.xa, h->red);
+    launch_scale_by<T>(h->s, n, L.xa, h->red, L.dv);  // x = dv / |dv|
   }
 }
 

@@ -269,8 +269,9 @@ void launch_coarse_tail(hipStream_t s, const TailArgs<T>& a);
 
 template <typename T>
 void launch_normsq(hipStream_t s, size_t n, const T* a, double* partial, int* nblk);
+// a = (src, or a itself when src is null) / sqrt(norm2[0]) (0 when norm2[0] is 0)
 template <typename T>
-void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2);
+void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2, const T* src = nullptr);
 
 // reference-layout transfer primitives (DMDA doubles), for verification
 void launch_ref_restrict(hipStream_t s, Dmda f, Dmda c, const double* nvf, const double* nvc, const double* xf,

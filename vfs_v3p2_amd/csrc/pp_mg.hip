@@ -473,11 +473,12 @@ __global__ __launch_bounds__(256) void k_normsq(size_t n, const T* __restrict__ 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_scale_by(size_t n, T* __restrict__ a, const double* __restrict__ nrm2) {
+__global__ __launch_bounds__(256) void k_scale_by(size_t n, T* __restrict__ a, const double* __restrict__ nrm2,
+                                                  const T* __restrict__ src) {
   const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
   const double s = nrm2[0] > 0. ? 1. / sqrt(nrm2[0]) : 0.;
-  a[t] = (T)((double)a[t] * s);
+  a[t] = (T)((double)(src ? src[t] : a[t]) * s);
 }
 
 // ---------------------------------------------------------- reference primitives
@@ -735,8 +736,8 @@ void launch_normsq(hipStream_t s, size_t n, const T* a, double* partial, int* nb
 }
 
 template <typename T>
-void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
-  k_scale_by<T><<<nb1(n), 256, 0, s>>>(n, a, norm2);
+void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2, const T* src) {
+  k_scale_by<T><<<nb1(n), 256, 0, s>>>(n, a, norm2, src);
   HIPCHECK(hipGetLastError());
 }
 
@@ -751,7 +752,7 @@ void launch_scale_by(hipStream_t s, size_t n, T* a, const double* norm2) {
   template void launch_dense_solve<T>(hipStream_t, int, const T*, const T*, T*);                                   \
   template void launch_coarse_tail<T>(hipStream_t, const TailArgs<T>&);                                            \
   template void launch_normsq<T>(hipStream_t, size_t, const T*, double*, int*);                                    \
-  template void launch_scale_by<T>(hipStream_t, size_t, T*, const double*);
+  template void launch_scale_by<T>(hipStream_t, size_t, T*, const double*, const T*);
 INST(float)
 INST(double)
 #undef INST
