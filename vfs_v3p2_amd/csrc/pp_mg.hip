@@ -39,10 +39,17 @@ __global__ __launch_bounds__(256) void k_smooth_point(LevelView<T> L, const T* _
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_line_factors(LevelView<T> L, T* __restrict__ lfac) {
+__global__ __launch_bounds__(256) void k_line_rows(LevelView<T> L, T* __restrict__ lfac) {
+  CELL_IJK(i, j, k);
+  if (i >= L.d.nx || j >= L.d.ny) return;
+  line_rows_cell(L, lfac, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_line_factors(Dims d, T* __restrict__ lfac) {
   const int i = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y * 4 + threadIdx.y;
-  if (i >= L.d.nx || k >= L.d.nz) return;
-  line_factors_col(L, lfac, i, k);
+  if (i >= d.nx || k >= d.nz) return;
+  line_factors_col(d, lfac, i, k);
 }
 
 template <typename T>
@@ -624,7 +631,10 @@ void launch_stream_floor(hipStream_t s, LevelView<T> L, const T* f, const T* x, 
 
 template <typename T>
 void launch_line_factors(hipStream_t s, LevelView<T> L, T* lfac) {
-  k_line_factors<T><<<line_grid(L.d), kBlk, 0, s>>>(L, lfac);
+  // every row's (a, b, c) in parallel, then the recurrence along each line
+  k_line_rows<T><<<cell_grid(L.d), kBlk, 0, s>>>(L, lfac);
+  HIPCHECK(hipGetLastError());
+  k_line_factors<T><<<line_grid(L.d), kBlk, 0, s>>>(L.d, lfac);
   HIPCHECK(hipGetLastError());
 }
 

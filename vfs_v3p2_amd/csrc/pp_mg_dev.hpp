@@ -157,37 +157,68 @@ __device__ __forceinline__ void smooth_point_cell(const LevelView<T>& L, const T
 // Thomas factors of the y-line tridiagonal part of A on the line (i, k), from the
 // rows' SP / CP / NP coefficients (cells without an unknown get the identity):
 // lfac = [a | cp | inv], each n long; den_j = b_j - a_j cp_{j-1}, inv_j = 1/den_j,
-// cp_j = c_j inv_j.
+// cp_j = c_j inv_j.  First pass, one thread per cell: the row's (a, b, c) into
+// lfac's three parts.
 template <typename T>
-__device__ __forceinline__ void line_factors_col(const LevelView<T>& L, T* __restrict__ lfac, int i, int k) {
+__device__ __forceinline__ void line_rows_cell(const LevelView<T>& L, T* __restrict__ lfac, int i, int j, int k) {
   const size_t n = L.d.n();
+  const size_t c = L.d.idx(i, j, k);
+  const uint32_t cd = L.code[c];
+  T a = T(0), b = T(1), cc = T(0);
+  if (cd & ROW_BIT) {
+    Nbr nb;
+    neighbours(L.d, i, j, k, nb);
+    T vol[19];
+    if (L.gh) {
+      build_row(L.gh, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+      a = vol[SP] * L.gs;
+      b = vol[CP] * L.gs;
+      cc = vol[NP] * L.gs;
+    } else {
+      build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
+      a = vol[SP];
+      b = vol[CP];
+      cc = vol[NP];
+    }
+  }
+  lfac[c] = a;
+  lfac[n + c] = b;
+  lfac[2 * n + c] = cc;
+}
+
+// The factorisation along the line (i, k), in place over line_rows_cell's rows:
+// den_j = b_j - a_j cp_{j-1}, inv_j = 1 / den_j (0 when den_j is 0), cp_j = c_j inv_j.
+// The rows' loads do not depend on the recurrence, so eight rows are loaded at a time.
+template <typename T>
+__device__ __forceinline__ void line_factors_col(const Dims& d, T* __restrict__ lfac, int i, int k) {
+  const size_t n = d.n();
+  const size_t sy = (size_t)d.nx;
+  const size_t c0 = d.idx(i, 0, k);
   T cprev = T(0);
-  for (int j = 0; j < L.d.ny; j++) {
-    const size_t c = L.d.idx(i, j, k);
-    const uint32_t cd = L.code[c];
-    T a = T(0), b = T(1), cc = T(0);
-    if (cd & ROW_BIT) {
-      Nbr nb;
-      neighbours(L.d, i, j, k, nb);
-      T vol[19];
-      if (L.gh) {
-        build_row(L.gh, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
-        a = vol[SP] * L.gs;
-        b = vol[CP] * L.gs;
-        cc = vol[NP] * L.gs;
-      } else {
-        build_row(L.g, n, cd, c, nb.s[WP], nb.s[SP], nb.s[BP], vol);
-        a = vol[SP];
-        b = vol[CP];
-        cc = vol[NP];
+  constexpr int LC = 8;
+  for (int j0 = 0; j0 < d.ny; j0 += LC) {
+    T b[LC], cc[LC], a[LC];
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      a[q] = b[q] = cc[q] = T(0);
+      if (j0 + q < d.ny) {
+        const size_t c = c0 + (size_t)(j0 + q) * sy;
+        a[q] = lfac[c];
+        b[q] = lfac[n + c];
+        cc[q] = lfac[2 * n + c];
       }
     }
-    const T den = b - a * cprev;
-    const T inv = den != T(0) ? T(1) / den : T(0);
-    cprev = cc * inv;
-    lfac[c] = a;
-    lfac[n + c] = cprev;
-    lfac[2 * n + c] = inv;
+#pragma unroll
+    for (int q = 0; q < LC; q++) {
+      if (j0 + q < d.ny) {
+        const size_t c = c0 + (size_t)(j0 + q) * sy;
+        const T den = b[q] - a[q] * cprev;
+        const T inv = den != T(0) ? T(1) / den : T(0);
+        cprev = cc[q] * inv;
+        lfac[n + c] = cprev;
+        lfac[2 * n + c] = inv;
+      }
+    }
   }
 }
 
