@@ -128,10 +128,10 @@ constexpr int COARSE_HOST_MAX = 96;
 
 // cinv (n x n, column-major, precision T) = the top-left block of B^-1, by
 // Gauss-Jordan with partial pivoting on the device (launch_gj_step; on the host
-// for small systems); the working matrices are allocated for the call and freed
-// before it returns.
+// for small systems).  The working matrices come from the set-up scratch `tmp`,
+// which a moving body's rebuilds keep from one call to the next.
 template <typename T>
-static void coarse_inverse_device(hipStream_t s, const std::vector<double>& B, int n, T* cinv) {
+static void coarse_inverse_device(hipStream_t s, Arena& tmp, const std::vector<double>& B, int n, T* cinv) {
   const int N = (int)std::lround(std::sqrt((double)B.size()));
   if (N <= COARSE_HOST_MAX) {
     const std::vector<double> ci = coarse_inverse_host(B, N, n);
@@ -142,15 +142,7 @@ static void coarse_inverse_device(hipStream_t s, const std::vector<double>& B, i
     return;
   }
   const size_t nn = (size_t)N * N;
-  struct Buf {
-    void* p = nullptr;
-    ~Buf() {
-      if (p) (void)hipFree(p);
-    }
-  } buf;
-  if (hipMalloc(&buf.p, (4 * nn + 1) * sizeof(double)) != hipSuccess)
-    throw StatusError(VFS_PP_ENOMEM, "coarsest inverse: no device memory for the elimination");
-  double* m = static_cast<double*>(buf.p);
+  double* m = tmp.alloc<double>(4 * nn + 1);
   double *b[2] = {m, m + nn}, *id[2] = {m + 2 * nn, m + 3 * nn};
   int* singular = reinterpret_cast<int*>(m + 4 * nn);
   h2d(s, b[0], B.data(), nn);
@@ -352,7 +344,7 @@ static std::unique_ptr<MG<T>> build_mg(vfs_pp_handle* h) {
   mg->ncoarse = (int)nc;
   mg->ncomp = ncomp;
   mg->cinv = A.alloc<T>(nc * nc);
-  coarse_inverse_device<T>(h->s, Bc, (int)nc, mg->cinv);
+  coarse_inverse_device<T>(h->s, h->setup_tmp, Bc, (int)nc, mg->cinv);
   // the coarse tail's thread-to-cell order, from the codes of the level above the
   // dense one (FP32 point levels that fit one workgroup)
   std::vector<uint16_t> ord;
